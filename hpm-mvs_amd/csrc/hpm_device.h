@@ -1,0 +1,80 @@
+/*
+ * hpm_device.h — device-side problem description shared by the HIP kernels
+ * (hpm_kernels.hip) and the host runtime (hpm_runtime.cpp).
+ *
+ * Layout in HBM for one reference view (W x H reference, nsrc sources):
+ *   image pool   float, all images back to back, row-major, one grey value
+ *                per pixel (the reference converts 8-bit JPEG to float,
+ *                HPM.cpp:492-494);
+ *   depth pool   float, previous-pass depth map of every image (geometric
+ *                consistency only, HPM.cpp:561-588);
+ *   hyp          float4 per reference pixel: camera-frame plane (n, d)
+ *                during an iteration, (world normal, depth) after finalize
+ *                (HPM.cu:1751-1764);
+ *   cost         float per pixel; sel: uint32 view mask per pixel;
+ *   prior_planes float4 per pixel, prior_mask uint32 per pixel (planar prior);
+ *   views        hpm_dev_view per image: camera + the camera-only terms of
+ *                the plane-induced homography (HPM.cu:229-255).
+ */
+#ifndef HPM_DEVICE_H_
+#define HPM_DEVICE_H_
+
+#include <stdint.h>
+
+#include "hpm_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hpm_dev_view {
+    hpm_camera cam;
+    float Rrel[9];      /* R_src * R_ref^T */
+    float trel[3];      /* R_src * (C_ref - C_src) */
+    int64_t img_off;    /* offset of this image in the image pool (elements) */
+    int64_t depth_off;  /* offset of this view's depth map in the depth pool */
+} hpm_dev_view;
+
+typedef struct hpm_dev_problem {
+    int32_t W, H, nsrc;
+    int32_t yy_limit;          /* checkerboard row-pair extent, see for_colour */
+    hpm_params params;
+    uint32_t call_index;       /* RNG stream of this RunPatchMatch call */
+    uint32_t salt;             /* RNG stream: reference image id */
+    float good[3][3];          /* JudgeExtend thresholds [iter][round] */
+    float thr[3];              /* view-selection thresholds per iteration */
+    float fallback[3];         /* expf(thr^2 / -0.32) */
+    const float* images;
+    const float* depths;
+    const hpm_dev_view* views;
+    float* hyp;                /* float4 per pixel */
+    float* cost;
+    uint32_t* sel;
+    const float* prior_planes; /* float4 per pixel */
+    const uint32_t* prior_mask;
+    unsigned long long* n_eval; /* NCC evaluation counter (may be null) */
+} hpm_dev_problem;
+
+/* Kernel launchers (hpm_kernels.hip). All launch on `stream` and return a
+ * hipError_t value as int. */
+int hpm_launch_init(const hpm_dev_problem* pb, void* stream);
+int hpm_launch_propagate(const hpm_dev_problem* pb, int iter, int colour, void* stream);
+int hpm_launch_finalize(const hpm_dev_problem* pb, void* stream);
+int hpm_launch_filter(const hpm_dev_problem* pb, int colour, void* stream);
+/* Cost vectors for fixed camera-frame planes at given pixels:
+ * out[i*nsrc + v]. */
+int hpm_launch_cost_vectors(const hpm_dev_problem* pb, const float* planes, const int32_t* pix, int npix,
+                            float* out, void* stream);
+/* Joint bilateral upsampling: guide w x h, src_nd float4 at sw x sh
+ * (normal xyz, depth w), outputs depth (w x h) and float4 normal. */
+int hpm_launch_jbu(const float* guide, int w, int h, const float* src_nd, int sw, int sh, int scale_i,
+                   float* out_depth, float* out_normal, void* stream);
+
+/* kind 0..3: exp/sin/cos/acos of in[i]; kind 4: Philox blocks */
+int hpm_launch_selftest(int kind, const void* in, void* out, int n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPM_DEVICE_H_ */

@@ -1,0 +1,1272 @@
+/*
+ * hpm_kernels.hip — CDNA4 (gfx950) kernels of the per-reference-view
+ * PatchMatch: random/prior/GC initialisation, red/black checkerboard
+ * propagation with NESP sampling, joint view selection and refinement,
+ * plane->depth finalisation, the red/black median filter, fixed-hypothesis
+ * cost vectors and joint bilateral upsampling.
+ *
+ * Reference semantics: HPM-MVS/HPM.cu (cited per function). The arithmetic
+ * is the shared FP32 definition documented in oracle/hpm_oracle.cpp's
+ * header: the same expression order, explicit fmaf where used, correctly
+ * rounded division and square root, hpm_fmath.h elementary functions and
+ * Philox random numbers, so results can be compared bit for bit with the
+ * CPU restatement. Build with -ffp-contract=off.
+ *
+ * Mapping: one lane per pixel of the active colour, 64-wide waves along x.
+ * The 36 reference taps, their bilateral weights and reference sums stay in
+ * VGPRs for the whole pixel update; source taps are bilinear gathers from
+ * the FP32 image pool (cache resident: a 1600x1066 view set is ~75 MB).
+ */
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <stdint.h>
+
+#include "hpm_device.h"
+#include "hpm_fmath.h"
+#include "hpm_philox.h"
+
+#define HPM_NTAP 36 /* 11x11 window at stride 2 (HPM.cu:347-356) */
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+__device__ __forceinline__ float4 ld4(const float* a, int i)
+{
+    return reinterpret_cast<const float4*>(a)[i];
+}
+
+__device__ __forceinline__ void st4(float* a, int i, float4 v)
+{
+    reinterpret_cast<float4*>(a)[i] = v;
+}
+
+__device__ __forceinline__ float texel(const float* img, int W, int H, int x, int y)
+{
+    return img[(size_t)clampi(y, 0, H - 1) * W + clampi(x, 0, W - 1)];
+}
+
+/* linear-filtered, clamp-addressed fetch at texel-centre coordinates */
+__device__ __forceinline__ float bilinear(const float* img, int W, int H, float x, float y)
+{
+    x = fminf(fmaxf(x, -1.0f), (float)W);
+    y = fminf(fmaxf(y, -1.0f), (float)H);
+    const float fx = floorf(x);
+    const float fy = floorf(y);
+    const float a = x - fx;
+    const float b = y - fy;
+    const int ix = (int)fx;
+    const int iy = (int)fy;
+    const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
+    const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
+    const float* r0 = img + (size_t)y0 * W;
+    const float* r1 = img + (size_t)y1 * W;
+    const float t00 = r0[x0];
+    const float t10 = r0[x1];
+    const float t01 = r1[x0];
+    const float t11 = r1[x1];
+    const float v0 = fmaf(a, t10 - t00, t00);
+    const float v1 = fmaf(a, t11 - t01, t01);
+    return fmaf(b, v1 - v0, v0);
+}
+
+/* plane-dependent part of ComputeHomography (HPM.cu:257-286) */
+__device__ __forceinline__ void homography(const hpm_camera& rc, const hpm_dev_view& sv, float4 n, float H[9])
+{
+    const hpm_camera& sc = sv.cam;
+    float h[9], tmp[9];
+    const float nn[3] = {n.x, n.y, n.z};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            h[3 * r + c] = sv.Rrel[3 * r + c] - sv.trel[r] * nn[c] / n.w;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        tmp[3 * r + 0] = h[3 * r + 0] / rc.K[0];
+        tmp[3 * r + 1] = h[3 * r + 1] / rc.K[4];
+        tmp[3 * r + 2] = -h[3 * r + 0] * rc.K[2] / rc.K[0] - h[3 * r + 1] * rc.K[5] / rc.K[4] + h[3 * r + 2];
+    }
+    H[0] = sc.K[0] * tmp[0] + sc.K[2] * tmp[6];
+    H[1] = sc.K[0] * tmp[1] + sc.K[2] * tmp[7];
+    H[2] = sc.K[0] * tmp[2] + sc.K[2] * tmp[8];
+    H[3] = sc.K[4] * tmp[3] + sc.K[5] * tmp[6];
+    H[4] = sc.K[4] * tmp[4] + sc.K[5] * tmp[7];
+    H[5] = sc.K[4] * tmp[5] + sc.K[5] * tmp[8];
+    H[6] = sc.K[8] * tmp[6];
+    H[7] = sc.K[8] * tmp[7];
+    H[8] = sc.K[8] * tmp[8];
+}
+
+/* ComputeCorrespondingPoint (HPM.cu:289-296), fma form, one reciprocal */
+__device__ __forceinline__ void warp(const float H[9], float X, float Y, float* sx, float* sy)
+{
+    const float px = fmaf(H[0], X, fmaf(H[1], Y, H[2]));
+    const float py = fmaf(H[3], X, fmaf(H[4], Y, H[5]));
+    const float pz = fmaf(H[6], X, fmaf(H[7], Y, H[8]));
+    const float inv = 1.0f / pz;
+    *sx = px * inv;
+    *sy = py * inv;
+}
+
+/* reference-only part of ComputeBilateralNCC + ComputeBilateralWeight
+ * (HPM.cu:318-369), held in registers for the whole pixel update */
+struct RefPatch {
+    float w[HPM_NTAP];
+    float wr[HPM_NTAP];
+    float c;
+    float inv_w;
+    float mr;
+    float var_r;
+};
+
+__device__ __forceinline__ void make_ref_patch(const float* img, int W, int H, int px, int py, const hpm_params& P,
+                                               RefPatch* rp)
+{
+    const float c = texel(img, W, H, px, py);
+    const float two_ss = 2.0f * P.sigma_spatial * P.sigma_spatial;
+    const float two_sc = 2.0f * P.sigma_color * P.sigma_color;
+    float wsum = 0.0f, sr = 0.0f, srr = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            const int i = 2 * a - 5, j = 2 * b - 5;
+            const int k = 6 * a + b;
+            const float r = texel(img, W, H, px + i, py + j);
+            const float fi = (float)i, fj = (float)j;
+            const float sd = sqrtf(fi * fi + fj * fj);
+            const float cd = fabsf(r - c);
+            const float w = hpm_expf(-sd / two_ss - cd / two_sc);
+            const float rr = r - c;
+            const float wr = w * rr;
+            rp->w[k] = w;
+            rp->wr[k] = wr;
+            wsum = wsum + w;
+            sr = sr + wr;
+            srr = fmaf(wr, rr, srr);
+        }
+    }
+    rp->c = c;
+    rp->inv_w = 1.0f / wsum;
+    rp->mr = sr * rp->inv_w;
+    rp->var_r = srr * rp->inv_w - rp->mr * rp->mr;
+}
+
+/* ComputeBilateralNCC (HPM.cu:325-397) for source view v */
+__device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& rp, int px, int py, int v,
+                                     float4 plane)
+{
+    const hpm_dev_view& sv = pb.views[v];
+    const int SW = sv.cam.width, SH = sv.cam.height;
+    float H[9];
+    homography(pb.views[0].cam, sv, plane, H);
+    float cx0, cy0;
+    warp(H, (float)px, (float)py, &cx0, &cy0);
+    if (cx0 >= (float)SW || cx0 < 0.0f || cy0 >= (float)SH || cy0 < 0.0f) {
+        return 2.0f;
+    }
+    const float* simg = pb.images + sv.img_off;
+    float ss = 0.0f, sss = 0.0f, srs = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            const int k = 6 * a + b;
+            float sx, sy;
+            warp(H, (float)(px + 2 * a - 5), (float)(py + 2 * b - 5), &sx, &sy);
+            const float s = bilinear(simg, SW, SH, sx, sy) - rp.c;
+            const float ws = rp.w[k] * s;
+            ss = ss + ws;
+            sss = fmaf(ws, s, sss);
+            srs = fmaf(rp.wr[k], s, srs);
+        }
+    }
+    const float ms = ss * rp.inv_w;
+    const float var_s = sss * rp.inv_w - ms * ms;
+    const float kMinVar = 1e-5f;
+    if (rp.var_r < kMinVar || var_s < kMinVar) {
+        return 2.0f;
+    }
+    const float cov = srs * rp.inv_w - rp.mr * ms;
+    const float den = sqrtf(rp.var_r * var_s);
+    return fmaxf(0.0f, fminf(2.0f, 1.0f - cov / den));
+}
+
+/* HPM.cu:102-133 */
+__device__ __forceinline__ void point3(const hpm_camera& c, int px, int py, float depth, float X[3])
+{
+    X[0] = depth * ((float)px - c.K[2]) / c.K[0];
+    X[1] = depth * ((float)py - c.K[5]) / c.K[4];
+    X[2] = depth;
+}
+
+__device__ __forceinline__ float4 view_direction(const hpm_camera& c, int px, int py, float depth)
+{
+    float X[3];
+    point3(c, px, py, depth, X);
+    const float norm = sqrtf(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+    return make_float4(X[0] / norm, X[1] / norm, X[2] / norm, 0.0f);
+}
+
+__device__ __forceinline__ float dist_to_origin(const hpm_camera& c, int px, int py, float depth, float4 n)
+{
+    float X[3];
+    point3(c, px, py, depth, X);
+    return -(n.x * X[0] + n.y * X[1] + n.z * X[2]);
+}
+
+__device__ __forceinline__ float depth_from_plane(const hpm_camera& c, float4 pl, int px, int py)
+{
+    return -pl.w * c.K[0] /
+           (((float)px - c.K[2]) * pl.x + (c.K[0] / c.K[4]) * ((float)py - c.K[5]) * pl.y + c.K[0] * pl.z);
+}
+
+/* NormalizeVec3 (HPM.cu:77-84) with 1/sqrtf */
+__device__ __forceinline__ void normalize3(float4* v)
+{
+    const float n2 = v->x * v->x + v->y * v->y + v->z * v->z;
+    const float inv = 1.0f / sqrtf(n2);
+    v->x *= inv;
+    v->y *= inv;
+    v->z *= inv;
+}
+
+__device__ __forceinline__ float dot3(float4 a, float4 b)
+{
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+/* GenerateRandomNormal (HPM.cu:135-161) */
+__device__ float4 random_normal(const hpm_camera& c, int px, int py, hpm_rng* rng, float depth)
+{
+    float q1 = 1.0f, q2 = 1.0f, s = 2.0f;
+    while (s >= 1.0f) {
+        q1 = 2.0f * hpm_rng_uniform(rng) - 1.0f;
+        q2 = 2.0f * hpm_rng_uniform(rng) - 1.0f;
+        s = q1 * q1 + q2 * q2;
+    }
+    const float sq = sqrtf(1.0f - s);
+    float4 n = make_float4(2.0f * q1 * sq, 2.0f * q2 * sq, 1.0f - 2.0f * s, 0.0f);
+    const float4 vd = view_direction(c, px, py, depth);
+    const float d = n.x * vd.x + n.y * vd.y + n.z * vd.z;
+    if (d > 0.0f) {
+        n.x = -n.x;
+        n.y = -n.y;
+        n.z = -n.z;
+    }
+    normalize3(&n);
+    return n;
+}
+
+/* GeneratePerturbedNormal (HPM.cu:163-198) */
+__device__ float4 perturbed_normal(const hpm_camera& c, int px, int py, float4 normal, hpm_rng* rng,
+                                   float perturbation)
+{
+    const float4 vd = view_direction(c, px, py, 1.0f);
+    const float a1 = (hpm_rng_uniform(rng) - 0.5f) * perturbation;
+    const float a2 = (hpm_rng_uniform(rng) - 0.5f) * perturbation;
+    const float a3 = (hpm_rng_uniform(rng) - 0.5f) * perturbation;
+    const float s1 = hpm_sinf(a1), s2 = hpm_sinf(a2), s3 = hpm_sinf(a3);
+    const float c1 = hpm_cosf(a1), c2 = hpm_cosf(a2), c3 = hpm_cosf(a3);
+    float R[9];
+    R[0] = c2 * c3;
+    R[1] = c3 * s1 * s2 - c1 * s3;
+    R[2] = s1 * s3 + c1 * c3 * s2;
+    R[3] = c2 * s3;
+    R[4] = c1 * c3 + s1 * s2 * s3;
+    R[5] = c1 * s2 * s3 - c3 * s1;
+    R[6] = -s2;
+    R[7] = c2 * s1;
+    R[8] = c1 * c2;
+    float4 np;
+    np.x = R[0] * normal.x + R[1] * normal.y + R[2] * normal.z;
+    np.y = R[3] * normal.x + R[4] * normal.y + R[5] * normal.z;
+    np.z = R[6] * normal.x + R[7] * normal.y + R[8] * normal.z;
+    np.w = normal.w;
+    if (dot3(np, vd) >= 0.0f) {
+        np = normal;
+    }
+    normalize3(&np);
+    return np;
+}
+
+/* Get3DPointonWorld_cu / ProjectonCamera_cu (HPM.cu:445-481) */
+__device__ __forceinline__ void point_world(float x, float y, float depth, const hpm_camera& c, float out[3])
+{
+    float p[3], t[3], C[3];
+    p[0] = depth * (x - c.K[2]) / c.K[0];
+    p[1] = depth * (y - c.K[5]) / c.K[4];
+    p[2] = depth;
+    t[0] = c.R[0] * p[0] + c.R[3] * p[1] + c.R[6] * p[2];
+    t[1] = c.R[1] * p[0] + c.R[4] * p[1] + c.R[7] * p[2];
+    t[2] = c.R[2] * p[0] + c.R[5] * p[1] + c.R[8] * p[2];
+    C[0] = -(c.R[0] * c.t[0] + c.R[3] * c.t[1] + c.R[6] * c.t[2]);
+    C[1] = -(c.R[1] * c.t[0] + c.R[4] * c.t[1] + c.R[7] * c.t[2]);
+    C[2] = -(c.R[2] * c.t[0] + c.R[5] * c.t[1] + c.R[8] * c.t[2]);
+    out[0] = t[0] + C[0];
+    out[1] = t[1] + C[1];
+    out[2] = t[2] + C[2];
+}
+
+__device__ __forceinline__ void project(const float X[3], const hpm_camera& c, float* u, float* v, float* d)
+{
+    float t[3];
+    t[0] = c.R[0] * X[0] + c.R[1] * X[1] + c.R[2] * X[2] + c.t[0];
+    t[1] = c.R[3] * X[0] + c.R[4] * X[1] + c.R[5] * X[2] + c.t[1];
+    t[2] = c.R[6] * X[0] + c.R[7] * X[1] + c.R[8] * X[2] + c.t[2];
+    *d = c.K[6] * t[0] + c.K[7] * t[1] + c.K[8] * t[2];
+    *u = (c.K[0] * t[0] + c.K[1] * t[1] + c.K[2] * t[2]) / *d;
+    *v = (c.K[3] * t[0] + c.K[4] * t[1] + c.K[5] * t[2]) / *d;
+}
+
+/* ComputeGeomConsistencyCost (HPM.cu:483-508) */
+__device__ float geom_cost(const hpm_dev_problem& pb, int v, float4 plane, int px, int py)
+{
+    const hpm_camera& rc = pb.views[0].cam;
+    const hpm_camera& sc = pb.views[v].cam;
+    const float depth = depth_from_plane(rc, plane, px, py);
+    float fw[3];
+    point_world((float)px, (float)py, depth, rc, fw);
+    float sx, sy, sd;
+    project(fw, sc, &sx, &sy, &sd);
+    const float cxf = fminf(fmaxf(sx, -1.0f), (float)sc.width);
+    const float cyf = fminf(fmaxf(sy, -1.0f), (float)sc.height);
+    const int ix = clampi((int)cxf, 0, sc.width - 1);
+    const int iy = clampi((int)cyf, 0, sc.height - 1);
+    const float src_depth = pb.depths[pb.views[v].depth_off + (int64_t)iy * sc.width + ix];
+    if (src_depth == 0.0f) {
+        return 5.0f;
+    }
+    float bw[3];
+    point_world(sx, sy, src_depth, sc, bw);
+    float bx, by, bd;
+    project(bw, rc, &bx, &by, &bd);
+    const float dx = (float)px - bx;
+    const float dy = (float)py - by;
+    return fminf(5.0f, sqrtf(dx * dx + dy * dy));
+}
+
+__device__ __forceinline__ int find_min_idx8(const float* c)
+{
+    float m = c[0];
+    int idx = 0;
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+        if (c[i] <= m) {
+            m = c[i];
+            idx = i;
+        }
+    }
+    return idx;
+}
+
+__device__ __forceinline__ int find_max_idx8(const float* c)
+{
+    float m = c[0];
+    int idx = 0;
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+        if (c[i] >= m) {
+            m = c[i];
+            idx = i;
+        }
+    }
+    return idx;
+}
+
+/* ComputeMultiViewCostVector (HPM.cu:438-443); views with a zero weight
+ * are skipped when weights is given (their terms are never read) */
+__device__ void cost_vector(const hpm_dev_problem& pb, const RefPatch& rp, int px, int py, float4 plane, float* out,
+                            const float* weights, uint32_t* nev)
+{
+    for (int v = 1; v <= pb.nsrc; ++v) {
+        if (weights != nullptr && !(weights[v - 1] > 0.0f)) {
+            out[v - 1] = 0.0f;
+            continue;
+        }
+        out[v - 1] = ncc(pb, rp, px, py, v, plane);
+        ++*nev;
+    }
+}
+
+/* ComputeMultiViewInitialCostandSelectedViews (HPM.cu:399-436) */
+__device__ float initial_cost(const hpm_dev_problem& pb, const RefPatch& rp, int px, int py, float4 plane,
+                              uint32_t* sel, uint32_t* nev)
+{
+    float cv[HPM_MAX_SOURCES], cvc[HPM_MAX_SOURCES];
+    int count = 0, valid = 0;
+    for (int v = 1; v <= pb.nsrc; ++v) {
+        const float c = ncc(pb, rp, px, py, v, plane);
+        ++*nev;
+        cv[v - 1] = c;
+        cvc[v - 1] = c;
+        ++count;
+        if (c < 2.0f) {
+            ++valid;
+        }
+    }
+    /* sort_small (HPM.cu:3-12) */
+    for (int i = 1; i < count; i++) {
+        const float tmp = cv[i];
+        int j;
+        for (j = i; j >= 1 && tmp < cv[j - 1]; j--) {
+            cv[j] = cv[j - 1];
+        }
+        cv[j] = tmp;
+    }
+    uint32_t m = 0u;
+    const int top_k = min(valid, pb.params.top_k);
+    float result = 2.0f;
+    if (top_k > 0) {
+        float cost = 0.0f;
+        for (int i = 0; i < top_k; ++i) {
+            cost += cv[i];
+        }
+        const float thr = cv[top_k - 1];
+        for (int i = 0; i < pb.nsrc; ++i) {
+            if (cvc[i] <= thr) {
+                m |= (1u << i);
+            }
+        }
+        result = cost / (float)top_k;
+    }
+    *sel = m;
+    return result;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off, 64);
+    }
+    return v;
+}
+
+__device__ __forceinline__ void count_evals(const hpm_dev_problem& pb, uint32_t nev)
+{
+    if (pb.n_eval == nullptr) {
+        return;
+    }
+    const unsigned long long s = wave_sum((unsigned long long)nev);
+    if ((threadIdx.x + blockDim.x * threadIdx.y) % 64 == 0 && s != 0ull) {
+        atomicAdd(pb.n_eval, s);
+    }
+}
+
+/* RandomInitialization (HPM.cu:510-558) */
+__device__ uint32_t init_pixel(const hpm_dev_problem& pb, int px, int py)
+{
+    const hpm_params& P = pb.params;
+    const hpm_camera& rc = pb.views[0].cam;
+    const int center = py * pb.W + px;
+    uint32_t nev = 0;
+    hpm_rng rng;
+    hpm_rng_init(&rng, P.seed_lo, P.seed_hi, (uint32_t)center, hpm_stream_init(pb.call_index), pb.salt);
+    RefPatch rp;
+    make_ref_patch(pb.images, pb.W, pb.H, px, py, P, &rp);
+    float4 plane;
+    if (P.geom_consistency) {
+        const float4 t = ld4(pb.hyp, center);
+        plane.x = rc.R[0] * t.x + rc.R[1] * t.y + rc.R[2] * t.z;
+        plane.y = rc.R[3] * t.x + rc.R[4] * t.y + rc.R[5] * t.z;
+        plane.z = rc.R[6] * t.x + rc.R[7] * t.y + rc.R[8] * t.z;
+        plane.w = t.w;
+        const float depth = plane.w;
+        plane.w = dist_to_origin(rc, px, py, depth, plane);
+    } else if (P.planar_prior) {
+        if (pb.prior_mask[center] > 0u && pb.cost[center] >= 0.1f) {
+            const float perturbation = 0.02f;
+            const float4 pl = ld4(pb.prior_planes, center);
+            float dp = pl.w;
+            const float dmin_p = (1 - 3 * perturbation) * dp;
+            const float dmax_p = (1 + 3 * perturbation) * dp;
+            dp = hpm_rng_uniform(&rng) * (dmax_p - dmin_p) + dmin_p;
+            plane = perturbed_normal(rc, px, py, pl, &rng, (float)((double)(3 * perturbation) * kPi));
+            plane.w = dp;
+        } else {
+            plane = ld4(pb.hyp, center);
+            const float depth = plane.w;
+            plane.w = dist_to_origin(rc, px, py, depth, plane);
+        }
+    } else {
+        const float depth = hpm_rng_uniform(&rng) * (P.depth_max - P.depth_min) + P.depth_min;
+        plane = random_normal(rc, px, py, &rng, depth);
+        plane.w = dist_to_origin(rc, px, py, depth, plane);
+    }
+    st4(pb.hyp, center, plane);
+    uint32_t m;
+    pb.cost[center] = initial_cost(pb, rp, px, py, plane, &m, &nev);
+    pb.sel[center] = m;
+    return nev;
+}
+
+struct PriorConsts {
+    float gamma, tdss, tass, beta;
+};
+
+__device__ __forceinline__ PriorConsts prior_consts(const hpm_params& P)
+{
+    PriorConsts k;
+    k.gamma = 0.5f;
+    const float ds = (P.depth_max - P.depth_min) / 64.0f;
+    k.tdss = 2 * ds * ds;
+    const float as = (float)(kPi * (double)(5.0f / 180.0f));
+    k.tass = 2 * as * as;
+    k.beta = 0.18f;
+    return k;
+}
+
+__device__ __forceinline__ float prior_term(const PriorConsts& k, float depth_diff, float4 prior_plane, float4 plane)
+{
+    const float ac = dot3(prior_plane, plane);
+    const float ad = hpm_acosf(ac);
+    return k.gamma + hpm_expf(-depth_diff * depth_diff / k.tdss) * hpm_expf(-ad * ad / k.tass);
+}
+
+/* scan of an axis strip (HPM.cu:1251-1341): start at `start`, up to 4 more
+ * samples `step` apart while `ok(i)`; keeps the minimum, or the maximum
+ * for right_far */
+/* CheckerboardPropagation (HPM.cu:1222-1725) + PlaneHypothesisRefinement
+ * (HPM.cu:1143-1220) */
+__device__ uint32_t propagate_pixel(const hpm_dev_problem& pb, int px, int py, int iter, int colour)
+{
+    const hpm_params& P = pb.params;
+    const hpm_camera& rc = pb.views[0].cam;
+    const int W = pb.W, H = pb.H, nsrc = pb.nsrc;
+    const float* cost = pb.cost;
+    const int center = py * W + px;
+    uint32_t nev = 0;
+    hpm_rng rng;
+    hpm_rng_init(&rng, P.seed_lo, P.seed_hi, (uint32_t)center, hpm_stream_update(pb.call_index, iter, colour),
+                 pb.salt);
+    RefPatch rp;
+    make_ref_patch(pb.images, W, H, px, py, P, &rp);
+
+    int pos[8];
+    bool flag[8];
+    pos[0] = center - 5 - 6 * W;  /* left_up */
+    pos[1] = center - W * 5;      /* up_far */
+    pos[2] = center + 6 - 5 * W;  /* right_up */
+    pos[3] = center + W * 5;      /* down_far */
+    pos[4] = center + 5 + 6 * W;  /* right_down */
+    pos[5] = center - 5;          /* left_far */
+    pos[6] = center - 6 + 5 * W;  /* left_down */
+    pos[7] = center + 5;          /* right_far */
+
+    /* axis strips (HPM.cu:1251-1341) */
+    flag[1] = py > 4;
+    if (flag[1]) {
+        float cmin = cost[pos[1]];
+        int cmp = pos[1];
+        for (int i = 1; i < 5 && py > 4 + 2 * i; ++i) {
+            const int pt = pos[1] - 2 * i * W;
+            if (cost[pt] < cmin) {
+                cmin = cost[pt];
+                cmp = pt;
+            }
+        }
+        pos[1] = cmp;
+    }
+    flag[3] = py < H - 5;
+    if (flag[3]) {
+        float cmin = cost[pos[3]];
+        int cmp = pos[3];
+        for (int i = 1; i < 5 && py < H - 5 - 2 * i; ++i) {
+            const int pt = pos[3] + 2 * i * W;
+            if (cost[pt] < cmin) {
+                cmin = cost[pt];
+                cmp = pt;
+            }
+        }
+        pos[3] = cmp;
+    }
+    flag[5] = px > 4;
+    if (flag[5]) {
+        float cmin = cost[pos[5]];
+        int cmp = pos[5];
+        for (int i = 1; i < 5 && px > 4 + 2 * i; ++i) {
+            const int pt = pos[5] - 2 * i;
+            if (cost[pt] < cmin) {
+                cmin = cost[pt];
+                cmp = pt;
+            }
+        }
+        pos[5] = cmp;
+    }
+    flag[7] = px < W - 5;
+    if (flag[7]) {
+        float cmin = cost[pos[7]];
+        int cmp = pos[7];
+        for (int i = 1; i < 5 && px < W - 5 - 2 * i; ++i) {
+            const int pt = pos[7] + 2 * i;
+            if (cmin < cost[pt]) { /* keeps the maximum (HPM.cu:1330) */
+                cmin = cost[pt];
+                cmp = pt;
+            }
+        }
+        pos[7] = cmp;
+    }
+    /* diagonal zig-zags (HPM.cu:1344-1471): start offset, first step axis,
+     * per-axis step signs */
+    {
+        const int sx0[4] = {-5, 6, 5, -6};
+        const int sy0[4] = {-6, -5, 6, 5};
+        const int first_x[4] = {1, 0, 1, 0}; /* even steps move x (1) or y (0) */
+        const int dxs[4] = {-2, 2, 2, -2};
+        const int dys[4] = {-2, -2, 2, 2};
+        const int slot[4] = {0, 2, 4, 6};
+        flag[0] = px > 4 && py > 5;
+        flag[2] = px < W - 6 && py > 4;
+        flag[4] = px < W - 5 && py < H - 6;
+        flag[6] = px > 5 && py < H - 5;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const int s = slot[d];
+            if (!flag[s]) {
+                continue;
+            }
+            float cmin = cost[pos[s]];
+            int cmp = pos[s];
+            int qx = px + sx0[d], qy = py + sy0[d];
+            for (int i = 0; i < 7; i++) {
+                const bool move_x = ((i % 2 == 0) == (first_x[d] == 1));
+                if (move_x) {
+                    qx += dxs[d];
+                } else {
+                    qy += dys[d];
+                }
+                if (qx >= 0 && qx < W && qy < H && qy >= 0) {
+                    const int pt = qy * W + qx;
+                    if (cost[pt] < cmin) {
+                        cmin = cost[pt];
+                        cmp = pt;
+                    }
+                } else if (d != 3) {
+                    break; /* left_down never breaks (HPM.cu:1461-1470) */
+                }
+            }
+            pos[s] = cmp;
+        }
+    }
+
+    /* cost_array[8][32] = { 2.0f } (HPM.cu:1244) */
+    float ca[8][HPM_MAX_SOURCES];
+    for (int d = 0; d < 8; ++d) {
+        for (int v = 0; v < nsrc; ++v) {
+            ca[d][v] = 0.0f;
+        }
+    }
+    ca[0][0] = 2.0f;
+    for (int d = 0; d < 8; ++d) {
+        if (flag[d]) {
+            cost_vector(pb, rp, px, py, ld4(pb.hyp, pos[d]), ca[d], nullptr, &nev);
+        }
+    }
+
+    /* NESP extension, orientation 0 only (HPM.cu:1473-1543, 780-843) */
+    for (int k = 0; k < 3; ++k) {
+        if (!flag[0]) {
+            break;
+        }
+        {
+            /* JudgeExtend (HPM.cu:1036-1059) */
+            const float good = pb.good[iter][k];
+            int good_sum = 0, bad_sum = 0;
+            for (int i = 0; i < nsrc; ++i) {
+                if (ca[0][i] < good) {
+                    ++good_sum;
+                }
+                if ((double)ca[0][i] > 1.2) {
+                    ++bad_sum;
+                }
+            }
+            if (good_sum >= 1 && bad_sum <= 2) {
+                break;
+            }
+        }
+        int lu = center - 5 - 6 * W - 4 * 2 * k - 4 * 2 * W * k;
+        if (px > 4 + 8 * k && py > 5 + 8 * k) {
+            bool symbol = true;
+            float cmin = cost[lu];
+            int cmp = lu;
+            int qx = px - 5 - 8 * k, qy = py - 6 - 8 * k;
+            for (int i = 0; i < 7; i++) {
+                if (i % 2 == 0) {
+                    qx -= 2;
+                } else {
+                    qy -= 2;
+                }
+                if (qx >= 0 && qx < W && qy < H && qy >= 0) {
+                    const int pt = qy * W + qx;
+                    if (cost[pt] < cmin) {
+                        cmin = cost[pt];
+                        cmp = pt;
+                    }
+                } else {
+                    break;
+                }
+            }
+            if (cmin > cost[pos[0]]) {
+                cmp = pos[0];
+                symbol = false;
+            }
+            lu = cmp;
+            if (symbol && lu != pos[0]) {
+                cost_vector(pb, rp, px, py, ld4(pb.hyp, lu), ca[0], nullptr, &nev);
+            }
+        } else {
+            lu = pos[0];
+        }
+        pos[0] = lu;
+    }
+
+    /* joint view selection (HPM.cu:1548-1613) */
+    float vw[HPM_MAX_SOURCES], probs[HPM_MAX_SOURCES];
+    for (int i = 0; i < nsrc; ++i) {
+        vw[i] = 0.0f;
+        probs[i] = 0.0f;
+    }
+    {
+        float vprior[HPM_MAX_SOURCES];
+        for (int i = 0; i < nsrc; ++i) {
+            vprior[i] = 0.0f;
+        }
+        const int nb[4] = {center - W * 5, center + W * 5, center - 5, center + 5};
+        for (int i = 0; i < 4; ++i) {
+            if (flag[2 * i + 1]) {
+                const uint32_t m = pb.sel[nb[i]];
+                for (int j = 0; j < nsrc; ++j) {
+                    if (((m >> j) & 1u) == 1u) {
+                        vprior[j] += 0.9f;
+                    } else {
+                        vprior[j] += 0.1f;
+                    }
+                }
+            }
+        }
+        const float thr = pb.thr[iter];
+        for (int i = 0; i < nsrc; i++) {
+            float count = 0.0f;
+            int count_false = 0;
+            float tmpw = 0.0f;
+            for (int j = 0; j < 8; j++) {
+                const float c = ca[j][i];
+                if (c < thr) {
+                    tmpw += hpm_expf(c * c / (-0.18f));
+                    count++;
+                }
+                if (c > 1.2f) {
+                    count_false++;
+                }
+            }
+            float pr = 0.0f;
+            if (count > 2 && count_false < 3) {
+                pr = tmpw / count;
+            } else if (count_false < 3) {
+                pr = pb.fallback[iter];
+            }
+            probs[i] = pr * vprior[i];
+        }
+    }
+    {
+        /* TransformPDFToCDF (HPM.cu:86-100) */
+        float sum = 0.0f;
+        for (int i = 0; i < nsrc; ++i) {
+            sum += probs[i];
+        }
+        const float inv = 1.0f / sum;
+        float cum = 0.0f;
+        for (int i = 0; i < nsrc; ++i) {
+            cum += probs[i] * inv;
+            probs[i] = cum;
+        }
+    }
+    for (int s = 0; s < 15; ++s) {
+        const float u = hpm_rng_uniform(&rng) - FLT_EPSILON;
+        for (int v = 0; v < nsrc; ++v) {
+            if (probs[v] > u) {
+                vw[v] += 1.0f;
+                break;
+            }
+        }
+    }
+    uint32_t tsel = 0u;
+    float wn = 0.0f;
+    for (int i = 0; i < nsrc; ++i) {
+        if (vw[i] > 0) {
+            tsel |= (1u << i);
+            wn += vw[i];
+        }
+    }
+
+    /* candidate scoring (HPM.cu:1615-1635) */
+    float fc[8];
+    for (int i = 0; i < 8; ++i) {
+        float acc = 0.0f;
+        float4 hp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (P.geom_consistency && flag[i]) {
+            hp = ld4(pb.hyp, pos[i]);
+        }
+        for (int j = 0; j < nsrc; ++j) {
+            if (vw[j] > 0) {
+                if (P.geom_consistency) {
+                    if (flag[i]) {
+                        acc += vw[j] * (ca[i][j] + 0.1f * geom_cost(pb, j + 1, hp, px, py));
+                    } else {
+                        acc += vw[j] * (ca[i][j] + 0.1f * 5.0f);
+                    }
+                } else {
+                    acc += vw[j] * ca[i][j];
+                }
+            }
+        }
+        fc[i] = acc / wn;
+    }
+    const int min_idx = find_min_idx8(fc);
+
+    /* re-score the current hypothesis (HPM.cu:1637-1650) */
+    const float4 cur = ld4(pb.hyp, center);
+    float cost_now = 0.0f;
+    {
+        float cvn[HPM_MAX_SOURCES];
+        cost_vector(pb, rp, px, py, cur, cvn, vw, &nev);
+        for (int i = 0; i < nsrc; ++i) {
+            if (!(vw[i] > 0.0f)) {
+                continue;
+            }
+            if (P.geom_consistency) {
+                cost_now += vw[i] * (cvn[i] + 0.1f * geom_cost(pb, i + 1, cur, px, py));
+            } else {
+                cost_now += vw[i] * cvn[i];
+            }
+        }
+    }
+    cost_now /= wn;
+    float cost_center = cost_now;
+    float4 plane_center = cur;
+    uint32_t sel_center = pb.sel[center];
+    float depth_now = depth_from_plane(rc, cur, px, py);
+
+    /* adoption (HPM.cu:1652-1722) */
+    float restricted_cost = 0.0f;
+    const PriorConsts pk = prior_consts(P);
+    const bool prior_on = P.planar_prior != 0;
+    const bool prior_pix = prior_on && pb.prior_mask[center] > 0u;
+    float4 prior_plane = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (prior_on) {
+        prior_plane = ld4(pb.prior_planes, center);
+        const float depth_prior = depth_from_plane(rc, prior_plane, px, py);
+        if (prior_pix) {
+            float rfc[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                rfc[i] = 0.0f;
+                if (flag[i]) {
+                    const float4 hp = ld4(pb.hyp, pos[i]);
+                    const float dn = depth_from_plane(rc, hp, px, py);
+                    const float pr = prior_term(pk, dn - depth_prior, prior_plane, hp);
+                    rfc[i] = hpm_expf(-fc[i] * fc[i] / pk.beta) * pr;
+                }
+            }
+            const int max_idx = find_max_idx8(rfc);
+            const float dn = depth_from_plane(rc, cur, px, py);
+            const float pr = prior_term(pk, dn - depth_prior, prior_plane, cur);
+            const float rcn = hpm_expf(-cost_now * cost_now / pk.beta) * pr;
+            if (flag[max_idx]) {
+                const float4 hp = ld4(pb.hyp, pos[max_idx]);
+                const float db = depth_from_plane(rc, hp, px, py);
+                if (db >= P.depth_min && db <= P.depth_max && rfc[max_idx] > rcn) {
+                    depth_now = db;
+                    plane_center = hp;
+                    cost_center = fc[max_idx];
+                    restricted_cost = rfc[max_idx];
+                    sel_center = tsel;
+                }
+            }
+        } else if (flag[min_idx]) {
+            const float4 hp = ld4(pb.hyp, pos[min_idx]);
+            const float db = depth_from_plane(rc, hp, px, py);
+            if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
+                depth_now = db;
+                plane_center = hp;
+                cost_center = fc[min_idx];
+            }
+        }
+    }
+    if (!prior_on && flag[min_idx]) {
+        const float4 hp = ld4(pb.hyp, pos[min_idx]);
+        const float db = depth_from_plane(rc, hp, px, py);
+        if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
+            depth_now = db;
+            plane_center = hp;
+            cost_center = fc[min_idx];
+            sel_center = tsel;
+        }
+        if (cost_center != cost_center) {
+            depth_now = db;
+            plane_center = hp;
+            cost_now = fc[min_idx];
+            sel_center = tsel;
+        }
+    }
+
+    /* PlaneHypothesisRefinement (HPM.cu:1143-1220) */
+    {
+        const float perturbation = 0.02f;
+        float depth_prior = 0.0f;
+        float depth_rand;
+        float4 n_rand;
+        const float ds = (P.depth_max - P.depth_min) / 64.0f;
+        const float as = (float)(kPi * (double)(5.0f / 180.0f));
+        if (prior_pix) {
+            depth_prior = depth_from_plane(rc, prior_plane, px, py);
+            depth_rand = hpm_rng_uniform(&rng) * 6 * ds + (depth_prior - 3 * ds);
+            n_rand = perturbed_normal(rc, px, py, prior_plane, &rng, as);
+        } else {
+            depth_rand = hpm_rng_uniform(&rng) * (P.depth_max - P.depth_min) + P.depth_min;
+            n_rand = random_normal(rc, px, py, &rng, depth_now);
+        }
+        const float dmin_p = (1 - perturbation) * depth_now;
+        const float dmax_p = (1 + perturbation) * depth_now;
+        const float depth_pert = hpm_rng_uniform(&rng) * (dmax_p - dmin_p) + dmin_p;
+        const float4 plane_now = plane_center;
+        const float4 n_pert = perturbed_normal(rc, px, py, plane_now, &rng, (float)((double)perturbation * kPi));
+        const float dnow = depth_now;
+        for (int i = 0; i < 5; ++i) {
+            const float di = (i == 0 || i == 2) ? depth_rand : (i == 4 ? depth_pert : dnow);
+            float4 t = (i == 0 || i == 4) ? plane_now : (i == 3 ? n_pert : n_rand);
+            t.w = dist_to_origin(rc, px, py, di, t);
+            float cvt[HPM_MAX_SOURCES];
+            cost_vector(pb, rp, px, py, t, cvt, vw, &nev);
+            float tc = 0.0f;
+            for (int j = 0; j < nsrc; ++j) {
+                if (vw[j] > 0) {
+                    if (P.geom_consistency) {
+                        tc += vw[j] * (cvt[j] + 0.1f * geom_cost(pb, j + 1, t, px, py));
+                    } else {
+                        tc += vw[j] * cvt[j];
+                    }
+                }
+            }
+            tc /= wn;
+            const float db = depth_from_plane(rc, t, px, py);
+            if (prior_pix) {
+                const float pr = prior_term(pk, di - depth_prior, prior_plane, t);
+                const float rtc = hpm_expf(-tc * tc / pk.beta) * pr;
+                if (db >= P.depth_min && db <= P.depth_max && rtc > restricted_cost) {
+                    depth_now = db;
+                    plane_center = t;
+                    cost_center = tc;
+                    restricted_cost = rtc;
+                }
+            } else {
+                if (db >= P.depth_min && db <= P.depth_max && tc < cost_center) {
+                    depth_now = db;
+                    plane_center = t;
+                    cost_center = tc;
+                }
+            }
+        }
+    }
+    st4(pb.hyp, center, plane_center);
+    pb.cost[center] = cost_center;
+    pb.sel[center] = sel_center;
+    return nev;
+}
+
+/* CheckerboardFilter (HPM.cu:1766-1880) */
+__device__ void filter_pixel(const hpm_dev_problem& pb, int px, int py)
+{
+    const int W = pb.W, H = pb.H;
+    const int center = py * W + px;
+    if (pb.cost[center] < 0.001f) {
+        return;
+    }
+    const float* hyp = pb.hyp;
+    float f[21];
+    int n = 0;
+    f[n++] = hyp[4 * center + 3];
+    const int left = center - 1, leftleft = center - 3;
+    const int up = center - W, upup = center - 3 * W;
+    const int down = center + W, downdown = center + 3 * W;
+    const int right = center + 1, rightright = center + 3;
+#define HPM_TAP(cond, idx)                  \
+    if (cond) {                             \
+        f[n++] = hyp[4 * (idx) + 3];        \
+    }
+    HPM_TAP(py > 0, up)
+    HPM_TAP(py > 2, upup)
+    HPM_TAP(py > 4, upup - W * 2)
+    HPM_TAP(py < H - 1, down)
+    HPM_TAP(py < H - 3, downdown)
+    HPM_TAP(py < H - 5, downdown + W * 2)
+    HPM_TAP(px > 0, left)
+    HPM_TAP(px > 2, leftleft)
+    HPM_TAP(px > 4, leftleft - 2)
+    HPM_TAP(px < W - 1, right)
+    HPM_TAP(px < W - 3, rightright)
+    HPM_TAP(px < W - 5, rightright + 2)
+    HPM_TAP(py > 0 && px < W - 2, up + 2)
+    HPM_TAP(py < H - 1 && px < W - 2, down + 2)
+    HPM_TAP(py > 0 && px > 1, up - 2)
+    HPM_TAP(py < H - 1 && px > 1, down - 2)
+    HPM_TAP(px > 0 && py > 2, left - W * 2)
+    HPM_TAP(px < W - 1 && py > 2, right - W * 2)
+    HPM_TAP(px > 0 && py < H - 2, left + W * 2)
+    HPM_TAP(px < W - 1 && py < H - 2, right + W * 2)
+#undef HPM_TAP
+    for (int i = 1; i < n; i++) {
+        const float tmp = f[i];
+        int j;
+        for (j = i; j >= 1 && tmp < f[j - 1]; j--) {
+            f[j] = f[j - 1];
+        }
+        f[j] = tmp;
+    }
+    const int m = n / 2;
+    pb.hyp[4 * center + 3] = (n % 2 == 0) ? (f[m - 1] + f[m]) / 2 : f[m];
+}
+
+/* pixel of this thread in a colour launch: rows 2*yy + parity, black when
+ * x+y is even (HPM.cu:1727-1749) */
+__device__ __forceinline__ bool colour_pixel(const hpm_dev_problem& pb, int colour, int* px, int* py)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yy = blockIdx.y * blockDim.y + threadIdx.y;
+    const int odd = x & 1;
+    const int y = 2 * yy + (colour == 0 ? odd : 1 - odd);
+    *px = x;
+    *py = y;
+    return yy < pb.yy_limit && x < pb.W && y < pb.H;
+}
+
+} // namespace
+
+__global__ __launch_bounds__(256) void hpm_k_init(hpm_dev_problem pb)
+{
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    const int py = blockIdx.y * blockDim.y + threadIdx.y;
+    uint32_t nev = 0;
+    if (px < pb.W && py < pb.H) {
+        nev = init_pixel(pb, px, py);
+    }
+    count_evals(pb, nev);
+}
+
+__global__ __launch_bounds__(256) void hpm_k_propagate(hpm_dev_problem pb, int iter, int colour)
+{
+    int px, py;
+    uint32_t nev = 0;
+    if (colour_pixel(pb, colour, &px, &py)) {
+        nev = propagate_pixel(pb, px, py, iter, colour);
+    }
+    count_evals(pb, nev);
+}
+
+__global__ __launch_bounds__(256) void hpm_k_finalize(hpm_dev_problem pb)
+{
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    const int py = blockIdx.y * blockDim.y + threadIdx.y;
+    if (px >= pb.W || py >= pb.H) {
+        return;
+    }
+    /* GetDepthandNormal (HPM.cu:1751-1764) */
+    const hpm_camera& rc = pb.views[0].cam;
+    const int center = py * pb.W + px;
+    float4 h = ld4(pb.hyp, center);
+    h.w = depth_from_plane(rc, h, px, py);
+    float4 o;
+    o.x = rc.R[0] * h.x + rc.R[3] * h.y + rc.R[6] * h.z;
+    o.y = rc.R[1] * h.x + rc.R[4] * h.y + rc.R[7] * h.z;
+    o.z = rc.R[2] * h.x + rc.R[5] * h.y + rc.R[8] * h.z;
+    o.w = h.w;
+    st4(pb.hyp, center, o);
+}
+
+__global__ __launch_bounds__(256) void hpm_k_filter(hpm_dev_problem pb, int colour)
+{
+    int px, py;
+    if (colour_pixel(pb, colour, &px, &py)) {
+        filter_pixel(pb, px, py);
+    }
+}
+
+__global__ __launch_bounds__(256) void hpm_k_cost_vectors(hpm_dev_problem pb, const float* planes,
+                                                          const int32_t* pix, int npix, float* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) {
+        return;
+    }
+    const int px = pix[2 * i], py = pix[2 * i + 1];
+    const float4 pl = ld4(planes, i);
+    RefPatch rp;
+    make_ref_patch(pb.images, pb.W, pb.H, px, py, pb.params, &rp);
+    for (int v = 1; v <= pb.nsrc; ++v) {
+        out[(size_t)i * pb.nsrc + v - 1] = ncc(pb, rp, px, py, v, pl);
+    }
+}
+
+/* JBU_cu (HPM.cu:1984-2052) */
+__global__ __launch_bounds__(256) void hpm_k_jbu(const float* guide, int w, int h, const float* src_nd, int sw,
+                                                 int sh, int scale_i, float* out_depth, float* out_normal)
+{
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    const int py = blockIdx.y * blockDim.y + threadIdx.y;
+    if (px >= w || py >= h) {
+        return;
+    }
+    const float scale = (float)(1.0 * sw / w);
+    const float sigmad = 0.50f, sigmar = 25.5f;
+    const int win = scale_i * scale_i + 1;
+    const int nn = win / 2;
+    const float o_y = (float)py * scale;
+    const float o_x = (float)px * scale;
+    const float ref_pix = texel(guide, w, h, px, py);
+    float nf = 0.0f, ctot = 0.0f;
+    float4 ntot = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int j = -nn; j <= nn; ++j) {
+        int r_y = (int)(o_y + (float)j);
+        r_y = (r_y > 0 ? (r_y < sh ? r_y : sh - 1) : 0);
+        const int r_ys = py + j;
+        for (int i = -nn; i <= nn; ++i) {
+            int r_x = (int)(o_x + (float)i);
+            r_x = (r_x > 0 ? (r_x < sw ? r_x : sw - 1) : 0);
+            const int s_center = r_y * sw + r_x;
+            float4 sn = ld4(src_nd, s_center);
+            const float src_pix = sn.w;
+            const float nb_pix = texel(guide, w, h, px + i, r_ys);
+            const float ddx = o_x - (float)r_x, ddy = o_y - (float)r_y;
+            const float dis = ddx * ddx + ddy * ddy;
+            const float sg = hpm_expf(-dis / (2 * sigmad * sigmad));
+            const float xp = fabsf(ref_pix - nb_pix);
+            const float rg = hpm_expf(-(xp * xp) / (2 * sigmar * sigmar));
+            const float tg = sg * rg;
+            nf += tg;
+            ctot += src_pix * tg;
+            sn.x = sn.x * tg;
+            sn.y = sn.y * tg;
+            sn.z = sn.z * tg;
+            ntot.x = ntot.x + sn.x;
+            ntot.y = ntot.y + sn.y;
+            ntot.z = ntot.z + sn.z;
+        }
+    }
+    ntot.x = ntot.x / nf;
+    ntot.y = ntot.y / nf;
+    ntot.z = ntot.z / nf;
+    normalize3(&ntot);
+    const int center = py * w + px;
+    out_depth[center] = ctot / nf;
+    st4(out_normal, center, make_float4(ntot.x, ntot.y, ntot.z, 0.0f));
+}
+
+__global__ __launch_bounds__(256) void hpm_k_selftest(int kind, const void* in, void* out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    if (kind == 4) {
+        const uint32_t* b = static_cast<const uint32_t*>(in) + 6 * (size_t)i;
+        const uint32_t ctr[4] = {b[0], b[1], b[2], b[3]};
+        const uint32_t key[2] = {b[4], b[5]};
+        uint32_t r[4];
+        hpm_philox4x32_10(ctr, key, r);
+        uint32_t* o = static_cast<uint32_t*>(out) + 4 * (size_t)i;
+        o[0] = r[0];
+        o[1] = r[1];
+        o[2] = r[2];
+        o[3] = r[3];
+        return;
+    }
+    const float x = static_cast<const float*>(in)[i];
+    float y;
+    switch (kind) {
+    case 0: y = hpm_expf(x); break;
+    case 1: y = hpm_sinf(x); break;
+    case 2: y = hpm_cosf(x); break;
+    default: y = hpm_acosf(x); break;
+    }
+    static_cast<float*>(out)[i] = y;
+}
+
+/* ---- launchers ---- */
+
+static dim3 full_grid(int W, int H, dim3 blk)
+{
+    return dim3((W + blk.x - 1) / blk.x, (H + blk.y - 1) / blk.y, 1);
+}
+
+static dim3 colour_grid(const hpm_dev_problem* pb, dim3 blk)
+{
+    return dim3((pb->W + blk.x - 1) / blk.x, (pb->yy_limit + blk.y - 1) / blk.y, 1);
+}
+
+extern "C" int hpm_launch_init(const hpm_dev_problem* pb, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_init, full_grid(pb->W, pb->H, blk), blk, 0, (hipStream_t)stream, *pb);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_propagate(const hpm_dev_problem* pb, int iter, int colour, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_propagate, colour_grid(pb, blk), blk, 0, (hipStream_t)stream, *pb, iter, colour);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_finalize(const hpm_dev_problem* pb, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_finalize, full_grid(pb->W, pb->H, blk), blk, 0, (hipStream_t)stream, *pb);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_filter(const hpm_dev_problem* pb, int colour, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_filter, colour_grid(pb, blk), blk, 0, (hipStream_t)stream, *pb, colour);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_cost_vectors(const hpm_dev_problem* pb, const float* planes, const int32_t* pix,
+                                       int npix, float* out, void* stream)
+{
+    if (npix <= 0) {
+        return 0;
+    }
+    const dim3 blk(256, 1, 1);
+    const dim3 grd((npix + 255) / 256, 1, 1);
+    hipLaunchKernelGGL(hpm_k_cost_vectors, grd, blk, 0, (hipStream_t)stream, *pb, planes, pix, npix, out);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_jbu(const float* guide, int w, int h, const float* src_nd, int sw, int sh, int scale_i,
+                              float* out_depth, float* out_normal, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_jbu, full_grid(w, h, blk), blk, 0, (hipStream_t)stream, guide, w, h, src_nd, sw, sh,
+                       scale_i, out_depth, out_normal);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_selftest(int kind, const void* in, void* out, int n, void* stream)
+{
+    if (n <= 0) {
+        return 0;
+    }
+    hipLaunchKernelGGL(hpm_k_selftest, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, kind, in, out, n);
+    return (int)hipGetLastError();
+}

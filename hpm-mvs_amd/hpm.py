@@ -1,0 +1,518 @@
+"""ctypes bindings of libhpm_mvs.so (include/hpm_capi.h, hpm_io.h, hpm_scene.h).
+
+`Problem` mirrors the reference's per-problem class ``HPM``
+(HPM-MVS/HPM.h:48-112) method for method; numpy arrays cross the boundary
+as plain host buffers. The library is the HIP path: if it is not built,
+loading fails loudly (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libhpm_mvs.so")
+
+HPM_MAX_SOURCES = 32
+HPM_MAX_IMAGES = HPM_MAX_SOURCES + 1
+
+
+class HpmCamera(C.Structure):
+    """`Camera` (HPM-MVS/main.h:44-52)."""
+
+    _fields_ = [
+        ("K", C.c_float * 9),
+        ("R", C.c_float * 9),
+        ("t", C.c_float * 3),
+        ("height", C.c_int32),
+        ("width", C.c_int32),
+        ("depth_min", C.c_float),
+        ("depth_max", C.c_float),
+    ]
+
+
+class HpmParams(C.Structure):
+    """`PatchMatchParams` (HPM-MVS/HPM.h:29-46) plus settings the reference hard-codes."""
+
+    _fields_ = [
+        ("max_iterations", C.c_int32),
+        ("patch_size", C.c_int32),
+        ("num_images", C.c_int32),
+        ("max_image_size", C.c_int32),
+        ("radius_increment", C.c_int32),
+        ("sigma_spatial", C.c_float),
+        ("sigma_color", C.c_float),
+        ("top_k", C.c_int32),
+        ("baseline", C.c_float),
+        ("depth_min", C.c_float),
+        ("depth_max", C.c_float),
+        ("disparity_min", C.c_float),
+        ("disparity_max", C.c_float),
+        ("geom_consistency", C.c_int32),
+        ("multi_geometry", C.c_int32),
+        ("planar_prior", C.c_int32),
+        ("knn_k", C.c_int32),
+        ("geom_iterations", C.c_int32),
+        ("seed_lo", C.c_uint32),
+        ("seed_hi", C.c_uint32),
+    ]
+
+
+def default_params() -> HpmParams:
+    """hpm_params_default (include/hpm_types.h)."""
+    p = HpmParams()
+    p.max_iterations = 3
+    p.patch_size = 11
+    p.num_images = 5
+    p.max_image_size = 3200
+    p.radius_increment = 2
+    p.sigma_spatial = 5.0
+    p.sigma_color = 3.0
+    p.top_k = 4
+    p.baseline = 0.54
+    p.depth_min = 0.0
+    p.depth_max = 1.0
+    p.disparity_min = 0.0
+    p.disparity_max = 1.0
+    p.geom_consistency = 0
+    p.multi_geometry = 0
+    p.planar_prior = 0
+    p.knn_k = 6
+    p.geom_iterations = 2
+    p.seed_lo = 0x5EED0001
+    p.seed_hi = 0x00C0FFEE
+    return p
+
+
+class HpmRunStats(C.Structure):
+    _fields_ = [
+        ("n_eval", C.c_uint64),
+        ("n_eval_propagate", C.c_uint64),
+        ("ms_total", C.c_float),
+        ("ms_init", C.c_float),
+        ("ms_propagate", C.c_float),
+        ("ms_finalize_filter", C.c_float),
+        ("launches", C.c_int32),
+    ]
+
+
+class HpmSceneConfig(C.Structure):
+    """hpm_scene_config (include/hpm_scene.h)."""
+
+    _fields_ = [
+        ("seed", C.c_uint32),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("fx", C.c_float),
+        ("fy", C.c_float),
+        ("cx", C.c_float),
+        ("cy", C.c_float),
+        ("num_views", C.c_int32),
+        ("arc_length", C.c_float),
+        ("look_depth", C.c_float),
+        ("num_planes", C.c_int32),
+        ("num_boxes", C.c_int32),
+        ("low_texture_fraction", C.c_float),
+        ("amp_high", C.c_float),
+        ("amp_low", C.c_float),
+        ("noise_sigma", C.c_float),
+        ("pitch_px", C.c_float),
+        ("depth_min", C.c_float),
+        ("depth_max", C.c_float),
+    ]
+
+
+_lib: Optional[C.CDLL] = None
+
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int32)
+_UP = C.POINTER(C.c_uint32)
+_BP = C.POINTER(C.c_uint8)
+
+# name -> (restype, argtypes); every symbol include/*.h declares for the library
+SIGNATURES = {
+    "hpm_create": (C.c_void_p, [C.c_int]),
+    "hpm_destroy": (None, [C.c_void_p]),
+    "hpm_last_error": (C.c_char_p, [C.c_void_p]),
+    "hpm_set_params": (C.c_int, [C.c_void_p, C.POINTER(HpmParams)]),
+    "hpm_get_params": (C.c_int, [C.c_void_p, C.POINTER(HpmParams)]),
+    "hpm_set_geom_consistency": (C.c_int, [C.c_void_p, C.c_int]),
+    "hpm_set_planar_prior": (C.c_int, [C.c_void_p]),
+    "hpm_input_init": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _IP, C.c_int]),
+    "hpm_input_init_memory": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(HpmCamera),
+                                        C.POINTER(_FP), C.POINTER(_FP)]),
+    "hpm_device_init": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "hpm_device_init_state": (C.c_int, [C.c_void_p, _FP, _FP]),
+    "hpm_run_patchmatch": (C.c_int, [C.c_void_p]),
+    "hpm_get_run_stats": (C.c_int, [C.c_void_p, C.POINTER(HpmRunStats)]),
+    "hpm_dims": (C.c_int, [C.c_void_p, _IP, _IP]),
+    "hpm_copy_reference_image": (C.c_int, [C.c_void_p, _FP]),
+    "hpm_get_plane": (C.c_int, [C.c_void_p, C.c_int, _FP]),
+    "hpm_get_cost": (C.c_float, [C.c_void_p, C.c_int]),
+    "hpm_copy_hypotheses": (C.c_int, [C.c_void_p, _FP]),
+    "hpm_copy_costs": (C.c_int, [C.c_void_p, _FP]),
+    "hpm_depth_range": (C.c_int, [C.c_void_p, _FP, _FP]),
+    "hpm_upload_prior": (C.c_int, [C.c_void_p, _FP, _UP]),
+    "hpm_release_prior": (C.c_int, [C.c_void_p]),
+    "hpm_release": (C.c_int, [C.c_void_p]),
+    "hpm_jbu": (C.c_int, [C.c_void_p, _FP, C.c_int, C.c_int, _FP, C.c_int, C.c_int, _FP, _FP]),
+    "hpm_cost_vectors": (C.c_int, [C.c_void_p, _FP, _IP, C.c_int, _FP]),
+    "hpm_debug_stage": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "hpm_debug_read_state": (C.c_int, [C.c_void_p, _FP, _FP, _UP]),
+    "hpm_debug_write_state": (C.c_int, [C.c_void_p, _FP, _FP, _UP]),
+    "hpm_device_selftest": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    # hpm_io.h
+    "hpm_read_camera": (C.c_int, [C.c_char_p, C.POINTER(HpmCamera)]),
+    "hpm_write_camera": (C.c_int, [C.c_char_p, C.POINTER(HpmCamera)]),
+    "hpm_read_pair": (C.c_int, [C.c_char_p, _IP, _IP, _IP, _IP, C.c_int]),
+    "hpm_read_dmb": (C.c_int, [C.c_char_p, _FP, _IP, _IP, _IP]),
+    "hpm_write_dmb": (C.c_int, [C.c_char_p, _FP, C.c_int, C.c_int, C.c_int]),
+    "hpm_read_pgm_f32": (C.c_int, [C.c_char_p, _FP, _IP, _IP]),
+    "hpm_write_pgm_u8": (C.c_int, [C.c_char_p, _BP, C.c_int, C.c_int]),
+    "hpm_resize_bilinear_c": (None, [_FP, C.c_int, C.c_int, C.c_int, _FP, C.c_int, C.c_int]),
+    "hpm_resize_bilinear": (None, [_FP, C.c_int, C.c_int, _FP, C.c_int, C.c_int]),
+    # hpm_scene.h
+    "hpm_scene_default": (None, [C.POINTER(HpmSceneConfig)]),
+    "hpm_scene_render": (C.c_int, [C.POINTER(HpmSceneConfig), C.POINTER(HpmCamera), _BP, _FP, _FP, _BP]),
+    "hpm_scene_write_folder": (C.c_int, [C.POINTER(HpmSceneConfig), C.c_char_p]),
+}
+
+
+def lib() -> C.CDLL:
+    """Loads libhpm_mvs.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() first")
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(_FP)
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(_IP)
+
+
+def uptr(a: np.ndarray):
+    return a.ctypes.data_as(_UP)
+
+
+def bptr(a: np.ndarray):
+    return a.ctypes.data_as(_BP)
+
+
+class HpmError(RuntimeError):
+    pass
+
+
+def camera_from_arrays(K, R, t, width, height, depth_min, depth_max) -> HpmCamera:
+    c = HpmCamera()
+    for i, v in enumerate(np.asarray(K, np.float32).reshape(9)):
+        c.K[i] = float(v)
+    for i, v in enumerate(np.asarray(R, np.float32).reshape(9)):
+        c.R[i] = float(v)
+    for i, v in enumerate(np.asarray(t, np.float32).reshape(3)):
+        c.t[i] = float(v)
+    c.width, c.height = int(width), int(height)
+    c.depth_min, c.depth_max = float(depth_min), float(depth_max)
+    return c
+
+
+class Problem:
+    """One reference-view problem on the GPU: the reference's `class HPM`."""
+
+    def __init__(self, device: int = 0):
+        self._h = lib().hpm_create(device)
+        if not self._h:
+            raise HpmError("hpm_create failed (no HIP device?)")
+        self._keep = []
+
+    def close(self):
+        if self._h:
+            lib().hpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            msg = lib().hpm_last_error(self._h)
+            raise HpmError(f"{what}: {msg.decode() if msg else rc}")
+        return rc
+
+    # --- configuration (HPM.cpp:468-480) ---
+    def set_params(self, p: HpmParams):
+        self._check(lib().hpm_set_params(self._h, C.byref(p)), "hpm_set_params")
+
+    def params(self) -> HpmParams:
+        p = HpmParams()
+        self._check(lib().hpm_get_params(self._h, C.byref(p)), "hpm_get_params")
+        return p
+
+    def SetGeomConsistencyParams(self, multi_geometry: bool = False):
+        self._check(lib().hpm_set_geom_consistency(self._h, int(multi_geometry)), "SetGeomConsistencyParams")
+
+    def SetPlanarPriorParams(self):
+        self._check(lib().hpm_set_planar_prior(self._h), "SetPlanarPriorParams")
+
+    # --- inputs (HPM.cpp:482-700) ---
+    def InuputInitialization(self, dense_folder: str, ref_image_id: int, src_image_ids: Sequence[int]):
+        src = np.ascontiguousarray(src_image_ids, dtype=np.int32)
+        self._check(lib().hpm_input_init(self._h, dense_folder.encode(), int(ref_image_id), iptr(src), len(src)),
+                    "InuputInitialization")
+
+    def input_init_memory(self, cams: Sequence[HpmCamera], images: Sequence[np.ndarray],
+                          depths: Optional[Sequence[np.ndarray]] = None, ref_image_id: int = 0):
+        n = len(cams)
+        cam_arr = (HpmCamera * n)(*cams)
+        imgs = [np.ascontiguousarray(im, dtype=np.float32) for im in images]
+        img_ptrs = (_FP * n)(*[fptr(im) for im in imgs])
+        dep_ptrs = None
+        deps = []
+        if depths is not None:
+            deps = [np.ascontiguousarray(d, dtype=np.float32) for d in depths]
+            dep_ptrs = (_FP * n)(*[fptr(d) for d in deps])
+        self._check(lib().hpm_input_init_memory(self._h, int(ref_image_id), n, cam_arr, img_ptrs, dep_ptrs),
+                    "hpm_input_init_memory")
+
+    def CudaSpaceInitialization(self, dense_folder: Optional[str] = None):
+        self._check(lib().hpm_device_init(self._h, dense_folder.encode() if dense_folder else None),
+                    "CudaSpaceInitialization")
+
+    def device_init_state(self, hyp_world: np.ndarray, cost: np.ndarray):
+        hw = np.ascontiguousarray(hyp_world, dtype=np.float32)
+        cs = np.ascontiguousarray(cost, dtype=np.float32)
+        self._check(lib().hpm_device_init_state(self._h, fptr(hw), fptr(cs)), "hpm_device_init_state")
+
+    # --- run (HPM.cu:1906-1957) ---
+    def RunPatchMatch(self):
+        self._check(lib().hpm_run_patchmatch(self._h), "RunPatchMatch")
+
+    def run_stats(self) -> HpmRunStats:
+        s = HpmRunStats()
+        self._check(lib().hpm_get_run_stats(self._h, C.byref(s)), "hpm_get_run_stats")
+        return s
+
+    # --- getters (HPM.cpp:724-757) ---
+    def dims(self):
+        w, h = C.c_int32(), C.c_int32()
+        self._check(lib().hpm_dims(self._h, C.byref(w), C.byref(h)), "hpm_dims")
+        return w.value, h.value
+
+    def GetReferenceImageWidth(self) -> int:
+        return self.dims()[0]
+
+    def GetReferenceImageHeight(self) -> int:
+        return self.dims()[1]
+
+    def GetReferenceImage(self) -> np.ndarray:
+        w, h = self.dims()
+        out = np.empty((h, w), np.float32)
+        self._check(lib().hpm_copy_reference_image(self._h, fptr(out)), "GetReferenceImage")
+        return out
+
+    def GetPlaneHypothesis(self, index: int) -> np.ndarray:
+        out = np.empty(4, np.float32)
+        self._check(lib().hpm_get_plane(self._h, int(index), fptr(out)), "GetPlaneHypothesis")
+        return out
+
+    def GetCost(self, index: int) -> float:
+        return float(lib().hpm_get_cost(self._h, int(index)))
+
+    def hypotheses(self) -> np.ndarray:
+        w, h = self.dims()
+        out = np.empty((h, w, 4), np.float32)
+        self._check(lib().hpm_copy_hypotheses(self._h, fptr(out)), "hpm_copy_hypotheses")
+        return out
+
+    def costs(self) -> np.ndarray:
+        w, h = self.dims()
+        out = np.empty((h, w), np.float32)
+        self._check(lib().hpm_copy_costs(self._h, fptr(out)), "hpm_copy_costs")
+        return out
+
+    def GetMinDepth(self) -> float:
+        a, b = C.c_float(), C.c_float()
+        self._check(lib().hpm_depth_range(self._h, C.byref(a), C.byref(b)), "hpm_depth_range")
+        return a.value
+
+    def GetMaxDepth(self) -> float:
+        a, b = C.c_float(), C.c_float()
+        self._check(lib().hpm_depth_range(self._h, C.byref(a), C.byref(b)), "hpm_depth_range")
+        return b.value
+
+    # --- prior (HPM.cpp:702-722, 901-921, 1099-1123, 1283-1286) ---
+    def upload_prior(self, planes: np.ndarray, mask: np.ndarray):
+        pl = np.ascontiguousarray(planes, dtype=np.float32)
+        mk = np.ascontiguousarray(mask, dtype=np.uint32)
+        self._check(lib().hpm_upload_prior(self._h, fptr(pl), uptr(mk)), "hpm_upload_prior")
+
+    def ReleasePriorCudaMemory(self):
+        self._check(lib().hpm_release_prior(self._h), "ReleasePriorCudaMemory")
+
+    def ReleaseProblemCudaMemory(self):
+        self._check(lib().hpm_release(self._h), "ReleaseProblemCudaMemory")
+
+    # --- JBU (HPM.cpp:1215-1281) ---
+    def JointBilateralUpsampling(self, guide: np.ndarray, src_nd: np.ndarray):
+        """guide (h, w) grey; src_nd (sh, sw, 4) = world normal xyz + depth.
+        Returns (depth (h, w), normal (h, w, 4)) or None when the scale is 1."""
+        g = np.ascontiguousarray(guide, dtype=np.float32)
+        s = np.ascontiguousarray(src_nd, dtype=np.float32)
+        h, w = g.shape
+        sh, sw = s.shape[:2]
+        od = np.empty((h, w), np.float32)
+        on = np.empty((h, w, 4), np.float32)
+        rc = self._check(lib().hpm_jbu(self._h, fptr(g), w, h, fptr(s), sw, sh, fptr(od), fptr(on)),
+                         "JointBilateralUpsampling")
+        return None if rc == 1 else (od, on)
+
+    # --- checks ---
+    def cost_vectors(self, planes: np.ndarray, pix: np.ndarray) -> np.ndarray:
+        pl = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
+        px = np.ascontiguousarray(pix, dtype=np.int32).reshape(-1, 2)
+        nsrc = self.params().num_images - 1
+        out = np.empty((len(pl), nsrc), np.float32)
+        self._check(lib().hpm_cost_vectors(self._h, fptr(pl), iptr(px), len(pl), fptr(out)), "hpm_cost_vectors")
+        return out
+
+    def debug_stage(self, stage: int, it: int = 0, colour: int = 0):
+        self._check(lib().hpm_debug_stage(self._h, stage, it, colour), "hpm_debug_stage")
+
+    def device_selftest(self, kind: int, data: np.ndarray) -> np.ndarray:
+        if kind == 4:
+            a = np.ascontiguousarray(data, dtype=np.uint32).reshape(-1, 6)
+            out = np.empty((len(a), 4), np.uint32)
+        else:
+            a = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+            out = np.empty(len(a), np.float32)
+        self._check(lib().hpm_device_selftest(self._h, kind, a.ctypes.data_as(C.c_void_p),
+                                              out.ctypes.data_as(C.c_void_p), len(a)), "hpm_device_selftest")
+        return out
+
+    def debug_state(self):
+        w, h = self.dims()
+        hyp = np.empty((h, w, 4), np.float32)
+        cost = np.empty((h, w), np.float32)
+        sel = np.empty((h, w), np.uint32)
+        self._check(lib().hpm_debug_read_state(self._h, fptr(hyp), fptr(cost), uptr(sel)), "hpm_debug_read_state")
+        return hyp, cost, sel
+
+
+# --- I/O (include/hpm_io.h) ---
+
+def read_camera(path: str) -> HpmCamera:
+    c = HpmCamera()
+    if lib().hpm_read_camera(path.encode(), C.byref(c)) != 0:
+        raise HpmError(f"cannot read camera {path}")
+    return c
+
+
+def write_camera(path: str, cam: HpmCamera):
+    if lib().hpm_write_camera(path.encode(), C.byref(cam)) != 0:
+        raise HpmError(f"cannot write camera {path}")
+
+
+def read_dmb(path: str) -> np.ndarray:
+    w, h, nb = C.c_int32(), C.c_int32(), C.c_int32()
+    if lib().hpm_read_dmb(path.encode(), None, C.byref(w), C.byref(h), C.byref(nb)) != 0:
+        raise HpmError(f"cannot read {path}")
+    out = np.empty((h.value, w.value, nb.value), np.float32)
+    if lib().hpm_read_dmb(path.encode(), fptr(out), C.byref(w), C.byref(h), C.byref(nb)) != 0:
+        raise HpmError(f"cannot read {path}")
+    return out[:, :, 0] if nb.value == 1 else out
+
+
+def write_dmb(path: str, data: np.ndarray):
+    a = np.ascontiguousarray(data, dtype=np.float32)
+    nb = 1 if a.ndim == 2 else a.shape[2]
+    if lib().hpm_write_dmb(path.encode(), fptr(a), a.shape[1], a.shape[0], nb) != 0:
+        raise HpmError(f"cannot write {path}")
+
+
+def read_pgm(path: str) -> np.ndarray:
+    w, h = C.c_int32(), C.c_int32()
+    if lib().hpm_read_pgm_f32(path.encode(), None, C.byref(w), C.byref(h)) != 0:
+        raise HpmError(f"cannot read {path}")
+    out = np.empty((h.value, w.value), np.float32)
+    lib().hpm_read_pgm_f32(path.encode(), fptr(out), C.byref(w), C.byref(h))
+    return out
+
+
+def read_pair(path: str):
+    n = C.c_int32()
+    if lib().hpm_read_pair(path.encode(), C.byref(n), None, None, None, 0) != 0:
+        raise HpmError(f"cannot read {path}")
+    refs = np.zeros(n.value, np.int32)
+    ns = np.zeros(n.value, np.int32)
+    srcs = np.zeros((n.value, HPM_MAX_SOURCES), np.int32)
+    if lib().hpm_read_pair(path.encode(), C.byref(n), iptr(refs), iptr(ns), iptr(srcs), HPM_MAX_SOURCES) != 0:
+        raise HpmError(f"cannot read {path}")
+    return [(int(refs[i]), [int(v) for v in srcs[i, : ns[i]]]) for i in range(n.value)]
+
+
+def resize_bilinear(src: np.ndarray, dw: int, dh: int) -> np.ndarray:
+    a = np.ascontiguousarray(src, dtype=np.float32)
+    ch = 1 if a.ndim == 2 else a.shape[2]
+    out = np.empty((dh, dw) + (() if a.ndim == 2 else (ch,)), np.float32)
+    lib().hpm_resize_bilinear_c(fptr(a), a.shape[1], a.shape[0], ch, fptr(out), dw, dh)
+    return out
+
+
+# --- synthetic scenes (include/hpm_scene.h) ---
+
+def scene_config(**kw) -> HpmSceneConfig:
+    cfg = HpmSceneConfig()
+    lib().hpm_scene_default(C.byref(cfg))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def workload_config(name: str) -> HpmSceneConfig:
+    """Scene settings of the two plain-method workloads:
+    'c1': 1 ref + 4 src, 640x480, 4 planes (fx=fy=500);
+    'c2': 11 views, 1600x1066, 20 planes + 50 boxes, 30% low texture."""
+    if name == "c1":
+        return scene_config(seed=1)
+    if name == "c2":
+        return scene_config(seed=2, width=1600, height=1066, fx=1250.0, fy=1250.0, cx=800.0, cy=533.0,
+                            num_views=11, arc_length=0.5, num_planes=20, num_boxes=50,
+                            low_texture_fraction=0.3)
+    raise ValueError(name)
+
+
+def render_scene(cfg: HpmSceneConfig, want_gt: bool = True):
+    """Returns (cams list, images (n, H, W) uint8, gt_depth (n, H, W) | None,
+    gt_normal (n, H, W, 3) | None, low_texture (H, W) uint8 of the middle view)."""
+    n, H, W = cfg.num_views, cfg.height, cfg.width
+    cams = (HpmCamera * n)()
+    imgs = np.empty((n, H, W), np.uint8)
+    gd = np.empty((n, H, W), np.float32) if want_gt else None
+    gn = np.empty((n, H, W, 3), np.float32) if want_gt else None
+    low = np.empty((H, W), np.uint8)
+    rc = lib().hpm_scene_render(C.byref(cfg), cams, bptr(imgs), fptr(gd) if want_gt else None,
+                                fptr(gn) if want_gt else None, bptr(low))
+    if rc != 0:
+        raise HpmError("hpm_scene_render failed")
+    return list(cams), imgs, gd, gn, low
+
+
+def write_scene_folder(cfg: HpmSceneConfig, folder: str):
+    if lib().hpm_scene_write_folder(C.byref(cfg), folder.encode()) != 0:
+        raise HpmError(f"hpm_scene_write_folder failed for {folder}")

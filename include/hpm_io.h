@@ -1,0 +1,59 @@
+/*
+ * hpm_io.h — the reference's on-disk formats, without OpenCV.
+ *
+ *   cams/%08d_cam.txt  ReadCamera (HPM-MVS/HPM.cpp:160-185): "extrinsic",
+ *                      4x4 [R|t], "intrinsic", 3x3 K, then
+ *                      depth_min interval depth_num depth_max.
+ *   pair.txt           GenerateSampleList (HPM-MVS/main.cpp:4-33): N, then
+ *                      per view: ref_id, n, n x (id score); score <= 0 skipped.
+ *   *.dmb              readDepthDmb/writeDepthDmb/readNormalDmb/writeNormalDmb
+ *                      (HPM-MVS/HPM.cpp:270-401): int32 type=1, h, w, nb, then
+ *                      h*w*nb float32 row-major.
+ *   images             8-bit grey, PGM (P5) here; the reference decodes JPEG
+ *                      with cv::imread(IMREAD_GRAYSCALE) (HPM.cpp:492).
+ * All functions return 0 on success and a negative value on failure; none
+ * exits the process.
+ */
+#ifndef HPM_IO_H_
+#define HPM_IO_H_
+
+#include <stdint.h>
+
+#include "hpm_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ReadCamera (HPM.cpp:160-185); width/height are left 0. */
+int hpm_read_camera(const char* path, hpm_camera* cam);
+/* Writer of the same layout (interval 0, depth_num 192). */
+int hpm_write_camera(const char* path, const hpm_camera* cam);
+
+/* GenerateSampleList (main.cpp:4-33). Call with ref_ids == NULL to get the
+ * problem count. src_ids is max_src entries per problem; sources whose
+ * score is <= 0 are dropped as in the reference. */
+int hpm_read_pair(const char* path, int* num_problems, int* ref_ids, int* num_src, int* src_ids, int max_src);
+
+/* read/write{Depth,Normal}Dmb (HPM.cpp:270-401). Call with data == NULL to
+ * read only the header. type != 1 is an error (HPM.cpp:315-318). */
+int hpm_read_dmb(const char* path, float* data, int* width, int* height, int* channels);
+int hpm_write_dmb(const char* path, const float* data, int width, int height, int channels);
+
+/* 8-bit binary PGM (P5). hpm_read_pgm_f32 converts to float (the
+ * convertTo(CV_32FC1) of HPM.cpp:494); data == NULL reads only the size. */
+int hpm_read_pgm_f32(const char* path, float* data, int* width, int* height);
+int hpm_write_pgm_u8(const char* path, const uint8_t* data, int width, int height);
+
+/* cv::resize(..., INTER_LINEAR) of a float image: half-pixel centres,
+ * src = (dst + 0.5) * (sw / dw) - 0.5, edge-clamped, exact float weights.
+ * Used for the >max_image_size downscale (HPM.cpp:524-551) and the prior
+ * hierarchy (main.cpp:84-86, 238-240). channels interleaved. */
+void hpm_resize_bilinear_c(const float* src, int sw, int sh, int channels, float* dst, int dw, int dh);
+void hpm_resize_bilinear(const float* src, int sw, int sh, float* dst, int dw, int dh);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPM_IO_H_ */

@@ -1,0 +1,1569 @@
+/*
+ * hpm_oracle.cpp — CPU (OpenMP) restatement of HPM-MVS's per-reference-view
+ * PatchMatch. TEST INFRASTRUCTURE: the checker for the HIP path, never part
+ * of the product.
+ *
+ * Every function cites the reference text it restates. Where the reference
+ * depends on CUDA hardware behaviour that cannot be reproduced (fast-math
+ * exp/sin/cos/rsqrt, 9-bit texture filter weights, cuRAND XORWOW seeded from
+ * clock64()), this file and the HIP kernels implement one shared definition
+ * instead:
+ *   - exp/sin/cos/acos: hpm_fmath.h; 1/sqrt(x) as 1.0f / sqrtf(x);
+ *   - bilinear sampling with exact FP32 weights, clamp-to-edge addressing,
+ *     texel centres at integer coordinates (the reference samples at +0.5);
+ *   - random numbers: Philox4x32-10 (hpm_philox.h), one counter stream per
+ *     (pixel, launch);
+ *   - warped tap coordinates from fma-evaluated homogeneous coordinates and
+ *     one reciprocal per tap;
+ *   - NCC window sums accumulated relative to the reference centre value
+ *     (NCC is invariant to a common shift; this keeps FP32 accurate on
+ *     low-texture patches).
+ * The file must be compiled with -ffp-contract=off and without fast-math.
+ */
+#include "hpm_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include <omp.h>
+
+#include "hpm_fmath.h"
+#include "hpm_philox.h"
+
+namespace {
+
+struct F4 {
+    float x, y, z, w;
+};
+
+static inline F4 ld4(const float* a, int i)
+{
+    F4 r = {a[4 * i + 0], a[4 * i + 1], a[4 * i + 2], a[4 * i + 3]};
+    return r;
+}
+
+static inline void st4(float* a, int i, F4 v)
+{
+    a[4 * i + 0] = v.x;
+    a[4 * i + 1] = v.y;
+    a[4 * i + 2] = v.z;
+    a[4 * i + 3] = v.w;
+}
+
+static inline int clampi(int v, int lo, int hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+static const double kPi = 3.14159265358979323846; /* main.h:41 */
+
+/* Per-iteration thresholds. JudgeExtend (HPM.cu:1040-1041) evaluates
+ * 0.8*exp(-iter^2*(3-k)/90) in double; view selection (HPM.cu:1568,1586)
+ * evaluates 0.8*expf(iter^2/-90) and expf(thr^2/-0.32) in float. */
+struct IterConsts {
+    float good[3][3];   /* [iter][k] */
+    float thr[3];       /* view-selection cost threshold */
+    float fallback[3];  /* expf(thr^2 / -0.32) */
+};
+
+static IterConsts make_iter_consts()
+{
+    IterConsts c;
+    for (int it = 0; it < 3; ++it) {
+        for (int k = 0; k < 3; ++k) {
+            const int iter_tmp = 3 - k;
+            c.good[it][k] = (float)(0.8 * exp((double)(-it * it * iter_tmp) / 90.0));
+        }
+        const float e = hpm_expf((float)(it * it) / (-90.0f));
+        c.thr[it] = (float)(0.8 * (double)e);
+        c.fallback[it] = hpm_expf(c.thr[it] * c.thr[it] / (-0.32f));
+    }
+    return c;
+}
+
+/* Camera-only part of ComputeHomography (HPM.cu:229-255). */
+struct ViewRel {
+    float Rrel[9];
+    float trel[3];
+};
+
+static ViewRel make_view_rel(const hpm_camera& rc, const hpm_camera& sc)
+{
+    ViewRel v;
+    float ref_C[3], src_C[3], C_rel[3];
+    ref_C[0] = -(rc.R[0] * rc.t[0] + rc.R[3] * rc.t[1] + rc.R[6] * rc.t[2]);
+    ref_C[1] = -(rc.R[1] * rc.t[0] + rc.R[4] * rc.t[1] + rc.R[7] * rc.t[2]);
+    ref_C[2] = -(rc.R[2] * rc.t[0] + rc.R[5] * rc.t[1] + rc.R[8] * rc.t[2]);
+    src_C[0] = -(sc.R[0] * sc.t[0] + sc.R[3] * sc.t[1] + sc.R[6] * sc.t[2]);
+    src_C[1] = -(sc.R[1] * sc.t[0] + sc.R[4] * sc.t[1] + sc.R[7] * sc.t[2]);
+    src_C[2] = -(sc.R[2] * sc.t[0] + sc.R[5] * sc.t[1] + sc.R[8] * sc.t[2]);
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) {
+            v.Rrel[3 * r + c] = sc.R[3 * r + 0] * rc.R[3 * c + 0] + sc.R[3 * r + 1] * rc.R[3 * c + 1] +
+                                sc.R[3 * r + 2] * rc.R[3 * c + 2];
+        }
+    }
+    C_rel[0] = ref_C[0] - src_C[0];
+    C_rel[1] = ref_C[1] - src_C[1];
+    C_rel[2] = ref_C[2] - src_C[2];
+    for (int r = 0; r < 3; ++r) {
+        v.trel[r] = sc.R[3 * r + 0] * C_rel[0] + sc.R[3 * r + 1] * C_rel[1] + sc.R[3 * r + 2] * C_rel[2];
+    }
+    return v;
+}
+
+/* Plane-dependent part of ComputeHomography (HPM.cu:257-286). */
+static void homography(const hpm_camera& rc, const hpm_camera& sc, const ViewRel& v, F4 n, float H[9])
+{
+    float h[9], tmp[9];
+    const float nn[3] = {n.x, n.y, n.z};
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) {
+            h[3 * r + c] = v.Rrel[3 * r + c] - v.trel[r] * nn[c] / n.w;
+        }
+    }
+    for (int r = 0; r < 3; ++r) {
+        tmp[3 * r + 0] = h[3 * r + 0] / rc.K[0];
+        tmp[3 * r + 1] = h[3 * r + 1] / rc.K[4];
+        tmp[3 * r + 2] = -h[3 * r + 0] * rc.K[2] / rc.K[0] - h[3 * r + 1] * rc.K[5] / rc.K[4] + h[3 * r + 2];
+    }
+    H[0] = sc.K[0] * tmp[0] + sc.K[2] * tmp[6];
+    H[1] = sc.K[0] * tmp[1] + sc.K[2] * tmp[7];
+    H[2] = sc.K[0] * tmp[2] + sc.K[2] * tmp[8];
+    H[3] = sc.K[4] * tmp[3] + sc.K[5] * tmp[6];
+    H[4] = sc.K[4] * tmp[4] + sc.K[5] * tmp[7];
+    H[5] = sc.K[4] * tmp[5] + sc.K[5] * tmp[8];
+    H[6] = sc.K[8] * tmp[6];
+    H[7] = sc.K[8] * tmp[7];
+    H[8] = sc.K[8] * tmp[8];
+}
+
+/* ComputeCorrespondingPoint (HPM.cu:289-296), fma form with one reciprocal. */
+static inline void warp(const float H[9], float X, float Y, float* sx, float* sy)
+{
+    const float px = fmaf(H[0], X, fmaf(H[1], Y, H[2]));
+    const float py = fmaf(H[3], X, fmaf(H[4], Y, H[5]));
+    const float pz = fmaf(H[6], X, fmaf(H[7], Y, H[8]));
+    const float inv = 1.0f / pz;
+    *sx = px * inv;
+    *sy = py * inv;
+}
+
+/* tex2D with cudaFilterModeLinear, clamp addressing (HPM.cpp:599-617);
+ * (x, y) are texel-centre coordinates (reference coordinate minus 0.5). */
+static inline float bilinear(const float* img, int W, int H, float x, float y)
+{
+    x = fminf(fmaxf(x, -1.0f), (float)W);
+    y = fminf(fmaxf(y, -1.0f), (float)H);
+    const float fx = floorf(x);
+    const float fy = floorf(y);
+    const float a = x - fx;
+    const float b = y - fy;
+    const int ix = (int)fx;
+    const int iy = (int)fy;
+    const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
+    const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
+    const float t00 = img[(size_t)y0 * W + x0];
+    const float t10 = img[(size_t)y0 * W + x1];
+    const float t01 = img[(size_t)y1 * W + x0];
+    const float t11 = img[(size_t)y1 * W + x1];
+    const float v0 = fmaf(a, t10 - t00, t00);
+    const float v1 = fmaf(a, t11 - t01, t01);
+    return fmaf(b, v1 - v0, v0);
+}
+
+static inline float texel(const float* img, int W, int H, int x, int y)
+{
+    return img[(size_t)clampi(y, 0, H - 1) * W + clampi(x, 0, W - 1)];
+}
+
+/* Reference-only part of ComputeBilateralNCC (HPM.cu:337-369): the window
+ * taps, their bilateral weights (ComputeBilateralWeight, HPM.cu:318-323) and
+ * the reference sums, all relative to the centre value. */
+struct RefPatch {
+    int ntap;
+    float dx[64], dy[64];
+    float w[64], rr[64], wr[64];
+    float c;
+    float inv_w;
+    float mr;
+    float var_r;
+};
+
+static void make_ref_patch(const float* img, int W, int H, int px, int py, const hpm_params& P, RefPatch* rp)
+{
+    const int radius = P.patch_size / 2;
+    const float c = texel(img, W, H, px, py);
+    float wsum = 0.0f, sr = 0.0f, srr = 0.0f;
+    int k = 0;
+    for (int i = -radius; i < radius + 1; i += P.radius_increment) {
+        for (int j = -radius; j < radius + 1; j += P.radius_increment) {
+            const float r = texel(img, W, H, px + i, py + j);
+            const float fi = (float)i, fj = (float)j;
+            const float sd = sqrtf(fi * fi + fj * fj);
+            const float cd = fabsf(r - c);
+            const float w = hpm_expf(-sd / (2.0f * P.sigma_spatial * P.sigma_spatial) -
+                                     cd / (2.0f * P.sigma_color * P.sigma_color));
+            const float rr = r - c;
+            const float wr = w * rr;
+            rp->dx[k] = (float)(px + i);
+            rp->dy[k] = (float)(py + j);
+            rp->w[k] = w;
+            rp->rr[k] = rr;
+            rp->wr[k] = wr;
+            wsum = wsum + w;
+            sr = sr + wr;
+            srr = fmaf(wr, rr, srr);
+            ++k;
+        }
+    }
+    rp->ntap = k;
+    rp->c = c;
+    rp->inv_w = 1.0f / wsum;
+    rp->mr = sr * rp->inv_w;
+    rp->var_r = srr * rp->inv_w - rp->mr * rp->mr;
+}
+
+struct Ctx {
+    const hpm_oracle_problem* pb;
+    int W, H, nsrc;
+    ViewRel rel[HPM_MAX_IMAGES];
+    IterConsts ic;
+    float yy_limit; /* unused placeholder for symmetry */
+};
+
+/* ComputeBilateralNCC (HPM.cu:325-397) for one source view, FP32. */
+static float ncc(const Ctx& cx, const RefPatch& rp, int px, int py, int v, F4 plane)
+{
+    const hpm_oracle_problem* pb = cx.pb;
+    const hpm_camera& rc = pb->cams[0];
+    const hpm_camera& sc = pb->cams[v];
+    const int SW = sc.width, SH = sc.height;
+    const float* simg = pb->images[v];
+    float H[9];
+    homography(rc, sc, cx.rel[v], plane, H);
+    float cx0, cy0;
+    warp(H, (float)px, (float)py, &cx0, &cy0);
+    if (cx0 >= (float)SW || cx0 < 0.0f || cy0 >= (float)SH || cy0 < 0.0f) {
+        return 2.0f;
+    }
+    float ss_ = 0.0f, sss = 0.0f, srs = 0.0f;
+    for (int k = 0; k < rp.ntap; ++k) {
+        float sx, sy;
+        warp(H, rp.dx[k], rp.dy[k], &sx, &sy);
+        const float s = bilinear(simg, SW, SH, sx, sy) - rp.c;
+        const float ws = rp.w[k] * s;
+        ss_ = ss_ + ws;
+        sss = fmaf(ws, s, sss);
+        srs = fmaf(rp.wr[k], s, srs);
+    }
+    const float ms = ss_ * rp.inv_w;
+    const float var_s = sss * rp.inv_w - ms * ms;
+    const float kMinVar = 1e-5f;
+    if (rp.var_r < kMinVar || var_s < kMinVar) {
+        return 2.0f;
+    }
+    const float cov = srs * rp.inv_w - rp.mr * ms;
+    const float den = sqrtf(rp.var_r * var_s);
+    return fmaxf(0.0f, fminf(2.0f, 1.0f - cov / den));
+}
+
+/* ComputeBilateralNCC in double precision, reference formula (raw sums). */
+static double ncc64(const hpm_oracle_problem* pb, const Ctx& cx, int px, int py, int v, F4 plane)
+{
+    (void)cx;
+    const hpm_camera& rc = pb->cams[0];
+    const hpm_camera& sc = pb->cams[v];
+    const hpm_params& P = pb->params;
+    const int RW = rc.width, RH = rc.height;
+    const int SW = sc.width, SH = sc.height;
+    const float* rimg = pb->images[0];
+    const float* simg = pb->images[v];
+    /* homography in double from the same float inputs */
+    double refC[3], srcC[3], Rr[9], Cr[3], tr[3], h[9], tmp[9], H[9];
+    for (int a = 0; a < 3; ++a) {
+        refC[a] = -((double)rc.R[a] * rc.t[0] + (double)rc.R[3 + a] * rc.t[1] + (double)rc.R[6 + a] * rc.t[2]);
+        srcC[a] = -((double)sc.R[a] * sc.t[0] + (double)sc.R[3 + a] * sc.t[1] + (double)sc.R[6 + a] * sc.t[2]);
+    }
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) {
+            Rr[3 * r + c] = (double)sc.R[3 * r] * rc.R[3 * c] + (double)sc.R[3 * r + 1] * rc.R[3 * c + 1] +
+                            (double)sc.R[3 * r + 2] * rc.R[3 * c + 2];
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        Cr[a] = refC[a] - srcC[a];
+    }
+    for (int r = 0; r < 3; ++r) {
+        tr[r] = (double)sc.R[3 * r] * Cr[0] + (double)sc.R[3 * r + 1] * Cr[1] + (double)sc.R[3 * r + 2] * Cr[2];
+    }
+    const double nn[3] = {plane.x, plane.y, plane.z};
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) {
+            h[3 * r + c] = Rr[3 * r + c] - tr[r] * nn[c] / (double)plane.w;
+        }
+    }
+    for (int r = 0; r < 3; ++r) {
+        tmp[3 * r + 0] = h[3 * r + 0] / rc.K[0];
+        tmp[3 * r + 1] = h[3 * r + 1] / rc.K[4];
+        tmp[3 * r + 2] = -h[3 * r + 0] * rc.K[2] / rc.K[0] - h[3 * r + 1] * rc.K[5] / rc.K[4] + h[3 * r + 2];
+    }
+    H[0] = sc.K[0] * tmp[0] + sc.K[2] * tmp[6];
+    H[1] = sc.K[0] * tmp[1] + sc.K[2] * tmp[7];
+    H[2] = sc.K[0] * tmp[2] + sc.K[2] * tmp[8];
+    H[3] = sc.K[4] * tmp[3] + sc.K[5] * tmp[6];
+    H[4] = sc.K[4] * tmp[4] + sc.K[5] * tmp[7];
+    H[5] = sc.K[4] * tmp[5] + sc.K[5] * tmp[8];
+    H[6] = sc.K[8] * tmp[6];
+    H[7] = sc.K[8] * tmp[7];
+    H[8] = sc.K[8] * tmp[8];
+    auto wp = [&](double X, double Y, double* sx, double* sy) {
+        const double z = H[6] * X + H[7] * Y + H[8];
+        *sx = (H[0] * X + H[1] * Y + H[2]) / z;
+        *sy = (H[3] * X + H[4] * Y + H[5]) / z;
+    };
+    auto bil = [&](double x, double y) {
+        x = std::min(std::max(x, -1.0), (double)SW);
+        y = std::min(std::max(y, -1.0), (double)SH);
+        const double fx = floor(x), fy = floor(y);
+        const double a = x - fx, b = y - fy;
+        const int ix = (int)fx, iy = (int)fy;
+        const int x0 = clampi(ix, 0, SW - 1), x1 = clampi(ix + 1, 0, SW - 1);
+        const int y0 = clampi(iy, 0, SH - 1), y1 = clampi(iy + 1, 0, SH - 1);
+        const double t00 = simg[(size_t)y0 * SW + x0], t10 = simg[(size_t)y0 * SW + x1];
+        const double t01 = simg[(size_t)y1 * SW + x0], t11 = simg[(size_t)y1 * SW + x1];
+        return (1 - a) * (1 - b) * t00 + a * (1 - b) * t10 + (1 - a) * b * t01 + a * b * t11;
+    };
+    double cxp, cyp;
+    wp(px, py, &cxp, &cyp);
+    if (cxp >= SW || cxp < 0.0 || cyp >= SH || cyp < 0.0) {
+        return 2.0;
+    }
+    const int radius = P.patch_size / 2;
+    const double c = texel(rimg, RW, RH, px, py);
+    double sw = 0, sr = 0, srr = 0, ss = 0, sss = 0, srs = 0;
+    for (int i = -radius; i < radius + 1; i += P.radius_increment) {
+        for (int j = -radius; j < radius + 1; j += P.radius_increment) {
+            const double r = texel(rimg, RW, RH, px + i, py + j);
+            double sx, sy;
+            wp(px + i, py + j, &sx, &sy);
+            const double s = bil(sx, sy);
+            const double w = exp(-sqrt((double)(i * i + j * j)) / (2.0 * P.sigma_spatial * P.sigma_spatial) -
+                                 fabs(r - c) / (2.0 * P.sigma_color * P.sigma_color));
+            sw += w;
+            sr += w * r;
+            srr += w * r * r;
+            ss += w * s;
+            sss += w * s * s;
+            srs += w * r * s;
+        }
+    }
+    sr /= sw;
+    srr /= sw;
+    ss /= sw;
+    sss /= sw;
+    srs /= sw;
+    const double vr = srr - sr * sr, vs = sss - ss * ss;
+    if (vr < 1e-5 || vs < 1e-5) {
+        return 2.0;
+    }
+    const double cost = 1.0 - (srs - sr * ss) / sqrt(vr * vs);
+    return std::max(0.0, std::min(2.0, cost));
+}
+
+/* Get3DPoint + GetViewDirection + GetDistance2Origin +
+ * ComputeDepthfromPlaneHypothesis (HPM.cu:102-133). */
+static inline void point3(const hpm_camera& c, int px, int py, float depth, float X[3])
+{
+    X[0] = depth * ((float)px - c.K[2]) / c.K[0];
+    X[1] = depth * ((float)py - c.K[5]) / c.K[4];
+    X[2] = depth;
+}
+
+static inline F4 view_direction(const hpm_camera& c, int px, int py, float depth)
+{
+    float X[3];
+    point3(c, px, py, depth, X);
+    const float norm = sqrtf(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+    F4 v = {X[0] / norm, X[1] / norm, X[2] / norm, 0.0f};
+    return v;
+}
+
+static inline float dist_to_origin(const hpm_camera& c, int px, int py, float depth, F4 n)
+{
+    float X[3];
+    point3(c, px, py, depth, X);
+    return -(n.x * X[0] + n.y * X[1] + n.z * X[2]);
+}
+
+static inline float depth_from_plane(const hpm_camera& c, F4 pl, int px, int py)
+{
+    return -pl.w * c.K[0] /
+           (((float)px - c.K[2]) * pl.x + (c.K[0] / c.K[4]) * ((float)py - c.K[5]) * pl.y + c.K[0] * pl.z);
+}
+
+/* NormalizeVec3 (HPM.cu:77-84); rsqrtf replaced by 1/sqrtf. */
+static inline void normalize3(F4* v)
+{
+    const float n2 = v->x * v->x + v->y * v->y + v->z * v->z;
+    const float inv = 1.0f / sqrtf(n2);
+    v->x *= inv;
+    v->y *= inv;
+    v->z *= inv;
+}
+
+static inline float dot3(F4 a, F4 b)
+{
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+/* GenerateRandomNormal (HPM.cu:135-161). */
+static F4 random_normal(const hpm_camera& c, int px, int py, hpm_rng* rng, float depth)
+{
+    float q1 = 1.0f, q2 = 1.0f, s = 2.0f;
+    while (s >= 1.0f) {
+        q1 = 2.0f * hpm_rng_uniform(rng) - 1.0f;
+        q2 = 2.0f * hpm_rng_uniform(rng) - 1.0f;
+        s = q1 * q1 + q2 * q2;
+    }
+    const float sq = sqrtf(1.0f - s);
+    F4 n = {2.0f * q1 * sq, 2.0f * q2 * sq, 1.0f - 2.0f * s, 0.0f};
+    const F4 vd = view_direction(c, px, py, depth);
+    const float d = n.x * vd.x + n.y * vd.y + n.z * vd.z;
+    if (d > 0.0f) {
+        n.x = -n.x;
+        n.y = -n.y;
+        n.z = -n.z;
+    }
+    normalize3(&n);
+    return n;
+}
+
+/* GeneratePerturbedNormal (HPM.cu:163-198). */
+static F4 perturbed_normal(const hpm_camera& c, int px, int py, F4 normal, hpm_rng* rng, float perturbation)
+{
+    const F4 vd = view_direction(c, px, py, 1.0f);
+    const float a1 = (hpm_rng_uniform(rng) - 0.5f) * perturbation;
+    const float a2 = (hpm_rng_uniform(rng) - 0.5f) * perturbation;
+    const float a3 = (hpm_rng_uniform(rng) - 0.5f) * perturbation;
+    const float s1 = hpm_sinf(a1), s2 = hpm_sinf(a2), s3 = hpm_sinf(a3);
+    const float c1 = hpm_cosf(a1), c2 = hpm_cosf(a2), c3 = hpm_cosf(a3);
+    float R[9];
+    R[0] = c2 * c3;
+    R[1] = c3 * s1 * s2 - c1 * s3;
+    R[2] = s1 * s3 + c1 * c3 * s2;
+    R[3] = c2 * s3;
+    R[4] = c1 * c3 + s1 * s2 * s3;
+    R[5] = c1 * s2 * s3 - c3 * s1;
+    R[6] = -s2;
+    R[7] = c2 * s1;
+    R[8] = c1 * c2;
+    F4 np;
+    np.x = R[0] * normal.x + R[1] * normal.y + R[2] * normal.z;
+    np.y = R[3] * normal.x + R[4] * normal.y + R[5] * normal.z;
+    np.z = R[6] * normal.x + R[7] * normal.y + R[8] * normal.z;
+    np.w = normal.w;
+    if (dot3(np, vd) >= 0.0f) {
+        np = normal;
+    }
+    normalize3(&np);
+    return np;
+}
+
+/* Get3DPointonWorld_cu / ProjectonCamera_cu (HPM.cu:445-481). */
+static inline void point_world(float x, float y, float depth, const hpm_camera& c, float out[3])
+{
+    float p[3], t[3], C[3];
+    p[0] = depth * (x - c.K[2]) / c.K[0];
+    p[1] = depth * (y - c.K[5]) / c.K[4];
+    p[2] = depth;
+    t[0] = c.R[0] * p[0] + c.R[3] * p[1] + c.R[6] * p[2];
+    t[1] = c.R[1] * p[0] + c.R[4] * p[1] + c.R[7] * p[2];
+    t[2] = c.R[2] * p[0] + c.R[5] * p[1] + c.R[8] * p[2];
+    C[0] = -(c.R[0] * c.t[0] + c.R[3] * c.t[1] + c.R[6] * c.t[2]);
+    C[1] = -(c.R[1] * c.t[0] + c.R[4] * c.t[1] + c.R[7] * c.t[2]);
+    C[2] = -(c.R[2] * c.t[0] + c.R[5] * c.t[1] + c.R[8] * c.t[2]);
+    out[0] = t[0] + C[0];
+    out[1] = t[1] + C[1];
+    out[2] = t[2] + C[2];
+}
+
+static inline void project(const float X[3], const hpm_camera& c, float* u, float* v, float* d)
+{
+    float t[3];
+    t[0] = c.R[0] * X[0] + c.R[1] * X[1] + c.R[2] * X[2] + c.t[0];
+    t[1] = c.R[3] * X[0] + c.R[4] * X[1] + c.R[5] * X[2] + c.t[1];
+    t[2] = c.R[6] * X[0] + c.R[7] * X[1] + c.R[8] * X[2] + c.t[2];
+    *d = c.K[6] * t[0] + c.K[7] * t[1] + c.K[8] * t[2];
+    *u = (c.K[0] * t[0] + c.K[1] * t[1] + c.K[2] * t[2]) / *d;
+    *v = (c.K[3] * t[0] + c.K[4] * t[1] + c.K[5] * t[2]) / *d;
+}
+
+/* ComputeGeomConsistencyCost (HPM.cu:483-508). */
+static float geom_cost(const Ctx& cx, int v, F4 plane, int px, int py)
+{
+    const hpm_oracle_problem* pb = cx.pb;
+    const hpm_camera& rc = pb->cams[0];
+    const hpm_camera& sc = pb->cams[v];
+    const float depth = depth_from_plane(rc, plane, px, py);
+    float fw[3];
+    point_world((float)px, (float)py, depth, rc, fw);
+    float sx, sy, sd;
+    project(fw, sc, &sx, &sy, &sd);
+    const float cxf = fminf(fmaxf(sx, -1.0f), (float)sc.width);
+    const float cyf = fminf(fmaxf(sy, -1.0f), (float)sc.height);
+    const int ix = clampi((int)cxf, 0, sc.width - 1);
+    const int iy = clampi((int)cyf, 0, sc.height - 1);
+    const float src_depth = pb->depths[v][(size_t)iy * sc.width + ix];
+    if (src_depth == 0.0f) {
+        return 5.0f;
+    }
+    float bw[3];
+    point_world(sx, sy, src_depth, sc, bw);
+    float bx, by, bd;
+    project(bw, rc, &bx, &by, &bd);
+    const float dx = (float)px - bx;
+    const float dy = (float)py - by;
+    return fminf(5.0f, sqrtf(dx * dx + dy * dy));
+}
+
+/* sort_small (HPM.cu:3-12) */
+static inline void sort_small(float* d, int n)
+{
+    int j;
+    for (int i = 1; i < n; i++) {
+        const float tmp = d[i];
+        for (j = i; j >= 1 && tmp < d[j - 1]; j--) {
+            d[j] = d[j - 1];
+        }
+        d[j] = tmp;
+    }
+}
+
+/* FindMinCostIndex / FindMaxCostIndex (HPM.cu:29-53): the last extremum wins. */
+static inline int find_min_idx(const float* c, int n)
+{
+    float m = c[0];
+    int idx = 0;
+    for (int i = 1; i < n; ++i) {
+        if (c[i] <= m) {
+            m = c[i];
+            idx = i;
+        }
+    }
+    return idx;
+}
+
+static inline int find_max_idx(const float* c, int n)
+{
+    float m = c[0];
+    int idx = 0;
+    for (int i = 1; i < n; ++i) {
+        if (c[i] >= m) {
+            m = c[i];
+            idx = i;
+        }
+    }
+    return idx;
+}
+
+/* ComputeMultiViewCostVector (HPM.cu:438-443); only views with want[v]
+ * set are evaluated (the others are never read by the caller). */
+static void cost_vector(const Ctx& cx, const RefPatch& rp, int px, int py, F4 plane, float* out,
+                        const float* weights, uint64_t* nev)
+{
+    for (int v = 1; v <= cx.nsrc; ++v) {
+        if (weights != nullptr && !(weights[v - 1] > 0.0f)) {
+            out[v - 1] = 0.0f;
+            continue;
+        }
+        out[v - 1] = ncc(cx, rp, px, py, v, plane);
+        ++*nev;
+    }
+}
+
+/* ComputeMultiViewInitialCostandSelectedViews (HPM.cu:399-436). */
+static float initial_cost(const Ctx& cx, const RefPatch& rp, int px, int py, F4 plane, uint32_t* sel,
+                          uint64_t* nev)
+{
+    float cv[32], cvc[32];
+    int count = 0, valid = 0;
+    for (int v = 1; v <= cx.nsrc; ++v) {
+        const float c = ncc(cx, rp, px, py, v, plane);
+        ++*nev;
+        cv[v - 1] = c;
+        cvc[v - 1] = c;
+        ++count;
+        if (c < 2.0f) {
+            ++valid;
+        }
+    }
+    sort_small(cv, count);
+    *sel = 0u;
+    const int top_k = std::min(valid, cx.pb->params.top_k);
+    if (top_k > 0) {
+        float cost = 0.0f;
+        for (int i = 0; i < top_k; ++i) {
+            cost += cv[i];
+        }
+        const float thr = cv[top_k - 1];
+        for (int i = 0; i < cx.nsrc; ++i) {
+            if (cvc[i] <= thr) {
+                *sel |= (1u << i);
+            }
+        }
+        return cost / (float)top_k;
+    }
+    return 2.0f;
+}
+
+static void make_ctx(const hpm_oracle_problem* pb, Ctx* cx)
+{
+    cx->pb = pb;
+    cx->W = pb->cams[0].width;
+    cx->H = pb->cams[0].height;
+    cx->nsrc = pb->num_images - 1;
+    for (int v = 1; v < pb->num_images; ++v) {
+        cx->rel[v] = make_view_rel(pb->cams[0], pb->cams[v]);
+    }
+    cx->ic = make_iter_consts();
+}
+
+/* RandomInitialization (HPM.cu:510-558). */
+static void init_pixel(const Ctx& cx, int px, int py, float* hyp, float* cost, uint32_t* sel, uint64_t* nev)
+{
+    const hpm_oracle_problem* pb = cx.pb;
+    const hpm_params& P = pb->params;
+    const hpm_camera& rc = pb->cams[0];
+    const int center = py * cx.W + px;
+    hpm_rng rng;
+    hpm_rng_init(&rng, P.seed_lo, P.seed_hi, (uint32_t)center, hpm_stream_init(pb->call_index), pb->salt);
+    RefPatch rp;
+    make_ref_patch(pb->images[0], cx.W, cx.H, px, py, P, &rp);
+    if (P.geom_consistency) {
+        F4 t = ld4(hyp, center);
+        F4 c;
+        c.x = rc.R[0] * t.x + rc.R[1] * t.y + rc.R[2] * t.z;
+        c.y = rc.R[3] * t.x + rc.R[4] * t.y + rc.R[5] * t.z;
+        c.z = rc.R[6] * t.x + rc.R[7] * t.y + rc.R[8] * t.z;
+        c.w = t.w;
+        const float depth = c.w;
+        c.w = dist_to_origin(rc, px, py, depth, c);
+        st4(hyp, center, c);
+        cost[center] = initial_cost(cx, rp, px, py, c, &sel[center], nev);
+    } else if (P.planar_prior) {
+        if (pb->prior_mask[center] > 0u && cost[center] >= 0.1f) {
+            const float perturbation = 0.02f;
+            const F4 pl = ld4(pb->prior_planes, center);
+            float dp = pl.w;
+            const float dmin_p = (1 - 3 * perturbation) * dp;
+            const float dmax_p = (1 + 3 * perturbation) * dp;
+            dp = hpm_rng_uniform(&rng) * (dmax_p - dmin_p) + dmin_p;
+            F4 np = perturbed_normal(rc, px, py, pl, &rng, (float)((double)(3 * perturbation) * kPi));
+            np.w = dp;
+            st4(hyp, center, np);
+            cost[center] = initial_cost(cx, rp, px, py, np, &sel[center], nev);
+        } else {
+            F4 t = ld4(hyp, center);
+            const float depth = t.w;
+            t.w = dist_to_origin(rc, px, py, depth, t);
+            st4(hyp, center, t);
+            cost[center] = initial_cost(cx, rp, px, py, t, &sel[center], nev);
+        }
+    } else {
+        /* GenerateRandomPlaneHypothesis (HPM.cu:200-206) */
+        const float depth = hpm_rng_uniform(&rng) * (P.depth_max - P.depth_min) + P.depth_min;
+        F4 n = random_normal(rc, px, py, &rng, depth);
+        n.w = dist_to_origin(rc, px, py, depth, n);
+        st4(hyp, center, n);
+        cost[center] = initial_cost(cx, rp, px, py, n, &sel[center], nev);
+    }
+}
+
+/* JudgeExtend (HPM.cu:1036-1059). */
+static bool judge_extend(const Ctx& cx, int iter, int k, const float* c, bool flag)
+{
+    if (!flag) {
+        return false;
+    }
+    const float good = cx.ic.good[iter][k];
+    const double bad = 1.2;
+    int good_sum = 0, bad_sum = 0;
+    for (int i = 0; i < cx.nsrc; ++i) {
+        if (c[i] < good) {
+            ++good_sum;
+        }
+        if ((double)c[i] > bad) {
+            ++bad_sum;
+        }
+    }
+    return !(good_sum >= 1 && bad_sum <= 2);
+}
+
+/* Restricted cost with the planar prior (HPM.cu:1667-1671, 1200-1204). */
+struct PriorConsts {
+    float gamma, tdss, tass, beta;
+};
+
+static inline PriorConsts prior_consts(const hpm_params& P)
+{
+    PriorConsts k;
+    k.gamma = 0.5f;
+    const float ds = (P.depth_max - P.depth_min) / 64.0f;
+    k.tdss = 2 * ds * ds;
+    const float as = (float)(kPi * (double)(5.0f / 180.0f));
+    k.tass = 2 * as * as;
+    k.beta = 0.18f;
+    return k;
+}
+
+static inline float prior_term(const PriorConsts& k, float depth_diff, F4 prior_plane, F4 plane)
+{
+    const float ac = dot3(prior_plane, plane);
+    const float ad = hpm_acosf(ac);
+    return k.gamma + hpm_expf(-depth_diff * depth_diff / k.tdss) * hpm_expf(-ad * ad / k.tass);
+}
+
+/* CheckerboardPropagation (HPM.cu:1222-1725) + PlaneHypothesisRefinement
+ * (HPM.cu:1143-1220). */
+static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour, float* hyp, float* cost,
+                            uint32_t* sel, uint64_t* nev)
+{
+    const hpm_oracle_problem* pb = cx.pb;
+    const hpm_params& P = pb->params;
+    const hpm_camera& rc = pb->cams[0];
+    const int W = cx.W, H = cx.H, nsrc = cx.nsrc;
+    if (px >= W || py >= H) {
+        return;
+    }
+    const int center = py * W + px;
+    hpm_rng rng;
+    hpm_rng_init(&rng, P.seed_lo, P.seed_hi, (uint32_t)center, hpm_stream_update(pb->call_index, iter, colour),
+                 pb->salt);
+    RefPatch rp;
+    make_ref_patch(pb->images[0], W, H, px, py, P, &rp);
+
+    int left_up = center - 5 - 6 * W;
+    int right_up = center + 6 - 5 * W;
+    int right_down = center + 5 + 6 * W;
+    int left_down = center - 6 + 5 * W;
+    int left_far = center - 5;
+    int right_far = center + 5;
+    int up_far = center - W * 5;
+    int down_far = center + W * 5;
+
+    float ca[8][32];
+    memset(ca, 0, sizeof(ca));
+    ca[0][0] = 2.0f; /* float cost_array[8][32] = { 2.0f } (HPM.cu:1244) */
+    bool flag[8] = {false, false, false, false, false, false, false, false};
+    float cmin;
+    int cmp;
+
+    /* up_far (HPM.cu:1251-1272) */
+    if (py > 4) {
+        flag[1] = true;
+        cmin = cost[up_far];
+        cmp = up_far;
+        for (int i = 1; i < 5; ++i) {
+            if (py > 4 + 2 * i) {
+                const int pt = up_far - 2 * i * W;
+                if (cost[pt] < cmin) {
+                    cmin = cost[pt];
+                    cmp = pt;
+                }
+            } else {
+                break;
+            }
+        }
+        up_far = cmp;
+        cost_vector(cx, rp, px, py, ld4(hyp, up_far), ca[1], nullptr, nev);
+    }
+    /* down_far (HPM.cu:1274-1295) */
+    if (py < H - 5) {
+        flag[3] = true;
+        cmin = cost[down_far];
+        cmp = down_far;
+        for (int i = 1; i < 5; ++i) {
+            if (py < H - 5 - 2 * i) {
+                const int pt = down_far + 2 * i * W;
+                if (cost[pt] < cmin) {
+                    cmin = cost[pt];
+                    cmp = pt;
+                }
+            } else {
+                break;
+            }
+        }
+        down_far = cmp;
+        cost_vector(cx, rp, px, py, ld4(hyp, down_far), ca[3], nullptr, nev);
+    }
+    /* left_far (HPM.cu:1297-1318) */
+    if (px > 4) {
+        flag[5] = true;
+        cmin = cost[left_far];
+        cmp = left_far;
+        for (int i = 1; i < 5; ++i) {
+            if (px > 4 + 2 * i) {
+                const int pt = left_far - 2 * i;
+                if (cost[pt] < cmin) {
+                    cmin = cost[pt];
+                    cmp = pt;
+                }
+            } else {
+                break;
+            }
+        }
+        left_far = cmp;
+        cost_vector(cx, rp, px, py, ld4(hyp, left_far), ca[5], nullptr, nev);
+    }
+    /* right_far keeps the maximum (HPM.cu:1320-1341) */
+    if (px < W - 5) {
+        flag[7] = true;
+        cmin = cost[right_far];
+        cmp = right_far;
+        for (int i = 1; i < 5; ++i) {
+            if (px < W - 5 - 2 * i) {
+                const int pt = right_far + 2 * i;
+                if (cmin < cost[pt]) {
+                    cmin = cost[pt];
+                    cmp = pt;
+                }
+            } else {
+                break;
+            }
+        }
+        right_far = cmp;
+        cost_vector(cx, rp, px, py, ld4(hyp, right_far), ca[7], nullptr, nev);
+    }
+    /* left_up (HPM.cu:1344-1376) */
+    if (px > 4 && py > 5) {
+        flag[0] = true;
+        cmin = cost[left_up];
+        cmp = left_up;
+        int qx = px - 5, qy = py - 6;
+        for (int i = 0; i < 7; i++) {
+            if (i % 2 == 0) {
+                qx -= 2;
+            } else {
+                qy -= 2;
+            }
+            if (qx >= 0 && qx < W && qy < H && qy >= 0) {
+                const int pt = qy * W + qx;
+                if (cost[pt] < cmin) {
+                    cmin = cost[pt];
+                    cmp = pt;
+                }
+            } else {
+                break;
+            }
+        }
+        left_up = cmp;
+        cost_vector(cx, rp, px, py, ld4(hyp, left_up), ca[0], nullptr, nev);
+    }
+    /* right_up (HPM.cu:1378-1409) */
+    if (px < W - 6 && py > 4) {
+        flag[2] = true;
+        cmin = cost[right_up];
+        cmp = right_up;
+        int qx = px + 6, qy = py - 5;
+        for (int i = 0; i < 7; i++) {
+            if (i % 2 == 0) {
+                qy -= 2;
+            } else {
+                qx += 2;
+            }
+            if (qx >= 0 && qx < W && qy < H && qy >= 0) {
+                const int pt = qy * W + qx;
+                if (cost[pt] < cmin) {
+                    cmin = cost[pt];
+                    cmp = pt;
+                }
+            } else {
+                break;
+            }
+        }
+        right_up = cmp;
+        cost_vector(cx, rp, px, py, ld4(hyp, right_up), ca[2], nullptr, nev);
+    }
+    /* right_down (HPM.cu:1411-1442) */
+    if (px < W - 5 && py < H - 6) {
+        flag[4] = true;
+        cmin = cost[right_down];
+        cmp = right_down;
+        int qx = px + 5, qy = py + 6;
+        for (int i = 0; i < 7; i++) {
+            if (i % 2 == 0) {
+                qx += 2;
+            } else {
+                qy += 2;
+            }
+            if (qx >= 0 && qx < W && qy < H && qy >= 0) {
+                const int pt = qy * W + qx;
+                if (cost[pt] < cmin) {
+                    cmin = cost[pt];
+                    cmp = pt;
+                }
+            } else {
+                break;
+            }
+        }
+        right_down = cmp;
+        cost_vector(cx, rp, px, py, ld4(hyp, right_down), ca[4], nullptr, nev);
+    }
+    /* left_down: the scan never breaks and re-evaluates the cost vector
+     * after every step (HPM.cu:1444-1471); the final state equals one
+     * evaluation at the final position. */
+    if (px > 5 && py < H - 5) {
+        flag[6] = true;
+        cmin = cost[left_down];
+        cmp = left_down;
+        int qx = px - 6, qy = py + 5;
+        for (int i = 0; i < 7; i++) {
+            if (i % 2 == 0) {
+                qy += 2;
+            } else {
+                qx -= 2;
+            }
+            if (qx >= 0 && qx < W && qy < H && qy >= 0) {
+                const int pt = qy * W + qx;
+                if (cost[pt] < cmin) {
+                    cmin = cost[pt];
+                    cmp = pt;
+                }
+            }
+        }
+        left_down = cmp;
+        cost_vector(cx, rp, px, py, ld4(hyp, left_down), ca[6], nullptr, nev);
+    }
+    int pos[8] = {left_up, up_far, right_up, down_far, right_down, left_far, left_down, right_far};
+
+    /* NESP extension: only orientation 0 is ever enabled (HPM.cu:1473-1543);
+     * ExtendedLeftUpPropagation (HPM.cu:780-843). */
+    for (int k = 0; k < 3; ++k) {
+        if (!judge_extend(cx, iter, k, ca[0], flag[0])) {
+            break;
+        }
+        int lu = center - 5 - 6 * W - 4 * 2 * k - 4 * 2 * W * k;
+        if (px > 4 + 8 * k && py > 5 + 8 * k) {
+            bool symbol = true;
+            cmin = cost[lu];
+            cmp = lu;
+            int qx = px - 5 - 8 * k, qy = py - 6 - 8 * k;
+            for (int i = 0; i < 7; i++) {
+                if (i % 2 == 0) {
+                    qx -= 2;
+                } else {
+                    qy -= 2;
+                }
+                if (qx >= 0 && qx < W && qy < H && qy >= 0) {
+                    const int pt = qy * W + qx;
+                    if (cost[pt] < cmin) {
+                        cmin = cost[pt];
+                        cmp = pt;
+                    }
+                } else {
+                    break;
+                }
+            }
+            if (cmin > cost[pos[0]]) {
+                cmin = cost[pos[0]];
+                cmp = pos[0];
+                symbol = false;
+            }
+            lu = cmp;
+            /* re-evaluating the hypothesis already at pos[0] reproduces
+             * ca[0] bit for bit, so it is skipped */
+            if (symbol && lu != pos[0]) {
+                cost_vector(cx, rp, px, py, ld4(hyp, lu), ca[0], nullptr, nev);
+            }
+        } else {
+            lu = pos[0]; /* flag[0] is true here */
+        }
+        pos[0] = lu;
+    }
+
+    /* Multi-hypothesis joint view selection (HPM.cu:1548-1613) */
+    float vw[32], vprior[32], probs[32];
+    for (int i = 0; i < 32; ++i) {
+        vw[i] = 0.0f;
+        vprior[i] = 0.0f;
+        probs[i] = 0.0f;
+    }
+    const int nb[4] = {center - W * 5, center + W * 5, center - 5, center + 5};
+    for (int i = 0; i < 4; ++i) {
+        if (flag[2 * i + 1]) {
+            const uint32_t m = sel[nb[i]];
+            for (int j = 0; j < nsrc; ++j) {
+                if (((m >> j) & 1u) == 1u) {
+                    vprior[j] += 0.9f;
+                } else {
+                    vprior[j] += 0.1f;
+                }
+            }
+        }
+    }
+    const float thr = cx.ic.thr[iter];
+    for (int i = 0; i < nsrc; i++) {
+        float count = 0.0f;
+        int count_false = 0;
+        float tmpw = 0.0f;
+        for (int j = 0; j < 8; j++) {
+            const float c = ca[j][i];
+            if (c < thr) {
+                tmpw += hpm_expf(c * c / (-0.18f));
+                count++;
+            }
+            if (c > 1.2f) {
+                count_false++;
+            }
+        }
+        if (count > 2 && count_false < 3) {
+            probs[i] = tmpw / count;
+        } else if (count_false < 3) {
+            probs[i] = cx.ic.fallback[iter];
+        }
+        probs[i] = probs[i] * vprior[i];
+    }
+    /* TransformPDFToCDF (HPM.cu:86-100) */
+    {
+        float sum = 0.0f;
+        for (int i = 0; i < nsrc; ++i) {
+            sum += probs[i];
+        }
+        const float inv = 1.0f / sum;
+        float cum = 0.0f;
+        for (int i = 0; i < nsrc; ++i) {
+            cum += probs[i] * inv;
+            probs[i] = cum;
+        }
+    }
+    for (int s = 0; s < 15; ++s) {
+        const float u = hpm_rng_uniform(&rng) - FLT_EPSILON;
+        for (int v = 0; v < nsrc; ++v) {
+            if (probs[v] > u) {
+                vw[v] += 1.0f;
+                break;
+            }
+        }
+    }
+    uint32_t tsel = 0u;
+    float wn = 0.0f;
+    for (int i = 0; i < nsrc; ++i) {
+        if (vw[i] > 0) {
+            tsel |= (1u << i);
+            wn += vw[i];
+        }
+    }
+
+    /* Candidate scoring (HPM.cu:1615-1633) */
+    float fc[8];
+    for (int i = 0; i < 8; ++i) {
+        fc[i] = 0.0f;
+        for (int j = 0; j < nsrc; ++j) {
+            if (vw[j] > 0) {
+                if (P.geom_consistency) {
+                    if (flag[i]) {
+                        fc[i] += vw[j] * (ca[i][j] + 0.1f * geom_cost(cx, j + 1, ld4(hyp, pos[i]), px, py));
+                    } else {
+                        fc[i] += vw[j] * (ca[i][j] + 0.1f * 5.0f);
+                    }
+                } else {
+                    fc[i] += vw[j] * ca[i][j];
+                }
+            }
+        }
+        fc[i] /= wn;
+    }
+    const int min_idx = find_min_idx(fc, 8);
+
+    /* Re-score the current hypothesis (HPM.cu:1637-1650) */
+    float cvn[32];
+    const F4 cur = ld4(hyp, center);
+    cost_vector(cx, rp, px, py, cur, cvn, vw, nev);
+    float cost_now = 0.0f;
+    for (int i = 0; i < nsrc; ++i) {
+        if (!(vw[i] > 0.0f)) {
+            continue; /* the term is +0 */
+        }
+        if (P.geom_consistency) {
+            cost_now += vw[i] * (cvn[i] + 0.1f * geom_cost(cx, i + 1, cur, px, py));
+        } else {
+            cost_now += vw[i] * cvn[i];
+        }
+    }
+    cost_now /= wn;
+    cost[center] = cost_now;
+    float depth_now = depth_from_plane(rc, cur, px, py);
+
+    /* Adoption (HPM.cu:1652-1722) */
+    float restricted_cost = 0.0f;
+    const PriorConsts pk = prior_consts(P);
+    if (P.planar_prior) {
+        const F4 prior_plane = ld4(pb->prior_planes, center);
+        const float depth_prior = depth_from_plane(rc, prior_plane, px, py);
+        if (pb->prior_mask[center] > 0u) {
+            float rfc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            for (int i = 0; i < 8; i++) {
+                if (flag[i]) {
+                    const F4 hp = ld4(hyp, pos[i]);
+                    const float dn = depth_from_plane(rc, hp, px, py);
+                    const float pr = prior_term(pk, dn - depth_prior, prior_plane, hp);
+                    rfc[i] = hpm_expf(-fc[i] * fc[i] / pk.beta) * pr;
+                }
+            }
+            const int max_idx = find_max_idx(rfc, 8);
+            const float dn = depth_from_plane(rc, cur, px, py);
+            const float pr = prior_term(pk, dn - depth_prior, prior_plane, cur);
+            const float rcn = hpm_expf(-cost_now * cost_now / pk.beta) * pr;
+            if (flag[max_idx]) {
+                const F4 hp = ld4(hyp, pos[max_idx]);
+                const float db = depth_from_plane(rc, hp, px, py);
+                if (db >= P.depth_min && db <= P.depth_max && rfc[max_idx] > rcn) {
+                    depth_now = db;
+                    st4(hyp, center, hp);
+                    cost[center] = fc[max_idx];
+                    restricted_cost = rfc[max_idx];
+                    sel[center] = tsel;
+                }
+            }
+        } else if (flag[min_idx]) {
+            const F4 hp = ld4(hyp, pos[min_idx]);
+            const float db = depth_from_plane(rc, hp, px, py);
+            if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
+                depth_now = db;
+                st4(hyp, center, hp);
+                cost[center] = fc[min_idx];
+            }
+        }
+    }
+    if (!P.planar_prior && flag[min_idx]) {
+        const F4 hp = ld4(hyp, pos[min_idx]);
+        const float db = depth_from_plane(rc, hp, px, py);
+        if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
+            depth_now = db;
+            st4(hyp, center, hp);
+            cost[center] = fc[min_idx];
+            sel[center] = tsel;
+        }
+        if (cost[center] != cost[center]) {
+            depth_now = db;
+            st4(hyp, center, hp);
+            cost_now = fc[min_idx];
+            sel[center] = tsel;
+        }
+    }
+
+    /* PlaneHypothesisRefinement (HPM.cu:1143-1220) */
+    {
+        const float perturbation = 0.02f;
+        float depth_prior = 0.0f;
+        float depth_rand;
+        F4 n_rand;
+        const bool use_prior = P.planar_prior && pb->prior_mask[center] > 0u;
+        F4 prior_plane = {0.0f, 0.0f, 0.0f, 0.0f};
+        const float ds = (P.depth_max - P.depth_min) / 64.0f;
+        const float as = (float)(kPi * (double)(5.0f / 180.0f));
+        if (use_prior) {
+            prior_plane = ld4(pb->prior_planes, center);
+            depth_prior = depth_from_plane(rc, prior_plane, px, py);
+            depth_rand = hpm_rng_uniform(&rng) * 6 * ds + (depth_prior - 3 * ds);
+            n_rand = perturbed_normal(rc, px, py, prior_plane, &rng, as);
+        } else {
+            depth_rand = hpm_rng_uniform(&rng) * (P.depth_max - P.depth_min) + P.depth_min;
+            n_rand = random_normal(rc, px, py, &rng, depth_now);
+        }
+        const float dmin_p = (1 - perturbation) * depth_now;
+        const float dmax_p = (1 + perturbation) * depth_now;
+        const float depth_pert = hpm_rng_uniform(&rng) * (dmax_p - dmin_p) + dmin_p;
+        const F4 plane_now = ld4(hyp, center);
+        const F4 n_pert = perturbed_normal(rc, px, py, plane_now, &rng, (float)((double)perturbation * kPi));
+        const float depths[5] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert};
+        const F4 normals[5] = {plane_now, n_rand, n_rand, n_pert, plane_now};
+        for (int i = 0; i < 5; ++i) {
+            float cvt[32];
+            F4 t = normals[i];
+            t.w = dist_to_origin(rc, px, py, depths[i], t);
+            cost_vector(cx, rp, px, py, t, cvt, vw, nev);
+            float tc = 0.0f;
+            for (int j = 0; j < nsrc; ++j) {
+                if (vw[j] > 0) {
+                    if (P.geom_consistency) {
+                        tc += vw[j] * (cvt[j] + 0.1f * geom_cost(cx, j + 1, t, px, py));
+                    } else {
+                        tc += vw[j] * cvt[j];
+                    }
+                }
+            }
+            tc /= wn;
+            const float db = depth_from_plane(rc, t, px, py);
+            if (use_prior) {
+                const float pr = prior_term(pk, depths[i] - depth_prior, prior_plane, t);
+                const float rtc = hpm_expf(-tc * tc / pk.beta) * pr;
+                if (db >= P.depth_min && db <= P.depth_max && rtc > restricted_cost) {
+                    depth_now = db;
+                    st4(hyp, center, t);
+                    cost[center] = tc;
+                    restricted_cost = rtc;
+                }
+            } else {
+                if (db >= P.depth_min && db <= P.depth_max && tc < cost[center]) {
+                    depth_now = db;
+                    st4(hyp, center, t);
+                    cost[center] = tc;
+                }
+            }
+        }
+    }
+}
+
+/* GetDepthandNormal (HPM.cu:1751-1764). */
+static void finalize_pixel(const Ctx& cx, int px, int py, float* hyp)
+{
+    const hpm_camera& rc = cx.pb->cams[0];
+    const int center = py * cx.W + px;
+    F4 h = ld4(hyp, center);
+    h.w = depth_from_plane(rc, h, px, py);
+    F4 o;
+    o.x = rc.R[0] * h.x + rc.R[3] * h.y + rc.R[6] * h.z;
+    o.y = rc.R[1] * h.x + rc.R[4] * h.y + rc.R[7] * h.z;
+    o.z = rc.R[2] * h.x + rc.R[5] * h.y + rc.R[8] * h.z;
+    o.w = h.w;
+    st4(hyp, center, o);
+}
+
+/* CheckerboardFilter (HPM.cu:1766-1880). */
+static void filter_pixel(const Ctx& cx, int px, int py, float* hyp, const float* cost)
+{
+    const int W = cx.W, H = cx.H;
+    if (px >= W || py >= H) {
+        return;
+    }
+    const int center = py * W + px;
+    float f[21];
+    int n = 0;
+    f[n++] = hyp[4 * center + 3];
+    const int left = center - 1, leftleft = center - 3;
+    const int up = center - W, upup = center - 3 * W;
+    const int down = center + W, downdown = center + 3 * W;
+    const int right = center + 1, rightright = center + 3;
+    if (cost[center] < 0.001f) {
+        return;
+    }
+#define HPM_TAP(cond, idx)                  \
+    if (cond) {                             \
+        f[n++] = hyp[4 * (idx) + 3];        \
+    }
+    HPM_TAP(py > 0, up)
+    HPM_TAP(py > 2, upup)
+    HPM_TAP(py > 4, upup - W * 2)
+    HPM_TAP(py < H - 1, down)
+    HPM_TAP(py < H - 3, downdown)
+    HPM_TAP(py < H - 5, downdown + W * 2)
+    HPM_TAP(px > 0, left)
+    HPM_TAP(px > 2, leftleft)
+    HPM_TAP(px > 4, leftleft - 2)
+    HPM_TAP(px < W - 1, right)
+    HPM_TAP(px < W - 3, rightright)
+    HPM_TAP(px < W - 5, rightright + 2)
+    HPM_TAP(py > 0 && px < W - 2, up + 2)
+    HPM_TAP(py < H - 1 && px < W - 2, down + 2)
+    HPM_TAP(py > 0 && px > 1, up - 2)
+    HPM_TAP(py < H - 1 && px > 1, down - 2)
+    HPM_TAP(px > 0 && py > 2, left - W * 2)
+    HPM_TAP(px < W - 1 && py > 2, right - W * 2)
+    HPM_TAP(px > 0 && py < H - 2, left + W * 2)
+    HPM_TAP(px < W - 1 && py < H - 2, right + W * 2)
+#undef HPM_TAP
+    sort_small(f, n);
+    const int m = n / 2;
+    if (n % 2 == 0) {
+        hyp[4 * center + 3] = (f[m - 1] + f[m]) / 2;
+    } else {
+        hyp[4 * center + 3] = f[m];
+    }
+}
+
+/* Pixels of one colour visited by the 32x16-thread checkerboard launch of
+ * HPM.cu:1924-1931 with the row mapping of HPM.cu:1729-1734 (black: x+y
+ * even). Rows are 2*yy or 2*yy+1 for yy below the launch's y extent, which
+ * decides whether the last row of an odd-height image is visited. */
+template <typename F>
+static void for_colour(const Ctx& cx, int colour, int nthreads, F&& fn)
+{
+    const int W = cx.W, H = cx.H;
+    const int yy_limit = ((H / 2 + 15) / 16) * 16;
+    const int ny = std::min(yy_limit, (H + 1) / 2);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int yy = 0; yy < ny; ++yy) {
+        for (int x = 0; x < W; ++x) {
+            const int odd = x & 1;
+            const int y = 2 * yy + (colour == 0 ? odd : 1 - odd);
+            if (y < H) {
+                fn(x, y);
+            }
+        }
+    }
+}
+
+static int check_problem(const hpm_oracle_problem* pb)
+{
+    if (pb == nullptr || pb->num_images < 2 || pb->num_images > HPM_MAX_IMAGES) {
+        return -1;
+    }
+    for (int i = 0; i < pb->num_images; ++i) {
+        if (pb->images[i] == nullptr || pb->cams[i].width <= 0 || pb->cams[i].height <= 0) {
+            return -1;
+        }
+    }
+    if (pb->params.radius_increment <= 0 || pb->params.patch_size < 1) {
+        return -1;
+    }
+    const int per_axis = 2 * (pb->params.patch_size / 2) / pb->params.radius_increment + 1;
+    if (per_axis > 8) { /* RefPatch holds at most 64 taps */
+        return -1;
+    }
+    if (pb->params.geom_consistency) {
+        for (int i = 1; i < pb->num_images; ++i) {
+            if (pb->depths[i] == nullptr) {
+                return -1;
+            }
+        }
+    }
+    if (pb->params.planar_prior && (pb->prior_planes == nullptr || pb->prior_mask == nullptr)) {
+        return -1;
+    }
+    return 0;
+}
+
+static int nthreads_or_default(int n)
+{
+    return n > 0 ? n : omp_get_max_threads();
+}
+
+} // namespace
+
+extern "C" int hpm_oracle_init(const hpm_oracle_problem* pb, float* hyp, float* cost, uint32_t* selmask,
+                               uint64_t* n_eval, int num_threads)
+{
+    if (check_problem(pb) != 0) {
+        return -1;
+    }
+    Ctx cx;
+    make_ctx(pb, &cx);
+    const int nt = nthreads_or_default(num_threads);
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt) reduction(+ : total)
+    for (int y = 0; y < cx.H; ++y) {
+        uint64_t local = 0;
+        for (int x = 0; x < cx.W; ++x) {
+            init_pixel(cx, x, y, hyp, cost, selmask, &local);
+        }
+        total += local;
+    }
+    if (n_eval) {
+        *n_eval += total;
+    }
+    return 0;
+}
+
+extern "C" int hpm_oracle_half_iteration(const hpm_oracle_problem* pb, int iter, int colour, float* hyp,
+                                         float* cost, uint32_t* selmask, uint64_t* n_eval, int num_threads)
+{
+    if (check_problem(pb) != 0 || iter < 0 || iter > 2 || colour < 0 || colour > 1) {
+        return -1;
+    }
+    Ctx cx;
+    make_ctx(pb, &cx);
+    const int nt = nthreads_or_default(num_threads);
+    std::vector<uint64_t> counts((size_t)cx.H, 0);
+    for_colour(cx, colour, nt, [&](int x, int y) {
+        uint64_t local = 0;
+        propagate_pixel(cx, x, y, iter, colour, hyp, cost, selmask, &local);
+        counts[(size_t)y] += local;
+    });
+    if (n_eval) {
+        for (uint64_t c : counts) {
+            *n_eval += c;
+        }
+    }
+    return 0;
+}
+
+extern "C" int hpm_oracle_finalize(const hpm_oracle_problem* pb, float* hyp, int num_threads)
+{
+    if (check_problem(pb) != 0) {
+        return -1;
+    }
+    Ctx cx;
+    make_ctx(pb, &cx);
+    const int nt = nthreads_or_default(num_threads);
+#pragma omp parallel for num_threads(nt)
+    for (int y = 0; y < cx.H; ++y) {
+        for (int x = 0; x < cx.W; ++x) {
+            finalize_pixel(cx, x, y, hyp);
+        }
+    }
+    return 0;
+}
+
+extern "C" int hpm_oracle_filter(const hpm_oracle_problem* pb, int colour, float* hyp, const float* cost,
+                                 int num_threads)
+{
+    if (check_problem(pb) != 0 || colour < 0 || colour > 1) {
+        return -1;
+    }
+    Ctx cx;
+    make_ctx(pb, &cx);
+    const int nt = nthreads_or_default(num_threads);
+    for_colour(cx, colour, nt, [&](int x, int y) { filter_pixel(cx, x, y, hyp, cost); });
+    return 0;
+}
+
+extern "C" int hpm_oracle_run_patchmatch(const hpm_oracle_problem* pb, float* hyp, float* cost, uint32_t* selmask,
+                                         uint64_t* n_eval, int num_threads)
+{
+    uint64_t total = 0;
+    if (hpm_oracle_init(pb, hyp, cost, selmask, &total, num_threads) != 0) {
+        return -1;
+    }
+    for (int it = 0; it < 3; ++it) { /* hard-coded 3 (HPM.cu:1933) */
+        hpm_oracle_half_iteration(pb, it, 0, hyp, cost, selmask, &total, num_threads);
+        hpm_oracle_half_iteration(pb, it, 1, hyp, cost, selmask, &total, num_threads);
+    }
+    hpm_oracle_finalize(pb, hyp, num_threads);
+    hpm_oracle_filter(pb, 0, hyp, cost, num_threads);
+    hpm_oracle_filter(pb, 1, hyp, cost, num_threads);
+    if (n_eval) {
+        *n_eval = total;
+    }
+    return 0;
+}
+
+extern "C" int hpm_oracle_cost_vectors(const hpm_oracle_problem* pb, const float* planes, const int32_t* pix,
+                                       int npix, int fp64, double* out, int num_threads)
+{
+    if (check_problem(pb) != 0 || npix < 0) {
+        return -1;
+    }
+    Ctx cx;
+    make_ctx(pb, &cx);
+    const int nt = nthreads_or_default(num_threads);
+    const int nsrc = cx.nsrc;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+    for (int i = 0; i < npix; ++i) {
+        const int px = pix[2 * i], py = pix[2 * i + 1];
+        const F4 pl = ld4(planes, i);
+        if (fp64) {
+            for (int v = 1; v <= nsrc; ++v) {
+                out[(size_t)i * nsrc + v - 1] = ncc64(pb, cx, px, py, v, pl);
+            }
+        } else {
+            RefPatch rp;
+            make_ref_patch(pb->images[0], cx.W, cx.H, px, py, pb->params, &rp);
+            for (int v = 1; v <= nsrc; ++v) {
+                out[(size_t)i * nsrc + v - 1] = ncc(cx, rp, px, py, v, pl);
+            }
+        }
+    }
+    return 0;
+}
+
+/* JBU_cu (HPM.cu:1984-2052) with SpatialGauss / RangeGauss (HPM.cu:1971-1981)
+ * in FP32; NormalizeVec3's rsqrtf as 1/sqrtf. */
+extern "C" int hpm_oracle_jbu(const float* guide, int w, int h, const float* src_nd, int sw, int sh,
+                              float* out_depth, float* out_normal, int num_threads)
+{
+    const int scale_i = std::max(h / sh, w / sw);
+    if (scale_i == 1) {
+        return 1;
+    }
+    const int nt = nthreads_or_default(num_threads);
+    const float scale = (float)(1.0 * sw / w);
+    const float sigmad = 0.50f, sigmar = 25.5f;
+    const int win = scale_i * scale_i + 1;
+    const int nn = win / 2;
+#pragma omp parallel for num_threads(nt)
+    for (int py = 0; py < h; ++py) {
+        for (int px = 0; px < w; ++px) {
+            const float o_y = (float)py * scale;
+            const float o_x = (float)px * scale;
+            const float ref_pix = texel(guide, w, h, px, py);
+            float nf = 0.0f, ctot = 0.0f;
+            F4 ntot = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int j = -nn; j <= nn; ++j) {
+                int r_y = (int)(o_y + (float)j);
+                r_y = (r_y > 0 ? (r_y < sh ? r_y : sh - 1) : 0);
+                const int r_ys = py + j;
+                for (int i = -nn; i <= nn; ++i) {
+                    int r_x = (int)(o_x + (float)i);
+                    r_x = (r_x > 0 ? (r_x < sw ? r_x : sw - 1) : 0);
+                    const int s_center = r_y * sw + r_x;
+                    const float src_pix = src_nd[4 * s_center + 3];
+                    F4 sn = ld4(src_nd, s_center);
+                    const int r_xs = px + i;
+                    const float nb_pix = texel(guide, w, h, r_xs, r_ys);
+                    const float ddx = o_x - (float)r_x, ddy = o_y - (float)r_y;
+                    const float dis = ddx * ddx + ddy * ddy;
+                    const float sg = hpm_expf(-dis / (2 * sigmad * sigmad));
+                    const float xp = fabsf(ref_pix - nb_pix);
+                    const float rg = hpm_expf(-(xp * xp) / (2 * sigmar * sigmar));
+                    const float tg = sg * rg;
+                    nf += tg;
+                    ctot += src_pix * tg;
+                    sn.x = sn.x * tg;
+                    sn.y = sn.y * tg;
+                    sn.z = sn.z * tg;
+                    ntot.x = ntot.x + sn.x;
+                    ntot.y = ntot.y + sn.y;
+                    ntot.z = ntot.z + sn.z;
+                }
+            }
+            ntot.x = ntot.x / nf;
+            ntot.y = ntot.y / nf;
+            ntot.z = ntot.z / nf;
+            normalize3(&ntot);
+            const int center = py * w + px;
+            out_depth[center] = ctot / nf;
+            out_normal[4 * center + 0] = ntot.x;
+            out_normal[4 * center + 1] = ntot.y;
+            out_normal[4 * center + 2] = ntot.z;
+            out_normal[4 * center + 3] = 0.0f;
+        }
+    }
+    return 0;
+}
+
+extern "C" int hpm_oracle_fmath(int fn, const float* x, float* y, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        switch (fn) {
+        case 0: y[i] = hpm_expf(x[i]); break;
+        case 1: y[i] = hpm_sinf(x[i]); break;
+        case 2: y[i] = hpm_cosf(x[i]); break;
+        case 3: y[i] = hpm_acosf(x[i]); break;
+        default: return -1;
+        }
+    }
+    return 0;
+}
+
+extern "C" int hpm_oracle_philox_block(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    hpm_philox4x32_10(ctr, key, out);
+    return 0;
+}
+
+extern "C" int hpm_oracle_uniforms(uint32_t seed_lo, uint32_t seed_hi, uint32_t pixel, uint32_t stream,
+                                   uint32_t salt, float* out, int n)
+{
+    hpm_rng r;
+    hpm_rng_init(&r, seed_lo, seed_hi, pixel, stream, salt);
+    for (int i = 0; i < n; ++i) {
+        out[i] = hpm_rng_uniform(&r);
+    }
+    return 0;
+}

@@ -1,0 +1,88 @@
+/*
+ * hpm_oracle.h — C interface of the CPU restatement of HPM-MVS's
+ * per-reference-view PatchMatch (HPM-MVS/HPM.cu:1-2052).
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker the HIP path is
+ * compared against; nothing in the product links or calls it. It is used by
+ * tests/, by the smoke check and by the CPU-baseline leg of bench.py.
+ *
+ * The restatement is plain C++ with OpenMP over pixels. It follows the
+ * reference's text (function by function, cited in hpm_oracle.cpp) with the
+ * same floating-point expression order as the HIP kernels, so that for the
+ * same inputs, seed and call index the two produce the same bits wherever
+ * the arithmetic is IEEE-exact (all of it: +, -, *, /, sqrt, explicit fma,
+ * and the shared elementary functions of hpm_fmath.h). An FP64 variant of
+ * the bilateral NCC cost (hpm_oracle_cost_vectors with fp64=1) evaluates the
+ * reference formula E[xy]-E[x]E[y] in double precision as the accuracy
+ * anchor for the FP32 cost.
+ */
+#ifndef HPM_ORACLE_H_
+#define HPM_ORACLE_H_
+
+#include <stdint.h>
+
+#include "hpm_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hpm_oracle_problem {
+    int32_t num_images;                      /* 1 reference + sources */
+    hpm_camera cams[HPM_MAX_IMAGES];         /* width/height filled */
+    const float* images[HPM_MAX_IMAGES];     /* row-major grey, cams[i].width x cams[i].height */
+    const float* depths[HPM_MAX_IMAGES];     /* geometric consistency: previous depth maps (NULL otherwise) */
+    hpm_params params;                       /* depth_min/depth_max already relaxed */
+    const float* prior_planes;               /* planar prior: float4 per ref pixel (camera-frame n, d) */
+    const uint32_t* prior_mask;              /* planar prior: u32 per ref pixel */
+    uint32_t call_index;                     /* RunPatchMatch call number on this problem (RNG stream) */
+    uint32_t salt;                           /* reference view id (RNG stream) */
+} hpm_oracle_problem;
+
+/* One full RunPatchMatch (HPM.cu:1906-1957): init, 3 red/black iterations,
+ * plane->depth/world normal, red/black median filter. hyp (float4 per pixel),
+ * cost and selmask are in/out state, as on the device. n_eval (may be NULL)
+ * receives the number of (pixel, hypothesis, source view) NCC evaluations.
+ * Returns 0 on success. */
+int hpm_oracle_run_patchmatch(const hpm_oracle_problem* pb, float* hyp, float* cost,
+                              uint32_t* selmask, uint64_t* n_eval, int num_threads);
+
+/* The stages of hpm_oracle_run_patchmatch, for step-by-step parity. */
+int hpm_oracle_init(const hpm_oracle_problem* pb, float* hyp, float* cost, uint32_t* selmask,
+                    uint64_t* n_eval, int num_threads);
+int hpm_oracle_half_iteration(const hpm_oracle_problem* pb, int iter, int colour, float* hyp,
+                              float* cost, uint32_t* selmask, uint64_t* n_eval, int num_threads);
+int hpm_oracle_finalize(const hpm_oracle_problem* pb, float* hyp, int num_threads);
+int hpm_oracle_filter(const hpm_oracle_problem* pb, int colour, float* hyp, const float* cost,
+                      int num_threads);
+
+/* Multi-view NCC cost vectors for fixed hypotheses (HPM.cu:325-397,438-443).
+ * planes: npix camera-frame planes (n, d); pix: npix (x, y) pairs.
+ * out: npix x (num_images-1) costs. fp64 selects the double-precision
+ * evaluation of the reference formula; otherwise the FP32 arithmetic of the
+ * HIP path. */
+int hpm_oracle_cost_vectors(const hpm_oracle_problem* pb, const float* planes, const int32_t* pix,
+                            int npix, int fp64, double* out, int num_threads);
+
+/* Joint bilateral upsampling (HPM.cu:1984-2052) of a low-resolution
+ * depth/normal map guided by a full-resolution grey image. src_nd is float4
+ * (world normal xyz, depth) at sw x sh; out_depth / out_normal (float4, w=0)
+ * are at w x h. Returns 1 without writing when the integer scale is 1
+ * (HPM.cpp:1218-1222). */
+int hpm_oracle_jbu(const float* guide, int w, int h, const float* src_nd, int sw, int sh,
+                   float* out_depth, float* out_normal, int num_threads);
+
+/* Host evaluation of the shared elementary functions (hpm_fmath.h):
+ * fn 0 exp, 1 sin, 2 cos, 3 acos. */
+int hpm_oracle_fmath(int fn, const float* x, float* y, int n);
+/* Host evaluation of Philox4x32-10 and of the per-pixel draw stream
+ * (hpm_philox.h): n uniforms of stream (seed, pixel, stream, salt). */
+int hpm_oracle_philox_block(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+int hpm_oracle_uniforms(uint32_t seed_lo, uint32_t seed_hi, uint32_t pixel, uint32_t stream, uint32_t salt,
+                        float* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPM_ORACLE_H_ */

@@ -1,0 +1,179 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Bars:
+  - shared arithmetic (exp/sin/cos/acos, Philox): bit-identical CPU vs GPU;
+  - NCC cost vectors with fixed hypotheses: GPU FP32 within 1e-4 of the
+    FP64 evaluation of the reference formula, away from the two decision
+    boundaries (var < 1e-5 and the source-image bounds test of the warped
+    centre), and bit-identical to the oracle's FP32 arithmetic;
+  - each stage of RunPatchMatch (init, 6 half-iterations, finalize, two
+    filter passes) started from the same state: bit-identical per pixel;
+  - a full RunPatchMatch on the 640x480 four-plane workload: at least 99% of
+    pixels within 1% relative depth and 5 degrees normal of the CPU run.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def make_problem(hpm, cams, images, params=None):
+    pr = hpm.Problem(0)
+    if params is not None:
+        pr.set_params(params)
+    pr.input_init_memory(cams, images, ref_image_id=0)
+    pr.CudaSpaceInitialization()
+    return pr
+
+
+def test_shared_math_bitwise(hpm, cpu):
+    say("case: shared math CPU vs GPU")
+    import ctypes as C
+
+    L = cpu.lib()
+    L.hpm_oracle_fmath.argtypes = [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]
+    L.hpm_oracle_philox_block.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    rng = np.random.default_rng(0)
+    pr = hpm.Problem(0)
+    ranges = {0: (-100.0, 90.0), 1: (-7.0, 7.0), 2: (-7.0, 7.0), 3: (-1.0, 1.0)}
+    for kind, (lo, hi) in ranges.items():
+        x = rng.uniform(lo, hi, 100000).astype(np.float32)
+        x[:5] = np.array([0.0, lo, hi, -0.0, (lo + hi) / 2], np.float32)
+        ref = np.empty_like(x)
+        L.hpm_oracle_fmath(kind, x.ctypes.data_as(C.POINTER(C.c_float)), ref.ctypes.data_as(C.POINTER(C.c_float)),
+                           len(x))
+        got = pr.device_selftest(kind, x)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), kind
+    blocks = rng.integers(0, 2**32, size=(4096, 6), dtype=np.uint64).astype(np.uint32)
+    got = pr.device_selftest(4, blocks)
+    for i in range(0, len(blocks), 97):
+        ctr = (C.c_uint32 * 4)(*[int(v) for v in blocks[i, :4]])
+        key = (C.c_uint32 * 2)(*[int(v) for v in blocks[i, 4:]])
+        out = (C.c_uint32 * 4)()
+        L.hpm_oracle_philox_block(ctr, key, out)
+        assert list(out) == [int(v) for v in got[i]]
+    pr.close()
+
+
+def _planes_near_truth(hpm, cams, gd, gn, order, rng, n, noise):
+    """Camera-frame planes at random pixels from the ground truth, perturbed."""
+    ref = cams[0]
+    W, H = ref.width, ref.height
+    xs = rng.integers(0, W, n)
+    ys = rng.integers(0, H, n)
+    R = np.array(ref.R, np.float64).reshape(3, 3)
+    K = np.array(ref.K, np.float64).reshape(3, 3)
+    d = gd[order[0]][ys, xs].astype(np.float64) * (1.0 + noise * rng.standard_normal(n))
+    nw = gn[order[0]][ys, xs].astype(np.float64)
+    nc = nw @ R.T
+    nc += noise * rng.standard_normal(nc.shape)
+    nc /= np.linalg.norm(nc, axis=1, keepdims=True)
+    X = np.stack([d * (xs - K[0, 2]) / K[0, 0], d * (ys - K[1, 2]) / K[1, 1], d], axis=1)
+    dist = -(nc * X).sum(axis=1)
+    planes = np.concatenate([nc, dist[:, None]], axis=1).astype(np.float32)
+    pix = np.stack([xs, ys], axis=1).astype(np.int32)
+    return planes, pix
+
+
+def test_cost_vectors_vs_fp64(hpm, cpu, c1_full):
+    cfg, (cams, images, gd, gn, low, order) = c1_full
+    say("case: cost vectors 640x480, 4 sources, fixed hypotheses")
+    rng = np.random.default_rng(1)
+    p1, x1 = _planes_near_truth(hpm, cams, gd, gn, order, rng, 20000, 0.01)
+    p2, x2 = _planes_near_truth(hpm, cams, gd, gn, order, rng, 20000, 0.3)
+    planes = np.concatenate([p1, p2])
+    pix = np.concatenate([x1, x2])
+    pr = make_problem(hpm, cams, images)
+    got = pr.cost_vectors(planes, pix).astype(np.float64)
+    params = pr.params()
+    o = cpu.Oracle(cams, images, params)
+    f32 = o.cost_vectors(planes, pix, fp64=False)
+    f64 = o.cost_vectors(planes, pix, fp64=True)
+    # bit-identical to the oracle's FP32 arithmetic
+    assert np.array_equal(got.astype(np.float32).view(np.uint32), f32.astype(np.float32).view(np.uint32))
+    # within 1e-4 of FP64 except where FP32 and FP64 land on different sides
+    # of a decision boundary (cost exactly 2 on one side only)
+    boundary = (got == 2.0) != (f64 == 2.0)
+    err = np.abs(got - f64)[~boundary]
+    say(f"max |fp32-fp64| = {err.max():.3e}, boundary flips = {boundary.sum()} of {boundary.size}")
+    assert err.max() <= 1e-4
+    assert boundary.mean() < 1e-3
+    pr.close()
+
+
+def test_stages_bitwise(hpm, cpu, c1_small):
+    """Every stage started from the oracle's state gives the oracle's bits."""
+    cfg, (cams, images, gd, gn, low, order) = c1_small
+    say("case: stagewise 160x120, 4 sources")
+    pr = make_problem(hpm, cams, images)
+    o = cpu.Oracle(cams, images, pr.params(), salt=0)
+
+    def compare(tag):
+        hyp, cost, sel = pr.debug_state()
+        same = (np.all(hyp.view(np.uint32) == o.hyp.view(np.uint32), axis=2)
+                & ((cost.view(np.uint32) == o.cost.view(np.uint32)) | (np.isnan(cost) & np.isnan(o.cost)))
+                & (sel == o.sel))
+        say(f"{tag}: identical pixels {same.mean():.6f}")
+        assert same.all(), tag
+
+    def sync():
+        import ctypes as C
+        hpm.lib().hpm_debug_write_state(pr._h, hpm.fptr(o.hyp), hpm.fptr(o.cost), hpm.uptr(o.sel))
+
+    o.init()
+    pr.debug_stage(0)
+    compare("init")
+    for it in range(3):
+        for colour in range(2):
+            sync()
+            o.half_iteration(it, colour)
+            pr.debug_stage(1, it, colour)
+            compare(f"iter {it} colour {colour}")
+    sync()
+    o.finalize()
+    pr.debug_stage(2)
+    compare("finalize")
+    for colour in range(2):
+        sync()
+        o.filter(colour)
+        pr.debug_stage(3, 0, colour)
+        compare(f"filter {colour}")
+    pr.close()
+
+
+def depth_normal_agreement(a_hyp, b_hyp):
+    da, db = a_hyp[..., 3], b_hyp[..., 3]
+    rel = np.abs(da - db) / np.maximum(np.abs(db), 1e-12)
+    na, nb = a_hyp[..., :3], b_hyp[..., :3]
+    cosang = np.clip((na * nb).sum(-1) / np.maximum(np.linalg.norm(na, axis=-1) * np.linalg.norm(nb, axis=-1),
+                                                    1e-12), -1, 1)
+    ang = np.degrees(np.arccos(cosang))
+    ok = (rel < 0.01) & (ang < 5.0)
+    return ok
+
+
+def test_full_run_parity_c1(hpm, cpu, c1_full):
+    cfg, (cams, images, gd, gn, low, order) = c1_full
+    say("case: full RunPatchMatch 640x480, 4 sources")
+    pr = make_problem(hpm, cams, images)
+    pr.RunPatchMatch()
+    g_hyp = pr.hypotheses()
+    g_cost = pr.costs()
+    stats = pr.run_stats()
+    o = cpu.Oracle(cams, images, pr.params(), salt=0)
+    o.run_patchmatch()
+    ok = depth_normal_agreement(g_hyp, o.hyp)
+    ident = np.all(g_hyp.view(np.uint32) == o.hyp.view(np.uint32), axis=2).mean()
+    gt = gd[order[0]]
+    acc_g = (np.abs(g_hyp[..., 3] - gt) / gt < 0.02).mean()
+    acc_o = (np.abs(o.hyp[..., 3] - gt) / gt < 0.02).mean()
+    say(f"agree(1%,5deg)={ok.mean():.5f} identical={ident:.5f} acc2%: gpu={acc_g:.4f} cpu={acc_o:.4f} "
+        f"n_eval gpu={stats.n_eval} cpu={o.n_eval} ms={stats.ms_total:.2f}")
+    assert ok.mean() >= 0.99
+    assert stats.n_eval == o.n_eval or ident < 1.0
+    assert acc_g > 0.8
+    pr.close()

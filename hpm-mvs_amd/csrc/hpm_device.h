@@ -27,17 +27,23 @@
 extern "C" {
 #endif
 
+/* Plane-induced homography of a source view (HPM.cu:227-287) split as
+ * H = A - b q^T with A = Ks Rrel Kr^-1, b = Ks trel (double, per view) and
+ * q = Kr^-T n / w (per hypothesis). */
 typedef struct hpm_dev_view {
+    double A[9];
+    double b[3];
     hpm_camera cam;
-    float Rrel[9];      /* R_src * R_ref^T */
-    float trel[3];      /* R_src * (C_ref - C_src) */
+    int32_t pad_;
     int64_t img_off;    /* offset of this image in the image pool (elements) */
     int64_t depth_off;  /* offset of this view's depth map in the depth pool */
 } hpm_dev_view;
 
 typedef struct hpm_dev_problem {
+    double rk_ifx, rk_ify;     /* 1/fx, 1/fy of the reference camera */
+    double rk_cx, rk_cy;
     int32_t W, H, nsrc;
-    int32_t yy_limit;          /* checkerboard row-pair extent, see for_colour */
+    int32_t yy_limit;          /* checkerboard row-pair extent (HPM.cu:1926) */
     hpm_params params;
     uint32_t call_index;       /* RNG stream of this RunPatchMatch call */
     uint32_t salt;             /* RNG stream: reference image id */

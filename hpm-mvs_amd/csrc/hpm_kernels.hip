@@ -52,17 +52,79 @@ __device__ __forceinline__ float texel(const float* img, int W, int H, int x, in
     return img[(size_t)clampi(y, 0, H - 1) * W + clampi(x, 0, W - 1)];
 }
 
-/* linear-filtered, clamp-addressed fetch at texel-centre coordinates */
-__device__ __forceinline__ float bilinear(const float* img, int W, int H, float x, float y)
+/* q = Kr^-T n / w of one hypothesis (double); the homography of source
+ * view v is then H = A_v - b_v q^T (HPM.cu:227-287) */
+__device__ __forceinline__ void plane_q(const hpm_dev_problem& pb, float4 n, double q[3])
 {
-    x = fminf(fmaxf(x, -1.0f), (float)W);
-    y = fminf(fmaxf(y, -1.0f), (float)H);
-    const float fx = floorf(x);
-    const float fy = floorf(y);
-    const float a = x - fx;
-    const float b = y - fy;
-    const int ix = (int)fx;
-    const int iy = (int)fy;
+    const double iw = 1.0 / (double)n.w;
+    const double nx = n.x, ny = n.y, nz = n.z;
+    q[0] = (nx * pb.rk_ifx) * iw;
+    q[1] = (ny * pb.rk_ify) * iw;
+    q[2] = ((nz - (nx * pb.rk_cx) * pb.rk_ifx) - (ny * pb.rk_cy) * pb.rk_ify) * iw;
+}
+
+/* Warp of one (hypothesis, source view) around reference pixel p: the
+ * warped centre (ComputeCorrespondingPoint, HPM.cu:289-296) in double, split
+ * into an integer texel and a float fraction, plus the float coefficients of
+ * the tap offsets s(p + d) - s0 = (g d) / (1 + e . d). */
+struct Warp {
+    int ix0, iy0;
+    float fx0, fy0;
+    float ex, ey, gxx, gxy, gyx, gyy;
+};
+
+/* false when the warped centre leaves the source image (HPM.cu:333-335) */
+__device__ __forceinline__ bool make_warp(const hpm_dev_view& sv, const double q[3], int px, int py, int SW, int SH,
+                                          Warp* w)
+{
+    double H[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            H[3 * r + c] = fma(-sv.b[r], q[c], sv.A[3 * r + c]);
+        }
+    }
+    const double X = (double)px, Y = (double)py;
+    const double hx = fma(H[0], X, fma(H[1], Y, H[2]));
+    const double hy = fma(H[3], X, fma(H[4], Y, H[5]));
+    const double hz = fma(H[6], X, fma(H[7], Y, H[8]));
+    const double icz = 1.0 / hz;
+    const double sx0 = hx * icz;
+    const double sy0 = hy * icz;
+    if (!(sx0 >= 0.0 && sx0 < (double)SW && sy0 >= 0.0 && sy0 < (double)SH)) {
+        return false;
+    }
+    const double fx = floor(sx0), fy = floor(sy0);
+    w->ix0 = (int)fx;
+    w->iy0 = (int)fy;
+    w->fx0 = (float)(sx0 - fx);
+    w->fy0 = (float)(sy0 - fy);
+    w->ex = (float)(H[6] * icz);
+    w->ey = (float)(H[7] * icz);
+    w->gxx = (float)((H[0] - sx0 * H[6]) * icz);
+    w->gxy = (float)((H[1] - sx0 * H[7]) * icz);
+    w->gyx = (float)((H[3] - sy0 * H[6]) * icz);
+    w->gyy = (float)((H[4] - sy0 * H[7]) * icz);
+    return true;
+}
+
+/* linear-filtered, clamp-addressed fetch (the reference's tex2D on a
+ * cudaFilterModeLinear texture, HPM.cpp:599-617) at tap (dx, dy) */
+__device__ __forceinline__ float sample_tap(const float* img, int W, int H, const Warp& w, float dx, float dy)
+{
+    const float den = fmaf(w.ex, dx, fmaf(w.ey, dy, 1.0f));
+    const float inv = 1.0f / den;
+    float lx = fmaf(fmaf(w.gxx, dx, w.gxy * dy), inv, w.fx0);
+    float ly = fmaf(fmaf(w.gyx, dx, w.gyy * dy), inv, w.fy0);
+    lx = fminf(fmaxf(lx, -1.0f - (float)w.ix0), (float)(W - w.ix0));
+    ly = fminf(fmaxf(ly, -1.0f - (float)w.iy0), (float)(H - w.iy0));
+    const float flx = floorf(lx);
+    const float fly = floorf(ly);
+    const float a = lx - flx;
+    const float b = ly - fly;
+    const int ix = w.ix0 + (int)flx;
+    const int iy = w.iy0 + (int)fly;
     const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
     const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
     const float* r0 = img + (size_t)y0 * W;
@@ -74,47 +136,6 @@ __device__ __forceinline__ float bilinear(const float* img, int W, int H, float 
     const float v0 = fmaf(a, t10 - t00, t00);
     const float v1 = fmaf(a, t11 - t01, t01);
     return fmaf(b, v1 - v0, v0);
-}
-
-/* plane-dependent part of ComputeHomography (HPM.cu:257-286) */
-__device__ __forceinline__ void homography(const hpm_camera& rc, const hpm_dev_view& sv, float4 n, float H[9])
-{
-    const hpm_camera& sc = sv.cam;
-    float h[9], tmp[9];
-    const float nn[3] = {n.x, n.y, n.z};
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            h[3 * r + c] = sv.Rrel[3 * r + c] - sv.trel[r] * nn[c] / n.w;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        tmp[3 * r + 0] = h[3 * r + 0] / rc.K[0];
-        tmp[3 * r + 1] = h[3 * r + 1] / rc.K[4];
-        tmp[3 * r + 2] = -h[3 * r + 0] * rc.K[2] / rc.K[0] - h[3 * r + 1] * rc.K[5] / rc.K[4] + h[3 * r + 2];
-    }
-    H[0] = sc.K[0] * tmp[0] + sc.K[2] * tmp[6];
-    H[1] = sc.K[0] * tmp[1] + sc.K[2] * tmp[7];
-    H[2] = sc.K[0] * tmp[2] + sc.K[2] * tmp[8];
-    H[3] = sc.K[4] * tmp[3] + sc.K[5] * tmp[6];
-    H[4] = sc.K[4] * tmp[4] + sc.K[5] * tmp[7];
-    H[5] = sc.K[4] * tmp[5] + sc.K[5] * tmp[8];
-    H[6] = sc.K[8] * tmp[6];
-    H[7] = sc.K[8] * tmp[7];
-    H[8] = sc.K[8] * tmp[8];
-}
-
-/* ComputeCorrespondingPoint (HPM.cu:289-296), fma form, one reciprocal */
-__device__ __forceinline__ void warp(const float H[9], float X, float Y, float* sx, float* sy)
-{
-    const float px = fmaf(H[0], X, fmaf(H[1], Y, H[2]));
-    const float py = fmaf(H[3], X, fmaf(H[4], Y, H[5]));
-    const float pz = fmaf(H[6], X, fmaf(H[7], Y, H[8]));
-    const float inv = 1.0f / pz;
-    *sx = px * inv;
-    *sy = py * inv;
 }
 
 /* reference-only part of ComputeBilateralNCC + ComputeBilateralWeight
@@ -161,17 +182,14 @@ __device__ __forceinline__ void make_ref_patch(const float* img, int W, int H, i
     rp->var_r = srr * rp->inv_w - rp->mr * rp->mr;
 }
 
-/* ComputeBilateralNCC (HPM.cu:325-397) for source view v */
+/* ComputeBilateralNCC (HPM.cu:325-397) for source view v; q from plane_q */
 __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& rp, int px, int py, int v,
-                                     float4 plane)
+                                     const double q[3])
 {
     const hpm_dev_view& sv = pb.views[v];
     const int SW = sv.cam.width, SH = sv.cam.height;
-    float H[9];
-    homography(pb.views[0].cam, sv, plane, H);
-    float cx0, cy0;
-    warp(H, (float)px, (float)py, &cx0, &cy0);
-    if (cx0 >= (float)SW || cx0 < 0.0f || cy0 >= (float)SH || cy0 < 0.0f) {
+    Warp w;
+    if (!make_warp(sv, q, px, py, SW, SH, &w)) {
         return 2.0f;
     }
     const float* simg = pb.images + sv.img_off;
@@ -181,9 +199,7 @@ __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& 
 #pragma unroll
         for (int b = 0; b < 6; ++b) {
             const int k = 6 * a + b;
-            float sx, sy;
-            warp(H, (float)(px + 2 * a - 5), (float)(py + 2 * b - 5), &sx, &sy);
-            const float s = bilinear(simg, SW, SH, sx, sy) - rp.c;
+            const float s = sample_tap(simg, SW, SH, w, (float)(2 * a - 5), (float)(2 * b - 5)) - rp.c;
             const float ws = rp.w[k] * s;
             ss = ss + ws;
             sss = fmaf(ws, s, sss);
@@ -388,12 +404,14 @@ __device__ __forceinline__ int find_max_idx8(const float* c)
 __device__ void cost_vector(const hpm_dev_problem& pb, const RefPatch& rp, int px, int py, float4 plane, float* out,
                             const float* weights, uint32_t* nev)
 {
+    double q[3];
+    plane_q(pb, plane, q);
     for (int v = 1; v <= pb.nsrc; ++v) {
         if (weights != nullptr && !(weights[v - 1] > 0.0f)) {
             out[v - 1] = 0.0f;
             continue;
         }
-        out[v - 1] = ncc(pb, rp, px, py, v, plane);
+        out[v - 1] = ncc(pb, rp, px, py, v, q);
         ++*nev;
     }
 }
@@ -404,8 +422,10 @@ __device__ float initial_cost(const hpm_dev_problem& pb, const RefPatch& rp, int
 {
     float cv[HPM_MAX_SOURCES], cvc[HPM_MAX_SOURCES];
     int count = 0, valid = 0;
+    double q[3];
+    plane_q(pb, plane, q);
     for (int v = 1; v <= pb.nsrc; ++v) {
-        const float c = ncc(pb, rp, px, py, v, plane);
+        const float c = ncc(pb, rp, px, py, v, q);
         ++*nev;
         cv[v - 1] = c;
         cvc[v - 1] = c;
@@ -1112,8 +1132,10 @@ __global__ __launch_bounds__(256) void hpm_k_cost_vectors(hpm_dev_problem pb, co
     const float4 pl = ld4(planes, i);
     RefPatch rp;
     make_ref_patch(pb.images, pb.W, pb.H, px, py, pb.params, &rp);
+    double q[3];
+    plane_q(pb, pl, q);
     for (int v = 1; v <= pb.nsrc; ++v) {
-        out[(size_t)i * pb.nsrc + v - 1] = ncc(pb, rp, px, py, v, pl);
+        out[(size_t)i * pb.nsrc + v - 1] = ncc(pb, rp, px, py, v, q);
     }
 }
 

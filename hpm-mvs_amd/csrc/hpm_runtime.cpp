@@ -93,28 +93,48 @@ void dfree(T*& p)
     }
 }
 
-/* camera-only part of ComputeHomography (HPM.cu:229-255) */
+/* Camera-only part of the plane-induced homography (ComputeHomography,
+ * HPM.cu:227-287): H = Ks (Rrel - trel n^T / w) Kr^-1 = A - b q^T with
+ * Ks = [sK0 0 sK2; 0 sK4 sK5; 0 0 sK8], Kr = [fx 0 cx; 0 fy cy; 0 0 1] as the
+ * reference reads them; A = Ks Rrel Kr^-1 and b = Ks trel in double. */
 void make_view_rel(const hpm_camera& rc, const hpm_camera& sc, hpm_dev_view* v)
 {
-    float ref_C[3], src_C[3], C_rel[3];
-    ref_C[0] = -(rc.R[0] * rc.t[0] + rc.R[3] * rc.t[1] + rc.R[6] * rc.t[2]);
-    ref_C[1] = -(rc.R[1] * rc.t[0] + rc.R[4] * rc.t[1] + rc.R[7] * rc.t[2]);
-    ref_C[2] = -(rc.R[2] * rc.t[0] + rc.R[5] * rc.t[1] + rc.R[8] * rc.t[2]);
-    src_C[0] = -(sc.R[0] * sc.t[0] + sc.R[3] * sc.t[1] + sc.R[6] * sc.t[2]);
-    src_C[1] = -(sc.R[1] * sc.t[0] + sc.R[4] * sc.t[1] + sc.R[7] * sc.t[2]);
-    src_C[2] = -(sc.R[2] * sc.t[0] + sc.R[5] * sc.t[1] + sc.R[8] * sc.t[2]);
+    double rR[9], sR[9], refC[3], srcC[3], Crel[3], Rrel[9], trel[3], M[9];
+    for (int i = 0; i < 9; ++i) {
+        rR[i] = (double)rc.R[i];
+        sR[i] = (double)sc.R[i];
+    }
+    for (int a = 0; a < 3; ++a) {
+        refC[a] = -(rR[a] * (double)rc.t[0] + rR[3 + a] * (double)rc.t[1] + rR[6 + a] * (double)rc.t[2]);
+        srcC[a] = -(sR[a] * (double)sc.t[0] + sR[3 + a] * (double)sc.t[1] + sR[6 + a] * (double)sc.t[2]);
+    }
     for (int r = 0; r < 3; ++r) {
         for (int c = 0; c < 3; ++c) {
-            v->Rrel[3 * r + c] = sc.R[3 * r + 0] * rc.R[3 * c + 0] + sc.R[3 * r + 1] * rc.R[3 * c + 1] +
-                                 sc.R[3 * r + 2] * rc.R[3 * c + 2];
+            Rrel[3 * r + c] = sR[3 * r] * rR[3 * c] + sR[3 * r + 1] * rR[3 * c + 1] + sR[3 * r + 2] * rR[3 * c + 2];
         }
     }
-    C_rel[0] = ref_C[0] - src_C[0];
-    C_rel[1] = ref_C[1] - src_C[1];
-    C_rel[2] = ref_C[2] - src_C[2];
-    for (int r = 0; r < 3; ++r) {
-        v->trel[r] = sc.R[3 * r + 0] * C_rel[0] + sc.R[3 * r + 1] * C_rel[1] + sc.R[3 * r + 2] * C_rel[2];
+    for (int a = 0; a < 3; ++a) {
+        Crel[a] = refC[a] - srcC[a];
     }
+    for (int r = 0; r < 3; ++r) {
+        trel[r] = sR[3 * r] * Crel[0] + sR[3 * r + 1] * Crel[1] + sR[3 * r + 2] * Crel[2];
+    }
+    const double sK0 = sc.K[0], sK2 = sc.K[2], sK4 = sc.K[4], sK5 = sc.K[5], sK8 = sc.K[8];
+    for (int c = 0; c < 3; ++c) {
+        M[c] = sK0 * Rrel[c] + sK2 * Rrel[6 + c];
+        M[3 + c] = sK4 * Rrel[3 + c] + sK5 * Rrel[6 + c];
+        M[6 + c] = sK8 * Rrel[6 + c];
+    }
+    const double ifx = 1.0 / (double)rc.K[0], ify = 1.0 / (double)rc.K[4];
+    const double cx = (double)rc.K[2], cy = (double)rc.K[5];
+    for (int r = 0; r < 3; ++r) {
+        v->A[3 * r + 0] = M[3 * r + 0] * ifx;
+        v->A[3 * r + 1] = M[3 * r + 1] * ify;
+        v->A[3 * r + 2] = M[3 * r + 2] - M[3 * r + 0] * cx * ifx - M[3 * r + 1] * cy * ify;
+    }
+    v->b[0] = sK0 * trel[0] + sK2 * trel[2];
+    v->b[1] = sK4 * trel[1] + sK5 * trel[2];
+    v->b[2] = sK8 * trel[2];
 }
 
 /* JudgeExtend thresholds (HPM.cu:1040-1041, double) and view-selection
@@ -199,6 +219,10 @@ int finish_input(hpm_handle* h)
 void fill_problem(const hpm_handle* h, hpm_dev_problem* pb)
 {
     memset(pb, 0, sizeof(*pb));
+    pb->rk_ifx = 1.0 / (double)h->cams[0].K[0];
+    pb->rk_ify = 1.0 / (double)h->cams[0].K[4];
+    pb->rk_cx = (double)h->cams[0].K[2];
+    pb->rk_cy = (double)h->cams[0].K[5];
     pb->W = h->cams[0].width;
     pb->H = h->cams[0].height;
     pb->nsrc = h->num_images - 1;

@@ -85,86 +85,143 @@ static IterConsts make_iter_consts()
     return c;
 }
 
-/* Camera-only part of ComputeHomography (HPM.cu:229-255). */
+/* Plane-induced homography (ComputeHomography, HPM.cu:227-287),
+ * H = Ks (Rrel - trel n^T / w) Kr^-1 with Ks = [sK0 0 sK2; 0 sK4 sK5; 0 0 sK8]
+ * and Kr = [fx 0 cx; 0 fy cy; 0 0 1] as the reference reads them, split as
+ * H = A - b q^T: A = Ks Rrel Kr^-1 and b = Ks trel per source view (double,
+ * from the float cameras), q = Kr^-T n / w per hypothesis. */
 struct ViewRel {
-    float Rrel[9];
-    float trel[3];
+    double A[9];
+    double b[3];
 };
+
+struct RefK {
+    double ifx, ify, cx, cy;
+};
+
+static RefK make_ref_k(const hpm_camera& rc)
+{
+    RefK k;
+    k.ifx = 1.0 / (double)rc.K[0];
+    k.ify = 1.0 / (double)rc.K[4];
+    k.cx = (double)rc.K[2];
+    k.cy = (double)rc.K[5];
+    return k;
+}
 
 static ViewRel make_view_rel(const hpm_camera& rc, const hpm_camera& sc)
 {
-    ViewRel v;
-    float ref_C[3], src_C[3], C_rel[3];
-    ref_C[0] = -(rc.R[0] * rc.t[0] + rc.R[3] * rc.t[1] + rc.R[6] * rc.t[2]);
-    ref_C[1] = -(rc.R[1] * rc.t[0] + rc.R[4] * rc.t[1] + rc.R[7] * rc.t[2]);
-    ref_C[2] = -(rc.R[2] * rc.t[0] + rc.R[5] * rc.t[1] + rc.R[8] * rc.t[2]);
-    src_C[0] = -(sc.R[0] * sc.t[0] + sc.R[3] * sc.t[1] + sc.R[6] * sc.t[2]);
-    src_C[1] = -(sc.R[1] * sc.t[0] + sc.R[4] * sc.t[1] + sc.R[7] * sc.t[2]);
-    src_C[2] = -(sc.R[2] * sc.t[0] + sc.R[5] * sc.t[1] + sc.R[8] * sc.t[2]);
+    double rR[9], sR[9], refC[3], srcC[3], Crel[3], Rrel[9], trel[3], M[9];
+    for (int i = 0; i < 9; ++i) {
+        rR[i] = (double)rc.R[i];
+        sR[i] = (double)sc.R[i];
+    }
+    for (int a = 0; a < 3; ++a) {
+        refC[a] = -(rR[a] * (double)rc.t[0] + rR[3 + a] * (double)rc.t[1] + rR[6 + a] * (double)rc.t[2]);
+        srcC[a] = -(sR[a] * (double)sc.t[0] + sR[3 + a] * (double)sc.t[1] + sR[6 + a] * (double)sc.t[2]);
+    }
     for (int r = 0; r < 3; ++r) {
         for (int c = 0; c < 3; ++c) {
-            v.Rrel[3 * r + c] = sc.R[3 * r + 0] * rc.R[3 * c + 0] + sc.R[3 * r + 1] * rc.R[3 * c + 1] +
-                                sc.R[3 * r + 2] * rc.R[3 * c + 2];
+            Rrel[3 * r + c] = sR[3 * r] * rR[3 * c] + sR[3 * r + 1] * rR[3 * c + 1] + sR[3 * r + 2] * rR[3 * c + 2];
         }
     }
-    C_rel[0] = ref_C[0] - src_C[0];
-    C_rel[1] = ref_C[1] - src_C[1];
-    C_rel[2] = ref_C[2] - src_C[2];
-    for (int r = 0; r < 3; ++r) {
-        v.trel[r] = sc.R[3 * r + 0] * C_rel[0] + sc.R[3 * r + 1] * C_rel[1] + sc.R[3 * r + 2] * C_rel[2];
+    for (int a = 0; a < 3; ++a) {
+        Crel[a] = refC[a] - srcC[a];
     }
+    for (int r = 0; r < 3; ++r) {
+        trel[r] = sR[3 * r] * Crel[0] + sR[3 * r + 1] * Crel[1] + sR[3 * r + 2] * Crel[2];
+    }
+    const double sK0 = sc.K[0], sK2 = sc.K[2], sK4 = sc.K[4], sK5 = sc.K[5], sK8 = sc.K[8];
+    for (int c = 0; c < 3; ++c) {
+        M[c] = sK0 * Rrel[c] + sK2 * Rrel[6 + c];
+        M[3 + c] = sK4 * Rrel[3 + c] + sK5 * Rrel[6 + c];
+        M[6 + c] = sK8 * Rrel[6 + c];
+    }
+    const RefK k = make_ref_k(rc);
+    ViewRel v;
+    for (int r = 0; r < 3; ++r) {
+        v.A[3 * r + 0] = M[3 * r + 0] * k.ifx;
+        v.A[3 * r + 1] = M[3 * r + 1] * k.ify;
+        v.A[3 * r + 2] = M[3 * r + 2] - M[3 * r + 0] * k.cx * k.ifx - M[3 * r + 1] * k.cy * k.ify;
+    }
+    v.b[0] = sK0 * trel[0] + sK2 * trel[2];
+    v.b[1] = sK4 * trel[1] + sK5 * trel[2];
+    v.b[2] = sK8 * trel[2];
     return v;
 }
 
-/* Plane-dependent part of ComputeHomography (HPM.cu:257-286). */
-static void homography(const hpm_camera& rc, const hpm_camera& sc, const ViewRel& v, F4 n, float H[9])
+/* q = Kr^-T n / w for one hypothesis (double) */
+static inline void plane_q(const RefK& k, F4 n, double q[3])
 {
-    float h[9], tmp[9];
-    const float nn[3] = {n.x, n.y, n.z};
+    const double iw = 1.0 / (double)n.w;
+    const double nx = n.x, ny = n.y, nz = n.z;
+    q[0] = (nx * k.ifx) * iw;
+    q[1] = (ny * k.ify) * iw;
+    q[2] = ((nz - (nx * k.cx) * k.ifx) - (ny * k.cy) * k.ify) * iw;
+}
+
+/* The warp of one (hypothesis, source view) around reference pixel p:
+ * the warped centre s0 = H p (ComputeCorrespondingPoint, HPM.cu:289-296)
+ * in double, split into an integer texel (ix0, iy0) and a float fraction,
+ * and the tap offsets of s(p + d) - s0 as a float rational function of d:
+ *   s(p + d) - s0 = (g d) / (1 + e . d). */
+struct Warp {
+    int ix0, iy0;
+    float fx0, fy0;
+    float ex, ey, gxx, gxy, gyx, gyy;
+};
+
+/* Returns false when the warped centre falls outside the source image (the
+ * reference returns the maximum cost there, HPM.cu:333-335). */
+static inline bool make_warp(const ViewRel& v, const double q[3], int px, int py, int SW, int SH, Warp* w)
+{
+    double H[9];
     for (int r = 0; r < 3; ++r) {
         for (int c = 0; c < 3; ++c) {
-            h[3 * r + c] = v.Rrel[3 * r + c] - v.trel[r] * nn[c] / n.w;
+            H[3 * r + c] = fma(-v.b[r], q[c], v.A[3 * r + c]);
         }
     }
-    for (int r = 0; r < 3; ++r) {
-        tmp[3 * r + 0] = h[3 * r + 0] / rc.K[0];
-        tmp[3 * r + 1] = h[3 * r + 1] / rc.K[4];
-        tmp[3 * r + 2] = -h[3 * r + 0] * rc.K[2] / rc.K[0] - h[3 * r + 1] * rc.K[5] / rc.K[4] + h[3 * r + 2];
+    const double X = (double)px, Y = (double)py;
+    const double hx = fma(H[0], X, fma(H[1], Y, H[2]));
+    const double hy = fma(H[3], X, fma(H[4], Y, H[5]));
+    const double hz = fma(H[6], X, fma(H[7], Y, H[8]));
+    const double icz = 1.0 / hz;
+    const double sx0 = hx * icz;
+    const double sy0 = hy * icz;
+    if (!(sx0 >= 0.0 && sx0 < (double)SW && sy0 >= 0.0 && sy0 < (double)SH)) {
+        return false;
     }
-    H[0] = sc.K[0] * tmp[0] + sc.K[2] * tmp[6];
-    H[1] = sc.K[0] * tmp[1] + sc.K[2] * tmp[7];
-    H[2] = sc.K[0] * tmp[2] + sc.K[2] * tmp[8];
-    H[3] = sc.K[4] * tmp[3] + sc.K[5] * tmp[6];
-    H[4] = sc.K[4] * tmp[4] + sc.K[5] * tmp[7];
-    H[5] = sc.K[4] * tmp[5] + sc.K[5] * tmp[8];
-    H[6] = sc.K[8] * tmp[6];
-    H[7] = sc.K[8] * tmp[7];
-    H[8] = sc.K[8] * tmp[8];
+    const double fx = floor(sx0), fy = floor(sy0);
+    w->ix0 = (int)fx;
+    w->iy0 = (int)fy;
+    w->fx0 = (float)(sx0 - fx);
+    w->fy0 = (float)(sy0 - fy);
+    w->ex = (float)(H[6] * icz);
+    w->ey = (float)(H[7] * icz);
+    w->gxx = (float)((H[0] - sx0 * H[6]) * icz);
+    w->gxy = (float)((H[1] - sx0 * H[7]) * icz);
+    w->gyx = (float)((H[3] - sy0 * H[6]) * icz);
+    w->gyy = (float)((H[4] - sy0 * H[7]) * icz);
+    return true;
 }
 
-/* ComputeCorrespondingPoint (HPM.cu:289-296), fma form with one reciprocal. */
-static inline void warp(const float H[9], float X, float Y, float* sx, float* sy)
+/* tex2D with cudaFilterModeLinear and clamp addressing (HPM.cpp:599-617) at
+ * the tap (dx, dy) of a warp; texel centres at integer coordinates (the
+ * reference samples at +0.5). */
+static inline float sample_tap(const float* img, int W, int H, const Warp& w, float dx, float dy)
 {
-    const float px = fmaf(H[0], X, fmaf(H[1], Y, H[2]));
-    const float py = fmaf(H[3], X, fmaf(H[4], Y, H[5]));
-    const float pz = fmaf(H[6], X, fmaf(H[7], Y, H[8]));
-    const float inv = 1.0f / pz;
-    *sx = px * inv;
-    *sy = py * inv;
-}
-
-/* tex2D with cudaFilterModeLinear, clamp addressing (HPM.cpp:599-617);
- * (x, y) are texel-centre coordinates (reference coordinate minus 0.5). */
-static inline float bilinear(const float* img, int W, int H, float x, float y)
-{
-    x = fminf(fmaxf(x, -1.0f), (float)W);
-    y = fminf(fmaxf(y, -1.0f), (float)H);
-    const float fx = floorf(x);
-    const float fy = floorf(y);
-    const float a = x - fx;
-    const float b = y - fy;
-    const int ix = (int)fx;
-    const int iy = (int)fy;
+    const float den = fmaf(w.ex, dx, fmaf(w.ey, dy, 1.0f));
+    const float inv = 1.0f / den;
+    float lx = fmaf(fmaf(w.gxx, dx, w.gxy * dy), inv, w.fx0);
+    float ly = fmaf(fmaf(w.gyx, dx, w.gyy * dy), inv, w.fy0);
+    lx = fminf(fmaxf(lx, -1.0f - (float)w.ix0), (float)(W - w.ix0));
+    ly = fminf(fmaxf(ly, -1.0f - (float)w.iy0), (float)(H - w.iy0));
+    const float flx = floorf(lx);
+    const float fly = floorf(ly);
+    const float a = lx - flx;
+    const float b = ly - fly;
+    const int ix = w.ix0 + (int)flx;
+    const int iy = w.iy0 + (int)fly;
     const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
     const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
     const float t00 = img[(size_t)y0 * W + x0];
@@ -210,8 +267,8 @@ static void make_ref_patch(const float* img, int W, int H, int px, int py, const
                                      cd / (2.0f * P.sigma_color * P.sigma_color));
             const float rr = r - c;
             const float wr = w * rr;
-            rp->dx[k] = (float)(px + i);
-            rp->dy[k] = (float)(py + j);
+            rp->dx[k] = fi;
+            rp->dy[k] = fj;
             rp->w[k] = w;
             rp->rr[k] = rr;
             rp->wr[k] = wr;
@@ -232,30 +289,26 @@ struct Ctx {
     const hpm_oracle_problem* pb;
     int W, H, nsrc;
     ViewRel rel[HPM_MAX_IMAGES];
+    RefK rk;
     IterConsts ic;
     float yy_limit; /* unused placeholder for symmetry */
 };
 
-/* ComputeBilateralNCC (HPM.cu:325-397) for one source view, FP32. */
-static float ncc(const Ctx& cx, const RefPatch& rp, int px, int py, int v, F4 plane)
+/* ComputeBilateralNCC (HPM.cu:325-397) for one source view, FP32 sums;
+ * q from plane_q of the hypothesis. */
+static float ncc(const Ctx& cx, const RefPatch& rp, int px, int py, int v, const double q[3])
 {
     const hpm_oracle_problem* pb = cx.pb;
-    const hpm_camera& rc = pb->cams[0];
     const hpm_camera& sc = pb->cams[v];
     const int SW = sc.width, SH = sc.height;
     const float* simg = pb->images[v];
-    float H[9];
-    homography(rc, sc, cx.rel[v], plane, H);
-    float cx0, cy0;
-    warp(H, (float)px, (float)py, &cx0, &cy0);
-    if (cx0 >= (float)SW || cx0 < 0.0f || cy0 >= (float)SH || cy0 < 0.0f) {
+    Warp w;
+    if (!make_warp(cx.rel[v], q, px, py, SW, SH, &w)) {
         return 2.0f;
     }
     float ss_ = 0.0f, sss = 0.0f, srs = 0.0f;
     for (int k = 0; k < rp.ntap; ++k) {
-        float sx, sy;
-        warp(H, rp.dx[k], rp.dy[k], &sx, &sy);
-        const float s = bilinear(simg, SW, SH, sx, sy) - rp.c;
+        const float s = sample_tap(simg, SW, SH, w, rp.dx[k], rp.dy[k]) - rp.c;
         const float ws = rp.w[k] * s;
         ss_ = ss_ + ws;
         sss = fmaf(ws, s, sss);
@@ -571,17 +624,19 @@ static inline int find_max_idx(const float* c, int n)
     return idx;
 }
 
-/* ComputeMultiViewCostVector (HPM.cu:438-443); only views with want[v]
- * set are evaluated (the others are never read by the caller). */
+/* ComputeMultiViewCostVector (HPM.cu:438-443); with weights given, views
+ * of zero weight are skipped (the caller never reads their terms). */
 static void cost_vector(const Ctx& cx, const RefPatch& rp, int px, int py, F4 plane, float* out,
                         const float* weights, uint64_t* nev)
 {
+    double q[3];
+    plane_q(cx.rk, plane, q);
     for (int v = 1; v <= cx.nsrc; ++v) {
         if (weights != nullptr && !(weights[v - 1] > 0.0f)) {
             out[v - 1] = 0.0f;
             continue;
         }
-        out[v - 1] = ncc(cx, rp, px, py, v, plane);
+        out[v - 1] = ncc(cx, rp, px, py, v, q);
         ++*nev;
     }
 }
@@ -592,8 +647,10 @@ static float initial_cost(const Ctx& cx, const RefPatch& rp, int px, int py, F4 
 {
     float cv[32], cvc[32];
     int count = 0, valid = 0;
+    double q[3];
+    plane_q(cx.rk, plane, q);
     for (int v = 1; v <= cx.nsrc; ++v) {
-        const float c = ncc(cx, rp, px, py, v, plane);
+        const float c = ncc(cx, rp, px, py, v, q);
         ++*nev;
         cv[v - 1] = c;
         cvc[v - 1] = c;
@@ -630,6 +687,7 @@ static void make_ctx(const hpm_oracle_problem* pb, Ctx* cx)
     for (int v = 1; v < pb->num_images; ++v) {
         cx->rel[v] = make_view_rel(pb->cams[0], pb->cams[v]);
     }
+    cx->rk = make_ref_k(pb->cams[0]);
     cx->ic = make_iter_consts();
 }
 
@@ -1464,8 +1522,10 @@ extern "C" int hpm_oracle_cost_vectors(const hpm_oracle_problem* pb, const float
         } else {
             RefPatch rp;
             make_ref_patch(pb->images[0], cx.W, cx.H, px, py, pb->params, &rp);
+            double q[3];
+            plane_q(cx.rk, pl, q);
             for (int v = 1; v <= nsrc; ++v) {
-                out[(size_t)i * nsrc + v - 1] = ncc(cx, rp, px, py, v, pl);
+                out[(size_t)i * nsrc + v - 1] = ncc(cx, rp, px, py, v, q);
             }
         }
     }

@@ -194,12 +194,17 @@ __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& 
     }
     const float* simg = pb.images + sv.img_off;
     float ss = 0.0f, sss = 0.0f, srs = 0.0f;
+    /* source values relative to the tap nearest the window centre (tap 14,
+     * offset (-1, -1)), sampled first: a constant patch sums to exactly 0 */
+    constexpr int kc = 14;
+    const float shift = sample_tap(simg, SW, SH, w, -1.0f, -1.0f);
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
 #pragma unroll
         for (int b = 0; b < 6; ++b) {
             const int k = 6 * a + b;
-            const float s = sample_tap(simg, SW, SH, w, (float)(2 * a - 5), (float)(2 * b - 5)) - rp.c;
+            const float raw = k == kc ? shift : sample_tap(simg, SW, SH, w, (float)(2 * a - 5), (float)(2 * b - 5));
+            const float s = raw - shift;
             const float ws = rp.w[k] * s;
             ss = ss + ws;
             sss = fmaf(ws, s, sss);

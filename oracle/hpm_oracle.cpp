@@ -13,11 +13,12 @@
  *     texel centres at integer coordinates (the reference samples at +0.5);
  *   - random numbers: Philox4x32-10 (hpm_philox.h), one counter stream per
  *     (pixel, launch);
- *   - warped tap coordinates from fma-evaluated homogeneous coordinates and
- *     one reciprocal per tap;
- *   - NCC window sums accumulated relative to the reference centre value
- *     (NCC is invariant to a common shift; this keeps FP32 accurate on
- *     low-texture patches).
+ *   - the homography and the warped window centre in FP64 per (hypothesis,
+ *     source view), the 36 tap offsets from the centre in FP32 with one
+ *     reciprocal per tap;
+ *   - NCC window sums of both signals accumulated relative to the reference
+ *     centre value: NCC is invariant to a shift of either signal, and the
+ *     shifted sums keep FP32 accurate on low-texture patches.
  * The file must be compiled with -ffp-contract=off and without fast-math.
  */
 #include "hpm_oracle.h"
@@ -243,6 +244,7 @@ static inline float texel(const float* img, int W, int H, int x, int y)
  * the reference sums, all relative to the centre value. */
 struct RefPatch {
     int ntap;
+    int centre_tap; /* first tap of smallest offset from the centre */
     float dx[64], dy[64];
     float w[64], rr[64], wr[64];
     float c;
@@ -279,6 +281,14 @@ static void make_ref_patch(const float* img, int W, int H, int px, int py, const
         }
     }
     rp->ntap = k;
+    rp->centre_tap = 0;
+    for (int t = 1; t < k; ++t) {
+        const float dt = rp->dx[t] * rp->dx[t] + rp->dy[t] * rp->dy[t];
+        const float db = rp->dx[rp->centre_tap] * rp->dx[rp->centre_tap] + rp->dy[rp->centre_tap] * rp->dy[rp->centre_tap];
+        if (dt < db) {
+            rp->centre_tap = t;
+        }
+    }
     rp->c = c;
     rp->inv_w = 1.0f / wsum;
     rp->mr = sr * rp->inv_w;
@@ -307,8 +317,13 @@ static float ncc(const Ctx& cx, const RefPatch& rp, int px, int py, int v, const
         return 2.0f;
     }
     float ss_ = 0.0f, sss = 0.0f, srs = 0.0f;
+    /* source values are taken relative to the tap nearest the window
+     * centre, sampled first: a constant source patch then sums to exactly 0 */
+    const int kc = rp.centre_tap;
+    const float shift = sample_tap(simg, SW, SH, w, rp.dx[kc], rp.dy[kc]);
     for (int k = 0; k < rp.ntap; ++k) {
-        const float s = sample_tap(simg, SW, SH, w, rp.dx[k], rp.dy[k]) - rp.c;
+        const float raw = k == kc ? shift : sample_tap(simg, SW, SH, w, rp.dx[k], rp.dy[k]);
+        const float s = raw - shift;
         const float ws = rp.w[k] * s;
         ss_ = ss_ + ws;
         sss = fmaf(ws, s, sss);

@@ -1,0 +1,286 @@
+"""Pins the CPU oracle against fixtures, closed forms and brute force.
+
+Each check is chosen so that a plausible mistake in the oracle fails it:
+  - Philox4x32-10 against known-answer vectors from PyTorch's source header;
+  - exp/sin/cos/acos against libm within a few ulp;
+  - bilateral NCC: identical / affine / negated / constant patches, the
+    out-of-image centre, and a depth sweep whose minimum must sit at the
+    analytic depth of an exact integer disparity;
+  - FP32 cost against the FP64 evaluation of the reference formula;
+  - initial cost and view mask (top-k) against a brute-force sort;
+  - finalisation (plane -> depth, camera -> world normal) in closed form;
+  - the red/black median filter against a brute-force median of its taps;
+  - a half-iteration leaves the other colour untouched; runs are deterministic.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def _lib(cpu):
+    L = cpu.lib()
+    L.hpm_oracle_fmath.argtypes = [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]
+    L.hpm_oracle_philox_block.argtypes = [C.POINTER(C.c_uint32)] * 3
+    L.hpm_oracle_uniforms.argtypes = [C.c_uint32] * 5 + [C.POINTER(C.c_float), C.c_int]
+    return L
+
+
+def test_philox_known_answers(cpu):
+    L = _lib(cpu)
+    kat = json.load(open(os.path.join(ROOT, "tests", "golden", "philox_kat.json")))
+    assert len(kat) >= 5
+    for case in kat:
+        ctr = (C.c_uint32 * 4)(*case["ctr"])
+        key = (C.c_uint32 * 2)(*case["key"])
+        out = (C.c_uint32 * 4)()
+        L.hpm_oracle_philox_block(ctr, key, out)
+        assert list(out) == case["out"], case
+
+
+def test_uniform_stream(cpu):
+    L = _lib(cpu)
+    n = 200000
+    out = np.empty(n, np.float32)
+    L.hpm_oracle_uniforms(1, 2, 3, 4, 5, out.ctypes.data_as(C.POINTER(C.c_float)), n)
+    assert out.min() > 0.0 and out.max() <= 1.0
+    assert abs(out.mean() - 0.5) < 0.005
+    assert abs(out.var() - 1.0 / 12.0) < 0.002
+    # 4 words per Philox block: draw k of the stream is word k % 4 of block k // 4
+    ctr = (C.c_uint32 * 4)(1, 3, 4, 5)
+    key = (C.c_uint32 * 2)(1, 2)
+    blk = (C.c_uint32 * 4)()
+    L.hpm_oracle_philox_block(ctr, key, blk)
+    expect = ((blk[2] >> 8) + 1) * 2.0 ** -24
+    assert out[6] == np.float32(expect)
+
+
+@pytest.mark.parametrize("fn,lo,hi,ref,tol_ulp", [
+    (0, -87.0, 88.0, np.exp, 3),
+    (1, -6.3, 6.3, np.sin, 3),
+    (2, -6.3, 6.3, np.cos, 3),
+    (3, -1.0, 1.0, np.arccos, 4),
+])
+def test_fmath_against_libm(cpu, fn, lo, hi, ref, tol_ulp):
+    L = _lib(cpu)
+    x = np.linspace(lo, hi, 200001, dtype=np.float32)
+    y = np.empty_like(x)
+    L.hpm_oracle_fmath(fn, x.ctypes.data_as(C.POINTER(C.c_float)), y.ctypes.data_as(C.POINTER(C.c_float)), len(x))
+    r = ref(x.astype(np.float64))
+    ulp = np.spacing(np.abs(r).astype(np.float32)).astype(np.float64)
+    err = np.abs(y.astype(np.float64) - r)
+    if fn in (1, 2):  # absolute error near the zeros of sin / cos
+        assert np.all(err <= np.maximum(tol_ulp * ulp, 2e-7))
+    else:
+        assert np.all(err <= tol_ulp * ulp)
+
+
+def _cam(hpm, fx, cx, cy, W, H, tx=0.0, dmin=1.0, dmax=10.0):
+    K = [fx, 0, cx, 0, fx, cy, 0, 0, 1]
+    R = np.eye(3)
+    return hpm.camera_from_arrays(K, R, [tx, 0.0, 0.0], W, H, dmin, dmax)
+
+
+def _texture(W, H, seed):
+    rng = np.random.default_rng(seed)
+    coarse = rng.uniform(40, 215, (H // 4 + 2, W // 4 + 2))
+    img = np.kron(coarse, np.ones((4, 4)))[:H, :W]
+    return (img + rng.normal(0, 3, (H, W))).clip(0, 255).round().astype(np.float32)
+
+
+def _fronto(hpm, cam, depth, px, py):
+    # plane n = (0, 0, -1), d = depth: n.X + d = 0 at Z = depth
+    return np.array([0.0, 0.0, -1.0, depth], np.float32)
+
+
+def _costs(cpu, cams, images, planes, pix, fp64=False):
+    o = cpu.Oracle(cams, images, cpu.relaxed_params(cams))
+    return o.cost_vectors(np.asarray(planes, np.float32), np.asarray(pix, np.int32), fp64=fp64)
+
+
+def test_ncc_invariants(hpm, cpu):
+    W, H = 64, 48
+    ref = _texture(W, H, 0)
+    cam = _cam(hpm, 60.0, 32.0, 24.0, W, H)
+    pix = [(32, 24), (20, 30), (40, 15)]
+    planes = [_fronto(hpm, cam, 4.0, x, y) for x, y in pix]
+    srcs = [ref.copy(), 2.0 * ref + 7.0, 255.0 - ref, np.full_like(ref, 100.0)]
+    cams = [cam] * 5
+    for fp64 in (False, True):
+        c = _costs(cpu, cams, [ref] + srcs, planes, pix, fp64)
+        # FP32 sums are centred on the reference centre value; a source far
+        # from it in intensity (2I + 7, 255 - I) keeps about 5 digits
+        tol = 1e-9 if fp64 else 5e-5
+        assert np.all(np.abs(c[:, 0]) <= 1e-6), c[:, 0]          # identical
+        assert np.all(np.abs(c[:, 1]) <= tol), c[:, 1]           # a*I + b, a > 0
+        assert np.all(np.abs(c[:, 2] - 2.0) <= tol), c[:, 2]     # negated contrast
+        assert np.all(c[:, 3] == 2.0)                            # constant source: var < 1e-5
+    # warped centre outside the source image -> 2
+    far = _cam(hpm, 60.0, 32.0, 24.0, W, H, tx=-10.0)
+    c = _costs(cpu, [cam, far], [ref, ref], planes, pix)
+    assert np.all(c == 2.0)
+
+
+def test_ncc_depth_sweep_closed_form(hpm, cpu):
+    """A source camera translated by B along x sees a fronto-parallel plane
+    at depth Z shifted by fx*B/Z pixels; with an exact 4-pixel shift the
+    cost is 0 at Z and the sweep minimum sits there."""
+    W, H = 96, 64
+    fx, Z, shift = 80.0, 5.0, 4
+    B = shift * Z / fx
+    ref = _texture(W, H, 1)
+    src = np.empty_like(ref)
+    src[:, :W - shift] = ref[:, shift:]
+    src[:, W - shift:] = ref[:, -1:]
+    # x_src = x_ref - fx*B/Z: camera centre at +B (t = -R C = -B)
+    rc = _cam(hpm, fx, 48.0, 32.0, W, H)
+    sc = _cam(hpm, fx, 48.0, 32.0, W, H, tx=-B)
+    pix = [(50, 30), (30, 20), (60, 40)]
+    depths = np.linspace(3.0, 8.0, 101)
+    best = []
+    for x, y in pix:
+        planes = [np.array([0, 0, -1, z], np.float32) for z in depths]
+        c = _costs(cpu, [rc, sc], [ref, src], planes, [(x, y)] * len(depths))[:, 0]
+        best.append(depths[np.argmin(c)])
+        at = _costs(cpu, [rc, sc], [ref, src], [np.array([0, 0, -1, Z], np.float32)], [(x, y)])[0, 0]
+        assert at <= 1e-5
+    assert np.allclose(best, Z, atol=0.051)
+
+
+def test_fp32_cost_matches_fp64(hpm, cpu, c1_small):
+    cfg, (cams, images, gd, gn, low, order) = c1_small
+    rng = np.random.default_rng(3)
+    W, H = cams[0].width, cams[0].height
+    n = 3000
+    xs, ys = rng.integers(0, W, n), rng.integers(0, H, n)
+    nrm = rng.normal(size=(n, 3))
+    nrm[:, 2] = -np.abs(nrm[:, 2]) - 0.5
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    d = rng.uniform(2.0, 8.0, n)
+    K = np.array(cams[0].K).reshape(3, 3)
+    X = np.stack([d * (xs - K[0, 2]) / K[0, 0], d * (ys - K[1, 2]) / K[1, 1], d], 1)
+    planes = np.concatenate([nrm, -(nrm * X).sum(1, keepdims=True)], 1).astype(np.float32)
+    pix = np.stack([xs, ys], 1)
+    f32 = _costs(cpu, cams, images, planes, pix)
+    f64 = _costs(cpu, cams, images, planes, pix, fp64=True)
+    flip = (f32 == 2.0) != (f64 == 2.0)
+    assert flip.mean() < 2e-3
+    assert np.abs(f32 - f64)[~flip].max() <= 1e-4
+
+
+def test_initial_cost_topk_brute_force(hpm, cpu, c1_small):
+    cfg, (cams, images, gd, gn, low, order) = c1_small
+    o = cpu.Oracle(cams, images, cpu.relaxed_params(cams))
+    o.init()
+    H, W = o.cost.shape
+    rng = np.random.default_rng(4)
+    idx = rng.integers(0, W * H, 500)
+    xs, ys = idx % W, idx // W
+    planes = o.hyp.reshape(-1, 4)[idx]
+    cv = _costs(cpu, cams, images, planes, np.stack([xs, ys], 1)).astype(np.float32)
+    for k in range(len(idx)):
+        c = cv[k]
+        valid = int((c < 2.0).sum())
+        top = min(valid, 4)
+        if top == 0:
+            exp_cost, exp_mask = 2.0, 0
+        else:
+            s = np.sort(c)
+            acc = np.float32(0)
+            for v in s[:top]:
+                acc = np.float32(acc + v)
+            exp_cost = np.float32(acc / np.float32(top))
+            exp_mask = int(sum(1 << i for i in range(len(c)) if c[i] <= s[top - 1]))
+        assert o.cost.reshape(-1)[idx[k]] == exp_cost
+        assert int(o.sel.reshape(-1)[idx[k]]) == exp_mask
+
+
+def test_finalize_closed_form(hpm, cpu, c1_small):
+    cfg, (cams, images, gd, gn, low, order) = c1_small
+    o = cpu.Oracle(cams, images, cpu.relaxed_params(cams))
+    o.init()
+    before = o.hyp.copy()
+    o.finalize()
+    H, W = o.cost.shape
+    K = np.array(cams[0].K, np.float64).reshape(3, 3)
+    R = np.array(cams[0].R, np.float64).reshape(3, 3)
+    ys, xs = np.mgrid[0:H, 0:W]
+    n = before[..., :3].astype(np.float64)
+    d = before[..., 3].astype(np.float64)
+    terms = [(xs - K[0, 2]) * n[..., 0], (K[0, 0] / K[1, 1]) * (ys - K[1, 2]) * n[..., 1], K[0, 0] * n[..., 2]]
+    den = terms[0] + terms[1] + terms[2]
+    depth = -d * K[0, 0] / den
+    # FP32 rounding bound of the reference expression: the denominator
+    # cancels when the plane is seen at grazing incidence
+    cond = (np.abs(terms[0]) + np.abs(terms[1]) + np.abs(terms[2])) / np.abs(den)
+    bound = 8 * 2.0 ** -24 * (cond + 2)
+    rel = np.abs(o.hyp[..., 3] - depth) / np.abs(depth)
+    assert np.all(rel <= bound)
+    assert np.allclose(o.hyp[..., :3], n @ R, atol=1e-6)  # R^T n per pixel
+
+
+FILTER_TAPS = [  # (dx, dy, condition) of CheckerboardFilter (HPM.cu:1801-1871)
+    (0, -1, lambda x, y, W, H: y > 0), (0, -3, lambda x, y, W, H: y > 2), (0, -5, lambda x, y, W, H: y > 4),
+    (0, 1, lambda x, y, W, H: y < H - 1), (0, 3, lambda x, y, W, H: y < H - 3), (0, 5, lambda x, y, W, H: y < H - 5),
+    (-1, 0, lambda x, y, W, H: x > 0), (-3, 0, lambda x, y, W, H: x > 2), (-5, 0, lambda x, y, W, H: x > 4),
+    (1, 0, lambda x, y, W, H: x < W - 1), (3, 0, lambda x, y, W, H: x < W - 3), (5, 0, lambda x, y, W, H: x < W - 5),
+    (2, -1, lambda x, y, W, H: y > 0 and x < W - 2), (2, 1, lambda x, y, W, H: y < H - 1 and x < W - 2),
+    (-2, -1, lambda x, y, W, H: y > 0 and x > 1), (-2, 1, lambda x, y, W, H: y < H - 1 and x > 1),
+    (-1, -2, lambda x, y, W, H: x > 0 and y > 2), (1, -2, lambda x, y, W, H: x < W - 1 and y > 2),
+    (-1, 2, lambda x, y, W, H: x > 0 and y < H - 2), (1, 2, lambda x, y, W, H: x < W - 1 and y < H - 2),
+]
+
+
+def test_filter_brute_force(hpm, cpu, c1_small):
+    cfg, (cams, images, gd, gn, low, order) = c1_small
+    o = cpu.Oracle(cams, images, cpu.relaxed_params(cams))
+    o.run_patchmatch()  # realistic state, then filter once more from here
+    before = o.hyp.copy()
+    o.filter(0)
+    H, W = o.cost.shape
+    for y in range(H):
+        for x in range(W):
+            if (x + y) % 2 == 1:
+                assert o.hyp[y, x, 3] == before[y, x, 3]
+                continue
+            if o.cost[y, x] < np.float32(0.001):
+                assert o.hyp[y, x, 3] == before[y, x, 3]
+                continue
+            vals = [before[y, x, 3]] + [before[y + dy, x + dx, 3] for dx, dy, ok in FILTER_TAPS
+                                         if ok(x, y, W, H)]
+            s = np.sort(np.array(vals, np.float32))
+            m = len(s) // 2
+            exp = s[m] if len(s) % 2 else np.float32((s[m - 1] + s[m]) / np.float32(2))
+            assert o.hyp[y, x, 3] == exp, (x, y)
+
+
+def test_half_iteration_touches_one_colour_and_is_deterministic(hpm, cpu, c1_small):
+    cfg, (cams, images, gd, gn, low, order) = c1_small
+    runs = []
+    for _ in range(2):
+        o = cpu.Oracle(cams, images, cpu.relaxed_params(cams))
+        o.init()
+        before = o.hyp.copy()
+        o.half_iteration(0, 0)
+        H, W = o.cost.shape
+        ys, xs = np.mgrid[0:H, 0:W]
+        red = (xs + ys) % 2 == 1
+        assert np.array_equal(o.hyp[red].view(np.uint32), before[red].view(np.uint32))
+        assert not np.array_equal(o.hyp[~red], before[~red])
+        runs.append(o.hyp.copy())
+    assert np.array_equal(runs[0].view(np.uint32), runs[1].view(np.uint32))
+
+
+def test_plain_run_reaches_ground_truth(hpm, cpu, c1_small):
+    cfg, (cams, images, gd, gn, low, order) = c1_small
+    o = cpu.Oracle(cams, images, cpu.relaxed_params(cams))
+    o.run_patchmatch()
+    g = gd[order[0]]
+    acc = np.mean(np.abs(o.hyp[..., 3] - g) / g < 0.02)
+    # a random hypothesis is within 2% about 2.5% of the time; at 160x120
+    # with a 7.5 cm baseline 2% of depth is under 0.1 px of disparity
+    assert acc > 0.3, acc

@@ -76,6 +76,17 @@ int hpm_launch_cost_vectors(const hpm_dev_problem* pb, const float* planes, cons
 int hpm_launch_jbu(const float* guide, int w, int h, const float* src_nd, int sw, int sh, int scale_i,
                    float* out_depth, float* out_normal, void* stream);
 
+/* Copies n finished hypotheses (float4) and costs into a pool slot and the
+ * depth (.w) into a depth map. */
+int hpm_launch_store_state(const float* hyp, const float* cost, int64_t n, float* out_hyp, float* out_cost,
+                           float* out_depth, void* stream);
+/* JBU of a low-resolution prior followed by the reload into camera-frame
+ * prior planes and masks (main.cpp:179-204): mask 1 where the upsampled
+ * depth lies in [depth_min, depth_max], else 0. */
+int hpm_launch_jbu_prior(const float* guide, int w, int h, const float* src_nd, int sw, int sh, int scale_i,
+                         const hpm_camera* ref, float depth_min, float depth_max, float* planes, uint32_t* mask,
+                         void* stream);
+
 /* kind 0..3: exp/sin/cos/acos of in[i]; kind 4: Philox blocks */
 int hpm_launch_selftest(int kind, const void* in, void* out, int n, void* stream);
 

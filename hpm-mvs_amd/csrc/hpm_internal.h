@@ -1,0 +1,99 @@
+/*
+ * hpm_internal.h — the runtime objects behind the C-ABI handles, shared by
+ * hpm_runtime.cpp (the per-problem `HPM` equivalent) and hpm_pipeline.cpp
+ * (the `ProcessProblem` / `main` pass schedule).
+ *
+ * hpm_pool: every view of a scene resident on one device. Images are
+ * uploaded once; per view the pool also keeps the last plain-pass depth map
+ * (the reference's depths.dmb), the last geometric-consistency depth map
+ * (depths_geom.dmb) and the last (world normal, depth) hypotheses and costs
+ * (normals.dmb, costs.dmb), so passes hand results to each other on the
+ * device instead of through files (HPM.cpp:561-588, 662-699).
+ */
+#ifndef HPM_INTERNAL_H_
+#define HPM_INTERNAL_H_
+
+#include <hip/hip_runtime_api.h>
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "hpm_capi.h"
+#include "hpm_device.h"
+
+struct hpm_pool {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n = 0;
+    std::vector<hpm_camera> cams;             /* after the max_image_size downscale */
+    std::vector<std::vector<float>> images;   /* host copies (JBU guide, GetReferenceImage) */
+    std::vector<int64_t> off;                 /* pixel offset of each view in the device arrays */
+    int64_t total = 0;
+    float* d_images = nullptr;
+    float* d_depth_plain = nullptr;           /* depths.dmb of each view */
+    float* d_depth_geom = nullptr;            /* depths_geom.dmb of each view */
+    float* d_state_hyp = nullptr;             /* float4: world normal, depth of the last pass */
+    float* d_state_cost = nullptr;
+    std::vector<char> have_state;
+    std::string err;
+};
+
+struct hpm_handle {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hpm_params params;
+    int ref_id = 0;
+    int num_images = 0;
+
+    hpm_pool* pool = nullptr;   /* images of this problem */
+    bool own_pool = false;
+    std::vector<int> views;     /* pool view of image i (0 = reference) */
+    std::vector<hpm_camera> cams;
+
+    bool have_input = false;
+    bool have_device = false;
+    uint32_t call_index = 0;
+
+    /* per-problem device state */
+    hpm_dev_view* d_views = nullptr;
+    float* d_hyp = nullptr;
+    float* d_cost = nullptr;
+    uint32_t* d_sel = nullptr;
+    float* d_prior = nullptr;
+    uint32_t* d_mask = nullptr;
+    unsigned long long* d_neval = nullptr;
+    size_t alloc_pixels = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool have_prior = false;
+
+    /* host mirrors */
+    std::vector<float> h_hyp;
+    std::vector<float> h_cost;
+    hpm_run_stats stats;
+};
+
+/* error helpers */
+int hpm_fail(hpm_handle* h, int code, const char* fmt, ...);
+
+/* RunPatchMatch with an optional host readback (the C-ABI always reads
+ * back; the pass schedule skips it where the host does not need the maps). */
+int hpm_run_internal(hpm_handle* h, bool readback);
+/* Copies the handle's finished (world normal, depth) hypotheses and costs
+ * into the pool slot of its reference view and the depth into the plain
+ * (which_depth 0) or geometric (1) depth map of that view. */
+int hpm_store_to_pool(hpm_handle* h, int which_depth);
+/* Loads the pool's last state of the reference view as the starting state
+ * of a geometric-consistency run (HPM.cpp:662-699). */
+int hpm_load_from_pool(hpm_handle* h);
+/* JBU of a low-resolution prior (float4 world normal + depth, sw x sh, on
+ * the host) guided by the reference image already on the device, followed by
+ * the reload of main.cpp:179-204 on the device: mask 1 where the upsampled
+ * depth is within the relaxed range, camera-frame planes there. Leaves the
+ * prior uploaded on the handle. Returns 1 (nothing done) when the integer
+ * scale is 1. */
+int hpm_jbu_prior(hpm_handle* h, const float* src_nd, int sw, int sh, float factor);
+
+#endif /* HPM_INTERNAL_H_ */

@@ -1144,15 +1144,11 @@ __global__ __launch_bounds__(256) void hpm_k_cost_vectors(hpm_dev_problem pb, co
     }
 }
 
-/* JBU_cu (HPM.cu:1984-2052) */
-__global__ __launch_bounds__(256) void hpm_k_jbu(const float* guide, int w, int h, const float* src_nd, int sw,
-                                                 int sh, int scale_i, float* out_depth, float* out_normal)
+/* JBU_cu (HPM.cu:1984-2052) for one output pixel: SpatialGauss /
+ * RangeGauss (HPM.cu:1971-1981) in FP32, NormalizeVec3 with 1/sqrtf */
+__device__ __forceinline__ void jbu_pixel(const float* guide, int w, int h, const float* src_nd, int sw, int sh,
+                                          int scale_i, int px, int py, float* depth_out, float4* normal_out)
 {
-    const int px = blockIdx.x * blockDim.x + threadIdx.x;
-    const int py = blockIdx.y * blockDim.y + threadIdx.y;
-    if (px >= w || py >= h) {
-        return;
-    }
     const float scale = (float)(1.0 * sw / w);
     const float sigmad = 0.50f, sigmar = 25.5f;
     const int win = scale_i * scale_i + 1;
@@ -1193,9 +1189,66 @@ __global__ __launch_bounds__(256) void hpm_k_jbu(const float* guide, int w, int 
     ntot.y = ntot.y / nf;
     ntot.z = ntot.z / nf;
     normalize3(&ntot);
+    *depth_out = ctot / nf;
+    *normal_out = make_float4(ntot.x, ntot.y, ntot.z, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void hpm_k_jbu(const float* guide, int w, int h, const float* src_nd, int sw,
+                                                 int sh, int scale_i, float* out_depth, float* out_normal)
+{
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    const int py = blockIdx.y * blockDim.y + threadIdx.y;
+    if (px >= w || py >= h) {
+        return;
+    }
+    float d;
+    float4 n;
+    jbu_pixel(guide, w, h, src_nd, sw, sh, scale_i, px, py, &d, &n);
     const int center = py * w + px;
-    out_depth[center] = ctot / nf;
-    st4(out_normal, center, make_float4(ntot.x, ntot.y, ntot.z, 0.0f));
+    out_depth[center] = d;
+    st4(out_normal, center, n);
+}
+
+/* JBU followed by the prior reload of main.cpp:179-204 (scale 0) and
+ * main.cpp:330-353 (scale 1): TransformNormal2RefCam (HPM.cpp:882-890) and
+ * GetDistance2Origin_factor (HPM.cpp:892-899, full-resolution K) */
+__global__ __launch_bounds__(256) void hpm_k_jbu_prior(const float* guide, int w, int h, const float* src_nd, int sw,
+                                                       int sh, int scale_i, hpm_camera ref, float depth_min,
+                                                       float depth_max, float* planes, uint32_t* mask)
+{
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    const int py = blockIdx.y * blockDim.y + threadIdx.y;
+    if (px >= w || py >= h) {
+        return;
+    }
+    float d;
+    float4 n;
+    jbu_pixel(guide, w, h, src_nd, sw, sh, scale_i, px, py, &d, &n);
+    const int center = py * w + px;
+    uint32_t m = 0u;
+    float4 pl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (d <= depth_max && d >= depth_min) {
+        m = 1u;
+        pl.x = ref.R[0] * n.x + ref.R[1] * n.y + ref.R[2] * n.z;
+        pl.y = ref.R[3] * n.x + ref.R[4] * n.y + ref.R[5] * n.z;
+        pl.z = ref.R[6] * n.x + ref.R[7] * n.y + ref.R[8] * n.z;
+        pl.w = dist_to_origin(ref, px, py, d, pl);
+    }
+    mask[center] = m;
+    st4(planes, center, pl);
+}
+
+__global__ __launch_bounds__(256) void hpm_k_store_state(const float* hyp, const float* cost, int64_t n,
+                                                         float* out_hyp, float* out_cost, float* out_depth)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const float4 v = reinterpret_cast<const float4*>(hyp)[i];
+    reinterpret_cast<float4*>(out_hyp)[i] = v;
+    out_cost[i] = cost[i];
+    out_depth[i] = v.w;
 }
 
 __global__ __launch_bounds__(256) void hpm_k_selftest(int kind, const void* in, void* out, int n)
@@ -1295,5 +1348,26 @@ extern "C" int hpm_launch_selftest(int kind, const void* in, void* out, int n, v
         return 0;
     }
     hipLaunchKernelGGL(hpm_k_selftest, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, kind, in, out, n);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_store_state(const float* hyp, const float* cost, int64_t n, float* out_hyp, float* out_cost,
+                                      float* out_depth, void* stream)
+{
+    if (n <= 0) {
+        return 0;
+    }
+    hipLaunchKernelGGL(hpm_k_store_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, hyp,
+                       cost, n, out_hyp, out_cost, out_depth);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_jbu_prior(const float* guide, int w, int h, const float* src_nd, int sw, int sh, int scale_i,
+                                    const hpm_camera* ref, float depth_min, float depth_max, float* planes,
+                                    uint32_t* mask, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_jbu_prior, full_grid(w, h, blk), blk, 0, (hipStream_t)stream, guide, w, h, src_nd, sw,
+                       sh, scale_i, *ref, depth_min, depth_max, planes, mask);
     return (int)hipGetLastError();
 }

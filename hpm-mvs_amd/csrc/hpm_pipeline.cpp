@@ -1,0 +1,454 @@
+/*
+ * hpm_pipeline.cpp — ProcessProblem and the pass schedule of main()
+ * (HPM-MVS/main.cpp:35-499, 636-674) on a device-resident pool
+ * (include/hpm_pipeline.h).
+ */
+#include "hpm_pipeline.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include <thread>
+
+#include "hpm_internal.h"
+#include "hpm_io.h"
+#include "hpm_prior.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_since(clk::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+}
+
+struct Ctx {
+    hpm_pool* pool;
+    const hpm_pipeline_options* opt;
+    hpm_pipeline_stats* st;
+    std::string err;
+    int threads;
+};
+
+int fail(Ctx& c, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c.err = buf;
+    return HPM_EINVAL;
+}
+
+int check(Ctx& c, hpm_handle* h, int rc, const char* what)
+{
+    if (rc < 0) {
+        c.err = std::string(what) + ": " + h->err;
+    }
+    return rc;
+}
+
+void account(Ctx& c, const hpm_handle* h)
+{
+    c.st->ms_device += h->stats.ms_total;
+    c.st->ms_propagate += h->stats.ms_propagate;
+    c.st->n_eval += h->stats.n_eval;
+    c.st->n_eval_propagate += h->stats.n_eval_propagate;
+    c.st->runs += 1;
+    c.st->propagate_launches += 6;
+}
+
+/* host maps of the current result: depth, world normal (3 floats), cost */
+struct Maps {
+    int W = 0, H = 0;
+    std::vector<float> depth, normal, cost;
+};
+
+void maps_from_handle(const hpm_handle* h, Maps* m)
+{
+    m->W = h->cams[0].width;
+    m->H = h->cams[0].height;
+    const size_t np = (size_t)m->W * m->H;
+    m->depth.resize(np);
+    m->normal.resize(np * 3);
+    m->cost.assign(h->h_cost.begin(), h->h_cost.end());
+    for (size_t i = 0; i < np; ++i) {
+        m->normal[3 * i + 0] = h->h_hyp[4 * i + 0];
+        m->normal[3 * i + 1] = h->h_hyp[4 * i + 1];
+        m->normal[3 * i + 2] = h->h_hyp[4 * i + 2];
+        m->depth[i] = h->h_hyp[4 * i + 3];
+    }
+}
+
+/* TransformNormal (HPM.cpp:872-880): camera -> world */
+void to_world(const hpm_camera& c, const float n[4], float out[3])
+{
+    out[0] = c.R[0] * n[0] + c.R[3] * n[1] + c.R[6] * n[2];
+    out[1] = c.R[1] * n[0] + c.R[4] * n[1] + c.R[7] * n[2];
+    out[2] = c.R[2] * n[0] + c.R[5] * n[1] + c.R[8] * n[2];
+}
+
+/* Support points, Delaunay triangles, rasterised triangle mask, triangle
+ * planes and the KNN supplement at one scale (main.cpp:84-121,
+ * 238-273, 387-424). mask: idx+1 inside triangles, -233 for KNN planes. */
+struct PriorScale {
+    int w = 0, h = 0;
+    std::vector<float> mask;
+    std::vector<float> tri_planes; /* float4 per triangle */
+    std::vector<float> knn_planes; /* float4 per pixel */
+};
+
+void build_prior_scale(Ctx& c, const hpm_camera& ref, const float* depth, const float* cost, int w, int h,
+                       float factor, float dmin, float dmax, PriorScale* ps)
+{
+    ps->w = w;
+    ps->h = h;
+    std::vector<int32_t> pts((size_t)(w / 5 + 1) * (h / 5 + 1) * 2);
+    const int npts = hpm_support_points(cost, w, h, pts.data(), (int)(pts.size() / 2));
+    pts.resize((size_t)npts * 2);
+    std::vector<int32_t> tris;
+    if (npts >= 3) {
+        const int nt = hpm_delaunay(pts.data(), npts, nullptr, 0);
+        tris.resize((size_t)std::max(nt, 0) * 3);
+        hpm_delaunay(pts.data(), npts, tris.data(), nt);
+    }
+    const int ntris = (int)(tris.size() / 3);
+    ps->mask.assign((size_t)w * h, 0.0f);
+    hpm_rasterize_triangles(pts.data(), tris.data(), ntris, ps->mask.data(), w, h);
+    ps->tri_planes.assign((size_t)ntris * 4, 0.0f);
+    for (int t = 0; t < ntris; ++t) {
+        const int32_t p[6] = {pts[2 * tris[3 * t]], pts[2 * tris[3 * t] + 1], pts[2 * tris[3 * t + 1]],
+                              pts[2 * tris[3 * t + 1] + 1], pts[2 * tris[3 * t + 2]], pts[2 * tris[3 * t + 2] + 1]};
+        hpm_plane_from_points(&ref, p, depth, w, factor, &ps->tri_planes[4 * (size_t)t]);
+    }
+    ps->knn_planes.assign((size_t)w * h * 4, 0.0f);
+    hpm_knn_supplement(&ref, pts.data(), npts, depth, w, h, factor, c.opt->params.knn_k, dmin, dmax, ps->mask.data(),
+                       ps->knn_planes.data(), c.threads);
+}
+
+/* ProcessProblem (main.cpp:35-499) for phase A. */
+int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
+{
+    hpm_params p = c.opt->params;
+    p.planar_prior = 0;
+    p.geom_consistency = 0;
+    p.multi_geometry = 0;
+    h->params = p;
+    int rc = check(c, h, hpm_input_init_pool(h, c.pool, ref, srcs, nsrc), "input init");
+    if (rc < 0) return rc;
+    rc = check(c, h, hpm_device_init(h, nullptr), "device init");
+    if (rc < 0) return rc;
+    rc = check(c, h, hpm_run_internal(h, true), "RunPatchMatch");
+    if (rc < 0) return rc;
+    account(c, h);
+    const hpm_camera ref_cam = h->cams[0];
+    const int W = ref_cam.width, H = ref_cam.height;
+    const float dmin = h->params.depth_min, dmax = h->params.depth_max;
+    if (c.opt->planar_prior) {
+        hpm_set_planar_prior(h); /* SetPlanarPriorParams (HPM.cpp:477-480) */
+        Maps m;
+        maps_from_handle(h, &m);
+        const int k = hpm_multiscale_count(W, H);
+        for (int i = 2 - k; i <= 2; ++i) {
+            auto t0 = clk::now();
+            if (i == 0 || i == 1) {
+                const float f = i == 0 ? 0.25f : 0.5f;
+                const int ws = (int)lround(W * (double)f), hs = (int)lround(H * (double)f);
+                std::vector<float> ds((size_t)ws * hs), cs((size_t)ws * hs), ns((size_t)ws * hs * 3);
+                hpm_resize_bilinear_c(m.depth.data(), W, H, 1, ds.data(), ws, hs);
+                hpm_resize_bilinear_c(m.cost.data(), W, H, 1, cs.data(), ws, hs);
+                hpm_resize_bilinear_c(m.normal.data(), W, H, 3, ns.data(), ws, hs);
+                PriorScale ps;
+                build_prior_scale(c, ref_cam, ds.data(), cs.data(), ws, hs, f, dmin, dmax, &ps);
+                /* low-scale prior maps (main.cpp:124-162, 277-314), world normals */
+                std::vector<float> src_nd((size_t)ws * hs * 4, 0.0f);
+                for (int x = 0; x < ws; ++x) {
+                    for (int y = 0; y < hs; ++y) {
+                        const size_t q = (size_t)y * ws + x;
+                        float& mk = ps.mask[q];
+                        const float* plane = nullptr;
+                        bool ok = false;
+                        if (mk > 0) {
+                            plane = &ps.tri_planes[4 * (size_t)((int)mk - 1)];
+                            const float d = hpm_depth_from_plane_factor(&ref_cam, plane, x, y, f);
+                            ok = d <= dmax * 1.2f && d >= dmin * 0.6f;
+                            if (ok) src_nd[4 * q + 3] = d;
+                        } else if (mk < -200) {
+                            plane = &ps.knn_planes[4 * q];
+                            const float d = hpm_depth_from_plane_factor(&ref_cam, plane, x, y, f);
+                            ok = d <= dmax && d >= dmin;
+                            if (ok) src_nd[4 * q + 3] = d;
+                        }
+                        if (ok) {
+                            to_world(ref_cam, plane, &src_nd[4 * q]);
+                        } else {
+                            mk = 0;
+                        }
+                    }
+                }
+                c.st->ms_prior_host += ms_since(t0);
+                auto t1 = clk::now();
+                rc = check(c, h, hpm_jbu_prior(h, src_nd.data(), ws, hs, f), "JBU prior");
+                if (rc < 0) return rc;
+                if (rc == 1) {
+                    /* integer scale 1: JointBilateralUpsampling returns without
+                     * output (HPM.cpp:1219-1222), the upsampled maps stay 0 and
+                     * every mask is 0 */
+                    const size_t np = (size_t)W * H;
+                    std::vector<float> zp(np * 4, 0.0f);
+                    std::vector<uint32_t> zm(np, 0u);
+                    rc = check(c, h, hpm_upload_prior(h, zp.data(), zm.data()), "prior upload");
+                    if (rc < 0) return rc;
+                }
+                c.st->ms_prior_device += ms_since(t1);
+            } else {
+                PriorScale ps;
+                build_prior_scale(c, ref_cam, m.depth.data(), m.cost.data(), W, H, 1.0f, dmin, dmax, &ps);
+                /* range checks of main.cpp:427-454 and the upload of
+                 * CudaPlanarPriorInitializationSupplement (HPM.cpp:1099-1123):
+                 * KNN pixels count as active (Q12) */
+                const size_t np = (size_t)W * H;
+                std::vector<float> planes(np * 4, 0.0f);
+                std::vector<uint32_t> mask(np, 0u);
+                for (int x = 0; x < W; ++x) {
+                    for (int y = 0; y < H; ++y) {
+                        const size_t q = (size_t)y * W + x;
+                        const float mk = ps.mask[q];
+                        const float* plane = nullptr;
+                        if (mk > 0) {
+                            plane = &ps.tri_planes[4 * (size_t)((int)mk - 1)];
+                        } else if (mk < -200) {
+                            plane = &ps.knn_planes[4 * q];
+                        }
+                        if (plane) {
+                            const float d = hpm_depth_from_plane_factor(&ref_cam, plane, x, y, 1.0f);
+                            if (d <= dmax && d >= dmin) {
+                                memcpy(&planes[4 * q], plane, 16);
+                                mask[q] = 1u;
+                            }
+                        }
+                    }
+                }
+                c.st->ms_prior_host += ms_since(t0);
+                auto t1 = clk::now();
+                rc = check(c, h, hpm_upload_prior(h, planes.data(), mask.data()), "prior upload");
+                if (rc < 0) return rc;
+                c.st->ms_prior_device += ms_since(t1);
+            }
+            rc = check(c, h, hpm_run_internal(h, true), "RunPatchMatch (prior)");
+            if (rc < 0) return rc;
+            account(c, h);
+            maps_from_handle(h, &m);
+        }
+    }
+    return check(c, h, hpm_store_to_pool(h, 0), "store");
+}
+
+/* ProcessProblem with geometric consistency (main.cpp:44-51, 481-490) */
+int process_phase_b(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc, bool multi)
+{
+    hpm_params p = c.opt->params;
+    p.planar_prior = 0;
+    p.geom_consistency = 0;
+    p.multi_geometry = 0;
+    h->params = p;
+    hpm_set_geom_consistency(h, multi ? 1 : 0);
+    int rc = check(c, h, hpm_input_init_pool(h, c.pool, ref, srcs, nsrc), "input init");
+    if (rc < 0) return rc;
+    rc = check(c, h, hpm_device_init(h, nullptr), "device init");
+    if (rc < 0) return rc;
+    rc = check(c, h, hpm_load_from_pool(h), "reload");
+    if (rc < 0) return rc;
+    rc = check(c, h, hpm_run_internal(h, false), "RunPatchMatch (geometric)");
+    if (rc < 0) return rc;
+    account(c, h);
+    return check(c, h, hpm_store_to_pool(h, 1), "store");
+}
+
+} // namespace
+
+extern "C" void hpm_pipeline_default(hpm_pipeline_options* opt)
+{
+    memset(opt, 0, sizeof(*opt));
+    hpm_params_default(&opt->params);
+    opt->planar_prior = 1;
+    opt->geom_consistency = 1;
+    opt->num_threads = 0;
+}
+
+extern "C" int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, float* cost)
+{
+    if (!pool || view < 0 || view >= pool->n) {
+        return HPM_EINVAL;
+    }
+    if (!pool->have_state[(size_t)view]) {
+        return HPM_EINVAL;
+    }
+    const int64_t off = pool->off[(size_t)view];
+    const size_t np = pool->images[(size_t)view].size();
+    if ((hyp_world4 && hipMemcpyAsync(hyp_world4, pool->d_state_hyp + 4 * off, np * 16, hipMemcpyDeviceToHost,
+                                      pool->stream) != hipSuccess) ||
+        (cost && hipMemcpyAsync(cost, pool->d_state_cost + off, np * 4, hipMemcpyDeviceToHost, pool->stream) !=
+                     hipSuccess) ||
+        hipStreamSynchronize(pool->stream) != hipSuccess) {
+        return HPM_EDEVICE;
+    }
+    return HPM_OK;
+}
+
+extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref_ids, const int* num_src,
+                                const int* src_ids, int max_src, const hpm_pipeline_options* opt,
+                                hpm_pipeline_stats* stats, char* err, int err_len)
+{
+    hpm_pipeline_stats local;
+    hpm_pipeline_stats* st = stats ? stats : &local;
+    memset(st, 0, sizeof(*st));
+    hpm_pipeline_options defaults;
+    hpm_pipeline_default(&defaults);
+    Ctx c;
+    c.pool = pool;
+    c.opt = opt ? opt : &defaults;
+    c.st = st;
+    c.threads = c.opt->num_threads > 0 ? c.opt->num_threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    int rc = HPM_OK;
+    auto t_all = clk::now();
+    hpm_handle* h = nullptr;
+    if (!pool || num_problems < 1 || !ref_ids || !num_src || !src_ids || max_src < 1) {
+        rc = fail(c, "bad pipeline arguments");
+    } else if (!(h = hpm_create(pool->device))) {
+        rc = fail(c, "cannot create a problem handle");
+    }
+    if (rc == HPM_OK) {
+        auto t0 = clk::now();
+        for (int i = 0; i < num_problems && rc >= 0; ++i) {
+            if (c.opt->verbose) {
+                fprintf(stderr, "phase A: view %d\n", ref_ids[i]);
+            }
+            rc = process_phase_a(c, h, ref_ids[i], src_ids + (size_t)i * max_src, num_src[i]);
+        }
+        st->ms_phase_a = ms_since(t0);
+    }
+    if (rc >= 0 && c.opt->geom_consistency) {
+        auto t0 = clk::now();
+        for (int g = 0; g < c.opt->params.geom_iterations && rc >= 0; ++g) {
+            for (int i = 0; i < num_problems && rc >= 0; ++i) {
+                if (c.opt->verbose) {
+                    fprintf(stderr, "geometric pass %d: view %d\n", g, ref_ids[i]);
+                }
+                rc = process_phase_b(c, h, ref_ids[i], src_ids + (size_t)i * max_src, num_src[i], g > 0);
+            }
+        }
+        st->ms_phase_b = ms_since(t0);
+    }
+    if (h) {
+        hpm_destroy(h);
+    }
+    st->ms_wall = ms_since(t_all);
+    if (rc < 0 && err && err_len > 0) {
+        snprintf(err, (size_t)err_len, "%s", c.err.c_str());
+    }
+    return rc < 0 ? rc : HPM_OK;
+}
+
+extern "C" int hpm_pipeline_run_folder(const char* dense_folder, const hpm_pipeline_options* opt,
+                                       hpm_pipeline_stats* stats, char* err, int err_len)
+{
+    auto say = [&](const char* msg) {
+        if (err && err_len > 0) {
+            snprintf(err, (size_t)err_len, "%s", msg);
+        }
+    };
+    if (!dense_folder) {
+        say("null folder");
+        return HPM_EINVAL;
+    }
+    const std::string base(dense_folder);
+    int np = 0;
+    if (hpm_read_pair((base + "/pair.txt").c_str(), &np, nullptr, nullptr, nullptr, 0) != 0 || np < 1) {
+        say("cannot read pair.txt");
+        return HPM_EIO;
+    }
+    std::vector<int> refs((size_t)np), ns((size_t)np), srcs((size_t)np * HPM_MAX_SOURCES);
+    hpm_read_pair((base + "/pair.txt").c_str(), &np, refs.data(), ns.data(), srcs.data(), HPM_MAX_SOURCES);
+    int maxid = 0;
+    for (int i = 0; i < np; ++i) {
+        maxid = std::max(maxid, refs[(size_t)i]);
+        for (int j = 0; j < ns[(size_t)i]; ++j) {
+            maxid = std::max(maxid, srcs[(size_t)i * HPM_MAX_SOURCES + j]);
+        }
+    }
+    /* the pool holds views 0..maxid by image id */
+    std::vector<hpm_camera> cams((size_t)maxid + 1);
+    std::vector<std::vector<float>> imgs((size_t)maxid + 1);
+    for (int id = 0; id <= maxid; ++id) {
+        char path[4096];
+        int w = 0, hh = 0;
+        snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder, id);
+        if (hpm_read_pgm_f32(path, nullptr, &w, &hh) != 0) {
+            snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder, id);
+            if (hpm_read_pgm_f32(path, nullptr, &w, &hh) != 0) {
+                say("cannot read an image (8-bit PGM expected)");
+                return HPM_EIO;
+            }
+        }
+        imgs[(size_t)id].resize((size_t)w * hh);
+        hpm_read_pgm_f32(path, imgs[(size_t)id].data(), &w, &hh);
+        snprintf(path, sizeof(path), "%s/cams/%08d_cam.txt", dense_folder, id);
+        if (hpm_read_camera(path, &cams[(size_t)id]) != 0) {
+            say("cannot read a camera file");
+            return HPM_EIO;
+        }
+        cams[(size_t)id].width = w;
+        cams[(size_t)id].height = hh;
+    }
+    std::vector<const float*> ip;
+    for (auto& v : imgs) ip.push_back(v.data());
+    hpm_pipeline_options defaults;
+    hpm_pipeline_default(&defaults);
+    const hpm_pipeline_options* o = opt ? opt : &defaults;
+    hpm_pool* pool = hpm_pool_create(0, maxid + 1, cams.data(), ip.data(), o->params.max_image_size);
+    if (!pool) {
+        say("cannot create the device pool");
+        return HPM_EDEVICE;
+    }
+    int rc = hpm_pipeline_run(pool, np, refs.data(), ns.data(), srcs.data(), HPM_MAX_SOURCES, o, stats, err, err_len);
+    if (rc == HPM_OK) {
+        mkdir((base + "/HPM").c_str(), 0755);
+        for (int i = 0; i < np && rc == HPM_OK; ++i) {
+            const int v = refs[(size_t)i];
+            const size_t n = pool->images[(size_t)v].size();
+            const int W = pool->cams[(size_t)v].width, H = pool->cams[(size_t)v].height;
+            std::vector<float> hyp(n * 4), cost(n), nrm(n * 3), d_plain(n), d_geom(n);
+            hpm_pool_get_state(pool, v, hyp.data(), cost.data());
+            hpm_pool_get_depth(pool, v, 0, d_plain.data());
+            hpm_pool_get_depth(pool, v, 1, d_geom.data());
+            for (size_t k = 0; k < n; ++k) {
+                nrm[3 * k + 0] = hyp[4 * k + 0];
+                nrm[3 * k + 1] = hyp[4 * k + 1];
+                nrm[3 * k + 2] = hyp[4 * k + 2];
+            }
+            char dir[4096];
+            snprintf(dir, sizeof(dir), "%s/HPM/2333_%08d", dense_folder, v);
+            mkdir(dir, 0755);
+            const std::string d(dir);
+            if (hpm_write_dmb((d + "/depths.dmb").c_str(), d_plain.data(), W, H, 1) != 0 ||
+                hpm_write_dmb((d + "/normals.dmb").c_str(), nrm.data(), W, H, 3) != 0 ||
+                hpm_write_dmb((d + "/costs.dmb").c_str(), cost.data(), W, H, 1) != 0 ||
+                (o->geom_consistency && hpm_write_dmb((d + "/depths_geom.dmb").c_str(), d_geom.data(), W, H, 1) != 0)) {
+                say("cannot write results");
+                rc = HPM_EIO;
+            }
+        }
+    }
+    hpm_pool_destroy(pool);
+    return rc;
+}

@@ -1,10 +1,13 @@
 /*
  * hpm_runtime.cpp — host side of the C-ABI (include/hpm_capi.h): the
- * per-problem handle, its device buffers, the RunPatchMatch launch schedule
- * and the host mirrors. Mirrors `class HPM` (HPM-MVS/HPM.h:48-112,
- * HPM.cpp:99-133, 468-757, 901-921, 1215-1307; HPM.cu:1906-1957).
+ * device-resident image pool, the per-problem handle (the reference's
+ * `class HPM`, HPM-MVS/HPM.h:48-112, HPM.cpp:99-133, 468-757, 901-921,
+ * 1215-1307) and the RunPatchMatch launch schedule (HPM.cu:1906-1957).
  *
  * Differences from the reference by design:
+ *   - the images of a scene are uploaded once into a pool shared by the
+ *     problems of that scene; passes exchange depth maps and hypotheses on
+ *     the device (the reference re-reads images and .dmb files per pass);
  *   - one HIP stream per handle and one synchronisation per RunPatchMatch
  *     (the reference synchronises the device after every launch);
  *   - no RNG state buffer: Philox counters replace the 48 B/px cuRAND state;
@@ -25,44 +28,10 @@
 #include "hpm_capi.h"
 #include "hpm_device.h"
 #include "hpm_fmath.h"
+#include "hpm_internal.h"
 #include "hpm_io.h"
 
-struct hpm_handle {
-    std::string err;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hpm_params params;
-    int ref_id = 0;
-    int num_images = 0;
-    std::vector<hpm_camera> cams;
-    std::vector<std::vector<float>> images;
-    std::vector<std::vector<float>> depths; /* geometric consistency inputs */
-    bool have_input = false;
-    bool have_device = false;
-    uint32_t call_index = 0;
-
-    /* device */
-    float* d_images = nullptr;
-    float* d_depths = nullptr;
-    hpm_dev_view* d_views = nullptr;
-    float* d_hyp = nullptr;
-    float* d_cost = nullptr;
-    uint32_t* d_sel = nullptr;
-    float* d_prior = nullptr;
-    uint32_t* d_mask = nullptr;
-    unsigned long long* d_neval = nullptr;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    bool have_prior = false;
-
-    /* host mirrors */
-    std::vector<float> h_hyp;
-    std::vector<float> h_cost;
-    hpm_run_stats stats;
-};
-
-namespace {
-
-int fail(hpm_handle* h, int code, const char* fmt, ...)
+int hpm_fail(hpm_handle* h, int code, const char* fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -79,10 +48,12 @@ int fail(hpm_handle* h, int code, const char* fmt, ...)
     do {                                                                                            \
         hipError_t e_ = (call);                                                                     \
         if (e_ != hipSuccess) {                                                                     \
-            return fail((h), HPM_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
-                        __FILE__, __LINE__);                                                        \
+            return hpm_fail((h), HPM_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                                    \
         }                                                                                           \
     } while (0)
+
+namespace {
 
 template <typename T>
 void dfree(T*& p)
@@ -154,66 +125,46 @@ void iter_consts(hpm_dev_problem* pb)
 int check_params(hpm_handle* h, const hpm_params& p)
 {
     if (p.patch_size != 11 || p.radius_increment != 2) {
-        return fail(h, HPM_EINVAL, "the HIP path implements the 11x11 window at stride 2 (patch_size=%d, "
-                                   "radius_increment=%d)", p.patch_size, p.radius_increment);
+        return hpm_fail(h, HPM_EINVAL, "the HIP path implements the 11x11 window at stride 2 (patch_size=%d, "
+                                       "radius_increment=%d)", p.patch_size, p.radius_increment);
     }
     if (p.top_k < 1 || p.top_k > HPM_MAX_SOURCES) {
-        return fail(h, HPM_EINVAL, "top_k out of range: %d", p.top_k);
+        return hpm_fail(h, HPM_EINVAL, "top_k out of range: %d", p.top_k);
     }
     return HPM_OK;
 }
 
-/* input checks shared by both input-init forms (HPM.cpp:482-589) */
-int finish_input(hpm_handle* h)
+void release_problem(hpm_handle* h)
 {
-    const int n = (int)h->images.size();
-    if (n < 2 || n > HPM_MAX_IMAGES) {
-        return fail(h, HPM_EINVAL, "a problem needs 1 reference and 1..%d sources (got %d images)",
-                    HPM_MAX_SOURCES, n);
+    dfree(h->d_views);
+    dfree(h->d_hyp);
+    dfree(h->d_cost);
+    dfree(h->d_sel);
+    dfree(h->d_prior);
+    dfree(h->d_mask);
+    dfree(h->d_neval);
+    h->alloc_pixels = 0;
+    h->have_prior = false;
+    h->have_device = false;
+}
+
+void drop_pool(hpm_handle* h)
+{
+    if (h->own_pool && h->pool) {
+        hpm_pool_destroy(h->pool);
     }
-    /* Scale cameras and images (HPM.cpp:524-551) */
-    for (int i = 0; i < n; ++i) {
-        const int W = h->cams[i].width, H = h->cams[i].height;
-        if (W <= 0 || H <= 0) {
-            return fail(h, HPM_EINVAL, "image %d has no size", i);
-        }
-        if (W <= h->params.max_image_size && H <= h->params.max_image_size) {
-            continue;
-        }
-        const float fx = (float)h->params.max_image_size / W;
-        const float fy = (float)h->params.max_image_size / H;
-        const float f = std::min(fx, fy);
-        const int nw = (int)lroundf(W * f);
-        const int nh = (int)lroundf(H * f);
-        std::vector<float> out((size_t)nw * nh);
-        hpm_resize_bilinear(h->images[i].data(), W, H, out.data(), nw, nh);
-        h->images[i].swap(out);
-        const float sx = nw / (float)W, sy = nh / (float)H;
-        h->cams[i].K[0] *= sx;
-        h->cams[i].K[2] *= sx;
-        h->cams[i].K[4] *= sy;
-        h->cams[i].K[5] *= sy;
-        h->cams[i].width = nw;
-        h->cams[i].height = nh;
-    }
+    h->pool = nullptr;
+    h->own_pool = false;
+}
+
+/* HPM.cpp:553-559 */
+void set_depth_range(hpm_handle* h)
+{
     h->params.depth_min = h->cams[0].depth_min * 0.6f;
     h->params.depth_max = h->cams[0].depth_max * 1.2f;
-    h->params.num_images = n;
+    h->params.num_images = h->num_images;
     h->params.disparity_min = h->cams[0].K[0] * h->params.baseline / h->params.depth_max;
     h->params.disparity_max = h->cams[0].K[0] * h->params.baseline / h->params.depth_min;
-    h->num_images = n;
-    if (h->params.geom_consistency) {
-        if ((int)h->depths.size() != n) {
-            return fail(h, HPM_EINVAL, "geometric consistency needs a depth map for every image");
-        }
-        for (int i = 0; i < n; ++i) {
-            if (h->depths[i].size() != (size_t)h->cams[i].width * h->cams[i].height) {
-                return fail(h, HPM_EINVAL, "depth map %d does not match its image size", i);
-            }
-        }
-    }
-    h->have_input = true;
-    return HPM_OK;
 }
 
 void fill_problem(const hpm_handle* h, hpm_dev_problem* pb)
@@ -232,8 +183,8 @@ void fill_problem(const hpm_handle* h, hpm_dev_problem* pb)
     pb->call_index = h->call_index;
     pb->salt = (uint32_t)h->ref_id;
     iter_consts(pb);
-    pb->images = h->d_images;
-    pb->depths = h->d_depths;
+    pb->images = h->pool->d_images;
+    pb->depths = h->params.multi_geometry ? h->pool->d_depth_geom : h->pool->d_depth_plain;
     pb->views = h->d_views;
     pb->hyp = h->d_hyp;
     pb->cost = h->d_cost;
@@ -254,22 +205,172 @@ int readback(hpm_handle* h)
     return HPM_OK;
 }
 
-void release_device(hpm_handle* h)
+int ensure_ready(hpm_handle* h)
 {
-    dfree(h->d_images);
-    dfree(h->d_depths);
-    dfree(h->d_views);
-    dfree(h->d_hyp);
-    dfree(h->d_cost);
-    dfree(h->d_sel);
-    dfree(h->d_prior);
-    dfree(h->d_mask);
-    dfree(h->d_neval);
-    h->have_prior = false;
+    if (!h->have_device) {
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
+    }
+    if (h->params.planar_prior && !h->have_prior) {
+        return hpm_fail(h, HPM_EINVAL, "planar prior enabled but no prior uploaded");
+    }
+    return check_params(h, h->params);
+}
+
+int attach_pool(hpm_handle* h, hpm_pool* pool, bool own, int ref_view, const int* srcs, int nsrc)
+{
+    if (nsrc < 1 || nsrc > HPM_MAX_SOURCES) {
+        return hpm_fail(h, HPM_EINVAL, "a problem needs 1..%d source views (got %d)", HPM_MAX_SOURCES, nsrc);
+    }
+    if (ref_view < 0 || ref_view >= pool->n) {
+        return hpm_fail(h, HPM_EINVAL, "reference view %d outside the pool", ref_view);
+    }
+    for (int i = 0; i < nsrc; ++i) {
+        if (srcs[i] < 0 || srcs[i] >= pool->n || srcs[i] == ref_view) {
+            return hpm_fail(h, HPM_EINVAL, "source view %d invalid for reference %d", srcs[i], ref_view);
+        }
+    }
+    if (h->pool != pool) {
+        drop_pool(h);
+    }
+    h->pool = pool;
+    h->own_pool = own;
+    h->views.assign(1, ref_view);
+    h->cams.assign(1, pool->cams[(size_t)ref_view]);
+    for (int i = 0; i < nsrc; ++i) {
+        h->views.push_back(srcs[i]);
+        h->cams.push_back(pool->cams[(size_t)srcs[i]]);
+    }
+    h->num_images = nsrc + 1;
+    set_depth_range(h);
+    h->have_input = true;
     h->have_device = false;
+    return HPM_OK;
 }
 
 } // namespace
+
+/* ------------------------------------------------------------------ pool */
+
+extern "C" hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera* cams, const float* const* images,
+                                     int max_image_size)
+{
+    if (num_views < 1 || !cams || !images) {
+        return nullptr;
+    }
+    hpm_pool* p = new hpm_pool();
+    p->device = device < 0 ? 0 : device;
+    p->n = num_views;
+    for (int i = 0; i < num_views; ++i) {
+        hpm_camera cam = cams[i];
+        const int W = cam.width, H = cam.height;
+        if (W <= 0 || H <= 0 || !images[i]) {
+            delete p;
+            return nullptr;
+        }
+        std::vector<float> img(images[i], images[i] + (size_t)W * H);
+        /* Scale cameras and images (HPM.cpp:524-551) */
+        if (max_image_size > 0 && (W > max_image_size || H > max_image_size)) {
+            const float fx = (float)max_image_size / W;
+            const float fy = (float)max_image_size / H;
+            const float f = std::min(fx, fy);
+            const int nw = (int)lroundf(W * f);
+            const int nh = (int)lroundf(H * f);
+            std::vector<float> out((size_t)nw * nh);
+            hpm_resize_bilinear(img.data(), W, H, out.data(), nw, nh);
+            img.swap(out);
+            const float sx = nw / (float)W, sy = nh / (float)H;
+            cam.K[0] *= sx;
+            cam.K[2] *= sx;
+            cam.K[4] *= sy;
+            cam.K[5] *= sy;
+            cam.width = nw;
+            cam.height = nh;
+        }
+        p->cams.push_back(cam);
+        p->images.push_back(std::move(img));
+        p->off.push_back(p->total);
+        p->total += (int64_t)p->cams.back().width * p->cams.back().height;
+    }
+    p->have_state.assign((size_t)num_views, 0);
+    bool ok = hipSetDevice(p->device) == hipSuccess &&
+              hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc((void**)&p->d_images, (size_t)p->total * 4) == hipSuccess &&
+              hipMalloc((void**)&p->d_depth_plain, (size_t)p->total * 4) == hipSuccess &&
+              hipMalloc((void**)&p->d_depth_geom, (size_t)p->total * 4) == hipSuccess &&
+              hipMalloc((void**)&p->d_state_hyp, (size_t)p->total * 16) == hipSuccess &&
+              hipMalloc((void**)&p->d_state_cost, (size_t)p->total * 4) == hipSuccess;
+    for (int i = 0; ok && i < num_views; ++i) {
+        ok = hipMemcpyAsync(p->d_images + p->off[i], p->images[i].data(), p->images[i].size() * 4,
+                            hipMemcpyHostToDevice, p->stream) == hipSuccess;
+    }
+    ok = ok && hipMemsetAsync(p->d_depth_plain, 0, (size_t)p->total * 4, p->stream) == hipSuccess &&
+         hipMemsetAsync(p->d_depth_geom, 0, (size_t)p->total * 4, p->stream) == hipSuccess &&
+         hipStreamSynchronize(p->stream) == hipSuccess;
+    if (!ok) {
+        hpm_pool_destroy(p);
+        return nullptr;
+    }
+    return p;
+}
+
+extern "C" void hpm_pool_destroy(hpm_pool* p)
+{
+    if (!p) {
+        return;
+    }
+    (void)hipSetDevice(p->device);
+    if (p->stream) {
+        (void)hipStreamSynchronize(p->stream);
+    }
+    dfree(p->d_images);
+    dfree(p->d_depth_plain);
+    dfree(p->d_depth_geom);
+    dfree(p->d_state_hyp);
+    dfree(p->d_state_cost);
+    if (p->stream) {
+        (void)hipStreamDestroy(p->stream);
+    }
+    delete p;
+}
+
+extern "C" int hpm_pool_camera(const hpm_pool* p, int view, hpm_camera* out)
+{
+    if (!p || !out || view < 0 || view >= p->n) {
+        return HPM_EINVAL;
+    }
+    *out = p->cams[(size_t)view];
+    return HPM_OK;
+}
+
+extern "C" int hpm_pool_set_depth(hpm_pool* p, int view, int which, const float* depth)
+{
+    if (!p || !depth || view < 0 || view >= p->n || which < 0 || which > 1) {
+        return HPM_EINVAL;
+    }
+    float* dst = (which == 0 ? p->d_depth_plain : p->d_depth_geom) + p->off[(size_t)view];
+    const size_t np = p->images[(size_t)view].size();
+    if (hipMemcpyAsync(dst, depth, np * 4, hipMemcpyHostToDevice, p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess) {
+        return HPM_EDEVICE;
+    }
+    return HPM_OK;
+}
+
+extern "C" int hpm_pool_get_depth(hpm_pool* p, int view, int which, float* depth)
+{
+    if (!p || !depth || view < 0 || view >= p->n || which < 0 || which > 1) {
+        return HPM_EINVAL;
+    }
+    const float* src = (which == 0 ? p->d_depth_plain : p->d_depth_geom) + p->off[(size_t)view];
+    const size_t np = p->images[(size_t)view].size();
+    if (hipMemcpyAsync(depth, src, np * 4, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess) {
+        return HPM_EDEVICE;
+    }
+    return HPM_OK;
+}
+
+/* --------------------------------------------------------------- handles */
 
 extern "C" hpm_handle* hpm_create(int device)
 {
@@ -277,11 +378,8 @@ extern "C" hpm_handle* hpm_create(int device)
     hpm_params_default(&h->params);
     memset(&h->stats, 0, sizeof(h->stats));
     h->device = device < 0 ? 0 : device;
-    if (hipSetDevice(h->device) != hipSuccess) {
-        delete h;
-        return nullptr;
-    }
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(h->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return nullptr;
     }
@@ -302,7 +400,8 @@ extern "C" void hpm_destroy(hpm_handle* h)
     if (h->stream) {
         (void)hipStreamSynchronize(h->stream);
     }
-    release_device(h);
+    release_problem(h);
+    drop_pool(h);
     for (auto& e : h->ev) {
         if (e) {
             (void)hipEventDestroy(e);
@@ -322,13 +421,16 @@ extern "C" const char* hpm_last_error(const hpm_handle* h)
 extern "C" int hpm_set_params(hpm_handle* h, const hpm_params* p)
 {
     if (!h || !p) {
-        return fail(h, HPM_EINVAL, "null argument");
+        return hpm_fail(h, HPM_EINVAL, "null argument");
     }
     const int rc = check_params(h, *p);
     if (rc != HPM_OK) {
         return rc;
     }
     h->params = *p;
+    if (h->have_input) {
+        set_depth_range(h);
+    }
     return HPM_OK;
 }
 
@@ -363,97 +465,128 @@ extern "C" int hpm_set_planar_prior(hpm_handle* h)
     return HPM_OK;
 }
 
-extern "C" int hpm_input_init(hpm_handle* h, const char* dense_folder, int ref_image_id, const int* src_image_ids,
-                              int num_src)
+extern "C" int hpm_input_init_pool(hpm_handle* h, hpm_pool* pool, int ref_view, const int* src_views, int num_src)
 {
-    if (!h || !dense_folder || (num_src > 0 && !src_image_ids)) {
-        return fail(h, HPM_EINVAL, "null argument");
+    if (!h || !pool || (num_src > 0 && !src_views)) {
+        return hpm_fail(h, HPM_EINVAL, "null argument");
     }
-    if (num_src < 1 || num_src > HPM_MAX_SOURCES) {
-        return fail(h, HPM_EINVAL, "num_src must be 1..%d (got %d)", HPM_MAX_SOURCES, num_src);
-    }
-    h->images.clear();
-    h->cams.clear();
-    h->depths.clear();
-    h->ref_id = ref_image_id;
-    std::vector<int> ids;
-    ids.push_back(ref_image_id);
-    for (int i = 0; i < num_src; ++i) {
-        ids.push_back(src_image_ids[i]);
-    }
-    for (int id : ids) {
-        char path[4096];
-        int w = 0, hh = 0;
-        std::vector<float> img;
-        snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder, id);
-        if (hpm_read_pgm_f32(path, nullptr, &w, &hh) != 0) {
-            snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder, id);
-            if (hpm_read_pgm_f32(path, nullptr, &w, &hh) != 0) {
-                return fail(h, HPM_EIO, "cannot read image %d under %s/images (8-bit PGM expected)", id,
-                            dense_folder);
-            }
-        }
-        img.resize((size_t)w * hh);
-        if (hpm_read_pgm_f32(path, img.data(), &w, &hh) != 0) {
-            return fail(h, HPM_EIO, "cannot decode %s", path);
-        }
-        hpm_camera cam;
-        snprintf(path, sizeof(path), "%s/cams/%08d_cam.txt", dense_folder, id);
-        if (hpm_read_camera(path, &cam) != 0) {
-            return fail(h, HPM_EIO, "cannot read camera %s", path);
-        }
-        cam.width = w;
-        cam.height = hh;
-        h->cams.push_back(cam);
-        h->images.push_back(std::move(img));
-    }
-    if (h->params.geom_consistency) {
-        const char* suffix = h->params.multi_geometry ? "depths_geom.dmb" : "depths.dmb";
-        for (int id : ids) {
-            char path[4096];
-            snprintf(path, sizeof(path), "%s/HPM/2333_%08d/%s", dense_folder, id, suffix);
-            int w = 0, hh = 0, nb = 0;
-            if (hpm_read_dmb(path, nullptr, &w, &hh, &nb) != 0 || nb != 1) {
-                return fail(h, HPM_EIO, "cannot read depth map %s", path);
-            }
-            std::vector<float> d((size_t)w * hh);
-            if (hpm_read_dmb(path, d.data(), &w, &hh, &nb) != 0) {
-                return fail(h, HPM_EIO, "cannot read depth map %s", path);
-            }
-            h->depths.push_back(std::move(d));
-        }
-    }
-    return finish_input(h);
+    h->ref_id = ref_view;
+    return attach_pool(h, pool, false, ref_view, src_views, num_src);
 }
 
 extern "C" int hpm_input_init_memory(hpm_handle* h, int ref_image_id, int num_images, const hpm_camera* cams,
                                      const float* const* images, const float* const* depths)
 {
     if (!h || !cams || !images) {
-        return fail(h, HPM_EINVAL, "null argument");
+        return hpm_fail(h, HPM_EINVAL, "null argument");
     }
     if (num_images < 2 || num_images > HPM_MAX_IMAGES) {
-        return fail(h, HPM_EINVAL, "num_images must be 2..%d (got %d)", HPM_MAX_IMAGES, num_images);
+        return hpm_fail(h, HPM_EINVAL, "num_images must be 2..%d (got %d)", HPM_MAX_IMAGES, num_images);
     }
-    h->images.clear();
-    h->cams.clear();
-    h->depths.clear();
-    h->ref_id = ref_image_id;
     for (int i = 0; i < num_images; ++i) {
         if (!images[i] || cams[i].width <= 0 || cams[i].height <= 0) {
-            return fail(h, HPM_EINVAL, "image %d missing or empty", i);
-        }
-        const size_t np = (size_t)cams[i].width * cams[i].height;
-        h->cams.push_back(cams[i]);
-        h->images.emplace_back(images[i], images[i] + np);
-        if (depths) {
-            if (!depths[i]) {
-                return fail(h, HPM_EINVAL, "depth map %d missing", i);
-            }
-            h->depths.emplace_back(depths[i], depths[i] + np);
+            return hpm_fail(h, HPM_EINVAL, "image %d missing or empty", i);
         }
     }
-    return finish_input(h);
+    hpm_pool* pool = hpm_pool_create(h->device, num_images, cams, images, h->params.max_image_size);
+    if (!pool) {
+        return hpm_fail(h, HPM_EDEVICE, "cannot create the device image pool");
+    }
+    if (depths) {
+        for (int i = 0; i < num_images; ++i) {
+            if (!depths[i]) {
+                hpm_pool_destroy(pool);
+                return hpm_fail(h, HPM_EINVAL, "depth map %d missing", i);
+            }
+            if (pool->cams[i].width != cams[i].width || pool->cams[i].height != cams[i].height) {
+                hpm_pool_destroy(pool);
+                return hpm_fail(h, HPM_EINVAL, "depth maps need images within max_image_size");
+            }
+            hpm_pool_set_depth(pool, i, 0, depths[i]);
+            hpm_pool_set_depth(pool, i, 1, depths[i]);
+        }
+    }
+    std::vector<int> srcs;
+    for (int i = 1; i < num_images; ++i) {
+        srcs.push_back(i);
+    }
+    h->ref_id = ref_image_id;
+    const int rc = attach_pool(h, pool, true, 0, srcs.data(), (int)srcs.size());
+    if (rc != HPM_OK) {
+        hpm_pool_destroy(pool);
+        h->pool = nullptr;
+        h->own_pool = false;
+        return rc;
+    }
+    h->ref_id = ref_image_id;
+    return HPM_OK;
+}
+
+extern "C" int hpm_input_init(hpm_handle* h, const char* dense_folder, int ref_image_id, const int* src_image_ids,
+                              int num_src)
+{
+    if (!h || !dense_folder || (num_src > 0 && !src_image_ids)) {
+        return hpm_fail(h, HPM_EINVAL, "null argument");
+    }
+    if (num_src < 1 || num_src > HPM_MAX_SOURCES) {
+        return hpm_fail(h, HPM_EINVAL, "num_src must be 1..%d (got %d)", HPM_MAX_SOURCES, num_src);
+    }
+    std::vector<int> ids;
+    ids.push_back(ref_image_id);
+    for (int i = 0; i < num_src; ++i) {
+        ids.push_back(src_image_ids[i]);
+    }
+    std::vector<std::vector<float>> imgs;
+    std::vector<hpm_camera> cams;
+    for (int id : ids) {
+        char path[4096];
+        int w = 0, hh = 0;
+        snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder, id);
+        if (hpm_read_pgm_f32(path, nullptr, &w, &hh) != 0) {
+            snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder, id);
+            if (hpm_read_pgm_f32(path, nullptr, &w, &hh) != 0) {
+                return hpm_fail(h, HPM_EIO, "cannot read image %d under %s/images (8-bit PGM expected)", id,
+                                dense_folder);
+            }
+        }
+        std::vector<float> img((size_t)w * hh);
+        if (hpm_read_pgm_f32(path, img.data(), &w, &hh) != 0) {
+            return hpm_fail(h, HPM_EIO, "cannot decode %s", path);
+        }
+        hpm_camera cam;
+        snprintf(path, sizeof(path), "%s/cams/%08d_cam.txt", dense_folder, id);
+        if (hpm_read_camera(path, &cam) != 0) {
+            return hpm_fail(h, HPM_EIO, "cannot read camera %s", path);
+        }
+        cam.width = w;
+        cam.height = hh;
+        cams.push_back(cam);
+        imgs.push_back(std::move(img));
+    }
+    std::vector<std::vector<float>> deps;
+    if (h->params.geom_consistency) {
+        /* HPM.cpp:561-588 */
+        const char* suffix = h->params.multi_geometry ? "depths_geom.dmb" : "depths.dmb";
+        for (size_t i = 0; i < ids.size(); ++i) {
+            char path[4096];
+            snprintf(path, sizeof(path), "%s/HPM/2333_%08d/%s", dense_folder, ids[i], suffix);
+            int w = 0, hh = 0, nb = 0;
+            if (hpm_read_dmb(path, nullptr, &w, &hh, &nb) != 0 || nb != 1 || w != cams[i].width ||
+                hh != cams[i].height) {
+                return hpm_fail(h, HPM_EIO, "cannot read depth map %s", path);
+            }
+            std::vector<float> d((size_t)w * hh);
+            if (hpm_read_dmb(path, d.data(), &w, &hh, &nb) != 0) {
+                return hpm_fail(h, HPM_EIO, "cannot read depth map %s", path);
+            }
+            deps.push_back(std::move(d));
+        }
+    }
+    std::vector<const float*> ip, dp;
+    for (auto& v : imgs) ip.push_back(v.data());
+    for (auto& v : deps) dp.push_back(v.data());
+    return hpm_input_init_memory(h, ref_image_id, (int)ids.size(), cams.data(), ip.data(),
+                                 deps.empty() ? nullptr : dp.data());
 }
 
 extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
@@ -462,53 +595,40 @@ extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
         return HPM_EINVAL;
     }
     if (!h->have_input) {
-        return fail(h, HPM_EINVAL, "input initialisation must come first");
+        return hpm_fail(h, HPM_EINVAL, "input initialisation must come first");
     }
     HPM_HIP(h, hipSetDevice(h->device));
-    release_device(h);
     const int n = h->num_images;
     const int W = h->cams[0].width, H = h->cams[0].height;
     const size_t np = (size_t)W * H;
     std::vector<hpm_dev_view> views((size_t)n);
-    int64_t off = 0, doff = 0;
     for (int i = 0; i < n; ++i) {
         memset(&views[i], 0, sizeof(hpm_dev_view));
         views[i].cam = h->cams[i];
         if (i > 0) {
             make_view_rel(h->cams[0], h->cams[i], &views[i]);
         }
-        views[i].img_off = off;
-        views[i].depth_off = doff;
-        off += (int64_t)h->cams[i].width * h->cams[i].height;
-        if (h->params.geom_consistency) {
-            doff += (int64_t)h->cams[i].width * h->cams[i].height;
-        }
+        views[i].img_off = h->pool->off[(size_t)h->views[i]];
+        views[i].depth_off = views[i].img_off;
     }
-    HPM_HIP(h, hipMalloc((void**)&h->d_images, (size_t)off * sizeof(float)));
-    for (int i = 0; i < n; ++i) {
-        HPM_HIP(h, hipMemcpyAsync(h->d_images + views[i].img_off, h->images[i].data(),
-                                  h->images[i].size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    if (h->alloc_pixels != np) {
+        release_problem(h);
+        HPM_HIP(h, hipMalloc((void**)&h->d_hyp, np * 16));
+        HPM_HIP(h, hipMalloc((void**)&h->d_cost, np * 4));
+        HPM_HIP(h, hipMalloc((void**)&h->d_sel, np * 4));
+        HPM_HIP(h, hipMalloc((void**)&h->d_neval, 2 * sizeof(unsigned long long)));
+        HPM_HIP(h, hipMalloc((void**)&h->d_views, HPM_MAX_IMAGES * sizeof(hpm_dev_view)));
+        h->alloc_pixels = np;
     }
-    if (h->params.geom_consistency) {
-        HPM_HIP(h, hipMalloc((void**)&h->d_depths, (size_t)doff * sizeof(float)));
-        for (int i = 0; i < n; ++i) {
-            HPM_HIP(h, hipMemcpyAsync(h->d_depths + views[i].depth_off, h->depths[i].data(),
-                                      h->depths[i].size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
-        }
-    }
-    HPM_HIP(h, hipMalloc((void**)&h->d_views, views.size() * sizeof(hpm_dev_view)));
     HPM_HIP(h, hipMemcpyAsync(h->d_views, views.data(), views.size() * sizeof(hpm_dev_view), hipMemcpyHostToDevice,
                               h->stream));
-    HPM_HIP(h, hipMalloc((void**)&h->d_hyp, np * 16));
-    HPM_HIP(h, hipMalloc((void**)&h->d_cost, np * 4));
-    HPM_HIP(h, hipMalloc((void**)&h->d_sel, np * 4));
-    HPM_HIP(h, hipMalloc((void**)&h->d_neval, 2 * sizeof(unsigned long long)));
     HPM_HIP(h, hipMemsetAsync(h->d_hyp, 0, np * 16, h->stream));
     HPM_HIP(h, hipMemsetAsync(h->d_cost, 0, np * 4, h->stream));
     HPM_HIP(h, hipMemsetAsync(h->d_sel, 0, np * 4, h->stream));
     h->h_hyp.assign(np * 4, 0.0f);
     h->h_cost.assign(np, 0.0f);
     h->have_device = true;
+    h->have_prior = false;
     h->call_index = 0;
     if (h->params.geom_consistency && dense_folder) {
         /* reload the previous hypotheses (HPM.cpp:662-699) */
@@ -520,15 +640,15 @@ extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
         std::vector<float> depth(np), normal(np * 3), cost(np);
         snprintf(path, sizeof(path), "%s/%s", base, suffix);
         if (hpm_read_dmb(path, depth.data(), &w, &hh, &nb) != 0 || w != W || hh != H || nb != 1) {
-            return fail(h, HPM_EIO, "cannot reload %s", path);
+            return hpm_fail(h, HPM_EIO, "cannot reload %s", path);
         }
         snprintf(path, sizeof(path), "%s/normals.dmb", base);
         if (hpm_read_dmb(path, normal.data(), &w, &hh, &nb) != 0 || w != W || hh != H || nb != 3) {
-            return fail(h, HPM_EIO, "cannot reload %s", path);
+            return hpm_fail(h, HPM_EIO, "cannot reload %s", path);
         }
         snprintf(path, sizeof(path), "%s/costs.dmb", base);
         if (hpm_read_dmb(path, cost.data(), &w, &hh, &nb) != 0 || w != W || hh != H || nb != 1) {
-            return fail(h, HPM_EIO, "cannot reload %s", path);
+            return hpm_fail(h, HPM_EIO, "cannot reload %s", path);
         }
         std::vector<float> hyp(np * 4);
         for (size_t i = 0; i < np; ++i) {
@@ -546,10 +666,10 @@ extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
 extern "C" int hpm_device_init_state(hpm_handle* h, const float* hyp_world, const float* cost)
 {
     if (!h || !hyp_world || !cost) {
-        return fail(h, HPM_EINVAL, "null argument");
+        return hpm_fail(h, HPM_EINVAL, "null argument");
     }
     if (!h->have_device) {
-        return fail(h, HPM_EINVAL, "device initialisation must come first");
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
     }
     const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
     memcpy(h->h_hyp.data(), hyp_world, np * 16);
@@ -560,25 +680,8 @@ extern "C" int hpm_device_init_state(hpm_handle* h, const float* hyp_world, cons
     return HPM_OK;
 }
 
-static int ensure_ready(hpm_handle* h)
+int hpm_run_internal(hpm_handle* h, bool want_readback)
 {
-    if (!h->have_device) {
-        return fail(h, HPM_EINVAL, "device initialisation must come first");
-    }
-    if (h->params.planar_prior && !h->have_prior) {
-        return fail(h, HPM_EINVAL, "planar prior enabled but no prior uploaded");
-    }
-    if (h->params.geom_consistency && !h->d_depths) {
-        return fail(h, HPM_EINVAL, "geometric consistency enabled after device initialisation");
-    }
-    return check_params(h, h->params);
-}
-
-extern "C" int hpm_run_patchmatch(hpm_handle* h)
-{
-    if (!h) {
-        return HPM_EINVAL;
-    }
     int rc = ensure_ready(h);
     if (rc != HPM_OK) {
         return rc;
@@ -602,9 +705,13 @@ extern "C" int hpm_run_patchmatch(hpm_handle* h)
     HPM_HIP(h, hipEventRecord(h->ev[3], h->stream));
     unsigned long long nev[2] = {0ull, 0ull};
     HPM_HIP(h, hipMemcpyAsync(nev, h->d_neval, sizeof(nev), hipMemcpyDeviceToHost, h->stream));
-    rc = readback(h);
-    if (rc != HPM_OK) {
-        return rc;
+    if (want_readback) {
+        rc = readback(h);
+        if (rc != HPM_OK) {
+            return rc;
+        }
+    } else {
+        HPM_HIP(h, hipStreamSynchronize(h->stream));
     }
     h->stats.n_eval = nev[0] + nev[1];
     h->stats.n_eval_propagate = nev[1];
@@ -614,6 +721,91 @@ extern "C" int hpm_run_patchmatch(hpm_handle* h)
     HPM_HIP(h, hipEventElapsedTime(&h->stats.ms_propagate, h->ev[1], h->ev[2]));
     HPM_HIP(h, hipEventElapsedTime(&h->stats.ms_finalize_filter, h->ev[2], h->ev[3]));
     h->call_index += 1;
+    return HPM_OK;
+}
+
+extern "C" int hpm_run_patchmatch(hpm_handle* h)
+{
+    if (!h) {
+        return HPM_EINVAL;
+    }
+    return hpm_run_internal(h, true);
+}
+
+int hpm_store_to_pool(hpm_handle* h, int which_depth)
+{
+    if (!h->have_device) {
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
+    }
+    hpm_pool* p = h->pool;
+    const int v = h->views[0];
+    const int64_t off = p->off[(size_t)v];
+    const int64_t np = (int64_t)h->cams[0].width * h->cams[0].height;
+    float* depth = (which_depth == 0 ? p->d_depth_plain : p->d_depth_geom) + off;
+    HPM_HIP(h, (hipError_t)hpm_launch_store_state(h->d_hyp, h->d_cost, np, p->d_state_hyp + 4 * off,
+                                                  p->d_state_cost + off, depth, h->stream));
+    HPM_HIP(h, hipStreamSynchronize(h->stream));
+    p->have_state[(size_t)v] = 1;
+    return HPM_OK;
+}
+
+int hpm_load_from_pool(hpm_handle* h)
+{
+    hpm_pool* p = h->pool;
+    const int v = h->views[0];
+    if (!h->have_device || !p->have_state[(size_t)v]) {
+        return hpm_fail(h, HPM_EINVAL, "no previous state of view %d in the pool", v);
+    }
+    const int64_t off = p->off[(size_t)v];
+    const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
+    HPM_HIP(h, hipMemcpyAsync(h->d_hyp, p->d_state_hyp + 4 * off, np * 16, hipMemcpyDeviceToDevice, h->stream));
+    HPM_HIP(h, hipMemcpyAsync(h->d_cost, p->d_state_cost + off, np * 4, hipMemcpyDeviceToDevice, h->stream));
+    HPM_HIP(h, hipStreamSynchronize(h->stream));
+    return HPM_OK;
+}
+
+static int ensure_prior_buffers(hpm_handle* h)
+{
+    const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
+    if (!h->d_prior) {
+        HPM_HIP(h, hipMalloc((void**)&h->d_prior, np * 16));
+        HPM_HIP(h, hipMalloc((void**)&h->d_mask, np * 4));
+    }
+    return HPM_OK;
+}
+
+int hpm_jbu_prior(hpm_handle* h, const float* src_nd, int sw, int sh, float factor)
+{
+    (void)factor; /* GetDistance2Origin_factor uses the full-resolution K (HPM.cpp:892-899) */
+    if (!h->have_device) {
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
+    }
+    const int w = h->cams[0].width, hgt = h->cams[0].height;
+    const int scale_i = std::max(hgt / sh, w / sw);
+    if (scale_i == 1) {
+        return 1;
+    }
+    int rc = ensure_prior_buffers(h);
+    if (rc != HPM_OK) {
+        return rc;
+    }
+    const size_t sn = (size_t)sw * sh;
+    float* d_src = nullptr;
+    HPM_HIP(h, hipMalloc((void**)&d_src, sn * 16));
+    hipError_t e = hipMemcpyAsync(d_src, src_nd, sn * 16, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        e = (hipError_t)hpm_launch_jbu_prior(h->pool->d_images + h->pool->off[(size_t)h->views[0]], w, hgt, d_src, sw,
+                                             sh, scale_i, &h->cams[0], h->params.depth_min, h->params.depth_max,
+                                             h->d_prior, h->d_mask, h->stream);
+    }
+    if (e == hipSuccess) {
+        e = hipStreamSynchronize(h->stream);
+    }
+    (void)hipFree(d_src);
+    if (e != hipSuccess) {
+        return hpm_fail(h, HPM_EDEVICE, "JBU prior failed: %s", hipGetErrorString(e));
+    }
+    h->have_prior = true;
     return HPM_OK;
 }
 
@@ -645,7 +837,8 @@ extern "C" int hpm_copy_reference_image(const hpm_handle* h, float* out)
     if (!h || !h->have_input || !out) {
         return HPM_EINVAL;
     }
-    memcpy(out, h->images[0].data(), h->images[0].size() * sizeof(float));
+    const std::vector<float>& img = h->pool->images[(size_t)h->views[0]];
+    memcpy(out, img.data(), img.size() * sizeof(float));
     return HPM_OK;
 }
 
@@ -701,16 +894,16 @@ extern "C" int hpm_depth_range(const hpm_handle* h, float* depth_min, float* dep
 extern "C" int hpm_upload_prior(hpm_handle* h, const float* planes4, const uint32_t* mask)
 {
     if (!h || !planes4 || !mask) {
-        return fail(h, HPM_EINVAL, "null argument");
+        return hpm_fail(h, HPM_EINVAL, "null argument");
     }
     if (!h->have_device) {
-        return fail(h, HPM_EINVAL, "device initialisation must come first");
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
     }
     HPM_HIP(h, hipSetDevice(h->device));
     const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
-    if (!h->d_prior) {
-        HPM_HIP(h, hipMalloc((void**)&h->d_prior, np * 16));
-        HPM_HIP(h, hipMalloc((void**)&h->d_mask, np * 4));
+    int rc = ensure_prior_buffers(h);
+    if (rc != HPM_OK) {
+        return rc;
     }
     /* planes are only defined where mask > 0 (HPM.cpp:910-918) */
     std::vector<float> planes(np * 4, 0.0f);
@@ -750,9 +943,8 @@ extern "C" int hpm_release(hpm_handle* h)
     if (h->stream) {
         (void)hipStreamSynchronize(h->stream);
     }
-    release_device(h);
-    h->images.clear();
-    h->depths.clear();
+    release_problem(h);
+    drop_pool(h);
     h->have_input = false;
     return HPM_OK;
 }
@@ -761,10 +953,10 @@ extern "C" int hpm_jbu(hpm_handle* h, const float* guide, int w, int hgt, const 
                        float* out_depth, float* out_normal)
 {
     if (!h || !guide || !src_nd || !out_depth || !out_normal) {
-        return fail(h, HPM_EINVAL, "null argument");
+        return hpm_fail(h, HPM_EINVAL, "null argument");
     }
     if (w <= 0 || hgt <= 0 || sw <= 0 || sh <= 0) {
-        return fail(h, HPM_EINVAL, "bad sizes");
+        return hpm_fail(h, HPM_EINVAL, "bad sizes");
     }
     /* Imagescale (HPM.cpp:1218-1222) */
     const int scale_i = std::max(hgt / sh, w / sw);
@@ -791,7 +983,7 @@ extern "C" int hpm_jbu(hpm_handle* h, const float* guide, int w, int hgt, const 
     dfree(d_depth);
     dfree(d_normal);
     if (e != hipSuccess) {
-        return fail(h, HPM_EDEVICE, "joint bilateral upsampling failed: %s", hipGetErrorString(e));
+        return hpm_fail(h, HPM_EDEVICE, "joint bilateral upsampling failed: %s", hipGetErrorString(e));
     }
     return HPM_OK;
 }
@@ -799,15 +991,15 @@ extern "C" int hpm_jbu(hpm_handle* h, const float* guide, int w, int hgt, const 
 extern "C" int hpm_cost_vectors(hpm_handle* h, const float* planes4, const int32_t* pix_xy, int npix, float* out)
 {
     if (!h || !planes4 || !pix_xy || !out || npix < 0) {
-        return fail(h, HPM_EINVAL, "null argument");
+        return hpm_fail(h, HPM_EINVAL, "null argument");
     }
     if (!h->have_device) {
-        return fail(h, HPM_EINVAL, "device initialisation must come first");
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
     }
     const int W = h->cams[0].width, H = h->cams[0].height;
     for (int i = 0; i < npix; ++i) {
         if (pix_xy[2 * i] < 0 || pix_xy[2 * i] >= W || pix_xy[2 * i + 1] < 0 || pix_xy[2 * i + 1] >= H) {
-            return fail(h, HPM_EINVAL, "pixel %d outside the reference image", i);
+            return hpm_fail(h, HPM_EINVAL, "pixel %d outside the reference image", i);
         }
     }
     if (npix == 0) {
@@ -833,7 +1025,7 @@ extern "C" int hpm_cost_vectors(hpm_handle* h, const float* planes4, const int32
     dfree(d_pix);
     dfree(d_out);
     if (e != hipSuccess) {
-        return fail(h, HPM_EDEVICE, "cost vectors failed: %s", hipGetErrorString(e));
+        return hpm_fail(h, HPM_EDEVICE, "cost vectors failed: %s", hipGetErrorString(e));
     }
     return HPM_OK;
 }
@@ -848,7 +1040,7 @@ extern "C" int hpm_debug_stage(hpm_handle* h, int stage, int iter, int colour)
         return rc;
     }
     if (iter < 0 || iter > 2 || colour < 0 || colour > 1) {
-        return fail(h, HPM_EINVAL, "iter must be 0..2 and colour 0..1");
+        return hpm_fail(h, HPM_EINVAL, "iter must be 0..2 and colour 0..1");
     }
     HPM_HIP(h, hipSetDevice(h->device));
     hpm_dev_problem pb;
@@ -860,7 +1052,7 @@ extern "C" int hpm_debug_stage(hpm_handle* h, int stage, int iter, int colour)
     case 2: HPM_HIP(h, (hipError_t)hpm_launch_finalize(&pb, h->stream)); break;
     case 3: HPM_HIP(h, (hipError_t)hpm_launch_filter(&pb, colour, h->stream)); break;
     case 4: return readback(h);
-    default: return fail(h, HPM_EINVAL, "unknown stage %d", stage);
+    default: return hpm_fail(h, HPM_EINVAL, "unknown stage %d", stage);
     }
     HPM_HIP(h, hipStreamSynchronize(h->stream));
     return HPM_OK;
@@ -869,7 +1061,7 @@ extern "C" int hpm_debug_stage(hpm_handle* h, int stage, int iter, int colour)
 extern "C" int hpm_debug_read_state(hpm_handle* h, float* hyp4, float* cost, uint32_t* selmask)
 {
     if (!h || !h->have_device) {
-        return fail(h, HPM_EINVAL, "device initialisation must come first");
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
     }
     const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
     if (hyp4) HPM_HIP(h, hipMemcpyAsync(hyp4, h->d_hyp, np * 16, hipMemcpyDeviceToHost, h->stream));
@@ -882,7 +1074,7 @@ extern "C" int hpm_debug_read_state(hpm_handle* h, float* hyp4, float* cost, uin
 extern "C" int hpm_debug_write_state(hpm_handle* h, const float* hyp4, const float* cost, const uint32_t* selmask)
 {
     if (!h || !h->have_device) {
-        return fail(h, HPM_EINVAL, "device initialisation must come first");
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
     }
     const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
     if (hyp4) HPM_HIP(h, hipMemcpyAsync(h->d_hyp, hyp4, np * 16, hipMemcpyHostToDevice, h->stream));
@@ -895,7 +1087,7 @@ extern "C" int hpm_debug_write_state(hpm_handle* h, const float* hyp4, const flo
 extern "C" int hpm_device_selftest(hpm_handle* h, int kind, const void* in, void* out, int n)
 {
     if (!h || !in || !out || n < 0 || kind < 0 || kind > 4) {
-        return fail(h, HPM_EINVAL, "bad selftest arguments");
+        return hpm_fail(h, HPM_EINVAL, "bad selftest arguments");
     }
     if (n == 0) {
         return HPM_OK;
@@ -913,7 +1105,7 @@ extern "C" int hpm_device_selftest(hpm_handle* h, int kind, const void* in, void
     if (d_in) (void)hipFree(d_in);
     if (d_out) (void)hipFree(d_out);
     if (e != hipSuccess) {
-        return fail(h, HPM_EDEVICE, "selftest failed: %s", hipGetErrorString(e));
+        return hpm_fail(h, HPM_EDEVICE, "selftest failed: %s", hipGetErrorString(e));
     }
     return HPM_OK;
 }

@@ -125,6 +125,34 @@ class HpmSceneConfig(C.Structure):
     ]
 
 
+class HpmPipelineOptions(C.Structure):
+    """hpm_pipeline_options (include/hpm_pipeline.h)."""
+
+    _fields_ = [
+        ("params", HpmParams),
+        ("planar_prior", C.c_int32),
+        ("geom_consistency", C.c_int32),
+        ("num_threads", C.c_int32),
+        ("verbose", C.c_int32),
+    ]
+
+
+class HpmPipelineStats(C.Structure):
+    _fields_ = [
+        ("ms_wall", C.c_double),
+        ("ms_phase_a", C.c_double),
+        ("ms_phase_b", C.c_double),
+        ("ms_prior_host", C.c_double),
+        ("ms_prior_device", C.c_double),
+        ("ms_device", C.c_double),
+        ("n_eval", C.c_uint64),
+        ("n_eval_propagate", C.c_uint64),
+        ("ms_propagate", C.c_double),
+        ("runs", C.c_int32),
+        ("propagate_launches", C.c_int32),
+    ]
+
+
 _lib: Optional[C.CDLL] = None
 
 _FP = C.POINTER(C.c_float)
@@ -178,6 +206,29 @@ SIGNATURES = {
     "hpm_scene_default": (None, [C.POINTER(HpmSceneConfig)]),
     "hpm_scene_render": (C.c_int, [C.POINTER(HpmSceneConfig), C.POINTER(HpmCamera), _BP, _FP, _FP, _BP]),
     "hpm_scene_write_folder": (C.c_int, [C.POINTER(HpmSceneConfig), C.c_char_p]),
+    # pool (hpm_capi.h)
+    "hpm_pool_create": (C.c_void_p, [C.c_int, C.c_int, C.POINTER(HpmCamera), C.POINTER(_FP), C.c_int]),
+    "hpm_pool_destroy": (None, [C.c_void_p]),
+    "hpm_pool_camera": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(HpmCamera)]),
+    "hpm_pool_set_depth": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _FP]),
+    "hpm_pool_get_depth": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _FP]),
+    "hpm_input_init_pool": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, _IP, C.c_int]),
+    # hpm_prior.h
+    "hpm_support_points": (C.c_int, [_FP, C.c_int, C.c_int, _IP, C.c_int]),
+    "hpm_delaunay": (C.c_int, [_IP, C.c_int, _IP, C.c_int]),
+    "hpm_rasterize_triangles": (None, [_IP, _IP, C.c_int, _FP, C.c_int, C.c_int]),
+    "hpm_plane_from_points": (None, [C.POINTER(HpmCamera), _IP, _FP, C.c_int, C.c_float, _FP]),
+    "hpm_depth_from_plane_factor": (C.c_float, [C.POINTER(HpmCamera), _FP, C.c_int, C.c_int, C.c_float]),
+    "hpm_knn_supplement": (C.c_int, [C.POINTER(HpmCamera), _IP, C.c_int, _FP, C.c_int, C.c_int, C.c_float, C.c_int,
+                                     C.c_float, C.c_float, _FP, _FP, C.c_int]),
+    "hpm_multiscale_count": (C.c_int, [C.c_int, C.c_int]),
+    # hpm_pipeline.h
+    "hpm_pipeline_default": (None, [C.POINTER(HpmPipelineOptions)]),
+    "hpm_pipeline_run": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _IP, C.c_int, C.POINTER(HpmPipelineOptions),
+                                   C.POINTER(HpmPipelineStats), C.c_char_p, C.c_int]),
+    "hpm_pool_get_state": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP]),
+    "hpm_pipeline_run_folder": (C.c_int, [C.c_char_p, C.POINTER(HpmPipelineOptions), C.POINTER(HpmPipelineStats),
+                                          C.c_char_p, C.c_int]),
 }
 
 
@@ -516,3 +567,147 @@ def render_scene(cfg: HpmSceneConfig, want_gt: bool = True):
 def write_scene_folder(cfg: HpmSceneConfig, folder: str):
     if lib().hpm_scene_write_folder(C.byref(cfg), folder.encode()) != 0:
         raise HpmError(f"hpm_scene_write_folder failed for {folder}")
+
+
+# --- device pool and pass schedule (hpm_capi.h, hpm_pipeline.h) ---
+
+class Pool:
+    """All views of a scene resident on the device."""
+
+    def __init__(self, cams: Sequence[HpmCamera], images: Sequence[np.ndarray], device: int = 0,
+                 max_image_size: int = 3200):
+        n = len(cams)
+        self._imgs = [np.ascontiguousarray(im, dtype=np.float32) for im in images]
+        ptrs = (_FP * n)(*[fptr(im) for im in self._imgs])
+        self._p = lib().hpm_pool_create(device, n, (HpmCamera * n)(*cams), ptrs, max_image_size)
+        if not self._p:
+            raise HpmError("hpm_pool_create failed")
+        self.n = n
+
+    def close(self):
+        if self._p:
+            lib().hpm_pool_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def camera(self, v: int) -> HpmCamera:
+        c = HpmCamera()
+        if lib().hpm_pool_camera(self._p, v, C.byref(c)) != 0:
+            raise HpmError("hpm_pool_camera")
+        return c
+
+    def state(self, v: int):
+        c = self.camera(v)
+        hyp = np.empty((c.height, c.width, 4), np.float32)
+        cost = np.empty((c.height, c.width), np.float32)
+        if lib().hpm_pool_get_state(self._p, v, fptr(hyp), fptr(cost)) != 0:
+            raise HpmError(f"no state for view {v}")
+        return hyp, cost
+
+    def depth(self, v: int, which: int) -> np.ndarray:
+        c = self.camera(v)
+        out = np.empty((c.height, c.width), np.float32)
+        if lib().hpm_pool_get_depth(self._p, v, which, fptr(out)) != 0:
+            raise HpmError("hpm_pool_get_depth")
+        return out
+
+    def set_depth(self, v: int, which: int, depth: np.ndarray):
+        d = np.ascontiguousarray(depth, dtype=np.float32)
+        if lib().hpm_pool_set_depth(self._p, v, which, fptr(d)) != 0:
+            raise HpmError("hpm_pool_set_depth")
+
+
+def pipeline_options(**kw) -> HpmPipelineOptions:
+    o = HpmPipelineOptions()
+    lib().hpm_pipeline_default(C.byref(o))
+    params = kw.pop("params", None)
+    if params is not None:
+        o.params = params
+    for k, v in kw.items():
+        if hasattr(o.params, k) and not hasattr(o, k):
+            setattr(o.params, k, v)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def run_pipeline(pool: Pool, problems, opt: Optional[HpmPipelineOptions] = None) -> HpmPipelineStats:
+    """problems: list of (ref_view, [source views])."""
+    n = len(problems)
+    refs = np.array([p[0] for p in problems], np.int32)
+    ns = np.array([len(p[1]) for p in problems], np.int32)
+    srcs = np.zeros((n, HPM_MAX_SOURCES), np.int32)
+    for i, (_, s) in enumerate(problems):
+        srcs[i, : len(s)] = s
+    st = HpmPipelineStats()
+    err = C.create_string_buffer(512)
+    o = opt if opt is not None else pipeline_options()
+    rc = lib().hpm_pipeline_run(pool._p, n, iptr(refs), iptr(ns), iptr(srcs), HPM_MAX_SOURCES, C.byref(o),
+                                C.byref(st), err, 512)
+    if rc != 0:
+        raise HpmError(f"hpm_pipeline_run: {err.value.decode()}")
+    return st
+
+
+def run_pipeline_folder(folder: str, opt: Optional[HpmPipelineOptions] = None) -> HpmPipelineStats:
+    st = HpmPipelineStats()
+    err = C.create_string_buffer(512)
+    o = opt if opt is not None else pipeline_options()
+    if lib().hpm_pipeline_run_folder(folder.encode(), C.byref(o), C.byref(st), err, 512) != 0:
+        raise HpmError(f"hpm_pipeline_run_folder: {err.value.decode()}")
+    return st
+
+
+# --- host prior construction (hpm_prior.h) ---
+
+def support_points(cost: np.ndarray) -> np.ndarray:
+    c = np.ascontiguousarray(cost, dtype=np.float32)
+    h, w = c.shape
+    out = np.empty(((w // 5 + 1) * (h // 5 + 1), 2), np.int32)
+    n = lib().hpm_support_points(fptr(c), w, h, iptr(out), len(out))
+    return out[:n].copy()
+
+
+def delaunay(points: np.ndarray) -> np.ndarray:
+    p = np.ascontiguousarray(points, dtype=np.int32).reshape(-1, 2)
+    n = lib().hpm_delaunay(iptr(p), len(p), None, 0)
+    if n < 0:
+        raise HpmError("hpm_delaunay")
+    out = np.empty((max(n, 1), 3), np.int32)
+    lib().hpm_delaunay(iptr(p), len(p), iptr(out), n)
+    return out[:n]
+
+
+def rasterize_triangles(points: np.ndarray, tris: np.ndarray, w: int, h: int) -> np.ndarray:
+    p = np.ascontiguousarray(points, dtype=np.int32)
+    t = np.ascontiguousarray(tris, dtype=np.int32)
+    mask = np.zeros((h, w), np.float32)
+    lib().hpm_rasterize_triangles(iptr(p), iptr(t), len(t), fptr(mask), w, h)
+    return mask
+
+
+def plane_from_points(ref: HpmCamera, pts3: np.ndarray, depth: np.ndarray, factor: float) -> np.ndarray:
+    p = np.ascontiguousarray(pts3, dtype=np.int32).reshape(6)
+    d = np.ascontiguousarray(depth, dtype=np.float32)
+    out = np.empty(4, np.float32)
+    lib().hpm_plane_from_points(C.byref(ref), iptr(p), fptr(d), d.shape[1], factor, fptr(out))
+    return out
+
+
+def knn_supplement(ref: HpmCamera, points: np.ndarray, depth: np.ndarray, factor: float, k: int, dmin: float,
+                   dmax: float, mask: np.ndarray, threads: int = 0):
+    p = np.ascontiguousarray(points, dtype=np.int32)
+    d = np.ascontiguousarray(depth, dtype=np.float32)
+    h, w = d.shape
+    m = np.ascontiguousarray(mask, dtype=np.float32).copy()
+    planes = np.zeros((h, w, 4), np.float32)
+    rc = lib().hpm_knn_supplement(C.byref(ref), iptr(p), len(p), fptr(d), w, h, factor, k, dmin, dmax, fptr(m),
+                                  fptr(planes), threads)
+    if rc != 0:
+        raise HpmError("hpm_knn_supplement")
+    return m, planes

@@ -28,6 +28,7 @@ extern "C" {
 #endif
 
 typedef struct hpm_handle hpm_handle;
+typedef struct hpm_pool hpm_pool;
 
 #define HPM_OK 0
 #define HPM_EINVAL (-1)   /* bad argument or call order */
@@ -45,6 +46,19 @@ typedef struct hpm_run_stats {
     float ms_finalize_filter;
     int32_t launches;
 } hpm_run_stats;
+
+/* Every view of a scene resident on one device: images uploaded once
+ * (downscaled above max_image_size with the camera, HPM.cpp:524-551), plus
+ * per view the last plain-pass depth map (which 0, the reference's
+ * depths.dmb), the last geometric-consistency depth map (which 1,
+ * depths_geom.dmb) and the last hypotheses and costs. No reference
+ * counterpart: the reference re-reads images and .dmb files per problem. */
+hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera* cams, const float* const* images,
+                          int max_image_size);
+void hpm_pool_destroy(hpm_pool* pool);
+int hpm_pool_camera(const hpm_pool* pool, int view, hpm_camera* out);
+int hpm_pool_set_depth(hpm_pool* pool, int view, int which, const float* depth);
+int hpm_pool_get_depth(hpm_pool* pool, int view, int which, float* depth);
 
 /* HPM::HPM / HPM::~HPM (HPM.cpp:99-133). device < 0 selects device 0. */
 hpm_handle* hpm_create(int device);
@@ -69,6 +83,9 @@ int hpm_set_planar_prior(hpm_handle* h);
  * PGM (P5) images are decoded here; a .jpg path is tried as PGM content. */
 int hpm_input_init(hpm_handle* h, const char* dense_folder, int ref_image_id, const int* src_image_ids,
                    int num_src);
+/* The same on a pool: the problem's reference and source views are pool
+ * views, nothing is copied. The pool must outlive the handle's use of it. */
+int hpm_input_init_pool(hpm_handle* h, hpm_pool* pool, int ref_view, const int* src_views, int num_src);
 /* In-memory form of the same: images[i] is cams[i].width x cams[i].height
  * row-major grey; depths (may be NULL) gives the previous-pass depth map of
  * every image for geometric consistency. ref_image_id only salts the RNG. */

@@ -1,0 +1,69 @@
+/*
+ * hpm_pipeline.h — the per-scene pass schedule of HPM-MVS (main.cpp:35-499,
+ * 636-674) over a device-resident pool:
+ *   phase A, every view: plain RunPatchMatch, then (planar prior) one prior
+ *     RunPatchMatch per scale i = 2-k .. 2 of ComputingMultiScaleSettings:
+ *     low scales build the prior at 0.25x / 0.5x and upsample it with JBU,
+ *     the full scale builds it at 1x (triangles + KNN supplement);
+ *   phase B: geom_iterations geometric-consistency passes over all views,
+ *     the first reading the phase-A depth maps, later ones the latest
+ *     geometric depth maps (views in order, as the reference runs them).
+ * Views within a pass run in problem order on one device.
+ */
+#ifndef HPM_PIPELINE_H_
+#define HPM_PIPELINE_H_
+
+#include <stdint.h>
+
+#include "hpm_capi.h"
+#include "hpm_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hpm_pipeline_options {
+    hpm_params params;          /* seeds, knn_k and geom_iterations are read from here */
+    int32_t planar_prior;       /* phase A with the multi-scale planar prior (main.cpp:654) */
+    int32_t geom_consistency;   /* run phase B (main.cpp:662-674) */
+    int32_t num_threads;        /* host threads for the prior construction (<= 0: all) */
+    int32_t verbose;
+} hpm_pipeline_options;
+
+typedef struct hpm_pipeline_stats {
+    double ms_wall;             /* whole schedule, host clock */
+    double ms_phase_a;
+    double ms_phase_b;
+    double ms_prior_host;       /* support points, Delaunay, rasterisation, plane fits, KNN */
+    double ms_prior_device;     /* JBU and prior uploads */
+    double ms_device;           /* device time of all RunPatchMatch calls (hipEvents) */
+    uint64_t n_eval;            /* NCC evaluations of all RunPatchMatch calls */
+    uint64_t n_eval_propagate;
+    double ms_propagate;        /* device time of the propagation launches */
+    int32_t runs;               /* RunPatchMatch calls */
+    int32_t propagate_launches;
+} hpm_pipeline_stats;
+
+void hpm_pipeline_default(hpm_pipeline_options* opt);
+
+/* Runs the schedule for num_problems problems on the pool: problem i has
+ * reference view ref_ids[i] and sources src_ids[i * max_src + j],
+ * j < num_src[i]. Final maps stay in the pool (hpm_pool_get_state). err
+ * receives a message on failure. */
+int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref_ids, const int* num_src, const int* src_ids,
+                     int max_src, const hpm_pipeline_options* opt, hpm_pipeline_stats* stats, char* err,
+                     int err_len);
+
+/* Last (world normal xyz, depth) hypotheses and costs of a view. */
+int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, float* cost);
+
+/* main() (main.cpp:636-677) on a folder: pair.txt, images/, cams/; writes
+ * HPM/2333_%08d/{depths.dmb, depths_geom.dmb, normals.dmb, costs.dmb}. */
+int hpm_pipeline_run_folder(const char* dense_folder, const hpm_pipeline_options* opt, hpm_pipeline_stats* stats,
+                            char* err, int err_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPM_PIPELINE_H_ */

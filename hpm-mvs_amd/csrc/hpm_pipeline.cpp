@@ -157,7 +157,11 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
         hpm_set_planar_prior(h); /* SetPlanarPriorParams (HPM.cpp:477-480) */
         Maps m;
         maps_from_handle(h, &m);
-        const int k = hpm_multiscale_count(W, H);
+        /* ComputingMultiScaleSettings (HPM.cpp:851-860) with its bound */
+        int k = 0;
+        for (int max_size = std::max(W, H); max_size > c.opt->scale_bound; max_size /= 2) {
+            ++k;
+        }
         for (int i = 2 - k; i <= 2; ++i) {
             auto t0 = clk::now();
             if (i == 0 || i == 1) {
@@ -254,8 +258,9 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
 }
 
 /* ProcessProblem with geometric consistency (main.cpp:44-51, 481-490) */
-int process_phase_b(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc, bool multi)
+int process_phase_b(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc, int pass)
 {
+    const bool multi = pass > 0;
     hpm_params p = c.opt->params;
     p.planar_prior = 0;
     p.geom_consistency = 0;
@@ -268,6 +273,7 @@ int process_phase_b(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc, b
     if (rc < 0) return rc;
     rc = check(c, h, hpm_load_from_pool(h), "reload");
     if (rc < 0) return rc;
+    h->call_index = 8u + (uint32_t)pass;
     rc = check(c, h, hpm_run_internal(h, false), "RunPatchMatch (geometric)");
     if (rc < 0) return rc;
     account(c, h);
@@ -283,6 +289,7 @@ extern "C" void hpm_pipeline_default(hpm_pipeline_options* opt)
     opt->planar_prior = 1;
     opt->geom_consistency = 1;
     opt->num_threads = 0;
+    opt->scale_bound = 1000;
 }
 
 extern "C" int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, float* cost)
@@ -344,7 +351,7 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
                 if (c.opt->verbose) {
                     fprintf(stderr, "geometric pass %d: view %d\n", g, ref_ids[i]);
                 }
-                rc = process_phase_b(c, h, ref_ids[i], src_ids + (size_t)i * max_src, num_src[i], g > 0);
+                rc = process_phase_b(c, h, ref_ids[i], src_ids + (size_t)i * max_src, num_src[i], g);
             }
         }
         st->ms_phase_b = ms_since(t0);

@@ -134,6 +134,7 @@ class HpmPipelineOptions(C.Structure):
         ("geom_consistency", C.c_int32),
         ("num_threads", C.c_int32),
         ("verbose", C.c_int32),
+        ("scale_bound", C.c_int32),
     ]
 
 

@@ -9,6 +9,10 @@
  *     the first reading the phase-A depth maps, later ones the latest
  *     geometric depth maps (views in order, as the reference runs them).
  * Views within a pass run in problem order on one device.
+ *
+ * Random streams: RunPatchMatch call c of phase A uses Philox stream
+ * c (0..k); geometric pass g uses stream 8 + g, so no two passes of a view
+ * share random numbers (the reference reseeds from clock64() per launch).
  */
 #ifndef HPM_PIPELINE_H_
 #define HPM_PIPELINE_H_
@@ -28,6 +32,7 @@ typedef struct hpm_pipeline_options {
     int32_t geom_consistency;   /* run phase B (main.cpp:662-674) */
     int32_t num_threads;        /* host threads for the prior construction (<= 0: all) */
     int32_t verbose;
+    int32_t scale_bound;        /* size_bound of ComputingMultiScaleSettings (HPM.cpp:852), 1000 */
 } hpm_pipeline_options;
 
 typedef struct hpm_pipeline_stats {

@@ -1642,3 +1642,34 @@ extern "C" int hpm_oracle_uniforms(uint32_t seed_lo, uint32_t seed_hi, uint32_t 
     }
     return 0;
 }
+
+extern "C" int hpm_oracle_jbu_prior(const float* guide, int w, int h, const float* src_nd, int sw, int sh,
+                                    const hpm_camera* ref, float depth_min, float depth_max, float* planes,
+                                    uint32_t* mask, int num_threads)
+{
+    std::vector<float> d((size_t)w * h), n((size_t)w * h * 4);
+    const int rc = hpm_oracle_jbu(guide, w, h, src_nd, sw, sh, d.data(), n.data(), num_threads);
+    if (rc != 0) {
+        return rc;
+    }
+    const hpm_camera& c = *ref;
+    for (int py = 0; py < h; ++py) {
+        for (int px = 0; px < w; ++px) {
+            const size_t q = (size_t)py * w + px;
+            const float dq = d[q];
+            F4 pl = {0.0f, 0.0f, 0.0f, 0.0f};
+            uint32_t m = 0u;
+            if (dq <= depth_max && dq >= depth_min) {
+                m = 1u;
+                const F4 nw = ld4(n.data(), (int)q);
+                pl.x = c.R[0] * nw.x + c.R[1] * nw.y + c.R[2] * nw.z;
+                pl.y = c.R[3] * nw.x + c.R[4] * nw.y + c.R[5] * nw.z;
+                pl.z = c.R[6] * nw.x + c.R[7] * nw.y + c.R[8] * nw.z;
+                pl.w = dist_to_origin(c, px, py, dq, pl);
+            }
+            mask[q] = m;
+            st4(planes, (int)q, pl);
+        }
+    }
+    return 0;
+}

@@ -72,6 +72,16 @@ int hpm_oracle_cost_vectors(const hpm_oracle_problem* pb, const float* planes, c
 int hpm_oracle_jbu(const float* guide, int w, int h, const float* src_nd, int sw, int sh,
                    float* out_depth, float* out_normal, int num_threads);
 
+/* JBU of a low-resolution prior followed by the reload of main.cpp:179-204
+ * (and 330-353): mask 1 where the upsampled depth lies within
+ * [depth_min, depth_max], camera-frame plane (R n, -n.X) there
+ * (TransformNormal2RefCam, GetDistance2Origin_factor with the
+ * full-resolution K, HPM.cpp:882-899); planes 0 and mask 0 elsewhere.
+ * Returns 1 without writing when the integer scale is 1. */
+int hpm_oracle_jbu_prior(const float* guide, int w, int h, const float* src_nd, int sw, int sh,
+                         const hpm_camera* ref, float depth_min, float depth_max, float* planes, uint32_t* mask,
+                         int num_threads);
+
 /* Host evaluation of the shared elementary functions (hpm_fmath.h):
  * fn 0 exp, 1 sin, 2 cos, 3 acos. */
 int hpm_oracle_fmath(int fn, const float* x, float* y, int n);

@@ -1,0 +1,66 @@
+"""The device pass schedule (hpm_pipeline_run: plain pass, 0.5x prior with
+JBU, 1x prior, two geometric-consistency passes) against its CPU
+restatement (oracle/pipeline.py) on a 4-view 320x240 scene, with the scale
+bound lowered to 200 so that the JBU level runs. Bar: at least 99% of pixels
+within 1% relative depth and 5 degrees normal of the CPU run (the kernels are
+bit-identical stage by stage, so the maps are expected to match exactly)."""
+import os
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import depth_normal_agreement, say
+
+pytestmark = pytest.mark.gpu
+
+
+def scene(hpm):
+    cfg = hpm.scene_config(seed=21, width=320, height=240, fx=250.0, fy=250.0, cx=160.0, cy=120.0, num_views=4,
+                           num_planes=4, num_boxes=6, low_texture_fraction=0.3)
+    cams, imgs, gd, gn, low = hpm.render_scene(cfg)
+    n = cfg.num_views
+    problems = [(r, sorted([v for v in range(n) if v != r], key=lambda v: (abs(v - r), v))) for r in range(n)]
+    return cfg, cams, [im.astype(np.float32) for im in imgs], gd, problems
+
+
+def test_pipeline_matches_cpu_schedule(hpm):
+    from oracle import pipeline
+
+    cfg, cams, imgs, gd, problems = scene(hpm)
+    opt = hpm.pipeline_options(scale_bound=200)
+    opt.params.geom_iterations = 2
+    say("case: pipeline 4 views 320x240, prior scales 0.5x+1x, 2 geometric passes")
+    pool = hpm.Pool(cams, imgs)
+    st = hpm.run_pipeline(pool, problems, opt)
+    say(f"gpu pipeline: runs={st.runs} n_eval={st.n_eval} wall={st.ms_wall:.1f} ms device={st.ms_device:.1f} ms "
+        f"prior host={st.ms_prior_host:.1f} ms")
+    assert st.runs == 4 * (3 + 2)
+    cpu_state, d_plain, d_geom = pipeline.run_cpu_pipeline(cams, imgs, problems, opt)
+    for v, _ in problems:
+        g_hyp, g_cost = pool.state(v)
+        c_hyp, c_cost = cpu_state[v]
+        ok = depth_normal_agreement(g_hyp, c_hyp)
+        ident = np.all(g_hyp.view(np.uint32) == c_hyp.view(np.uint32), axis=2).mean()
+        gt = gd[v]
+        acc = np.mean(np.abs(g_hyp[..., 3] - gt) / gt < 0.02)
+        say(f"view {v}: agree={ok.mean():.5f} identical={ident:.5f} acc2%={acc:.4f}")
+        assert ok.mean() >= 0.99
+        assert np.array_equal(pool.depth(v, 0), d_plain[v]) or \
+            np.mean(np.abs(pool.depth(v, 0) - d_plain[v]) <= 0.01 * np.abs(d_plain[v])) >= 0.99
+    pool.close()
+
+
+def test_pipeline_folder_round_trip(hpm, tmp_path):
+    cfg = hpm.scene_config(seed=4, width=160, height=120, fx=125.0, fy=125.0, cx=80.0, cy=60.0, num_views=3)
+    hpm.write_scene_folder(cfg, str(tmp_path))
+    opt = hpm.pipeline_options()
+    opt.params.geom_iterations = 1
+    say("case: pipeline on a 3-view 160x120 folder")
+    st = hpm.run_pipeline_folder(str(tmp_path), opt)
+    assert st.runs == 3 * 2  # k = 0: plain + 1x prior, then one geometric pass
+    for v in range(3):
+        d = hpm.read_dmb(str(tmp_path / "HPM" / f"2333_{v:08d}" / "depths_geom.dmb"))
+        n = hpm.read_dmb(str(tmp_path / "HPM" / f"2333_{v:08d}" / "normals.dmb"))
+        assert d.shape == (120, 160) and n.shape == (120, 160, 3)
+        gt = hpm.read_dmb(str(tmp_path / "gt" / f"{v:08d}_depth.dmb"))
+        assert np.mean(np.abs(d - gt) / gt < 0.05) > 0.3

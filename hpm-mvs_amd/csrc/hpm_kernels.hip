@@ -498,7 +498,7 @@ __device__ uint32_t init_pixel(const hpm_dev_problem& pb, int px, int py)
     hpm_rng rng;
     hpm_rng_init(&rng, P.seed_lo, P.seed_hi, (uint32_t)center, hpm_stream_init(pb.call_index), pb.salt);
     RefPatch rp;
-    make_ref_patch(pb.images, pb.W, pb.H, px, py, P, &rp);
+    make_ref_patch(pb.images + pb.views[0].img_off, pb.W, pb.H, px, py, P, &rp);
     float4 plane;
     if (P.geom_consistency) {
         const float4 t = ld4(pb.hyp, center);
@@ -575,7 +575,7 @@ __device__ uint32_t propagate_pixel(const hpm_dev_problem& pb, int px, int py, i
     hpm_rng_init(&rng, P.seed_lo, P.seed_hi, (uint32_t)center, hpm_stream_update(pb.call_index, iter, colour),
                  pb.salt);
     RefPatch rp;
-    make_ref_patch(pb.images, W, H, px, py, P, &rp);
+    make_ref_patch(pb.images + pb.views[0].img_off, W, H, px, py, P, &rp);
 
     int pos[8];
     bool flag[8];
@@ -1136,7 +1136,7 @@ __global__ __launch_bounds__(256) void hpm_k_cost_vectors(hpm_dev_problem pb, co
     const int px = pix[2 * i], py = pix[2 * i + 1];
     const float4 pl = ld4(planes, i);
     RefPatch rp;
-    make_ref_patch(pb.images, pb.W, pb.H, px, py, pb.params, &rp);
+    make_ref_patch(pb.images + pb.views[0].img_off, pb.W, pb.H, px, py, pb.params, &rp);
     double q[3];
     plane_q(pb, pl, q);
     for (int v = 1; v <= pb.nsrc; ++v) {

@@ -57,7 +57,7 @@ def test_pipeline_folder_round_trip(hpm, tmp_path):
     opt.params.geom_iterations = 1
     say("case: pipeline on a 3-view 160x120 folder")
     st = hpm.run_pipeline_folder(str(tmp_path), opt)
-    assert st.runs == 3 * 2  # k = 0: plain + 1x prior, then one geometric pass
+    assert st.runs == 3 * (2 + 1)  # k = 0: plain + 1x prior per view, then one geometric pass
     for v in range(3):
         d = hpm.read_dmb(str(tmp_path / "HPM" / f"2333_{v:08d}" / "depths_geom.dmb"))
         n = hpm.read_dmb(str(tmp_path / "HPM" / f"2333_{v:08d}" / "normals.dmb"))

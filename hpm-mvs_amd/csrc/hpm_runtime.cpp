@@ -832,6 +832,15 @@ extern "C" int hpm_dims(const hpm_handle* h, int* width, int* height)
     return HPM_OK;
 }
 
+extern "C" int hpm_get_camera(const hpm_handle* h, int image, hpm_camera* out)
+{
+    if (!h || !h->have_input || !out || image < 0 || image >= h->num_images) {
+        return HPM_EINVAL;
+    }
+    *out = h->cams[(size_t)image];
+    return HPM_OK;
+}
+
 extern "C" int hpm_copy_reference_image(const hpm_handle* h, float* out)
 {
     if (!h || !h->have_input || !out) {

@@ -179,6 +179,7 @@ SIGNATURES = {
     "hpm_get_run_stats": (C.c_int, [C.c_void_p, C.POINTER(HpmRunStats)]),
     "hpm_dims": (C.c_int, [C.c_void_p, _IP, _IP]),
     "hpm_copy_reference_image": (C.c_int, [C.c_void_p, _FP]),
+    "hpm_get_camera": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(HpmCamera)]),
     "hpm_get_plane": (C.c_int, [C.c_void_p, C.c_int, _FP]),
     "hpm_get_cost": (C.c_float, [C.c_void_p, C.c_int]),
     "hpm_copy_hypotheses": (C.c_int, [C.c_void_p, _FP]),

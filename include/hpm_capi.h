@@ -106,6 +106,9 @@ int hpm_get_run_stats(const hpm_handle* h, hpm_run_stats* out);
 
 /* HPM::GetReferenceImageWidth / Height (HPM.cpp:724-732) */
 int hpm_dims(const hpm_handle* h, int* width, int* height);
+/* Camera of image i of the problem (0 = reference) after input init,
+ * including the max_image_size downscale (HPM.cpp:524-551). */
+int hpm_get_camera(const hpm_handle* h, int image, hpm_camera* out);
 /* HPM::GetReferenceImage (HPM.cpp:734-737): copies W*H floats. */
 int hpm_copy_reference_image(const hpm_handle* h, float* out);
 /* HPM::GetPlaneHypothesis / GetCost (HPM.cpp:739-747), one pixel. */

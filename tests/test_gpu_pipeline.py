@@ -50,6 +50,27 @@ def test_pipeline_matches_cpu_schedule(hpm):
     pool.close()
 
 
+@pytest.mark.parametrize("mode", [[], ["--class-api"]])
+def test_executable_on_folder(hpm, tmp_path, mode):
+    """The `HPM <dense_folder>` executable: device schedule, and the
+    reference's ProcessProblem written against the drop-in C++ class."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(hpm.LIB_PATH), "hpm_mvs")
+    cfg = hpm.scene_config(seed=6, width=192, height=144, fx=150.0, fy=150.0, cx=96.0, cy=72.0, num_views=3)
+    hpm.write_scene_folder(cfg, str(tmp_path))
+    say(f"case: hpm_mvs {' '.join(mode)} on a 3-view 192x144 folder")
+    r = subprocess.run([exe, str(tmp_path), "--geom-iterations", "1"] + mode, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    for v in range(3):
+        base = tmp_path / "HPM" / f"2333_{v:08d}"
+        d = hpm.read_dmb(str(base / "depths_geom.dmb"))
+        gt = hpm.read_dmb(str(tmp_path / "gt" / f"{v:08d}_depth.dmb"))
+        assert np.mean(np.abs(d - gt) / gt < 0.05) > 0.3
+        assert hpm.read_dmb(str(base / "costs.dmb")).shape == (144, 192)
+
+
 def test_pipeline_folder_round_trip(hpm, tmp_path):
     cfg = hpm.scene_config(seed=4, width=160, height=120, fx=125.0, fy=125.0, cx=80.0, cy=60.0, num_views=3)
     hpm.write_scene_folder(cfg, str(tmp_path))

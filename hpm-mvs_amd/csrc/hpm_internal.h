@@ -65,6 +65,8 @@ struct hpm_handle {
     float* d_prior = nullptr;
     uint32_t* d_mask = nullptr;
     unsigned long long* d_neval = nullptr;
+    float* d_jbu_src = nullptr;   /* low-resolution prior input of the JBU */
+    size_t jbu_src_cap = 0;       /* bytes */
     size_t alloc_pixels = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool have_prior = false;

@@ -11,11 +11,13 @@
 #include <string.h>
 #include <sys/stat.h>
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <mutex>
 #include <string>
-#include <vector>
-
 #include <thread>
+#include <vector>
 
 #include "hpm_internal.h"
 #include "hpm_io.h"
@@ -35,7 +37,8 @@ struct Ctx {
     const hpm_pipeline_options* opt;
     hpm_pipeline_stats* st;
     std::string err;
-    int threads;
+    int threads;      /* host threads for the KNN supplement of one worker */
+    std::mutex mu;    /* guards st and err across phase-A workers */
 };
 
 int fail(Ctx& c, const char* fmt, ...)
@@ -45,6 +48,7 @@ int fail(Ctx& c, const char* fmt, ...)
     va_start(ap, fmt);
     vsnprintf(buf, sizeof(buf), fmt, ap);
     va_end(ap);
+    std::lock_guard<std::mutex> lk(c.mu);
     c.err = buf;
     return HPM_EINVAL;
 }
@@ -52,13 +56,23 @@ int fail(Ctx& c, const char* fmt, ...)
 int check(Ctx& c, hpm_handle* h, int rc, const char* what)
 {
     if (rc < 0) {
-        c.err = std::string(what) + ": " + h->err;
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (c.err.empty()) {
+            c.err = std::string(what) + ": " + h->err;
+        }
     }
     return rc;
 }
 
+void add_ms(Ctx& c, double* field, double ms)
+{
+    std::lock_guard<std::mutex> lk(c.mu);
+    *field += ms;
+}
+
 void account(Ctx& c, const hpm_handle* h)
 {
+    std::lock_guard<std::mutex> lk(c.mu);
     c.st->ms_device += h->stats.ms_total;
     c.st->ms_propagate += h->stats.ms_propagate;
     c.st->n_eval += h->stats.n_eval;
@@ -199,7 +213,7 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
                         }
                     }
                 }
-                c.st->ms_prior_host += ms_since(t0);
+                add_ms(c, &c.st->ms_prior_host, ms_since(t0));
                 auto t1 = clk::now();
                 rc = check(c, h, hpm_jbu_prior(h, src_nd.data(), ws, hs, f), "JBU prior");
                 if (rc < 0) return rc;
@@ -213,7 +227,7 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
                     rc = check(c, h, hpm_upload_prior(h, zp.data(), zm.data()), "prior upload");
                     if (rc < 0) return rc;
                 }
-                c.st->ms_prior_device += ms_since(t1);
+                add_ms(c, &c.st->ms_prior_device, ms_since(t1));
             } else {
                 PriorScale ps;
                 build_prior_scale(c, ref_cam, m.depth.data(), m.cost.data(), W, H, 1.0f, dmin, dmax, &ps);
@@ -242,11 +256,11 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
                         }
                     }
                 }
-                c.st->ms_prior_host += ms_since(t0);
+                add_ms(c, &c.st->ms_prior_host, ms_since(t0));
                 auto t1 = clk::now();
                 rc = check(c, h, hpm_upload_prior(h, planes.data(), mask.data()), "prior upload");
                 if (rc < 0) return rc;
-                c.st->ms_prior_device += ms_since(t1);
+                add_ms(c, &c.st->ms_prior_device, ms_since(t1));
             }
             rc = check(c, h, hpm_run_internal(h, true), "RunPatchMatch (prior)");
             if (rc < 0) return rc;
@@ -290,6 +304,7 @@ extern "C" void hpm_pipeline_default(hpm_pipeline_options* opt)
     opt->geom_consistency = 1;
     opt->num_threads = 0;
     opt->scale_bound = 1000;
+    opt->workers = 4;
 }
 
 extern "C" int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, float* cost)
@@ -335,13 +350,52 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
         rc = fail(c, "cannot create a problem handle");
     }
     if (rc == HPM_OK) {
+        /* phase A: views are independent (main.cpp:658-661); host workers,
+         * each with its own handle and stream, overlap one view's host prior
+         * construction with the device work of the others */
         auto t0 = clk::now();
-        for (int i = 0; i < num_problems && rc >= 0; ++i) {
-            if (c.opt->verbose) {
-                fprintf(stderr, "phase A: view %d\n", ref_ids[i]);
+        const int workers = std::max(1, std::min(c.opt->workers, num_problems));
+        c.threads = std::max(1, c.threads / workers);
+        std::atomic<int> next(0);
+        std::atomic<int> worst(HPM_OK);
+        auto work = [&](hpm_handle* wh) {
+            for (;;) {
+                const int i = next.fetch_add(1);
+                if (i >= num_problems || worst.load() < 0) {
+                    return;
+                }
+                if (c.opt->verbose) {
+                    fprintf(stderr, "phase A: view %d\n", ref_ids[i]);
+                }
+                const int r = process_phase_a(c, wh, ref_ids[i], src_ids + (size_t)i * max_src, num_src[i]);
+                if (r < 0) {
+                    worst.store(r);
+                }
             }
-            rc = process_phase_a(c, h, ref_ids[i], src_ids + (size_t)i * max_src, num_src[i]);
+        };
+        std::vector<hpm_handle*> handles((size_t)workers, nullptr);
+        handles[0] = h;
+        for (int w = 1; w < workers; ++w) {
+            handles[(size_t)w] = hpm_create(pool->device);
+            if (!handles[(size_t)w]) {
+                rc = fail(c, "cannot create a problem handle");
+            }
         }
+        if (rc == HPM_OK) {
+            std::vector<std::thread> pool_threads;
+            for (int w = 1; w < workers; ++w) {
+                pool_threads.emplace_back(work, handles[(size_t)w]);
+            }
+            work(h);
+            for (auto& t : pool_threads) {
+                t.join();
+            }
+            rc = worst.load();
+        }
+        for (int w = 1; w < workers; ++w) {
+            hpm_destroy(handles[(size_t)w]);
+        }
+        c.threads = c.threads * workers;
         st->ms_phase_a = ms_since(t0);
     }
     if (rc >= 0 && c.opt->geom_consistency) {

@@ -143,6 +143,8 @@ void release_problem(hpm_handle* h)
     dfree(h->d_prior);
     dfree(h->d_mask);
     dfree(h->d_neval);
+    dfree(h->d_jbu_src);
+    h->jbu_src_cap = 0;
     h->alloc_pixels = 0;
     h->have_prior = false;
     h->have_device = false;
@@ -790,18 +792,22 @@ int hpm_jbu_prior(hpm_handle* h, const float* src_nd, int sw, int sh, float fact
         return rc;
     }
     const size_t sn = (size_t)sw * sh;
-    float* d_src = nullptr;
-    HPM_HIP(h, hipMalloc((void**)&d_src, sn * 16));
-    hipError_t e = hipMemcpyAsync(d_src, src_nd, sn * 16, hipMemcpyHostToDevice, h->stream);
+    if (h->jbu_src_cap < sn * 16) {
+        /* kept across calls: a free would synchronise the device */
+        dfree(h->d_jbu_src);
+        h->jbu_src_cap = 0;
+        HPM_HIP(h, hipMalloc((void**)&h->d_jbu_src, sn * 16));
+        h->jbu_src_cap = sn * 16;
+    }
+    hipError_t e = hipMemcpyAsync(h->d_jbu_src, src_nd, sn * 16, hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) {
-        e = (hipError_t)hpm_launch_jbu_prior(h->pool->d_images + h->pool->off[(size_t)h->views[0]], w, hgt, d_src, sw,
-                                             sh, scale_i, &h->cams[0], h->params.depth_min, h->params.depth_max,
-                                             h->d_prior, h->d_mask, h->stream);
+        e = (hipError_t)hpm_launch_jbu_prior(h->pool->d_images + h->pool->off[(size_t)h->views[0]], w, hgt,
+                                             h->d_jbu_src, sw, sh, scale_i, &h->cams[0], h->params.depth_min,
+                                             h->params.depth_max, h->d_prior, h->d_mask, h->stream);
     }
     if (e == hipSuccess) {
         e = hipStreamSynchronize(h->stream);
     }
-    (void)hipFree(d_src);
     if (e != hipSuccess) {
         return hpm_fail(h, HPM_EDEVICE, "JBU prior failed: %s", hipGetErrorString(e));
     }

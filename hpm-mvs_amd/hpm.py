@@ -135,6 +135,7 @@ class HpmPipelineOptions(C.Structure):
         ("num_threads", C.c_int32),
         ("verbose", C.c_int32),
         ("scale_bound", C.c_int32),
+        ("workers", C.c_int32),
     ]
 
 

@@ -33,6 +33,7 @@ typedef struct hpm_pipeline_options {
     int32_t num_threads;        /* host threads for the prior construction (<= 0: all) */
     int32_t verbose;
     int32_t scale_bound;        /* size_bound of ComputingMultiScaleSettings (HPM.cpp:852), 1000 */
+    int32_t workers;            /* phase-A host workers, each with its own handle and stream (4) */
 } hpm_pipeline_options;
 
 typedef struct hpm_pipeline_stats {

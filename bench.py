@@ -117,15 +117,13 @@ def main():
 
     tot = {k: 0.0 for k in ("ms_device", "ms_propagate", "ms_prior_host", "ms_prior_device", "ms_phase_a",
                             "ms_phase_b")}
-    evals = evals_prop = launches = runs = 0
+    evals = runs = 0
     t0 = time.perf_counter()
     for s in range(args.steps):
         st = hpm.run_pipeline(pool, problems, opt)
         for k in tot:
             tot[k] += getattr(st, k)
         evals += st.n_eval
-        evals_prop += st.n_eval_propagate
-        launches += st.propagate_launches
         runs += st.runs
         log(f"step {s}: {st.ms_wall:.0f} ms wall, {st.ms_device:.0f} ms device, {st.runs} RunPatchMatch, "
             f"{st.n_eval} evals")
@@ -140,8 +138,17 @@ def main():
         valid = g > 0
         acc.append(float(np.mean(np.abs(hyp[..., 3][valid] - g[valid]) / g[valid] < 0.02)))
 
-    avg_launch_ms = tot["ms_propagate"] / launches
-    flops_per_launch = evals_prop / launches * FLOPS_PER_EVAL
+    # Roofline of the propagation kernel, measured alone: during the timed
+    # schedule several workers' launches overlap on the device, so their
+    # event durations overstate a launch's own time. One plain RunPatchMatch
+    # per view on a single worker gives the isolated launch duration.
+    ropt = hpm.pipeline_options()
+    ropt.planar_prior = 0
+    ropt.geom_consistency = 0
+    ropt.workers = 1
+    rst = hpm.run_pipeline(pool, problems, ropt)
+    avg_launch_ms = rst.ms_propagate / rst.propagate_launches
+    flops_per_launch = rst.n_eval_propagate / rst.propagate_launches * FLOPS_PER_EVAL
     achieved = flops_per_launch / (avg_launch_ms * 1e-3) / 1e12
     per_view = lambda k: round(tot[k] / args.steps / n, 2)  # noqa: E731
 
@@ -172,6 +179,8 @@ def main():
             "accuracy_2pct": round(float(np.mean(acc)), 4),
             "runs_per_view": runs / args.steps / n,
             "n_eval_per_view": int(evals / args.steps / n),
+            "ms_per_view_phase_a": per_view("ms_phase_a"),
+            "ms_per_view_phase_b": per_view("ms_phase_b"),
             "ms_per_view_device": per_view("ms_device"),
             "ms_per_view_prior_host": per_view("ms_prior_host"),
             "ms_per_view_prior_device": per_view("ms_prior_device"),

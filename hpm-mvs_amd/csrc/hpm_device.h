@@ -6,6 +6,12 @@
  *   image pool   float, all images back to back, row-major, one grey value
  *                per pixel (the reference converts 8-bit JPEG to float,
  *                HPM.cpp:492-494);
+ *   padded pool  the same images with a border of 1 texel left and top, 2
+ *                right and bottom repeating the edge texels (pitch W + 3):
+ *                the bilinear footprint of every clamped source coordinate
+ *                in [-1, W] x [-1, H] is in bounds, so a source tap is two
+ *                8-byte row-pair gathers with 32-bit offsets and no integer
+ *                clamps;
  *   depth pool   float, previous-pass depth map of every image (geometric
  *                consistency only, HPM.cpp:561-588);
  *   hyp          float4 per reference pixel: camera-frame plane (n, d)
@@ -34,9 +40,10 @@ typedef struct hpm_dev_view {
     double A[9];
     double b[3];
     hpm_camera cam;
-    int32_t pad_;
+    int32_t pitch;      /* row pitch of the padded image (width + 3) */
     int64_t img_off;    /* offset of this image in the image pool (elements) */
     int64_t depth_off;  /* offset of this view's depth map in the depth pool */
+    int64_t pad_off;    /* offset of texel (-1, -1) of this image in the padded pool */
 } hpm_dev_view;
 
 typedef struct hpm_dev_problem {
@@ -51,6 +58,7 @@ typedef struct hpm_dev_problem {
     float thr[3];              /* view-selection thresholds per iteration */
     float fallback[3];         /* expf(thr^2 / -0.32) */
     const float* images;
+    const float* images_pad;   /* padded pool */
     const float* depths;
     const hpm_dev_view* views;
     float* hyp;                /* float4 per pixel */

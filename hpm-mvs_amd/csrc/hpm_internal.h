@@ -32,6 +32,9 @@ struct hpm_pool {
     std::vector<int64_t> off;                 /* pixel offset of each view in the device arrays */
     int64_t total = 0;
     float* d_images = nullptr;
+    std::vector<int64_t> pad_off;             /* offset of texel (-1, -1) of each view, padded pool */
+    int64_t pad_total = 0;
+    float* d_images_pad = nullptr;            /* replicated border: 1 left/top, 2 right/bottom */
     float* d_depth_plain = nullptr;           /* depths.dmb of each view */
     float* d_depth_geom = nullptr;            /* depths_geom.dmb of each view */
     float* d_state_hyp = nullptr;             /* float4: world normal, depth of the last pass */

@@ -28,9 +28,46 @@
 
 #define HPM_NTAP 36 /* 11x11 window at stride 2 (HPM.cu:347-356) */
 
+/* Waves per SIMD the NCC kernels are register-budgeted for: 2 leaves 256
+ * VGPRs, room for the 72 reference weights plus ~25 gathers in flight. */
+#ifndef HPM_NCC_WAVES
+#define HPM_NCC_WAVES 2
+#endif
+
+
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
+
+/* The view table is written by the host before a launch and only read by
+ * the kernels: reading it through the constant address space lets the
+ * compiler use scalar loads wherever the view index is wave-uniform. */
+typedef const __attribute__((address_space(4))) hpm_dev_view* hpm_cview;
+
+__device__ __forceinline__ hpm_cview cview(const hpm_dev_problem& pb, int v)
+{
+    return (hpm_cview)pb.views + v;
+}
+
+__device__ __forceinline__ hpm_camera camera(const hpm_dev_problem& pb, int v)
+{
+    const hpm_cview sv = cview(pb, v);
+    hpm_camera c;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        c.K[i] = sv->cam.K[i];
+        c.R[i] = sv->cam.R[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        c.t[i] = sv->cam.t[i];
+    }
+    c.height = sv->cam.height;
+    c.width = sv->cam.width;
+    c.depth_min = sv->cam.depth_min;
+    c.depth_max = sv->cam.depth_max;
+    return c;
+}
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi)
 {
@@ -68,21 +105,24 @@ __device__ __forceinline__ void plane_q(const hpm_dev_problem& pb, float4 n, dou
  * into an integer texel and a float fraction, plus the float coefficients of
  * the tap offsets s(p + d) - s0 = (g d) / (1 + e . d). */
 struct Warp {
-    int ix0, iy0;
+    uint32_t rowbase;     /* padded-pool element starting the row of iy0 (the pool holds < 2^30 texels) */
+    int cx;               /* ix0 + 1: padded column of ix0 */
     float fx0, fy0;
     float ex, ey, gxx, gxy, gyx, gyy;
+    float lox, hix, loy, hiy; /* clamp of the offsets: source coordinate in [-1, W] x [-1, H] */
+    bool small;           /* |e . d| <= 0.01 over the window: cubic reciprocal */
 };
 
 /* false when the warped centre leaves the source image (HPM.cu:333-335) */
-__device__ __forceinline__ bool make_warp(const hpm_dev_view& sv, const double q[3], int px, int py, int SW, int SH,
-                                          Warp* w)
+__device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int px, int py, Warp* w)
 {
+    const int SW = sv->cam.width, SH = sv->cam.height;
     double H[9];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            H[3 * r + c] = fma(-sv.b[r], q[c], sv.A[3 * r + c]);
+            H[3 * r + c] = fma(-sv->b[r], q[c], sv->A[3 * r + c]);
         }
     }
     const double X = (double)px, Y = (double)py;
@@ -96,8 +136,9 @@ __device__ __forceinline__ bool make_warp(const hpm_dev_view& sv, const double q
         return false;
     }
     const double fx = floor(sx0), fy = floor(sy0);
-    w->ix0 = (int)fx;
-    w->iy0 = (int)fy;
+    const int ix0 = (int)fx, iy0 = (int)fy;
+    w->rowbase = (uint32_t)sv->pad_off + (uint32_t)((iy0 + 1) * sv->pitch);
+    w->cx = ix0 + 1;
     w->fx0 = (float)(sx0 - fx);
     w->fy0 = (float)(sy0 - fy);
     w->ex = (float)(H[6] * icz);
@@ -106,43 +147,71 @@ __device__ __forceinline__ bool make_warp(const hpm_dev_view& sv, const double q
     w->gxy = (float)((H[1] - sx0 * H[7]) * icz);
     w->gyx = (float)((H[3] - sy0 * H[6]) * icz);
     w->gyy = (float)((H[4] - sy0 * H[7]) * icz);
+    w->lox = -1.0f - (float)ix0;
+    w->hix = (float)(SW - ix0);
+    w->loy = -1.0f - (float)iy0;
+    w->hiy = (float)(SH - iy0);
+    w->small = (fabsf(w->ex) + fabsf(w->ey)) * 5.0f <= 0.01f;
     return true;
 }
 
-/* linear-filtered, clamp-addressed fetch (the reference's tex2D on a
- * cudaFilterModeLinear texture, HPM.cpp:599-617) at tap (dx, dy) */
-__device__ __forceinline__ float sample_tap(const float* img, int W, int H, const Warp& w, float dx, float dy)
+/* Source coordinate of tap (dx, dy): byte offset of the top-left texel of
+ * its bilinear footprint in the padded pool and the bilinear fractions.
+ * kCubic: every active lane of the wave has small warps, so 1 / (1 + u) is
+ * the cubic without a per-lane choice; both forms give the same value per
+ * lane. */
+template <bool kCubic>
+__device__ __forceinline__ void tap_coord(const Warp& w, uint32_t pitch, float dx, float dy, uint32_t* off,
+                                          float* fa, float* fb)
 {
-    const float den = fmaf(w.ex, dx, fmaf(w.ey, dy, 1.0f));
-    const float inv = 1.0f / den;
+    const float u = fmaf(w.ex, dx, w.ey * dy);
+    const float cubic = fmaf(u, fmaf(u, 1.0f - u, -1.0f), 1.0f);
+    const float inv = (kCubic || w.small) ? cubic : 1.0f / (1.0f + u);
     float lx = fmaf(fmaf(w.gxx, dx, w.gxy * dy), inv, w.fx0);
     float ly = fmaf(fmaf(w.gyx, dx, w.gyy * dy), inv, w.fy0);
-    lx = fminf(fmaxf(lx, -1.0f - (float)w.ix0), (float)(W - w.ix0));
-    ly = fminf(fmaxf(ly, -1.0f - (float)w.iy0), (float)(H - w.iy0));
+    lx = fminf(fmaxf(lx, w.lox), w.hix);
+    ly = fminf(fmaxf(ly, w.loy), w.hiy);
     const float flx = floorf(lx);
     const float fly = floorf(ly);
-    const float a = lx - flx;
-    const float b = ly - fly;
-    const int ix = w.ix0 + (int)flx;
-    const int iy = w.iy0 + (int)fly;
-    const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
-    const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
-    const float* r0 = img + (size_t)y0 * W;
-    const float* r1 = img + (size_t)y1 * W;
-    const float t00 = r0[x0];
-    const float t10 = r0[x1];
-    const float t01 = r1[x0];
-    const float t11 = r1[x1];
-    const float v0 = fmaf(a, t10 - t00, t00);
-    const float v1 = fmaf(a, t11 - t01, t01);
-    return fmaf(b, v1 - v0, v0);
+    *fa = lx - flx;
+    *fb = ly - fly;
+    /* (ix0 + flx, iy0 + fly) lies in [-1, W] x [-1, H]: its footprint is
+     * inside the padded image, whose border repeats the edge texels */
+    const uint32_t qx = (uint32_t)(w.cx + (int)flx);
+    const uint32_t row = w.rowbase + (uint32_t)__mul24((int)fly, (int)pitch);
+    *off = (row + qx) * 4u;
 }
 
+/* pair of adjacent texels of the padded layout, 4-byte aligned */
+typedef float hpm_f2u __attribute__((ext_vector_type(2), aligned(4)));
+
+/* the 2x2 texels at byte offset off: two row pairs, pitch texels apart */
+__device__ __forceinline__ float4 footprint(const float* pool, uint32_t off, uint32_t pitch)
+{
+    const hpm_f2u top = *reinterpret_cast<const hpm_f2u*>(reinterpret_cast<const char*>(pool) + off);
+    const hpm_f2u bot = *reinterpret_cast<const hpm_f2u*>(reinterpret_cast<const char*>(pool) + (off + pitch * 4u));
+    return make_float4(top.x, top.y, bot.x, bot.y);
+}
+
+/* linear-filtered, clamp-addressed fetch (the reference's tex2D on a
+ * cudaFilterModeLinear texture, HPM.cpp:599-617) from a footprint: texels
+ * (x, y), (x + 1, y), (x, y + 1), (x + 1, y + 1) */
+__device__ __forceinline__ float bilerp(float4 q, float fa, float fb)
+{
+    const float v0 = fmaf(fa, q.y - q.x, q.x);
+    const float v1 = fmaf(fa, q.w - q.z, q.z);
+    return fmaf(fb, v1 - v0, v0);
+}
+
+/* Bilateral weights of the reference taps, (w, w * (r - c)) per tap and
+ * lane of a 256-lane block: [tap][lane]. In LDS they leave the VGPRs to the
+ * gathers in flight; every lane reads only its own entries. */
+__shared__ float2 s_refw[HPM_NTAP][256];
+
 /* reference-only part of ComputeBilateralNCC + ComputeBilateralWeight
- * (HPM.cu:318-369), held in registers for the whole pixel update */
+ * (HPM.cu:318-369): the window weights go to s_refw, the sums stay here */
 struct RefPatch {
-    float w[HPM_NTAP];
-    float wr[HPM_NTAP];
+    int lane;
     float c;
     float inv_w;
     float mr;
@@ -152,6 +221,7 @@ struct RefPatch {
 __device__ __forceinline__ void make_ref_patch(const float* img, int W, int H, int px, int py, const hpm_params& P,
                                                RefPatch* rp)
 {
+    const int lane = (int)(threadIdx.x + blockDim.x * threadIdx.y);
     const float c = texel(img, W, H, px, py);
     const float two_ss = 2.0f * P.sigma_spatial * P.sigma_spatial;
     const float two_sc = 2.0f * P.sigma_color * P.sigma_color;
@@ -169,47 +239,104 @@ __device__ __forceinline__ void make_ref_patch(const float* img, int W, int H, i
             const float w = hpm_expf(-sd / two_ss - cd / two_sc);
             const float rr = r - c;
             const float wr = w * rr;
-            rp->w[k] = w;
-            rp->wr[k] = wr;
+            s_refw[k][lane] = make_float2(w, wr);
             wsum = wsum + w;
             sr = sr + wr;
             srr = fmaf(wr, rr, srr);
         }
     }
+    rp->lane = lane;
     rp->c = c;
     rp->inv_w = 1.0f / wsum;
     rp->mr = sr * rp->inv_w;
     rp->var_r = srr * rp->inv_w - rp->mr * rp->mr;
 }
 
+/* One column of the window (6 taps at x offset dx): coordinates, then the
+ * 6 gathers. */
+template <bool kCubic>
+__device__ __forceinline__ void fetch_column(const float* pool, uint32_t pitch, const Warp& w, float dx, float4* quad,
+                                             float* fa, float* fb)
+{
+    uint32_t off[6];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        tap_coord<kCubic>(w, pitch, dx, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        quad[b] = footprint(pool, off[b], pitch);
+    }
+}
+
+/* Adds the 6 taps of window column a (taps 6a .. 6a + 5) to the sums, in
+ * tap order; tap 14 is the shift itself. */
+__device__ __forceinline__ void accumulate_column(const RefPatch& rp, int a, float shift, const float4* quad,
+                                                  const float* fa, const float* fb, float* ss, float* sss, float* srs)
+{
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const float raw = (a == 2 && b == 2) ? shift : bilerp(quad[b], fa[b], fb[b]);
+        const float s = raw - shift;
+        const float2 wk = s_refw[6 * a + b][rp.lane];
+        const float ws = wk.x * s;
+        *ss = *ss + ws;
+        *sss = fmaf(ws, s, *sss);
+        *srs = fmaf(wk.y, s, *srs);
+    }
+}
+
+/* Window sums of one (hypothesis, source view), software-pipelined by
+ * column: the gathers of column a + 1 are in flight while column a is
+ * accumulated. Tap k = 6 a + b sits at offset (2a - 5, 2b - 5). */
+template <bool kCubic>
+__device__ __forceinline__ void ncc_sums(const float* pool, uint32_t pitch, const RefPatch& rp, const Warp& w,
+                                         float* ss_out, float* sss_out, float* srs_out)
+{
+    float ss = 0.0f, sss = 0.0f, srs = 0.0f;
+    float4 cur[6], nxt[6];
+    float fa[6], fb[6], nfa[6], nfb[6];
+    /* source values relative to the tap nearest the window centre (tap 14,
+     * offset (-1, -1)), sampled first: a constant patch sums to exactly 0 */
+    uint32_t coff;
+    float cfa, cfb;
+    tap_coord<kCubic>(w, pitch, -1.0f, -1.0f, &coff, &cfa, &cfb);
+    const float4 cq = footprint(pool, coff, pitch);
+    fetch_column<kCubic>(pool, pitch, w, -5.0f, cur, fa, fb);
+    const float shift = bilerp(cq, cfa, cfb);
+    /* two columns per trip with the buffers alternating; the last two
+     * columns follow the loop, so every prefetch is unconditional and the
+     * waits stay on the older loads */
+#pragma unroll 1
+    for (int a = 0; a < 4; a += 2) {
+        fetch_column<kCubic>(pool, pitch, w, (float)(2 * a - 3), nxt, nfa, nfb);
+        accumulate_column(rp, a, shift, cur, fa, fb, &ss, &sss, &srs);
+        fetch_column<kCubic>(pool, pitch, w, (float)(2 * a - 1), cur, fa, fb);
+        accumulate_column(rp, a + 1, shift, nxt, nfa, nfb, &ss, &sss, &srs);
+    }
+    fetch_column<kCubic>(pool, pitch, w, 5.0f, nxt, nfa, nfb);
+    accumulate_column(rp, 4, shift, cur, fa, fb, &ss, &sss, &srs);
+    accumulate_column(rp, 5, shift, nxt, nfa, nfb, &ss, &sss, &srs);
+    *ss_out = ss;
+    *sss_out = sss;
+    *srs_out = srs;
+}
+
 /* ComputeBilateralNCC (HPM.cu:325-397) for source view v; q from plane_q */
 __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& rp, int px, int py, int v,
                                      const double q[3])
 {
-    const hpm_dev_view& sv = pb.views[v];
-    const int SW = sv.cam.width, SH = sv.cam.height;
+    const hpm_cview sv = cview(pb, v);
     Warp w;
-    if (!make_warp(sv, q, px, py, SW, SH, &w)) {
+    if (!make_warp(sv, q, px, py, &w)) {
         return 2.0f;
     }
-    const float* simg = pb.images + sv.img_off;
-    float ss = 0.0f, sss = 0.0f, srs = 0.0f;
-    /* source values relative to the tap nearest the window centre (tap 14,
-     * offset (-1, -1)), sampled first: a constant patch sums to exactly 0 */
-    constexpr int kc = 14;
-    const float shift = sample_tap(simg, SW, SH, w, -1.0f, -1.0f);
-#pragma unroll
-    for (int a = 0; a < 6; ++a) {
-#pragma unroll
-        for (int b = 0; b < 6; ++b) {
-            const int k = 6 * a + b;
-            const float raw = k == kc ? shift : sample_tap(simg, SW, SH, w, (float)(2 * a - 5), (float)(2 * b - 5));
-            const float s = raw - shift;
-            const float ws = rp.w[k] * s;
-            ss = ss + ws;
-            sss = fmaf(ws, s, sss);
-            srs = fmaf(rp.wr[k], s, srs);
-        }
+    const uint32_t pitch = (uint32_t)sv->pitch;
+    float ss, sss, srs;
+    if (__all(w.small)) {
+        ncc_sums<true>(pb.images_pad, pitch, rp, w, &ss, &sss, &srs);
+    } else {
+        ncc_sums<false>(pb.images_pad, pitch, rp, w, &ss, &sss, &srs);
     }
     const float ms = ss * rp.inv_w;
     const float var_s = sss * rp.inv_w - ms * ms;
@@ -352,8 +479,8 @@ __device__ __forceinline__ void project(const float X[3], const hpm_camera& c, f
 /* ComputeGeomConsistencyCost (HPM.cu:483-508) */
 __device__ float geom_cost(const hpm_dev_problem& pb, int v, float4 plane, int px, int py)
 {
-    const hpm_camera& rc = pb.views[0].cam;
-    const hpm_camera& sc = pb.views[v].cam;
+    const hpm_camera rc = camera(pb, 0);
+    const hpm_camera sc = camera(pb, v);
     const float depth = depth_from_plane(rc, plane, px, py);
     float fw[3];
     point_world((float)px, (float)py, depth, rc, fw);
@@ -363,7 +490,7 @@ __device__ float geom_cost(const hpm_dev_problem& pb, int v, float4 plane, int p
     const float cyf = fminf(fmaxf(sy, -1.0f), (float)sc.height);
     const int ix = clampi((int)cxf, 0, sc.width - 1);
     const int iy = clampi((int)cyf, 0, sc.height - 1);
-    const float src_depth = pb.depths[pb.views[v].depth_off + (int64_t)iy * sc.width + ix];
+    const float src_depth = pb.depths[cview(pb, v)->depth_off + (int64_t)iy * sc.width + ix];
     if (src_depth == 0.0f) {
         return 5.0f;
     }
@@ -492,13 +619,13 @@ __device__ __forceinline__ void count_evals(const hpm_dev_problem& pb, uint32_t 
 __device__ uint32_t init_pixel(const hpm_dev_problem& pb, int px, int py)
 {
     const hpm_params& P = pb.params;
-    const hpm_camera& rc = pb.views[0].cam;
+    const hpm_camera rc = camera(pb, 0);
     const int center = py * pb.W + px;
     uint32_t nev = 0;
     hpm_rng rng;
     hpm_rng_init(&rng, P.seed_lo, P.seed_hi, (uint32_t)center, hpm_stream_init(pb.call_index), pb.salt);
     RefPatch rp;
-    make_ref_patch(pb.images + pb.views[0].img_off, pb.W, pb.H, px, py, P, &rp);
+    make_ref_patch(pb.images + cview(pb, 0)->img_off, pb.W, pb.H, px, py, P, &rp);
     float4 plane;
     if (P.geom_consistency) {
         const float4 t = ld4(pb.hyp, center);
@@ -558,15 +685,45 @@ __device__ __forceinline__ float prior_term(const PriorConsts& k, float depth_di
     return k.gamma + hpm_expf(-depth_diff * depth_diff / k.tdss) * hpm_expf(-ad * ad / k.tass);
 }
 
-/* scan of an axis strip (HPM.cu:1251-1341): start at `start`, up to 4 more
- * samples `step` apart while `ok(i)`; keeps the minimum, or the maximum
- * for right_far */
+/* sum over the selected views (vw > 0) in view order of vw * term: the
+ * weighted costs of HPM.cu:1637-1650 and 1191-1200 */
+__device__ __forceinline__ float selected_sum(const float* vw, const float* term, int nsrc)
+{
+    float acc = 0.0f;
+    for (int i = 0; i < nsrc; ++i) {
+        if (vw[i] > 0.0f) {
+            acc += vw[i] * term[i];
+        }
+    }
+    return acc;
+}
+
+/* Jobs of the per-lane evaluation machine of propagate_pixel. Every NCC
+ * evaluation of a pixel update belongs to one job: a hypothesis and the set
+ * of source views it is scored on. */
+enum : int {
+    kJobDir = 0,      /* 0..7: neighbour hypothesis of direction d, every view */
+    kJobExt = 8,      /* NESP extension of direction 0, every view */
+    kJobRescore = 9,  /* current hypothesis, selected views */
+    kJobRefine = 10,  /* 10..14: refinement candidates, selected views */
+    kJobDone = 15
+};
+
 /* CheckerboardPropagation (HPM.cu:1222-1725) + PlaneHypothesisRefinement
- * (HPM.cu:1143-1220) */
-__device__ uint32_t propagate_pixel(const hpm_dev_problem& pb, int px, int py, int iter, int colour)
+ * (HPM.cu:1143-1220).
+ *
+ * The reference scores hypotheses phase by phase (eight neighbours, the
+ * NESP extension, the current hypothesis, five refinement candidates); here
+ * each lane walks the same sequence as a queue of jobs, and every loop trip
+ * scores one source view for each lane whose current job still needs it, so
+ * a wave runs one NCC code path while its lanes may sit in different phases.
+ * Per-view results are stored and summed in view order, and the decisions
+ * between phases run when a lane's queue reaches them, so every value and
+ * every sum order is the sequential one. */
+__device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, int px, int py, int iter, int colour)
 {
     const hpm_params& P = pb.params;
-    const hpm_camera& rc = pb.views[0].cam;
+    const hpm_camera rc = camera(pb, 0);
     const int W = pb.W, H = pb.H, nsrc = pb.nsrc;
     const float* cost = pb.cost;
     const int center = py * W + px;
@@ -575,7 +732,7 @@ __device__ uint32_t propagate_pixel(const hpm_dev_problem& pb, int px, int py, i
     hpm_rng_init(&rng, P.seed_lo, P.seed_hi, (uint32_t)center, hpm_stream_update(pb.call_index, iter, colour),
                  pb.salt);
     RefPatch rp;
-    make_ref_patch(pb.images + pb.views[0].img_off, W, H, px, py, P, &rp);
+    make_ref_patch(pb.images + cview(pb, 0)->img_off, W, H, px, py, P, &rp);
 
     int pos[8];
     bool flag[8];
@@ -588,7 +745,8 @@ __device__ uint32_t propagate_pixel(const hpm_dev_problem& pb, int px, int py, i
     pos[6] = center - 6 + 5 * W;  /* left_down */
     pos[7] = center + 5;          /* right_far */
 
-    /* axis strips (HPM.cu:1251-1341) */
+    /* axis strips (HPM.cu:1251-1341): start sample, up to 4 more two apart;
+     * keeps the minimum, the maximum for right_far */
     flag[1] = py > 4;
     if (flag[1]) {
         float cmin = cost[pos[1]];
@@ -692,314 +850,396 @@ __device__ uint32_t propagate_pixel(const hpm_dev_problem& pb, int px, int py, i
         }
     }
     ca[0][0] = 2.0f;
-    for (int d = 0; d < 8; ++d) {
-        if (flag[d]) {
-            cost_vector(pb, rp, px, py, ld4(pb.hyp, pos[d]), ca[d], nullptr, &nev);
-        }
-    }
+    const uint32_t all_views = nsrc >= 32 ? 0xffffffffu : ((1u << nsrc) - 1u);
 
-    /* NESP extension, orientation 0 only (HPM.cu:1473-1543, 780-843) */
-    for (int k = 0; k < 3; ++k) {
-        if (!flag[0]) {
-            break;
-        }
-        {
-            /* JudgeExtend (HPM.cu:1036-1059) */
-            const float good = pb.good[iter][k];
-            int good_sum = 0, bad_sum = 0;
-            for (int i = 0; i < nsrc; ++i) {
-                if (ca[0][i] < good) {
-                    ++good_sum;
-                }
-                if ((double)ca[0][i] > 1.2) {
-                    ++bad_sum;
-                }
-            }
-            if (good_sum >= 1 && bad_sum <= 2) {
-                break;
-            }
-        }
-        int lu = center - 5 - 6 * W - 4 * 2 * k - 4 * 2 * W * k;
-        if (px > 4 + 8 * k && py > 5 + 8 * k) {
-            bool symbol = true;
-            float cmin = cost[lu];
-            int cmp = lu;
-            int qx = px - 5 - 8 * k, qy = py - 6 - 8 * k;
-            for (int i = 0; i < 7; i++) {
-                if (i % 2 == 0) {
-                    qx -= 2;
-                } else {
-                    qy -= 2;
-                }
-                if (qx >= 0 && qx < W && qy < H && qy >= 0) {
-                    const int pt = qy * W + qx;
-                    if (cost[pt] < cmin) {
-                        cmin = cost[pt];
-                        cmp = pt;
-                    }
-                } else {
-                    break;
-                }
-            }
-            if (cmin > cost[pos[0]]) {
-                cmp = pos[0];
-                symbol = false;
-            }
-            lu = cmp;
-            if (symbol && lu != pos[0]) {
-                cost_vector(pb, rp, px, py, ld4(pb.hyp, lu), ca[0], nullptr, &nev);
-            }
-        } else {
-            lu = pos[0];
-        }
-        pos[0] = lu;
-    }
-
-    /* joint view selection (HPM.cu:1548-1613) */
-    float vw[HPM_MAX_SOURCES], probs[HPM_MAX_SOURCES];
-    for (int i = 0; i < nsrc; ++i) {
-        vw[i] = 0.0f;
-        probs[i] = 0.0f;
-    }
-    {
-        float vprior[HPM_MAX_SOURCES];
-        for (int i = 0; i < nsrc; ++i) {
-            vprior[i] = 0.0f;
-        }
-        const int nb[4] = {center - W * 5, center + W * 5, center - 5, center + 5};
-        for (int i = 0; i < 4; ++i) {
-            if (flag[2 * i + 1]) {
-                const uint32_t m = pb.sel[nb[i]];
-                for (int j = 0; j < nsrc; ++j) {
-                    if (((m >> j) & 1u) == 1u) {
-                        vprior[j] += 0.9f;
-                    } else {
-                        vprior[j] += 0.1f;
-                    }
-                }
-            }
-        }
-        const float thr = pb.thr[iter];
-        for (int i = 0; i < nsrc; i++) {
-            float count = 0.0f;
-            int count_false = 0;
-            float tmpw = 0.0f;
-            for (int j = 0; j < 8; j++) {
-                const float c = ca[j][i];
-                if (c < thr) {
-                    tmpw += hpm_expf(c * c / (-0.18f));
-                    count++;
-                }
-                if (c > 1.2f) {
-                    count_false++;
-                }
-            }
-            float pr = 0.0f;
-            if (count > 2 && count_false < 3) {
-                pr = tmpw / count;
-            } else if (count_false < 3) {
-                pr = pb.fallback[iter];
-            }
-            probs[i] = pr * vprior[i];
-        }
-    }
-    {
-        /* TransformPDFToCDF (HPM.cu:86-100) */
-        float sum = 0.0f;
-        for (int i = 0; i < nsrc; ++i) {
-            sum += probs[i];
-        }
-        const float inv = 1.0f / sum;
-        float cum = 0.0f;
-        for (int i = 0; i < nsrc; ++i) {
-            cum += probs[i] * inv;
-            probs[i] = cum;
-        }
-    }
-    for (int s = 0; s < 15; ++s) {
-        const float u = hpm_rng_uniform(&rng) - FLT_EPSILON;
-        for (int v = 0; v < nsrc; ++v) {
-            if (probs[v] > u) {
-                vw[v] += 1.0f;
-                break;
-            }
-        }
-    }
+    /* state handed between the phases */
+    float vw[HPM_MAX_SOURCES];
+    uint32_t sel_views = 0u; /* views with vw > 0 */
     uint32_t tsel = 0u;
     float wn = 0.0f;
-    for (int i = 0; i < nsrc; ++i) {
-        if (vw[i] > 0) {
-            tsel |= (1u << i);
-            wn += vw[i];
-        }
-    }
-
-    /* candidate scoring (HPM.cu:1615-1635) */
     float fc[8];
-    for (int i = 0; i < 8; ++i) {
-        float acc = 0.0f;
-        float4 hp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (P.geom_consistency && flag[i]) {
-            hp = ld4(pb.hyp, pos[i]);
-        }
-        for (int j = 0; j < nsrc; ++j) {
-            if (vw[j] > 0) {
-                if (P.geom_consistency) {
-                    if (flag[i]) {
-                        acc += vw[j] * (ca[i][j] + 0.1f * geom_cost(pb, j + 1, hp, px, py));
-                    } else {
-                        acc += vw[j] * (ca[i][j] + 0.1f * 5.0f);
-                    }
-                } else {
-                    acc += vw[j] * ca[i][j];
-                }
-            }
-        }
-        fc[i] = acc / wn;
-    }
-    const int min_idx = find_min_idx8(fc);
-
-    /* re-score the current hypothesis (HPM.cu:1637-1650) */
+    int min_idx = 0;
     const float4 cur = ld4(pb.hyp, center);
     float cost_now = 0.0f;
-    {
-        float cvn[HPM_MAX_SOURCES];
-        cost_vector(pb, rp, px, py, cur, cvn, vw, &nev);
-        for (int i = 0; i < nsrc; ++i) {
-            if (!(vw[i] > 0.0f)) {
-                continue;
-            }
-            if (P.geom_consistency) {
-                cost_now += vw[i] * (cvn[i] + 0.1f * geom_cost(pb, i + 1, cur, px, py));
-            } else {
-                cost_now += vw[i] * cvn[i];
-            }
-        }
-    }
-    cost_now /= wn;
-    float cost_center = cost_now;
+    float cost_center = 0.0f;
     float4 plane_center = cur;
-    uint32_t sel_center = pb.sel[center];
-    float depth_now = depth_from_plane(rc, cur, px, py);
-
-    /* adoption (HPM.cu:1652-1722) */
+    uint32_t sel_center = 0u;
+    float depth_now = 0.0f;
     float restricted_cost = 0.0f;
     const PriorConsts pk = prior_consts(P);
     const bool prior_on = P.planar_prior != 0;
     const bool prior_pix = prior_on && pb.prior_mask[center] > 0u;
     float4 prior_plane = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (prior_on) {
-        prior_plane = ld4(pb.prior_planes, center);
-        const float depth_prior = depth_from_plane(rc, prior_plane, px, py);
-        if (prior_pix) {
-            float rfc[8];
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                rfc[i] = 0.0f;
-                if (flag[i]) {
-                    const float4 hp = ld4(pb.hyp, pos[i]);
-                    const float dn = depth_from_plane(rc, hp, px, py);
-                    const float pr = prior_term(pk, dn - depth_prior, prior_plane, hp);
-                    rfc[i] = hpm_expf(-fc[i] * fc[i] / pk.beta) * pr;
-                }
+    float depth_prior = 0.0f;
+    float4 cand[5];
+    float cand_d[5];
+
+    double q[3];
+    /* the eight neighbour hypotheses on every view: the same work for every
+     * lane away from the image border, so the wave runs it in lockstep */
+#pragma unroll 1
+    for (int d = 0; d < 8; ++d) {
+        if (flag[d]) {
+            plane_q(pb, ld4(pb.hyp, pos[d]), q);
+#pragma unroll 1
+            for (int v = 0; v < nsrc; ++v) {
+                ca[d][v] = ncc(pb, rp, px, py, v + 1, q);
             }
-            const int max_idx = find_max_idx8(rfc);
-            const float dn = depth_from_plane(rc, cur, px, py);
-            const float pr = prior_term(pk, dn - depth_prior, prior_plane, cur);
-            const float rcn = hpm_expf(-cost_now * cost_now / pk.beta) * pr;
-            if (flag[max_idx]) {
-                const float4 hp = ld4(pb.hyp, pos[max_idx]);
-                const float db = depth_from_plane(rc, hp, px, py);
-                if (db >= P.depth_min && db <= P.depth_max && rfc[max_idx] > rcn) {
-                    depth_now = db;
-                    plane_center = hp;
-                    cost_center = fc[max_idx];
-                    restricted_cost = rfc[max_idx];
-                    sel_center = tsel;
-                }
-            }
-        } else if (flag[min_idx]) {
-            const float4 hp = ld4(pb.hyp, pos[min_idx]);
-            const float db = depth_from_plane(rc, hp, px, py);
-            if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
-                depth_now = db;
-                plane_center = hp;
-                cost_center = fc[min_idx];
-            }
-        }
-    }
-    if (!prior_on && flag[min_idx]) {
-        const float4 hp = ld4(pb.hyp, pos[min_idx]);
-        const float db = depth_from_plane(rc, hp, px, py);
-        if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
-            depth_now = db;
-            plane_center = hp;
-            cost_center = fc[min_idx];
-            sel_center = tsel;
-        }
-        if (cost_center != cost_center) {
-            depth_now = db;
-            plane_center = hp;
-            cost_now = fc[min_idx];
-            sel_center = tsel;
+            nev += (uint32_t)nsrc;
         }
     }
 
-    /* PlaneHypothesisRefinement (HPM.cu:1143-1220) */
-    {
-        const float perturbation = 0.02f;
-        float depth_prior = 0.0f;
-        float depth_rand;
-        float4 n_rand;
-        const float ds = (P.depth_max - P.depth_min) / 64.0f;
-        const float as = (float)(kPi * (double)(5.0f / 180.0f));
-        if (prior_pix) {
-            depth_prior = depth_from_plane(rc, prior_plane, px, py);
-            depth_rand = hpm_rng_uniform(&rng) * 6 * ds + (depth_prior - 3 * ds);
-            n_rand = perturbed_normal(rc, px, py, prior_plane, &rng, as);
-        } else {
-            depth_rand = hpm_rng_uniform(&rng) * (P.depth_max - P.depth_min) + P.depth_min;
-            n_rand = random_normal(rc, px, py, &rng, depth_now);
-        }
-        const float dmin_p = (1 - perturbation) * depth_now;
-        const float dmax_p = (1 + perturbation) * depth_now;
-        const float depth_pert = hpm_rng_uniform(&rng) * (dmax_p - dmin_p) + dmin_p;
-        const float4 plane_now = plane_center;
-        const float4 n_pert = perturbed_normal(rc, px, py, plane_now, &rng, (float)((double)perturbation * kPi));
-        const float dnow = depth_now;
-        for (int i = 0; i < 5; ++i) {
-            const float di = (i == 0 || i == 2) ? depth_rand : (i == 4 ? depth_pert : dnow);
-            float4 t = (i == 0 || i == 4) ? plane_now : (i == 3 ? n_pert : n_rand);
-            t.w = dist_to_origin(rc, px, py, di, t);
-            float cvt[HPM_MAX_SOURCES];
-            cost_vector(pb, rp, px, py, t, cvt, vw, &nev);
-            float tc = 0.0f;
-            for (int j = 0; j < nsrc; ++j) {
-                if (vw[j] > 0) {
-                    if (P.geom_consistency) {
-                        tc += vw[j] * (cvt[j] + 0.1f * geom_cost(pb, j + 1, t, px, py));
+    /* the rest runs as a job machine: the extension rounds, the rescore and
+     * the refinement candidates take a different number of evaluations on
+     * every lane */
+    int job = kJobDir + 7;
+    int ext_k = 0;
+    uint32_t todo = 0u;
+    float4 jplane = cur;
+    bool fresh = false; /* q not yet computed for the current job */
+    float jc[HPM_MAX_SOURCES]; /* per-view terms of a rescore / refinement job */
+
+    for (;;) {
+        /* finish the lane's job and start the next while no view is left */
+        while (todo == 0u && job != kJobDone) {
+            if (job == kJobDir + 7 || job == kJobExt) {
+                if (job == kJobExt) {
+                    ++ext_k;
+                }
+                /* NESP extension, orientation 0 only (HPM.cu:1473-1543,
+                 * 780-843): up to 3 rounds, each judged on ca[0] */
+                bool extend = false;
+                while (ext_k < 3 && flag[0]) {
+                    /* JudgeExtend (HPM.cu:1036-1059) */
+                    const float good = pb.good[iter][ext_k];
+                    int good_sum = 0, bad_sum = 0;
+                    for (int i = 0; i < nsrc; ++i) {
+                        if (ca[0][i] < good) {
+                            ++good_sum;
+                        }
+                        if ((double)ca[0][i] > 1.2) {
+                            ++bad_sum;
+                        }
+                    }
+                    if (good_sum >= 1 && bad_sum <= 2) {
+                        break;
+                    }
+                    const int k = ext_k;
+                    int lu = center - 5 - 6 * W - 4 * 2 * k - 4 * 2 * W * k;
+                    if (px > 4 + 8 * k && py > 5 + 8 * k) {
+                        bool symbol = true;
+                        float cmin = cost[lu];
+                        int cmp = lu;
+                        int qx = px - 5 - 8 * k, qy = py - 6 - 8 * k;
+                        for (int i = 0; i < 7; i++) {
+                            if (i % 2 == 0) {
+                                qx -= 2;
+                            } else {
+                                qy -= 2;
+                            }
+                            if (qx >= 0 && qx < W && qy < H && qy >= 0) {
+                                const int pt = qy * W + qx;
+                                if (cost[pt] < cmin) {
+                                    cmin = cost[pt];
+                                    cmp = pt;
+                                }
+                            } else {
+                                break;
+                            }
+                        }
+                        if (cmin > cost[pos[0]]) {
+                            cmp = pos[0];
+                            symbol = false;
+                        }
+                        lu = cmp;
+                        extend = symbol && lu != pos[0];
                     } else {
-                        tc += vw[j] * cvt[j];
+                        lu = pos[0];
+                    }
+                    pos[0] = lu;
+                    if (extend) {
+                        break;
+                    }
+                    ++ext_k;
+                }
+                if (extend) {
+                    job = kJobExt;
+                    jplane = ld4(pb.hyp, pos[0]);
+                    todo = all_views;
+                    fresh = true;
+                    continue;
+                }
+
+                /* joint view selection (HPM.cu:1548-1613) */
+                float probs[HPM_MAX_SOURCES];
+                for (int i = 0; i < nsrc; ++i) {
+                    vw[i] = 0.0f;
+                    probs[i] = 0.0f;
+                }
+                {
+                    float vprior[HPM_MAX_SOURCES];
+                    for (int i = 0; i < nsrc; ++i) {
+                        vprior[i] = 0.0f;
+                    }
+                    const int nb[4] = {center - W * 5, center + W * 5, center - 5, center + 5};
+                    for (int i = 0; i < 4; ++i) {
+                        if (flag[2 * i + 1]) {
+                            const uint32_t m = pb.sel[nb[i]];
+                            for (int j = 0; j < nsrc; ++j) {
+                                if (((m >> j) & 1u) == 1u) {
+                                    vprior[j] += 0.9f;
+                                } else {
+                                    vprior[j] += 0.1f;
+                                }
+                            }
+                        }
+                    }
+                    const float thr = pb.thr[iter];
+                    for (int i = 0; i < nsrc; i++) {
+                        float count = 0.0f;
+                        int count_false = 0;
+                        float tmpw = 0.0f;
+                        for (int j = 0; j < 8; j++) {
+                            const float c = ca[j][i];
+                            if (c < thr) {
+                                tmpw += hpm_expf(c * c / (-0.18f));
+                                count++;
+                            }
+                            if (c > 1.2f) {
+                                count_false++;
+                            }
+                        }
+                        float pr = 0.0f;
+                        if (count > 2 && count_false < 3) {
+                            pr = tmpw / count;
+                        } else if (count_false < 3) {
+                            pr = pb.fallback[iter];
+                        }
+                        probs[i] = pr * vprior[i];
                     }
                 }
+                {
+                    /* TransformPDFToCDF (HPM.cu:86-100) */
+                    float sum = 0.0f;
+                    for (int i = 0; i < nsrc; ++i) {
+                        sum += probs[i];
+                    }
+                    const float inv = 1.0f / sum;
+                    float cum = 0.0f;
+                    for (int i = 0; i < nsrc; ++i) {
+                        cum += probs[i] * inv;
+                        probs[i] = cum;
+                    }
+                }
+                for (int s = 0; s < 15; ++s) {
+                    const float u = hpm_rng_uniform(&rng) - FLT_EPSILON;
+                    for (int v = 0; v < nsrc; ++v) {
+                        if (probs[v] > u) {
+                            vw[v] += 1.0f;
+                            break;
+                        }
+                    }
+                }
+                for (int i = 0; i < nsrc; ++i) {
+                    if (vw[i] > 0) {
+                        tsel |= (1u << i);
+                        wn += vw[i];
+                    }
+                }
+                sel_views = tsel;
+
+                /* candidate scoring (HPM.cu:1615-1635) */
+                for (int i = 0; i < 8; ++i) {
+                    float a = 0.0f;
+                    float4 hp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (P.geom_consistency && flag[i]) {
+                        hp = ld4(pb.hyp, pos[i]);
+                    }
+                    for (int j = 0; j < nsrc; ++j) {
+                        if (vw[j] > 0) {
+                            if (P.geom_consistency) {
+                                if (flag[i]) {
+                                    a += vw[j] * (ca[i][j] + 0.1f * geom_cost(pb, j + 1, hp, px, py));
+                                } else {
+                                    a += vw[j] * (ca[i][j] + 0.1f * 5.0f);
+                                }
+                            } else {
+                                a += vw[j] * ca[i][j];
+                            }
+                        }
+                    }
+                    fc[i] = a / wn;
+                }
+                min_idx = find_min_idx8(fc);
+
+                /* re-score the current hypothesis on the selected views
+                 * (HPM.cu:1637-1650) */
+                job = kJobRescore;
+                jplane = cur;
+                todo = sel_views;
+                fresh = true;
+                continue;
             }
-            tc /= wn;
-            const float db = depth_from_plane(rc, t, px, py);
-            if (prior_pix) {
-                const float pr = prior_term(pk, di - depth_prior, prior_plane, t);
-                const float rtc = hpm_expf(-tc * tc / pk.beta) * pr;
-                if (db >= P.depth_min && db <= P.depth_max && rtc > restricted_cost) {
-                    depth_now = db;
-                    plane_center = t;
-                    cost_center = tc;
-                    restricted_cost = rtc;
+            if (job == kJobRescore) {
+                cost_now = selected_sum(vw, jc, nsrc) / wn;
+                cost_center = cost_now;
+                plane_center = cur;
+                sel_center = pb.sel[center];
+                depth_now = depth_from_plane(rc, cur, px, py);
+
+                /* adoption (HPM.cu:1652-1722) */
+                if (prior_on) {
+                    prior_plane = ld4(pb.prior_planes, center);
+                    depth_prior = depth_from_plane(rc, prior_plane, px, py);
+                    if (prior_pix) {
+                        float rfc[8];
+#pragma unroll
+                        for (int i = 0; i < 8; i++) {
+                            rfc[i] = 0.0f;
+                            if (flag[i]) {
+                                const float4 hp = ld4(pb.hyp, pos[i]);
+                                const float dn = depth_from_plane(rc, hp, px, py);
+                                const float pr = prior_term(pk, dn - depth_prior, prior_plane, hp);
+                                rfc[i] = hpm_expf(-fc[i] * fc[i] / pk.beta) * pr;
+                            }
+                        }
+                        const int max_idx = find_max_idx8(rfc);
+                        const float dn = depth_from_plane(rc, cur, px, py);
+                        const float pr = prior_term(pk, dn - depth_prior, prior_plane, cur);
+                        const float rcn = hpm_expf(-cost_now * cost_now / pk.beta) * pr;
+                        if (flag[max_idx]) {
+                            const float4 hp = ld4(pb.hyp, pos[max_idx]);
+                            const float db = depth_from_plane(rc, hp, px, py);
+                            if (db >= P.depth_min && db <= P.depth_max && rfc[max_idx] > rcn) {
+                                depth_now = db;
+                                plane_center = hp;
+                                cost_center = fc[max_idx];
+                                restricted_cost = rfc[max_idx];
+                                sel_center = tsel;
+                            }
+                        }
+                    } else if (flag[min_idx]) {
+                        const float4 hp = ld4(pb.hyp, pos[min_idx]);
+                        const float db = depth_from_plane(rc, hp, px, py);
+                        if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
+                            depth_now = db;
+                            plane_center = hp;
+                            cost_center = fc[min_idx];
+                        }
+                    }
                 }
+                if (!prior_on && flag[min_idx]) {
+                    const float4 hp = ld4(pb.hyp, pos[min_idx]);
+                    const float db = depth_from_plane(rc, hp, px, py);
+                    if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
+                        depth_now = db;
+                        plane_center = hp;
+                        cost_center = fc[min_idx];
+                        sel_center = tsel;
+                    }
+                    if (cost_center != cost_center) {
+                        depth_now = db;
+                        plane_center = hp;
+                        cost_now = fc[min_idx];
+                        sel_center = tsel;
+                    }
+                }
+
+                /* PlaneHypothesisRefinement candidates (HPM.cu:1143-1190) */
+                {
+                    const float perturbation = 0.02f;
+                    float depth_rand;
+                    float4 n_rand;
+                    const float ds = (P.depth_max - P.depth_min) / 64.0f;
+                    const float as = (float)(kPi * (double)(5.0f / 180.0f));
+                    if (prior_pix) {
+                        depth_rand = hpm_rng_uniform(&rng) * 6 * ds + (depth_prior - 3 * ds);
+                        n_rand = perturbed_normal(rc, px, py, prior_plane, &rng, as);
+                    } else {
+                        depth_rand = hpm_rng_uniform(&rng) * (P.depth_max - P.depth_min) + P.depth_min;
+                        n_rand = random_normal(rc, px, py, &rng, depth_now);
+                    }
+                    const float dmin_p = (1 - perturbation) * depth_now;
+                    const float dmax_p = (1 + perturbation) * depth_now;
+                    const float depth_pert = hpm_rng_uniform(&rng) * (dmax_p - dmin_p) + dmin_p;
+                    const float4 plane_now = plane_center;
+                    const float4 n_pert =
+                        perturbed_normal(rc, px, py, plane_now, &rng, (float)((double)perturbation * kPi));
+                    const float dnow = depth_now;
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) {
+                        const float di = (i == 0 || i == 2) ? depth_rand : (i == 4 ? depth_pert : dnow);
+                        float4 t = (i == 0 || i == 4) ? plane_now : (i == 3 ? n_pert : n_rand);
+                        t.w = dist_to_origin(rc, px, py, di, t);
+                        cand[i] = t;
+                        cand_d[i] = di;
+                    }
+                }
+                job = kJobRefine;
+                jplane = cand[0];
+                todo = sel_views;
+                fresh = true;
+                continue;
+            }
+            /* refinement candidate job - kJobRefine scored (HPM.cu:1191-1218) */
+            {
+                const int i = job - kJobRefine;
+                const float4 t = jplane;
+                const float tc = selected_sum(vw, jc, nsrc) / wn;
+                const float db = depth_from_plane(rc, t, px, py);
+                if (prior_pix) {
+                    const float pr = prior_term(pk, cand_d[i] - depth_prior, prior_plane, t);
+                    const float rtc = hpm_expf(-tc * tc / pk.beta) * pr;
+                    if (db >= P.depth_min && db <= P.depth_max && rtc > restricted_cost) {
+                        depth_now = db;
+                        plane_center = t;
+                        cost_center = tc;
+                        restricted_cost = rtc;
+                    }
+                } else {
+                    if (db >= P.depth_min && db <= P.depth_max && tc < cost_center) {
+                        depth_now = db;
+                        plane_center = t;
+                        cost_center = tc;
+                    }
+                }
+                if (i < 4) {
+                    ++job;
+                    jplane = cand[i + 1];
+                    todo = sel_views;
+                    fresh = true;
+                    } else {
+                    job = kJobDone;
+                }
+            }
+        }
+        if (job == kJobDone) {
+            break;
+        }
+        if (fresh) { /* first view of a job: q of its plane */
+            plane_q(pb, jplane, q);
+            fresh = false;
+        }
+        /* the wave scores one source view per trip: the one most of its
+         * lanes still need, so all gathers of a trip hit one image */
+        int vi = 0, best = 0;
+        for (int i = 0; i < nsrc; ++i) {
+            const int n = __popcll(__ballot((todo >> i) & 1u));
+            if (n > best) {
+                best = n;
+                vi = i;
+            }
+        }
+        if ((todo >> vi) & 1u) {
+            todo &= ~(1u << vi);
+            const float c = ncc(pb, rp, px, py, vi + 1, q);
+            ++nev;
+            if (job == kJobExt) {
+                ca[0][vi] = c;
+            } else if (P.geom_consistency) {
+                jc[vi] = c + 0.1f * geom_cost(pb, vi + 1, jplane, px, py);
             } else {
-                if (db >= P.depth_min && db <= P.depth_max && tc < cost_center) {
-                    depth_now = db;
-                    plane_center = t;
-                    cost_center = tc;
-                }
+                jc[vi] = c;
             }
         }
     }
@@ -1064,23 +1304,47 @@ __device__ void filter_pixel(const hpm_dev_problem& pb, int px, int py)
 
 /* pixel of this thread in a colour launch: rows 2*yy + parity, black when
  * x+y is even (HPM.cu:1727-1749) */
+/* Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+ * L2. Renumbering them so that XCD k runs the k-th contiguous band of the
+ * grid keeps the source-image rows a band samples in one L2. The mapping
+ * is a bijection of the grid, whatever the dispatch order. */
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ void xcd_block(int* bx, int* by)
+{
+    const int gx = (int)gridDim.x;
+    const int n = gx * (int)gridDim.y;
+    const int id = (int)blockIdx.x + (int)blockIdx.y * gx;
+    const int xcd = id % kXcds, i = id / kXcds;
+    const int base = n / kXcds, rem = n % kXcds;
+    const int l = xcd * base + min(xcd, rem) + i;
+    *bx = l % gx;
+    *by = l / gx;
+}
+
 __device__ __forceinline__ bool colour_pixel(const hpm_dev_problem& pb, int colour, int* px, int* py)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int yy = blockIdx.y * blockDim.y + threadIdx.y;
-    const int odd = x & 1;
-    const int y = 2 * yy + (colour == 0 ? odd : 1 - odd);
+    /* pixel (x, y) has colour (x + y) & 1; the launch of HPM.cu:1924-1931
+     * visits rows y = 2 yy + parity with yy < yy_limit. Here a wave takes
+     * every second pixel of one row, so its lanes sample neighbouring
+     * source columns. */
+    int bx, by;
+    xcd_block(&bx, &by);
+    const int y = by * blockDim.y + threadIdx.y;
+    const int x = 2 * (bx * blockDim.x + threadIdx.x) + ((y + colour) & 1);
     *px = x;
     *py = y;
-    return yy < pb.yy_limit && x < pb.W && y < pb.H;
+    return (y >> 1) < pb.yy_limit && x < pb.W && y < pb.H;
 }
 
 } // namespace
 
-__global__ __launch_bounds__(256) void hpm_k_init(hpm_dev_problem pb)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_init(hpm_dev_problem pb)
 {
-    const int px = blockIdx.x * blockDim.x + threadIdx.x;
-    const int py = blockIdx.y * blockDim.y + threadIdx.y;
+    int bx, by;
+    xcd_block(&bx, &by);
+    const int px = bx * blockDim.x + threadIdx.x;
+    const int py = by * blockDim.y + threadIdx.y;
     uint32_t nev = 0;
     if (px < pb.W && py < pb.H) {
         nev = init_pixel(pb, px, py);
@@ -1088,7 +1352,7 @@ __global__ __launch_bounds__(256) void hpm_k_init(hpm_dev_problem pb)
     count_evals(pb, nev);
 }
 
-__global__ __launch_bounds__(256) void hpm_k_propagate(hpm_dev_problem pb, int iter, int colour)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_propagate(hpm_dev_problem pb, int iter, int colour)
 {
     int px, py;
     uint32_t nev = 0;
@@ -1106,7 +1370,7 @@ __global__ __launch_bounds__(256) void hpm_k_finalize(hpm_dev_problem pb)
         return;
     }
     /* GetDepthandNormal (HPM.cu:1751-1764) */
-    const hpm_camera& rc = pb.views[0].cam;
+    const hpm_camera rc = camera(pb, 0);
     const int center = py * pb.W + px;
     float4 h = ld4(pb.hyp, center);
     h.w = depth_from_plane(rc, h, px, py);
@@ -1126,7 +1390,7 @@ __global__ __launch_bounds__(256) void hpm_k_filter(hpm_dev_problem pb, int colo
     }
 }
 
-__global__ __launch_bounds__(256) void hpm_k_cost_vectors(hpm_dev_problem pb, const float* planes,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_cost_vectors(hpm_dev_problem pb, const float* planes,
                                                           const int32_t* pix, int npix, float* out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1136,7 +1400,7 @@ __global__ __launch_bounds__(256) void hpm_k_cost_vectors(hpm_dev_problem pb, co
     const int px = pix[2 * i], py = pix[2 * i + 1];
     const float4 pl = ld4(planes, i);
     RefPatch rp;
-    make_ref_patch(pb.images + pb.views[0].img_off, pb.W, pb.H, px, py, pb.params, &rp);
+    make_ref_patch(pb.images + cview(pb, 0)->img_off, pb.W, pb.H, px, py, pb.params, &rp);
     double q[3];
     plane_q(pb, pl, q);
     for (int v = 1; v <= pb.nsrc; ++v) {
@@ -1290,7 +1554,9 @@ static dim3 full_grid(int W, int H, dim3 blk)
 
 static dim3 colour_grid(const hpm_dev_problem* pb, dim3 blk)
 {
-    return dim3((pb->W + blk.x - 1) / blk.x, (pb->yy_limit + blk.y - 1) / blk.y, 1);
+    const int half_w = (pb->W + 1) / 2;
+    const int rows = pb->H < 2 * pb->yy_limit ? pb->H : 2 * pb->yy_limit;
+    return dim3((half_w + blk.x - 1) / blk.x, (rows + blk.y - 1) / blk.y, 1);
 }
 
 extern "C" int hpm_launch_init(const hpm_dev_problem* pb, void* stream)

@@ -186,6 +186,7 @@ void fill_problem(const hpm_handle* h, hpm_dev_problem* pb)
     pb->salt = (uint32_t)h->ref_id;
     iter_consts(pb);
     pb->images = h->pool->d_images;
+    pb->images_pad = h->pool->d_images_pad;
     pb->depths = h->params.multi_geometry ? h->pool->d_depth_geom : h->pool->d_depth_plain;
     pb->views = h->d_views;
     pb->hyp = h->d_hyp;
@@ -292,18 +293,44 @@ extern "C" hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera
         p->images.push_back(std::move(img));
         p->off.push_back(p->total);
         p->total += (int64_t)p->cams.back().width * p->cams.back().height;
+        p->pad_off.push_back(p->pad_total);
+        p->pad_total += (int64_t)(p->cams.back().width + 3) * (p->cams.back().height + 3);
     }
     p->have_state.assign((size_t)num_views, 0);
+    if (p->pad_total >= ((int64_t)1 << 30)) {
+        /* the kernels address the padded pool with 32-bit byte offsets */
+        delete p;
+        return nullptr;
+    }
     bool ok = hipSetDevice(p->device) == hipSuccess &&
               hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc((void**)&p->d_images, (size_t)p->total * 4) == hipSuccess &&
+              hipMalloc((void**)&p->d_images_pad, (size_t)p->pad_total * 4) == hipSuccess &&
               hipMalloc((void**)&p->d_depth_plain, (size_t)p->total * 4) == hipSuccess &&
               hipMalloc((void**)&p->d_depth_geom, (size_t)p->total * 4) == hipSuccess &&
               hipMalloc((void**)&p->d_state_hyp, (size_t)p->total * 16) == hipSuccess &&
               hipMalloc((void**)&p->d_state_cost, (size_t)p->total * 4) == hipSuccess;
+    std::vector<float> padded;
     for (int i = 0; ok && i < num_views; ++i) {
         ok = hipMemcpyAsync(p->d_images + p->off[i], p->images[i].data(), p->images[i].size() * 4,
                             hipMemcpyHostToDevice, p->stream) == hipSuccess;
+        const int W = p->cams[i].width, H = p->cams[i].height;
+        const float* img = p->images[i].data();
+        auto at = [&](int x, int y) {
+            return img[(size_t)std::min(std::max(y, 0), H - 1) * W + std::min(std::max(x, 0), W - 1)];
+        };
+        /* padded copy: texel (x, y), x in [-1, W + 1], y in [-1, H + 1], at
+         * (y + 1) * (W + 3) + x + 1, borders repeating the edge texels */
+        const int pitch = W + 3;
+        padded.assign((size_t)pitch * (H + 3), 0.0f);
+        for (int y = -1; y <= H + 1; ++y) {
+            float* dst = padded.data() + (size_t)(y + 1) * pitch;
+            for (int x = -1; x <= W + 1; ++x) {
+                dst[x + 1] = at(x, y);
+            }
+        }
+        ok = ok && hipMemcpy(p->d_images_pad + p->pad_off[i], padded.data(), padded.size() * 4,
+                             hipMemcpyHostToDevice) == hipSuccess;
     }
     ok = ok && hipMemsetAsync(p->d_depth_plain, 0, (size_t)p->total * 4, p->stream) == hipSuccess &&
          hipMemsetAsync(p->d_depth_geom, 0, (size_t)p->total * 4, p->stream) == hipSuccess &&
@@ -325,6 +352,7 @@ extern "C" void hpm_pool_destroy(hpm_pool* p)
         (void)hipStreamSynchronize(p->stream);
     }
     dfree(p->d_images);
+    dfree(p->d_images_pad);
     dfree(p->d_depth_plain);
     dfree(p->d_depth_geom);
     dfree(p->d_state_hyp);
@@ -612,6 +640,8 @@ extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
         }
         views[i].img_off = h->pool->off[(size_t)h->views[i]];
         views[i].depth_off = views[i].img_off;
+        views[i].pad_off = h->pool->pad_off[(size_t)h->views[i]];
+        views[i].pitch = views[i].cam.width + 3;
     }
     if (h->alloc_pixels != np) {
         release_problem(h);

@@ -15,7 +15,9 @@
  *     (pixel, launch);
  *   - the homography and the warped window centre in FP64 per (hypothesis,
  *     source view), the 36 tap offsets from the centre in FP32 with one
- *     reciprocal per tap;
+ *     reciprocal 1 / (1 + u) per tap: the cubic 1 - u + u^2 - u^3 when the
+ *     window bounds |u| by 0.01 (truncation below 1.02e-8 relative, under
+ *     half an FP32 ulp), the correctly rounded quotient otherwise;
  *   - NCC window sums of both signals accumulated relative to the reference
  *     centre value: NCC is invariant to a shift of either signal, and the
  *     shifted sums keep FP32 accurate on low-texture patches.
@@ -170,11 +172,14 @@ struct Warp {
     int ix0, iy0;
     float fx0, fy0;
     float ex, ey, gxx, gxy, gyx, gyy;
+    bool small; /* |e . d| <= 0.01 over the window: cubic reciprocal */
 };
 
 /* Returns false when the warped centre falls outside the source image (the
- * reference returns the maximum cost there, HPM.cu:333-335). */
-static inline bool make_warp(const ViewRel& v, const double q[3], int px, int py, int SW, int SH, Warp* w)
+ * reference returns the maximum cost there, HPM.cu:333-335). radius is the
+ * largest tap offset of the window. */
+static inline bool make_warp(const ViewRel& v, const double q[3], int px, int py, int SW, int SH, float radius,
+                             Warp* w)
 {
     double H[9];
     for (int r = 0; r < 3; ++r) {
@@ -203,7 +208,16 @@ static inline bool make_warp(const ViewRel& v, const double q[3], int px, int py
     w->gxy = (float)((H[1] - sx0 * H[7]) * icz);
     w->gyx = (float)((H[3] - sy0 * H[6]) * icz);
     w->gyy = (float)((H[4] - sy0 * H[7]) * icz);
+    w->small = (fabsf(w->ex) + fabsf(w->ey)) * radius <= 0.01f;
     return true;
+}
+
+/* 1 / (1 + u) of a tap: the cubic 1 - u + u^2 - u^3 in Horner form when the
+ * warp bounds |u| by 0.01 (truncation u^4 / (1 + u) <= 1.02e-8 relative),
+ * else the correctly rounded quotient. */
+static inline float tap_inv(float u, bool small)
+{
+    return small ? fmaf(u, fmaf(u, 1.0f - u, -1.0f), 1.0f) : 1.0f / (1.0f + u);
 }
 
 /* tex2D with cudaFilterModeLinear and clamp addressing (HPM.cpp:599-617) at
@@ -211,8 +225,8 @@ static inline bool make_warp(const ViewRel& v, const double q[3], int px, int py
  * reference samples at +0.5). */
 static inline float sample_tap(const float* img, int W, int H, const Warp& w, float dx, float dy)
 {
-    const float den = fmaf(w.ex, dx, fmaf(w.ey, dy, 1.0f));
-    const float inv = 1.0f / den;
+    const float u = fmaf(w.ex, dx, w.ey * dy);
+    const float inv = tap_inv(u, w.small);
     float lx = fmaf(fmaf(w.gxx, dx, w.gxy * dy), inv, w.fx0);
     float ly = fmaf(fmaf(w.gyx, dx, w.gyy * dy), inv, w.fy0);
     lx = fminf(fmaxf(lx, -1.0f - (float)w.ix0), (float)(W - w.ix0));
@@ -313,7 +327,7 @@ static float ncc(const Ctx& cx, const RefPatch& rp, int px, int py, int v, const
     const int SW = sc.width, SH = sc.height;
     const float* simg = pb->images[v];
     Warp w;
-    if (!make_warp(cx.rel[v], q, px, py, SW, SH, &w)) {
+    if (!make_warp(cx.rel[v], q, px, py, SW, SH, (float)(pb->params.patch_size / 2), &w)) {
         return 2.0f;
     }
     float ss_ = 0.0f, sss = 0.0f, srs = 0.0f;

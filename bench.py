@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--workload", choices=["full", "plain"], default="full")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workers", type=int, default=0, help="host workers of the schedule (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by reference "
@@ -110,6 +111,8 @@ def main():
     if args.workload == "plain":
         opt.planar_prior = 0
         opt.geom_consistency = 0
+    if args.workers > 0:
+        opt.workers = args.workers
 
     for s in range(args.warmup):
         st = hpm.run_pipeline(pool, problems, opt)

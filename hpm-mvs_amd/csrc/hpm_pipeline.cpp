@@ -126,27 +126,41 @@ void build_prior_scale(Ctx& c, const hpm_camera& ref, const float* depth, const 
 {
     ps->w = w;
     ps->h = h;
+    auto t0 = clk::now();
     std::vector<int32_t> pts((size_t)(w / 5 + 1) * (h / 5 + 1) * 2);
     const int npts = hpm_support_points(cost, w, h, pts.data(), (int)(pts.size() / 2));
     pts.resize((size_t)npts * 2);
     std::vector<int32_t> tris;
     if (npts >= 3) {
-        const int nt = hpm_delaunay(pts.data(), npts, nullptr, 0);
+        /* a triangulation of n points has fewer than 2n triangles */
+        tris.resize((size_t)npts * 2 * 3);
+        const int nt = hpm_delaunay(pts.data(), npts, tris.data(), npts * 2);
         tris.resize((size_t)std::max(nt, 0) * 3);
-        hpm_delaunay(pts.data(), npts, tris.data(), nt);
     }
     const int ntris = (int)(tris.size() / 3);
+    auto t1 = clk::now();
     ps->mask.assign((size_t)w * h, 0.0f);
     hpm_rasterize_triangles(pts.data(), tris.data(), ntris, ps->mask.data(), w, h);
+    auto t2 = clk::now();
     ps->tri_planes.assign((size_t)ntris * 4, 0.0f);
     for (int t = 0; t < ntris; ++t) {
         const int32_t p[6] = {pts[2 * tris[3 * t]], pts[2 * tris[3 * t] + 1], pts[2 * tris[3 * t + 1]],
                               pts[2 * tris[3 * t + 1] + 1], pts[2 * tris[3 * t + 2]], pts[2 * tris[3 * t + 2] + 1]};
         hpm_plane_from_points(&ref, p, depth, w, factor, &ps->tri_planes[4 * (size_t)t]);
     }
+    auto t3 = clk::now();
     ps->knn_planes.assign((size_t)w * h * 4, 0.0f);
     hpm_knn_supplement(&ref, pts.data(), npts, depth, w, h, factor, c.opt->params.knn_k, dmin, dmax, ps->mask.data(),
                        ps->knn_planes.data(), c.threads);
+    if (c.opt->verbose >= 2) {
+        auto ms = [](clk::time_point a, clk::time_point b) {
+            return std::chrono::duration<double, std::milli>(b - a).count();
+        };
+        fprintf(stderr,
+                "prior %dx%d: %d points %d triangles | support+delaunay %.1f ms, raster %.1f ms, planes %.1f ms, "
+                "knn %.1f ms (%d threads)\n",
+                w, h, npts, ntris, ms(t0, t1), ms(t1, t2), ms(t2, t3), ms(t3, clk::now()), c.threads);
+    }
 }
 
 /* ProcessProblem (main.cpp:35-499) for phase A. */
@@ -214,6 +228,9 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
                     }
                 }
                 add_ms(c, &c.st->ms_prior_host, ms_since(t0));
+                if (c.opt->verbose >= 2) {
+                    fprintf(stderr, "view %d prior %.2fx: host %.1f ms\n", ref, f, ms_since(t0));
+                }
                 auto t1 = clk::now();
                 rc = check(c, h, hpm_jbu_prior(h, src_nd.data(), ws, hs, f), "JBU prior");
                 if (rc < 0) return rc;
@@ -257,6 +274,9 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
                     }
                 }
                 add_ms(c, &c.st->ms_prior_host, ms_since(t0));
+                if (c.opt->verbose >= 2) {
+                    fprintf(stderr, "view %d prior 1x: host %.1f ms\n", ref, ms_since(t0));
+                }
                 auto t1 = clk::now();
                 rc = check(c, h, hpm_upload_prior(h, planes.data(), mask.data()), "prior upload");
                 if (rc < 0) return rc;
@@ -304,7 +324,7 @@ extern "C" void hpm_pipeline_default(hpm_pipeline_options* opt)
     opt->geom_consistency = 1;
     opt->num_threads = 0;
     opt->scale_bound = 1000;
-    opt->workers = 4;
+    opt->workers = 12;
 }
 
 extern "C" int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, float* cost)
@@ -349,13 +369,21 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
     } else if (!(h = hpm_create(pool->device))) {
         rc = fail(c, "cannot create a problem handle");
     }
-    if (rc == HPM_OK) {
-        /* phase A: views are independent (main.cpp:658-661); host workers,
-         * each with its own handle and stream, overlap one view's host prior
-         * construction with the device work of the others */
-        auto t0 = clk::now();
-        const int workers = std::max(1, std::min(c.opt->workers, num_problems));
-        c.threads = std::max(1, c.threads / workers);
+    /* host workers, each with its own handle and stream: views that do not
+     * depend on each other run concurrently, overlapping one view's host
+     * prior construction with the device work of the others */
+    const int workers = std::max(1, std::min(c.opt->workers, num_problems));
+    std::vector<hpm_handle*> handles((size_t)workers, nullptr);
+    handles[0] = h;
+    for (int w = 1; rc == HPM_OK && w < workers; ++w) {
+        handles[(size_t)w] = hpm_create(pool->device);
+        if (!handles[(size_t)w]) {
+            rc = fail(c, "cannot create a problem handle");
+        }
+    }
+    const int all_threads = c.threads;
+    /* runs fn(handle, problem) for every problem on the workers */
+    auto parallel = [&](auto&& fn) {
         std::atomic<int> next(0);
         std::atomic<int> worst(HPM_OK);
         auto work = [&](hpm_handle* wh) {
@@ -364,51 +392,63 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
                 if (i >= num_problems || worst.load() < 0) {
                     return;
                 }
-                if (c.opt->verbose) {
-                    fprintf(stderr, "phase A: view %d\n", ref_ids[i]);
-                }
-                const int r = process_phase_a(c, wh, ref_ids[i], src_ids + (size_t)i * max_src, num_src[i]);
+                const int r = fn(wh, i);
                 if (r < 0) {
                     worst.store(r);
                 }
             }
         };
-        std::vector<hpm_handle*> handles((size_t)workers, nullptr);
-        handles[0] = h;
+        c.threads = std::max(1, all_threads / workers);
+        std::vector<std::thread> pool_threads;
         for (int w = 1; w < workers; ++w) {
-            handles[(size_t)w] = hpm_create(pool->device);
-            if (!handles[(size_t)w]) {
-                rc = fail(c, "cannot create a problem handle");
-            }
+            pool_threads.emplace_back(work, handles[(size_t)w]);
         }
-        if (rc == HPM_OK) {
-            std::vector<std::thread> pool_threads;
-            for (int w = 1; w < workers; ++w) {
-                pool_threads.emplace_back(work, handles[(size_t)w]);
-            }
-            work(h);
-            for (auto& t : pool_threads) {
-                t.join();
-            }
-            rc = worst.load();
+        work(h);
+        for (auto& t : pool_threads) {
+            t.join();
         }
-        for (int w = 1; w < workers; ++w) {
-            hpm_destroy(handles[(size_t)w]);
-        }
-        c.threads = c.threads * workers;
+        c.threads = all_threads;
+        return worst.load();
+    };
+    if (rc == HPM_OK) {
+        /* phase A: views are independent (main.cpp:658-661) */
+        auto t0 = clk::now();
+        rc = parallel([&](hpm_handle* wh, int i) {
+            if (c.opt->verbose) {
+                fprintf(stderr, "phase A: view %d\n", ref_ids[i]);
+            }
+            return process_phase_a(c, wh, ref_ids[i], src_ids + (size_t)i * max_src, num_src[i]);
+        });
         st->ms_phase_a = ms_since(t0);
     }
     if (rc >= 0 && c.opt->geom_consistency) {
         auto t0 = clk::now();
         for (int g = 0; g < c.opt->params.geom_iterations && rc >= 0; ++g) {
-            for (int i = 0; i < num_problems && rc >= 0; ++i) {
+            auto pass = [&](hpm_handle* wh, int i) {
                 if (c.opt->verbose) {
                     fprintf(stderr, "geometric pass %d: view %d\n", g, ref_ids[i]);
                 }
-                rc = process_phase_b(c, h, ref_ids[i], src_ids + (size_t)i * max_src, num_src[i], g);
+                return process_phase_b(c, wh, ref_ids[i], src_ids + (size_t)i * max_src, num_src[i], g);
+            };
+            if (g == 0) {
+                /* the first pass reads only the phase-A depth maps and
+                 * writes each view's own state: views are independent */
+                rc = parallel(pass);
+            } else {
+                /* later passes read the geometric depth maps of views
+                 * already updated in the same pass (main.cpp:662-674):
+                 * problem order */
+                for (int i = 0; i < num_problems && rc >= 0; ++i) {
+                    rc = pass(h, i);
+                }
             }
         }
         st->ms_phase_b = ms_since(t0);
+    }
+    for (int w = 1; w < workers; ++w) {
+        if (handles[(size_t)w]) {
+            hpm_destroy(handles[(size_t)w]);
+        }
     }
     if (h) {
         hpm_destroy(h);

@@ -25,7 +25,21 @@ inline int64_t orient(const Pt& a, const Pt& b, const Pt& c)
 }
 
 /* > 0 when d lies strictly inside the circumcircle of the positively
- * oriented triangle (a, b, c) */
+ * oriented triangle (a, b, c); exact in 64 bits while every coordinate
+ * difference is below 2^14 (12 D^4 < 2^63) */
+inline int incircle64(const Pt& a, const Pt& b, const Pt& c, const Pt& d)
+{
+    const int64_t adx = a.x - d.x, ady = a.y - d.y;
+    const int64_t bdx = b.x - d.x, bdy = b.y - d.y;
+    const int64_t cdx = c.x - d.x, cdy = c.y - d.y;
+    const int64_t ad = adx * adx + ady * ady;
+    const int64_t bd = bdx * bdx + bdy * bdy;
+    const int64_t cd = cdx * cdx + cdy * cdy;
+    const int64_t det = adx * (bdy * cd - bd * cdy) - ady * (bdx * cd - bd * cdx) + ad * (bdx * cdy - bdy * cdx);
+    return det > 0 ? 1 : (det < 0 ? -1 : 0);
+}
+
+/* the same with 128-bit products, for triangles of the enclosing triangle */
 inline int incircle(const Pt& a, const Pt& b, const Pt& c, const Pt& d)
 {
     const i128 adx = a.x - d.x, ady = a.y - d.y;
@@ -56,9 +70,12 @@ public:
         }
         const int64_t S = 64 * (maxc + 16);
         n_real = (int)P.size();
+        small_coords = 2 * maxc < ((int64_t)1 << 14);
         P.push_back({-S, -S});
         P.push_back({3 * S, -S});
         P.push_back({-S, 3 * S});
+        T.reserve(2 * (size_t)n_real + 16);
+        marks.assign(2 * (size_t)n_real + 16, 0);
         Tri t;
         t.v[0] = n_real;
         t.v[1] = n_real + 1;
@@ -100,7 +117,11 @@ public:
                 if (n < 0 || marks[n] == mark_gen) {
                     continue;
                 }
-                if (incircle(P[T[n].v[0]], P[T[n].v[1]], P[T[n].v[2]], p) > 0) {
+                const Tri& tn = T[n];
+                const bool real = tn.v[0] < n_real && tn.v[1] < n_real && tn.v[2] < n_real && small_coords;
+                const int in = real ? incircle64(P[tn.v[0]], P[tn.v[1]], P[tn.v[2]], p)
+                                    : incircle(P[tn.v[0]], P[tn.v[1]], P[tn.v[2]], p);
+                if (in > 0) {
                     marks[n] = mark_gen;
                     stack.push_back(n);
                 }
@@ -183,6 +204,7 @@ public:
     /* triangles made only of real points, in slot order */
     void output(std::vector<int32_t>& out) const
     {
+        out.reserve(3 * T.size());
         for (const Tri& t : T) {
             if (!t.alive || t.v[0] >= n_real || t.v[1] >= n_real || t.v[2] >= n_real) {
                 continue;
@@ -245,12 +267,149 @@ private:
     std::vector<Pt> P;
     std::vector<Tri> T;
     int n_real;
+    bool small_coords; /* real points differ by less than 2^14 per axis */
     int last;
     std::vector<int> cavity, stack, free_slots, created;
     std::vector<Edge> boundary;
     std::vector<std::pair<int, int>> first_of;
     std::vector<uint32_t> marks;
     uint32_t mark_gen = 0;
+};
+
+/* position of (x, y) along a Hilbert curve over [0, 2^16)^2 */
+inline uint64_t hilbert_key(uint32_t x, uint32_t y)
+{
+    uint64_t d = 0;
+    for (uint32_t s = 1u << 15; s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u;
+        const uint32_t ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = s - 1 - (x & (s - 1)) + (x & ~(s - 1));
+                y = s - 1 - (y & (s - 1)) + (y & ~(s - 1));
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
+}
+
+/* Static 2-D tree over integer points for exact k-nearest-neighbour
+ * queries ordered by (squared distance, index), the order the KNN
+ * supplement defines (its reference uses a FLANN kd-tree, HPM.cpp:935-947).
+ * Subtrees are pruned only when their box lies strictly farther than the
+ * current k-th candidate, so equal-distance points of lower index are never
+ * missed. */
+class KdTree {
+public:
+    KdTree(const int32_t* pts, int n) : P(pts), idx((size_t)n)
+    {
+        for (int i = 0; i < n; ++i) {
+            idx[i] = i;
+        }
+        nodes.reserve(2 * (size_t)n / kLeaf + 16);
+        if (n > 0) {
+            build(0, n);
+        }
+    }
+
+    /* k nearest of (x, y): squared distances bd and indices bi, ascending */
+    int knn(int x, int y, int k, int64_t* bd, int* bi) const
+    {
+        int found = 0;
+        if (nodes.empty()) {
+            return 0;
+        }
+        int stack[64];
+        int top = 0;
+        stack[top++] = 0;
+        while (top > 0) {
+            const Node& nd = nodes[stack[--top]];
+            if (found == k && box_dist2(nd, x, y) > bd[k - 1]) {
+                continue;
+            }
+            if (nd.left < 0) {
+                for (int q = nd.lo; q < nd.hi; ++q) {
+                    const int s = idx[q];
+                    const int64_t dx = P[2 * s] - x, dy = P[2 * s + 1] - y;
+                    const int64_t d2 = dx * dx + dy * dy;
+                    if (found == k && (d2 > bd[k - 1] || (d2 == bd[k - 1] && s > bi[k - 1]))) {
+                        continue;
+                    }
+                    int pos = found < k ? found : k - 1;
+                    while (pos > 0 && (bd[pos - 1] > d2 || (bd[pos - 1] == d2 && bi[pos - 1] > s))) {
+                        bd[pos] = bd[pos - 1];
+                        bi[pos] = bi[pos - 1];
+                        --pos;
+                    }
+                    bd[pos] = d2;
+                    bi[pos] = s;
+                    if (found < k) {
+                        ++found;
+                    }
+                }
+                continue;
+            }
+            /* nearer child last, so it is popped first */
+            const bool left_first = box_dist2(nodes[nd.left], x, y) <= box_dist2(nodes[nd.right], x, y);
+            stack[top++] = left_first ? nd.right : nd.left;
+            stack[top++] = left_first ? nd.left : nd.right;
+        }
+        return found;
+    }
+
+private:
+    static constexpr int kLeaf = 8;
+    struct Node {
+        int lo, hi;          /* range of idx */
+        int left, right;     /* children, -1 for a leaf */
+        int x0, y0, x1, y1;  /* bounding box */
+    };
+
+    static int64_t box_dist2(const Node& n, int x, int y)
+    {
+        const int64_t dx = x < n.x0 ? n.x0 - x : (x > n.x1 ? x - n.x1 : 0);
+        const int64_t dy = y < n.y0 ? n.y0 - y : (y > n.y1 ? y - n.y1 : 0);
+        return dx * dx + dy * dy;
+    }
+
+    int build(int lo, int hi)
+    {
+        const int id = (int)nodes.size();
+        nodes.push_back(Node());
+        Node nd;
+        nd.lo = lo;
+        nd.hi = hi;
+        nd.left = nd.right = -1;
+        nd.x0 = nd.y0 = INT32_MAX;
+        nd.x1 = nd.y1 = INT32_MIN;
+        for (int q = lo; q < hi; ++q) {
+            const int s = idx[q];
+            nd.x0 = std::min(nd.x0, P[2 * s]);
+            nd.x1 = std::max(nd.x1, P[2 * s]);
+            nd.y0 = std::min(nd.y0, P[2 * s + 1]);
+            nd.y1 = std::max(nd.y1, P[2 * s + 1]);
+        }
+        if (hi - lo > kLeaf) {
+            const int dim = (nd.x1 - nd.x0) >= (nd.y1 - nd.y0) ? 0 : 1;
+            const int mid = lo + (hi - lo) / 2;
+            std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int a, int b) {
+                const int va = P[2 * a + dim], vb = P[2 * b + dim];
+                return va < vb || (va == vb && a < b);
+            });
+            nd.left = build(lo, mid);
+            nd.right = build(mid, hi);
+        }
+        nodes[id] = nd;
+        return id;
+    }
+
+    const int32_t* P;
+    std::vector<int> idx;
+    std::vector<Node> nodes;
 };
 
 /* Get3DPointonRefCam_factor (HPM.cpp:236-245) */
@@ -307,8 +466,20 @@ extern "C" int hpm_delaunay(const int32_t* points_xy, int n, int32_t* tris, int 
         pts[i] = {points_xy[2 * i], points_xy[2 * i + 1]};
     }
     Delaunay dt(pts);
+    /* insertion along a Hilbert curve: short walks and small cavities (in
+     * grid order the cavities of near-lattice support points are large) */
+    int32_t minx = (int32_t)pts[0].x, miny = (int32_t)pts[0].y;
+    for (const Pt& p : pts) {
+        minx = (int32_t)std::min<int64_t>(minx, p.x);
+        miny = (int32_t)std::min<int64_t>(miny, p.y);
+    }
+    std::vector<std::pair<uint64_t, int>> order((size_t)n);
     for (int i = 0; i < n; ++i) {
-        dt.insert(i);
+        order[i] = {hilbert_key((uint32_t)(pts[i].x - minx) & 0xffffu, (uint32_t)(pts[i].y - miny) & 0xffffu), i};
+    }
+    std::sort(order.begin(), order.end());
+    for (const auto& o : order) {
+        dt.insert(o.second);
     }
     std::vector<int32_t> out;
     dt.output(out);
@@ -411,17 +582,9 @@ extern "C" int hpm_knn_supplement(const hpm_camera* ref, const int32_t* pts, int
                 order.push_back(b);
                 order.push_back(m);
             }
-    /* grid of 5x5 cells: support points are at most one per block */
-    const int cell = 5;
-    const int gw = (width + cell - 1) / cell, gh = (height + cell - 1) / cell;
-    std::vector<int> head((size_t)gw * gh, -1), next((size_t)npoints, -1);
-    for (int i = npoints - 1; i >= 0; --i) {
-        const int cx = std::min(std::max(pts[2 * i] / cell, 0), gw - 1);
-        const int cy = std::min(std::max(pts[2 * i + 1] / cell, 0), gh - 1);
-        next[i] = head[(size_t)cy * gw + cx];
-        head[(size_t)cy * gw + cx] = i;
-    }
+    const KdTree tree(pts, npoints);
     const int nt = num_threads > 0 ? num_threads : 1;
+    const int kk = std::min(k, npoints);
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
     for (int i = 0; i < height; i++) {
         int64_t bd[32];
@@ -430,52 +593,7 @@ extern "C" int hpm_knn_supplement(const hpm_camera* ref, const int32_t* pts, int
             if (mask[(size_t)i * width + j] > 0) {
                 continue;
             }
-            int found = 0;
-            const int qcx = j / cell, qcy = i / cell;
-            const int kk = std::min(k, npoints);
-            for (int r = 0;; ++r) {
-                if (found == kk) {
-                    const int64_t lb = (int64_t)std::max(0, (r - 1) * cell);
-                    if (lb * lb > bd[found - 1]) {
-                        break;
-                    }
-                }
-                if (r > gw + gh) {
-                    break;
-                }
-                for (int gy = qcy - r; gy <= qcy + r; ++gy) {
-                    if (gy < 0 || gy >= gh) {
-                        continue;
-                    }
-                    const bool edge_row = (gy == qcy - r || gy == qcy + r);
-                    for (int gx = qcx - r; gx <= qcx + r; gx += (edge_row ? 1 : 2 * r)) {
-                        if (gx >= 0 && gx < gw) {
-                            for (int s = head[(size_t)gy * gw + gx]; s >= 0; s = next[s]) {
-                                const int64_t dx = pts[2 * s] - j, dy = pts[2 * s + 1] - i;
-                                const int64_t d2 = dx * dx + dy * dy;
-                                /* insert by (distance, index) */
-                                if (found == kk && (d2 > bd[kk - 1] || (d2 == bd[kk - 1] && s > bi[kk - 1]))) {
-                                    continue;
-                                }
-                                int pos = found < kk ? found : kk - 1;
-                                while (pos > 0 && (bd[pos - 1] > d2 || (bd[pos - 1] == d2 && bi[pos - 1] > s))) {
-                                    bd[pos] = bd[pos - 1];
-                                    bi[pos] = bi[pos - 1];
-                                    --pos;
-                                }
-                                bd[pos] = d2;
-                                bi[pos] = s;
-                                if (found < kk) {
-                                    ++found;
-                                }
-                            }
-                        }
-                        if (r == 0) {
-                            break;
-                        }
-                    }
-                }
-            }
+            const int found = tree.knn(j, i, kk, bd, bi);
             if (found < 3) {
                 continue;
             }

@@ -678,11 +678,11 @@ def support_points(cost: np.ndarray) -> np.ndarray:
 
 def delaunay(points: np.ndarray) -> np.ndarray:
     p = np.ascontiguousarray(points, dtype=np.int32).reshape(-1, 2)
-    n = lib().hpm_delaunay(iptr(p), len(p), None, 0)
+    cap = max(2 * len(p), 1)  # a triangulation of n points has fewer than 2n triangles
+    out = np.empty((cap, 3), np.int32)
+    n = lib().hpm_delaunay(iptr(p), len(p), iptr(out), cap)
     if n < 0:
         raise HpmError("hpm_delaunay")
-    out = np.empty((max(n, 1), 3), np.int32)
-    lib().hpm_delaunay(iptr(p), len(p), iptr(out), n)
     return out[:n]
 
 

@@ -8,7 +8,10 @@
  *   phase B: geom_iterations geometric-consistency passes over all views,
  *     the first reading the phase-A depth maps, later ones the latest
  *     geometric depth maps (views in order, as the reference runs them).
- * Views within a pass run in problem order on one device.
+ * Phase A and the first geometric pass run their views concurrently on
+ * `workers` host threads (views are independent there); later geometric
+ * passes run in problem order, each view reading the geometric depth maps
+ * its predecessors wrote in the same pass.
  *
  * Random streams: RunPatchMatch call c of phase A uses Philox stream
  * c (0..k); geometric pass g uses stream 8 + g, so no two passes of a view
@@ -33,7 +36,7 @@ typedef struct hpm_pipeline_options {
     int32_t num_threads;        /* host threads for the prior construction (<= 0: all) */
     int32_t verbose;
     int32_t scale_bound;        /* size_bound of ComputingMultiScaleSettings (HPM.cpp:852), 1000 */
-    int32_t workers;            /* phase-A host workers, each with its own handle and stream (4) */
+    int32_t workers;            /* host workers of the independent passes, each with its own handle and stream (12) */
 } hpm_pipeline_options;
 
 typedef struct hpm_pipeline_stats {

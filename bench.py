@@ -44,6 +44,23 @@ def load_pkg():
     return hpm
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hpm_k_propagate_1600x1066_10src_traffic.md")
+
+
+def measured_traffic():
+    """HBM (data-fabric) bytes per propagation launch of the plain
+    1600x1066 pass, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    committed under profiles/ (None when absent)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def view_order(ref, n):
     return [ref] + sorted([i for i in range(n) if i != ref], key=lambda i: (abs(i - ref), i))
 
@@ -86,7 +103,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workers", type=int, default=0, help="host workers of the schedule (0: library default)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB on the device 3 times (the known-size kernel that scales the FETCH_SIZE "
+                         "and WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied 3 x 1 GiB")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by reference "
               f"view and no multi-GPU run is part of this build", file=sys.stderr)
@@ -195,7 +225,7 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "avg_launch_ms": round(avg_launch_ms, 3),
             "flops_per_launch": flops_per_launch,
         },

@@ -52,7 +52,10 @@ typedef struct hpm_run_stats {
  * per view the last plain-pass depth map (which 0, the reference's
  * depths.dmb), the last geometric-consistency depth map (which 1,
  * depths_geom.dmb) and the last hypotheses and costs. No reference
- * counterpart: the reference re-reads images and .dmb files per problem. */
+ * counterpart: the reference re-reads images and .dmb files per problem.
+ * Returns NULL on bad arguments, device errors, or when the views together
+ * exceed 2^30 texels (the kernels address the images with 32-bit byte
+ * offsets; split larger scenes into several pools). */
 hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera* cams, const float* const* images,
                           int max_image_size);
 void hpm_pool_destroy(hpm_pool* pool);

@@ -94,12 +94,86 @@ def cpu_baseline(cams, imgs, ref, evals_per_view, host_ms_per_view, threads):
     }
 
 
+def batch_bench(hpm, args):
+    """32 independent references, W ~ U{640..1920}, H ~ U{480..1080} (even),
+    4-10 sources each, rendered from seeded scenes (seed 3 + i); one plain
+    RunPatchMatch per reference. Timed: the batched form (one launch per
+    stage over all references, replayed hipGraph, autotuned grid) against
+    the same references one RunPatchMatch at a time. Prints one JSON line
+    (ms per reference view of the batched form)."""
+    import numpy as np
+
+    rng = np.random.default_rng(3)
+    problems = []
+    t0 = time.perf_counter()
+    pixels = 0
+    for i in range(32):
+        w = int(rng.integers(640, 1921))
+        h = 2 * int(rng.integers(240, 541))
+        nsrc = int(rng.integers(4, 11))
+        f = 0.78 * w
+        cfg = hpm.scene_config(seed=3 + i, width=w, height=h, fx=f, fy=f, cx=w / 2.0, cy=h / 2.0,
+                               num_views=nsrc + 1, num_planes=20, num_boxes=50, low_texture_fraction=0.3)
+        cams, imgs, gd, gn, low = hpm.render_scene(cfg, want_gt=False)
+        ref = (nsrc + 1) // 2
+        order = view_order(ref, nsrc + 1)
+        pr = hpm.Problem(0)
+        pr.input_init_memory([cams[j] for j in order], [imgs[j].astype(np.float32) for j in order], ref_image_id=0)
+        pr.CudaSpaceInitialization()
+        problems.append(pr)
+        pixels += w * h
+    log(f"32 references rendered and uploaded ({pixels / 1e6:.1f} Mpx) in {time.perf_counter() - t0:.1f} s")
+    batch = hpm.Batch(0)
+    for s in range(args.warmup):
+        st = batch.run(problems)
+        log(f"warmup {s}: {st.ms_host:.0f} ms host, grid {st.grid}")
+    t0 = time.perf_counter()
+    dev = 0.0
+    for s in range(args.steps):
+        st = batch.run(problems)
+        dev += st.ms_device
+    ms_batch = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"batched: {ms_batch:.1f} ms per step ({dev / args.steps:.1f} ms device), {st.n_eval} evals")
+    t0 = time.perf_counter()
+    for pr in problems:
+        pr.RunPatchMatch()
+    ms_seq = (time.perf_counter() - t0) * 1e3
+    log(f"one at a time: {ms_seq:.1f} ms")
+    out = {
+        "metric": "ms/ref view, batched plain RunPatchMatch over 32 mixed-size references",
+        "value": round(ms_batch / 32, 3),
+        "unit": "ms/ref view",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_batch, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (32 seeded scenes rendered on the host)",
+        "config": {
+            "workload": "32 references, W in 640..1920, H in 480..1080, 4-10 sources, one plain RunPatchMatch each",
+            "megapixels": round(pixels / 1e6, 2),
+            "ms_device_per_step": round(dev / args.steps, 3),
+            "ms_one_at_a_time": round(ms_seq, 3),
+            "speedup_vs_one_at_a_time": round(ms_seq / ms_batch, 3),
+            "grid": st.grid,
+            "autotune_log": batch.log().strip(),
+        },
+    }
+    batch.close()
+    for pr in problems:
+        pr.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["full", "plain"], default="full")
+    ap.add_argument("--workload", choices=["full", "plain", "batch"], default="full")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workers", type=int, default=0, help="host workers of the schedule (0: library default)")
@@ -125,6 +199,9 @@ def main():
     import numpy as np
 
     hpm = load_pkg()
+    if args.workload == "batch":
+        batch_bench(hpm, args)
+        return
     cfg = hpm.workload_config("c2")
     t0 = time.perf_counter()
     cams, imgs, gd, gn, low = hpm.render_scene(cfg)

@@ -1,0 +1,360 @@
+/*
+ * hpm_batch.cpp — RunPatchMatch of many independent problems as one batch
+ * (no reference counterpart; the reference runs problems one at a time,
+ * HPM.cu:1906-1957).
+ *
+ * All problems advance through the same schedule (init, 3 x red/black
+ * propagation, finalize, red/black filter); each stage is one launch of a
+ * fixed grid of resident blocks that walk a table of 64x4 tiles covering
+ * every problem (hpm_launch_batch). The 11 launches are captured once per
+ * batch shape in a hipGraph and replayed; the problem table (pointers,
+ * sizes, RNG call index) is uploaded before every replay. The grid size is
+ * chosen by timing candidates on the first call of a shape, with the
+ * problems' state saved before and restored after every trial, so tuning
+ * leaves the results untouched. Per pixel the work is that of
+ * hpm_run_patchmatch, so every problem's maps equal an unbatched run's.
+ */
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "hpm_batch.h"
+#include "hpm_internal.h"
+
+struct hpm_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::string err;
+    std::string log;
+    /* device tables */
+    hpm_dev_problem* d_pbs = nullptr; /* [2n]: init counters, then propagation counters */
+    hpm_tile* d_tiles = nullptr;      /* full-grid tiles, then colour-grid tiles */
+    unsigned long long* d_neval = nullptr;
+    float* d_save = nullptr;          /* state saved around tuning trials */
+    size_t cap_pbs = 0, cap_tiles = 0, cap_neval = 0, cap_save = 0;
+    hpm_dev_problem* h_pbs = nullptr; /* pinned */
+    size_t cap_h_pbs = 0;
+    /* captured schedule and the shape it was built for */
+    hipGraphExec_t exec = nullptr;
+    std::vector<int> key;
+    int grid = 0;
+    int n_full = 0, n_colour = 0;
+};
+
+namespace {
+
+int bfail(hpm_batch* b, int code, const std::string& msg)
+{
+    b->err = msg;
+    return code;
+}
+
+#define BHIP(b, expr)                                                                                              \
+    do {                                                                                                           \
+        hipError_t e_ = (expr);                                                                                    \
+        if (e_ != hipSuccess) {                                                                                    \
+            return bfail((b), HPM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+        }                                                                                                          \
+    } while (0)
+
+template <typename T>
+int grow(hpm_batch* b, T** p, size_t* cap, size_t count)
+{
+    if (*cap >= count) {
+        return HPM_OK;
+    }
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    BHIP(b, hipMalloc((void**)p, count * sizeof(T)));
+    *cap = count;
+    return HPM_OK;
+}
+
+/* tiles of the full grid (x in [0, W), y in [0, H)) and of the colour grid
+ * (x = 2i + parity, rows below min(H, 2 yy_limit)) of every problem, 64x4 */
+void make_tiles(const std::vector<hpm_dev_problem>& pbs, int n, std::vector<hpm_tile>* full,
+                std::vector<hpm_tile>* colour)
+{
+    full->clear();
+    colour->clear();
+    for (int i = 0; i < n; ++i) {
+        const hpm_dev_problem& pb = pbs[(size_t)i];
+        for (int by = 0; by < (pb.H + 3) / 4; ++by) {
+            for (int bx = 0; bx < (pb.W + 63) / 64; ++bx) {
+                full->push_back({i, bx, by, 0});
+            }
+        }
+        const int half_w = (pb.W + 1) / 2;
+        const int rows = std::min(pb.H, 2 * pb.yy_limit);
+        for (int by = 0; by < (rows + 3) / 4; ++by) {
+            for (int bx = 0; bx < (half_w + 63) / 64; ++bx) {
+                colour->push_back({i, bx, by, 0});
+            }
+        }
+    }
+}
+
+/* the 11 launches of one schedule */
+int enqueue(hpm_batch* b, int n, int grid)
+{
+    const hpm_tile* tf = b->d_tiles;
+    const hpm_tile* tc = b->d_tiles + b->n_full;
+    const hpm_dev_problem* p_init = b->d_pbs;
+    const hpm_dev_problem* p_prop = b->d_pbs + n;
+    BHIP(b, (hipError_t)hpm_launch_zero_u64(b->d_neval, 2 * n, b->stream));
+    BHIP(b, (hipError_t)hpm_launch_batch(HPM_STAGE_INIT, p_init, tf, b->n_full, grid, 0, 0, b->stream));
+    for (int it = 0; it < 3; ++it) { /* hard-coded 3 (HPM.cu:1933) */
+        for (int colour = 0; colour < 2; ++colour) {
+            BHIP(b, (hipError_t)hpm_launch_batch(HPM_STAGE_PROPAGATE, p_prop, tc, b->n_colour, grid, it, colour,
+                                                 b->stream));
+        }
+    }
+    BHIP(b, (hipError_t)hpm_launch_batch(HPM_STAGE_FINALIZE, p_prop, tf, b->n_full, grid, 0, 0, b->stream));
+    BHIP(b, (hipError_t)hpm_launch_batch(HPM_STAGE_FILTER, p_prop, tc, b->n_colour, grid, 0, 0, b->stream));
+    BHIP(b, (hipError_t)hpm_launch_batch(HPM_STAGE_FILTER, p_prop, tc, b->n_colour, grid, 0, 1, b->stream));
+    return HPM_OK;
+}
+
+/* hypotheses, costs and view masks of every problem <-> d_save */
+int save_state(hpm_batch* b, hpm_handle* const* hs, int n, bool restore)
+{
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {
+        const size_t np = (size_t)hs[i]->cams[0].width * hs[i]->cams[0].height;
+        float* s = b->d_save + off;
+        if (restore) {
+            BHIP(b, hipMemcpyAsync(hs[i]->d_hyp, s, np * 16, hipMemcpyDeviceToDevice, b->stream));
+            BHIP(b, hipMemcpyAsync(hs[i]->d_cost, s + 4 * np, np * 4, hipMemcpyDeviceToDevice, b->stream));
+            BHIP(b, hipMemcpyAsync(hs[i]->d_sel, s + 5 * np, np * 4, hipMemcpyDeviceToDevice, b->stream));
+        } else {
+            BHIP(b, hipMemcpyAsync(s, hs[i]->d_hyp, np * 16, hipMemcpyDeviceToDevice, b->stream));
+            BHIP(b, hipMemcpyAsync(s + 4 * np, hs[i]->d_cost, np * 4, hipMemcpyDeviceToDevice, b->stream));
+            BHIP(b, hipMemcpyAsync(s + 5 * np, hs[i]->d_sel, np * 4, hipMemcpyDeviceToDevice, b->stream));
+        }
+        off += 6 * np;
+    }
+    return HPM_OK;
+}
+
+} // namespace
+
+extern "C" hpm_batch* hpm_batch_create(int device)
+{
+    hpm_batch* b = new hpm_batch();
+    b->device = device < 0 ? 0 : device;
+    if (hipSetDevice(b->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess) {
+        hpm_batch_destroy(b);
+        return nullptr;
+    }
+    return b;
+}
+
+extern "C" void hpm_batch_destroy(hpm_batch* b)
+{
+    if (!b) {
+        return;
+    }
+    (void)hipSetDevice(b->device);
+    if (b->stream) {
+        (void)hipStreamSynchronize(b->stream);
+    }
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+    }
+    if (b->d_pbs) (void)hipFree(b->d_pbs);
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->d_neval) (void)hipFree(b->d_neval);
+    if (b->d_save) (void)hipFree(b->d_save);
+    if (b->h_pbs) (void)hipHostFree(b->h_pbs);
+    for (hipEvent_t e : b->ev) {
+        if (e) (void)hipEventDestroy(e);
+    }
+    if (b->stream) {
+        (void)hipStreamDestroy(b->stream);
+    }
+    delete b;
+}
+
+extern "C" const char* hpm_batch_last_error(const hpm_batch* b)
+{
+    return b ? b->err.c_str() : "null batch";
+}
+
+extern "C" const char* hpm_batch_log(const hpm_batch* b)
+{
+    return b ? b->log.c_str() : "";
+}
+
+extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int want_readback, hpm_batch_stats* stats)
+{
+    if (!b || !hs || n < 1) {
+        return HPM_EINVAL;
+    }
+    auto t_host = std::chrono::steady_clock::now();
+    BHIP(b, hipSetDevice(b->device));
+    std::vector<hpm_dev_problem> pbs((size_t)n);
+    std::vector<int> key;
+    key.push_back(n);
+    size_t save_floats = 0;
+    for (int i = 0; i < n; ++i) {
+        hpm_handle* h = hs[i];
+        if (!h) {
+            return bfail(b, HPM_EINVAL, "null handle in the batch");
+        }
+        if (h->device != b->device) {
+            return bfail(b, HPM_EINVAL, "handles of a batch must be on the batch's device");
+        }
+        const int rc = hpm_ensure_ready(h);
+        if (rc != HPM_OK) {
+            return bfail(b, rc, "problem " + std::to_string(i) + ": " + h->err);
+        }
+        hpm_fill_problem(h, &pbs[(size_t)i]);
+        key.push_back(pbs[(size_t)i].W);
+        key.push_back(pbs[(size_t)i].H);
+        save_floats += (size_t)pbs[(size_t)i].W * pbs[(size_t)i].H * 6;
+    }
+    /* problem tables: init counts into 2i, propagation into 2i + 1 */
+    int rc = grow(b, &b->d_neval, &b->cap_neval, 2 * (size_t)n);
+    if (rc == HPM_OK) rc = grow(b, &b->d_pbs, &b->cap_pbs, 2 * (size_t)n);
+    if (rc != HPM_OK) return rc;
+    if (b->cap_h_pbs < 2 * (size_t)n) {
+        if (b->h_pbs) (void)hipHostFree(b->h_pbs);
+        b->h_pbs = nullptr;
+        BHIP(b, hipHostMalloc((void**)&b->h_pbs, 2 * (size_t)n * sizeof(hpm_dev_problem), hipHostMallocDefault));
+        b->cap_h_pbs = 2 * (size_t)n;
+    }
+    for (int i = 0; i < n; ++i) {
+        pbs[(size_t)i].n_eval = b->d_neval + 2 * i;
+        b->h_pbs[i] = pbs[(size_t)i];
+        b->h_pbs[n + i] = pbs[(size_t)i];
+        b->h_pbs[n + i].n_eval = b->d_neval + 2 * i + 1;
+    }
+    BHIP(b, hipMemcpyAsync(b->d_pbs, b->h_pbs, 2 * (size_t)n * sizeof(hpm_dev_problem), hipMemcpyHostToDevice,
+                           b->stream));
+    /* every handle's earlier work (uploads, reloads) precedes the batch */
+    for (int i = 0; i < n; ++i) {
+        BHIP(b, hipStreamSynchronize(hs[i]->stream));
+    }
+
+    bool reused = b->exec && key == b->key;
+    if (!reused) {
+        if (b->exec) {
+            BHIP(b, hipGraphExecDestroy(b->exec));
+            b->exec = nullptr;
+        }
+        std::vector<hpm_tile> full, colour;
+        make_tiles(pbs, n, &full, &colour);
+        b->n_full = (int)full.size();
+        b->n_colour = (int)colour.size();
+        rc = grow(b, &b->d_tiles, &b->cap_tiles, full.size() + colour.size());
+        if (rc != HPM_OK) return rc;
+        BHIP(b, hipMemcpyAsync(b->d_tiles, full.data(), full.size() * sizeof(hpm_tile), hipMemcpyHostToDevice,
+                               b->stream));
+        BHIP(b, hipMemcpyAsync(b->d_tiles + full.size(), colour.data(), colour.size() * sizeof(hpm_tile),
+                               hipMemcpyHostToDevice, b->stream));
+        BHIP(b, hipStreamSynchronize(b->stream));
+        /* autotuner: resident blocks per stage launch; 2 blocks of 256
+         * lanes fit a CU (LDS), more blocks balance uneven tiles */
+        hipDeviceProp_t prop;
+        BHIP(b, hipGetDeviceProperties(&prop, b->device));
+        const int cus = std::max(1, prop.multiProcessorCount);
+        const int cands[4] = {2 * cus, 4 * cus, 8 * cus, 16 * cus};
+        rc = grow(b, &b->d_save, &b->cap_save, save_floats);
+        if (rc != HPM_OK) return rc;
+        rc = save_state(b, hs, n, false);
+        if (rc != HPM_OK) return rc;
+        char line[256];
+        snprintf(line, sizeof(line), "batch of %d problems, %d full / %d colour tiles, %d CUs:", n, b->n_full,
+                 b->n_colour, cus);
+        std::string log = line;
+        float best_ms = -1.0f;
+        int best = cands[0];
+        for (int c : cands) {
+            rc = save_state(b, hs, n, true);
+            if (rc != HPM_OK) return rc;
+            BHIP(b, hipEventRecord(b->ev[0], b->stream));
+            rc = enqueue(b, n, c);
+            if (rc != HPM_OK) return rc;
+            BHIP(b, hipEventRecord(b->ev[1], b->stream));
+            BHIP(b, hipEventSynchronize(b->ev[1]));
+            float ms = 0.0f;
+            BHIP(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+            snprintf(line, sizeof(line), " grid %d: %.3f ms;", c, ms);
+            log += line;
+            if (best_ms < 0.0f || ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        rc = save_state(b, hs, n, true);
+        if (rc != HPM_OK) return rc;
+        snprintf(line, sizeof(line), " chosen grid %d\n", best);
+        log += line;
+        b->log += log;
+        b->grid = best;
+        /* capture the schedule */
+        hipGraph_t graph = nullptr;
+        BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+        rc = enqueue(b, n, b->grid);
+        hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+        if (rc != HPM_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        BHIP(b, ce);
+        hipError_t ie = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        BHIP(b, ie);
+        b->key = key;
+    }
+    BHIP(b, hipEventRecord(b->ev[0], b->stream));
+    BHIP(b, hipGraphLaunch(b->exec, b->stream));
+    BHIP(b, hipEventRecord(b->ev[1], b->stream));
+    std::vector<unsigned long long> nev(2 * (size_t)n, 0ull);
+    BHIP(b, hipMemcpyAsync(nev.data(), b->d_neval, nev.size() * 8, hipMemcpyDeviceToHost, b->stream));
+    if (want_readback) {
+        for (int i = 0; i < n; ++i) {
+            hpm_handle* h = hs[i];
+            const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
+            h->h_hyp.resize(np * 4);
+            h->h_cost.resize(np);
+            BHIP(b, hipMemcpyAsync(h->h_hyp.data(), h->d_hyp, np * 16, hipMemcpyDeviceToHost, b->stream));
+            BHIP(b, hipMemcpyAsync(h->h_cost.data(), h->d_cost, np * 4, hipMemcpyDeviceToHost, b->stream));
+        }
+    }
+    BHIP(b, hipStreamSynchronize(b->stream));
+    float ms = 0.0f;
+    BHIP(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    uint64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        hpm_handle* h = hs[i];
+        h->stats.n_eval = nev[2 * (size_t)i] + nev[2 * (size_t)i + 1];
+        h->stats.n_eval_propagate = nev[2 * (size_t)i + 1];
+        h->stats.launches = 11;
+        h->stats.ms_total = ms; /* the whole batch */
+        h->stats.ms_init = h->stats.ms_propagate = h->stats.ms_finalize_filter = 0.0f;
+        h->call_index += 1;
+        total += h->stats.n_eval;
+    }
+    if (stats) {
+        stats->ms_device = ms;
+        stats->ms_host = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host).count();
+        stats->n_eval = total;
+        stats->problems = n;
+        stats->tiles_full = b->n_full;
+        stats->tiles_colour = b->n_colour;
+        stats->grid = b->grid;
+        stats->graph_reused = reused ? 1 : 0;
+    }
+    return HPM_OK;
+}

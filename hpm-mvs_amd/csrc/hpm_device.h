@@ -69,6 +69,15 @@ typedef struct hpm_dev_problem {
     unsigned long long* n_eval; /* NCC evaluation counter (may be null) */
 } hpm_dev_problem;
 
+/* One 64x4 tile of a problem in the batched kernels. */
+typedef struct hpm_tile {
+    int32_t problem;  /* index into the problem table */
+    int32_t bx, by;   /* tile of the problem's full grid (init, finalize) or colour grid (propagate, filter) */
+    int32_t pad_;
+} hpm_tile;
+
+enum { HPM_STAGE_INIT = 0, HPM_STAGE_PROPAGATE = 1, HPM_STAGE_FINALIZE = 2, HPM_STAGE_FILTER = 3 };
+
 /* Kernel launchers (hpm_kernels.hip). All launch on `stream` and return a
  * hipError_t value as int. */
 int hpm_launch_init(const hpm_dev_problem* pb, void* stream);
@@ -94,6 +103,14 @@ int hpm_launch_store_state(const float* hyp, const float* cost, int64_t n, float
 int hpm_launch_jbu_prior(const float* guide, int w, int h, const float* src_nd, int sw, int sh, int scale_i,
                          const hpm_camera* ref, float depth_min, float depth_max, float* planes, uint32_t* mask,
                          void* stream);
+
+/* One stage of the batched schedule over a tile table: `grid` resident
+ * blocks each take a contiguous range of the ntiles tiles. pbs and tiles
+ * are device arrays. */
+int hpm_launch_batch(int stage, const hpm_dev_problem* pbs, const hpm_tile* tiles, int ntiles, int grid, int iter,
+                     int colour, void* stream);
+/* p[0..n) = 0 as a kernel (graph-capturable, unlike a memset node) */
+int hpm_launch_zero_u64(unsigned long long* p, int n, void* stream);
 
 /* kind 0..3: exp/sin/cos/acos of in[i]; kind 4: Philox blocks */
 int hpm_launch_selftest(int kind, const void* in, void* out, int n, void* stream);

@@ -83,6 +83,11 @@ struct hpm_handle {
 /* error helpers */
 int hpm_fail(hpm_handle* h, int code, const char* fmt, ...);
 
+/* The device problem description of a ready handle (its pool, views, state,
+ * call index), and the readiness check of RunPatchMatch. */
+void hpm_fill_problem(const hpm_handle* h, hpm_dev_problem* pb);
+int hpm_ensure_ready(hpm_handle* h);
+
 /* RunPatchMatch with an optional host readback (the C-ABI always reads
  * back; the pass schedule skips it where the host does not need the maps). */
 int hpm_run_internal(hpm_handle* h, bool readback);

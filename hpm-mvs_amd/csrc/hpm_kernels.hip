@@ -1322,19 +1322,26 @@ __device__ __forceinline__ void xcd_block(int* bx, int* by)
     *by = l / gx;
 }
 
-__device__ __forceinline__ bool colour_pixel(const hpm_dev_problem& pb, int colour, int* px, int* py)
+/* pixel of lane (threadIdx) in tile (bx, by) of the colour grid */
+__device__ __forceinline__ bool colour_pixel_at(const hpm_dev_problem& pb, int colour, int bx, int by, int* px,
+                                                int* py)
 {
     /* pixel (x, y) has colour (x + y) & 1; the launch of HPM.cu:1924-1931
      * visits rows y = 2 yy + parity with yy < yy_limit. Here a wave takes
      * every second pixel of one row, so its lanes sample neighbouring
      * source columns. */
-    int bx, by;
-    xcd_block(&bx, &by);
     const int y = by * blockDim.y + threadIdx.y;
     const int x = 2 * (bx * blockDim.x + threadIdx.x) + ((y + colour) & 1);
     *px = x;
     *py = y;
     return (y >> 1) < pb.yy_limit && x < pb.W && y < pb.H;
+}
+
+__device__ __forceinline__ bool colour_pixel(const hpm_dev_problem& pb, int colour, int* px, int* py)
+{
+    int bx, by;
+    xcd_block(&bx, &by);
+    return colour_pixel_at(pb, colour, bx, by, px, py);
 }
 
 } // namespace
@@ -1362,14 +1369,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAV
     count_evals(pb, nev);
 }
 
-__global__ __launch_bounds__(256) void hpm_k_finalize(hpm_dev_problem pb)
+/* GetDepthandNormal (HPM.cu:1751-1764) */
+__device__ __forceinline__ void finalize_pixel(const hpm_dev_problem& pb, int px, int py)
 {
-    const int px = blockIdx.x * blockDim.x + threadIdx.x;
-    const int py = blockIdx.y * blockDim.y + threadIdx.y;
-    if (px >= pb.W || py >= pb.H) {
-        return;
-    }
-    /* GetDepthandNormal (HPM.cu:1751-1764) */
     const hpm_camera rc = camera(pb, 0);
     const int center = py * pb.W + px;
     float4 h = ld4(pb.hyp, center);
@@ -1382,11 +1384,124 @@ __global__ __launch_bounds__(256) void hpm_k_finalize(hpm_dev_problem pb)
     st4(pb.hyp, center, o);
 }
 
+__global__ __launch_bounds__(256) void hpm_k_finalize(hpm_dev_problem pb)
+{
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    const int py = blockIdx.y * blockDim.y + threadIdx.y;
+    if (px < pb.W && py < pb.H) {
+        finalize_pixel(pb, px, py);
+    }
+}
+
 __global__ __launch_bounds__(256) void hpm_k_filter(hpm_dev_problem pb, int colour)
 {
     int px, py;
     if (colour_pixel(pb, colour, &px, &py)) {
         filter_pixel(pb, px, py);
+    }
+}
+
+/* ---------------------------------------------------------- batched form
+ * Several independent problems in one launch per stage: a fixed grid of
+ * resident blocks walks a table of 64x4 tiles (problem, tile x, tile y).
+ * Logical block l (XCD-renumbered) takes the contiguous tile range
+ * [l n / G, (l + 1) n / G), so each XCD works through a contiguous stretch
+ * of problems and rows. Per pixel the work is the unbatched kernels' own. */
+
+__device__ __forceinline__ hpm_tile load_tile(const hpm_tile* tiles, int t)
+{
+    typedef const __attribute__((address_space(4))) hpm_tile* ctile;
+    const ctile p = (ctile)tiles + t;
+    hpm_tile r;
+    r.problem = p->problem;
+    r.bx = p->bx;
+    r.by = p->by;
+    r.pad_ = 0;
+    return r;
+}
+
+__device__ __forceinline__ void tile_range(int ntiles, int* first, int* last)
+{
+    const int g = (int)gridDim.x;
+    const int xcd = (int)blockIdx.x % kXcds, i = (int)blockIdx.x / kXcds;
+    const int base = g / kXcds, rem = g % kXcds;
+    const int l = xcd * base + min(xcd, rem) + i;
+    *first = (int)((int64_t)l * ntiles / g);
+    *last = (int)((int64_t)(l + 1) * ntiles / g);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_init_batch(
+    const hpm_dev_problem* __restrict__ pbs, const hpm_tile* __restrict__ tiles, int ntiles)
+{
+    int first, last;
+    tile_range(ntiles, &first, &last);
+    for (int t = first; t < last; ++t) {
+        const hpm_tile tl = load_tile(tiles, t);
+        const hpm_dev_problem& pb = pbs[tl.problem];
+        const int px = tl.bx * blockDim.x + threadIdx.x;
+        const int py = tl.by * blockDim.y + threadIdx.y;
+        uint32_t nev = 0;
+        if (px < pb.W && py < pb.H) {
+            nev = init_pixel(pb, px, py);
+        }
+        count_evals(pb, nev);
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_propagate_batch(
+    const hpm_dev_problem* __restrict__ pbs, const hpm_tile* __restrict__ tiles, int ntiles, int iter, int colour)
+{
+    int first, last;
+    tile_range(ntiles, &first, &last);
+    for (int t = first; t < last; ++t) {
+        const hpm_tile tl = load_tile(tiles, t);
+        const hpm_dev_problem& pb = pbs[tl.problem];
+        int px, py;
+        uint32_t nev = 0;
+        if (colour_pixel_at(pb, colour, tl.bx, tl.by, &px, &py)) {
+            nev = propagate_pixel(pb, px, py, iter, colour);
+        }
+        count_evals(pb, nev);
+    }
+}
+
+__global__ __launch_bounds__(256) void hpm_k_finalize_batch(const hpm_dev_problem* __restrict__ pbs,
+                                                            const hpm_tile* __restrict__ tiles, int ntiles)
+{
+    int first, last;
+    tile_range(ntiles, &first, &last);
+    for (int t = first; t < last; ++t) {
+        const hpm_tile tl = load_tile(tiles, t);
+        const hpm_dev_problem& pb = pbs[tl.problem];
+        const int px = tl.bx * blockDim.x + threadIdx.x;
+        const int py = tl.by * blockDim.y + threadIdx.y;
+        if (px < pb.W && py < pb.H) {
+            finalize_pixel(pb, px, py);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void hpm_k_filter_batch(const hpm_dev_problem* __restrict__ pbs,
+                                                          const hpm_tile* __restrict__ tiles, int ntiles, int colour)
+{
+    int first, last;
+    tile_range(ntiles, &first, &last);
+    for (int t = first; t < last; ++t) {
+        const hpm_tile tl = load_tile(tiles, t);
+        const hpm_dev_problem& pb = pbs[tl.problem];
+        int px, py;
+        if (colour_pixel_at(pb, colour, tl.bx, tl.by, &px, &py)) {
+            filter_pixel(pb, px, py);
+        }
+    }
+}
+
+/* zeroes n counters (a kernel, so the schedule can be captured in a graph) */
+__global__ __launch_bounds__(64) void hpm_k_zero_u64(unsigned long long* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[i] = 0ull;
     }
 }
 
@@ -1605,6 +1720,42 @@ extern "C" int hpm_launch_jbu(const float* guide, int w, int h, const float* src
     const dim3 blk(64, 4, 1);
     hipLaunchKernelGGL(hpm_k_jbu, full_grid(w, h, blk), blk, 0, (hipStream_t)stream, guide, w, h, src_nd, sw, sh,
                        scale_i, out_depth, out_normal);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_batch(int stage, const hpm_dev_problem* pbs, const hpm_tile* tiles, int ntiles, int grid,
+                                int iter, int colour, void* stream)
+{
+    if (ntiles <= 0) {
+        return 0;
+    }
+    const dim3 blk(64, 4, 1);
+    const dim3 g(grid > 0 ? grid : 1, 1, 1);
+    switch (stage) {
+    case HPM_STAGE_INIT:
+        hipLaunchKernelGGL(hpm_k_init_batch, g, blk, 0, (hipStream_t)stream, pbs, tiles, ntiles);
+        break;
+    case HPM_STAGE_PROPAGATE:
+        hipLaunchKernelGGL(hpm_k_propagate_batch, g, blk, 0, (hipStream_t)stream, pbs, tiles, ntiles, iter, colour);
+        break;
+    case HPM_STAGE_FINALIZE:
+        hipLaunchKernelGGL(hpm_k_finalize_batch, g, blk, 0, (hipStream_t)stream, pbs, tiles, ntiles);
+        break;
+    case HPM_STAGE_FILTER:
+        hipLaunchKernelGGL(hpm_k_filter_batch, g, blk, 0, (hipStream_t)stream, pbs, tiles, ntiles, colour);
+        break;
+    default:
+        return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_zero_u64(unsigned long long* p, int n, void* stream)
+{
+    if (n <= 0) {
+        return 0;
+    }
+    hipLaunchKernelGGL(hpm_k_zero_u64, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, p, n);
     return (int)hipGetLastError();
 }
 

@@ -1154,3 +1154,15 @@ extern "C" int hpm_device_selftest(hpm_handle* h, int kind, const void* in, void
     }
     return HPM_OK;
 }
+
+/* ----------------------------------------------- shared with hpm_batch.cpp */
+
+void hpm_fill_problem(const hpm_handle* h, hpm_dev_problem* pb)
+{
+    fill_problem(h, pb);
+}
+
+int hpm_ensure_ready(hpm_handle* h)
+{
+    return ensure_ready(h);
+}

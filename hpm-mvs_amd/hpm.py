@@ -155,6 +155,19 @@ class HpmPipelineStats(C.Structure):
     ]
 
 
+class HpmBatchStats(C.Structure):
+    _fields_ = [
+        ("ms_device", C.c_double),
+        ("ms_host", C.c_double),
+        ("n_eval", C.c_uint64),
+        ("problems", C.c_int32),
+        ("tiles_full", C.c_int32),
+        ("tiles_colour", C.c_int32),
+        ("grid", C.c_int32),
+        ("graph_reused", C.c_int32),
+    ]
+
+
 _lib: Optional[C.CDLL] = None
 
 _FP = C.POINTER(C.c_float)
@@ -230,6 +243,12 @@ SIGNATURES = {
     "hpm_pipeline_run": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _IP, C.c_int, C.POINTER(HpmPipelineOptions),
                                    C.POINTER(HpmPipelineStats), C.c_char_p, C.c_int]),
     "hpm_pool_get_state": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP]),
+    # hpm_batch.h
+    "hpm_batch_create": (C.c_void_p, [C.c_int]),
+    "hpm_batch_destroy": (None, [C.c_void_p]),
+    "hpm_run_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(HpmBatchStats)]),
+    "hpm_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "hpm_batch_log": (C.c_char_p, [C.c_void_p]),
     "hpm_pipeline_run_folder": (C.c_int, [C.c_char_p, C.POINTER(HpmPipelineOptions), C.POINTER(HpmPipelineStats),
                                           C.c_char_p, C.c_int]),
 }
@@ -469,6 +488,39 @@ class Problem:
 
 
 # --- I/O (include/hpm_io.h) ---
+
+class Batch:
+    """RunPatchMatch of many Problems together (hpm_batch.h): one launch per
+    stage over all problems, the schedule replayed from a hipGraph, the
+    launch grid autotuned on the first call of a batch shape."""
+
+    def __init__(self, device: int = 0):
+        self._b = lib().hpm_batch_create(device)
+        if not self._b:
+            raise HpmError("hpm_batch_create failed (no HIP device?)")
+
+    def close(self):
+        if self._b:
+            lib().hpm_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, problems: Sequence["Problem"], readback: bool = True) -> HpmBatchStats:
+        hs = (C.c_void_p * len(problems))(*[p._h for p in problems])
+        st = HpmBatchStats()
+        rc = lib().hpm_run_batch(self._b, hs, len(problems), int(readback), C.byref(st))
+        if rc < 0:
+            raise HpmError(f"hpm_run_batch: {lib().hpm_batch_last_error(self._b).decode()}")
+        return st
+
+    def log(self) -> str:
+        return lib().hpm_batch_log(self._b).decode()
+
 
 def read_camera(path: str) -> HpmCamera:
     c = HpmCamera()

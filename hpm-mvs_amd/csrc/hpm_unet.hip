@@ -1,0 +1,277 @@
+/*
+ * hpm_unet.hip — BF16 MFMA kernels of the learned depth prior (a small
+ * U-Net): 3x3 / 1x1 convolution forward as an implicit GEMM on the gfx950
+ * matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulation), its weight
+ * gradient, and the memory-bound glue (2x2 average pooling, nearest 2x
+ * upsampling, packing of the network input, BF16 conversion).
+ *
+ * No counterpart in the reference, whose prior is the planar/KNN
+ * construction only (HPM.cpp:759-1179); the learned prior is an addition
+ * of this build, gated per pixel by the NCC judge and with the planar prior
+ * as fallback.
+ *
+ * Layout: activations NHWC per image (pixel-major, channels contiguous) with
+ * a channel stride (ld) so concatenations are slices of one buffer; weights
+ * [Co][k*k][Ci] BF16. Channel counts: Ci a multiple of 16, Co of 32.
+ *
+ * MFMA lane layout (32x32x16, 64 lanes): lane l supplies A[row l%32][k
+ * 8(l/32) .. +7] and B[k 8(l/32) .. +7][col l%32] and receives
+ * D[8j + 4(l/32) + i][l%32] in accumulator 4j + i (j, i < 4).
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+typedef __bf16 hpm_bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+__device__ __forceinline__ f32x16 mfma16(bf16x8 a, bf16x8 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 ld8(const hpm_bf16* p)
+{
+    return *reinterpret_cast<const bf16x8*>(p);
+}
+
+} // namespace
+
+/* MFMA lane-layout probe: one 32x32x16 product of the given operands,
+ * a and b as each lane supplies them, d as each lane receives it. */
+__global__ __launch_bounds__(64) void hpm_k_mfma_probe(const bf16x8* a, const bf16x8* b, f32x16* d)
+{
+    const int l = threadIdx.x;
+    f32x16 acc = {};
+    d[l] = mfma16(a[l], b[l], acc);
+}
+
+/* out[p][co] = act(bias[co] + sum_{tap, ci} in[p + tap][ci] w[co][tap][ci]),
+ * zero padding k/2. One wave: 32 consecutive pixels of a row x 32 output
+ * channels; a block: 4 waves over 4 output-channel tiles. */
+__global__ __launch_bounds__(256) void hpm_k_conv_fwd(const hpm_bf16* __restrict__ in, int ldi, int ci,
+                                                      const hpm_bf16* __restrict__ w, const float* __restrict__ bias,
+                                                      float* out32, hpm_bf16* out16, int ldo, int co, int height,
+                                                      int width, int ks, int relu)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * 32;
+    const int y = blockIdx.y;
+    const int n0 = (blockIdx.z * 4 + wave) * 32;
+    if (n0 >= co) {
+        return;
+    }
+    const int r = lane & 31;
+    const int kh = lane >> 5;
+    const int px = x0 + r;
+    const int taps = ks * ks;
+    const int half = ks / 2;
+    f32x16 acc = {};
+    const bf16x8 zero = {};
+    for (int tap = 0; tap < taps; ++tap) {
+        const int sy = y + tap / ks - half;
+        const int sx = px + tap % ks - half;
+        const bool valid = px < width && sy >= 0 && sy < height && sx >= 0 && sx < width;
+        const hpm_bf16* arow = in + ((int64_t)(valid ? sy : 0) * width + (valid ? sx : 0)) * ldi + kh * 8;
+        const hpm_bf16* brow = w + ((int64_t)(n0 + r) * taps + tap) * ci + kh * 8;
+        for (int c0 = 0; c0 < ci; c0 += 16) {
+            const bf16x8 a = valid ? ld8(arow + c0) : zero;
+            const bf16x8 b = ld8(brow + c0);
+            acc = mfma16(a, b, acc);
+        }
+    }
+    const int n = n0 + r;
+    const float bn = bias ? bias[n] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = x0 + 8 * j + 4 * kh + i;
+            if (x < width) {
+                float v = acc[4 * j + i] + bn;
+                if (relu) {
+                    v = fmaxf(v, 0.0f);
+                }
+                const int64_t o = ((int64_t)y * width + x) * ldo + n;
+                if (out32) {
+                    out32[o] = v;
+                }
+                if (out16) {
+                    out16[o] = (hpm_bf16)v;
+                }
+            }
+        }
+    }
+}
+
+/* dw[co][tap][ci] += sum_p dy[co][p] x[p + tap][ci] over one chunk of
+ * pixels, FP32 atomics into dw. dy is channel-major ([co][H*W], BF16), x
+ * pixel-major with stride ldx. One wave: a 32x32 (co, ci) tile of one tap;
+ * A = dy rows (8 consecutive pixels per lane), B = x gathered per pixel. */
+__global__ __launch_bounds__(256) void hpm_k_conv_wgrad(const hpm_bf16* __restrict__ dy_cm, int co,
+                                                        const hpm_bf16* __restrict__ x, int ldx, int ci,
+                                                        float* dw, int height, int width, int ks, int chunk)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int ci_tiles = ci / 32;
+    const int tile = blockIdx.x * 4 + wave;
+    const int taps = ks * ks;
+    if (tile >= (co / 32) * ci_tiles * taps) {
+        return;
+    }
+    const int tap = tile % taps;
+    const int cit = (tile / taps) % ci_tiles;
+    const int cot = tile / (taps * ci_tiles);
+    const int r = lane & 31;
+    const int kh = lane >> 5;
+    const int ty = tap / ks - ks / 2, tx = tap % ks - ks / 2;
+    const int64_t npx = (int64_t)height * width;
+    const int64_t p_begin = (int64_t)blockIdx.y * chunk;
+    const int64_t p_end = p_begin + chunk < npx ? p_begin + chunk : npx;
+    const hpm_bf16* arow = dy_cm + (int64_t)(cot * 32 + r) * npx;
+    const int cidx = cit * 32 + r;
+    f32x16 acc = {};
+    for (int64_t p0 = p_begin; p0 < p_end; p0 += 16) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int64_t p = p0 + kh * 8 + e;
+            hpm_bf16 av = (hpm_bf16)0.0f, bv = (hpm_bf16)0.0f;
+            if (p < p_end) {
+                av = arow[p];
+                const int py = (int)(p / width), pxx = (int)(p % width);
+                const int sy = py + ty, sx = pxx + tx;
+                if (sy >= 0 && sy < height && sx >= 0 && sx < width) {
+                    bv = x[((int64_t)sy * width + sx) * ldx + cidx];
+                }
+            }
+            a[e] = av;
+            b[e] = bv;
+        }
+        acc = mfma16(a, b, acc);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = 8 * j + 4 * kh + i; /* co row */
+            atomicAdd(&dw[((int64_t)(cot * 32 + m) * taps + tap) * ci + cidx], acc[4 * j + i]);
+        }
+    }
+}
+
+/* 2x2 average pooling (BF16 NHWC with strides), odd edges take what exists */
+__global__ __launch_bounds__(256) void hpm_k_avgpool2(const hpm_bf16* __restrict__ in, int ldi, hpm_bf16* out,
+                                                      int ldo, int c, int height, int width)
+{
+    const int oh = (height + 1) / 2, ow = (width + 1) / 2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)oh * ow * c) {
+        return;
+    }
+    const int ch = (int)(i % c);
+    const int64_t p = i / c;
+    const int oy = (int)(p / ow), ox = (int)(p % ow);
+    float s = 0.0f;
+    int n = 0;
+    for (int dy = 0; dy < 2; ++dy) {
+        for (int dx = 0; dx < 2; ++dx) {
+            const int y = 2 * oy + dy, x = 2 * ox + dx;
+            if (y < height && x < width) {
+                s += (float)in[((int64_t)y * width + x) * ldi + ch];
+                ++n;
+            }
+        }
+    }
+    out[p * ldo + ch] = (hpm_bf16)(s / (float)n);
+}
+
+/* nearest 2x upsampling of a (h+1)/2 x (w+1)/2 map to h x w */
+__global__ __launch_bounds__(256) void hpm_k_upsample2(const hpm_bf16* __restrict__ in, int ldi, hpm_bf16* out,
+                                                       int ldo, int c, int height, int width)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)height * width * c) {
+        return;
+    }
+    const int ch = (int)(i % c);
+    const int64_t p = i / c;
+    const int y = (int)(p / width), x = (int)(p % width);
+    const int iw = (width + 1) / 2;
+    out[p * ldo + ch] = in[((int64_t)(y / 2) * iw + x / 2) * ldi + ch];
+}
+
+/* BF16 <- FP32 */
+__global__ __launch_bounds__(256) void hpm_k_to_bf16(const float* __restrict__ in, hpm_bf16* out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[i] = (hpm_bf16)in[i];
+    }
+}
+
+/* --------------------------------------------------------------- launchers */
+
+extern "C" int hpm_unet_mfma_probe(const void* a, const void* b, float* d, void* stream)
+{
+    hipLaunchKernelGGL(hpm_k_mfma_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, (const bf16x8*)a,
+                       (const bf16x8*)b, (f32x16*)d);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_unet_conv_fwd(const void* in, int ldi, int ci, const void* w, const float* bias, float* out32,
+                                 void* out16, int ldo, int co, int height, int width, int ks, int relu, void* stream)
+{
+    if (ci % 16 != 0 || co % 32 != 0 || (ks != 1 && ks != 3) || ldi % 8 != 0 || height < 1 || width < 1) {
+        return (int)hipErrorInvalidValue;
+    }
+    const dim3 grid((width + 31) / 32, height, (co / 32 + 3) / 4);
+    hipLaunchKernelGGL(hpm_k_conv_fwd, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)in, ldi, ci,
+                       (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, ks, relu);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_unet_conv_wgrad(const void* dy_cm, int co, const void* x, int ldx, int ci, float* dw, int height,
+                                   int width, int ks, void* stream)
+{
+    if (ci % 32 != 0 || co % 32 != 0 || (ks != 1 && ks != 3)) {
+        return (int)hipErrorInvalidValue;
+    }
+    const int tiles = (co / 32) * (ci / 32) * ks * ks;
+    const int64_t npx = (int64_t)height * width;
+    const int chunk = 4096;
+    const dim3 grid((tiles + 3) / 4, (unsigned)((npx + chunk - 1) / chunk), 1);
+    hipLaunchKernelGGL(hpm_k_conv_wgrad, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)dy_cm, co,
+                       (const hpm_bf16*)x, ldx, ci, dw, height, width, ks, chunk);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_unet_avgpool2(const void* in, int ldi, void* out, int ldo, int c, int height, int width,
+                                 void* stream)
+{
+    const int64_t n = (int64_t)((height + 1) / 2) * ((width + 1) / 2) * c;
+    hipLaunchKernelGGL(hpm_k_avgpool2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const hpm_bf16*)in, ldi, (hpm_bf16*)out, ldo, c, height, width);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_unet_upsample2(const void* in, int ldi, void* out, int ldo, int c, int height, int width,
+                                  void* stream)
+{
+    const int64_t n = (int64_t)height * width * c;
+    hipLaunchKernelGGL(hpm_k_upsample2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const hpm_bf16*)in, ldi, (hpm_bf16*)out, ldo, c, height, width);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_unet_to_bf16(const float* in, void* out, int64_t n, void* stream)
+{
+    hipLaunchKernelGGL(hpm_k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, in,
+                       (hpm_bf16*)out, n);
+    return (int)hipGetLastError();
+}

@@ -243,6 +243,17 @@ SIGNATURES = {
     "hpm_pipeline_run": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _IP, C.c_int, C.POINTER(HpmPipelineOptions),
                                    C.POINTER(HpmPipelineStats), C.c_char_p, C.c_int]),
     "hpm_pool_get_state": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP]),
+    # hpm_unet.h (device pointers, e.g. torch tensors' data_ptr())
+    "hpm_unet_mfma_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hpm_unet_conv_fwd": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "hpm_unet_conv_wgrad": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                      C.c_int, C.c_int, C.c_void_p]),
+    "hpm_unet_avgpool2": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_void_p]),
+    "hpm_unet_upsample2": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_void_p]),
+    "hpm_unet_to_bf16": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     # hpm_batch.h
     "hpm_batch_create": (C.c_void_p, [C.c_int]),
     "hpm_batch_destroy": (None, [C.c_void_p]),

@@ -177,6 +177,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workers", type=int, default=0, help="host workers of the schedule (0: library default)")
+    ap.add_argument("--learned-prior", default="", help="U-Net weight file: phase A with the learned prior")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB on the device 3 times (the known-size kernel that scales the FETCH_SIZE "
                          "and WRITE_SIZE counters) and exit")
@@ -220,6 +221,8 @@ def main():
         opt.geom_consistency = 0
     if args.workers > 0:
         opt.workers = args.workers
+    if args.learned_prior:
+        opt.learned_prior = os.path.abspath(args.learned_prior).encode()
 
     for s in range(args.warmup):
         st = hpm.run_pipeline(pool, problems, opt)
@@ -289,6 +292,9 @@ def main():
             "accuracy_2pct": round(float(np.mean(acc)), 4),
             "runs_per_view": runs / args.steps / n,
             "n_eval_per_view": int(evals / args.steps / n),
+            "prior": ("learned (" + os.path.basename(args.learned_prior) + f"), {st.learned_views} views kept it, "
+                      f"{st.learned_fallbacks} fell back, mean kept fraction {st.learned_accept:.3f}")
+            if args.learned_prior else "planar (Delaunay triangles + KNN supplement)",
             "ms_per_view_phase_a": per_view("ms_phase_a"),
             "ms_per_view_phase_b": per_view("ms_phase_b"),
             "ms_per_view_device": per_view("ms_device"),

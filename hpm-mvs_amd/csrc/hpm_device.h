@@ -109,6 +109,11 @@ int hpm_launch_jbu_prior(const float* guide, int w, int h, const float* src_nd, 
  * are device arrays. */
 int hpm_launch_batch(int stage, const hpm_dev_problem* pbs, const hpm_tile* tiles, int ntiles, int grid, int iter,
                      int colour, void* stream);
+/* learned prior: network input (x: W*H*32 BF16) of a finished plain pass,
+ * then prediction + judge into prior planes / masks (head: W*H*32 FP32) */
+int hpm_launch_lprior_features(const hpm_dev_problem* pb, void* x, void* stream);
+int hpm_launch_lprior_judge(const hpm_dev_problem* pb, const void* x, const float* head, float* planes,
+                            uint32_t* mask, unsigned long long* accepted, void* stream);
 /* p[0..n) = 0 as a kernel (graph-capturable, unlike a memset node) */
 int hpm_launch_zero_u64(unsigned long long* p, int n, void* stream);
 

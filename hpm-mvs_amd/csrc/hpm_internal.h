@@ -70,6 +70,8 @@ struct hpm_handle {
     unsigned long long* d_neval = nullptr;
     float* d_jbu_src = nullptr;   /* low-resolution prior input of the JBU */
     size_t jbu_src_cap = 0;       /* bytes */
+    void* d_ws = nullptr;         /* learned-prior activations */
+    size_t ws_cap = 0;            /* bytes */
     size_t alloc_pixels = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool have_prior = false;
@@ -83,10 +85,25 @@ struct hpm_handle {
 /* error helpers */
 int hpm_fail(hpm_handle* h, int code, const char* fmt, ...);
 
+/* returns HPM_EDEVICE with the HIP error in the handle's message */
+#define HPM_HIP(h, call)                                                                            \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) {                                                                     \
+            return hpm_fail((h), HPM_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                                    \
+        }                                                                                           \
+    } while (0)
+
 /* The device problem description of a ready handle (its pool, views, state,
  * call index), and the readiness check of RunPatchMatch. */
 void hpm_fill_problem(const hpm_handle* h, hpm_dev_problem* pb);
 int hpm_ensure_ready(hpm_handle* h);
+/* Allocates the prior planes / masks if needed and enables the planar prior
+ * as if a prior had been uploaded (the caller fills d_prior / d_mask). */
+int hpm_prior_on_device(hpm_handle* h);
+/* A device scratch area of at least `bytes`, kept with the handle. */
+int hpm_workspace(hpm_handle* h, size_t bytes, void** ws);
 
 /* RunPatchMatch with an optional host readback (the C-ABI always reads
  * back; the pass schedule skips it where the host does not need the maps). */

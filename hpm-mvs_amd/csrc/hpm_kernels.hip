@@ -1401,6 +1401,101 @@ __global__ __launch_bounds__(256) void hpm_k_filter(hpm_dev_problem pb, int colo
     }
 }
 
+/* ----------------------------------------------------------- learned prior
+ * Network input and the judge of the learned depth prior (no reference
+ * counterpart; the pipeline uses it in place of the planar prior passes
+ * when a weight file is given, falling back to the planar prior). */
+
+/* x[p] (32 BF16 channels): grey / 255, (depth - dmin) / (dmax - dmin),
+ * camera-frame normal R n_world, cost / 2, zeros; non-finite values -> 0.
+ * hyp holds (world normal, depth) of the finished plain pass. */
+__global__ __launch_bounds__(256) void hpm_k_lprior_features(hpm_dev_problem pb, __bf16* x)
+{
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    const int py = blockIdx.y * blockDim.y + threadIdx.y;
+    if (px >= pb.W || py >= pb.H) {
+        return;
+    }
+    const hpm_camera rc = camera(pb, 0);
+    const int center = py * pb.W + px;
+    const float4 h = ld4(pb.hyp, center);
+    const float dmin = pb.params.depth_min, dmax = pb.params.depth_max;
+    float f[6];
+    f[0] = pb.images[cview(pb, 0)->img_off + center] / 255.0f;
+    f[1] = (h.w - dmin) / (dmax - dmin);
+    f[2] = rc.R[0] * h.x + rc.R[1] * h.y + rc.R[2] * h.z;
+    f[3] = rc.R[3] * h.x + rc.R[4] * h.y + rc.R[5] * h.z;
+    f[4] = rc.R[6] * h.x + rc.R[7] * h.y + rc.R[8] * h.z;
+    f[5] = pb.cost[center] / 2.0f;
+    __bf16* o = x + (size_t)center * 32;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        const float v = c < 6 ? f[c] : 0.0f;
+        o[c] = (__bf16)(isfinite(v) ? v : 0.0f);
+    }
+}
+
+/* Prediction and judge per pixel: depth = dmin + clamp(x1 + h0, 0, 1)
+ * (dmax - dmin), normal = normalise(x[2..4] + h[1..3]) turned toward the
+ * camera; the proposal is kept as prior (mask 1, camera-frame plane) when
+ * its multi-view cost (mean of the top-k NCC, as RandomInitialization
+ * scores, HPM.cu:399-436) is no worse than that of the plain pass's
+ * hypothesis at the pixel. accepted[0] counts kept pixels. */
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_lprior_judge(
+    hpm_dev_problem pb, const __bf16* __restrict__ x, const float* __restrict__ head, float* planes, uint32_t* mask,
+    unsigned long long* accepted)
+{
+    int bx, by;
+    xcd_block(&bx, &by);
+    const int px = bx * blockDim.x + threadIdx.x;
+    const int py = by * blockDim.y + threadIdx.y;
+    uint32_t keep = 0u;
+    if (px < pb.W && py < pb.H) {
+        const hpm_camera rc = camera(pb, 0);
+        const int center = py * pb.W + px;
+        const float dmin = pb.params.depth_min, dmax = pb.params.depth_max;
+        const __bf16* xi = x + (size_t)center * 32;
+        const float* hi = head + (size_t)center * 32;
+        const float dn = fminf(fmaxf((float)xi[1] + hi[0], 0.0f), 1.0f);
+        const float depth = dmin + dn * (dmax - dmin);
+        float4 n = make_float4((float)xi[2] + hi[1], (float)xi[3] + hi[2], (float)xi[4] + hi[3], 0.0f);
+        const float len2 = n.x * n.x + n.y * n.y + n.z * n.z;
+        float4 prop = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool ok = len2 > 1e-12f && isfinite(len2);
+        if (ok) {
+            normalize3(&n);
+            const float4 vd = view_direction(rc, px, py, depth);
+            if (dot3(n, vd) > 0.0f) {
+                n.x = -n.x;
+                n.y = -n.y;
+                n.z = -n.z;
+            }
+            prop = n;
+            prop.w = dist_to_origin(rc, px, py, depth, prop);
+            /* current hypothesis: (world normal, depth) -> camera plane */
+            const float4 t = ld4(pb.hyp, center);
+            float4 cur;
+            cur.x = rc.R[0] * t.x + rc.R[1] * t.y + rc.R[2] * t.z;
+            cur.y = rc.R[3] * t.x + rc.R[4] * t.y + rc.R[5] * t.z;
+            cur.z = rc.R[6] * t.x + rc.R[7] * t.y + rc.R[8] * t.z;
+            cur.w = dist_to_origin(rc, px, py, t.w, cur);
+            RefPatch rp;
+            make_ref_patch(pb.images + cview(pb, 0)->img_off, pb.W, pb.H, px, py, pb.params, &rp);
+            uint32_t sel, nev = 0;
+            const float c_prop = initial_cost(pb, rp, px, py, prop, &sel, &nev);
+            const float c_cur = isfinite(t.w) ? initial_cost(pb, rp, px, py, cur, &sel, &nev) : 2.0f;
+            ok = c_prop <= c_cur && depth >= dmin && depth <= dmax;
+        }
+        st4(planes, center, ok ? prop : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        mask[center] = ok ? 1u : 0u;
+        keep = ok ? 1u : 0u;
+    }
+    const unsigned long long s = wave_sum((unsigned long long)keep);
+    if ((threadIdx.x + blockDim.x * threadIdx.y) % 64 == 0 && s != 0ull) {
+        atomicAdd(accepted, s);
+    }
+}
+
 /* ---------------------------------------------------------- batched form
  * Several independent problems in one launch per stage: a fixed grid of
  * resident blocks walks a table of 64x4 tiles (problem, tile x, tile y).
@@ -1747,6 +1842,23 @@ extern "C" int hpm_launch_batch(int stage, const hpm_dev_problem* pbs, const hpm
     default:
         return (int)hipErrorInvalidValue;
     }
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_lprior_features(const hpm_dev_problem* pb, void* x, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_lprior_features, full_grid(pb->W, pb->H, blk), blk, 0, (hipStream_t)stream, *pb,
+                       (__bf16*)x);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_lprior_judge(const hpm_dev_problem* pb, const void* x, const float* head, float* planes,
+                                       uint32_t* mask, unsigned long long* accepted, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_lprior_judge, full_grid(pb->W, pb->H, blk), blk, 0, (hipStream_t)stream, *pb,
+                       (const __bf16*)x, head, planes, mask, accepted);
     return (int)hipGetLastError();
 }
 

@@ -22,6 +22,7 @@
 #include "hpm_internal.h"
 #include "hpm_io.h"
 #include "hpm_prior.h"
+#include "hpm_unet.h"
 
 namespace {
 
@@ -39,6 +40,7 @@ struct Ctx {
     std::string err;
     int threads;      /* host threads for the KNN supplement of one worker */
     std::mutex mu;    /* guards st and err across phase-A workers */
+    hpm_lprior* lprior = nullptr; /* learned prior, when a weight file is given */
 };
 
 int fail(Ctx& c, const char* fmt, ...)
@@ -181,6 +183,35 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
     const hpm_camera ref_cam = h->cams[0];
     const int W = ref_cam.width, H = ref_cam.height;
     const float dmin = h->params.depth_min, dmax = h->params.depth_max;
+    if (c.opt->planar_prior && c.lprior) {
+        /* learned prior: network + judge on the plain pass, one prior pass */
+        auto t0 = clk::now();
+        float kept = 0.0f;
+        rc = check(c, h, hpm_lprior_apply(c.lprior, h, &kept), "learned prior");
+        if (rc < 0) return rc;
+        add_ms(c, &c.st->ms_prior_device, ms_since(t0));
+        {
+            std::lock_guard<std::mutex> lk(c.mu);
+            c.st->learned_accept += kept;
+            if (kept >= c.opt->learned_min_accept) {
+                c.st->learned_views += 1;
+            } else {
+                c.st->learned_fallbacks += 1;
+            }
+        }
+        if (c.opt->verbose >= 2) {
+            fprintf(stderr, "view %d learned prior: %.3f of the pixels kept\n", ref, kept);
+        }
+        if (kept >= c.opt->learned_min_accept) {
+            rc = check(c, h, hpm_run_internal(h, false), "RunPatchMatch (learned prior)");
+            if (rc < 0) return rc;
+            account(c, h);
+            return check(c, h, hpm_store_to_pool(h, 0), "store");
+        }
+        /* fallback: the planar prior below, from the plain pass */
+        h->params.planar_prior = 0;
+        h->have_prior = false;
+    }
     if (c.opt->planar_prior) {
         hpm_set_planar_prior(h); /* SetPlanarPriorParams (HPM.cpp:477-480) */
         Maps m;
@@ -325,6 +356,8 @@ extern "C" void hpm_pipeline_default(hpm_pipeline_options* opt)
     opt->num_threads = 0;
     opt->scale_bound = 1000;
     opt->workers = 12;
+    opt->learned_prior = nullptr;
+    opt->learned_min_accept = 0.3f;
 }
 
 extern "C" int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, float* cost)
@@ -368,6 +401,9 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
         rc = fail(c, "bad pipeline arguments");
     } else if (!(h = hpm_create(pool->device))) {
         rc = fail(c, "cannot create a problem handle");
+    } else if (c.opt->learned_prior && c.opt->planar_prior &&
+               !(c.lprior = hpm_lprior_load(c.opt->learned_prior, pool->device))) {
+        rc = fail(c, "cannot load the learned prior %s", c.opt->learned_prior);
     }
     /* host workers, each with its own handle and stream: views that do not
      * depend on each other run concurrently, overlapping one view's host
@@ -452,6 +488,12 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
     }
     if (h) {
         hpm_destroy(h);
+    }
+    if (c.lprior) {
+        hpm_lprior_destroy(c.lprior);
+    }
+    if (st->learned_views + st->learned_fallbacks > 0) {
+        st->learned_accept /= (double)(st->learned_views + st->learned_fallbacks);
     }
     st->ms_wall = ms_since(t_all);
     if (rc < 0 && err && err_len > 0) {

@@ -44,14 +44,6 @@ int hpm_fail(hpm_handle* h, int code, const char* fmt, ...)
     return code;
 }
 
-#define HPM_HIP(h, call)                                                                            \
-    do {                                                                                            \
-        hipError_t e_ = (call);                                                                     \
-        if (e_ != hipSuccess) {                                                                     \
-            return hpm_fail((h), HPM_EDEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                            __FILE__, __LINE__);                                                    \
-        }                                                                                           \
-    } while (0)
 
 namespace {
 
@@ -145,6 +137,8 @@ void release_problem(hpm_handle* h)
     dfree(h->d_neval);
     dfree(h->d_jbu_src);
     h->jbu_src_cap = 0;
+    dfree(h->d_ws);
+    h->ws_cap = 0;
     h->alloc_pixels = 0;
     h->have_prior = false;
     h->have_device = false;
@@ -1165,4 +1159,27 @@ void hpm_fill_problem(const hpm_handle* h, hpm_dev_problem* pb)
 int hpm_ensure_ready(hpm_handle* h)
 {
     return ensure_ready(h);
+}
+
+int hpm_prior_on_device(hpm_handle* h)
+{
+    const int rc = ensure_prior_buffers(h);
+    if (rc != HPM_OK) {
+        return rc;
+    }
+    h->params.planar_prior = 1;
+    h->have_prior = true;
+    return HPM_OK;
+}
+
+int hpm_workspace(hpm_handle* h, size_t bytes, void** ws)
+{
+    if (h->ws_cap < bytes) {
+        dfree(h->d_ws);
+        h->ws_cap = 0;
+        HPM_HIP(h, hipMalloc(&h->d_ws, bytes));
+        h->ws_cap = bytes;
+    }
+    *ws = h->d_ws;
+    return HPM_OK;
 }

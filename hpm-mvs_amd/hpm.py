@@ -136,6 +136,8 @@ class HpmPipelineOptions(C.Structure):
         ("verbose", C.c_int32),
         ("scale_bound", C.c_int32),
         ("workers", C.c_int32),
+        ("learned_prior", C.c_char_p),
+        ("learned_min_accept", C.c_float),
     ]
 
 
@@ -152,6 +154,9 @@ class HpmPipelineStats(C.Structure):
         ("ms_propagate", C.c_double),
         ("runs", C.c_int32),
         ("propagate_launches", C.c_int32),
+        ("learned_views", C.c_int32),
+        ("learned_fallbacks", C.c_int32),
+        ("learned_accept", C.c_double),
     ]
 
 
@@ -254,6 +259,9 @@ SIGNATURES = {
     "hpm_unet_upsample2": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_void_p]),
     "hpm_unet_to_bf16": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "hpm_lprior_load": (C.c_void_p, [C.c_char_p, C.c_int]),
+    "hpm_lprior_destroy": (None, [C.c_void_p]),
+    "hpm_lprior_apply": (C.c_int, [C.c_void_p, C.c_void_p, _FP]),
     # hpm_batch.h
     "hpm_batch_create": (C.c_void_p, [C.c_int]),
     "hpm_batch_destroy": (None, [C.c_void_p]),

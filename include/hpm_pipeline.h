@@ -37,6 +37,13 @@ typedef struct hpm_pipeline_options {
     int32_t verbose;
     int32_t scale_bound;        /* size_bound of ComputingMultiScaleSettings (HPM.cpp:852), 1000 */
     int32_t workers;            /* host workers of the independent passes, each with its own handle and stream (12) */
+    /* learned prior (no reference counterpart): weight file of the U-Net
+     * (hpm_unet.h); when set, phase A runs the plain pass, the network and
+     * its judge, and one prior pass instead of the planar prior scales, and
+     * falls back to the planar prior for a view where fewer than
+     * learned_min_accept of the pixels keep the network's proposal */
+    const char* learned_prior;  /* NULL: planar prior */
+    float learned_min_accept;   /* 0.3 */
 } hpm_pipeline_options;
 
 typedef struct hpm_pipeline_stats {
@@ -51,6 +58,9 @@ typedef struct hpm_pipeline_stats {
     double ms_propagate;        /* device time of the propagation launches */
     int32_t runs;               /* RunPatchMatch calls */
     int32_t propagate_launches;
+    int32_t learned_views;      /* views whose prior pass used the learned prior */
+    int32_t learned_fallbacks;  /* views that fell back to the planar prior */
+    double learned_accept;      /* mean kept fraction over the views that ran the network */
 } hpm_pipeline_stats;
 
 void hpm_pipeline_default(hpm_pipeline_options* opt);

@@ -39,6 +39,23 @@ int hpm_unet_avgpool2(const void* in, int ldi, void* out, int ldo, int c, int he
 int hpm_unet_upsample2(const void* in, int ldi, void* out, int ldo, int c, int height, int width, void* stream);
 int hpm_unet_to_bf16(const float* in, void* out, int64_t n, void* stream);
 
+/* ------------------------------------------------------ the learned prior
+ * Weight file of hpm-mvs_amd/unet_prior.py (train_unet_prior.py writes it).
+ * hpm_lprior_load returns NULL if the file is missing or of another
+ * network. hpm_lprior_apply runs on a handle whose last hpm_run_patchmatch
+ * was a plain pass: the network proposes a depth and normal per pixel from
+ * the pass's maps and the reference image, the judge keeps a proposal where
+ * its multi-view cost (top-k NCC mean) is no worse than the plain
+ * hypothesis's, and the kept proposals become the handle's planar prior
+ * (as hpm_upload_prior would leave it; the planar prior is enabled).
+ * accepted receives the fraction of pixels kept. A loaded prior may be
+ * applied from several threads to different handles. */
+typedef struct hpm_lprior hpm_lprior;
+typedef struct hpm_handle hpm_handle;
+hpm_lprior* hpm_lprior_load(const char* path, int device);
+void hpm_lprior_destroy(hpm_lprior* lp);
+int hpm_lprior_apply(hpm_lprior* lp, hpm_handle* h, float* accepted);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,12 +2,22 @@
 
 Workload (default, "full"): the 1600x1066 scene with 11 views (20 plane
 patches, 50 boxes, 30% low-texture area); every view is a reference with the
-other 10 as sources. One step is the whole per-scene schedule of the plain
-method: per view a plain RunPatchMatch, a 0.5x planar prior upsampled with
-JBU and a 1x planar prior (KNN K=8), each followed by RunPatchMatch, then
-3 geometric-consistency passes over all views. ms per reference view =
-step time / 11. Images are resident in HBM before the timed region.
-"--workload plain" times one plain RunPatchMatch per view instead.
+other 10 as sources. One step is the whole per-scene schedule, timed for two
+methods:
+- plain method with the planar prior: per view a plain RunPatchMatch, a 0.5x
+  planar prior upsampled with JBU and a 1x planar prior (KNN K=8), each
+  followed by RunPatchMatch, then 3 geometric-consistency passes over all
+  views;
+- learned prior: per view a plain RunPatchMatch, the U-Net proposal
+  (hpm-mvs_amd/unet_prior.bin) judged by the NCC cost, one RunPatchMatch with
+  the kept proposals as prior (planar prior where too few are kept), then 1
+  geometric-consistency pass (--learned-geom-iterations; the run with 3
+  passes is reported beside it).
+value = ms per reference view (step time / 11) of the learned-prior schedule
+when its depth accuracy (fraction of pixels within 2% of ground truth) is at
+least the plain method's, else of the plain method; both are in config.
+Images are resident in HBM before the timed region. "--workload plain" times
+one plain RunPatchMatch per view instead; "--workload batch" the batched form.
 Prints one JSON line.
 """
 import argparse
@@ -177,7 +187,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workers", type=int, default=0, help="host workers of the schedule (0: library default)")
-    ap.add_argument("--learned-prior", default="", help="U-Net weight file: phase A with the learned prior")
+    ap.add_argument("--learned-prior", default=os.path.join(ROOT, "hpm-mvs_amd", "unet_prior.bin"),
+                    help="U-Net weight file of the learned-prior schedule ('none': time the planar prior only)")
+    ap.add_argument("--learned-geom-iterations", type=int, default=1,
+                    help="geometric-consistency passes of the learned-prior schedule (the plain method runs 3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB on the device 3 times (the known-size kernel that scales the FETCH_SIZE "
                          "and WRITE_SIZE counters) and exit")
@@ -213,43 +226,71 @@ def main():
     problems = [(r, view_order(r, n)[1:]) for r in range(n)]
     log("images resident on the device")
 
-    opt = hpm.pipeline_options()
-    opt.params.knn_k = 8
-    opt.params.geom_iterations = 3
-    if args.workload == "plain":
-        opt.planar_prior = 0
-        opt.geom_consistency = 0
-    if args.workers > 0:
-        opt.workers = args.workers
-    if args.learned_prior:
-        opt.learned_prior = os.path.abspath(args.learned_prior).encode()
+    def options(learned, geom_iterations=3):
+        opt = hpm.pipeline_options()
+        opt.params.knn_k = 8
+        opt.params.geom_iterations = geom_iterations
+        if args.workload == "plain":
+            opt.planar_prior = 0
+            opt.geom_consistency = 0
+        if args.workers > 0:
+            opt.workers = args.workers
+        if learned:
+            opt.learned_prior = os.path.abspath(args.learned_prior).encode()
+        return opt
 
-    for s in range(args.warmup):
-        st = hpm.run_pipeline(pool, problems, opt)
-        log(f"warmup step {s}: {st.ms_wall:.0f} ms")
+    def time_method(name, opt):
+        for s in range(args.warmup):
+            st = hpm.run_pipeline(pool, problems, opt)
+            log(f"{name} warmup step {s}: {st.ms_wall:.0f} ms")
+        tot = {k: 0.0 for k in ("ms_device", "ms_propagate", "ms_prior_host", "ms_prior_device", "ms_phase_a",
+                                "ms_phase_b")}
+        evals = runs = 0
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            st = hpm.run_pipeline(pool, problems, opt)
+            for k in tot:
+                tot[k] += getattr(st, k)
+            evals += st.n_eval
+            runs += st.runs
+            log(f"{name} step {s}: {st.ms_wall:.0f} ms wall, {st.ms_device:.0f} ms device, {st.runs} "
+                f"RunPatchMatch, {st.n_eval} evals")
+        t1 = time.perf_counter()
+        ms_per_step = (t1 - t0) * 1e3 / args.steps
+        acc = []
+        for v in range(n):
+            hyp, _ = pool.state(v)
+            g = gd[v]
+            valid = g > 0
+            acc.append(float(np.mean(np.abs(hyp[..., 3][valid] - g[valid]) / g[valid] < 0.02)))
+        r = {
+            "ms_per_step": ms_per_step,
+            "ms_per_view": round(ms_per_step / n, 3),
+            "accuracy_2pct": round(float(np.mean(acc)), 4),
+            "runs_per_view": runs / args.steps / n,
+            "n_eval_per_view": int(evals / args.steps / n),
+        }
+        for k in ("ms_phase_a", "ms_phase_b", "ms_device", "ms_prior_host", "ms_prior_device"):
+            r["ms_per_view_" + k[3:]] = round(tot[k] / args.steps / n, 2)
+        if opt.learned_prior:
+            r["learned"] = (f"{st.learned_views} views kept the learned prior, {st.learned_fallbacks} fell back to the "
+                            f"planar prior, mean kept fraction {st.learned_accept:.3f}")
+        log(f"{name}: {r['ms_per_view']} ms/view, accuracy {r['accuracy_2pct']}")
+        return r
 
-    tot = {k: 0.0 for k in ("ms_device", "ms_propagate", "ms_prior_host", "ms_prior_device", "ms_phase_a",
-                            "ms_phase_b")}
-    evals = runs = 0
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        st = hpm.run_pipeline(pool, problems, opt)
-        for k in tot:
-            tot[k] += getattr(st, k)
-        evals += st.n_eval
-        runs += st.runs
-        log(f"step {s}: {st.ms_wall:.0f} ms wall, {st.ms_device:.0f} ms device, {st.runs} RunPatchMatch, "
-            f"{st.n_eval} evals")
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    ms_per_view = ms_per_step / n
-
-    acc = []
-    for v in range(n):
-        hyp, _ = pool.state(v)
-        g = gd[v]
-        valid = g > 0
-        acc.append(float(np.mean(np.abs(hyp[..., 3][valid] - g[valid]) / g[valid] < 0.02)))
+    learned = args.workload == "full" and args.learned_prior.lower() != "none"
+    plain = time_method("planar prior", options(False))
+    res = same = None
+    if learned:
+        res = time_method(f"learned prior, {args.learned_geom_iterations} GC passes",
+                          options(True, args.learned_geom_iterations))
+        if args.learned_geom_iterations != 3:
+            same = time_method("learned prior, 3 GC passes", options(True, 3))
+    # the metric: ms per view at equal or better depth accuracy than the
+    # plain method; the learned-prior schedule counts when it meets that bar
+    chosen = res if res is not None and res["accuracy_2pct"] >= plain["accuracy_2pct"] else plain
+    ms_per_step = chosen["ms_per_step"]
+    ms_per_view = chosen["ms_per_view"]
 
     # Roofline of the propagation kernel, measured alone: during the timed
     # schedule several workers' launches overlap on the device, so their
@@ -263,13 +304,22 @@ def main():
     avg_launch_ms = rst.ms_propagate / rst.propagate_launches
     flops_per_launch = rst.n_eval_propagate / rst.propagate_launches * FLOPS_PER_EVAL
     achieved = flops_per_launch / (avg_launch_ms * 1e-3) / 1e12
-    per_view = lambda k: round(tot[k] / args.steps / n, 2)  # noqa: E731
 
-    workload = ("full plain method per reference view: plain RunPatchMatch, 0.5x planar prior with JBU, 1x planar "
-                "prior (KNN K=8), 3 geometric-consistency passes; 1600x1066, 10 sources, 11x11 window stride 2"
-                if args.workload == "full" else
-                "one plain RunPatchMatch per reference view (no prior, no geometric consistency); 1600x1066, "
-                "10 sources")
+    if args.workload == "full":
+        workload = ("per reference view: plain RunPatchMatch, then the prior (learned: U-Net proposal + NCC judge, "
+                    "one prior pass, planar prior as fallback, then "
+                    f"{args.learned_geom_iterations} geometric-consistency passes; plain method: 0.5x planar prior "
+                    "with JBU and 1x planar prior with KNN K=8, then 3 geometric-consistency passes); 1600x1066, "
+                    "10 sources, 11x11 window stride 2")
+    else:
+        workload = ("one plain RunPatchMatch per reference view (no prior, no geometric consistency); 1600x1066, "
+                    "10 sources")
+    methods = {"plain_gpu_planar_prior": {k: v for k, v in plain.items() if k != "ms_per_step"}}
+    if res is not None:
+        methods["learned_prior"] = {k: v for k, v in res.items() if k != "ms_per_step"}
+        methods["learned_prior"]["geom_iterations"] = args.learned_geom_iterations
+    if same is not None:
+        methods["learned_prior_3_gc_passes"] = {k: v for k, v in same.items() if k != "ms_per_step"}
     out = {
         "metric": METRIC,
         "value": round(ms_per_view, 3),
@@ -289,17 +339,10 @@ def main():
             "width": cfg.width,
             "height": cfg.height,
             "sources": n - 1,
-            "accuracy_2pct": round(float(np.mean(acc)), 4),
-            "runs_per_view": runs / args.steps / n,
-            "n_eval_per_view": int(evals / args.steps / n),
-            "prior": ("learned (" + os.path.basename(args.learned_prior) + f"), {st.learned_views} views kept it, "
-                      f"{st.learned_fallbacks} fell back, mean kept fraction {st.learned_accept:.3f}")
-            if args.learned_prior else "planar (Delaunay triangles + KNN supplement)",
-            "ms_per_view_phase_a": per_view("ms_phase_a"),
-            "ms_per_view_phase_b": per_view("ms_phase_b"),
-            "ms_per_view_device": per_view("ms_device"),
-            "ms_per_view_prior_host": per_view("ms_prior_host"),
-            "ms_per_view_prior_device": per_view("ms_prior_device"),
+            "value_method": "learned_prior" if chosen is res else "plain_gpu_planar_prior",
+            "accuracy_2pct": chosen["accuracy_2pct"],
+            "speedup_vs_plain": round(plain["ms_per_view"] / ms_per_view, 3),
+            "methods": methods,
         },
         "roofline": {
             "bound": "valu",
@@ -316,8 +359,8 @@ def main():
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or (os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")
-        out["cpu_baseline"] = cpu_baseline(cams, imgs, 5, int(evals / args.steps / n),
-                                           tot["ms_prior_host"] / args.steps / n, threads)
+        out["cpu_baseline"] = cpu_baseline(cams, imgs, 5, plain["n_eval_per_view"],
+                                           plain["ms_per_view_prior_host"], threads)
     pool.close()
     print(json.dumps(out), flush=True)
 

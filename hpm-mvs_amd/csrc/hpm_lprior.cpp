@@ -104,6 +104,117 @@ extern "C" hpm_lprior* hpm_lprior_load(const char* path, int device)
     return lp;
 }
 
+namespace {
+
+/* activation buffers of one forward pass at H x W, packed in one area */
+struct Plan {
+    size_t x, t0, cat0, p1, t1, cat1, p2, t2, cat2, p3, t3, b3, d2, d1, d0, head, cnt, total;
+};
+
+Plan plan(int W, int H)
+{
+    const int w1 = (W + 1) / 2, h1 = (H + 1) / 2;
+    const int w2 = (w1 + 1) / 2, h2 = (h1 + 1) / 2;
+    const int w3 = (w2 + 1) / 2, h3 = (h2 + 1) / 2;
+    const size_t p0 = (size_t)W * H, p1 = (size_t)w1 * h1, p2 = (size_t)w2 * h2, p3 = (size_t)w3 * h3;
+    Plan pl;
+    size_t total = 0;
+    auto take = [&](size_t bytes) {
+        const size_t off = total;
+        total += (bytes + 255) & ~(size_t)255;
+        return off;
+    };
+    pl.x = take(p0 * 32 * 2);
+    pl.t0 = take(p0 * 32 * 2);
+    pl.cat0 = take(p0 * 96 * 2);
+    pl.p1 = take(p1 * 32 * 2);
+    pl.t1 = take(p1 * 64 * 2);
+    pl.cat1 = take(p1 * 160 * 2);
+    pl.p2 = take(p2 * 64 * 2);
+    pl.t2 = take(p2 * 96 * 2);
+    pl.cat2 = take(p2 * 224 * 2);
+    pl.p3 = take(p3 * 96 * 2);
+    pl.t3 = take(p3 * 128 * 2);
+    pl.b3 = take(p3 * 128 * 2);
+    pl.d2 = take(p2 * 96 * 2);
+    pl.d1 = take(p1 * 64 * 2);
+    pl.d0 = take(p0 * 32 * 2);
+    pl.head = take(p0 * 32 * 4);
+    pl.cnt = take(8);
+    pl.total = total;
+    return pl;
+}
+
+/* the network on features x (BF16, W*H*32) into head (FP32, W*H*32); base is
+ * a workspace laid out by plan() */
+hipError_t forward(const hpm_lprior* lp, const void* x, int W, int H, float* head, char* base, const Plan& pl,
+                   hipStream_t s)
+{
+    const int w1 = (W + 1) / 2, h1 = (H + 1) / 2;
+    const int w2 = (w1 + 1) / 2, h2 = (h1 + 1) / 2;
+    const int w3 = (w2 + 1) / 2, h3 = (h2 + 1) / 2;
+    auto at = [&](size_t off, size_t elem_off = 0) { return (void*)(base + off + elem_off * 2); };
+    const Layer* L = lp->layers;
+    hipError_t e = hipSuccess;
+    auto conv = [&](int li, const void* in, int ldi, void* out, int ldo, int hh, int ww, float* out32) {
+        if (e == hipSuccess) {
+            const Layer& l = L[li];
+            e = (hipError_t)hpm_unet_conv_fwd(in, ldi, l.ci, l.w, l.b, out32, out32 ? nullptr : out, ldo, l.co, hh,
+                                              ww, l.ks, l.relu, s);
+        }
+    };
+    auto pool = [&](const void* in, int ldi, void* out, int c, int hh, int ww) {
+        if (e == hipSuccess) {
+            e = (hipError_t)hpm_unet_avgpool2(in, ldi, out, c, c, hh, ww, s);
+        }
+    };
+    auto up = [&](const void* in, int c, void* out, int ldo, int hh, int ww) {
+        if (e == hipSuccess) {
+            e = (hipError_t)hpm_unet_upsample2(in, c, out, ldo, c, hh, ww, s);
+        }
+    };
+    /* encoder; s0 = cat0[64:96], s1 = cat1[96:160], s2 = cat2[128:224] */
+    conv(E0A, x, 32, at(pl.t0), 32, H, W, nullptr);
+    conv(E0B, at(pl.t0), 32, at(pl.cat0, 64), 96, H, W, nullptr);
+    pool(at(pl.cat0, 64), 96, at(pl.p1), 32, H, W);
+    conv(E1A, at(pl.p1), 32, at(pl.t1), 64, h1, w1, nullptr);
+    conv(E1B, at(pl.t1), 64, at(pl.cat1, 96), 160, h1, w1, nullptr);
+    pool(at(pl.cat1, 96), 160, at(pl.p2), 64, h1, w1);
+    conv(E2A, at(pl.p2), 64, at(pl.t2), 96, h2, w2, nullptr);
+    conv(E2B, at(pl.t2), 96, at(pl.cat2, 128), 224, h2, w2, nullptr);
+    pool(at(pl.cat2, 128), 224, at(pl.p3), 96, h2, w2);
+    conv(E3A, at(pl.p3), 96, at(pl.t3), 128, h3, w3, nullptr);
+    conv(E3B, at(pl.t3), 128, at(pl.b3), 128, h3, w3, nullptr);
+    /* decoder */
+    up(at(pl.b3), 128, at(pl.cat2), 224, h2, w2);
+    conv(D2, at(pl.cat2), 224, at(pl.d2), 96, h2, w2, nullptr);
+    up(at(pl.d2), 96, at(pl.cat1), 160, h1, w1);
+    conv(D1, at(pl.cat1), 160, at(pl.d1), 64, h1, w1, nullptr);
+    up(at(pl.d1), 64, at(pl.cat0), 96, H, W);
+    conv(D0, at(pl.cat0), 96, at(pl.d0), 32, H, W, nullptr);
+    conv(HEAD, at(pl.d0), 32, nullptr, 32, H, W, head);
+    return e;
+}
+
+} // namespace
+
+extern "C" int hpm_lprior_forward(hpm_lprior* lp, const void* x, int height, int width, float* head, void* stream)
+{
+    if (!lp || !x || !head || height < 1 || width < 1) {
+        return (int)hipErrorInvalidValue;
+    }
+    const Plan pl = plan(width, height);
+    char* ws = nullptr;
+    hipError_t e = hipMalloc((void**)&ws, pl.total);
+    if (e != hipSuccess) {
+        return (int)e;
+    }
+    e = forward(lp, x, width, height, head, ws, pl, (hipStream_t)stream);
+    const hipError_t e2 = hipStreamSynchronize((hipStream_t)stream);
+    (void)hipFree(ws);
+    return (int)(e != hipSuccess ? e : e2);
+}
+
 extern "C" int hpm_lprior_apply(hpm_lprior* lp, hpm_handle* h, float* accepted)
 {
     if (!lp || !h) {
@@ -116,75 +227,30 @@ extern "C" int hpm_lprior_apply(hpm_lprior* lp, hpm_handle* h, float* accepted)
     hpm_dev_problem pb;
     hpm_fill_problem(h, &pb);
     const int W = pb.W, H = pb.H;
-    const int w1 = (W + 1) / 2, h1 = (H + 1) / 2;
-    const int w2 = (w1 + 1) / 2, h2 = (h1 + 1) / 2;
-    const int w3 = (w2 + 1) / 2, h3 = (h2 + 1) / 2;
-    const size_t p0 = (size_t)W * H, p1 = (size_t)w1 * h1, p2 = (size_t)w2 * h2, p3 = (size_t)w3 * h3;
-    /* workspace (BF16 elements unless noted) */
-    struct Buf {
-        size_t off, bytes;
-    };
-    size_t total = 0;
-    auto take = [&](size_t bytes) {
-        Buf b{total, bytes};
-        total += (bytes + 255) & ~(size_t)255;
-        return b;
-    };
-    const Buf bx = take(p0 * 32 * 2), bt0 = take(p0 * 32 * 2), bcat0 = take(p0 * 96 * 2);
-    const Buf bp1 = take(p1 * 32 * 2), bt1 = take(p1 * 64 * 2), bcat1 = take(p1 * 160 * 2);
-    const Buf bp2 = take(p2 * 64 * 2), bt2 = take(p2 * 96 * 2), bcat2 = take(p2 * 224 * 2);
-    const Buf bp3 = take(p3 * 96 * 2), bt3 = take(p3 * 128 * 2), bb3 = take(p3 * 128 * 2);
-    const Buf bd2 = take(p2 * 96 * 2), bd1 = take(p1 * 64 * 2), bd0 = take(p0 * 32 * 2);
-    const Buf bhead = take(p0 * 32 * 4), bcnt = take(8);
+    const Plan pl = plan(W, H);
     void* ws = nullptr;
-    int rc = hpm_workspace(h, total, &ws);
+    int rc = hpm_workspace(h, pl.total, &ws);
     if (rc != HPM_OK) {
         return rc;
     }
     char* base = (char*)ws;
-    auto at = [&](const Buf& b, size_t elem_off = 0) { return (void*)(base + b.off + elem_off * 2); };
     hipStream_t s = h->stream;
-    const Layer* L = lp->layers;
-    auto conv = [&](int li, const void* in, int ldi, void* out, int ldo, int hh, int ww, float* out32) {
-        const Layer& l = L[li];
-        return (hipError_t)hpm_unet_conv_fwd(in, ldi, l.ci, l.w, l.b, out32, out32 ? nullptr : out, ldo, l.co, hh,
-                                             ww, l.ks, l.relu, s);
-    };
-    HPM_HIP(h, (hipError_t)hpm_launch_lprior_features(&pb, at(bx), s));
-    /* encoder; s0 = cat0[64:96], s1 = cat1[96:160], s2 = cat2[128:224] */
-    HPM_HIP(h, conv(E0A, at(bx), 32, at(bt0), 32, H, W, nullptr));
-    HPM_HIP(h, conv(E0B, at(bt0), 32, at(bcat0, 64), 96, H, W, nullptr));
-    HPM_HIP(h, (hipError_t)hpm_unet_avgpool2(at(bcat0, 64), 96, at(bp1), 32, 32, H, W, s));
-    HPM_HIP(h, conv(E1A, at(bp1), 32, at(bt1), 64, h1, w1, nullptr));
-    HPM_HIP(h, conv(E1B, at(bt1), 64, at(bcat1, 96), 160, h1, w1, nullptr));
-    HPM_HIP(h, (hipError_t)hpm_unet_avgpool2(at(bcat1, 96), 160, at(bp2), 64, 64, h1, w1, s));
-    HPM_HIP(h, conv(E2A, at(bp2), 64, at(bt2), 96, h2, w2, nullptr));
-    HPM_HIP(h, conv(E2B, at(bt2), 96, at(bcat2, 128), 224, h2, w2, nullptr));
-    HPM_HIP(h, (hipError_t)hpm_unet_avgpool2(at(bcat2, 128), 224, at(bp3), 96, 96, h2, w2, s));
-    HPM_HIP(h, conv(E3A, at(bp3), 96, at(bt3), 128, h3, w3, nullptr));
-    HPM_HIP(h, conv(E3B, at(bt3), 128, at(bb3), 128, h3, w3, nullptr));
-    /* decoder */
-    HPM_HIP(h, (hipError_t)hpm_unet_upsample2(at(bb3), 128, at(bcat2), 224, 128, h2, w2, s));
-    HPM_HIP(h, conv(D2, at(bcat2), 224, at(bd2), 96, h2, w2, nullptr));
-    HPM_HIP(h, (hipError_t)hpm_unet_upsample2(at(bd2), 96, at(bcat1), 160, 96, h1, w1, s));
-    HPM_HIP(h, conv(D1, at(bcat1), 160, at(bd1), 64, h1, w1, nullptr));
-    HPM_HIP(h, (hipError_t)hpm_unet_upsample2(at(bd1), 64, at(bcat0), 96, 64, H, W, s));
-    HPM_HIP(h, conv(D0, at(bcat0), 96, at(bd0), 32, H, W, nullptr));
-    float* head = (float*)(base + bhead.off);
-    HPM_HIP(h, conv(HEAD, at(bd0), 32, nullptr, 32, H, W, head));
+    HPM_HIP(h, (hipError_t)hpm_launch_lprior_features(&pb, base + pl.x, s));
+    float* head = (float*)(base + pl.head);
+    HPM_HIP(h, forward(lp, base + pl.x, W, H, head, base, pl, s));
     /* judge into the handle's prior */
     rc = hpm_prior_on_device(h);
     if (rc != HPM_OK) {
         return rc;
     }
-    unsigned long long* cnt = (unsigned long long*)(base + bcnt.off);
+    unsigned long long* cnt = (unsigned long long*)(base + pl.cnt);
     HPM_HIP(h, (hipError_t)hpm_launch_zero_u64(cnt, 1, s));
-    HPM_HIP(h, (hipError_t)hpm_launch_lprior_judge(&pb, at(bx), head, h->d_prior, h->d_mask, cnt, s));
+    HPM_HIP(h, (hipError_t)hpm_launch_lprior_judge(&pb, base + pl.x, head, h->d_prior, h->d_mask, cnt, s));
     unsigned long long kept = 0;
     HPM_HIP(h, hipMemcpyAsync(&kept, cnt, 8, hipMemcpyDeviceToHost, s));
     HPM_HIP(h, hipStreamSynchronize(s));
     if (accepted) {
-        *accepted = (float)((double)kept / (double)p0);
+        *accepted = (float)((double)kept / ((double)W * H));
     }
     return HPM_OK;
 }

@@ -6,7 +6,10 @@
  * against the drop-in `hpm::HPM` class, file for file (.dmb between passes),
  * to show the class API driving the same path as main.cpp does.
  *
- * Options: --geom-iterations N (2), --knn-k K (6), --no-prior, --no-geom.
+ * Options: --geom-iterations N (2), --knn-k K (6), --no-prior, --no-geom,
+ * --learned-prior FILE (device schedule only: the U-Net weight file of the
+ * learned prior, which then replaces the planar prior passes where its judge
+ * keeps enough pixels).
  */
 #include <math.h>
 #include <stdio.h>
@@ -256,11 +259,12 @@ int main(int argc, char** argv)
 {
     if (argc < 2) {
         printf("USAGE: hpm_mvs dense_folder [--class-api] [--geom-iterations N] [--knn-k K] [--no-prior] "
-               "[--no-geom]\n");
+               "[--no-geom] [--learned-prior FILE]\n");
         return -1;
     }
     const std::string dense_folder = argv[1];
     Options opt;
+    const char* learned = nullptr;
     for (int i = 2; i < argc; ++i) {
         if (!strcmp(argv[i], "--class-api")) {
             opt.class_api = true;
@@ -272,6 +276,8 @@ int main(int argc, char** argv)
             opt.planar_prior = false;
         } else if (!strcmp(argv[i], "--no-geom")) {
             opt.geom = false;
+        } else if (!strcmp(argv[i], "--learned-prior") && i + 1 < argc) {
+            learned = argv[++i];
         } else {
             fprintf(stderr, "unknown option %s\n", argv[i]);
             return -1;
@@ -285,6 +291,7 @@ int main(int argc, char** argv)
         po.params.geom_iterations = opt.geom_iterations;
         po.planar_prior = opt.planar_prior ? 1 : 0;
         po.geom_consistency = opt.geom ? 1 : 0;
+        po.learned_prior = learned;
         hpm_pipeline_stats st;
         char err[512] = {0};
         if (hpm_pipeline_run_folder(dense_folder.c_str(), &po, &st, err, sizeof(err)) != 0) {
@@ -293,7 +300,15 @@ int main(int argc, char** argv)
         }
         printf("%d RunPatchMatch calls, %.1f ms wall, %.1f ms device, %.1f ms host prior\n", st.runs, st.ms_wall,
                st.ms_device, st.ms_prior_host);
+        if (learned) {
+            printf("learned prior: %d views kept it, %d fell back to the planar prior\n", st.learned_views,
+                   st.learned_fallbacks);
+        }
         return 0;
+    }
+    if (learned) {
+        fprintf(stderr, "hpm_mvs: --learned-prior runs with the device schedule, not with --class-api\n");
+        return -1;
     }
     try {
         const std::vector<Problem> problems = GenerateSampleList(dense_folder);

@@ -55,6 +55,11 @@ typedef struct hpm_handle hpm_handle;
 hpm_lprior* hpm_lprior_load(const char* path, int device);
 void hpm_lprior_destroy(hpm_lprior* lp);
 int hpm_lprior_apply(hpm_lprior* lp, hpm_handle* h, float* accepted);
+/* The network alone: head (FP32, height*width*32; channel 0 the depth
+ * residual, 1..3 the normal residual) of the features x (BF16,
+ * height*width*32, as hpm-mvs_amd/unet_prior.py builds them); synchronous,
+ * returns a hipError_t value. */
+int hpm_lprior_forward(hpm_lprior* lp, const void* x, int height, int width, float* head, void* stream);
 
 #ifdef __cplusplus
 }

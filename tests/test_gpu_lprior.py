@@ -1,0 +1,235 @@
+"""The learned prior (include/hpm_unet.h: hpm_lprior_*) on the GPU.
+
+- Network forward of the pipeline (hpm_lprior_forward, the C++ graph over the
+  BF16 MFMA kernels) against a float64 CPU restatement that rounds to BF16
+  exactly where the device stores BF16 (every convolution output and pooled
+  map). FP32 accumulation against float64 flips a BF16 rounding now and then;
+  bar: max abs error of the head within 2% of its largest value, mean abs
+  error within 0.2%.
+- The same forward against the training graph (unet_prior.forward: torch
+  glue around the same kernels, FP32 between layers), to the same bar: the
+  weights trained on one graph are run on the other.
+- Pipeline with the learned prior on a 4-view 320x240 scene: statistics
+  consistent, depth accuracy above the planar-prior pipeline's, and with
+  learned_min_accept above 1 (every view falls back) the maps bit-identical
+  to the planar-prior pipeline's.
+- Network gradients of training against float64 CPU autograd.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import say
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "..", "hpm-mvs_amd", "unet_prior.bin")
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).double()
+
+
+def _cpu_forward(x, ws, bs, up):
+    """float64 restatement of the C++ graph; x (H, W, 32) BF16 CPU tensor"""
+    F = torch.nn.functional
+    L = {name: i for i, (name, *_) in enumerate(up.LAYERS)}
+
+    def conv(name, t, store_bf16=True):
+        i = L[name]
+        _, ci, co, ks, relu = up.LAYERS[i]
+        w = ws[i].double().reshape(co, ks, ks, ci).permute(0, 3, 1, 2)
+        y = F.conv2d(t.permute(2, 0, 1)[None], w, bs[i].double(), padding=ks // 2)[0].permute(1, 2, 0)
+        if relu:
+            y = torch.relu(y)
+        return _bf(y) if store_bf16 else y
+
+    def pool(t):
+        h, w, c = t.shape
+        oh, ow = (h + 1) // 2, (w + 1) // 2
+        tp = F.pad(t.permute(2, 0, 1), (0, 2 * ow - w, 0, 2 * oh - h))
+        n = F.pad(torch.ones(1, h, w, dtype=torch.float64), (0, 2 * ow - w, 0, 2 * oh - h))
+        s = F.avg_pool2d(tp[None], 2, divisor_override=1)[0]
+        cnt = F.avg_pool2d(n[None], 2, divisor_override=1)[0]
+        return _bf((s / cnt).permute(1, 2, 0))
+
+    def upsample(t, h, w):
+        return t[torch.arange(h) // 2][:, torch.arange(w) // 2]
+
+    x = x.double()
+    s0 = conv("e0b", conv("e0a", x))
+    s1 = conv("e1b", conv("e1a", pool(s0)))
+    s2 = conv("e2b", conv("e2a", pool(s1)))
+    b3 = conv("e3b", conv("e3a", pool(s2)))
+    d2 = conv("d2", torch.cat([upsample(b3, *s2.shape[:2]), s2], dim=2))
+    d1 = conv("d1", torch.cat([upsample(d2, *s1.shape[:2]), s1], dim=2))
+    d0 = conv("d0", torch.cat([upsample(d1, *s0.shape[:2]), s0], dim=2))
+    return conv("head", d0, store_bf16=False)
+
+
+def _features(up, h, w, seed):
+    """network input with the value ranges of a plain pass: grey image,
+    normalised depth, unit camera normal facing the camera, cost"""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+    image = 127 + 100 * np.sin(xx / 3.0) * np.cos(yy / 5.0) + rng.normal(0, 10, (h, w))
+    depth = 2.0 + 0.5 * np.sin(xx / 17.0) + 0.01 * rng.normal(size=(h, w))
+    n = np.stack([0.2 * rng.normal(size=(h, w)), 0.2 * rng.normal(size=(h, w)), -np.ones((h, w))], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    cost = rng.uniform(0, 2, (h, w))
+    return up.features(image.astype(np.float32), depth.astype(np.float32), n.astype(np.float32),
+                       cost.astype(np.float32), 1.0, 3.5)
+
+
+@pytest.fixture(scope="module")
+def lprior(hpm):
+    lp = hpm.lib().hpm_lprior_load(WEIGHTS.encode(), 0)
+    assert lp, f"cannot load {WEIGHTS}"
+    yield lp
+    hpm.lib().hpm_lprior_destroy(lp)
+
+
+def _device_forward(hpm, lp, x_bf):
+    h, w, _ = x_bf.shape
+    x_d = x_bf.cuda().contiguous()
+    head = torch.full((h, w, 32), float("nan"), dtype=torch.float32, device="cuda")
+    rc = hpm.lib().hpm_lprior_forward(lp, C.c_void_p(x_d.data_ptr()), h, w, C.c_void_p(head.data_ptr()),
+                                      C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    torch.cuda.synchronize()
+    return head.cpu().double()
+
+
+def _close(got, ref, what):
+    err = (got - ref).abs()
+    scale = ref.abs().max().item()
+    say(f"{what}: max abs err {err.max().item():.3e}, mean {err.mean().item():.3e}, scale {scale:.3e}")
+    assert torch.isfinite(got).all()
+    assert err.max().item() <= 2e-2 * scale
+    assert err.mean().item() <= 2e-3 * scale
+
+
+@pytest.mark.parametrize("h,w", [(37, 53), (64, 96), (121, 170)])
+def test_forward_matches_cpu(hpm, lprior, h, w):
+    from hpm_mvs_amd import unet_prior as up
+
+    say(f"case: learned-prior forward {h}x{w} against float64 CPU graph")
+    ws, bs = up.load(WEIGHTS)
+    x = torch.from_numpy(_features(up, h, w, seed=h)).to(torch.bfloat16)
+    got = _device_forward(hpm, lprior, x)
+    ref = _cpu_forward(x, ws, bs, up)
+    _close(got, ref, "device vs CPU")
+
+
+def test_forward_matches_training_graph(hpm, lprior):
+    from hpm_mvs_amd import unet_prior as up
+
+    h, w = 90, 130
+    say(f"case: learned-prior forward {h}x{w} against the training graph")
+    ws, bs = up.load(WEIGHTS)
+    x = torch.from_numpy(_features(up, h, w, seed=3)).to(torch.bfloat16)
+    got = _device_forward(hpm, lprior, x)
+    with torch.no_grad():
+        ref = up.forward(x.cuda(), [t.float().cuda() for t in ws], [t.cuda() for t in bs]).cpu().double()
+    _close(got, ref, "pipeline vs training graph")
+
+
+def test_pipeline_learned_prior(hpm):
+    from test_gpu_pipeline import scene
+
+    cfg, cams, imgs, gd, problems = scene(hpm)
+    say("case: pipeline 4 views 320x240, planar prior / learned prior / learned prior forced to fall back")
+    res = {}
+    for name, kw in [("planar", {}), ("learned", {"learned_prior": WEIGHTS.encode()}),
+                     ("fallback", {"learned_prior": WEIGHTS.encode(), "learned_min_accept": 1.5})]:
+        opt = hpm.pipeline_options(**kw)
+        opt.params.geom_iterations = 1
+        pool = hpm.Pool(cams, imgs)
+        st = hpm.run_pipeline(pool, problems, opt)
+        maps = [pool.state(v) for v, _ in problems]
+        pool.close()
+        acc = np.mean([np.mean(np.abs(m[0][..., 3] - gd[v]) / gd[v] < 0.02) for (v, _), m in zip(problems, maps)])
+        say(f"{name}: runs={st.runs} learned_views={st.learned_views} fallbacks={st.learned_fallbacks} "
+            f"kept={st.learned_accept:.3f} acc2%={acc:.4f}")
+        res[name] = (st, maps, acc)
+    st, _, _ = res["planar"]
+    assert st.learned_views == 0 and st.learned_fallbacks == 0
+    st, _, acc = res["learned"]
+    assert st.learned_views + st.learned_fallbacks == 4
+    assert 0.0 < st.learned_accept <= 1.0
+    # what the trained network gains: 0.580 against 0.542 (measured) of the
+    # pixels within 2% of ground truth; the run is deterministic
+    assert acc > res["planar"][2]
+    st, maps, _ = res["fallback"]
+    assert st.learned_views == 0 and st.learned_fallbacks == 4
+    assert st.runs == res["planar"][0].runs
+    for (hf, cf), (hp, cp) in zip(maps, res["planar"][1]):
+        assert np.array_equal(hf.view(np.uint32), hp.view(np.uint32))
+        assert np.array_equal(cf.view(np.uint32), cp.view(np.uint32))
+
+
+def test_network_gradients(hpm):
+    """Training gradients (the hand-written forward, input-gradient and
+    weight-gradient kernels under torch autograd) against float64 CPU
+    autograd of the same graph and loss. BF16 operands (about 2^-9 relative
+    rounding per value, through 12 layers forward and back) bound the
+    agreement: relative Frobenius error of every weight gradient within 5%,
+    cosine similarity at least 0.998."""
+    from hpm_mvs_amd import unet_prior as up
+
+    h, w = 40, 56
+    say(f"case: learned-prior network gradients {h}x{w} against float64 CPU autograd")
+    ws, bs = up.load(WEIGHTS)
+    x = torch.from_numpy(_features(up, h, w, seed=9)).to(torch.bfloat16)
+    g = torch.Generator().manual_seed(1)
+    target = torch.rand(h, w, generator=g, dtype=torch.float64)
+
+    def loss_of(d):
+        return ((d - target.to(d.device, d.dtype)).abs()).mean()
+
+    gw = [t.float().cuda().requires_grad_(True) for t in ws]
+    gb = [t.cuda().requires_grad_(True) for t in bs]
+    xd = x.cuda()
+    d, _ = up.predict(xd, up.forward(xd, gw, gb))
+    loss_of(d).backward()
+
+    cw = [t.double().requires_grad_(True) for t in ws]
+    cb = [t.double().requires_grad_(True) for t in bs]
+    F = torch.nn.functional
+    L = {name: i for i, (name, *_) in enumerate(up.LAYERS)}
+
+    def conv(name, t):
+        i = L[name]
+        _, ci, co, ks, relu = up.LAYERS[i]
+        wt = cw[i].reshape(co, ks, ks, ci).permute(0, 3, 1, 2)
+        y = F.conv2d(t.permute(2, 0, 1)[None], wt, cb[i], padding=ks // 2)[0].permute(1, 2, 0)
+        return torch.relu(y) if relu else y
+
+    def pool(t):
+        return F.avg_pool2d(t.permute(2, 0, 1)[None], 2, ceil_mode=True, count_include_pad=False)[0].permute(1, 2, 0)
+
+    def upsample(t, hh, ww):
+        return t.repeat_interleave(2, dim=0)[:hh].repeat_interleave(2, dim=1)[:, :ww]
+
+    xc = x.double()
+    s0 = conv("e0b", conv("e0a", xc))
+    s1 = conv("e1b", conv("e1a", pool(s0)))
+    s2 = conv("e2b", conv("e2a", pool(s1)))
+    b3 = conv("e3b", conv("e3a", pool(s2)))
+    d2 = conv("d2", torch.cat([upsample(b3, *s2.shape[:2]), s2], dim=2))
+    d1 = conv("d1", torch.cat([upsample(d2, *s1.shape[:2]), s1], dim=2))
+    d0 = conv("d0", torch.cat([upsample(d1, h, w), s0], dim=2))
+    dc, _ = up.predict(xc, conv("head", d0))
+    loss_of(dc).backward()
+
+    for (name, *_), a, b in zip(up.LAYERS, gw, cw):
+        ga, gb_ = a.grad.cpu().double().flatten(), b.grad.flatten()
+        rel = (ga - gb_).norm().item() / max(gb_.norm().item(), 1e-30)
+        cos = torch.dot(ga, gb_).item() / max(ga.norm().item() * gb_.norm().item(), 1e-30)
+        say(f"{name}: |grad| {gb_.norm().item():.3e}, relative error {rel:.3e}, cosine {cos:.6f}")
+        assert rel <= 5e-2 and cos >= 0.998

@@ -50,10 +50,14 @@ def test_pipeline_matches_cpu_schedule(hpm):
     pool.close()
 
 
-@pytest.mark.parametrize("mode", [[], ["--class-api"]])
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "hpm-mvs_amd", "unet_prior.bin")
+
+
+@pytest.mark.parametrize("mode", [[], ["--class-api"], ["--learned-prior", WEIGHTS]])
 def test_executable_on_folder(hpm, tmp_path, mode):
-    """The `HPM <dense_folder>` executable: device schedule, and the
-    reference's ProcessProblem written against the drop-in C++ class."""
+    """The `HPM <dense_folder>` executable: device schedule, the reference's
+    ProcessProblem written against the drop-in C++ class, and the device
+    schedule with the learned prior."""
     import subprocess
 
     exe = os.path.join(os.path.dirname(hpm.LIB_PATH), "hpm_mvs")

@@ -58,7 +58,9 @@ typedef struct hpm_dev_problem {
     float thr[3];              /* view-selection thresholds per iteration */
     float fallback[3];         /* expf(thr^2 / -0.32) */
     const float* images;
-    const float* images_pad;   /* padded pool */
+    const uint16_t* images_pad; /* padded pool, FP16 texels */
+    float one;                  /* 1.0f, opaque to the compiler: keeps p1 - p0 of two FP16 texels one
+                                   v_fma_mix_f32 (p1 * one - p0) */
     const float* depths;
     const hpm_dev_view* views;
     float* hyp;                /* float4 per pixel */

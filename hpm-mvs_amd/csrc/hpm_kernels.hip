@@ -105,12 +105,12 @@ __device__ __forceinline__ void plane_q(const hpm_dev_problem& pb, float4 n, dou
  * into an integer texel and a float fraction, plus the float coefficients of
  * the tap offsets s(p + d) - s0 = (g d) / (1 + e . d). */
 struct Warp {
-    uint32_t rowbase;     /* padded-pool element starting the row of iy0 (the pool holds < 2^30 texels) */
-    int cx;               /* ix0 + 1: padded column of ix0 */
+    uint32_t base;        /* padded-pool element of texel (ix0, iy0) (the pool holds < 2^30 texels) */
     float fx0, fy0;
     float ex, ey, gxx, gxy, gyx, gyy;
     float lox, hix, loy, hiy; /* clamp of the offsets: source coordinate in [-1, W] x [-1, H] */
     bool small;           /* |e . d| <= 0.01 over the window: cubic reciprocal */
+    bool inner;           /* small, and every tap lies inside the clamp: the clamp is the identity */
 };
 
 /* false when the warped centre leaves the source image (HPM.cu:333-335) */
@@ -137,8 +137,7 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
     }
     const double fx = floor(sx0), fy = floor(sy0);
     const int ix0 = (int)fx, iy0 = (int)fy;
-    w->rowbase = (uint32_t)sv->pad_off + (uint32_t)((iy0 + 1) * sv->pitch);
-    w->cx = ix0 + 1;
+    w->base = (uint32_t)sv->pad_off + (uint32_t)((iy0 + 1) * sv->pitch) + (uint32_t)(ix0 + 1);
     w->fx0 = (float)(sx0 - fx);
     w->fy0 = (float)(sy0 - fy);
     w->ex = (float)(H[6] * icz);
@@ -152,54 +151,80 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
     w->loy = -1.0f - (float)iy0;
     w->hiy = (float)(SH - iy0);
     w->small = (fabsf(w->ex) + fabsf(w->ey)) * 5.0f <= 0.01f;
+    /* |tap offset| <= 5 (|g.x| + |g.y|) / (1 - 0.01) for small warps; 1.0102
+     * covers the cubic's maximum (1.010101) and the FP32 roundings, 1e-3 px
+     * the roundings of these bounds */
+    const float bx = fmaf(5.0f * (fabsf(w->gxx) + fabsf(w->gxy)), 1.0102f, 1e-3f);
+    const float by = fmaf(5.0f * (fabsf(w->gyx) + fabsf(w->gyy)), 1.0102f, 1e-3f);
+    w->inner = w->small && w->fx0 - bx >= w->lox && w->fx0 + bx <= w->hix && w->fy0 - by >= w->loy &&
+               w->fy0 + by <= w->hiy;
     return true;
 }
 
 /* Source coordinate of tap (dx, dy): byte offset of the top-left texel of
  * its bilinear footprint in the padded pool and the bilinear fractions.
- * kCubic: every active lane of the wave has small warps, so 1 / (1 + u) is
- * the cubic without a per-lane choice; both forms give the same value per
- * lane. */
-template <bool kCubic>
-__device__ __forceinline__ void tap_coord(const Warp& w, uint32_t pitch, float dx, float dy, uint32_t* off,
-                                          float* fa, float* fb)
+ * kInner: every active lane of the wave has an inner warp, so 1 / (1 + u) is
+ * the cubic and the clamp is the identity, without per-lane choices; both
+ * forms give the same values per lane. */
+template <bool kInner>
+__device__ __forceinline__ void tap_coord(const Warp& w, float pitchf, float dx, float dy, uint32_t* off, float* fa,
+                                          float* fb)
 {
     const float u = fmaf(w.ex, dx, w.ey * dy);
     const float cubic = fmaf(u, fmaf(u, 1.0f - u, -1.0f), 1.0f);
-    const float inv = (kCubic || w.small) ? cubic : 1.0f / (1.0f + u);
+    const float inv = (kInner || w.small) ? cubic : 1.0f / (1.0f + u);
     float lx = fmaf(fmaf(w.gxx, dx, w.gxy * dy), inv, w.fx0);
     float ly = fmaf(fmaf(w.gyx, dx, w.gyy * dy), inv, w.fy0);
-    lx = fminf(fmaxf(lx, w.lox), w.hix);
-    ly = fminf(fmaxf(ly, w.loy), w.hiy);
+    if (!kInner) {
+        lx = fminf(fmaxf(lx, w.lox), w.hix);
+        ly = fminf(fmaxf(ly, w.loy), w.hiy);
+    }
     const float flx = floorf(lx);
     const float fly = floorf(ly);
     *fa = lx - flx;
     *fb = ly - fly;
     /* (ix0 + flx, iy0 + fly) lies in [-1, W] x [-1, H]: its footprint is
-     * inside the padded image, whose border repeats the edge texels */
-    const uint32_t qx = (uint32_t)(w.cx + (int)flx);
-    const uint32_t row = w.rowbase + (uint32_t)__mul24((int)fly, (int)pitch);
-    *off = (row + qx) * 4u;
+     * inside the padded image, whose border repeats the edge texels. The
+     * texel offset fly * pitch + flx from (ix0, iy0) is an integer below the
+     * image's padded size (< 2^24), so the FP32 fma is exact. */
+    *off = (w.base + (uint32_t)(int)fmaf(fly, pitchf, flx)) * 2u;
 }
 
-/* pair of adjacent texels of the padded layout, 4-byte aligned */
-typedef float hpm_f2u __attribute__((ext_vector_type(2), aligned(4)));
-
-/* the 2x2 texels at byte offset off: two row pairs, pitch texels apart */
-__device__ __forceinline__ float4 footprint(const float* pool, uint32_t off, uint32_t pitch)
+/* The padded pool as a buffer resource (gfx9 word 3), so that a footprint's
+ * two row pairs are two buffer loads from one VGPR offset, the second row
+ * pitch texels on through the scalar offset. */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pool_rsrc(const uint16_t* pool)
 {
-    const hpm_f2u top = *reinterpret_cast<const hpm_f2u*>(reinterpret_cast<const char*>(pool) + off);
-    const hpm_f2u bot = *reinterpret_cast<const hpm_f2u*>(reinterpret_cast<const char*>(pool) + (off + pitch * 4u));
-    return make_float4(top.x, top.y, bot.x, bot.y);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)pool, (short)0, 0x7fffffff, 0x00020000);
 }
+
+/* the 2x2 FP16 texels at byte offset off: two row pairs of adjacent texels
+ * (2-byte aligned dwords), the rows row_bytes apart */
+struct Footprint {
+    uint32_t top, bot;
+};
+
+__device__ __forceinline__ Footprint footprint(__amdgpu_buffer_rsrc_t pool, uint32_t off, uint32_t row_bytes)
+{
+    Footprint f;
+    f.top = __builtin_amdgcn_raw_buffer_load_b32(pool, off, 0, 0);
+    f.bot = __builtin_amdgcn_raw_buffer_load_b32(pool, off, row_bytes, 0);
+    return f;
+}
+
+typedef _Float16 hpm_h2 __attribute__((ext_vector_type(2)));
 
 /* linear-filtered, clamp-addressed fetch (the reference's tex2D on a
  * cudaFilterModeLinear texture, HPM.cpp:599-617) from a footprint: texels
- * (x, y), (x + 1, y), (x, y + 1), (x + 1, y + 1) */
-__device__ __forceinline__ float bilerp(float4 q, float fa, float fb)
+ * (x, y), (x + 1, y), (x, y + 1), (x + 1, y + 1), in FP32 arithmetic on the
+ * (exact) FP16 values; one = 1.0f lets each row difference be one
+ * v_fma_mix_f32 on the FP16 halves */
+__device__ __forceinline__ float bilerp(Footprint q, float fa, float fb, float one)
 {
-    const float v0 = fmaf(fa, q.y - q.x, q.x);
-    const float v1 = fmaf(fa, q.w - q.z, q.z);
+    const hpm_h2 t = __builtin_bit_cast(hpm_h2, q.top);
+    const hpm_h2 b = __builtin_bit_cast(hpm_h2, q.bot);
+    const float v0 = fmaf(fa, fmaf((float)t.y, one, -(float)t.x), (float)t.x);
+    const float v1 = fmaf(fa, fmaf((float)b.y, one, -(float)b.x), (float)b.x);
     return fmaf(fb, v1 - v0, v0);
 }
 
@@ -254,29 +279,30 @@ __device__ __forceinline__ void make_ref_patch(const float* img, int W, int H, i
 
 /* One column of the window (6 taps at x offset dx): coordinates, then the
  * 6 gathers. */
-template <bool kCubic>
-__device__ __forceinline__ void fetch_column(const float* pool, uint32_t pitch, const Warp& w, float dx, float4* quad,
-                                             float* fa, float* fb)
+template <bool kInner>
+__device__ __forceinline__ void fetch_column(__amdgpu_buffer_rsrc_t pool, uint32_t pitch, const Warp& w, float dx,
+                                             Footprint* quad, float* fa, float* fb)
 {
     uint32_t off[6];
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        tap_coord<kCubic>(w, pitch, dx, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
+        tap_coord<kInner>(w, (float)pitch, dx, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
     }
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        quad[b] = footprint(pool, off[b], pitch);
+        quad[b] = footprint(pool, off[b], pitch * 2u);
     }
 }
 
 /* Adds the 6 taps of window column a (taps 6a .. 6a + 5) to the sums, in
  * tap order; tap 14 is the shift itself. */
-__device__ __forceinline__ void accumulate_column(const RefPatch& rp, int a, float shift, const float4* quad,
-                                                  const float* fa, const float* fb, float* ss, float* sss, float* srs)
+__device__ __forceinline__ void accumulate_column(const RefPatch& rp, int a, float shift, const Footprint* quad,
+                                                  const float* fa, const float* fb, float one, float* ss, float* sss,
+                                                  float* srs)
 {
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        const float raw = (a == 2 && b == 2) ? shift : bilerp(quad[b], fa[b], fb[b]);
+        const float raw = (a == 2 && b == 2) ? shift : bilerp(quad[b], fa[b], fb[b], one);
         const float s = raw - shift;
         const float2 wk = s_refw[6 * a + b][rp.lane];
         const float ws = wk.x * s;
@@ -289,34 +315,34 @@ __device__ __forceinline__ void accumulate_column(const RefPatch& rp, int a, flo
 /* Window sums of one (hypothesis, source view), software-pipelined by
  * column: the gathers of column a + 1 are in flight while column a is
  * accumulated. Tap k = 6 a + b sits at offset (2a - 5, 2b - 5). */
-template <bool kCubic>
-__device__ __forceinline__ void ncc_sums(const float* pool, uint32_t pitch, const RefPatch& rp, const Warp& w,
-                                         float* ss_out, float* sss_out, float* srs_out)
+template <bool kInner>
+__device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, uint32_t pitch, float one, const RefPatch& rp,
+                                         const Warp& w, float* ss_out, float* sss_out, float* srs_out)
 {
     float ss = 0.0f, sss = 0.0f, srs = 0.0f;
-    float4 cur[6], nxt[6];
+    Footprint cur[6], nxt[6];
     float fa[6], fb[6], nfa[6], nfb[6];
     /* source values relative to the tap nearest the window centre (tap 14,
      * offset (-1, -1)), sampled first: a constant patch sums to exactly 0 */
     uint32_t coff;
     float cfa, cfb;
-    tap_coord<kCubic>(w, pitch, -1.0f, -1.0f, &coff, &cfa, &cfb);
-    const float4 cq = footprint(pool, coff, pitch);
-    fetch_column<kCubic>(pool, pitch, w, -5.0f, cur, fa, fb);
-    const float shift = bilerp(cq, cfa, cfb);
+    tap_coord<kInner>(w, (float)pitch, -1.0f, -1.0f, &coff, &cfa, &cfb);
+    const Footprint cq = footprint(pool, coff, pitch * 2u);
+    fetch_column<kInner>(pool, pitch, w, -5.0f, cur, fa, fb);
+    const float shift = bilerp(cq, cfa, cfb, one);
     /* two columns per trip with the buffers alternating; the last two
      * columns follow the loop, so every prefetch is unconditional and the
      * waits stay on the older loads */
 #pragma unroll 1
     for (int a = 0; a < 4; a += 2) {
-        fetch_column<kCubic>(pool, pitch, w, (float)(2 * a - 3), nxt, nfa, nfb);
-        accumulate_column(rp, a, shift, cur, fa, fb, &ss, &sss, &srs);
-        fetch_column<kCubic>(pool, pitch, w, (float)(2 * a - 1), cur, fa, fb);
-        accumulate_column(rp, a + 1, shift, nxt, nfa, nfb, &ss, &sss, &srs);
+        fetch_column<kInner>(pool, pitch, w, (float)(2 * a - 3), nxt, nfa, nfb);
+        accumulate_column(rp, a, shift, cur, fa, fb, one, &ss, &sss, &srs);
+        fetch_column<kInner>(pool, pitch, w, (float)(2 * a - 1), cur, fa, fb);
+        accumulate_column(rp, a + 1, shift, nxt, nfa, nfb, one, &ss, &sss, &srs);
     }
-    fetch_column<kCubic>(pool, pitch, w, 5.0f, nxt, nfa, nfb);
-    accumulate_column(rp, 4, shift, cur, fa, fb, &ss, &sss, &srs);
-    accumulate_column(rp, 5, shift, nxt, nfa, nfb, &ss, &sss, &srs);
+    fetch_column<kInner>(pool, pitch, w, 5.0f, nxt, nfa, nfb);
+    accumulate_column(rp, 4, shift, cur, fa, fb, one, &ss, &sss, &srs);
+    accumulate_column(rp, 5, shift, nxt, nfa, nfb, one, &ss, &sss, &srs);
     *ss_out = ss;
     *sss_out = sss;
     *srs_out = srs;
@@ -331,12 +357,14 @@ __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& 
     if (!make_warp(sv, q, px, py, &w)) {
         return 2.0f;
     }
-    const uint32_t pitch = (uint32_t)sv->pitch;
+    /* the view is wave-uniform (every caller scores one view per wave) */
+    const uint32_t pitch = (uint32_t)__builtin_amdgcn_readfirstlane(sv->pitch);
+    const __amdgpu_buffer_rsrc_t pool = pool_rsrc(pb.images_pad);
     float ss, sss, srs;
-    if (__all(w.small)) {
-        ncc_sums<true>(pb.images_pad, pitch, rp, w, &ss, &sss, &srs);
+    if (__all(w.inner)) {
+        ncc_sums<true>(pool, pitch, pb.one, rp, w, &ss, &sss, &srs);
     } else {
-        ncc_sums<false>(pb.images_pad, pitch, rp, w, &ss, &sss, &srs);
+        ncc_sums<false>(pool, pitch, pb.one, rp, w, &ss, &sss, &srs);
     }
     const float ms = ss * rp.inv_w;
     const float var_s = sss * rp.inv_w - ms * ms;

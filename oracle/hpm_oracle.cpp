@@ -20,7 +20,9 @@
  *     half an FP32 ulp), the correctly rounded quotient otherwise;
  *   - NCC window sums of both signals accumulated relative to the reference
  *     centre value: NCC is invariant to a shift of either signal, and the
- *     shifted sums keep FP32 accurate on low-texture patches.
+ *     shifted sums keep FP32 accurate on low-texture patches;
+ *   - source images sampled as their FP16-rounded texels (hpm_half.h; exact
+ *     for 8-bit grey images), the reference image at full FP32.
  * The file must be compiled with -ffp-contract=off and without fast-math.
  */
 #include "hpm_oracle.h"
@@ -35,6 +37,7 @@
 #include <omp.h>
 
 #include "hpm_fmath.h"
+#include "hpm_half.h"
 #include "hpm_philox.h"
 
 namespace {
@@ -311,6 +314,7 @@ static void make_ref_patch(const float* img, int W, int H, int px, int py, const
 
 struct Ctx {
     const hpm_oracle_problem* pb;
+    std::vector<float> src16[HPM_MAX_IMAGES]; /* source images as sampled: FP16-rounded texels */
     int W, H, nsrc;
     ViewRel rel[HPM_MAX_IMAGES];
     RefK rk;
@@ -325,7 +329,7 @@ static float ncc(const Ctx& cx, const RefPatch& rp, int px, int py, int v, const
     const hpm_oracle_problem* pb = cx.pb;
     const hpm_camera& sc = pb->cams[v];
     const int SW = sc.width, SH = sc.height;
-    const float* simg = pb->images[v];
+    const float* simg = cx.src16[v].data();
     Warp w;
     if (!make_warp(cx.rel[v], q, px, py, SW, SH, (float)(pb->params.patch_size / 2), &w)) {
         return 2.0f;
@@ -357,14 +361,13 @@ static float ncc(const Ctx& cx, const RefPatch& rp, int px, int py, int v, const
 /* ComputeBilateralNCC in double precision, reference formula (raw sums). */
 static double ncc64(const hpm_oracle_problem* pb, const Ctx& cx, int px, int py, int v, F4 plane)
 {
-    (void)cx;
     const hpm_camera& rc = pb->cams[0];
     const hpm_camera& sc = pb->cams[v];
     const hpm_params& P = pb->params;
     const int RW = rc.width, RH = rc.height;
     const int SW = sc.width, SH = sc.height;
     const float* rimg = pb->images[0];
-    const float* simg = pb->images[v];
+    const float* simg = cx.src16[v].data(); /* the texels the FP32 path samples */
     /* homography in double from the same float inputs */
     double refC[3], srcC[3], Rr[9], Cr[3], tr[3], h[9], tmp[9], H[9];
     for (int a = 0; a < 3; ++a) {
@@ -715,6 +718,13 @@ static void make_ctx(const hpm_oracle_problem* pb, Ctx* cx)
     cx->nsrc = pb->num_images - 1;
     for (int v = 1; v < pb->num_images; ++v) {
         cx->rel[v] = make_view_rel(pb->cams[0], pb->cams[v]);
+        const size_t n = (size_t)pb->cams[v].width * pb->cams[v].height;
+        std::vector<float>& dst = cx->src16[v];
+        dst.resize(n);
+        const float* src = pb->images[v];
+        for (size_t i = 0; i < n; ++i) {
+            dst[i] = hpm_round_f16(src[i]);
+        }
     }
     cx->rk = make_ref_k(pb->cams[0]);
     cx->ic = make_iter_consts();
@@ -1622,6 +1632,14 @@ extern "C" int hpm_oracle_jbu(const float* guide, int w, int h, const float* src
             out_normal[4 * center + 2] = ntot.z;
             out_normal[4 * center + 3] = 0.0f;
         }
+    }
+    return 0;
+}
+
+extern "C" int hpm_oracle_round_f16(const float* x, float* y, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        y[i] = hpm_round_f16(x[i]);
     }
     return 0;
 }

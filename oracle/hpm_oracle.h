@@ -85,6 +85,8 @@ int hpm_oracle_jbu_prior(const float* guide, int w, int h, const float* src_nd, 
 /* Host evaluation of the shared elementary functions (hpm_fmath.h):
  * fn 0 exp, 1 sin, 2 cos, 3 acos. */
 int hpm_oracle_fmath(int fn, const float* x, float* y, int n);
+/* The value a source texel is sampled as (hpm_half.h): nearest FP16. */
+int hpm_oracle_round_f16(const float* x, float* y, int n);
 /* Host evaluation of Philox4x32-10 and of the per-pixel draw stream
  * (hpm_philox.h): n uniforms of stream (seed, pixel, stream, salt). */
 int hpm_oracle_philox_block(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -105,6 +105,42 @@ def test_cost_vectors_vs_fp64(hpm, cpu, c1_full):
     pr.close()
 
 
+def test_cost_vectors_fractional_images_and_borders(hpm, cpu, c1_full):
+    """Source texels are sampled as their nearest FP16 value (exact for 8-bit
+    grey images). Images with fractional grey values, which FP16 rounds, and
+    hypotheses at every pixel within 8 of the image border, where windows
+    leave the source image and the clamp decides: bit-identical to the
+    oracle."""
+    cfg, (cams, images, gd, gn, low, order) = c1_full
+    say("case: cost vectors 640x480, fractional images, border pixels")
+    frac = [(im * np.float32(0.731) + np.float32(0.1234)).astype(np.float32) for im in images]
+    assert any(not np.array_equal(f.astype(np.float16).astype(np.float32), f) for f in frac)
+    W, H = cams[0].width, cams[0].height
+    yy, xx = np.mgrid[0:H, 0:W]
+    border = (xx < 8) | (xx >= W - 8) | (yy < 8) | (yy >= H - 8)
+    rng = np.random.default_rng(4)
+    p1, x1 = _planes_near_truth(hpm, cams, gd, gn, order, rng, 20000, 0.05)
+    bx, by = xx[border], yy[border]
+    R = np.array(cams[0].R, np.float64).reshape(3, 3)
+    K = np.array(cams[0].K, np.float64).reshape(3, 3)
+    d = gd[order[0]][by, bx].astype(np.float64) * (1.0 + 0.05 * rng.standard_normal(bx.size))
+    nc = gn[order[0]][by, bx].astype(np.float64) @ R.T + 0.05 * rng.standard_normal((bx.size, 3))
+    nc /= np.linalg.norm(nc, axis=1, keepdims=True)
+    X = np.stack([d * (bx - K[0, 2]) / K[0, 0], d * (by - K[1, 2]) / K[1, 1], d], axis=1)
+    p2 = np.concatenate([nc, -(nc * X).sum(axis=1)[:, None]], axis=1).astype(np.float32)
+    x2 = np.stack([bx, by], axis=1).astype(np.int32)
+    planes = np.concatenate([p1, p2])
+    pix = np.concatenate([x1, x2])
+    pr = make_problem(hpm, cams, frac)
+    got = pr.cost_vectors(planes, pix)
+    o = cpu.Oracle(cams, frac, pr.params())
+    want = o.cost_vectors(planes, pix, fp64=False)
+    same = got.astype(np.float32).view(np.uint32) == want.astype(np.float32).view(np.uint32)
+    say(f"identical {same.mean():.6f} of {same.size}; valid costs {(want < 2).mean():.3f}")
+    assert same.all()
+    pr.close()
+
+
 def test_stages_bitwise(hpm, cpu, c1_small):
     """Every stage started from the oracle's state gives the oracle's bits."""
     cfg, (cams, images, gd, gn, low, order) = c1_small

@@ -59,6 +59,28 @@ def test_uniform_stream(cpu):
     assert out[6] == np.float32(expect)
 
 
+def test_round_f16_against_numpy(cpu):
+    """hpm_half.h (the FP16 texels of the source pool) against numpy's IEEE
+    binary16 conversion: every FP16 value, the midpoints between neighbours
+    (ties to even), values just beside them, subnormals, overflow, the 8-bit
+    grey levels and a random sweep."""
+    L = _lib(cpu)
+    L.hpm_oracle_round_f16.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]
+    h = np.arange(0, 0x7c00, dtype=np.uint16).view(np.float16).astype(np.float32)
+    mids = ((h[:-1].astype(np.float64) + h[1:]) / 2).astype(np.float32)
+    near = np.concatenate([np.nextafter(mids, np.float32(np.inf)), np.nextafter(mids, np.float32(-np.inf))])
+    rng = np.random.default_rng(0)
+    x = np.concatenate([h, mids, near, np.arange(256, dtype=np.float32), [65504, 65519.99, 65520, 1e6, 2.0 ** -25,
+                        2.0 ** -26, 3 * 2.0 ** -26, 0.0],
+                        rng.uniform(0, 255, 200000).astype(np.float32),
+                        (rng.standard_normal(100000) * 2.0 ** rng.uniform(-30, 17, 100000)).astype(np.float32)])
+    x = np.concatenate([x, -x]).astype(np.float32)
+    y = np.empty_like(x)
+    L.hpm_oracle_round_f16(x.ctypes.data_as(C.POINTER(C.c_float)), y.ctypes.data_as(C.POINTER(C.c_float)), len(x))
+    want = x.astype(np.float16).astype(np.float32)
+    assert np.array_equal(y.view(np.uint32), want.view(np.uint32))
+
+
 @pytest.mark.parametrize("fn,lo,hi,ref,tol_ulp", [
     (0, -87.0, 88.0, np.exp, 3),
     (1, -6.3, 6.3, np.sin, 3),

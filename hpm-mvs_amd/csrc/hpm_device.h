@@ -40,10 +40,10 @@ typedef struct hpm_dev_view {
     double A[9];
     double b[3];
     hpm_camera cam;
-    int32_t pitch;      /* row pitch of the padded image (width + 3) */
+    int32_t pitch;      /* row pitch of the padded image in entries (width + 3) */
     int64_t img_off;    /* offset of this image in the image pool (elements) */
     int64_t depth_off;  /* offset of this view's depth map in the depth pool */
-    int64_t pad_off;    /* offset of texel (-1, -1) of this image in the padded pool */
+    int64_t pad_off;    /* offset of entry (-1, -1) of this image in the padded pool */
 } hpm_dev_view;
 
 typedef struct hpm_dev_problem {
@@ -58,7 +58,7 @@ typedef struct hpm_dev_problem {
     float thr[3];              /* view-selection thresholds per iteration */
     float fallback[3];         /* expf(thr^2 / -0.32) */
     const float* images;
-    const uint16_t* images_pad; /* padded pool, FP16 texels */
+    const uint32_t* images_pad; /* padded pool: entry (x, y) = FP16 texels (x, y), (x, y + 1) */
     float one;                  /* 1.0f, opaque to the compiler: keeps p1 - p0 of two FP16 texels one
                                    v_fma_mix_f32 (p1 * one - p0) */
     const float* depths;

@@ -187,28 +187,29 @@ __device__ __forceinline__ void tap_coord(const Warp& w, float pitchf, float dx,
      * inside the padded image, whose border repeats the edge texels. The
      * texel offset fly * pitch + flx from (ix0, iy0) is an integer below the
      * image's padded size (< 2^24), so the FP32 fma is exact. */
-    *off = (w.base + (uint32_t)(int)fmaf(fly, pitchf, flx)) * 2u;
+    *off = (w.base + (uint32_t)(int)fmaf(fly, pitchf, flx)) * 4u;
 }
 
-/* The padded pool as a buffer resource (gfx9 word 3), so that a footprint's
- * two row pairs are two buffer loads from one VGPR offset, the second row
- * pitch texels on through the scalar offset. */
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t pool_rsrc(const uint16_t* pool)
+/* The padded pool as a buffer resource (gfx9 word 3). */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pool_rsrc(const uint32_t* pool)
 {
     return __builtin_amdgcn_make_buffer_rsrc((void*)pool, (short)0, 0x7fffffff, 0x00020000);
 }
 
-/* the 2x2 FP16 texels at byte offset off: two row pairs of adjacent texels
- * (2-byte aligned dwords), the rows row_bytes apart */
+/* the 2x2 texels of a footprint: entry (x, y) of the pool holds the FP16
+ * texels (x, y) and (x, y + 1), so the footprint at byte offset off is the
+ * two entries (x, y), (x + 1, y): one 8-byte load */
 struct Footprint {
-    uint32_t top, bot;
+    uint32_t left, right;
 };
 
-__device__ __forceinline__ Footprint footprint(__amdgpu_buffer_rsrc_t pool, uint32_t off, uint32_t row_bytes)
+__device__ __forceinline__ Footprint footprint(__amdgpu_buffer_rsrc_t pool, uint32_t off)
 {
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 v = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(pool, off, 0, 0));
     Footprint f;
-    f.top = __builtin_amdgcn_raw_buffer_load_b32(pool, off, 0, 0);
-    f.bot = __builtin_amdgcn_raw_buffer_load_b32(pool, off, row_bytes, 0);
+    f.left = v.x;
+    f.right = v.y;
     return f;
 }
 
@@ -221,10 +222,10 @@ typedef _Float16 hpm_h2 __attribute__((ext_vector_type(2)));
  * v_fma_mix_f32 on the FP16 halves */
 __device__ __forceinline__ float bilerp(Footprint q, float fa, float fb, float one)
 {
-    const hpm_h2 t = __builtin_bit_cast(hpm_h2, q.top);
-    const hpm_h2 b = __builtin_bit_cast(hpm_h2, q.bot);
-    const float v0 = fmaf(fa, fmaf((float)t.y, one, -(float)t.x), (float)t.x);
-    const float v1 = fmaf(fa, fmaf((float)b.y, one, -(float)b.x), (float)b.x);
+    const hpm_h2 l = __builtin_bit_cast(hpm_h2, q.left);
+    const hpm_h2 r = __builtin_bit_cast(hpm_h2, q.right);
+    const float v0 = fmaf(fa, fmaf((float)r.x, one, -(float)l.x), (float)l.x);
+    const float v1 = fmaf(fa, fmaf((float)r.y, one, -(float)l.y), (float)l.y);
     return fmaf(fb, v1 - v0, v0);
 }
 
@@ -290,7 +291,7 @@ __device__ __forceinline__ void fetch_column(__amdgpu_buffer_rsrc_t pool, uint32
     }
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        quad[b] = footprint(pool, off[b], pitch * 2u);
+        quad[b] = footprint(pool, off[b]);
     }
 }
 
@@ -327,7 +328,7 @@ __device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, uint32_t p
     uint32_t coff;
     float cfa, cfb;
     tap_coord<kInner>(w, (float)pitch, -1.0f, -1.0f, &coff, &cfa, &cfb);
-    const Footprint cq = footprint(pool, coff, pitch * 2u);
+    const Footprint cq = footprint(pool, coff);
     fetch_column<kInner>(pool, pitch, w, -5.0f, cur, fa, fb);
     const float shift = bilerp(cq, cfa, cfb, one);
     /* two columns per trip with the buffers alternating; the last two
@@ -357,8 +358,7 @@ __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& 
     if (!make_warp(sv, q, px, py, &w)) {
         return 2.0f;
     }
-    /* the view is wave-uniform (every caller scores one view per wave) */
-    const uint32_t pitch = (uint32_t)__builtin_amdgcn_readfirstlane(sv->pitch);
+    const uint32_t pitch = (uint32_t)sv->pitch;
     const __amdgpu_buffer_rsrc_t pool = pool_rsrc(pb.images_pad);
     float ss, sss, srs;
     if (__all(w.inner)) {

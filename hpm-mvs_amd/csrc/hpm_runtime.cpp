@@ -290,13 +290,13 @@ extern "C" hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera
         p->off.push_back(p->total);
         p->total += (int64_t)p->cams.back().width * p->cams.back().height;
         p->pad_off.push_back(p->pad_total);
-        p->pad_total += (int64_t)(p->cams.back().width + 3) * (p->cams.back().height + 3);
+        p->pad_total += (int64_t)(p->cams.back().width + 3) * (p->cams.back().height + 2);
     }
     p->have_state.assign((size_t)num_views, 0);
-    bool fits = p->pad_total < ((int64_t)1 << 30); /* 32-bit byte offsets into the padded pool */
+    bool fits = p->pad_total < ((int64_t)1 << 29); /* 31-bit byte offsets into the padded pool */
     for (const hpm_camera& c : p->cams) {
         /* texel offsets inside one padded image are exact FP32 integers */
-        fits = fits && (int64_t)(c.width + 3) * (c.height + 3) < ((int64_t)1 << 24);
+        fits = fits && (int64_t)(c.width + 3) * (c.height + 2) < ((int64_t)1 << 24);
     }
     if (!fits) {
         delete p;
@@ -305,12 +305,12 @@ extern "C" hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera
     bool ok = hipSetDevice(p->device) == hipSuccess &&
               hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc((void**)&p->d_images, (size_t)p->total * 4) == hipSuccess &&
-              hipMalloc((void**)&p->d_images_pad, (size_t)p->pad_total * 2) == hipSuccess &&
+              hipMalloc((void**)&p->d_images_pad, (size_t)p->pad_total * 4) == hipSuccess &&
               hipMalloc((void**)&p->d_depth_plain, (size_t)p->total * 4) == hipSuccess &&
               hipMalloc((void**)&p->d_depth_geom, (size_t)p->total * 4) == hipSuccess &&
               hipMalloc((void**)&p->d_state_hyp, (size_t)p->total * 16) == hipSuccess &&
               hipMalloc((void**)&p->d_state_cost, (size_t)p->total * 4) == hipSuccess;
-    std::vector<uint16_t> padded;
+    std::vector<uint32_t> padded;
     for (int i = 0; ok && i < num_views; ++i) {
         ok = hipMemcpyAsync(p->d_images + p->off[i], p->images[i].data(), p->images[i].size() * 4,
                             hipMemcpyHostToDevice, p->stream) == hipSuccess;
@@ -319,17 +319,19 @@ extern "C" hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera
         auto at = [&](int x, int y) {
             return img[(size_t)std::min(std::max(y, 0), H - 1) * W + std::min(std::max(x, 0), W - 1)];
         };
-        /* padded FP16 copy: texel (x, y), x in [-1, W + 1], y in [-1, H + 1],
-         * at (y + 1) * (W + 3) + x + 1, borders repeating the edge texels */
+        /* padded FP16 copy with vertical pairs: entry (x, y), x in [-1, W + 1],
+         * y in [-1, H], at (y + 1) * (W + 3) + x + 1, holds texel (x, y) in
+         * its low half and (x, y + 1) in its high half; borders repeat the
+         * edge texels */
         const int pitch = W + 3;
-        padded.assign((size_t)pitch * (H + 3), 0);
-        for (int y = -1; y <= H + 1; ++y) {
-            uint16_t* dst = padded.data() + (size_t)(y + 1) * pitch;
+        padded.assign((size_t)pitch * (H + 2), 0u);
+        for (int y = -1; y <= H; ++y) {
+            uint32_t* dst = padded.data() + (size_t)(y + 1) * pitch;
             for (int x = -1; x <= W + 1; ++x) {
-                dst[x + 1] = hpm_f16_bits(at(x, y));
+                dst[x + 1] = (uint32_t)hpm_f16_bits(at(x, y)) | ((uint32_t)hpm_f16_bits(at(x, y + 1)) << 16);
             }
         }
-        ok = ok && hipMemcpy(p->d_images_pad + p->pad_off[i], padded.data(), padded.size() * 2,
+        ok = ok && hipMemcpy(p->d_images_pad + p->pad_off[i], padded.data(), padded.size() * 4,
                              hipMemcpyHostToDevice) == hipSuccess;
     }
     ok = ok && hipMemsetAsync(p->d_depth_plain, 0, (size_t)p->total * 4, p->stream) == hipSuccess &&

@@ -53,9 +53,12 @@ typedef struct hpm_run_stats {
  * depths.dmb), the last geometric-consistency depth map (which 1,
  * depths_geom.dmb) and the last hypotheses and costs. No reference
  * counterpart: the reference re-reads images and .dmb files per problem.
- * Returns NULL on bad arguments, device errors, or when the views together
- * exceed 2^30 texels (the kernels address the images with 32-bit byte
- * offsets; split larger scenes into several pools). */
+ * Returns NULL on bad arguments, device errors, or when the padded source
+ * copies, (width + 3) * (height + 2) entries per view, exceed 2^29 entries
+ * together (the kernels address them with 31-bit byte offsets; split larger
+ * scenes into several pools) or 2^24 entries in one view (about 16
+ * megapixels). Source texels are sampled as their nearest FP16 values
+ * (exact for 8-bit grey images). */
 hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera* cams, const float* const* images,
                           int max_image_size);
 void hpm_pool_destroy(hpm_pool* pool);

@@ -78,8 +78,9 @@ int grow(hpm_batch* b, T** p, size_t* cap, size_t count)
     return HPM_OK;
 }
 
-/* tiles of the full grid (x in [0, W), y in [0, H)) and of the colour grid
- * (x = 2i + parity, rows below min(H, 2 yy_limit)) of every problem, 64x4 */
+/* tiles of the full grid (x in [0, W), y in [0, H); 64x4) and of the colour
+ * grid (x = 2i + parity, rows below min(H, 2 yy_limit); HPM_COLOUR_TILE_X x
+ * HPM_COLOUR_TILE_Y) of every problem */
 void make_tiles(const std::vector<hpm_dev_problem>& pbs, int n, std::vector<hpm_tile>* full,
                 std::vector<hpm_tile>* colour)
 {
@@ -94,8 +95,8 @@ void make_tiles(const std::vector<hpm_dev_problem>& pbs, int n, std::vector<hpm_
         }
         const int half_w = (pb.W + 1) / 2;
         const int rows = std::min(pb.H, 2 * pb.yy_limit);
-        for (int by = 0; by < (rows + 3) / 4; ++by) {
-            for (int bx = 0; bx < (half_w + 63) / 64; ++bx) {
+        for (int by = 0; by < (rows + HPM_COLOUR_TILE_Y - 1) / HPM_COLOUR_TILE_Y; ++by) {
+            for (int bx = 0; bx < (half_w + HPM_COLOUR_TILE_X - 1) / HPM_COLOUR_TILE_X; ++bx) {
                 colour->push_back({i, bx, by, 0});
             }
         }

@@ -71,7 +71,17 @@ typedef struct hpm_dev_problem {
     unsigned long long* n_eval; /* NCC evaluation counter (may be null) */
 } hpm_dev_problem;
 
-/* One 64x4 tile of a problem in the batched kernels. */
+/* Colour-grid mapping of the checkerboard kernels: a wave (64 lanes) takes
+ * every second pixel of HPM_WAVE_ROWS consecutive rows, 64 / HPM_WAVE_ROWS
+ * lanes per row; a 256-lane block four such waves stacked. */
+#ifndef HPM_WAVE_ROWS
+#define HPM_WAVE_ROWS 2
+#endif
+#define HPM_COLOUR_TILE_X (64 / HPM_WAVE_ROWS)  /* colour-grid columns (pixel pairs) per block */
+#define HPM_COLOUR_TILE_Y (4 * HPM_WAVE_ROWS)   /* rows per block */
+
+/* One tile of a problem in the batched kernels: 64x4 pixels of the full
+ * grid, or HPM_COLOUR_TILE_X x HPM_COLOUR_TILE_Y of the colour grid. */
 typedef struct hpm_tile {
     int32_t problem;  /* index into the problem table */
     int32_t bx, by;   /* tile of the problem's full grid (init, finalize) or colour grid (propagate, filter) */

@@ -1356,10 +1356,12 @@ __device__ __forceinline__ bool colour_pixel_at(const hpm_dev_problem& pb, int c
 {
     /* pixel (x, y) has colour (x + y) & 1; the launch of HPM.cu:1924-1931
      * visits rows y = 2 yy + parity with yy < yy_limit. Here a wave takes
-     * every second pixel of one row, so its lanes sample neighbouring
-     * source columns. */
-    const int y = by * blockDim.y + threadIdx.y;
-    const int x = 2 * (bx * blockDim.x + threadIdx.x) + ((y + colour) & 1);
+     * every second pixel of HPM_WAVE_ROWS consecutive rows, so its lanes
+     * sample neighbouring source columns and the windows of its rows
+     * overlap. */
+    const int lane = (int)threadIdx.x; /* blockDim = (64, 4) */
+    const int y = (by * (int)blockDim.y + (int)threadIdx.y) * HPM_WAVE_ROWS + lane / HPM_COLOUR_TILE_X;
+    const int x = 2 * (bx * HPM_COLOUR_TILE_X + lane % HPM_COLOUR_TILE_X) + ((y + colour) & 1);
     *px = x;
     *py = y;
     return (y >> 1) < pb.yy_limit && x < pb.W && y < pb.H;
@@ -1790,11 +1792,12 @@ static dim3 full_grid(int W, int H, dim3 blk)
     return dim3((W + blk.x - 1) / blk.x, (H + blk.y - 1) / blk.y, 1);
 }
 
-static dim3 colour_grid(const hpm_dev_problem* pb, dim3 blk)
+static dim3 colour_grid(const hpm_dev_problem* pb)
 {
     const int half_w = (pb->W + 1) / 2;
     const int rows = pb->H < 2 * pb->yy_limit ? pb->H : 2 * pb->yy_limit;
-    return dim3((half_w + blk.x - 1) / blk.x, (rows + blk.y - 1) / blk.y, 1);
+    return dim3((half_w + HPM_COLOUR_TILE_X - 1) / HPM_COLOUR_TILE_X,
+                (rows + HPM_COLOUR_TILE_Y - 1) / HPM_COLOUR_TILE_Y, 1);
 }
 
 extern "C" int hpm_launch_init(const hpm_dev_problem* pb, void* stream)
@@ -1807,7 +1810,7 @@ extern "C" int hpm_launch_init(const hpm_dev_problem* pb, void* stream)
 extern "C" int hpm_launch_propagate(const hpm_dev_problem* pb, int iter, int colour, void* stream)
 {
     const dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(hpm_k_propagate, colour_grid(pb, blk), blk, 0, (hipStream_t)stream, *pb, iter, colour);
+    hipLaunchKernelGGL(hpm_k_propagate, colour_grid(pb), blk, 0, (hipStream_t)stream, *pb, iter, colour);
     return (int)hipGetLastError();
 }
 
@@ -1821,7 +1824,7 @@ extern "C" int hpm_launch_finalize(const hpm_dev_problem* pb, void* stream)
 extern "C" int hpm_launch_filter(const hpm_dev_problem* pb, int colour, void* stream)
 {
     const dim3 blk(64, 4, 1);
-    hipLaunchKernelGGL(hpm_k_filter, colour_grid(pb, blk), blk, 0, (hipStream_t)stream, *pb, colour);
+    hipLaunchKernelGGL(hpm_k_filter, colour_grid(pb), blk, 0, (hipStream_t)stream, *pb, colour);
     return (int)hipGetLastError();
 }
 

@@ -726,27 +726,29 @@ __device__ __forceinline__ float selected_sum(const float* vw, const float* term
     return acc;
 }
 
-/* Jobs of the per-lane evaluation machine of propagate_pixel. Every NCC
- * evaluation of a pixel update belongs to one job: a hypothesis and the set
- * of source views it is scored on. */
-enum : int {
-    kJobDir = 0,      /* 0..7: neighbour hypothesis of direction d, every view */
-    kJobExt = 8,      /* NESP extension of direction 0, every view */
-    kJobRescore = 9,  /* current hypothesis, selected views */
-    kJobRefine = 10,  /* 10..14: refinement candidates, selected views */
-    kJobDone = 15
-};
+/* the source view most lanes of the wave still need (lowest index on ties) */
+__device__ __forceinline__ int majority_view(uint32_t todo, int nsrc)
+{
+    int vi = 0, best = 0;
+    for (int i = 0; i < nsrc; ++i) {
+        const int n = __popcll(__ballot((todo >> i) & 1u));
+        if (n > best) {
+            best = n;
+            vi = i;
+        }
+    }
+    return vi;
+}
 
 /* CheckerboardPropagation (HPM.cu:1222-1725) + PlaneHypothesisRefinement
  * (HPM.cu:1143-1220).
  *
  * The reference scores hypotheses phase by phase (eight neighbours, the
  * NESP extension, the current hypothesis, five refinement candidates); here
- * each lane walks the same sequence as a queue of jobs, and every loop trip
- * scores one source view for each lane whose current job still needs it, so
- * a wave runs one NCC code path while its lanes may sit in different phases.
- * Per-view results are stored and summed in view order, and the decisions
- * between phases run when a lane's queue reaches them, so every value and
+ * the wave runs the phases in step, so the decision code between them (view
+ * selection, adoption, candidate generation, the refinement decisions) runs
+ * once per wave rather than once for every group of lanes that reaches it.
+ * Per-view results are stored and summed in view order, so every value and
  * every sum order is the sequential one. */
 __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, int px, int py, int iter, int colour)
 {
@@ -917,357 +919,341 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
         }
     }
 
-    /* the rest runs as a job machine: the extension rounds, the rescore and
-     * the refinement candidates take a different number of evaluations on
-     * every lane */
-    int job = kJobDir + 7;
-    int ext_k = 0;
-    uint32_t todo = 0u;
-    float4 jplane = cur;
-    bool fresh = false; /* q not yet computed for the current job */
-    float jc[HPM_MAX_SOURCES]; /* per-view terms of a rescore / refinement job */
+    /* The rest: up to three NESP extension rounds, joint view selection,
+     * the rescore of the current hypothesis, adoption and five refinement
+     * candidates. The phases run in step across the wave, so each decision
+     * runs once per wave; inside the rescore and refinement phases every
+     * trip scores one source view (the one most lanes still need) for the
+     * lanes that selected it. Values, sum orders and decisions per lane are
+     * those of the sequential order. */
+    float jc[HPM_MAX_SOURCES];    /* rescore terms per view */
+    float jr[5][HPM_MAX_SOURCES]; /* refinement terms per candidate and view */
 
-    for (;;) {
-        /* finish the lane's job and start the next while no view is left */
-        while (todo == 0u && job != kJobDone) {
-            if (job == kJobDir + 7 || job == kJobExt) {
-                if (job == kJobExt) {
-                    ++ext_k;
+    /* NESP extension, orientation 0 only (HPM.cu:1473-1543, 780-843): up to
+     * 3 rounds, each judged on ca[0] */
+    bool ext_on = flag[0];
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {
+        bool extend = false;
+        if (ext_on) {
+            /* JudgeExtend (HPM.cu:1036-1059) */
+            const float good = pb.good[iter][k];
+            int good_sum = 0, bad_sum = 0;
+            for (int i = 0; i < nsrc; ++i) {
+                if (ca[0][i] < good) {
+                    ++good_sum;
                 }
-                /* NESP extension, orientation 0 only (HPM.cu:1473-1543,
-                 * 780-843): up to 3 rounds, each judged on ca[0] */
-                bool extend = false;
-                while (ext_k < 3 && flag[0]) {
-                    /* JudgeExtend (HPM.cu:1036-1059) */
-                    const float good = pb.good[iter][ext_k];
-                    int good_sum = 0, bad_sum = 0;
-                    for (int i = 0; i < nsrc; ++i) {
-                        if (ca[0][i] < good) {
-                            ++good_sum;
+                if ((double)ca[0][i] > 1.2) {
+                    ++bad_sum;
+                }
+            }
+            if (good_sum >= 1 && bad_sum <= 2) {
+                ext_on = false;
+            } else {
+                int lu = center - 5 - 6 * W - 4 * 2 * k - 4 * 2 * W * k;
+                if (px > 4 + 8 * k && py > 5 + 8 * k) {
+                    bool symbol = true;
+                    float cmin = cost[lu];
+                    int cmp = lu;
+                    int qx = px - 5 - 8 * k, qy = py - 6 - 8 * k;
+                    for (int i = 0; i < 7; i++) {
+                        if (i % 2 == 0) {
+                            qx -= 2;
+                        } else {
+                            qy -= 2;
                         }
-                        if ((double)ca[0][i] > 1.2) {
-                            ++bad_sum;
-                        }
-                    }
-                    if (good_sum >= 1 && bad_sum <= 2) {
-                        break;
-                    }
-                    const int k = ext_k;
-                    int lu = center - 5 - 6 * W - 4 * 2 * k - 4 * 2 * W * k;
-                    if (px > 4 + 8 * k && py > 5 + 8 * k) {
-                        bool symbol = true;
-                        float cmin = cost[lu];
-                        int cmp = lu;
-                        int qx = px - 5 - 8 * k, qy = py - 6 - 8 * k;
-                        for (int i = 0; i < 7; i++) {
-                            if (i % 2 == 0) {
-                                qx -= 2;
-                            } else {
-                                qy -= 2;
+                        if (qx >= 0 && qx < W && qy < H && qy >= 0) {
+                            const int pt = qy * W + qx;
+                            if (cost[pt] < cmin) {
+                                cmin = cost[pt];
+                                cmp = pt;
                             }
-                            if (qx >= 0 && qx < W && qy < H && qy >= 0) {
-                                const int pt = qy * W + qx;
-                                if (cost[pt] < cmin) {
-                                    cmin = cost[pt];
-                                    cmp = pt;
-                                }
-                            } else {
-                                break;
-                            }
-                        }
-                        if (cmin > cost[pos[0]]) {
-                            cmp = pos[0];
-                            symbol = false;
-                        }
-                        lu = cmp;
-                        extend = symbol && lu != pos[0];
-                    } else {
-                        lu = pos[0];
-                    }
-                    pos[0] = lu;
-                    if (extend) {
-                        break;
-                    }
-                    ++ext_k;
-                }
-                if (extend) {
-                    job = kJobExt;
-                    jplane = ld4(pb.hyp, pos[0]);
-                    todo = all_views;
-                    fresh = true;
-                    continue;
-                }
-
-                /* joint view selection (HPM.cu:1548-1613) */
-                float probs[HPM_MAX_SOURCES];
-                for (int i = 0; i < nsrc; ++i) {
-                    vw[i] = 0.0f;
-                    probs[i] = 0.0f;
-                }
-                {
-                    float vprior[HPM_MAX_SOURCES];
-                    for (int i = 0; i < nsrc; ++i) {
-                        vprior[i] = 0.0f;
-                    }
-                    const int nb[4] = {center - W * 5, center + W * 5, center - 5, center + 5};
-                    for (int i = 0; i < 4; ++i) {
-                        if (flag[2 * i + 1]) {
-                            const uint32_t m = pb.sel[nb[i]];
-                            for (int j = 0; j < nsrc; ++j) {
-                                if (((m >> j) & 1u) == 1u) {
-                                    vprior[j] += 0.9f;
-                                } else {
-                                    vprior[j] += 0.1f;
-                                }
-                            }
-                        }
-                    }
-                    const float thr = pb.thr[iter];
-                    for (int i = 0; i < nsrc; i++) {
-                        float count = 0.0f;
-                        int count_false = 0;
-                        float tmpw = 0.0f;
-                        for (int j = 0; j < 8; j++) {
-                            const float c = ca[j][i];
-                            if (c < thr) {
-                                tmpw += hpm_expf(c * c / (-0.18f));
-                                count++;
-                            }
-                            if (c > 1.2f) {
-                                count_false++;
-                            }
-                        }
-                        float pr = 0.0f;
-                        if (count > 2 && count_false < 3) {
-                            pr = tmpw / count;
-                        } else if (count_false < 3) {
-                            pr = pb.fallback[iter];
-                        }
-                        probs[i] = pr * vprior[i];
-                    }
-                }
-                {
-                    /* TransformPDFToCDF (HPM.cu:86-100) */
-                    float sum = 0.0f;
-                    for (int i = 0; i < nsrc; ++i) {
-                        sum += probs[i];
-                    }
-                    const float inv = 1.0f / sum;
-                    float cum = 0.0f;
-                    for (int i = 0; i < nsrc; ++i) {
-                        cum += probs[i] * inv;
-                        probs[i] = cum;
-                    }
-                }
-                for (int s = 0; s < 15; ++s) {
-                    const float u = hpm_rng_uniform(&rng) - FLT_EPSILON;
-                    for (int v = 0; v < nsrc; ++v) {
-                        if (probs[v] > u) {
-                            vw[v] += 1.0f;
+                        } else {
                             break;
                         }
                     }
-                }
-                for (int i = 0; i < nsrc; ++i) {
-                    if (vw[i] > 0) {
-                        tsel |= (1u << i);
-                        wn += vw[i];
+                    if (cmin > cost[pos[0]]) {
+                        cmp = pos[0];
+                        symbol = false;
                     }
-                }
-                sel_views = tsel;
-
-                /* candidate scoring (HPM.cu:1615-1635) */
-                for (int i = 0; i < 8; ++i) {
-                    float a = 0.0f;
-                    float4 hp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    if (P.geom_consistency && flag[i]) {
-                        hp = ld4(pb.hyp, pos[i]);
-                    }
-                    for (int j = 0; j < nsrc; ++j) {
-                        if (vw[j] > 0) {
-                            if (P.geom_consistency) {
-                                if (flag[i]) {
-                                    a += vw[j] * (ca[i][j] + 0.1f * geom_cost(pb, j + 1, hp, px, py));
-                                } else {
-                                    a += vw[j] * (ca[i][j] + 0.1f * 5.0f);
-                                }
-                            } else {
-                                a += vw[j] * ca[i][j];
-                            }
-                        }
-                    }
-                    fc[i] = a / wn;
-                }
-                min_idx = find_min_idx8(fc);
-
-                /* re-score the current hypothesis on the selected views
-                 * (HPM.cu:1637-1650) */
-                job = kJobRescore;
-                jplane = cur;
-                todo = sel_views;
-                fresh = true;
-                continue;
-            }
-            if (job == kJobRescore) {
-                cost_now = selected_sum(vw, jc, nsrc) / wn;
-                cost_center = cost_now;
-                plane_center = cur;
-                sel_center = pb.sel[center];
-                depth_now = depth_from_plane(rc, cur, px, py);
-
-                /* adoption (HPM.cu:1652-1722) */
-                if (prior_on) {
-                    prior_plane = ld4(pb.prior_planes, center);
-                    depth_prior = depth_from_plane(rc, prior_plane, px, py);
-                    if (prior_pix) {
-                        float rfc[8];
-#pragma unroll
-                        for (int i = 0; i < 8; i++) {
-                            rfc[i] = 0.0f;
-                            if (flag[i]) {
-                                const float4 hp = ld4(pb.hyp, pos[i]);
-                                const float dn = depth_from_plane(rc, hp, px, py);
-                                const float pr = prior_term(pk, dn - depth_prior, prior_plane, hp);
-                                rfc[i] = hpm_expf(-fc[i] * fc[i] / pk.beta) * pr;
-                            }
-                        }
-                        const int max_idx = find_max_idx8(rfc);
-                        const float dn = depth_from_plane(rc, cur, px, py);
-                        const float pr = prior_term(pk, dn - depth_prior, prior_plane, cur);
-                        const float rcn = hpm_expf(-cost_now * cost_now / pk.beta) * pr;
-                        if (flag[max_idx]) {
-                            const float4 hp = ld4(pb.hyp, pos[max_idx]);
-                            const float db = depth_from_plane(rc, hp, px, py);
-                            if (db >= P.depth_min && db <= P.depth_max && rfc[max_idx] > rcn) {
-                                depth_now = db;
-                                plane_center = hp;
-                                cost_center = fc[max_idx];
-                                restricted_cost = rfc[max_idx];
-                                sel_center = tsel;
-                            }
-                        }
-                    } else if (flag[min_idx]) {
-                        const float4 hp = ld4(pb.hyp, pos[min_idx]);
-                        const float db = depth_from_plane(rc, hp, px, py);
-                        if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
-                            depth_now = db;
-                            plane_center = hp;
-                            cost_center = fc[min_idx];
-                        }
-                    }
-                }
-                if (!prior_on && flag[min_idx]) {
-                    const float4 hp = ld4(pb.hyp, pos[min_idx]);
-                    const float db = depth_from_plane(rc, hp, px, py);
-                    if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
-                        depth_now = db;
-                        plane_center = hp;
-                        cost_center = fc[min_idx];
-                        sel_center = tsel;
-                    }
-                    if (cost_center != cost_center) {
-                        depth_now = db;
-                        plane_center = hp;
-                        cost_now = fc[min_idx];
-                        sel_center = tsel;
-                    }
-                }
-
-                /* PlaneHypothesisRefinement candidates (HPM.cu:1143-1190) */
-                {
-                    const float perturbation = 0.02f;
-                    float depth_rand;
-                    float4 n_rand;
-                    const float ds = (P.depth_max - P.depth_min) / 64.0f;
-                    const float as = (float)(kPi * (double)(5.0f / 180.0f));
-                    if (prior_pix) {
-                        depth_rand = hpm_rng_uniform(&rng) * 6 * ds + (depth_prior - 3 * ds);
-                        n_rand = perturbed_normal(rc, px, py, prior_plane, &rng, as);
-                    } else {
-                        depth_rand = hpm_rng_uniform(&rng) * (P.depth_max - P.depth_min) + P.depth_min;
-                        n_rand = random_normal(rc, px, py, &rng, depth_now);
-                    }
-                    const float dmin_p = (1 - perturbation) * depth_now;
-                    const float dmax_p = (1 + perturbation) * depth_now;
-                    const float depth_pert = hpm_rng_uniform(&rng) * (dmax_p - dmin_p) + dmin_p;
-                    const float4 plane_now = plane_center;
-                    const float4 n_pert =
-                        perturbed_normal(rc, px, py, plane_now, &rng, (float)((double)perturbation * kPi));
-                    const float dnow = depth_now;
-#pragma unroll
-                    for (int i = 0; i < 5; ++i) {
-                        const float di = (i == 0 || i == 2) ? depth_rand : (i == 4 ? depth_pert : dnow);
-                        float4 t = (i == 0 || i == 4) ? plane_now : (i == 3 ? n_pert : n_rand);
-                        t.w = dist_to_origin(rc, px, py, di, t);
-                        cand[i] = t;
-                        cand_d[i] = di;
-                    }
-                }
-                job = kJobRefine;
-                jplane = cand[0];
-                todo = sel_views;
-                fresh = true;
-                continue;
-            }
-            /* refinement candidate job - kJobRefine scored (HPM.cu:1191-1218) */
-            {
-                const int i = job - kJobRefine;
-                const float4 t = jplane;
-                const float tc = selected_sum(vw, jc, nsrc) / wn;
-                const float db = depth_from_plane(rc, t, px, py);
-                if (prior_pix) {
-                    const float pr = prior_term(pk, cand_d[i] - depth_prior, prior_plane, t);
-                    const float rtc = hpm_expf(-tc * tc / pk.beta) * pr;
-                    if (db >= P.depth_min && db <= P.depth_max && rtc > restricted_cost) {
-                        depth_now = db;
-                        plane_center = t;
-                        cost_center = tc;
-                        restricted_cost = rtc;
-                    }
+                    lu = cmp;
+                    extend = symbol && lu != pos[0];
                 } else {
-                    if (db >= P.depth_min && db <= P.depth_max && tc < cost_center) {
-                        depth_now = db;
-                        plane_center = t;
-                        cost_center = tc;
+                    lu = pos[0];
+                }
+                pos[0] = lu;
+            }
+        }
+        if (__any(extend)) {
+            if (extend) {
+                plane_q(pb, ld4(pb.hyp, pos[0]), q);
+                nev += (uint32_t)nsrc;
+            }
+#pragma unroll 1
+            for (int v = 0; v < nsrc; ++v) {
+                if (extend) {
+                    ca[0][v] = ncc(pb, rp, px, py, v + 1, q);
+                }
+            }
+        }
+    }
+
+    {
+        /* joint view selection (HPM.cu:1548-1613) */
+        float probs[HPM_MAX_SOURCES];
+        for (int i = 0; i < nsrc; ++i) {
+            vw[i] = 0.0f;
+            probs[i] = 0.0f;
+        }
+        {
+            float vprior[HPM_MAX_SOURCES];
+            for (int i = 0; i < nsrc; ++i) {
+                vprior[i] = 0.0f;
+            }
+            const int nb[4] = {center - W * 5, center + W * 5, center - 5, center + 5};
+            for (int i = 0; i < 4; ++i) {
+                if (flag[2 * i + 1]) {
+                    const uint32_t m = pb.sel[nb[i]];
+                    for (int j = 0; j < nsrc; ++j) {
+                        if (((m >> j) & 1u) == 1u) {
+                            vprior[j] += 0.9f;
+                        } else {
+                            vprior[j] += 0.1f;
+                        }
                     }
                 }
-                if (i < 4) {
-                    ++job;
-                    jplane = cand[i + 1];
-                    todo = sel_views;
-                    fresh = true;
-                    } else {
-                    job = kJobDone;
+            }
+            const float thr = pb.thr[iter];
+            for (int i = 0; i < nsrc; i++) {
+                float count = 0.0f;
+                int count_false = 0;
+                float tmpw = 0.0f;
+                for (int j = 0; j < 8; j++) {
+                    const float c = ca[j][i];
+                    if (c < thr) {
+                        tmpw += hpm_expf(c * c / (-0.18f));
+                        count++;
+                    }
+                    if (c > 1.2f) {
+                        count_false++;
+                    }
+                }
+                float pr = 0.0f;
+                if (count > 2 && count_false < 3) {
+                    pr = tmpw / count;
+                } else if (count_false < 3) {
+                    pr = pb.fallback[iter];
+                }
+                probs[i] = pr * vprior[i];
+            }
+        }
+        {
+            /* TransformPDFToCDF (HPM.cu:86-100) */
+            float sum = 0.0f;
+            for (int i = 0; i < nsrc; ++i) {
+                sum += probs[i];
+            }
+            const float inv = 1.0f / sum;
+            float cum = 0.0f;
+            for (int i = 0; i < nsrc; ++i) {
+                cum += probs[i] * inv;
+                probs[i] = cum;
+            }
+        }
+        for (int s = 0; s < 15; ++s) {
+            const float u = hpm_rng_uniform(&rng) - FLT_EPSILON;
+            for (int v = 0; v < nsrc; ++v) {
+                if (probs[v] > u) {
+                    vw[v] += 1.0f;
+                    break;
                 }
             }
         }
-        if (job == kJobDone) {
-            break;
-        }
-        if (fresh) { /* first view of a job: q of its plane */
-            plane_q(pb, jplane, q);
-            fresh = false;
-        }
-        /* the wave scores one source view per trip: the one most of its
-         * lanes still need, so all gathers of a trip hit one image */
-        int vi = 0, best = 0;
         for (int i = 0; i < nsrc; ++i) {
-            const int n = __popcll(__ballot((todo >> i) & 1u));
-            if (n > best) {
-                best = n;
-                vi = i;
+            if (vw[i] > 0) {
+                tsel |= (1u << i);
+                wn += vw[i];
             }
         }
-        if ((todo >> vi) & 1u) {
-            todo &= ~(1u << vi);
-            const float c = ncc(pb, rp, px, py, vi + 1, q);
-            ++nev;
-            if (job == kJobExt) {
-                ca[0][vi] = c;
-            } else if (P.geom_consistency) {
-                jc[vi] = c + 0.1f * geom_cost(pb, vi + 1, jplane, px, py);
+        sel_views = tsel;
+
+        /* candidate scoring (HPM.cu:1615-1635) */
+        for (int i = 0; i < 8; ++i) {
+            float a = 0.0f;
+            float4 hp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (P.geom_consistency && flag[i]) {
+                hp = ld4(pb.hyp, pos[i]);
+            }
+            for (int j = 0; j < nsrc; ++j) {
+                if (vw[j] > 0) {
+                    if (P.geom_consistency) {
+                        if (flag[i]) {
+                            a += vw[j] * (ca[i][j] + 0.1f * geom_cost(pb, j + 1, hp, px, py));
+                        } else {
+                            a += vw[j] * (ca[i][j] + 0.1f * 5.0f);
+                        }
+                    } else {
+                        a += vw[j] * ca[i][j];
+                    }
+                }
+            }
+            fc[i] = a / wn;
+        }
+        min_idx = find_min_idx8(fc);
+    }
+
+    /* re-score the current hypothesis on the selected views
+     * (HPM.cu:1637-1650) */
+    plane_q(pb, cur, q);
+    {
+        uint32_t todo = sel_views;
+        while (__any(todo != 0u)) {
+            const int vi = majority_view(todo, nsrc);
+            if ((todo >> vi) & 1u) {
+                todo &= ~(1u << vi);
+                const float c = ncc(pb, rp, px, py, vi + 1, q);
+                ++nev;
+                jc[vi] = P.geom_consistency ? c + 0.1f * geom_cost(pb, vi + 1, cur, px, py) : c;
+            }
+        }
+    }
+
+    {
+        cost_now = selected_sum(vw, jc, nsrc) / wn;
+        cost_center = cost_now;
+        plane_center = cur;
+        sel_center = pb.sel[center];
+        depth_now = depth_from_plane(rc, cur, px, py);
+
+        /* adoption (HPM.cu:1652-1722) */
+        if (prior_on) {
+            prior_plane = ld4(pb.prior_planes, center);
+            depth_prior = depth_from_plane(rc, prior_plane, px, py);
+            if (prior_pix) {
+                float rfc[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    rfc[i] = 0.0f;
+                    if (flag[i]) {
+                        const float4 hp = ld4(pb.hyp, pos[i]);
+                        const float dn = depth_from_plane(rc, hp, px, py);
+                        const float pr = prior_term(pk, dn - depth_prior, prior_plane, hp);
+                        rfc[i] = hpm_expf(-fc[i] * fc[i] / pk.beta) * pr;
+                    }
+                }
+                const int max_idx = find_max_idx8(rfc);
+                const float dn = depth_from_plane(rc, cur, px, py);
+                const float pr = prior_term(pk, dn - depth_prior, prior_plane, cur);
+                const float rcn = hpm_expf(-cost_now * cost_now / pk.beta) * pr;
+                if (flag[max_idx]) {
+                    const float4 hp = ld4(pb.hyp, pos[max_idx]);
+                    const float db = depth_from_plane(rc, hp, px, py);
+                    if (db >= P.depth_min && db <= P.depth_max && rfc[max_idx] > rcn) {
+                        depth_now = db;
+                        plane_center = hp;
+                        cost_center = fc[max_idx];
+                        restricted_cost = rfc[max_idx];
+                        sel_center = tsel;
+                    }
+                }
+            } else if (flag[min_idx]) {
+                const float4 hp = ld4(pb.hyp, pos[min_idx]);
+                const float db = depth_from_plane(rc, hp, px, py);
+                if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
+                    depth_now = db;
+                    plane_center = hp;
+                    cost_center = fc[min_idx];
+                }
+            }
+        }
+        if (!prior_on && flag[min_idx]) {
+            const float4 hp = ld4(pb.hyp, pos[min_idx]);
+            const float db = depth_from_plane(rc, hp, px, py);
+            if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
+                depth_now = db;
+                plane_center = hp;
+                cost_center = fc[min_idx];
+                sel_center = tsel;
+            }
+            if (cost_center != cost_center) {
+                depth_now = db;
+                plane_center = hp;
+                cost_now = fc[min_idx];
+                sel_center = tsel;
+            }
+        }
+
+        /* PlaneHypothesisRefinement candidates (HPM.cu:1143-1190) */
+        {
+            const float perturbation = 0.02f;
+            float depth_rand;
+            float4 n_rand;
+            const float ds = (P.depth_max - P.depth_min) / 64.0f;
+            const float as = (float)(kPi * (double)(5.0f / 180.0f));
+            if (prior_pix) {
+                depth_rand = hpm_rng_uniform(&rng) * 6 * ds + (depth_prior - 3 * ds);
+                n_rand = perturbed_normal(rc, px, py, prior_plane, &rng, as);
             } else {
-                jc[vi] = c;
+                depth_rand = hpm_rng_uniform(&rng) * (P.depth_max - P.depth_min) + P.depth_min;
+                n_rand = random_normal(rc, px, py, &rng, depth_now);
+            }
+            const float dmin_p = (1 - perturbation) * depth_now;
+            const float dmax_p = (1 + perturbation) * depth_now;
+            const float depth_pert = hpm_rng_uniform(&rng) * (dmax_p - dmin_p) + dmin_p;
+            const float4 plane_now = plane_center;
+            const float4 n_pert =
+                perturbed_normal(rc, px, py, plane_now, &rng, (float)((double)perturbation * kPi));
+            const float dnow = depth_now;
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                const float di = (i == 0 || i == 2) ? depth_rand : (i == 4 ? depth_pert : dnow);
+                float4 t = (i == 0 || i == 4) ? plane_now : (i == 3 ? n_pert : n_rand);
+                t.w = dist_to_origin(rc, px, py, di, t);
+                cand[i] = t;
+                cand_d[i] = di;
+            }
+        }
+    }
+
+    /* the refinement candidates on the selected views, all five on one view
+     * in turn (HPM.cu:1191-1200) */
+    {
+        uint32_t todo = sel_views;
+        while (__any(todo != 0u)) {
+            const int vi = majority_view(todo, nsrc);
+            if ((todo >> vi) & 1u) {
+                todo &= ~(1u << vi);
+#pragma unroll 1
+                for (int c = 0; c < 5; ++c) {
+                    const float4 t = cand[c];
+                    plane_q(pb, t, q);
+                    const float cc = ncc(pb, rp, px, py, vi + 1, q);
+                    jr[c][vi] = P.geom_consistency ? cc + 0.1f * geom_cost(pb, vi + 1, t, px, py) : cc;
+                }
+                nev += 5u;
+            }
+        }
+    }
+
+    /* the candidates decided in order (HPM.cu:1191-1218) */
+    for (int i = 0; i < 5; ++i) {
+        const float4 t = cand[i];
+        const float tc = selected_sum(vw, jr[i], nsrc) / wn;
+        const float db = depth_from_plane(rc, t, px, py);
+        if (prior_pix) {
+            const float pr = prior_term(pk, cand_d[i] - depth_prior, prior_plane, t);
+            const float rtc = hpm_expf(-tc * tc / pk.beta) * pr;
+            if (db >= P.depth_min && db <= P.depth_max && rtc > restricted_cost) {
+                depth_now = db;
+                plane_center = t;
+                cost_center = tc;
+                restricted_cost = rtc;
+            }
+        } else {
+            if (db >= P.depth_min && db <= P.depth_max && tc < cost_center) {
+                depth_now = db;
+                plane_center = t;
+                cost_center = tc;
             }
         }
     }

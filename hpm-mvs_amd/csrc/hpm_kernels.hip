@@ -713,31 +713,15 @@ __device__ __forceinline__ float prior_term(const PriorConsts& k, float depth_di
     return k.gamma + hpm_expf(-depth_diff * depth_diff / k.tdss) * hpm_expf(-ad * ad / k.tass);
 }
 
-/* sum over the selected views (vw > 0) in view order of vw * term: the
- * weighted costs of HPM.cu:1637-1650 and 1191-1200 */
-__device__ __forceinline__ float selected_sum(const float* vw, const float* term, int nsrc)
+/* the lowest source view some lane of the wave still needs */
+__device__ __forceinline__ int lowest_view(uint32_t todo, int nsrc)
 {
-    float acc = 0.0f;
     for (int i = 0; i < nsrc; ++i) {
-        if (vw[i] > 0.0f) {
-            acc += vw[i] * term[i];
+        if (__ballot((todo >> i) & 1u) != 0ull) {
+            return i;
         }
     }
-    return acc;
-}
-
-/* the source view most lanes of the wave still need (lowest index on ties) */
-__device__ __forceinline__ int majority_view(uint32_t todo, int nsrc)
-{
-    int vi = 0, best = 0;
-    for (int i = 0; i < nsrc; ++i) {
-        const int n = __popcll(__ballot((todo >> i) & 1u));
-        if (n > best) {
-            best = n;
-            vi = i;
-        }
-    }
-    return vi;
+    return 0;
 }
 
 /* CheckerboardPropagation (HPM.cu:1222-1725) + PlaneHypothesisRefinement
@@ -926,8 +910,6 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
      * trip scores one source view (the one most lanes still need) for the
      * lanes that selected it. Values, sum orders and decisions per lane are
      * those of the sequential order. */
-    float jc[HPM_MAX_SOURCES];    /* rescore terms per view */
-    float jr[5][HPM_MAX_SOURCES]; /* refinement terms per candidate and view */
 
     /* NESP extension, orientation 0 only (HPM.cu:1473-1543, 780-843): up to
      * 3 rounds, each judged on ca[0] */
@@ -1001,30 +983,26 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     {
         /* joint view selection (HPM.cu:1548-1613) */
         float probs[HPM_MAX_SOURCES];
-        for (int i = 0; i < nsrc; ++i) {
-            vw[i] = 0.0f;
-            probs[i] = 0.0f;
-        }
         {
-            float vprior[HPM_MAX_SOURCES];
-            for (int i = 0; i < nsrc; ++i) {
-                vprior[i] = 0.0f;
-            }
+            /* view priors from the four axis neighbours' selections, summed
+             * in neighbour order per view */
             const int nb[4] = {center - W * 5, center + W * 5, center - 5, center + 5};
-            for (int i = 0; i < 4; ++i) {
-                if (flag[2 * i + 1]) {
-                    const uint32_t m = pb.sel[nb[i]];
-                    for (int j = 0; j < nsrc; ++j) {
-                        if (((m >> j) & 1u) == 1u) {
-                            vprior[j] += 0.9f;
-                        } else {
-                            vprior[j] += 0.1f;
-                        }
-                    }
-                }
+            bool nbf[4];
+            uint32_t nbm[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                nbf[k] = flag[2 * k + 1];
+                nbm[k] = nbf[k] ? pb.sel[nb[k]] : 0u;
             }
             const float thr = pb.thr[iter];
             for (int i = 0; i < nsrc; i++) {
+                float vprior = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (nbf[k]) {
+                        vprior += ((nbm[k] >> i) & 1u) == 1u ? 0.9f : 0.1f;
+                    }
+                }
                 float count = 0.0f;
                 int count_false = 0;
                 float tmpw = 0.0f;
@@ -1044,29 +1022,36 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 } else if (count_false < 3) {
                     pr = pb.fallback[iter];
                 }
-                probs[i] = pr * vprior[i];
+                probs[i] = pr * vprior;
             }
         }
         {
-            /* TransformPDFToCDF (HPM.cu:86-100) */
+            /* TransformPDFToCDF (HPM.cu:86-100) and the 15 draws: every draw
+             * goes to the first view whose CDF value exceeds it; the draws
+             * are made first and assigned in one pass over the views */
             float sum = 0.0f;
             for (int i = 0; i < nsrc; ++i) {
                 sum += probs[i];
             }
             const float inv = 1.0f / sum;
+            float u[15];
+#pragma unroll
+            for (int k = 0; k < 15; ++k) {
+                u[k] = hpm_rng_uniform(&rng) - FLT_EPSILON;
+            }
+            uint32_t open = 0x7fffu; /* draws not yet assigned */
             float cum = 0.0f;
             for (int i = 0; i < nsrc; ++i) {
                 cum += probs[i] * inv;
-                probs[i] = cum;
-            }
-        }
-        for (int s = 0; s < 15; ++s) {
-            const float u = hpm_rng_uniform(&rng) - FLT_EPSILON;
-            for (int v = 0; v < nsrc; ++v) {
-                if (probs[v] > u) {
-                    vw[v] += 1.0f;
-                    break;
+                float n = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 15; ++k) {
+                    if (((open >> k) & 1u) != 0u && cum > u[k]) {
+                        open &= ~(1u << k);
+                        n += 1.0f;
+                    }
                 }
+                vw[i] = n;
             }
         }
         for (int i = 0; i < nsrc; ++i) {
@@ -1103,23 +1088,26 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     }
 
     /* re-score the current hypothesis on the selected views
-     * (HPM.cu:1637-1650) */
+     * (HPM.cu:1637-1650); the trips take the views in ascending order, so
+     * the weighted sum accumulates in view order */
     plane_q(pb, cur, q);
+    float rescore_sum = 0.0f;
     {
         uint32_t todo = sel_views;
         while (__any(todo != 0u)) {
-            const int vi = majority_view(todo, nsrc);
+            const int vi = lowest_view(todo, nsrc);
             if ((todo >> vi) & 1u) {
                 todo &= ~(1u << vi);
                 const float c = ncc(pb, rp, px, py, vi + 1, q);
                 ++nev;
-                jc[vi] = P.geom_consistency ? c + 0.1f * geom_cost(pb, vi + 1, cur, px, py) : c;
+                const float term = P.geom_consistency ? c + 0.1f * geom_cost(pb, vi + 1, cur, px, py) : c;
+                rescore_sum += vw[vi] * term;
             }
         }
     }
 
     {
-        cost_now = selected_sum(vw, jc, nsrc) / wn;
+        cost_now = rescore_sum / wn;
         cost_center = cost_now;
         plane_center = cur;
         sel_center = pb.sel[center];
@@ -1216,29 +1204,41 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     }
 
     /* the refinement candidates on the selected views, all five on one view
-     * in turn (HPM.cu:1191-1200) */
+     * in turn, views in ascending order (HPM.cu:1191-1200): the weighted
+     * sums accumulate in view order; rs0 is the sum of the candidate being
+     * scored and the five rotate once per view */
+    float rs0 = 0.0f, rs1 = 0.0f, rs2 = 0.0f, rs3 = 0.0f, rs4 = 0.0f;
     {
         uint32_t todo = sel_views;
         while (__any(todo != 0u)) {
-            const int vi = majority_view(todo, nsrc);
+            const int vi = lowest_view(todo, nsrc);
             if ((todo >> vi) & 1u) {
                 todo &= ~(1u << vi);
+                const float wv = vw[vi];
 #pragma unroll 1
                 for (int c = 0; c < 5; ++c) {
                     const float4 t = cand[c];
                     plane_q(pb, t, q);
                     const float cc = ncc(pb, rp, px, py, vi + 1, q);
-                    jr[c][vi] = P.geom_consistency ? cc + 0.1f * geom_cost(pb, vi + 1, t, px, py) : cc;
+                    const float term = P.geom_consistency ? cc + 0.1f * geom_cost(pb, vi + 1, t, px, py) : cc;
+                    const float next = rs0 + wv * term;
+                    rs0 = rs1;
+                    rs1 = rs2;
+                    rs2 = rs3;
+                    rs3 = rs4;
+                    rs4 = next;
                 }
                 nev += 5u;
             }
         }
     }
+    const float rsum[5] = {rs0, rs1, rs2, rs3, rs4};
 
     /* the candidates decided in order (HPM.cu:1191-1218) */
+#pragma unroll
     for (int i = 0; i < 5; ++i) {
         const float4 t = cand[i];
-        const float tc = selected_sum(vw, jr[i], nsrc) / wn;
+        const float tc = rsum[i] / wn;
         const float db = depth_from_plane(rc, t, px, py);
         if (prior_pix) {
             const float pr = prior_term(pk, cand_d[i] - depth_prior, prior_plane, t);

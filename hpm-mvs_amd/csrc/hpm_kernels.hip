@@ -1062,27 +1062,46 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
         }
         sel_views = tsel;
 
-        /* candidate scoring (HPM.cu:1615-1635) */
-        for (int i = 0; i < 8; ++i) {
-            float a = 0.0f;
-            float4 hp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (P.geom_consistency && flag[i]) {
-                hp = ld4(pb.hyp, pos[i]);
+        /* candidate scoring (HPM.cu:1615-1635); views outer, so the eight
+         * directions' terms of a view load together, each direction's sum
+         * still in view order */
+        {
+            float a[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                a[i] = 0.0f;
             }
-            for (int j = 0; j < nsrc; ++j) {
-                if (vw[j] > 0) {
-                    if (P.geom_consistency) {
-                        if (flag[i]) {
-                            a += vw[j] * (ca[i][j] + 0.1f * geom_cost(pb, j + 1, hp, px, py));
-                        } else {
-                            a += vw[j] * (ca[i][j] + 0.1f * 5.0f);
+            if (P.geom_consistency) {
+                float4 hp[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    hp[i] = flag[i] ? ld4(pb.hyp, pos[i]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+                for (int j = 0; j < nsrc; ++j) {
+                    if ((tsel >> j) & 1u) {
+                        const float w = vw[j];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            const float g = flag[i] ? geom_cost(pb, j + 1, hp[i], px, py) : 5.0f;
+                            a[i] += w * (ca[i][j] + 0.1f * g);
                         }
-                    } else {
-                        a += vw[j] * ca[i][j];
+                    }
+                }
+            } else {
+                for (int j = 0; j < nsrc; ++j) {
+                    if ((tsel >> j) & 1u) {
+                        const float w = vw[j];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            a[i] += w * ca[i][j];
+                        }
                     }
                 }
             }
-            fc[i] = a / wn;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                fc[i] = a[i] / wn;
+            }
         }
         min_idx = find_min_idx8(fc);
     }

@@ -734,8 +734,25 @@ __device__ __forceinline__ int lowest_view(uint32_t todo, int nsrc)
  * once per wave rather than once for every group of lanes that reaches it.
  * Per-view results are stored and summed in view order, so every value and
  * every sum order is the sequential one. */
-__device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, int px, int py, int iter, int colour)
+/* per wave of a 256-lane block: the lanes that selected the view being
+ * refined, in lane order */
+__shared__ int s_own[4][64];
+
+__device__ __forceinline__ int lane_rank(uint64_t m)
 {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+/* Every lane of the wave calls this (the refinement packs evaluations of
+ * all lanes into the wave's trips); lanes without a pixel (valid false)
+ * only help. */
+__device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, int px, int py, int iter, int colour,
+                                                    bool valid)
+{
+    if (!valid) { /* any pixel inside the image: its reads stay in bounds */
+        px = 0;
+        py = 0;
+    }
     const hpm_params& P = pb.params;
     const hpm_camera rc = camera(pb, 0);
     const int W = pb.W, H = pb.H, nsrc = pb.nsrc;
@@ -888,6 +905,11 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     float4 cand[5];
     float cand_d[5];
 
+    if (!valid) {
+        for (int d = 0; d < 8; ++d) {
+            flag[d] = false;
+        }
+    }
     double q[3];
     /* the eight neighbour hypotheses on every view: the same work for every
      * lane away from the image border, so the wave runs it in lockstep */
@@ -1060,7 +1082,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 wn += vw[i];
             }
         }
-        sel_views = tsel;
+        sel_views = valid ? tsel : 0u;
 
         /* candidate scoring (HPM.cu:1615-1635); views outer, so the eight
          * directions' terms of a view load together, each direction's sum
@@ -1222,36 +1244,84 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
         }
     }
 
-    /* the refinement candidates on the selected views, all five on one view
-     * in turn, views in ascending order (HPM.cu:1191-1200): the weighted
-     * sums accumulate in view order; rs0 is the sum of the candidate being
-     * scored and the five rotate once per view */
-    float rs0 = 0.0f, rs1 = 0.0f, rs2 = 0.0f, rs3 = 0.0f, rs4 = 0.0f;
+    /* the refinement candidates on the selected views (HPM.cu:1191-1200),
+     * views in ascending order, so the weighted sums accumulate in view
+     * order. The (lane, candidate) evaluations of the lanes that selected a
+     * view are packed into full trips on that view: lane j scores task
+     * t0 + j, candidate t / n of the (t mod n)-th selecting lane, whose
+     * plane, pixel and reference sums come by lane permutes and whose
+     * window weights are in LDS; the owners take their terms back. */
+    float rsum[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     {
-        uint32_t todo = sel_views;
-        while (__any(todo != 0u)) {
-            const int vi = lowest_view(todo, nsrc);
-            if ((todo >> vi) & 1u) {
-                todo &= ~(1u << vi);
-                const float wv = vw[vi];
+        const int wv = (int)threadIdx.y, ln = (int)threadIdx.x; /* blockDim = (64, 4) */
+        const int pxy = px | (py << 16);
 #pragma unroll 1
-                for (int c = 0; c < 5; ++c) {
-                    const float4 t = cand[c];
-                    plane_q(pb, t, q);
-                    const float cc = ncc(pb, rp, px, py, vi + 1, q);
-                    const float term = P.geom_consistency ? cc + 0.1f * geom_cost(pb, vi + 1, t, px, py) : cc;
-                    const float next = rs0 + wv * term;
-                    rs0 = rs1;
-                    rs1 = rs2;
-                    rs2 = rs3;
-                    rs3 = rs4;
-                    rs4 = next;
+        for (int vi = 0; vi < nsrc; ++vi) {
+            const bool mine = ((sel_views >> vi) & 1u) != 0u;
+            const uint64_t m = __ballot(mine);
+            if (m == 0ull) {
+                continue;
+            }
+            const int n = __popcll(m);
+            const int rank = lane_rank(m);
+            if (mine) {
+                s_own[wv][rank] = ln;
+            }
+            __builtin_amdgcn_wave_barrier();
+            const float wview = vw[vi];
+            const int total = 5 * n;
+#pragma unroll 1
+            for (int t0 = 0; t0 < total; t0 += 64) {
+                const int t = t0 + ln;
+                const bool act = t < total;
+                const int c = act ? t / n : 0;
+                const int owner = act ? s_own[wv][t - c * n] : ln;
+                const int c_lo = t0 / n;
+                const int c_hi = (t0 + 63) / n < 4 ? (t0 + 63) / n : 4;
+                const int opxy = __shfl(pxy, owner);
+                RefPatch orp;
+                orp.lane = wv * 64 + owner;
+                orp.c = __shfl(rp.c, owner);
+                orp.inv_w = __shfl(rp.inv_w, owner);
+                orp.mr = __shfl(rp.mr, owner);
+                orp.var_r = __shfl(rp.var_r, owner);
+                float4 plane = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                for (int cc = 0; cc < 5; ++cc) {
+                    if (cc >= c_lo && cc <= c_hi) {
+                        const float4 pc = make_float4(__shfl(cand[cc].x, owner), __shfl(cand[cc].y, owner),
+                                                      __shfl(cand[cc].z, owner), __shfl(cand[cc].w, owner));
+                        if (c == cc) {
+                            plane = pc;
+                        }
+                    }
                 }
+                float term = 0.0f;
+                if (act) {
+                    const int opx = opxy & 0xffff, opy = opxy >> 16;
+                    double tq[3];
+                    plane_q(pb, plane, tq);
+                    const float cv = ncc(pb, orp, opx, opy, vi + 1, tq);
+                    term = P.geom_consistency ? cv + 0.1f * geom_cost(pb, vi + 1, plane, opx, opy) : cv;
+                }
+#pragma unroll
+                for (int cc = 0; cc < 5; ++cc) {
+                    if (cc >= c_lo && cc <= c_hi) {
+                        const int tt = cc * n + rank - t0; /* lane that scored (cc, this lane) */
+                        const bool got = mine && tt >= 0 && tt < 64;
+                        const float val = __shfl(term, got ? tt : ln);
+                        if (got) {
+                            rsum[cc] = rsum[cc] + wview * val;
+                        }
+                    }
+                }
+            }
+            if (mine) {
                 nev += 5u;
             }
+            __builtin_amdgcn_wave_barrier();
         }
     }
-    const float rsum[5] = {rs0, rs1, rs2, rs3, rs4};
 
     /* the candidates decided in order (HPM.cu:1191-1218) */
 #pragma unroll
@@ -1276,9 +1346,11 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             }
         }
     }
-    st4(pb.hyp, center, plane_center);
-    pb.cost[center] = cost_center;
-    pb.sel[center] = sel_center;
+    if (valid) {
+        st4(pb.hyp, center, plane_center);
+        pb.cost[center] = cost_center;
+        pb.sel[center] = sel_center;
+    }
     return nev;
 }
 
@@ -1397,10 +1469,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAV
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_propagate(hpm_dev_problem pb, int iter, int colour)
 {
     int px, py;
-    uint32_t nev = 0;
-    if (colour_pixel(pb, colour, &px, &py)) {
-        nev = propagate_pixel(pb, px, py, iter, colour);
-    }
+    const bool valid = colour_pixel(pb, colour, &px, &py);
+    const uint32_t nev = propagate_pixel(pb, px, py, iter, colour, valid);
     count_evals(pb, nev);
 }
 
@@ -1587,10 +1657,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAV
         const hpm_tile tl = load_tile(tiles, t);
         const hpm_dev_problem& pb = pbs[tl.problem];
         int px, py;
-        uint32_t nev = 0;
-        if (colour_pixel_at(pb, colour, tl.bx, tl.by, &px, &py)) {
-            nev = propagate_pixel(pb, px, py, iter, colour);
-        }
+        const bool valid = colour_pixel_at(pb, colour, tl.bx, tl.by, &px, &py);
+        const uint32_t nev = propagate_pixel(pb, px, py, iter, colour, valid);
         count_evals(pb, nev);
     }
 }

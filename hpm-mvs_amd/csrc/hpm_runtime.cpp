@@ -1118,6 +1118,21 @@ extern "C" int hpm_debug_read_state(hpm_handle* h, float* hyp4, float* cost, uin
     return HPM_OK;
 }
 
+extern "C" int hpm_debug_read_prior(hpm_handle* h, float* planes4, uint32_t* mask)
+{
+    if (!h || !h->have_device) {
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
+    }
+    if (!h->d_prior || !h->d_mask) {
+        return hpm_fail(h, HPM_EINVAL, "no prior on the device");
+    }
+    const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
+    if (planes4) HPM_HIP(h, hipMemcpyAsync(planes4, h->d_prior, np * 16, hipMemcpyDeviceToHost, h->stream));
+    if (mask) HPM_HIP(h, hipMemcpyAsync(mask, h->d_mask, np * 4, hipMemcpyDeviceToHost, h->stream));
+    HPM_HIP(h, hipStreamSynchronize(h->stream));
+    return HPM_OK;
+}
+
 extern "C" int hpm_debug_write_state(hpm_handle* h, const float* hyp4, const float* cost, const uint32_t* selmask)
 {
     if (!h || !h->have_device) {

@@ -212,6 +212,7 @@ SIGNATURES = {
     "hpm_debug_stage": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "hpm_debug_read_state": (C.c_int, [C.c_void_p, _FP, _FP, _UP]),
     "hpm_debug_write_state": (C.c_int, [C.c_void_p, _FP, _FP, _UP]),
+    "hpm_debug_read_prior": (C.c_int, [C.c_void_p, _FP, _UP]),
     "hpm_device_selftest": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     # hpm_io.h
     "hpm_read_camera": (C.c_int, [C.c_char_p, C.POINTER(HpmCamera)]),

@@ -157,6 +157,10 @@ int hpm_debug_stage(hpm_handle* h, int stage, int iter, int colour);
 /* Device state without finalize: hyp (4 floats/px), cost, view masks. */
 int hpm_debug_read_state(hpm_handle* h, float* hyp4, float* cost, uint32_t* selmask);
 int hpm_debug_write_state(hpm_handle* h, const float* hyp4, const float* cost, const uint32_t* selmask);
+/* The device prior of the handle: camera-frame planes (n, d; 4 floats/px)
+ * and masks, as a prior run reads them. Fails when no prior is on the
+ * device. */
+int hpm_debug_read_prior(hpm_handle* h, float* planes4, uint32_t* mask);
 
 /* Evaluates the shared device arithmetic for comparison with a host build:
  * kind 0..3 = exp, sin, cos, acos of the n floats in `in` (out: n floats);

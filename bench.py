@@ -178,12 +178,69 @@ def batch_bench(hpm, args):
     print(json.dumps(out), flush=True)
 
 
+def fusion_bench(hpm, args):
+    """RunFusion (main.cpp:502-633) of the 11-view 1600x1066 scene after the
+    learned-prior schedule (1 geometric pass): device consistency tests plus
+    the host mask scan (hpm_fusion_run), timed over the steps; the oracle's
+    sequential fusion of the same maps timed once beside it and checked
+    equal. Prints one JSON line (ms per fused view)."""
+    import numpy as np
+    from oracle import cpu as oracle_cpu
+
+    cfg = hpm.workload_config("c2")
+    cams, imgs, gd, gn, low = hpm.render_scene(cfg)
+    n = cfg.num_views
+    pool = hpm.Pool(cams, [im.astype(np.float32) for im in imgs])
+    problems = [(r, view_order(r, n)[1:]) for r in range(n)]
+    opt = hpm.pipeline_options()
+    opt.params.knn_k = 8
+    opt.params.geom_iterations = 1
+    if os.path.exists(args.learned_prior):
+        opt.learned_prior = os.path.abspath(args.learned_prior).encode()
+    hpm.run_pipeline(pool, problems, opt)
+    log("maps ready")
+    for _ in range(args.warmup):
+        pts = hpm.fusion(pool, problems, geom_consistency=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pts = hpm.fusion(pool, problems, geom_consistency=True)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"fusion: {ms:.1f} ms, {len(pts)} points")
+    depths = [pool.depth(v, 1) for v in range(n)]
+    normals = [pool.state(v)[0][..., :3].copy() for v in range(n)]
+    colors = [np.repeat(np.clip(im, 0, 255).astype(np.uint8)[..., None], 3, axis=2) for im in imgs]
+    t0 = time.perf_counter()
+    want = oracle_cpu.fusion([pool.camera(v) for v in range(n)], depths, normals, colors, problems, hpm.POINT_DTYPE)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    same = len(want) == len(pts) and np.array_equal(want.view(np.uint8), pts.view(np.uint8))
+    log(f"oracle fusion: {cpu_ms:.0f} ms, {len(want)} points, identical {same}")
+    pool.close()
+    print(json.dumps({
+        "metric": "ms per fused reference view, RunFusion of 11 views at 1600x1066",
+        "value": round(ms / n, 3),
+        "unit": "ms/view",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 11-view scene; maps from the learned-prior schedule)",
+        "config": {"workload": "RunFusion over 11 views of 1600x1066, 10 sources each", "points": int(len(pts)),
+                   "identical_to_oracle": bool(same)},
+        "cpu_baseline": {"value": round(cpu_ms / n, 1), "unit": "ms/view", "cores": 1, "kind": "port",
+                         "sample": "the oracle's sequential RunFusion of the same 11 maps"},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["full", "plain", "batch"], default="full")
+    ap.add_argument("--workload", choices=["full", "plain", "batch", "fusion"], default="full")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workers", type=int, default=0, help="host workers of the schedule (0: library default)")
@@ -215,6 +272,9 @@ def main():
     hpm = load_pkg()
     if args.workload == "batch":
         batch_bench(hpm, args)
+        return
+    if args.workload == "fusion":
+        fusion_bench(hpm, args)
         return
     cfg = hpm.workload_config("c2")
     t0 = time.perf_counter()

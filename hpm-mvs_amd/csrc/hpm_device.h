@@ -126,6 +126,26 @@ int hpm_launch_batch(int stage, const hpm_dev_problem* pbs, const hpm_tile* tile
 int hpm_launch_lprior_features(const hpm_dev_problem* pb, void* x, void* stream);
 int hpm_launch_lprior_judge(const hpm_dev_problem* pb, const void* x, const float* head, float* planes,
                             uint32_t* mask, unsigned long long* accepted, void* stream);
+/* Fusion (main.cpp:547-604): the consistency tests of reference view ref
+ * against its sources, one lane per reference pixel. rec_pix[p * nn + j]
+ * receives the source pixel of source j when the pair passes the
+ * reprojection, depth and angle tests (-1 otherwise), rec_w the pair's
+ * exp(-(err + 200 rdd + 10 angle)), any[p] the bitmask of passing sources.
+ * cams and off (pixel offset of each view in depth / hyp) are device
+ * arrays over all pool views; hyp holds the world normals (float4). */
+typedef struct hpm_fusion_args {
+    const hpm_camera* cams;
+    const int64_t* off;
+    const float* depth;
+    const float* hyp;
+    int32_t ref;
+    int32_t nn;
+    int32_t src[32];
+    int32_t* rec_pix;
+    float* rec_w;
+    uint32_t* any;
+} hpm_fusion_args;
+int hpm_launch_fusion_geometry(const hpm_fusion_args* a, int width, int height, void* stream);
 /* p[0..n) = 0 as a kernel (graph-capturable, unlike a memset node) */
 int hpm_launch_zero_u64(unsigned long long* p, int n, void* stream);
 

@@ -1966,6 +1966,100 @@ extern "C" int hpm_launch_lprior_judge(const hpm_dev_problem* pb, const void* x,
     return (int)hipGetLastError();
 }
 
+/* ---- fusion (main.cpp:502-633): per (pixel, source) consistency ---- */
+
+namespace {
+
+struct P3 {
+    float x, y, z;
+};
+
+/* Get3DPointonWorld (HPM.cpp:209-233) */
+__device__ __forceinline__ P3 fusion_world_point(int x, int y, float depth, const hpm_camera& c)
+{
+    const float px = depth * ((float)x - c.K[2]) / c.K[0];
+    const float py = depth * ((float)y - c.K[5]) / c.K[4];
+    const float pz = depth;
+    const float tx = c.R[0] * px + c.R[3] * py + c.R[6] * pz;
+    const float ty = c.R[1] * px + c.R[4] * py + c.R[7] * pz;
+    const float tz = c.R[2] * px + c.R[5] * py + c.R[8] * pz;
+    const float cx = -(c.R[0] * c.t[0] + c.R[3] * c.t[1] + c.R[6] * c.t[2]);
+    const float cy = -(c.R[1] * c.t[0] + c.R[4] * c.t[1] + c.R[7] * c.t[2]);
+    const float cz = -(c.R[2] * c.t[0] + c.R[5] * c.t[1] + c.R[8] * c.t[2]);
+    return P3{tx + cx, ty + cy, tz + cz};
+}
+
+/* ProjectonCamera (HPM.cpp:246-256) */
+__device__ __forceinline__ void fusion_project(const P3& X, const hpm_camera& c, float* u, float* v, float* depth)
+{
+    const float tx = c.R[0] * X.x + c.R[1] * X.y + c.R[2] * X.z + c.t[0];
+    const float ty = c.R[3] * X.x + c.R[4] * X.y + c.R[5] * X.z + c.t[1];
+    const float tz = c.R[6] * X.x + c.R[7] * X.y + c.R[8] * X.z + c.t[2];
+    *depth = c.K[6] * tx + c.K[7] * ty + c.K[8] * tz;
+    *u = (c.K[0] * tx + c.K[1] * ty + c.K[2] * tz) / *depth;
+    *v = (c.K[3] * tx + c.K[4] * ty + c.K[5] * tz) / *depth;
+}
+
+} // namespace
+
+__global__ __launch_bounds__(256) void hpm_k_fusion_geometry(hpm_fusion_args a, int width, int height)
+{
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int r = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (c >= width || r >= height) {
+        return;
+    }
+    const size_t p = (size_t)r * width + c;
+    const hpm_camera rc = a.cams[a.ref];
+    const float ref_depth = a.depth[a.off[a.ref] + p];
+    const float4 rn = reinterpret_cast<const float4*>(a.hyp)[a.off[a.ref] + p];
+    uint32_t any = 0u;
+    for (int j = 0; j < a.nn; ++j) {
+        int32_t pix = -1;
+        float w = 0.0f;
+        if (ref_depth > 0.0f) {
+            const P3 X = fusion_world_point(c, r, ref_depth, rc);
+            const int s = a.src[j];
+            const hpm_camera sc = a.cams[s];
+            float u, v, proj_depth;
+            fusion_project(X, sc, &u, &v, &proj_depth);
+            const int sr = (int)(v + 0.5f), scol = (int)(u + 0.5f); /* truncation, as int(x + 0.5f) */
+            if (scol >= 0 && scol < sc.width && sr >= 0 && sr < sc.height) {
+                const size_t q = (size_t)sr * sc.width + scol;
+                const float src_depth = a.depth[a.off[s] + q];
+                if (src_depth > 0.0f) {
+                    const float4 sn = reinterpret_cast<const float4*>(a.hyp)[a.off[s] + q];
+                    const P3 Y = fusion_world_point(scol, sr, src_depth, sc);
+                    float tu, tv;
+                    fusion_project(Y, rc, &tu, &tv, &proj_depth);
+                    const double dx = (double)((float)c - tu), dy = (double)((float)r - tv);
+                    const float reproj = (float)sqrt(dx * dx + dy * dy);
+                    const float rdd = fabsf(proj_depth - ref_depth) / ref_depth;
+                    const float dot = rn.x * sn.x + rn.y * sn.y + rn.z * sn.z;
+                    float angle = hpm_acosf(dot);
+                    angle = angle != angle ? 0.0f : angle; /* GetAngle, HPM.cpp:258-268 */
+                    if (reproj < 2.0f && rdd < 0.01f && angle < 0.174533f) {
+                        pix = (int32_t)q;
+                        w = hpm_expf(-(reproj + 200.0f * rdd + angle * 10.0f));
+                        any |= 1u << j;
+                    }
+                }
+            }
+        }
+        a.rec_pix[p * a.nn + j] = pix;
+        a.rec_w[p * a.nn + j] = w;
+    }
+    a.any[p] = any;
+}
+
+extern "C" int hpm_launch_fusion_geometry(const hpm_fusion_args* a, int width, int height, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_fusion_geometry, full_grid(width, height, blk), blk, 0, (hipStream_t)stream, *a, width,
+                       height);
+    return (int)hipGetLastError();
+}
+
 extern "C" int hpm_launch_zero_u64(unsigned long long* p, int n, void* stream)
 {
     if (n <= 0) {

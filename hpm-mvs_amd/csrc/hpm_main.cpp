@@ -9,7 +9,8 @@
  * Options: --geom-iterations N (2), --knn-k K (6), --no-prior, --no-geom,
  * --learned-prior FILE (device schedule only: the U-Net weight file of the
  * learned prior, which then replaces the planar prior passes where its judge
- * keeps enough pixels).
+ * keeps enough pixels), --no-fusion (skip RunFusion, main.cpp:675, which
+ * otherwise writes HPM/HPM_model.ply).
  */
 #include <math.h>
 #include <stdio.h>
@@ -23,6 +24,7 @@
 
 #include "HPM.h"
 #include "hpm_io.h"
+#include "hpm_fusion.h"
 #include "hpm_pipeline.h"
 
 using namespace hpm;
@@ -259,12 +261,27 @@ int main(int argc, char** argv)
 {
     if (argc < 2) {
         printf("USAGE: hpm_mvs dense_folder [--class-api] [--geom-iterations N] [--knn-k K] [--no-prior] "
-               "[--no-geom] [--learned-prior FILE]\n");
+               "[--no-geom] [--learned-prior FILE] [--no-fusion]\n");
         return -1;
     }
     const std::string dense_folder = argv[1];
     Options opt;
     const char* learned = nullptr;
+    bool fusion = true;
+    /* RunFusion (main.cpp:675) on the maps just written */
+    auto run_fusion = [&]() -> int {
+        if (!fusion) {
+            return 0;
+        }
+        int64_t n = 0;
+        char ferr[512] = {0};
+        if (hpm_fusion_run_folder(dense_folder.c_str(), opt.geom ? 1 : 0, &n, ferr, sizeof(ferr)) != 0) {
+            fprintf(stderr, "hpm_mvs: %s\n", ferr);
+            return 1;
+        }
+        printf("fusion: %lld points -> %s/HPM/HPM_model.ply\n", (long long)n, dense_folder.c_str());
+        return 0;
+    };
     for (int i = 2; i < argc; ++i) {
         if (!strcmp(argv[i], "--class-api")) {
             opt.class_api = true;
@@ -278,6 +295,8 @@ int main(int argc, char** argv)
             opt.geom = false;
         } else if (!strcmp(argv[i], "--learned-prior") && i + 1 < argc) {
             learned = argv[++i];
+        } else if (!strcmp(argv[i], "--no-fusion")) {
+            fusion = false;
         } else {
             fprintf(stderr, "unknown option %s\n", argv[i]);
             return -1;
@@ -304,7 +323,7 @@ int main(int argc, char** argv)
             printf("learned prior: %d views kept it, %d fell back to the planar prior\n", st.learned_views,
                    st.learned_fallbacks);
         }
-        return 0;
+        return run_fusion();
     }
     if (learned) {
         fprintf(stderr, "hpm_mvs: --learned-prior runs with the device schedule, not with --class-api\n");
@@ -327,5 +346,5 @@ int main(int argc, char** argv)
         fprintf(stderr, "hpm_mvs: %s\n", e.what());
         return 1;
     }
-    return 0;
+    return run_fusion();
 }

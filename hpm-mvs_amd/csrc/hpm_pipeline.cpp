@@ -380,6 +380,28 @@ extern "C" int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, f
     return HPM_OK;
 }
 
+extern "C" int hpm_pool_set_state(hpm_pool* pool, int view, const float* hyp_world4, const float* cost)
+{
+    if (!pool || !hyp_world4 || view < 0 || view >= pool->n) {
+        return HPM_EINVAL;
+    }
+    const int64_t off = pool->off[(size_t)view];
+    const size_t np = pool->images[(size_t)view].size();
+    std::vector<float> zeros;
+    if (!cost) {
+        zeros.assign(np, 0.0f);
+        cost = zeros.data();
+    }
+    if (hipMemcpyAsync(pool->d_state_hyp + 4 * off, hyp_world4, np * 16, hipMemcpyHostToDevice, pool->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(pool->d_state_cost + off, cost, np * 4, hipMemcpyHostToDevice, pool->stream) != hipSuccess ||
+        hipStreamSynchronize(pool->stream) != hipSuccess) {
+        return HPM_EDEVICE;
+    }
+    pool->have_state[(size_t)view] = 1;
+    return HPM_OK;
+}
+
 extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref_ids, const int* num_src,
                                 const int* src_ids, int max_src, const hpm_pipeline_options* opt,
                                 hpm_pipeline_stats* stats, char* err, int err_len)

@@ -213,6 +213,12 @@ SIGNATURES = {
     "hpm_debug_read_state": (C.c_int, [C.c_void_p, _FP, _FP, _UP]),
     "hpm_debug_write_state": (C.c_int, [C.c_void_p, _FP, _FP, _UP]),
     "hpm_debug_read_prior": (C.c_int, [C.c_void_p, _FP, _UP]),
+    "hpm_fusion_run": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, C.c_int, C.c_int, C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_int64), C.c_char_p, C.c_int]),
+    "hpm_fusion_free": (None, [C.c_void_p]),
+    "hpm_fusion_run_folder": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.c_char_p, C.c_int]),
+    "hpm_pool_set_state": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP]),
+    "hpm_write_ply": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int64]),
     "hpm_device_selftest": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     # hpm_io.h
     "hpm_read_camera": (C.c_int, [C.c_char_p, C.POINTER(HpmCamera)]),
@@ -728,6 +734,51 @@ def run_pipeline(pool: Pool, problems, opt: Optional[HpmPipelineOptions] = None)
     if rc != 0:
         raise HpmError(f"hpm_pipeline_run: {err.value.decode()}")
     return st
+
+
+# --- fusion (hpm_fusion.h) ---
+
+POINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("nx", "<f4"), ("ny", "<f4"), ("nz", "<f4"),
+                        ("r", "u1"), ("g", "u1"), ("b", "u1"), ("pad", "u1")])
+
+
+def _problem_arrays(problems):
+    n = len(problems)
+    ns = np.array([len(p[1]) for p in problems], np.int32)
+    srcs = np.zeros((n, HPM_MAX_SOURCES), np.int32)
+    for i, (_, s) in enumerate(problems):
+        srcs[i, : len(s)] = s
+    return ns, srcs
+
+
+def fusion(pool: Pool, problems, geom_consistency: bool = True) -> np.ndarray:
+    """RunFusion over the pool's maps; problems[i] = (i, [source views]).
+    Returns the points as a POINT_DTYPE array in the reference's order."""
+    for i, (r, _) in enumerate(problems):
+        if r != i:
+            raise HpmError("fusion: problem i must have reference view i")
+    ns, srcs = _problem_arrays(problems)
+    pts = C.c_void_p()
+    npts = C.c_int64()
+    err = C.create_string_buffer(512)
+    rc = lib().hpm_fusion_run(pool._p, len(problems), iptr(ns), iptr(srcs), HPM_MAX_SOURCES, int(geom_consistency),
+                              C.byref(pts), C.byref(npts), err, 512)
+    if rc != 0:
+        raise HpmError(f"hpm_fusion_run: {err.value.decode()}")
+    try:
+        n = npts.value
+        out = np.empty(n, POINT_DTYPE)
+        if n:
+            C.memmove(out.ctypes.data, pts.value, n * POINT_DTYPE.itemsize)
+    finally:
+        lib().hpm_fusion_free(pts)
+    return out
+
+
+def write_ply(path: str, points: np.ndarray):
+    p = np.ascontiguousarray(points, dtype=POINT_DTYPE)
+    if lib().hpm_write_ply(path.encode(), C.c_void_p(p.ctypes.data), len(p)) != 0:
+        raise HpmError(f"hpm_write_ply failed for {path}")
 
 
 def run_pipeline_folder(folder: str, opt: Optional[HpmPipelineOptions] = None) -> HpmPipelineStats:

@@ -75,6 +75,9 @@ int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref_ids, const
 
 /* Last (world normal xyz, depth) hypotheses and costs of a view. */
 int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, float* cost);
+/* Sets them (for instance from normals.dmb and a depth map, to fuse maps
+ * written by an earlier run); cost may be NULL (zeros). */
+int hpm_pool_set_state(hpm_pool* pool, int view, const float* hyp_world4, const float* cost);
 
 /* main() (main.cpp:636-677) on a folder: pair.txt, images/, cams/; writes
  * HPM/2333_%08d/{depths.dmb, depths_geom.dmb, normals.dmb, costs.dmb}. */

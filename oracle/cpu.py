@@ -177,6 +177,35 @@ class Oracle:
         return out
 
 
+def fusion(cams, depths, normals, colors, problems, point_dtype):
+    """RunFusion (main.cpp:502-633), sequential: depths[i] (H, W), normals[i]
+    (H, W, 3) world, colors[i] (H, W, 3) uint8 of view i; problems[i] =
+    (i, [sources]). Returns the kept points as point_dtype."""
+    L = lib()
+    n = len(cams)
+    max_src = 32
+    ns = np.array([len(p[1]) for p in problems], np.int32)
+    srcs = np.zeros((n, max_src), np.int32)
+    for i, (_, s) in enumerate(problems):
+        srcs[i, : len(s)] = s
+    keep = [np.ascontiguousarray(d, dtype=np.float32) for d in depths]
+    keep_n = [np.ascontiguousarray(x, dtype=np.float32) for x in normals]
+    keep_c = [np.ascontiguousarray(x, dtype=np.uint8) for x in colors]
+    dp = (_FP * n)(*[_f(a) for a in keep])
+    np_ = (_FP * n)(*[_f(a) for a in keep_n])
+    cp = (C.POINTER(C.c_uint8) * n)(*[a.ctypes.data_as(C.POINTER(C.c_uint8)) for a in keep_c])
+    cam_arr = (type(cams[0]) * n)(*cams)
+    cap = sum(int(d.size) for d in depths)
+    out = np.empty(cap, point_dtype)
+    L.hpm_oracle_fusion.restype = C.c_int64
+    L.hpm_oracle_fusion.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _IP, _IP, C.c_int,
+                                    C.c_void_p, C.c_int64]
+    k = L.hpm_oracle_fusion(n, C.cast(cam_arr, C.c_void_p), C.cast(dp, C.c_void_p), C.cast(np_, C.c_void_p),
+                            C.cast(cp, C.c_void_p), ns.ctypes.data_as(_IP), srcs.ctypes.data_as(_IP), max_src,
+                            C.c_void_p(out.ctypes.data), cap)
+    return out[:k]
+
+
 def jbu(guide: np.ndarray, src_nd: np.ndarray, threads: int = 0):
     g = np.ascontiguousarray(guide, dtype=np.float32)
     s = np.ascontiguousarray(src_nd, dtype=np.float32)

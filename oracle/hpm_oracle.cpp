@@ -37,6 +37,7 @@
 #include <omp.h>
 
 #include "hpm_fmath.h"
+#include "hpm_fusion.h"
 #include "hpm_half.h"
 #include "hpm_philox.h"
 
@@ -1704,4 +1705,142 @@ extern "C" int hpm_oracle_jbu_prior(const float* guide, int w, int h, const floa
         }
     }
     return 0;
+}
+
+/* ---- RunFusion (main.cpp:502-633) ---- */
+
+namespace {
+
+struct P3 {
+    float x, y, z;
+};
+
+/* Get3DPointonWorld (HPM.cpp:209-233) */
+P3 fusion_world_point(int x, int y, float depth, const hpm_camera& c)
+{
+    const float px = depth * ((float)x - c.K[2]) / c.K[0];
+    const float py = depth * ((float)y - c.K[5]) / c.K[4];
+    const float pz = depth;
+    const float tx = c.R[0] * px + c.R[3] * py + c.R[6] * pz;
+    const float ty = c.R[1] * px + c.R[4] * py + c.R[7] * pz;
+    const float tz = c.R[2] * px + c.R[5] * py + c.R[8] * pz;
+    const float cx = -(c.R[0] * c.t[0] + c.R[3] * c.t[1] + c.R[6] * c.t[2]);
+    const float cy = -(c.R[1] * c.t[0] + c.R[4] * c.t[1] + c.R[7] * c.t[2]);
+    const float cz = -(c.R[2] * c.t[0] + c.R[5] * c.t[1] + c.R[8] * c.t[2]);
+    return P3{tx + cx, ty + cy, tz + cz};
+}
+
+/* ProjectonCamera (HPM.cpp:246-256) */
+void fusion_project(const P3& X, const hpm_camera& c, float* u, float* v, float* depth)
+{
+    const float tx = c.R[0] * X.x + c.R[1] * X.y + c.R[2] * X.z + c.t[0];
+    const float ty = c.R[3] * X.x + c.R[4] * X.y + c.R[5] * X.z + c.t[1];
+    const float tz = c.R[6] * X.x + c.R[7] * X.y + c.R[8] * X.z + c.t[2];
+    *depth = c.K[6] * tx + c.K[7] * ty + c.K[8] * tz;
+    *u = (c.K[0] * tx + c.K[1] * ty + c.K[2] * tz) / *depth;
+    *v = (c.K[3] * tx + c.K[4] * ty + c.K[5] * tz) / *depth;
+}
+
+/* GetAngle (HPM.cpp:258-268) with the shared acos */
+float fusion_angle(const float* a, const float* b)
+{
+    const float dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    const float angle = hpm_acosf(dot);
+    return angle != angle ? 0.0f : angle;
+}
+
+} // namespace
+
+extern "C" int64_t hpm_oracle_fusion(int num_views, const hpm_camera* cams, const float* const* depths,
+                                     const float* const* normals, const uint8_t* const* colors, const int* num_src,
+                                     const int* src_ids, int max_src, hpm_point* out, int64_t cap)
+{
+    std::vector<std::vector<uint8_t>> masks((size_t)num_views);
+    for (int i = 0; i < num_views; ++i) {
+        masks[(size_t)i].assign((size_t)cams[i].width * cams[i].height, 0);
+    }
+    int64_t n = 0;
+    for (int i = 0; i < num_views; ++i) {
+        const hpm_camera& rc = cams[i];
+        const int cols = rc.width, rows = rc.height;
+        const int nn = num_src[i];
+        const int* srcs = src_ids + (size_t)i * max_src;
+        int ux[32], uy[32]; /* used_list: per view, not per pixel (main.cpp:556) */
+        for (int j = 0; j < nn; ++j) {
+            ux[j] = -1;
+            uy[j] = -1;
+        }
+        for (int r = 0; r < rows; ++r) {
+            for (int c = 0; c < cols; ++c) {
+                const size_t p = (size_t)r * cols + c;
+                if (masks[(size_t)i][p] == 1) {
+                    continue;
+                }
+                const float ref_depth = depths[i][p];
+                const float* ref_normal = normals[i] + 3 * p;
+                if (ref_depth <= 0.0f) {
+                    continue;
+                }
+                const P3 X = fusion_world_point(c, r, ref_depth, rc);
+                int num_consistent = 0;
+                float dyn = 0.0f;
+                for (int j = 0; j < nn; ++j) {
+                    const int s = srcs[j];
+                    const hpm_camera& sc = cams[s];
+                    float u, v, proj_depth;
+                    fusion_project(X, sc, &u, &v, &proj_depth);
+                    const int sr = (int)(v + 0.5f), sc_ = (int)(u + 0.5f);
+                    if (!(sc_ >= 0 && sc_ < sc.width && sr >= 0 && sr < sc.height)) {
+                        continue;
+                    }
+                    const size_t q = (size_t)sr * sc.width + sc_;
+                    if (masks[(size_t)s][q] == 1) {
+                        continue;
+                    }
+                    const float src_depth = depths[s][q];
+                    if (src_depth <= 0.0f) {
+                        continue;
+                    }
+                    const P3 Y = fusion_world_point(sc_, sr, src_depth, sc);
+                    float tu, tv;
+                    fusion_project(Y, rc, &tu, &tv, &proj_depth);
+                    const double dx = (double)((float)c - tu), dy = (double)((float)r - tv);
+                    const float reproj = (float)sqrt(dx * dx + dy * dy);
+                    const float rdd = fabsf(proj_depth - ref_depth) / ref_depth;
+                    const float angle = fusion_angle(ref_normal, normals[s] + 3 * q);
+                    if (reproj < 2.0f && rdd < 0.01f && angle < 0.174533f) {
+                        ux[j] = sc_;
+                        uy[j] = sr;
+                        const float index = reproj + 200.0f * rdd + angle * 10.0f;
+                        dyn += hpm_expf(-index);
+                        ++num_consistent;
+                    }
+                }
+                if (num_consistent >= 1 && (double)dyn > 0.3 * (double)num_consistent) {
+                    if (n < cap) {
+                        hpm_point& pt = out[n];
+                        pt.x = X.x;
+                        pt.y = X.y;
+                        pt.z = X.z;
+                        pt.nx = ref_normal[0];
+                        pt.ny = ref_normal[1];
+                        pt.nz = ref_normal[2];
+                        pt.r = colors[i][3 * p];
+                        pt.g = colors[i][3 * p + 1];
+                        pt.b = colors[i][3 * p + 2];
+                        pt.pad_ = 0;
+                    }
+                    ++n;
+                    for (int j = 0; j < nn; ++j) {
+                        if (ux[j] == -1) {
+                            continue;
+                        }
+                        const int s = srcs[j];
+                        masks[(size_t)s][(size_t)uy[j] * cams[s].width + ux[j]] = 1;
+                    }
+                }
+            }
+        }
+    }
+    return n;
 }

@@ -21,6 +21,7 @@
 
 #include <stdint.h>
 
+#include "hpm_fusion.h"
 #include "hpm_types.h"
 
 #ifdef __cplusplus
@@ -92,6 +93,16 @@ int hpm_oracle_round_f16(const float* x, float* y, int n);
 int hpm_oracle_philox_block(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 int hpm_oracle_uniforms(uint32_t seed_lo, uint32_t seed_hi, uint32_t pixel, uint32_t stream, uint32_t salt,
                         float* out, int n);
+
+/* RunFusion (main.cpp:502-633) on the CPU, sequential in the reference's
+ * order: views in turn, pixels row-major, masks set as points are kept,
+ * used_list kept per view (main.cpp:556). depths[i] and normals[i] (world,
+ * 3 floats/px) and colors[i] (r g b bytes/px) of view i at cams[i]'s size;
+ * sources src_ids[i * max_src + j], j < num_src[i]. Writes up to cap points
+ * to out and returns how many the fusion kept. */
+int64_t hpm_oracle_fusion(int num_views, const hpm_camera* cams, const float* const* depths,
+                          const float* const* normals, const uint8_t* const* colors, const int* num_src,
+                          const int* src_ids, int max_src, hpm_point* out, int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -7,7 +7,8 @@ import re
 from conftest import ROOT
 
 # headers of the product library; hpm_types/philox/fmath hold inline code only
-HEADERS = ["hpm_capi.h", "hpm_io.h", "hpm_scene.h", "hpm_prior.h", "hpm_pipeline.h", "hpm_batch.h", "hpm_unet.h"]
+HEADERS = ["hpm_capi.h", "hpm_io.h", "hpm_scene.h", "hpm_prior.h", "hpm_pipeline.h", "hpm_batch.h", "hpm_unet.h",
+           "hpm_fusion.h"]
 
 
 def declared_functions(path):

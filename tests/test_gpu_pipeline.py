@@ -73,6 +73,32 @@ def test_executable_on_folder(hpm, tmp_path, mode):
         gt = hpm.read_dmb(str(tmp_path / "gt" / f"{v:08d}_depth.dmb"))
         assert np.mean(np.abs(d - gt) / gt < 0.05) > 0.3
         assert hpm.read_dmb(str(base / "costs.dmb")).shape == (144, 192)
+    # RunFusion (main.cpp:675) after the passes: the fused cloud equals the
+    # oracle's fusion of the maps the run wrote
+    from oracle import cpu
+
+    ply = (tmp_path / "HPM" / "HPM_model.ply").read_bytes()
+    head, _, body = ply.partition(b"end_header\n")
+    n = int([ln for ln in head.split(b"\n") if ln.startswith(b"element vertex")][0].split()[-1])
+    assert len(body) == 27 * n
+    rows = np.frombuffer(body, dtype=np.dtype([("xyz", "<f4", 3), ("n", "<f4", 3), ("rgb", "u1", 3)]))
+    problems = hpm.read_pair(str(tmp_path / "pair.txt"))
+    cams, depths, normals, colors = [], [], [], []
+    for v in range(3):
+        cam = hpm.read_camera(str(tmp_path / "cams" / f"{v:08d}_cam.txt"))
+        img = hpm.read_pgm(str(tmp_path / "images" / f"{v:08d}.pgm"))
+        cam.height, cam.width = img.shape
+        base = tmp_path / "HPM" / f"2333_{v:08d}"
+        cams.append(cam)
+        depths.append(hpm.read_dmb(str(base / "depths_geom.dmb")))
+        normals.append(hpm.read_dmb(str(base / "normals.dmb")))
+        colors.append(np.repeat(np.clip(img, 0, 255).astype(np.uint8)[..., None], 3, axis=2))
+    want = cpu.fusion(cams, depths, normals, colors, [(r, list(s)) for r, s in problems], hpm.POINT_DTYPE)
+    say(f"fused points: {n} (oracle {len(want)})")
+    assert n == len(want) and n > 0
+    assert np.array_equal(rows["xyz"], np.stack([want["x"], want["y"], want["z"]], axis=1))
+    assert np.array_equal(rows["n"], np.stack([want["nx"], want["ny"], want["nz"]], axis=1))
+    assert np.array_equal(rows["rgb"], np.stack([want["r"], want["g"], want["b"]], axis=1))
 
 
 def test_pipeline_folder_round_trip(hpm, tmp_path):

@@ -309,15 +309,15 @@ extern "C" int hpm_fusion_run_folder(const char* dense_folder, int geom_consiste
     char path[4096];
     for (int i = 0; i < np; ++i) {
         int w = 0, h = 0;
-        snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder, i);
-        if (hpm_read_pgm_f32(path, nullptr, &w, &h) != 0) {
-            snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder, i);
-            if (hpm_read_pgm_f32(path, nullptr, &w, &h) != 0) {
-                return fail(err, err_len, "fusion: cannot read image %d (8-bit PGM expected)", i);
+        snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder, i);
+        if (hpm_read_image_grey(path, nullptr, &w, &h) != 0) {
+            snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder, i);
+            if (hpm_read_image_grey(path, nullptr, &w, &h) != 0) {
+                return fail(err, err_len, "fusion: cannot read image %d (JPEG or 8-bit PGM expected)", i);
             }
         }
         imgs[(size_t)i].resize((size_t)w * h);
-        hpm_read_pgm_f32(path, imgs[(size_t)i].data(), &w, &h);
+        hpm_read_image_grey(path, imgs[(size_t)i].data(), &w, &h);
         snprintf(path, sizeof(path), "%s/cams/%08d_cam.txt", dense_folder, i);
         if (hpm_read_camera(path, &cams[(size_t)i]) != 0) {
             return fail(err, err_len, "fusion: cannot read %s", path);

@@ -557,16 +557,16 @@ extern "C" int hpm_pipeline_run_folder(const char* dense_folder, const hpm_pipel
     for (int id = 0; id <= maxid; ++id) {
         char path[4096];
         int w = 0, hh = 0;
-        snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder, id);
-        if (hpm_read_pgm_f32(path, nullptr, &w, &hh) != 0) {
-            snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder, id);
-            if (hpm_read_pgm_f32(path, nullptr, &w, &hh) != 0) {
-                say("cannot read an image (8-bit PGM expected)");
+        snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder, id);
+        if (hpm_read_image_grey(path, nullptr, &w, &hh) != 0) {
+            snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder, id);
+            if (hpm_read_image_grey(path, nullptr, &w, &hh) != 0) {
+                say("cannot read an image (JPEG or 8-bit PGM expected)");
                 return HPM_EIO;
             }
         }
         imgs[(size_t)id].resize((size_t)w * hh);
-        hpm_read_pgm_f32(path, imgs[(size_t)id].data(), &w, &hh);
+        hpm_read_image_grey(path, imgs[(size_t)id].data(), &w, &hh);
         snprintf(path, sizeof(path), "%s/cams/%08d_cam.txt", dense_folder, id);
         if (hpm_read_camera(path, &cams[(size_t)id]) != 0) {
             say("cannot read a camera file");

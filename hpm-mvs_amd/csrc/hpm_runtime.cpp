@@ -573,16 +573,16 @@ extern "C" int hpm_input_init(hpm_handle* h, const char* dense_folder, int ref_i
     for (int id : ids) {
         char path[4096];
         int w = 0, hh = 0;
-        snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder, id);
-        if (hpm_read_pgm_f32(path, nullptr, &w, &hh) != 0) {
-            snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder, id);
-            if (hpm_read_pgm_f32(path, nullptr, &w, &hh) != 0) {
-                return hpm_fail(h, HPM_EIO, "cannot read image %d under %s/images (8-bit PGM expected)", id,
+        snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder, id);
+        if (hpm_read_image_grey(path, nullptr, &w, &hh) != 0) {
+            snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder, id);
+            if (hpm_read_image_grey(path, nullptr, &w, &hh) != 0) {
+                return hpm_fail(h, HPM_EIO, "cannot read image %d under %s/images (JPEG or 8-bit PGM expected)", id,
                                 dense_folder);
             }
         }
         std::vector<float> img((size_t)w * hh);
-        if (hpm_read_pgm_f32(path, img.data(), &w, &hh) != 0) {
+        if (hpm_read_image_grey(path, img.data(), &w, &hh) != 0) {
             return hpm_fail(h, HPM_EIO, "cannot decode %s", path);
         }
         hpm_camera cam;

@@ -224,6 +224,8 @@ SIGNATURES = {
     "hpm_read_camera": (C.c_int, [C.c_char_p, C.POINTER(HpmCamera)]),
     "hpm_write_camera": (C.c_int, [C.c_char_p, C.POINTER(HpmCamera)]),
     "hpm_read_pair": (C.c_int, [C.c_char_p, _IP, _IP, _IP, _IP, C.c_int]),
+    "hpm_read_image_grey": (C.c_int, [C.c_char_p, _FP, _IP, _IP]),
+    "hpm_decode_jpeg_grey": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, _IP, _IP]),
     "hpm_read_dmb": (C.c_int, [C.c_char_p, _FP, _IP, _IP, _IP]),
     "hpm_write_dmb": (C.c_int, [C.c_char_p, _FP, C.c_int, C.c_int, C.c_int]),
     "hpm_read_pgm_f32": (C.c_int, [C.c_char_p, _FP, _IP, _IP]),
@@ -584,6 +586,17 @@ def read_pgm(path: str) -> np.ndarray:
         raise HpmError(f"cannot read {path}")
     out = np.empty((h.value, w.value), np.float32)
     lib().hpm_read_pgm_f32(path.encode(), fptr(out), C.byref(w), C.byref(h))
+    return out
+
+
+def read_image_grey(path: str) -> np.ndarray:
+    """JPEG (luminance, as libjpeg's grayscale decode) or 8-bit PGM, as float."""
+    w, h = C.c_int32(), C.c_int32()
+    if lib().hpm_read_image_grey(path.encode(), None, C.byref(w), C.byref(h)) != 0:
+        raise HpmError(f"cannot read {path}")
+    out = np.empty((h.value, w.value), np.float32)
+    if lib().hpm_read_image_grey(path.encode(), fptr(out), C.byref(w), C.byref(h)) != 0:
+        raise HpmError(f"cannot decode {path}")
     return out
 
 

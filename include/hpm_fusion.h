@@ -41,7 +41,7 @@ int hpm_fusion_run(hpm_pool* pool, int num_views, const int* num_src, const int*
 void hpm_fusion_free(hpm_point* points);
 
 /* RunFusion as the reference's main calls it (main.cpp:675): reads
- * pair.txt, cams/%08d_cam.txt, images/%08d.pgm (grey, as colour) and
+ * pair.txt, cams/%08d_cam.txt, images/%08d.jpg or .pgm (grey, as colour) and
  * HPM/2333_%08d/{depths_geom.dmb or depths.dmb, normals.dmb} of every view
  * (views 0..n-1 in pair.txt order), fuses them with hpm_fusion_run and
  * writes HPM/HPM_model.ply. *num_points (may be NULL) receives the point

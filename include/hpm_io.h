@@ -17,6 +17,7 @@
 #ifndef HPM_IO_H_
 #define HPM_IO_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "hpm_types.h"
@@ -44,6 +45,16 @@ int hpm_write_dmb(const char* path, const float* data, int width, int height, in
  * convertTo(CV_32FC1) of HPM.cpp:494); data == NULL reads only the size. */
 int hpm_read_pgm_f32(const char* path, float* data, int* width, int* height);
 int hpm_write_pgm_u8(const char* path, const uint8_t* data, int width, int height);
+
+/* cv::imread(path, cv::IMREAD_GRAYSCALE) + convertTo(CV_32FC1)
+ * (HPM.cpp:492-494) for JPEG and PGM files, chosen by content: for a JPEG
+ * the luminance component as libjpeg's grayscale output computes it
+ * (integer inverse DCT); baseline / extended sequential Huffman, 8-bit, grey
+ * or YCbCr only. data == NULL reads only the size. Returns 0 or -1. */
+int hpm_read_image_grey(const char* path, float* data, int* width, int* height);
+/* The same decode of a JPEG in memory into width * height bytes; out ==
+ * NULL returns the size. */
+int hpm_decode_jpeg_grey(const uint8_t* data, size_t size, uint8_t* out, int* width, int* height);
 
 /* cv::resize(..., INTER_LINEAR) of a float image: half-pixel centres,
  * src = (dst + 0.5) * (sw / dw) - 0.5, edge-clamped, exact float weights.

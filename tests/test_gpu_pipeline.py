@@ -115,3 +115,30 @@ def test_pipeline_folder_round_trip(hpm, tmp_path):
         assert d.shape == (120, 160) and n.shape == (120, 160, 3)
         gt = hpm.read_dmb(str(tmp_path / "gt" / f"{v:08d}_depth.dmb"))
         assert np.mean(np.abs(d - gt) / gt < 0.05) > 0.3
+
+
+def test_executable_on_jpeg_folder(hpm, tmp_path):
+    """The reference's folder layout with images/%08d.jpg (as real datasets
+    ship them): hpm_mvs decodes the JPEGs (luminance) and reconstructs."""
+    import subprocess
+
+    Image = pytest.importorskip("PIL.Image")
+    exe = os.path.join(os.path.dirname(hpm.LIB_PATH), "hpm_mvs")
+    cfg = hpm.scene_config(seed=6, width=192, height=144, fx=150.0, fy=150.0, cx=96.0, cy=72.0, num_views=3)
+    hpm.write_scene_folder(cfg, str(tmp_path))
+    for v in range(3):
+        pgm = tmp_path / "images" / f"{v:08d}.pgm"
+        img = hpm.read_image_grey(str(pgm)).astype(np.uint8)
+        Image.fromarray(img).save(str(tmp_path / "images" / f"{v:08d}.jpg"), "JPEG", quality=95)
+        pgm.unlink()
+    say("case: hpm_mvs on a 3-view 192x144 folder of JPEG images")
+    r = subprocess.run([exe, str(tmp_path), "--geom-iterations", "1"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    accs = []
+    for v in range(3):
+        d = hpm.read_dmb(str(tmp_path / "HPM" / f"2333_{v:08d}" / "depths_geom.dmb"))
+        gt = hpm.read_dmb(str(tmp_path / "gt" / f"{v:08d}_depth.dmb"))
+        accs.append(float(np.mean(np.abs(d - gt) / gt < 0.05)))
+    say(f"accuracy (5%) per view: {accs}")
+    assert min(accs) > 0.3
+    assert (tmp_path / "HPM" / "HPM_model.ply").exists()

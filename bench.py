@@ -34,6 +34,7 @@ if ROOT not in sys.path:
 FLOPS_PER_EVAL = 1190  # 36 taps x 32 + 24 homography + 14 NCC finalize
 PEAK_FP32_TFLOPS = 157.3
 METRIC = "ms/ref view @1600x1066,10 src: plain vs learned-prior, equal depth accuracy"
+METRIC_LARGE = "ms/ref view @3840x2160,20 src: plain vs learned-prior, equal depth accuracy"
 
 
 def log(msg):
@@ -240,7 +241,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["full", "plain", "batch", "fusion"], default="full")
+    ap.add_argument("--workload", choices=["full", "plain", "batch", "fusion", "large"], default="full")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workers", type=int, default=0, help="host workers of the schedule (0: library default)")
@@ -276,13 +277,15 @@ def main():
     if args.workload == "fusion":
         fusion_bench(hpm, args)
         return
-    cfg = hpm.workload_config("c2")
+    large = args.workload == "large"
+    cfg = hpm.workload_config("c5" if large else "c2")
+    max_size = 4096 if large else 3200  # the large views stay at full resolution
     t0 = time.perf_counter()
     cams, imgs, gd, gn, low = hpm.render_scene(cfg)
     n = cfg.num_views
     log(f"scene rendered: {n} views {cfg.width}x{cfg.height} in {time.perf_counter() - t0:.1f} s")
     fimgs = [im.astype(np.float32) for im in imgs]
-    pool = hpm.Pool(cams, fimgs)
+    pool = hpm.Pool(cams, fimgs, max_image_size=max_size)
     problems = [(r, view_order(r, n)[1:]) for r in range(n)]
     log("images resident on the device")
 
@@ -290,6 +293,7 @@ def main():
         opt = hpm.pipeline_options()
         opt.params.knn_k = 8
         opt.params.geom_iterations = geom_iterations
+        opt.params.max_image_size = max_size
         if args.workload == "plain":
             opt.planar_prior = 0
             opt.geom_consistency = 0
@@ -338,7 +342,7 @@ def main():
         log(f"{name}: {r['ms_per_view']} ms/view, accuracy {r['accuracy_2pct']}")
         return r
 
-    learned = args.workload == "full" and args.learned_prior.lower() != "none"
+    learned = args.workload in ("full", "large") and args.learned_prior.lower() != "none"
     plain = time_method("planar prior", options(False))
     res = same = None
     if learned:
@@ -357,6 +361,7 @@ def main():
     # event durations overstate a launch's own time. One plain RunPatchMatch
     # per view on a single worker gives the isolated launch duration.
     ropt = hpm.pipeline_options()
+    ropt.params.max_image_size = max_size
     ropt.planar_prior = 0
     ropt.geom_consistency = 0
     ropt.workers = 1
@@ -365,12 +370,13 @@ def main():
     flops_per_launch = rst.n_eval_propagate / rst.propagate_launches * FLOPS_PER_EVAL
     achieved = flops_per_launch / (avg_launch_ms * 1e-3) / 1e12
 
-    if args.workload == "full":
+    if args.workload in ("full", "large"):
+        scales = "0.25x and 0.5x planar priors with JBU" if large else "0.5x planar prior with JBU"
         workload = ("per reference view: plain RunPatchMatch, then the prior (learned: U-Net proposal + NCC judge, "
                     "one prior pass, planar prior as fallback, then "
-                    f"{args.learned_geom_iterations} geometric-consistency passes; plain method: 0.5x planar prior "
-                    "with JBU and 1x planar prior with KNN K=8, then 3 geometric-consistency passes); 1600x1066, "
-                    "10 sources, 11x11 window stride 2")
+                    f"{args.learned_geom_iterations} geometric-consistency passes; plain method: {scales} "
+                    "and 1x planar prior with KNN K=8, then 3 geometric-consistency passes); "
+                    f"{cfg.width}x{cfg.height}, {n - 1} sources, 11x11 window stride 2")
     else:
         workload = ("one plain RunPatchMatch per reference view (no prior, no geometric consistency); 1600x1066, "
                     "10 sources")
@@ -381,7 +387,7 @@ def main():
     if same is not None:
         methods["learned_prior_3_gc_passes"] = {k: v for k, v in same.items() if k != "ms_per_step"}
     out = {
-        "metric": METRIC,
+        "metric": METRIC_LARGE if large else METRIC,
         "value": round(ms_per_view, 3),
         "unit": "ms/ref view",
         "n_gpus": 1,
@@ -392,7 +398,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded 11-view scene rendered on the host)",
+        "data": f"synthetic (seeded {n}-view scene rendered on the host)",
         "config": {
             "workload": workload,
             "views": n,

@@ -631,15 +631,21 @@ def scene_config(**kw) -> HpmSceneConfig:
 
 
 def workload_config(name: str) -> HpmSceneConfig:
-    """Scene settings of the two plain-method workloads:
+    """Scene settings of the workloads:
     'c1': 1 ref + 4 src, 640x480, 4 planes (fx=fy=500);
-    'c2': 11 views, 1600x1066, 20 planes + 50 boxes, 30% low texture."""
+    'c2': 11 views, 1600x1066, 20 planes + 50 boxes, 30% low texture;
+    'c5': 21 views, 3840x2160 (the field of view of c2), 20 planes + 50
+    boxes, 50% low texture."""
     if name == "c1":
         return scene_config(seed=1)
     if name == "c2":
         return scene_config(seed=2, width=1600, height=1066, fx=1250.0, fy=1250.0, cx=800.0, cy=533.0,
                             num_views=11, arc_length=0.5, num_planes=20, num_boxes=50,
                             low_texture_fraction=0.3)
+    if name == "c5":
+        return scene_config(seed=5, width=3840, height=2160, fx=3000.0, fy=3000.0, cx=1920.0, cy=1080.0,
+                            num_views=21, arc_length=0.5, num_planes=20, num_boxes=50,
+                            low_texture_fraction=0.5)
     raise ValueError(name)
 
 

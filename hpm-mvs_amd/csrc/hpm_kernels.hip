@@ -161,21 +161,24 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
     return true;
 }
 
+/* Paths of the window sums, chosen per wave: every active lane has an inner
+ * warp (the cubic reciprocal, no clamp), a small warp (the cubic, with the
+ * clamp), or any warp (per lane the cubic or the quotient, with the clamp).
+ * The three give the same values per lane. */
+enum : int { kPathInner = 0, kPathSmall = 1, kPathAny = 2 };
+
 /* Source coordinate of tap (dx, dy): byte offset of the top-left texel of
- * its bilinear footprint in the padded pool and the bilinear fractions.
- * kInner: every active lane of the wave has an inner warp, so 1 / (1 + u) is
- * the cubic and the clamp is the identity, without per-lane choices; both
- * forms give the same values per lane. */
-template <bool kInner>
+ * its bilinear footprint in the padded pool and the bilinear fractions. */
+template <int kPath>
 __device__ __forceinline__ void tap_coord(const Warp& w, float pitchf, float dx, float dy, uint32_t* off, float* fa,
                                           float* fb)
 {
     const float u = fmaf(w.ex, dx, w.ey * dy);
     const float cubic = fmaf(u, fmaf(u, 1.0f - u, -1.0f), 1.0f);
-    const float inv = (kInner || w.small) ? cubic : 1.0f / (1.0f + u);
+    const float inv = (kPath != kPathAny || w.small) ? cubic : 1.0f / (1.0f + u);
     float lx = fmaf(fmaf(w.gxx, dx, w.gxy * dy), inv, w.fx0);
     float ly = fmaf(fmaf(w.gyx, dx, w.gyy * dy), inv, w.fy0);
-    if (!kInner) {
+    if (kPath != kPathInner) {
         lx = fminf(fmaxf(lx, w.lox), w.hix);
         ly = fminf(fmaxf(ly, w.loy), w.hiy);
     }
@@ -280,14 +283,14 @@ __device__ __forceinline__ void make_ref_patch(const float* img, int W, int H, i
 
 /* One column of the window (6 taps at x offset dx): coordinates, then the
  * 6 gathers. */
-template <bool kInner>
+template <int kPath>
 __device__ __forceinline__ void fetch_column(__amdgpu_buffer_rsrc_t pool, uint32_t pitch, const Warp& w, float dx,
                                              Footprint* quad, float* fa, float* fb)
 {
     uint32_t off[6];
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        tap_coord<kInner>(w, (float)pitch, dx, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
+        tap_coord<kPath>(w, (float)pitch, dx, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
     }
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
@@ -316,7 +319,7 @@ __device__ __forceinline__ void accumulate_column(const RefPatch& rp, int a, flo
 /* Window sums of one (hypothesis, source view), software-pipelined by
  * column: the gathers of column a + 1 are in flight while column a is
  * accumulated. Tap k = 6 a + b sits at offset (2a - 5, 2b - 5). */
-template <bool kInner>
+template <int kPath>
 __device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, uint32_t pitch, float one, const RefPatch& rp,
                                          const Warp& w, float* ss_out, float* sss_out, float* srs_out)
 {
@@ -327,21 +330,21 @@ __device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, uint32_t p
      * offset (-1, -1)), sampled first: a constant patch sums to exactly 0 */
     uint32_t coff;
     float cfa, cfb;
-    tap_coord<kInner>(w, (float)pitch, -1.0f, -1.0f, &coff, &cfa, &cfb);
+    tap_coord<kPath>(w, (float)pitch, -1.0f, -1.0f, &coff, &cfa, &cfb);
     const Footprint cq = footprint(pool, coff);
-    fetch_column<kInner>(pool, pitch, w, -5.0f, cur, fa, fb);
+    fetch_column<kPath>(pool, pitch, w, -5.0f, cur, fa, fb);
     const float shift = bilerp(cq, cfa, cfb, one);
     /* two columns per trip with the buffers alternating; the last two
      * columns follow the loop, so every prefetch is unconditional and the
      * waits stay on the older loads */
 #pragma unroll 1
     for (int a = 0; a < 4; a += 2) {
-        fetch_column<kInner>(pool, pitch, w, (float)(2 * a - 3), nxt, nfa, nfb);
+        fetch_column<kPath>(pool, pitch, w, (float)(2 * a - 3), nxt, nfa, nfb);
         accumulate_column(rp, a, shift, cur, fa, fb, one, &ss, &sss, &srs);
-        fetch_column<kInner>(pool, pitch, w, (float)(2 * a - 1), cur, fa, fb);
+        fetch_column<kPath>(pool, pitch, w, (float)(2 * a - 1), cur, fa, fb);
         accumulate_column(rp, a + 1, shift, nxt, nfa, nfb, one, &ss, &sss, &srs);
     }
-    fetch_column<kInner>(pool, pitch, w, 5.0f, nxt, nfa, nfb);
+    fetch_column<kPath>(pool, pitch, w, 5.0f, nxt, nfa, nfb);
     accumulate_column(rp, 4, shift, cur, fa, fb, one, &ss, &sss, &srs);
     accumulate_column(rp, 5, shift, nxt, nfa, nfb, one, &ss, &sss, &srs);
     *ss_out = ss;
@@ -362,9 +365,11 @@ __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& 
     const __amdgpu_buffer_rsrc_t pool = pool_rsrc(pb.images_pad);
     float ss, sss, srs;
     if (__all(w.inner)) {
-        ncc_sums<true>(pool, pitch, pb.one, rp, w, &ss, &sss, &srs);
+        ncc_sums<kPathInner>(pool, pitch, pb.one, rp, w, &ss, &sss, &srs);
+    } else if (__all(w.small)) {
+        ncc_sums<kPathSmall>(pool, pitch, pb.one, rp, w, &ss, &sss, &srs);
     } else {
-        ncc_sums<false>(pool, pitch, pb.one, rp, w, &ss, &sss, &srs);
+        ncc_sums<kPathAny>(pool, pitch, pb.one, rp, w, &ss, &sss, &srs);
     }
     const float ms = ss * rp.inv_w;
     const float var_s = sss * rp.inv_w - ms * ms;

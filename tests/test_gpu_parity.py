@@ -213,3 +213,53 @@ def test_full_run_parity_c1(hpm, cpu, c1_full):
     assert stats.n_eval == o.n_eval or ident < 1.0
     assert acc_g > 0.8
     pr.close()
+
+
+def _resized_view(hpm, cam, img, w, h):
+    """a source view at another resolution: the image resized, K scaled"""
+    import copy
+
+    c = copy.copy(cam)
+    sx, sy = w / cam.width, h / cam.height
+    K = list(c.K)
+    K[0] *= sx
+    K[2] *= sx
+    K[4] *= sy
+    K[5] *= sy
+    for i in range(9):
+        c.K[i] = K[i]
+    c.width, c.height = w, h
+    return c, hpm.resize_bilinear(img, w, h)
+
+
+@pytest.mark.parametrize("case", ["one_source", "max_sources", "tiny", "odd", "mixed_sizes"])
+def test_full_run_edge_cases(hpm, cpu, case):
+    """Full RunPatchMatch against the oracle, bit for bit (hypotheses, costs,
+    evaluation count), on shapes at the edges of the problem: a single
+    source, the maximum of 32 sources, an image smaller than the 11x11
+    window, odd sizes, and sources at other resolutions than the reference."""
+    from conftest import problem_inputs
+
+    sizes = {"one_source": (40, 30, 2), "max_sources": (48, 36, 33), "tiny": (9, 7, 5), "odd": (97, 53, 5),
+             "mixed_sizes": (96, 72, 5)}
+    w, h, nv = sizes[case]
+    f = 0.8 * w
+    cfg = hpm.scene_config(seed=40 + len(case), width=w, height=h, fx=f, fy=f, cx=w / 2.0, cy=h / 2.0, num_views=nv)
+    cams, images, gd, gn, low, order = problem_inputs(hpm, cfg, nv // 2)
+    if case == "mixed_sizes":
+        cams[1], images[1] = _resized_view(hpm, cams[1], images[1], 64, 48)
+        cams[3], images[3] = _resized_view(hpm, cams[3], images[3], 120, 90)
+    say(f"case: full run {case}: {w}x{h}, {len(cams) - 1} sources")
+    pr = make_problem(hpm, cams, images)
+    pr.RunPatchMatch()
+    g_hyp, g_cost = pr.hypotheses(), pr.costs()
+    stats = pr.run_stats()
+    o = cpu.Oracle(cams, images, pr.params(), salt=0)
+    o.run_patchmatch()
+    same_h = np.all(g_hyp.view(np.uint32) == o.hyp.view(np.uint32), axis=2)
+    same_c = (g_cost.view(np.uint32) == o.cost.view(np.uint32)) | (np.isnan(g_cost) & np.isnan(o.cost))
+    say(f"identical hypotheses {same_h.mean():.5f}, costs {same_c.mean():.5f}, n_eval gpu={stats.n_eval} "
+        f"cpu={o.n_eval}")
+    assert same_h.all() and same_c.all()
+    assert stats.n_eval == o.n_eval
+    pr.close()

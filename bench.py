@@ -16,9 +16,11 @@ methods:
 value = ms per reference view (step time / 11) of the learned-prior schedule
 when its depth accuracy (fraction of pixels within 2% of ground truth) is at
 least the plain method's, else of the plain method; both are in config.
-Images are resident in HBM before the timed region. "--workload plain" times
-one plain RunPatchMatch per view instead; "--workload batch" the batched form.
-Prints one JSON line.
+Images are resident in HBM before the timed region. Other workloads:
+"plain" (one plain RunPatchMatch per view), "batch" (the batched form over 32
+mixed-size references), "fusion" (RunFusion of the 11 views), "large" (21
+views of 3840x2160, 20 sources), "lprior" (the learned prior alone on one
+view, with the MFMA roofline of its network). Prints one JSON line.
 """
 import argparse
 import importlib.util
@@ -33,6 +35,8 @@ if ROOT not in sys.path:
 
 FLOPS_PER_EVAL = 1190  # 36 taps x 32 + 24 homography + 14 NCC finalize
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0  # dense MFMA
+PEAK_HBM_GBPS = 8000.0
 METRIC = "ms/ref view @1600x1066,10 src: plain vs learned-prior, equal depth accuracy"
 METRIC_LARGE = "ms/ref view @3840x2160,20 src: plain vs learned-prior, equal depth accuracy"
 
@@ -59,17 +63,20 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hpm_k_propagate_1600x1066_10sr
 
 
 def measured_traffic():
-    """HBM (data-fabric) bytes per propagation launch of the plain
-    1600x1066 pass, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    committed under profiles/ (None when absent)."""
+    """(HBM bytes per propagation launch, L2 hit rate) of the plain 1600x1066
+    pass, from the rocprofv3 FETCH_SIZE / WRITE_SIZE and TCC_HIT / TCC_MISS
+    runs committed under profiles/ (None where absent)."""
+    found = {"traffic_bytes_per_launch": None, "l2_hit_rate": None}
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return int(line.split(":")[1])
+                key = line.split(":")[0]
+                if key in found:
+                    found[key] = float(line.split(":")[1])
     except OSError:
         pass
-    return None
+    t = found["traffic_bytes_per_launch"]
+    return (int(t) if t is not None else None), found["l2_hit_rate"]
 
 
 def view_order(ref, n):
@@ -99,6 +106,7 @@ def cpu_baseline(cams, imgs, ref, evals_per_view, host_ms_per_view, threads):
         "unit": "ms/ref view",
         "cores": threads,
         "kind": "port",
+        "cpu_model": cpu_model(),
         "sample": f"init + iteration-0 black half of view {ref} at 1600x1066 with 10 sources "
                   f"({n} NCC evaluations in {dt:.1f} s), scaled to the {evals_per_view} evaluations of a whole "
                   f"view, plus {host_ms_per_view:.0f} ms of host prior construction",
@@ -236,12 +244,110 @@ def fusion_bench(hpm, args):
     }), flush=True)
 
 
+def lprior_bench(hpm, args):
+    """The learned prior alone on view 5 of the 11-view 1600x1066 scene:
+    one plain RunPatchMatch, then hpm_lprior_apply_timed (input features,
+    U-Net forward on the BF16 MFMA convolutions, NCC judge) timed over the
+    steps with events on the handle's stream. roofline: the network's
+    convolution flops (2 H W Ci Co k^2 per layer) over the network's device
+    time (pooling and upsampling included) against the dense BF16 peak.
+    cpu_baseline: the same network in float32 on the host cores
+    (oracle/unet_cpu.py). Prints one JSON line (ms per application)."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from hpm_mvs_amd import unet_prior as up
+    from oracle import unet_cpu
+
+    cfg = hpm.workload_config("c2")
+    cams, imgs, gd, gn, low = hpm.render_scene(cfg)
+    order = view_order(5, cfg.num_views)
+    pr = hpm.Problem(0)
+    pr.input_init_memory([cams[i] for i in order], [imgs[i].astype(np.float32) for i in order], ref_image_id=0)
+    pr.CudaSpaceInitialization()
+    pr.RunPatchMatch()
+    log("plain pass done")
+    lib = hpm.lib()
+    lp = lib.hpm_lprior_load(os.path.abspath(args.learned_prior).encode(), 0)
+    if not lp:
+        raise SystemExit(f"cannot load {args.learned_prior}")
+    kept = C.c_float()
+    ms = (C.c_float * 3)()
+    for _ in range(args.warmup):
+        if lib.hpm_lprior_apply_timed(lp, pr._h, C.byref(kept), ms) != 0:
+            raise SystemExit("hpm_lprior_apply_timed failed")
+    parts = np.zeros(3)
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        if lib.hpm_lprior_apply_timed(lp, pr._h, C.byref(kept), ms) != 0:
+            raise SystemExit("hpm_lprior_apply_timed failed")
+        parts += np.array(ms[:])
+    wall = (time.perf_counter() - t0) * 1e3 / args.steps
+    parts /= args.steps
+    lib.hpm_lprior_destroy(lp)
+    pr.close()
+    log(f"apply: {wall:.2f} ms wall; features {parts[0]:.3f}, network {parts[1]:.3f}, judge {parts[2]:.3f} ms; "
+        f"kept {kept.value:.3f}")
+    W, H = cfg.width, cfg.height
+    flops = unet_cpu.flops(up.LAYERS, H, W)
+    achieved = flops / (parts[1] * 1e-3) / 1e12
+    out = {
+        "metric": "ms per learned-prior application (U-Net + judge) @1600x1066, 10 src",
+        "value": round(wall, 3),
+        "unit": "ms/view",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded 11-view scene; trained weights hpm-mvs_amd/unet_prior.bin)",
+        "config": {"workload": "hpm_lprior_apply on view 5 after a plain RunPatchMatch: features, U-Net "
+                               "(32-64-96-128, 12 convolutions), NCC judge; 1600x1066, 10 sources",
+                   "ms_features": round(parts[0], 3), "ms_network": round(parts[1], 3),
+                   "ms_judge": round(parts[2], 3), "kept_fraction": round(kept.value, 4)},
+        "roofline": {"bound": "mfma", "kernel": "hpm_k_conv_fwd (+ pooling, upsampling)",
+                     "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_BF16_TFLOPS, 4), "traffic": None,
+                     "flops_per_forward": flops},
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or (os.cpu_count() or 1)
+        torch.set_num_threads(threads)
+        ws, bs = up.load(args.learned_prior)
+        x = torch.randn(H, W, 32).to(torch.bfloat16)
+        t0 = time.perf_counter()
+        unet_cpu.forward(x, ws, bs, up.LAYERS, dtype=torch.float32, store_bf16=False)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        out["cpu_baseline"] = {"value": round(cpu_ms, 1), "unit": "ms/view (network only)", "cores": threads,
+                               "kind": "port", "cpu_model": cpu_model(),
+                               "sample": "one float32 forward of the same network at 1600x1066 "
+                                         "(oracle/unet_cpu.py, torch CPU convolutions)"}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_model():
+    """the host CPU's model name (/proc/cpuinfo), for the CPU baseline"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["full", "plain", "batch", "fusion", "large"], default="full")
+    ap.add_argument("--workload", choices=["full", "plain", "batch", "fusion", "large", "lprior"], default="full")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workers", type=int, default=0, help="host workers of the schedule (0: library default)")
@@ -276,6 +382,9 @@ def main():
         return
     if args.workload == "fusion":
         fusion_bench(hpm, args)
+        return
+    if args.workload == "lprior":
+        lprior_bench(hpm, args)
         return
     large = args.workload == "large"
     cfg = hpm.workload_config("c5" if large else "c2")
@@ -369,6 +478,7 @@ def main():
     avg_launch_ms = rst.ms_propagate / rst.propagate_launches
     flops_per_launch = rst.n_eval_propagate / rst.propagate_launches * FLOPS_PER_EVAL
     achieved = flops_per_launch / (avg_launch_ms * 1e-3) / 1e12
+    traffic, l2_hit = measured_traffic()
 
     if args.workload in ("full", "large"):
         scales = "0.25x and 0.5x planar priors with JBU" if large else "0.5x planar prior with JBU"
@@ -417,9 +527,12 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": measured_traffic(),
+            "traffic": traffic,
             "avg_launch_ms": round(avg_launch_ms, 3),
             "flops_per_launch": flops_per_launch,
+            "hbm_gbps": round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1) if traffic else None,
+            "hbm_frac": round(traffic / (avg_launch_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4) if traffic else None,
+            "l2_hit_rate": l2_hit,
         },
     }
     if not args.no_cpu_baseline:

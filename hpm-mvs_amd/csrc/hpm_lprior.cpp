@@ -215,7 +215,7 @@ extern "C" int hpm_lprior_forward(hpm_lprior* lp, const void* x, int height, int
     return (int)(e != hipSuccess ? e : e2);
 }
 
-extern "C" int hpm_lprior_apply(hpm_lprior* lp, hpm_handle* h, float* accepted)
+extern "C" int hpm_lprior_apply_timed(hpm_lprior* lp, hpm_handle* h, float* accepted, float* ms)
 {
     if (!lp || !h) {
         return HPM_EINVAL;
@@ -233,24 +233,51 @@ extern "C" int hpm_lprior_apply(hpm_lprior* lp, hpm_handle* h, float* accepted)
     if (rc != HPM_OK) {
         return rc;
     }
-    char* base = (char*)ws;
-    hipStream_t s = h->stream;
-    HPM_HIP(h, (hipError_t)hpm_launch_lprior_features(&pb, base + pl.x, s));
-    float* head = (float*)(base + pl.head);
-    HPM_HIP(h, forward(lp, base + pl.x, W, H, head, base, pl, s));
-    /* judge into the handle's prior */
     rc = hpm_prior_on_device(h);
     if (rc != HPM_OK) {
         return rc;
     }
+    /* events of this call only: several threads may apply one lp */
+    hipEvent_t ev[4] = {};
+    hipError_t e = hipSuccess;
+    if (ms) {
+        for (hipEvent_t& x : ev) {
+            if (e == hipSuccess) e = hipEventCreate(&x);
+        }
+    }
+    auto mark = [&](int i) { return ms ? hipEventRecord(ev[i], h->stream) : hipSuccess; };
+    char* base = (char*)ws;
+    hipStream_t s = h->stream;
+    float* head = (float*)(base + pl.head);
     unsigned long long* cnt = (unsigned long long*)(base + pl.cnt);
-    HPM_HIP(h, (hipError_t)hpm_launch_zero_u64(cnt, 1, s));
-    HPM_HIP(h, (hipError_t)hpm_launch_lprior_judge(&pb, base + pl.x, head, h->d_prior, h->d_mask, cnt, s));
     unsigned long long kept = 0;
-    HPM_HIP(h, hipMemcpyAsync(&kept, cnt, 8, hipMemcpyDeviceToHost, s));
-    HPM_HIP(h, hipStreamSynchronize(s));
+    if (e == hipSuccess) e = mark(0);
+    if (e == hipSuccess) e = (hipError_t)hpm_launch_lprior_features(&pb, base + pl.x, s);
+    if (e == hipSuccess) e = mark(1);
+    if (e == hipSuccess) e = forward(lp, base + pl.x, W, H, head, base, pl, s);
+    if (e == hipSuccess) e = mark(2);
+    /* judge into the handle's prior */
+    if (e == hipSuccess) e = (hipError_t)hpm_launch_zero_u64(cnt, 1, s);
+    if (e == hipSuccess) e = (hipError_t)hpm_launch_lprior_judge(&pb, base + pl.x, head, h->d_prior, h->d_mask, cnt, s);
+    if (e == hipSuccess) e = mark(3);
+    if (e == hipSuccess) e = hipMemcpyAsync(&kept, cnt, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (ms) {
+        for (int i = 0; i < 3; ++i) {
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+        }
+        for (hipEvent_t& x : ev) {
+            if (x) (void)hipEventDestroy(x);
+        }
+    }
+    HPM_HIP(h, e);
     if (accepted) {
         *accepted = (float)((double)kept / ((double)W * H));
     }
     return HPM_OK;
+}
+
+extern "C" int hpm_lprior_apply(hpm_lprior* lp, hpm_handle* h, float* accepted)
+{
+    return hpm_lprior_apply_timed(lp, h, accepted, nullptr);
 }

@@ -55,6 +55,10 @@ typedef struct hpm_handle hpm_handle;
 hpm_lprior* hpm_lprior_load(const char* path, int device);
 void hpm_lprior_destroy(hpm_lprior* lp);
 int hpm_lprior_apply(hpm_lprior* lp, hpm_handle* h, float* accepted);
+/* hpm_lprior_apply, also timing its parts on the handle's stream with
+ * events of its own: ms[0] the input features, ms[1] the network, ms[2] the
+ * judge (device milliseconds; ms may be NULL). */
+int hpm_lprior_apply_timed(hpm_lprior* lp, hpm_handle* h, float* accepted, float* ms);
 /* The network alone: head (FP32, height*width*32; channel 0 the depth
  * residual, 1..3 the normal residual) of the features x (BF16,
  * height*width*32, as hpm-mvs_amd/unet_prior.py builds them); synchronous,

@@ -31,45 +31,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "..", "hpm-mvs_amd", "unet_prior.bin")
 
 
-def _bf(t):
-    return t.to(torch.bfloat16).double()
-
-
 def _cpu_forward(x, ws, bs, up):
-    """float64 restatement of the C++ graph; x (H, W, 32) BF16 CPU tensor"""
-    F = torch.nn.functional
-    L = {name: i for i, (name, *_) in enumerate(up.LAYERS)}
+    """float64 restatement of the C++ graph (oracle/unet_cpu.py); x (H, W,
+    32) BF16 CPU tensor"""
+    from oracle import unet_cpu
 
-    def conv(name, t, store_bf16=True):
-        i = L[name]
-        _, ci, co, ks, relu = up.LAYERS[i]
-        w = ws[i].double().reshape(co, ks, ks, ci).permute(0, 3, 1, 2)
-        y = F.conv2d(t.permute(2, 0, 1)[None], w, bs[i].double(), padding=ks // 2)[0].permute(1, 2, 0)
-        if relu:
-            y = torch.relu(y)
-        return _bf(y) if store_bf16 else y
-
-    def pool(t):
-        h, w, c = t.shape
-        oh, ow = (h + 1) // 2, (w + 1) // 2
-        tp = F.pad(t.permute(2, 0, 1), (0, 2 * ow - w, 0, 2 * oh - h))
-        n = F.pad(torch.ones(1, h, w, dtype=torch.float64), (0, 2 * ow - w, 0, 2 * oh - h))
-        s = F.avg_pool2d(tp[None], 2, divisor_override=1)[0]
-        cnt = F.avg_pool2d(n[None], 2, divisor_override=1)[0]
-        return _bf((s / cnt).permute(1, 2, 0))
-
-    def upsample(t, h, w):
-        return t[torch.arange(h) // 2][:, torch.arange(w) // 2]
-
-    x = x.double()
-    s0 = conv("e0b", conv("e0a", x))
-    s1 = conv("e1b", conv("e1a", pool(s0)))
-    s2 = conv("e2b", conv("e2a", pool(s1)))
-    b3 = conv("e3b", conv("e3a", pool(s2)))
-    d2 = conv("d2", torch.cat([upsample(b3, *s2.shape[:2]), s2], dim=2))
-    d1 = conv("d1", torch.cat([upsample(d2, *s1.shape[:2]), s1], dim=2))
-    d0 = conv("d0", torch.cat([upsample(d1, *s0.shape[:2]), s0], dim=2))
-    return conv("head", d0, store_bf16=False)
+    return unet_cpu.forward(x, ws, bs, up.LAYERS)
 
 
 def _features(up, h, w, seed):

@@ -20,7 +20,8 @@ Images are resident in HBM before the timed region. Other workloads:
 "plain" (one plain RunPatchMatch per view), "batch" (the batched form over 32
 mixed-size references), "fusion" (RunFusion of the 11 views), "large" (21
 views of 3840x2160, 20 sources), "lprior" (the learned prior alone on one
-view, with the MFMA roofline of its network). Prints one JSON line.
+view, with the MFMA roofline of its network), "acmm" (the ACMM+NESP
+coarse-to-fine schedule). Prints one JSON line.
 """
 import argparse
 import importlib.util
@@ -244,6 +245,63 @@ def fusion_bench(hpm, args):
     }), flush=True)
 
 
+def acmm_bench(hpm, args):
+    """The ACMM+NESP coarse-to-fine schedule (hpm_acmm_run) on the 11-view
+    1600x1066 scene: levels 800x533 and 1600x1066, each a first pass (plain,
+    then hierarchy) and 2 geometric passes of 2 iterations per view. Timed
+    over the steps; depth accuracy of the final maps. Prints one JSON line
+    (ms per reference view)."""
+    import numpy as np
+
+    cfg = hpm.workload_config("c2")
+    cams, imgs, gd, gn, low = hpm.render_scene(cfg)
+    n = cfg.num_views
+    fimgs = [im.astype(np.float32) for im in imgs]
+    problems = [(r, view_order(r, n)[1:]) for r in range(n)]
+    opt = hpm.acmm_options()
+    if args.workers > 0:
+        opt.workers = args.workers
+    for s in range(args.warmup):
+        pool, st = hpm.run_acmm(cams, fimgs, problems, opt)
+        pool.close()
+        log(f"acmm warmup {s}: {st.runs} RunPatchMatch")
+    t0 = time.perf_counter()
+    evals = 0
+    for s in range(args.steps):
+        pool, st = hpm.run_acmm(cams, fimgs, problems, opt)
+        evals += st.n_eval
+        if s + 1 < args.steps:
+            pool.close()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    acc = []
+    for v in range(n):
+        hyp, _ = pool.state(v)
+        g = gd[v]
+        valid = g > 0
+        acc.append(float(np.mean(np.abs(hyp[..., 3][valid] - g[valid]) / g[valid] < 0.02)))
+    pool.close()
+    log(f"acmm: {ms / n:.1f} ms/view, accuracy {np.mean(acc):.4f}")
+    print(json.dumps({
+        "metric": "ms/ref view, ACMM+NESP coarse-to-fine schedule @1600x1066, 10 src",
+        "value": round(ms / n, 3),
+        "unit": "ms/ref view",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic (seeded {n}-view scene rendered on the host)",
+        "config": {"workload": "ACMM+NESP: levels 800x533 and 1600x1066 (scale bound 1000), per level a first "
+                               "pass (plain, then hierarchy from the coarser maps) and 2 geometric passes of 2 "
+                               "iterations, every view a reference with the other 10 as sources",
+                   "runs_per_view": st.runs / n, "n_eval_per_view": int(evals / args.steps / n),
+                   "accuracy_2pct": round(float(np.mean(acc)), 4)},
+    }), flush=True)
+
+
 def lprior_bench(hpm, args):
     """The learned prior alone on view 5 of the 11-view 1600x1066 scene:
     one plain RunPatchMatch, then hpm_lprior_apply_timed (input features,
@@ -347,7 +405,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["full", "plain", "batch", "fusion", "large", "lprior"], default="full")
+    ap.add_argument("--workload", choices=["full", "plain", "batch", "fusion", "large", "lprior", "acmm"], default="full")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workers", type=int, default=0, help="host workers of the schedule (0: library default)")
@@ -385,6 +443,9 @@ def main():
         return
     if args.workload == "lprior":
         lprior_bench(hpm, args)
+        return
+    if args.workload == "acmm":
+        acmm_bench(hpm, args)
         return
     large = args.workload == "large"
     cfg = hpm.workload_config("c5" if large else "c2")

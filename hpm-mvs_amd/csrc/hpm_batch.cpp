@@ -219,6 +219,9 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         if (rc != HPM_OK) {
             return bfail(b, rc, "problem " + std::to_string(i) + ": " + h->err);
         }
+        if (h->params.acmm) {
+            return bfail(b, HPM_EINVAL, "problem " + std::to_string(i) + ": the batched form runs HPM-MVS problems");
+        }
         hpm_fill_problem(h, &pbs[(size_t)i]);
         key.push_back(pbs[(size_t)i].W);
         key.push_back(pbs[(size_t)i].H);

@@ -69,6 +69,12 @@ typedef struct hpm_dev_problem {
     const float* prior_planes; /* float4 per pixel */
     const uint32_t* prior_mask;
     unsigned long long* n_eval; /* NCC evaluation counter (may be null) */
+    /* ACMM hierarchy (params.hierarchy): the coarser map, float4 per coarse
+     * pixel (world normal xyz, depth), its JBU window scale, and the
+     * pre-cost per pixel */
+    const float* coarse;
+    int32_t coarse_w, coarse_h, coarse_scale;
+    float* pre_cost;
 } hpm_dev_problem;
 
 /* Colour-grid mapping of the checkerboard kernels: a wave (64 lanes) takes

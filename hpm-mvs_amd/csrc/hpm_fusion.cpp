@@ -69,10 +69,13 @@ uint8_t grey_byte(float v)
     return (uint8_t)(i < 0 ? 0 : i > 255 ? 255 : i);
 }
 
-} // namespace
-
-extern "C" int hpm_fusion_run(hpm_pool* pool, int num_views, const int* num_src, const int* src_ids, int max_src,
-                              int geom_consistency, hpm_point** points, int64_t* num_points, char* err, int err_len)
+/* RunFusion over the pool: HPM-MVS's rule, or with acmm ACMM+NESP's
+ * (ACMM+NESP/windows/main.cpp:164-300: the point, normal and colour are the
+ * means of the reference pixel's and its consistent sources', the normal
+ * through cv::Vec3f /= float, a product with the float reciprocal; a point
+ * needs 2 consistent sources) */
+int fusion_run(bool acmm, hpm_pool* pool, int num_views, const int* num_src, const int* src_ids, int max_src,
+               int geom_consistency, hpm_point** points, int64_t* num_points, char* err, int err_len)
 {
     if (!pool || !num_src || !src_ids || !points || !num_points || num_views < 1 || num_views > pool->n ||
         max_src < 1) {
@@ -131,6 +134,17 @@ extern "C" int hpm_fusion_run(hpm_pool* pool, int num_views, const int* num_src,
     for (int i = 0; i < num_views; ++i) {
         masks[(size_t)i].assign((size_t)pool->cams[(size_t)i].width * pool->cams[(size_t)i].height, 0);
     }
+    /* ACMM: every view's depth and normals on the host, for the sums */
+    std::vector<float> all_depth, all_hyp;
+    if (ok && acmm) {
+        all_depth.resize((size_t)pool->total);
+        all_hyp.resize((size_t)pool->total * 4);
+        ok = hipMemcpyAsync(all_depth.data(), depth_pool, all_depth.size() * 4, hipMemcpyDeviceToHost, s) ==
+                 hipSuccess &&
+             hipMemcpyAsync(all_hyp.data(), pool->d_state_hyp, all_hyp.size() * 4, hipMemcpyDeviceToHost, s) ==
+                 hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+    }
     std::vector<hpm_point> out;
     for (int i = 0; ok && i < num_views; ++i) {
         const hpm_camera& rc = pool->cams[(size_t)i];
@@ -179,6 +193,13 @@ extern "C" int hpm_fusion_run(hpm_pool* pool, int num_views, const int* num_src,
             }
             int num_consistent = 0;
             float dyn = 0.0f;
+            const int c = (int)(p % (size_t)cols), r = (int)(p / (size_t)cols);
+            P3 sx{0.0f, 0.0f, 0.0f};
+            float sn[3] = {h_hyp[4 * p], h_hyp[4 * p + 1], h_hyp[4 * p + 2]};
+            float sc = (float)grey_byte(grey[p]);
+            if (acmm) {
+                sx = world_point(c, r, h_depth[p], rc);
+            }
             for (int j = 0; j < nn; ++j) {
                 if (((any >> j) & 1u) == 0u) {
                     continue;
@@ -191,20 +212,45 @@ extern "C" int hpm_fusion_run(hpm_pool* pool, int num_views, const int* num_src,
                 const int sw = pool->cams[(size_t)sv].width;
                 ux[j] = q % sw;
                 uy[j] = q / sw;
+                if (acmm) {
+                    /* the source's point as the consistency test built it */
+                    const size_t g = (size_t)pool->off[(size_t)sv] + (size_t)q;
+                    const P3 Y = world_point(ux[j], uy[j], all_depth[g], pool->cams[(size_t)sv]);
+                    sx.x += Y.x;
+                    sx.y += Y.y;
+                    sx.z += Y.z;
+                    sn[0] = sn[0] + all_hyp[4 * g];
+                    sn[1] = sn[1] + all_hyp[4 * g + 1];
+                    sn[2] = sn[2] + all_hyp[4 * g + 2];
+                    sc += (float)grey_byte(pool->images[(size_t)sv][(size_t)q]);
+                }
                 dyn += h_w[p * nn + j];
                 ++num_consistent;
             }
-            if (num_consistent >= 1 && (double)dyn > 0.3 * (double)num_consistent) {
-                const int c = (int)(p % (size_t)cols), r = (int)(p / (size_t)cols);
-                const P3 X = world_point(c, r, h_depth[p], rc);
+            const bool keep =
+                acmm ? num_consistent >= 2 : (num_consistent >= 1 && (double)dyn > 0.3 * (double)num_consistent);
+            if (keep) {
                 hpm_point pt;
-                pt.x = X.x;
-                pt.y = X.y;
-                pt.z = X.z;
-                pt.nx = h_hyp[4 * p];
-                pt.ny = h_hyp[4 * p + 1];
-                pt.nz = h_hyp[4 * p + 2];
-                pt.r = pt.g = pt.b = grey_byte(grey[p]);
+                if (acmm) {
+                    const float k = num_consistent + 1.0f;
+                    const float ik = 1.0f / k;
+                    pt.x = sx.x / k;
+                    pt.y = sx.y / k;
+                    pt.z = sx.z / k;
+                    pt.nx = sn[0] * ik;
+                    pt.ny = sn[1] * ik;
+                    pt.nz = sn[2] * ik;
+                    pt.r = pt.g = pt.b = (uint8_t)(int)(sc / k);
+                } else {
+                    const P3 X = world_point(c, r, h_depth[p], rc);
+                    pt.x = X.x;
+                    pt.y = X.y;
+                    pt.z = X.z;
+                    pt.nx = sn[0];
+                    pt.ny = sn[1];
+                    pt.nz = sn[2];
+                    pt.r = pt.g = pt.b = grey_byte(grey[p]);
+                }
                 pt.pad_ = 0;
                 out.push_back(pt);
                 for (int j = 0; j < nn; ++j) {
@@ -240,6 +286,23 @@ extern "C" int hpm_fusion_run(hpm_pool* pool, int num_views, const int* num_src,
     *points = arr;
     *num_points = (int64_t)out.size();
     return HPM_OK;
+}
+
+} // namespace
+
+extern "C" int hpm_fusion_run(hpm_pool* pool, int num_views, const int* num_src, const int* src_ids, int max_src,
+                              int geom_consistency, hpm_point** points, int64_t* num_points, char* err, int err_len)
+{
+    return fusion_run(false, pool, num_views, num_src, src_ids, max_src, geom_consistency, points, num_points, err,
+                      err_len);
+}
+
+extern "C" int hpm_fusion_run_acmm(hpm_pool* pool, int num_views, const int* num_src, const int* src_ids,
+                                   int max_src, int geom_consistency, hpm_point** points, int64_t* num_points,
+                                   char* err, int err_len)
+{
+    return fusion_run(true, pool, num_views, num_src, src_ids, max_src, geom_consistency, points, num_points, err,
+                      err_len);
 }
 
 extern "C" void hpm_fusion_free(hpm_point* points)

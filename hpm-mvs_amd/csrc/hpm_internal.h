@@ -71,6 +71,11 @@ struct hpm_handle {
     unsigned long long* d_neval = nullptr;
     float* d_jbu_src = nullptr;   /* low-resolution prior input of the JBU */
     size_t jbu_src_cap = 0;       /* bytes */
+    float* d_coarse = nullptr;    /* ACMM hierarchy: coarser map, float4 (world normal, depth) */
+    float* d_pre_cost = nullptr;  /* ACMM hierarchy: pre-cost per pixel */
+    size_t coarse_cap = 0;        /* coarse pixels */
+    int coarse_w = 0, coarse_h = 0, coarse_scale = 0;
+    bool have_coarse = false;
     void* d_ws = nullptr;         /* learned-prior activations */
     size_t ws_cap = 0;            /* bytes */
     size_t alloc_pixels = 0;
@@ -109,6 +114,9 @@ int hpm_workspace(hpm_handle* h, size_t bytes, void** ws);
 /* RunPatchMatch with an optional host readback (the C-ABI always reads
  * back; the pass schedule skips it where the host does not need the maps). */
 int hpm_run_internal(hpm_handle* h, bool readback);
+/* hpm_set_hierarchy_input from a host or a device buffer. */
+int hpm_set_hierarchy_internal(hpm_handle* h, const float* coarse_world4, int coarse_w, int coarse_h,
+                               bool on_device);
 /* Copies the handle's finished (world normal, depth) hypotheses and costs
  * into the pool slot of its reference view and the depth into the plain
  * (which_depth 0) or geometric (1) depth map of that view. */

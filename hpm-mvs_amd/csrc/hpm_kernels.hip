@@ -509,9 +509,10 @@ __device__ __forceinline__ void project(const float X[3], const hpm_camera& c, f
     *v = (c.K[3] * t[0] + c.K[4] * t[1] + c.K[5] * t[2]) / *d;
 }
 
-/* ComputeGeomConsistencyCost (HPM.cu:483-508) */
+/* ComputeGeomConsistencyCost (HPM.cu:483-508; ACMM.cu:494-519 caps it at 3) */
 __device__ float geom_cost(const hpm_dev_problem& pb, int v, float4 plane, int px, int py)
 {
+    const float max_cost = pb.params.acmm ? 3.0f : 5.0f;
     const hpm_camera rc = camera(pb, 0);
     const hpm_camera sc = camera(pb, v);
     const float depth = depth_from_plane(rc, plane, px, py);
@@ -525,7 +526,7 @@ __device__ float geom_cost(const hpm_dev_problem& pb, int v, float4 plane, int p
     const int iy = clampi((int)cyf, 0, sc.height - 1);
     const float src_depth = pb.depths[cview(pb, v)->depth_off + (int64_t)iy * sc.width + ix];
     if (src_depth == 0.0f) {
-        return 5.0f;
+        return max_cost;
     }
     float bw[3];
     point_world(sx, sy, src_depth, sc, bw);
@@ -533,7 +534,7 @@ __device__ float geom_cost(const hpm_dev_problem& pb, int v, float4 plane, int p
     project(bw, rc, &bx, &by, &bd);
     const float dx = (float)px - bx;
     const float dy = (float)py - by;
-    return fminf(5.0f, sqrtf(dx * dx + dy * dy));
+    return fminf(max_cost, sqrtf(dx * dx + dy * dy));
 }
 
 __device__ __forceinline__ int find_min_idx8(const float* c)
@@ -648,7 +649,57 @@ __device__ __forceinline__ void count_evals(const hpm_dev_problem& pb, uint32_t 
     }
 }
 
-/* RandomInitialization (HPM.cu:510-558) */
+/* JBU_cu (HPM.cu:1984-2052) for one output pixel: SpatialGauss /
+ * RangeGauss (HPM.cu:1971-1981) in FP32, NormalizeVec3 with 1/sqrtf */
+__device__ __forceinline__ void jbu_pixel(const float* guide, int w, int h, const float* src_nd, int sw, int sh,
+                                          int scale_i, int px, int py, float* depth_out, float4* normal_out)
+{
+    const float scale = (float)(1.0 * sw / w);
+    const float sigmad = 0.50f, sigmar = 25.5f;
+    const int win = scale_i * scale_i + 1;
+    const int nn = win / 2;
+    const float o_y = (float)py * scale;
+    const float o_x = (float)px * scale;
+    const float ref_pix = texel(guide, w, h, px, py);
+    float nf = 0.0f, ctot = 0.0f;
+    float4 ntot = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int j = -nn; j <= nn; ++j) {
+        int r_y = (int)(o_y + (float)j);
+        r_y = (r_y > 0 ? (r_y < sh ? r_y : sh - 1) : 0);
+        const int r_ys = py + j;
+        for (int i = -nn; i <= nn; ++i) {
+            int r_x = (int)(o_x + (float)i);
+            r_x = (r_x > 0 ? (r_x < sw ? r_x : sw - 1) : 0);
+            const int s_center = r_y * sw + r_x;
+            float4 sn = ld4(src_nd, s_center);
+            const float src_pix = sn.w;
+            const float nb_pix = texel(guide, w, h, px + i, r_ys);
+            const float ddx = o_x - (float)r_x, ddy = o_y - (float)r_y;
+            const float dis = ddx * ddx + ddy * ddy;
+            const float sg = hpm_expf(-dis / (2 * sigmad * sigmad));
+            const float xp = fabsf(ref_pix - nb_pix);
+            const float rg = hpm_expf(-(xp * xp) / (2 * sigmar * sigmar));
+            const float tg = sg * rg;
+            nf += tg;
+            ctot += src_pix * tg;
+            sn.x = sn.x * tg;
+            sn.y = sn.y * tg;
+            sn.z = sn.z * tg;
+            ntot.x = ntot.x + sn.x;
+            ntot.y = ntot.y + sn.y;
+            ntot.z = ntot.z + sn.z;
+        }
+    }
+    ntot.x = ntot.x / nf;
+    ntot.y = ntot.y / nf;
+    ntot.z = ntot.z / nf;
+    normalize3(&ntot);
+    *depth_out = ctot / nf;
+    *normal_out = make_float4(ntot.x, ntot.y, ntot.z, 0.0f);
+}
+
+/* RandomInitialization (HPM.cu:510-558; ACMM.cu:521-621 adds the hierarchy
+ * branch) */
 __device__ uint32_t init_pixel(const hpm_dev_problem& pb, int px, int py)
 {
     const hpm_params& P = pb.params;
@@ -667,6 +718,23 @@ __device__ uint32_t init_pixel(const hpm_dev_problem& pb, int px, int py)
         plane.z = rc.R[6] * t.x + rc.R[7] * t.y + rc.R[8] * t.z;
         plane.w = t.w;
         const float depth = plane.w;
+        plane.w = dist_to_origin(rc, px, py, depth, plane);
+    } else if (P.hierarchy) {
+        /* the upsample branch (ACMM.cu:532-601): the depth is the JBU of
+         * the coarse depth (RunJBU, ACMM.cpp:804-853, written to and read
+         * back from depths.dmb), the normal the JBU of the coarse normals in
+         * the same window, normalised. The pre-cost is the cost of the
+         * plane (0, 0, 0, depth): the reference reads the normal of a fresh
+         * host array there (ACMM.cpp:711-717), taken as zeros. */
+        float depth;
+        float4 n;
+        jbu_pixel(pb.images + cview(pb, 0)->img_off, pb.W, pb.H, pb.coarse, pb.coarse_w, pb.coarse_h,
+                  pb.coarse_scale, px, py, &depth, &n);
+        uint32_t m0;
+        pb.pre_cost[center] = initial_cost(pb, rp, px, py, make_float4(0.0f, 0.0f, 0.0f, depth), &m0, &nev);
+        plane.x = rc.R[0] * n.x + rc.R[1] * n.y + rc.R[2] * n.z;
+        plane.y = rc.R[3] * n.x + rc.R[4] * n.y + rc.R[5] * n.z;
+        plane.z = rc.R[6] * n.x + rc.R[7] * n.y + rc.R[8] * n.z;
         plane.w = dist_to_origin(rc, px, py, depth, plane);
     } else if (P.planar_prior) {
         if (pb.prior_mask[center] > 0u && pb.cost[center] >= 0.1f) {
@@ -1137,6 +1205,9 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
      * (HPM.cu:1637-1650); the trips take the views in ascending order, so
      * the weighted sum accumulates in view order */
     plane_q(pb, cur, q);
+    /* weight of the geometric term in the re-score and the refinement:
+     * 0.1 (HPM.cu:1640, 1194), 0.2 in ACMM (ACMM.cu:1596, 651) */
+    const float gc_weight = P.acmm ? 0.2f : 0.1f;
     float rescore_sum = 0.0f;
     {
         uint32_t todo = sel_views;
@@ -1146,7 +1217,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 todo &= ~(1u << vi);
                 const float c = ncc(pb, rp, px, py, vi + 1, q);
                 ++nev;
-                const float term = P.geom_consistency ? c + 0.1f * geom_cost(pb, vi + 1, cur, px, py) : c;
+                const float term = P.geom_consistency ? c + gc_weight * geom_cost(pb, vi + 1, cur, px, py) : c;
                 rescore_sum += vw[vi] * term;
             }
         }
@@ -1209,7 +1280,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 cost_center = fc[min_idx];
                 sel_center = tsel;
             }
-            if (cost_center != cost_center) {
+            if (!P.acmm && cost_center != cost_center) {
                 depth_now = db;
                 plane_center = hp;
                 cost_now = fc[min_idx];
@@ -1307,7 +1378,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                     double tq[3];
                     plane_q(pb, plane, tq);
                     const float cv = ncc(pb, orp, opx, opy, vi + 1, tq);
-                    term = P.geom_consistency ? cv + 0.1f * geom_cost(pb, vi + 1, plane, opx, opy) : cv;
+                    term = P.geom_consistency ? cv + gc_weight * geom_cost(pb, vi + 1, plane, opx, opy) : cv;
                 }
 #pragma unroll
                 for (int cc = 0; cc < 5; ++cc) {
@@ -1352,9 +1423,16 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
         }
     }
     if (valid) {
-        st4(pb.hyp, center, plane_center);
-        pb.cost[center] = cost_center;
-        pb.sel[center] = sel_center;
+        if (P.hierarchy && !(cost_center < pb.pre_cost[center] - 0.1f)) {
+            /* ACMM hierarchy (ACMM.cu:1619-1624): the hypothesis stays, the
+             * cost is the re-scored one, the views those of the adoption */
+            pb.cost[center] = cost_now;
+            pb.sel[center] = sel_center;
+        } else {
+            st4(pb.hyp, center, plane_center);
+            pb.cost[center] = cost_center;
+            pb.sel[center] = sel_center;
+        }
     }
     return nev;
 }
@@ -1724,55 +1802,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAV
     for (int v = 1; v <= pb.nsrc; ++v) {
         out[(size_t)i * pb.nsrc + v - 1] = ncc(pb, rp, px, py, v, q);
     }
-}
-
-/* JBU_cu (HPM.cu:1984-2052) for one output pixel: SpatialGauss /
- * RangeGauss (HPM.cu:1971-1981) in FP32, NormalizeVec3 with 1/sqrtf */
-__device__ __forceinline__ void jbu_pixel(const float* guide, int w, int h, const float* src_nd, int sw, int sh,
-                                          int scale_i, int px, int py, float* depth_out, float4* normal_out)
-{
-    const float scale = (float)(1.0 * sw / w);
-    const float sigmad = 0.50f, sigmar = 25.5f;
-    const int win = scale_i * scale_i + 1;
-    const int nn = win / 2;
-    const float o_y = (float)py * scale;
-    const float o_x = (float)px * scale;
-    const float ref_pix = texel(guide, w, h, px, py);
-    float nf = 0.0f, ctot = 0.0f;
-    float4 ntot = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (int j = -nn; j <= nn; ++j) {
-        int r_y = (int)(o_y + (float)j);
-        r_y = (r_y > 0 ? (r_y < sh ? r_y : sh - 1) : 0);
-        const int r_ys = py + j;
-        for (int i = -nn; i <= nn; ++i) {
-            int r_x = (int)(o_x + (float)i);
-            r_x = (r_x > 0 ? (r_x < sw ? r_x : sw - 1) : 0);
-            const int s_center = r_y * sw + r_x;
-            float4 sn = ld4(src_nd, s_center);
-            const float src_pix = sn.w;
-            const float nb_pix = texel(guide, w, h, px + i, r_ys);
-            const float ddx = o_x - (float)r_x, ddy = o_y - (float)r_y;
-            const float dis = ddx * ddx + ddy * ddy;
-            const float sg = hpm_expf(-dis / (2 * sigmad * sigmad));
-            const float xp = fabsf(ref_pix - nb_pix);
-            const float rg = hpm_expf(-(xp * xp) / (2 * sigmar * sigmar));
-            const float tg = sg * rg;
-            nf += tg;
-            ctot += src_pix * tg;
-            sn.x = sn.x * tg;
-            sn.y = sn.y * tg;
-            sn.z = sn.z * tg;
-            ntot.x = ntot.x + sn.x;
-            ntot.y = ntot.y + sn.y;
-            ntot.z = ntot.z + sn.z;
-        }
-    }
-    ntot.x = ntot.x / nf;
-    ntot.y = ntot.y / nf;
-    ntot.z = ntot.z / nf;
-    normalize3(&ntot);
-    *depth_out = ctot / nf;
-    *normal_out = make_float4(ntot.x, ntot.y, ntot.z, 0.0f);
 }
 
 __global__ __launch_bounds__(256) void hpm_k_jbu(const float* guide, int w, int h, const float* src_nd, int sw,

@@ -11,6 +11,11 @@
  * learned prior, which then replaces the planar prior passes where its judge
  * keeps enough pixels), --no-fusion (skip RunFusion, main.cpp:675, which
  * otherwise writes HPM/HPM_model.ply).
+ *
+ * --acmm: the ACMM+NESP executable instead (ACMM+NESP/windows/main.cpp:
+ * 302-378, hpm_acmm_run_folder): coarse-to-fine levels with the ACMM kernel
+ * variant, results in ACMM_NESP/, fusion to ACMM_NESP/ACMM_NESP_model.ply;
+ * --geom-iterations N sets its geometric passes per level (2).
  */
 #include <math.h>
 #include <stdio.h>
@@ -26,6 +31,7 @@
 #include "hpm_io.h"
 #include "hpm_fusion.h"
 #include "hpm_pipeline.h"
+#include "hpm_acmm.h"
 
 using namespace hpm;
 
@@ -261,13 +267,14 @@ int main(int argc, char** argv)
 {
     if (argc < 2) {
         printf("USAGE: hpm_mvs dense_folder [--class-api] [--geom-iterations N] [--knn-k K] [--no-prior] "
-               "[--no-geom] [--learned-prior FILE] [--no-fusion]\n");
+               "[--no-geom] [--learned-prior FILE] [--no-fusion] [--acmm]\n");
         return -1;
     }
     const std::string dense_folder = argv[1];
     Options opt;
     const char* learned = nullptr;
     bool fusion = true;
+    bool acmm = false;
     /* RunFusion (main.cpp:675) on the maps just written */
     auto run_fusion = [&]() -> int {
         if (!fusion) {
@@ -297,10 +304,33 @@ int main(int argc, char** argv)
             learned = argv[++i];
         } else if (!strcmp(argv[i], "--no-fusion")) {
             fusion = false;
+        } else if (!strcmp(argv[i], "--acmm")) {
+            acmm = true;
         } else {
             fprintf(stderr, "unknown option %s\n", argv[i]);
             return -1;
         }
+    }
+    if (acmm) {
+        if (opt.class_api || learned) {
+            fprintf(stderr, "hpm_mvs: --acmm runs the ACMM+NESP schedule alone\n");
+            return -1;
+        }
+        hpm_acmm_options ao;
+        hpm_acmm_default(&ao);
+        ao.geom_iterations = opt.geom ? opt.geom_iterations : 0;
+        hpm_pipeline_stats st;
+        int64_t n = 0;
+        char err[512] = {0};
+        if (hpm_acmm_run_folder(dense_folder.c_str(), &ao, fusion ? 1 : 0, &n, &st, err, sizeof(err)) != 0) {
+            fprintf(stderr, "hpm_mvs: %s\n", err);
+            return 1;
+        }
+        printf("ACMM+NESP: %d RunPatchMatch calls, %.1f ms device\n", st.runs, st.ms_device);
+        if (fusion) {
+            printf("fusion: %lld points -> %s/ACMM_NESP/ACMM_NESP_model.ply\n", (long long)n, dense_folder.c_str());
+        }
+        return 0;
     }
     mkdir((dense_folder + "/HPM").c_str(), 0755);
     if (!opt.class_api) {

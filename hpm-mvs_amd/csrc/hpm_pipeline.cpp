@@ -177,7 +177,20 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
     if (rc < 0) return rc;
     rc = check(c, h, hpm_device_init(h, nullptr), "device init");
     if (rc < 0) return rc;
-    rc = check(c, h, hpm_run_internal(h, true), "RunPatchMatch");
+    h->call_index = (uint32_t)c.opt->stream_base;
+    if (const hpm_pool* cp = c.opt->hierarchy_from) {
+        /* ACMM: the view's final maps at the coarser scale (normals.dmb,
+         * depths_geom.dmb there; ACMM.cpp:660-717) */
+        if (ref >= cp->n || !cp->have_state[(size_t)ref]) {
+            return fail(c, "view %d: no state in the coarser pool", ref);
+        }
+        rc = check(c, h,
+                   hpm_set_hierarchy_internal(h, cp->d_state_hyp + 4 * cp->off[(size_t)ref], cp->cams[(size_t)ref].width,
+                                              cp->cams[(size_t)ref].height, true),
+                   "hierarchy input");
+        if (rc < 0) return rc;
+    }
+    rc = check(c, h, hpm_run_internal(h, c.opt->planar_prior != 0), "RunPatchMatch");
     if (rc < 0) return rc;
     account(c, h);
     const hpm_camera ref_cam = h->cams[0];
@@ -338,7 +351,7 @@ int process_phase_b(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc, i
     if (rc < 0) return rc;
     rc = check(c, h, hpm_load_from_pool(h), "reload");
     if (rc < 0) return rc;
-    h->call_index = 8u + (uint32_t)pass;
+    h->call_index = (uint32_t)c.opt->stream_base + 8u + (uint32_t)pass;
     rc = check(c, h, hpm_run_internal(h, false), "RunPatchMatch (geometric)");
     if (rc < 0) return rc;
     account(c, h);
@@ -358,6 +371,8 @@ extern "C" void hpm_pipeline_default(hpm_pipeline_options* opt)
     opt->workers = 12;
     opt->learned_prior = nullptr;
     opt->learned_min_accept = 0.3f;
+    opt->hierarchy_from = nullptr;
+    opt->stream_base = 0;
 }
 
 extern "C" int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, float* cost)

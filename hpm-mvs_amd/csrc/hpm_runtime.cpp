@@ -124,6 +124,15 @@ int check_params(hpm_handle* h, const hpm_params& p)
     if (p.top_k < 1 || p.top_k > HPM_MAX_SOURCES) {
         return hpm_fail(h, HPM_EINVAL, "top_k out of range: %d", p.top_k);
     }
+    if (p.acmm && (p.max_iterations < 1 || p.max_iterations > 3)) {
+        return hpm_fail(h, HPM_EINVAL, "the ACMM variant runs 1..3 iterations (max_iterations=%d)", p.max_iterations);
+    }
+    if (p.acmm && p.planar_prior) {
+        return hpm_fail(h, HPM_EINVAL, "the ACMM variant has no planar prior");
+    }
+    if (p.hierarchy && (!p.acmm || p.geom_consistency)) {
+        return hpm_fail(h, HPM_EINVAL, "hierarchy needs the ACMM variant and no geometric consistency");
+    }
     return HPM_OK;
 }
 
@@ -138,6 +147,10 @@ void release_problem(hpm_handle* h)
     dfree(h->d_neval);
     dfree(h->d_jbu_src);
     h->jbu_src_cap = 0;
+    dfree(h->d_coarse);
+    dfree(h->d_pre_cost);
+    h->coarse_cap = 0;
+    h->have_coarse = false;
     dfree(h->d_ws);
     h->ws_cap = 0;
     h->alloc_pixels = 0;
@@ -191,6 +204,13 @@ void fill_problem(const hpm_handle* h, hpm_dev_problem* pb)
     pb->prior_planes = h->d_prior;
     pb->prior_mask = h->d_mask;
     pb->n_eval = h->d_neval;
+    if (h->have_coarse) {
+        pb->coarse = h->d_coarse;
+        pb->coarse_w = h->coarse_w;
+        pb->coarse_h = h->coarse_h;
+        pb->coarse_scale = h->coarse_scale;
+        pb->pre_cost = h->d_pre_cost;
+    }
 }
 
 int readback(hpm_handle* h)
@@ -211,6 +231,9 @@ int ensure_ready(hpm_handle* h)
     }
     if (h->params.planar_prior && !h->have_prior) {
         return hpm_fail(h, HPM_EINVAL, "planar prior enabled but no prior uploaded");
+    }
+    if (h->params.hierarchy && !h->have_coarse) {
+        return hpm_fail(h, HPM_EINVAL, "hierarchy enabled but no coarser map given (hpm_set_hierarchy_input)");
     }
     return check_params(h, h->params);
 }
@@ -481,7 +504,7 @@ extern "C" int hpm_set_geom_consistency(hpm_handle* h, int multi_geometry)
         return HPM_EINVAL;
     }
     h->params.geom_consistency = 1;
-    h->params.max_iterations = 2; /* stored only: RunPatchMatch runs 3 (HPM.cu:1933) */
+    h->params.max_iterations = 2; /* HPM-MVS stores it only (RunPatchMatch runs 3, HPM.cu:1933); ACMM runs it */
     if (multi_geometry) {
         h->params.multi_geometry = 1;
     }
@@ -495,6 +518,56 @@ extern "C" int hpm_set_planar_prior(hpm_handle* h)
     }
     h->params.planar_prior = 1;
     return HPM_OK;
+}
+
+int hpm_set_hierarchy_internal(hpm_handle* h, const float* coarse_world4, int coarse_w, int coarse_h,
+                               bool on_device)
+{
+    if (!h || !coarse_world4 || coarse_w < 1 || coarse_h < 1) {
+        return hpm_fail(h, HPM_EINVAL, "bad hierarchy input");
+    }
+    if (!h->have_device) {
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
+    }
+    if (!h->params.acmm) {
+        return hpm_fail(h, HPM_EINVAL, "hierarchy needs the ACMM variant (params.acmm)");
+    }
+    const int W = h->cams[0].width, H = h->cams[0].height;
+    /* RunJBU (ACMM.cpp:804-853): window scale max(rows / s_rows, cols /
+     * s_cols); at 1 the reference writes no upsampled map */
+    const int scale = std::max(H / coarse_h, W / coarse_w);
+    if (scale < 2) {
+        return hpm_fail(h, HPM_EINVAL, "hierarchy input %dx%d is not coarser than %dx%d by 2 or more", coarse_w,
+                        coarse_h, W, H);
+    }
+    HPM_HIP(h, hipSetDevice(h->device));
+    const size_t nc = (size_t)coarse_w * coarse_h;
+    const size_t np = (size_t)W * H;
+    if (h->coarse_cap < nc || !h->d_pre_cost) {
+        dfree(h->d_coarse);
+        dfree(h->d_pre_cost);
+        h->coarse_cap = 0;
+        HPM_HIP(h, hipMalloc((void**)&h->d_coarse, nc * 16));
+        HPM_HIP(h, hipMalloc((void**)&h->d_pre_cost, h->alloc_pixels * 4));
+        h->coarse_cap = nc;
+    }
+    HPM_HIP(h, hipMemcpyAsync(h->d_coarse, coarse_world4, nc * 16,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream));
+    HPM_HIP(h, hipMemsetAsync(h->d_pre_cost, 0, np * 4, h->stream));
+    HPM_HIP(h, hipStreamSynchronize(h->stream));
+    h->coarse_w = coarse_w;
+    h->coarse_h = coarse_h;
+    /* the in-kernel window scale: int(max(width / scaled_cols, height /
+     * scaled_rows)) in float (ACMM.cu:536) */
+    h->coarse_scale = (int)std::max((float)W / (float)coarse_w, (float)H / (float)coarse_h);
+    h->have_coarse = true;
+    h->params.hierarchy = 1;
+    return HPM_OK;
+}
+
+extern "C" int hpm_set_hierarchy_input(hpm_handle* h, const float* coarse_world4, int coarse_w, int coarse_h)
+{
+    return hpm_set_hierarchy_internal(h, coarse_world4, coarse_w, coarse_h, false);
 }
 
 extern "C" int hpm_input_init_pool(hpm_handle* h, hpm_pool* pool, int ref_view, const int* src_views, int num_src)
@@ -663,11 +736,13 @@ extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
     h->h_cost.assign(np, 0.0f);
     h->have_device = true;
     h->have_prior = false;
+    h->have_coarse = false;
     h->call_index = 0;
     if (h->params.geom_consistency && dense_folder) {
         /* reload the previous hypotheses (HPM.cpp:662-699) */
         char base[4096];
-        snprintf(base, sizeof(base), "%s/HPM/2333_%08d", dense_folder, h->ref_id);
+        snprintf(base, sizeof(base), "%s/%s/2333_%08d", dense_folder, h->params.acmm ? "ACMM_NESP" : "HPM",
+                 h->ref_id);
         const char* suffix = h->params.multi_geometry ? "depths_geom.dmb" : "depths.dmb";
         char path[4200];
         int w = 0, hh = 0, nb = 0;
@@ -728,7 +803,9 @@ int hpm_run_internal(hpm_handle* h, bool want_readback)
     HPM_HIP(h, (hipError_t)hpm_launch_init(&pb, h->stream));
     HPM_HIP(h, hipEventRecord(h->ev[1], h->stream));
     pb.n_eval = h->d_neval + 1; /* propagation evaluations counted apart */
-    for (int it = 0; it < 3; ++it) { /* hard-coded 3 (HPM.cu:1933) */
+    /* hard-coded 3 (HPM.cu:1933); ACMM runs params.max_iterations (ACMM.cu:1836) */
+    const int iterations = h->params.acmm ? h->params.max_iterations : 3;
+    for (int it = 0; it < iterations; ++it) {
         HPM_HIP(h, (hipError_t)hpm_launch_propagate(&pb, it, 0, h->stream));
         HPM_HIP(h, (hipError_t)hpm_launch_propagate(&pb, it, 1, h->stream));
     }
@@ -749,7 +826,7 @@ int hpm_run_internal(hpm_handle* h, bool want_readback)
     }
     h->stats.n_eval = nev[0] + nev[1];
     h->stats.n_eval_propagate = nev[1];
-    h->stats.launches = 10;
+    h->stats.launches = 4 + 2 * iterations;
     HPM_HIP(h, hipEventElapsedTime(&h->stats.ms_total, h->ev[0], h->ev[3]));
     HPM_HIP(h, hipEventElapsedTime(&h->stats.ms_init, h->ev[0], h->ev[1]));
     HPM_HIP(h, hipEventElapsedTime(&h->stats.ms_propagate, h->ev[1], h->ev[2]));

@@ -58,6 +58,8 @@ class HpmParams(C.Structure):
         ("geom_iterations", C.c_int32),
         ("seed_lo", C.c_uint32),
         ("seed_hi", C.c_uint32),
+        ("acmm", C.c_int32),
+        ("hierarchy", C.c_int32),
     ]
 
 
@@ -138,6 +140,20 @@ class HpmPipelineOptions(C.Structure):
         ("workers", C.c_int32),
         ("learned_prior", C.c_char_p),
         ("learned_min_accept", C.c_float),
+        ("hierarchy_from", C.c_void_p),
+        ("stream_base", C.c_int32),
+    ]
+
+
+class HpmAcmmOptions(C.Structure):
+    """hpm_acmm_options (include/hpm_acmm.h)."""
+
+    _fields_ = [
+        ("params", HpmParams),
+        ("scale_bound", C.c_int32),
+        ("geom_iterations", C.c_int32),
+        ("workers", C.c_int32),
+        ("verbose", C.c_int32),
     ]
 
 
@@ -189,6 +205,7 @@ SIGNATURES = {
     "hpm_get_params": (C.c_int, [C.c_void_p, C.POINTER(HpmParams)]),
     "hpm_set_geom_consistency": (C.c_int, [C.c_void_p, C.c_int]),
     "hpm_set_planar_prior": (C.c_int, [C.c_void_p]),
+    "hpm_set_hierarchy_input": (C.c_int, [C.c_void_p, _FP, C.c_int, C.c_int]),
     "hpm_input_init": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _IP, C.c_int]),
     "hpm_input_init_memory": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(HpmCamera),
                                         C.POINTER(_FP), C.POINTER(_FP)]),
@@ -215,6 +232,8 @@ SIGNATURES = {
     "hpm_debug_read_prior": (C.c_int, [C.c_void_p, _FP, _UP]),
     "hpm_fusion_run": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, C.c_int, C.c_int, C.POINTER(C.c_void_p),
                                  C.POINTER(C.c_int64), C.c_char_p, C.c_int]),
+    "hpm_fusion_run_acmm": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, C.c_int, C.c_int, C.POINTER(C.c_void_p),
+                                      C.POINTER(C.c_int64), C.c_char_p, C.c_int]),
     "hpm_fusion_free": (None, [C.c_void_p]),
     "hpm_fusion_run_folder": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.c_char_p, C.c_int]),
     "hpm_pool_set_state": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP]),
@@ -281,6 +300,13 @@ SIGNATURES = {
     "hpm_batch_log": (C.c_char_p, [C.c_void_p]),
     "hpm_pipeline_run_folder": (C.c_int, [C.c_char_p, C.POINTER(HpmPipelineOptions), C.POINTER(HpmPipelineStats),
                                           C.c_char_p, C.c_int]),
+    # hpm_acmm.h
+    "hpm_acmm_default": (None, [C.c_void_p]),
+    "hpm_acmm_run": (C.c_int, [C.c_int, C.c_int, C.POINTER(HpmCamera), C.POINTER(_FP), C.c_int, _IP, _IP, _IP,
+                               C.c_int, C.POINTER(HpmAcmmOptions), C.POINTER(C.c_void_p),
+                               C.POINTER(HpmPipelineStats), C.c_char_p, C.c_int]),
+    "hpm_acmm_run_folder": (C.c_int, [C.c_char_p, C.POINTER(HpmAcmmOptions), C.c_int, C.POINTER(C.c_int64),
+                                      C.POINTER(HpmPipelineStats), C.c_char_p, C.c_int]),
 }
 
 
@@ -677,7 +703,11 @@ class Pool:
     """All views of a scene resident on the device."""
 
     def __init__(self, cams: Sequence[HpmCamera], images: Sequence[np.ndarray], device: int = 0,
-                 max_image_size: int = 3200):
+                 max_image_size: int = 3200, _handle=None):
+        if _handle is not None:  # a pool the library made (hpm_acmm_run)
+            self._p = _handle
+            self.n = len(cams)
+            return
         n = len(cams)
         self._imgs = [np.ascontiguousarray(im, dtype=np.float32) for im in images]
         ptrs = (_FP * n)(*[fptr(im) for im in self._imgs])
@@ -771,9 +801,10 @@ def _problem_arrays(problems):
     return ns, srcs
 
 
-def fusion(pool: Pool, problems, geom_consistency: bool = True) -> np.ndarray:
-    """RunFusion over the pool's maps; problems[i] = (i, [source views]).
-    Returns the points as a POINT_DTYPE array in the reference's order."""
+def fusion(pool: Pool, problems, geom_consistency: bool = True, acmm: bool = False) -> np.ndarray:
+    """RunFusion over the pool's maps (acmm: ACMM+NESP's rule,
+    hpm_fusion_run_acmm); problems[i] = (i, [source views]). Returns the
+    points as a POINT_DTYPE array in the reference's order."""
     for i, (r, _) in enumerate(problems):
         if r != i:
             raise HpmError("fusion: problem i must have reference view i")
@@ -781,8 +812,9 @@ def fusion(pool: Pool, problems, geom_consistency: bool = True) -> np.ndarray:
     pts = C.c_void_p()
     npts = C.c_int64()
     err = C.create_string_buffer(512)
-    rc = lib().hpm_fusion_run(pool._p, len(problems), iptr(ns), iptr(srcs), HPM_MAX_SOURCES, int(geom_consistency),
-                              C.byref(pts), C.byref(npts), err, 512)
+    fn = lib().hpm_fusion_run_acmm if acmm else lib().hpm_fusion_run
+    rc = fn(pool._p, len(problems), iptr(ns), iptr(srcs), HPM_MAX_SOURCES, int(geom_consistency), C.byref(pts),
+            C.byref(npts), err, 512)
     if rc != 0:
         raise HpmError(f"hpm_fusion_run: {err.value.decode()}")
     try:
@@ -799,6 +831,49 @@ def write_ply(path: str, points: np.ndarray):
     p = np.ascontiguousarray(points, dtype=POINT_DTYPE)
     if lib().hpm_write_ply(path.encode(), C.c_void_p(p.ctypes.data), len(p)) != 0:
         raise HpmError(f"hpm_write_ply failed for {path}")
+
+
+def acmm_options(**kw) -> HpmAcmmOptions:
+    """hpm_acmm_default, then the given fields (params fields by name)."""
+    o = HpmAcmmOptions()
+    lib().hpm_acmm_default(C.byref(o))
+    for k, v in kw.items():
+        if hasattr(o.params, k) and not hasattr(o, k):
+            setattr(o.params, k, v)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def run_acmm(cams, images, problems, opt: Optional[HpmAcmmOptions] = None, device: int = 0):
+    """The ACMM+NESP coarse-to-fine schedule (include/hpm_acmm.h) on full-size
+    views; problems: list of (ref_view, [source views]). Returns (Pool of the
+    last level, HpmPipelineStats)."""
+    n = len(cams)
+    imgs = [np.ascontiguousarray(im, dtype=np.float32) for im in images]
+    ptrs = (_FP * n)(*[fptr(im) for im in imgs])
+    refs = np.array([p[0] for p in problems], np.int32)
+    ns, srcs = _problem_arrays(problems)
+    st = HpmPipelineStats()
+    err = C.create_string_buffer(512)
+    out = C.c_void_p()
+    o = opt if opt is not None else acmm_options()
+    rc = lib().hpm_acmm_run(device, n, (HpmCamera * n)(*cams), ptrs, len(problems), iptr(refs), iptr(ns),
+                            iptr(srcs), HPM_MAX_SOURCES, C.byref(o), C.byref(out), C.byref(st), err, 512)
+    if rc != 0:
+        raise HpmError(f"hpm_acmm_run: {err.value.decode()}")
+    return Pool(cams, [], _handle=out.value), st
+
+
+def run_acmm_folder(folder: str, opt: Optional[HpmAcmmOptions] = None, fuse: bool = True):
+    """hpm_acmm_run_folder: returns (HpmPipelineStats, fused point count)."""
+    st = HpmPipelineStats()
+    err = C.create_string_buffer(512)
+    npts = C.c_int64(0)
+    o = opt if opt is not None else acmm_options()
+    if lib().hpm_acmm_run_folder(folder.encode(), C.byref(o), int(fuse), C.byref(npts), C.byref(st), err, 512) != 0:
+        raise HpmError(f"hpm_acmm_run_folder: {err.value.decode()}")
+    return st, npts.value
 
 
 def run_pipeline_folder(folder: str, opt: Optional[HpmPipelineOptions] = None) -> HpmPipelineStats:

@@ -81,6 +81,17 @@ int hpm_get_params(const hpm_handle* h, hpm_params* p);
 int hpm_set_geom_consistency(hpm_handle* h, int multi_geometry);
 /* HPM::SetPlanarPriorParams (HPM.cpp:477-480) */
 int hpm_set_planar_prior(hpm_handle* h);
+/* ACMM::SetHierarchyParams (ACMM+NESP/windows/ACMM.cpp:443-446) with the
+ * hierarchy branch of ACMM::CudaSpaceInitialization (ACMM.cpp:660-717), after
+ * hpm_device_init, on a handle with params.acmm set: coarse_world4 holds
+ * (world normal xyz, depth) per pixel of the previous, coarser scale
+ * (normals.dmb and depths_geom.dmb there). RunPatchMatch then starts from
+ * their joint bilateral upsampling guided by the reference image (RunJBU,
+ * ACMM.cpp:804-853, and the upsample branch of RandomInitialization,
+ * ACMM.cu:532-601) and keeps a pixel's new hypothesis only where its cost
+ * beats the pre-cost by 0.1 (ACMM.cu:1619-1628). The coarse map must be
+ * smaller by an integer factor of 2 or more. */
+int hpm_set_hierarchy_input(hpm_handle* h, const float* coarse_world4, int coarse_w, int coarse_h);
 
 /* HPM::InuputInitialization (HPM.cpp:482-589): reads images/%08d.{pgm,jpg}
  * as grey, cams/%08d_cam.txt, downscales images above max_image_size, sets

@@ -38,6 +38,14 @@ typedef struct hpm_point {
  * Returns 0, or a negative code with a message in err. */
 int hpm_fusion_run(hpm_pool* pool, int num_views, const int* num_src, const int* src_ids, int max_src,
                    int geom_consistency, hpm_point** points, int64_t* num_points, char* err, int err_len);
+/* RunFusion of ACMM+NESP (ACMM+NESP/windows/main.cpp:164-300) on the same
+ * inputs: a point needs 2 consistent sources, and its position, normal and
+ * colour are the means of the reference pixel's and its consistent
+ * sources' (the normal as cv::Vec3f /= float computes it, a product with
+ * the float reciprocal; the colour truncated as the reference's PLY writer
+ * does, ACMM.cpp:396-398). */
+int hpm_fusion_run_acmm(hpm_pool* pool, int num_views, const int* num_src, const int* src_ids, int max_src,
+                        int geom_consistency, hpm_point** points, int64_t* num_points, char* err, int err_len);
 void hpm_fusion_free(hpm_point* points);
 
 /* RunFusion as the reference's main calls it (main.cpp:675): reads

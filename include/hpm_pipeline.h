@@ -44,6 +44,11 @@ typedef struct hpm_pipeline_options {
      * learned_min_accept of the pixels keep the network's proposal */
     const char* learned_prior;  /* NULL: planar prior */
     float learned_min_accept;   /* 0.3 */
+    /* ACMM hierarchy (params.acmm, no planar prior): when set, phase A runs
+     * each view once from its final state in this coarser pool
+     * (hpm_set_hierarchy_input) instead of a plain pass */
+    hpm_pool* hierarchy_from;
+    int32_t stream_base;        /* added to every RunPatchMatch call's Philox stream (0) */
 } hpm_pipeline_options;
 
 typedef struct hpm_pipeline_stats {

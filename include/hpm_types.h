@@ -57,6 +57,18 @@ typedef struct hpm_params {
     int32_t geom_iterations;  /* 2 (main.cpp:656) */
     uint32_t seed_lo;         /* Philox key; the reference seeds from clock64() (HPM.cu:521) */
     uint32_t seed_hi;
+    /* The ACMM+NESP variant (ACMM+NESP/windows/ACMM.cu, PatchMatchParams at
+     * ACMM.h:21-45); both 0 for HPM-MVS. acmm: the geometric cost is capped at
+     * 3 (ACMM.cu:496) and weighted 0.2 in the re-score and the refinement
+     * (ACMM.cu:1596, 651), adoption has no NaN fallback (ACMM.cu:1607-1616),
+     * and RunPatchMatch runs max_iterations (1..3) red/black iterations
+     * (ACMM.cu:1836), 2 in geometric passes (ACMM.cpp:434-441). hierarchy
+     * (SetHierarchyParams, ACMM.cpp:443-446): the initialisation upsamples a
+     * coarser map (hpm_set_hierarchy_input) and a pixel keeps its new
+     * hypothesis only when its cost beats the pre-cost by 0.1
+     * (ACMM.cu:1619-1628). */
+    int32_t acmm;
+    int32_t hierarchy;
 } hpm_params;
 
 /* Fills p with the reference defaults. */
@@ -82,6 +94,8 @@ static inline void hpm_params_default(hpm_params* p)
     p->geom_iterations = 2;
     p->seed_lo = 0x5eed0001u;
     p->seed_hi = 0x00c0ffeeu;
+    p->acmm = 0;
+    p->hierarchy = 0;
 }
 
 #ifdef __cplusplus

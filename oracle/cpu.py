@@ -55,6 +55,10 @@ class OracleProblem(C.Structure):
         ("prior_mask", _UP),
         ("call_index", C.c_uint32),
         ("salt", C.c_uint32),
+        ("coarse", _FP),
+        ("coarse_w", C.c_int32),
+        ("coarse_h", C.c_int32),
+        ("pre_cost", _FP),
     ]
 
 
@@ -139,6 +143,17 @@ class Oracle:
         self.pb.prior_planes = _f(pl)
         self.pb.prior_mask = mk.ctypes.data_as(_UP)
 
+    def set_hierarchy(self, coarse_world4: np.ndarray):
+        """ACMM hierarchy: start from the coarser map (world normal xyz,
+        depth per coarse pixel); pre_cost receives the pre-costs."""
+        c = np.ascontiguousarray(coarse_world4, dtype=np.float32)
+        self.pre_cost = np.zeros((self.H, self.W), np.float32)
+        self._keep += [c, self.pre_cost]
+        self.pb.coarse = _f(c)
+        self.pb.coarse_h, self.pb.coarse_w = c.shape[:2]
+        self.pb.pre_cost = _f(self.pre_cost)
+        self.pb.params.hierarchy = 1
+
     def _state(self):
         return (_f(self.hyp), _f(self.cost), self.sel.ctypes.data_as(_UP))
 
@@ -177,10 +192,11 @@ class Oracle:
         return out
 
 
-def fusion(cams, depths, normals, colors, problems, point_dtype):
-    """RunFusion (main.cpp:502-633), sequential: depths[i] (H, W), normals[i]
-    (H, W, 3) world, colors[i] (H, W, 3) uint8 of view i; problems[i] =
-    (i, [sources]). Returns the kept points as point_dtype."""
+def fusion(cams, depths, normals, colors, problems, point_dtype, acmm=False):
+    """RunFusion (main.cpp:502-633; with acmm, ACMM+NESP's, main.cpp:164-300),
+    sequential: depths[i] (H, W), normals[i] (H, W, 3) world, colors[i]
+    (H, W, 3) uint8 of view i; problems[i] = (i, [sources]). Returns the
+    kept points as point_dtype."""
     L = lib()
     n = len(cams)
     max_src = 32
@@ -197,10 +213,11 @@ def fusion(cams, depths, normals, colors, problems, point_dtype):
     cam_arr = (type(cams[0]) * n)(*cams)
     cap = sum(int(d.size) for d in depths)
     out = np.empty(cap, point_dtype)
-    L.hpm_oracle_fusion.restype = C.c_int64
-    L.hpm_oracle_fusion.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _IP, _IP, C.c_int,
-                                    C.c_void_p, C.c_int64]
-    k = L.hpm_oracle_fusion(n, C.cast(cam_arr, C.c_void_p), C.cast(dp, C.c_void_p), C.cast(np_, C.c_void_p),
+    fn = L.hpm_oracle_fusion_acmm if acmm else L.hpm_oracle_fusion
+    fn.restype = C.c_int64
+    fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _IP, _IP, C.c_int, C.c_void_p,
+                   C.c_int64]
+    k = fn(n, C.cast(cam_arr, C.c_void_p), C.cast(dp, C.c_void_p), C.cast(np_, C.c_void_p),
                             C.cast(cp, C.c_void_p), ns.ctypes.data_as(_IP), srcs.ctypes.data_as(_IP), max_src,
                             C.c_void_p(out.ctypes.data), cap)
     return out[:k]

@@ -589,10 +589,11 @@ static inline void project(const float X[3], const hpm_camera& c, float* u, floa
     *v = (c.K[3] * t[0] + c.K[4] * t[1] + c.K[5] * t[2]) / *d;
 }
 
-/* ComputeGeomConsistencyCost (HPM.cu:483-508). */
+/* ComputeGeomConsistencyCost (HPM.cu:483-508; ACMM.cu:494-519 caps it at 3). */
 static float geom_cost(const Ctx& cx, int v, F4 plane, int px, int py)
 {
     const hpm_oracle_problem* pb = cx.pb;
+    const float max_cost = pb->params.acmm ? 3.0f : 5.0f;
     const hpm_camera& rc = pb->cams[0];
     const hpm_camera& sc = pb->cams[v];
     const float depth = depth_from_plane(rc, plane, px, py);
@@ -606,7 +607,7 @@ static float geom_cost(const Ctx& cx, int v, F4 plane, int px, int py)
     const int iy = clampi((int)cyf, 0, sc.height - 1);
     const float src_depth = pb->depths[v][(size_t)iy * sc.width + ix];
     if (src_depth == 0.0f) {
-        return 5.0f;
+        return max_cost;
     }
     float bw[3];
     point_world(sx, sy, src_depth, sc, bw);
@@ -614,7 +615,7 @@ static float geom_cost(const Ctx& cx, int v, F4 plane, int px, int py)
     project(bw, rc, &bx, &by, &bd);
     const float dx = (float)px - bx;
     const float dy = (float)py - by;
-    return fminf(5.0f, sqrtf(dx * dx + dy * dy));
+    return fminf(max_cost, sqrtf(dx * dx + dy * dy));
 }
 
 /* sort_small (HPM.cu:3-12) */
@@ -731,6 +732,59 @@ static void make_ctx(const hpm_oracle_problem* pb, Ctx* cx)
     cx->ic = make_iter_consts();
 }
 
+/* One output pixel of JBU_cu (HPM.cu:1984-2052; the same window and
+ * weights in RunJBU and the upsample branch of ACMM's RandomInitialization,
+ * ACMM.cu:532-601 and 1862-1919): the weighted depth (src_nd.w) and the
+ * normalised weighted normal (src_nd.xyz), SpatialGauss / RangeGauss in
+ * FP32, NormalizeVec3's rsqrtf as 1/sqrtf. */
+static void jbu_pixel(const float* guide, int w, int h, const float* src_nd, int sw, int sh, int scale_i, int px,
+                      int py, float* depth_out, F4* normal_out)
+{
+    const float scale = (float)(1.0 * sw / w);
+    const float sigmad = 0.50f, sigmar = 25.5f;
+    const int win = scale_i * scale_i + 1;
+    const int nn = win / 2;
+    const float o_y = (float)py * scale;
+    const float o_x = (float)px * scale;
+    const float ref_pix = texel(guide, w, h, px, py);
+    float nf = 0.0f, ctot = 0.0f;
+    F4 ntot = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int j = -nn; j <= nn; ++j) {
+        int r_y = (int)(o_y + (float)j);
+        r_y = (r_y > 0 ? (r_y < sh ? r_y : sh - 1) : 0);
+        const int r_ys = py + j;
+        for (int i = -nn; i <= nn; ++i) {
+            int r_x = (int)(o_x + (float)i);
+            r_x = (r_x > 0 ? (r_x < sw ? r_x : sw - 1) : 0);
+            const int s_center = r_y * sw + r_x;
+            const float src_pix = src_nd[4 * s_center + 3];
+            F4 sn = ld4(src_nd, s_center);
+            const int r_xs = px + i;
+            const float nb_pix = texel(guide, w, h, r_xs, r_ys);
+            const float ddx = o_x - (float)r_x, ddy = o_y - (float)r_y;
+            const float dis = ddx * ddx + ddy * ddy;
+            const float sg = hpm_expf(-dis / (2 * sigmad * sigmad));
+            const float xp = fabsf(ref_pix - nb_pix);
+            const float rg = hpm_expf(-(xp * xp) / (2 * sigmar * sigmar));
+            const float tg = sg * rg;
+            nf += tg;
+            ctot += src_pix * tg;
+            sn.x = sn.x * tg;
+            sn.y = sn.y * tg;
+            sn.z = sn.z * tg;
+            ntot.x = ntot.x + sn.x;
+            ntot.y = ntot.y + sn.y;
+            ntot.z = ntot.z + sn.z;
+        }
+    }
+    ntot.x = ntot.x / nf;
+    ntot.y = ntot.y / nf;
+    ntot.z = ntot.z / nf;
+    normalize3(&ntot);
+    *depth_out = ctot / nf;
+    *normal_out = ntot;
+}
+
 /* RandomInitialization (HPM.cu:510-558). */
 static void init_pixel(const Ctx& cx, int px, int py, float* hyp, float* cost, uint32_t* sel, uint64_t* nev)
 {
@@ -750,6 +804,27 @@ static void init_pixel(const Ctx& cx, int px, int py, float* hyp, float* cost, u
         c.z = rc.R[6] * t.x + rc.R[7] * t.y + rc.R[8] * t.z;
         c.w = t.w;
         const float depth = c.w;
+        c.w = dist_to_origin(rc, px, py, depth, c);
+        st4(hyp, center, c);
+        cost[center] = initial_cost(cx, rp, px, py, c, &sel[center], nev);
+    } else if (P.hierarchy) {
+        /* ACMM's upsample branch (ACMM.cu:532-601): depth = the JBU of the
+         * coarse depth (RunJBU, ACMM.cpp:804-853, through depths.dmb),
+         * normal = the JBU of the coarse normals, normalised, then
+         * TransformNormal2RefCam and GetDistance2Origin. pre-cost = the cost
+         * of the plane (0, 0, 0, depth): the normal there comes from a fresh
+         * host array (ACMM.cpp:711-717), taken as zeros. */
+        const int scale_i = (int)std::max((float)cx.W / (float)pb->coarse_w, (float)cx.H / (float)pb->coarse_h);
+        float depth;
+        F4 n;
+        jbu_pixel(pb->images[0], cx.W, cx.H, pb->coarse, pb->coarse_w, pb->coarse_h, scale_i, px, py, &depth, &n);
+        const F4 z = {0.0f, 0.0f, 0.0f, depth};
+        uint32_t m0;
+        pb->pre_cost[center] = initial_cost(cx, rp, px, py, z, &m0, nev);
+        F4 c;
+        c.x = rc.R[0] * n.x + rc.R[1] * n.y + rc.R[2] * n.z;
+        c.y = rc.R[3] * n.x + rc.R[4] * n.y + rc.R[5] * n.z;
+        c.z = rc.R[6] * n.x + rc.R[7] * n.y + rc.R[8] * n.z;
         c.w = dist_to_origin(rc, px, py, depth, c);
         st4(hyp, center, c);
         cost[center] = initial_cost(cx, rp, px, py, c, &sel[center], nev);
@@ -1178,7 +1253,9 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
     }
     const int min_idx = find_min_idx(fc, 8);
 
-    /* Re-score the current hypothesis (HPM.cu:1637-1650) */
+    /* Re-score the current hypothesis (HPM.cu:1637-1650); the geometric
+     * term weighs 0.1 there and 0.2 in ACMM (ACMM.cu:1596, 651) */
+    const float gc_weight = P.acmm ? 0.2f : 0.1f;
     float cvn[32];
     const F4 cur = ld4(hyp, center);
     cost_vector(cx, rp, px, py, cur, cvn, vw, nev);
@@ -1188,7 +1265,7 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
             continue; /* the term is +0 */
         }
         if (P.geom_consistency) {
-            cost_now += vw[i] * (cvn[i] + 0.1f * geom_cost(cx, i + 1, cur, px, py));
+            cost_now += vw[i] * (cvn[i] + gc_weight * geom_cost(cx, i + 1, cur, px, py));
         } else {
             cost_now += vw[i] * cvn[i];
         }
@@ -1247,7 +1324,7 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
             cost[center] = fc[min_idx];
             sel[center] = tsel;
         }
-        if (cost[center] != cost[center]) {
+        if (!P.acmm && cost[center] != cost[center]) { /* no such fallback in ACMM (ACMM.cu:1607-1616) */
             depth_now = db;
             st4(hyp, center, hp);
             cost_now = fc[min_idx];
@@ -1290,7 +1367,7 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
             for (int j = 0; j < nsrc; ++j) {
                 if (vw[j] > 0) {
                     if (P.geom_consistency) {
-                        tc += vw[j] * (cvt[j] + 0.1f * geom_cost(cx, j + 1, t, px, py));
+                        tc += vw[j] * (cvt[j] + gc_weight * geom_cost(cx, j + 1, t, px, py));
                     } else {
                         tc += vw[j] * cvt[j];
                     }
@@ -1315,6 +1392,17 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
                 }
             }
         }
+    }
+
+    /* ACMM hierarchy (ACMM.cu:1619-1628): the adopted and refined
+     * hypothesis replaces the current one only when its cost beats the
+     * pre-cost by 0.1; otherwise the hypothesis stays and the cost is the
+     * re-scored one (written before the adoption). The reference refines a
+     * local copy that is left uninitialised when nothing is adopted
+     * (ACMM.cu:1605); it is taken as the current hypothesis. */
+    if (P.hierarchy && !(cost[center] < pb->pre_cost[center] - 0.1f)) {
+        st4(hyp, center, cur);
+        cost[center] = cost_now;
     }
 }
 
@@ -1434,6 +1522,16 @@ static int check_problem(const hpm_oracle_problem* pb)
     if (pb->params.planar_prior && (pb->prior_planes == nullptr || pb->prior_mask == nullptr)) {
         return -1;
     }
+    if (pb->params.acmm && (pb->params.max_iterations < 1 || pb->params.max_iterations > 3 ||
+                            pb->params.planar_prior)) {
+        return -1;
+    }
+    if (pb->params.hierarchy &&
+        (!pb->params.acmm || pb->params.geom_consistency || pb->coarse == nullptr || pb->pre_cost == nullptr ||
+         pb->coarse_w < 1 || pb->coarse_h < 1 ||
+         std::max(pb->cams[0].height / pb->coarse_h, pb->cams[0].width / pb->coarse_w) < 2)) {
+        return -1;
+    }
     return 0;
 }
 
@@ -1528,7 +1626,9 @@ extern "C" int hpm_oracle_run_patchmatch(const hpm_oracle_problem* pb, float* hy
     if (hpm_oracle_init(pb, hyp, cost, selmask, &total, num_threads) != 0) {
         return -1;
     }
-    for (int it = 0; it < 3; ++it) { /* hard-coded 3 (HPM.cu:1933) */
+    /* hard-coded 3 (HPM.cu:1933); ACMM runs max_iterations (ACMM.cu:1836) */
+    const int iterations = pb->params.acmm ? pb->params.max_iterations : 3;
+    for (int it = 0; it < iterations; ++it) {
         hpm_oracle_half_iteration(pb, it, 0, hyp, cost, selmask, &total, num_threads);
         hpm_oracle_half_iteration(pb, it, 1, hyp, cost, selmask, &total, num_threads);
     }
@@ -1582,55 +1682,15 @@ extern "C" int hpm_oracle_jbu(const float* guide, int w, int h, const float* src
         return 1;
     }
     const int nt = nthreads_or_default(num_threads);
-    const float scale = (float)(1.0 * sw / w);
-    const float sigmad = 0.50f, sigmar = 25.5f;
-    const int win = scale_i * scale_i + 1;
-    const int nn = win / 2;
 #pragma omp parallel for num_threads(nt)
     for (int py = 0; py < h; ++py) {
         for (int px = 0; px < w; ++px) {
-            const float o_y = (float)py * scale;
-            const float o_x = (float)px * scale;
-            const float ref_pix = texel(guide, w, h, px, py);
-            float nf = 0.0f, ctot = 0.0f;
-            F4 ntot = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (int j = -nn; j <= nn; ++j) {
-                int r_y = (int)(o_y + (float)j);
-                r_y = (r_y > 0 ? (r_y < sh ? r_y : sh - 1) : 0);
-                const int r_ys = py + j;
-                for (int i = -nn; i <= nn; ++i) {
-                    int r_x = (int)(o_x + (float)i);
-                    r_x = (r_x > 0 ? (r_x < sw ? r_x : sw - 1) : 0);
-                    const int s_center = r_y * sw + r_x;
-                    const float src_pix = src_nd[4 * s_center + 3];
-                    F4 sn = ld4(src_nd, s_center);
-                    const int r_xs = px + i;
-                    const float nb_pix = texel(guide, w, h, r_xs, r_ys);
-                    const float ddx = o_x - (float)r_x, ddy = o_y - (float)r_y;
-                    const float dis = ddx * ddx + ddy * ddy;
-                    const float sg = hpm_expf(-dis / (2 * sigmad * sigmad));
-                    const float xp = fabsf(ref_pix - nb_pix);
-                    const float rg = hpm_expf(-(xp * xp) / (2 * sigmar * sigmar));
-                    const float tg = sg * rg;
-                    nf += tg;
-                    ctot += src_pix * tg;
-                    sn.x = sn.x * tg;
-                    sn.y = sn.y * tg;
-                    sn.z = sn.z * tg;
-                    ntot.x = ntot.x + sn.x;
-                    ntot.y = ntot.y + sn.y;
-                    ntot.z = ntot.z + sn.z;
-                }
-            }
-            ntot.x = ntot.x / nf;
-            ntot.y = ntot.y / nf;
-            ntot.z = ntot.z / nf;
-            normalize3(&ntot);
             const int center = py * w + px;
-            out_depth[center] = ctot / nf;
-            out_normal[4 * center + 0] = ntot.x;
-            out_normal[4 * center + 1] = ntot.y;
-            out_normal[4 * center + 2] = ntot.z;
+            F4 n;
+            jbu_pixel(guide, w, h, src_nd, sw, sh, scale_i, px, py, &out_depth[center], &n);
+            out_normal[4 * center + 0] = n.x;
+            out_normal[4 * center + 1] = n.y;
+            out_normal[4 * center + 2] = n.z;
             out_normal[4 * center + 3] = 0.0f;
         }
     }
@@ -1751,9 +1811,14 @@ float fusion_angle(const float* a, const float* b)
 
 } // namespace
 
-extern "C" int64_t hpm_oracle_fusion(int num_views, const hpm_camera* cams, const float* const* depths,
-                                     const float* const* normals, const uint8_t* const* colors, const int* num_src,
-                                     const int* src_ids, int max_src, hpm_point* out, int64_t cap)
+/* RunFusion of HPM-MVS (main.cpp:502-633) or, with acmm, of ACMM+NESP
+ * (ACMM+NESP/windows/main.cpp:164-300): there the point, normal and colour
+ * are the means of the reference pixel's and the consistent sources' (the
+ * normal through cv::Vec3f /= float, a product with the float reciprocal),
+ * and a point needs 2 consistent sources. */
+static int64_t fusion(bool acmm, int num_views, const hpm_camera* cams, const float* const* depths,
+                      const float* const* normals, const uint8_t* const* colors, const int* num_src,
+                      const int* src_ids, int max_src, hpm_point* out, int64_t cap)
 {
     std::vector<std::vector<uint8_t>> masks((size_t)num_views);
     for (int i = 0; i < num_views; ++i) {
@@ -1784,6 +1849,10 @@ extern "C" int64_t hpm_oracle_fusion(int num_views, const hpm_camera* cams, cons
                 const P3 X = fusion_world_point(c, r, ref_depth, rc);
                 int num_consistent = 0;
                 float dyn = 0.0f;
+                /* ACMM's sums, from the reference pixel */
+                P3 sum_x = X;
+                float sum_n[3] = {ref_normal[0], ref_normal[1], ref_normal[2]};
+                float sum_c[3] = {(float)colors[i][3 * p], (float)colors[i][3 * p + 1], (float)colors[i][3 * p + 2]};
                 for (int j = 0; j < nn; ++j) {
                     const int s = srcs[j];
                     const hpm_camera& sc = cams[s];
@@ -1809,6 +1878,18 @@ extern "C" int64_t hpm_oracle_fusion(int num_views, const hpm_camera* cams, cons
                     const float rdd = fabsf(proj_depth - ref_depth) / ref_depth;
                     const float angle = fusion_angle(ref_normal, normals[s] + 3 * q);
                     if (reproj < 2.0f && rdd < 0.01f && angle < 0.174533f) {
+                        if (acmm) {
+                            const float* sn = normals[s] + 3 * q;
+                            sum_x.x += Y.x;
+                            sum_x.y += Y.y;
+                            sum_x.z += Y.z;
+                            sum_n[0] = sum_n[0] + sn[0];
+                            sum_n[1] = sum_n[1] + sn[1];
+                            sum_n[2] = sum_n[2] + sn[2];
+                            sum_c[0] += (float)colors[s][3 * q];
+                            sum_c[1] += (float)colors[s][3 * q + 1];
+                            sum_c[2] += (float)colors[s][3 * q + 2];
+                        }
                         ux[j] = sc_;
                         uy[j] = sr;
                         const float index = reproj + 200.0f * rdd + angle * 10.0f;
@@ -1816,18 +1897,34 @@ extern "C" int64_t hpm_oracle_fusion(int num_views, const hpm_camera* cams, cons
                         ++num_consistent;
                     }
                 }
-                if (num_consistent >= 1 && (double)dyn > 0.3 * (double)num_consistent) {
+                const bool keep = acmm ? num_consistent >= 2
+                                       : (num_consistent >= 1 && (double)dyn > 0.3 * (double)num_consistent);
+                if (keep) {
                     if (n < cap) {
                         hpm_point& pt = out[n];
-                        pt.x = X.x;
-                        pt.y = X.y;
-                        pt.z = X.z;
-                        pt.nx = ref_normal[0];
-                        pt.ny = ref_normal[1];
-                        pt.nz = ref_normal[2];
-                        pt.r = colors[i][3 * p];
-                        pt.g = colors[i][3 * p + 1];
-                        pt.b = colors[i][3 * p + 2];
+                        if (acmm) {
+                            const float k = num_consistent + 1.0f;
+                            const float ik = 1.0f / k;
+                            pt.x = sum_x.x / k;
+                            pt.y = sum_x.y / k;
+                            pt.z = sum_x.z / k;
+                            pt.nx = sum_n[0] * ik;
+                            pt.ny = sum_n[1] * ik;
+                            pt.nz = sum_n[2] * ik;
+                            pt.r = (uint8_t)(int)(sum_c[0] / k);
+                            pt.g = (uint8_t)(int)(sum_c[1] / k);
+                            pt.b = (uint8_t)(int)(sum_c[2] / k);
+                        } else {
+                            pt.x = X.x;
+                            pt.y = X.y;
+                            pt.z = X.z;
+                            pt.nx = ref_normal[0];
+                            pt.ny = ref_normal[1];
+                            pt.nz = ref_normal[2];
+                            pt.r = colors[i][3 * p];
+                            pt.g = colors[i][3 * p + 1];
+                            pt.b = colors[i][3 * p + 2];
+                        }
                         pt.pad_ = 0;
                     }
                     ++n;
@@ -1843,4 +1940,35 @@ extern "C" int64_t hpm_oracle_fusion(int num_views, const hpm_camera* cams, cons
         }
     }
     return n;
+}
+
+extern "C" int64_t hpm_oracle_fusion(int num_views, const hpm_camera* cams, const float* const* depths,
+                                     const float* const* normals, const uint8_t* const* colors, const int* num_src,
+                                     const int* src_ids, int max_src, hpm_point* out, int64_t cap)
+{
+    return fusion(false, num_views, cams, depths, normals, colors, num_src, src_ids, max_src, out, cap);
+}
+
+extern "C" int64_t hpm_oracle_fusion_acmm(int num_views, const hpm_camera* cams, const float* const* depths,
+                                          const float* const* normals, const uint8_t* const* colors,
+                                          const int* num_src, const int* src_ids, int max_src, hpm_point* out,
+                                          int64_t cap)
+{
+    return fusion(true, num_views, cams, depths, normals, colors, num_src, src_ids, max_src, out, cap);
+}
+
+/* ComputeGeomConsistencyCost for given camera-frame planes at given pixels
+ * against source view v (for the tests). */
+extern "C" int hpm_oracle_geom_cost(const hpm_oracle_problem* pb, int v, const float* planes, const int32_t* pix,
+                                    int npix, float* out)
+{
+    if (check_problem(pb) != 0 || v < 1 || v >= pb->num_images || pb->depths[v] == nullptr) {
+        return -1;
+    }
+    Ctx cx;
+    make_ctx(pb, &cx);
+    for (int i = 0; i < npix; ++i) {
+        out[i] = geom_cost(cx, v, ld4(planes, i), pix[2 * i], pix[2 * i + 1]);
+    }
+    return 0;
 }

@@ -38,6 +38,12 @@ typedef struct hpm_oracle_problem {
     const uint32_t* prior_mask;              /* planar prior: u32 per ref pixel */
     uint32_t call_index;                     /* RunPatchMatch call number on this problem (RNG stream) */
     uint32_t salt;                           /* reference view id (RNG stream) */
+    /* ACMM hierarchy (params.hierarchy): the coarser map, float4 per coarse
+     * pixel (world normal xyz, depth), and the pre-cost per pixel (written
+     * by the initialisation, read by the propagation) */
+    const float* coarse;
+    int32_t coarse_w, coarse_h;
+    float* pre_cost;
 } hpm_oracle_problem;
 
 /* One full RunPatchMatch (HPM.cu:1906-1957): init, 3 red/black iterations,
@@ -64,6 +70,12 @@ int hpm_oracle_filter(const hpm_oracle_problem* pb, int colour, float* hyp, cons
  * HIP path. */
 int hpm_oracle_cost_vectors(const hpm_oracle_problem* pb, const float* planes, const int32_t* pix,
                             int npix, int fp64, double* out, int num_threads);
+
+/* ComputeGeomConsistencyCost (HPM.cu:483-508; capped at 3 instead of 5 with
+ * params.acmm, ACMM.cu:494-519) of npix camera-frame planes at pixels pix
+ * ((x, y) pairs) against source view v; needs depths[v]. */
+int hpm_oracle_geom_cost(const hpm_oracle_problem* pb, int v, const float* planes, const int32_t* pix, int npix,
+                         float* out);
 
 /* Joint bilateral upsampling (HPM.cu:1984-2052) of a low-resolution
  * depth/normal map guided by a full-resolution grey image. src_nd is float4
@@ -103,6 +115,10 @@ int hpm_oracle_uniforms(uint32_t seed_lo, uint32_t seed_hi, uint32_t pixel, uint
 int64_t hpm_oracle_fusion(int num_views, const hpm_camera* cams, const float* const* depths,
                           const float* const* normals, const uint8_t* const* colors, const int* num_src,
                           const int* src_ids, int max_src, hpm_point* out, int64_t cap);
+/* The same with ACMM+NESP's rule (ACMM+NESP/windows/main.cpp:164-300). */
+int64_t hpm_oracle_fusion_acmm(int num_views, const hpm_camera* cams, const float* const* depths,
+                               const float* const* normals, const uint8_t* const* colors, const int* num_src,
+                               const int* src_ids, int max_src, hpm_point* out, int64_t cap);
 
 #ifdef __cplusplus
 }

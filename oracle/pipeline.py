@@ -1,4 +1,6 @@
-"""CPU restatement of the HPM-MVS pass schedule (main.cpp:35-499, 636-674).
+"""CPU restatement of the HPM-MVS pass schedule (main.cpp:35-499, 636-674)
+and of the ACMM+NESP coarse-to-fine schedule (ACMM+NESP/windows/main.cpp:
+35-378).
 
 TEST INFRASTRUCTURE: the checker of hpm_pipeline_run. It runs every
 RunPatchMatch and JBU on the CPU oracle and builds the planar prior with the
@@ -158,3 +160,89 @@ def run_cpu_pipeline(cams, images, problems, opt, threads=0):
                 state[ref] = (o.hyp.copy(), o.cost.copy())
                 d_geom[ref] = np.ascontiguousarray(o.hyp[..., 3]).copy()
     return state, d_plain, d_geom
+
+
+def _scale_view(cam, image, cur):
+    """ACMM::InuputInitialization's rescale (ACMM.cpp:497-523) in float32,
+    with the library's bilinear resize (the device schedule's)."""
+    f32 = np.float32
+    W, H = cam.width, cam.height
+    c = _hpm.HpmCamera()
+    C.pointer(c)[0] = cam
+    if W <= cur and H <= cur:
+        return c, np.ascontiguousarray(image, np.float32)
+    f = min(f32(cur) / f32(W), f32(cur) / f32(H))
+    nw = int(np.floor(np.float64(f32(W) * f) + 0.5))
+    nh = int(np.floor(np.float64(f32(H) * f) + 0.5))
+    out = _hpm.resize_bilinear(np.ascontiguousarray(image, np.float32), nw, nh)
+    sx, sy = f32(nw) / f32(W), f32(nh) / f32(H)
+    c.K[0] = float(f32(c.K[0]) * sx)
+    c.K[2] = float(f32(c.K[2]) * sx)
+    c.K[4] = float(f32(c.K[4]) * sy)
+    c.K[5] = float(f32(c.K[5]) * sy)
+    c.width, c.height = nw, nh
+    return c, out
+
+
+def run_cpu_acmm(cams, images, problems, opt, threads=0):
+    """The ACMM+NESP levels (main.cpp:302-374) on the CPU oracle, in the
+    device schedule's order and random streams (hpm_acmm_run). opt: an
+    HpmAcmmOptions. Returns (state {view: (hyp_world, cost)}, first-pass
+    depths, geometric depths, level cameras) of the last level."""
+    params = opt.params
+    prob_of = {r: i for i, (r, _) in enumerate(problems)}
+    max_size, left, cur = [], [], []
+    for r, _ in problems:
+        m = min(max(cams[r].width, cams[r].height), params.max_image_size)
+        max_size.append(m)
+        k = 0
+        while m > opt.scale_bound:
+            m //= 2
+            k += 1
+        left.append(k)
+        cur.append(max_size[-1])
+    prev = None
+    for step in range(max(left) + 1):
+        for i in range(len(problems)):
+            if left[i] >= 0:
+                cur[i] = int(max_size[i] / 2.0 ** left[i])
+                left[i] -= 1
+        lc, li = [], []
+        for v in range(len(cams)):
+            size = cur[prob_of[v]] if v in prob_of else max(cams[v].width, cams[v].height)
+            c, im = _scale_view(cams[v], images[v], size)
+            lc.append(c)
+            li.append(im)
+        state, d_first, d_geom = {}, {}, {}
+        for ref, srcs in problems:
+            order = [ref] + list(srcs)
+            cs = [lc[v] for v in order]
+            p = cpu.relaxed_params(cs, params)
+            p.planar_prior = p.geom_consistency = p.multi_geometry = p.hierarchy = 0
+            o = cpu.Oracle(cs, [li[v] for v in order], p, salt=ref, threads=threads)
+            o.pb.call_index = 16 * step
+            if prev is not None:
+                o.set_hierarchy(prev[ref][0])
+            o.run_patchmatch()
+            state[ref] = (o.hyp.copy(), o.cost.copy())
+            d_first[ref] = np.ascontiguousarray(o.hyp[..., 3]).copy()
+        for g in range(opt.geom_iterations):
+            for ref, srcs in problems:
+                order = [ref] + list(srcs)
+                cs = [lc[v] for v in order]
+                maps = d_geom if g > 0 else d_first
+                p = cpu.relaxed_params(cs, params)
+                p.planar_prior = p.hierarchy = 0
+                p.geom_consistency = 1
+                p.multi_geometry = 1 if g > 0 else 0
+                p.max_iterations = 2
+                o = cpu.Oracle(cs, [li[v] for v in order], p, depths=[maps[v] for v in order], salt=ref,
+                               threads=threads)
+                o.hyp[...] = state[ref][0]
+                o.cost[...] = state[ref][1]
+                o.pb.call_index = 16 * step + 8 + g
+                o.run_patchmatch()
+                state[ref] = (o.hyp.copy(), o.cost.copy())
+                d_geom[ref] = np.ascontiguousarray(o.hyp[..., 3]).copy()
+        prev = state
+    return state, d_first, d_geom, lc

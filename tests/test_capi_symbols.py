@@ -8,7 +8,7 @@ from conftest import ROOT
 
 # headers of the product library; hpm_types/philox/fmath hold inline code only
 HEADERS = ["hpm_capi.h", "hpm_io.h", "hpm_scene.h", "hpm_prior.h", "hpm_pipeline.h", "hpm_batch.h", "hpm_unet.h",
-           "hpm_fusion.h"]
+           "hpm_fusion.h", "hpm_acmm.h"]
 
 
 def declared_functions(path):
@@ -42,6 +42,6 @@ def test_bindings_cover_every_declared_symbol(hpm):
 
 
 def test_params_struct_matches_header(hpm):
-    # hpm_params: 20 four-byte fields; hpm_camera: 25 four-byte fields
-    assert ctypes.sizeof(hpm.HpmParams) == 80
+    # hpm_params: 22 four-byte fields; hpm_camera: 25 four-byte fields
+    assert ctypes.sizeof(hpm.HpmParams) == 88
     assert ctypes.sizeof(hpm.HpmCamera) == 100

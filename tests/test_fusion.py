@@ -68,7 +68,10 @@ def _project(c, X):
     return u, v, d
 
 
-def _python_fusion(cpu, cams, depths, normals, colors, problems):
+def _python_fusion(cpu, cams, depths, normals, colors, problems, acmm=False):
+    """acmm: ACMM+NESP's RunFusion (ACMM+NESP/windows/main.cpp:164-300): means
+    of the reference's and the consistent sources' points, normals (times the
+    float reciprocal, cv::Vec3f /= float) and colours, 2 sources needed"""
     masks = [np.zeros(d.shape, np.uint8) for d in depths]
     out = []
     for i, (_, srcs) in enumerate(problems):
@@ -83,31 +86,46 @@ def _python_fusion(cpu, cams, depths, normals, colors, problems):
                     continue
                 X = _world(rc, c, r, ref_depth)
                 n_cons, dyn = 0, F(0)
+                sx = list(X)
+                sn = [F(v) for v in normals[i][r, c]]
+                scol = [F(v) for v in colors[i][r, c]]
                 for j, s in enumerate(srcs):
                     sc = cams[s]
                     u, v, _ = _project(sc, X)
-                    sr, scol = int(F(v + F(0.5))), int(F(u + F(0.5)))
-                    if not (0 <= scol < sc.width and 0 <= sr < sc.height):
+                    sr, sc_ = int(F(v + F(0.5))), int(F(u + F(0.5)))
+                    if not (0 <= sc_ < sc.width and 0 <= sr < sc.height):
                         continue
-                    if masks[s][sr, scol] == 1:
+                    if masks[s][sr, sc_] == 1:
                         continue
-                    sd = F(depths[s][sr, scol])
+                    sd = F(depths[s][sr, sc_])
                     if sd <= 0:
                         continue
-                    tu, tv, pd = _project(rc, _world(sc, scol, sr, sd))
+                    Y = _world(sc, sc_, sr, sd)
+                    tu, tv, pd = _project(rc, Y)
                     dx, dy = float(F(F(c) - tu)), float(F(F(r) - tv))
                     reproj = F(math.sqrt(dx * dx + dy * dy))
                     rdd = F(abs(F(pd - ref_depth)) / ref_depth)
-                    a, b = normals[i][r, c], normals[s][sr, scol]
+                    a, b = normals[i][r, c], normals[s][sr, sc_]
                     dot = F(F(F(a[0] * b[0]) + F(a[1] * b[1])) + F(a[2] * b[2]))
                     ang = _shared(cpu, 3, dot)
                     ang = F(0) if ang != ang else ang
                     if reproj < F(2) and rdd < F(0.01) and ang < F(0.174533):
-                        used[j] = (scol, sr)
+                        sx = [F(sx[k] + Y[k]) for k in range(3)]
+                        sn = [F(sn[k] + F(b[k])) for k in range(3)]
+                        scol = [F(scol[k] + F(colors[s][sr, sc_][k])) for k in range(3)]
+                        used[j] = (sc_, sr)
                         idx = F(F(reproj + F(F(200) * rdd)) + F(ang * F(10)))
                         dyn = F(dyn + _shared(cpu, 0, -idx))
                         n_cons += 1
-                if n_cons >= 1 and float(dyn) > 0.3 * n_cons:
+                if acmm and n_cons >= 2:
+                    k = F(F(n_cons) + F(1))
+                    ik = F(F(1) / k)
+                    out.append((tuple(F(v / k) for v in sx), tuple(F(v * ik) for v in sn),
+                                tuple(int(F(v / k)) for v in scol)))
+                    for j, s in enumerate(srcs):
+                        if used[j][0] != -1:
+                            masks[s][used[j][1], used[j][0]] = 1
+                elif not acmm and n_cons >= 1 and float(dyn) > 0.3 * n_cons:
                     out.append((X, tuple(normals[i][r, c]), tuple(colors[i][r, c])))
                     for j, s in enumerate(srcs):
                         if used[j][0] != -1:
@@ -133,6 +151,32 @@ def test_oracle_fusion_matches_python(hpm, cpu):
     got = cpu.fusion(cams, depths, normals, colors, problems, hpm.POINT_DTYPE)
     want = _python_fusion(cpu, cams, depths, normals, colors, problems)
     assert len(got) == len(want) and len(got) > 100
+    for g, (X, n, col) in zip(got, want):
+        assert (g["x"], g["y"], g["z"]) == X
+        assert (g["nx"], g["ny"], g["nz"]) == n
+        assert (g["r"], g["g"], g["b"]) == col
+
+
+def test_oracle_acmm_fusion_matches_python(hpm, cpu):
+    """ACMM+NESP's RunFusion (means over the consistent sources, 2 needed)
+    in the oracle against the pure-Python loops, bit for bit"""
+    cfg = hpm.scene_config(seed=9, width=40, height=30, fx=32.0, fy=32.0, cx=20.0, cy=15.0, num_views=4,
+                           num_planes=3, num_boxes=2)
+    cams, imgs, gd, gn, low = hpm.render_scene(cfg)
+    rng = np.random.default_rng(1)
+    depths = [(d * (1 + 0.003 * rng.standard_normal(d.shape))).astype(np.float32) for d in gd]
+    for d in depths:
+        d[rng.random(d.shape) < 0.05] = 0.0
+    normals = [n.astype(np.float32).copy() for n in gn]
+    for n in normals:
+        flip = rng.random(n.shape[:2]) < 0.05
+        n[flip] = -n[flip]
+    colors = [np.repeat(im.astype(np.uint8)[..., None], 3, axis=2) for im in imgs]
+    problems = [(i, [j for j in range(4) if j != i]) for i in range(4)]
+    got = cpu.fusion(cams, depths, normals, colors, problems, hpm.POINT_DTYPE, acmm=True)
+    want = _python_fusion(cpu, cams, depths, normals, colors, problems, acmm=True)
+    plain = cpu.fusion(cams, depths, normals, colors, problems, hpm.POINT_DTYPE)
+    assert len(got) == len(want) and len(got) > 100 and len(got) != len(plain)
     for g, (X, n, col) in zip(got, want):
         assert (g["x"], g["y"], g["z"]) == X
         assert (g["nx"], g["ny"], g["nz"]) == n

@@ -190,7 +190,9 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
                    "hierarchy input");
         if (rc < 0) return rc;
     }
-    rc = check(c, h, hpm_run_internal(h, c.opt->planar_prior != 0), "RunPatchMatch");
+    /* the host builds the planar prior from the maps it reads back */
+    const bool host_prior = c.opt->planar_prior != 0 && !c.opt->prior_on_device;
+    rc = check(c, h, hpm_run_internal(h, host_prior), "RunPatchMatch");
     if (rc < 0) return rc;
     account(c, h);
     const hpm_camera ref_cam = h->cams[0];
@@ -224,6 +226,30 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
         /* fallback: the planar prior below, from the plain pass */
         h->params.planar_prior = 0;
         h->have_prior = false;
+    }
+    if (c.opt->planar_prior && c.opt->prior_on_device) {
+        hpm_set_planar_prior(h); /* SetPlanarPriorParams (HPM.cpp:477-480) */
+        int k = 0;
+        for (int max_size = std::max(W, H); max_size > c.opt->scale_bound; max_size /= 2) {
+            ++k;
+        }
+        for (int i = 2 - k; i <= 2; ++i) {
+            const float f = i == 0 ? 0.25f : (i == 1 ? 0.5f : 1.0f);
+            auto t0 = clk::now();
+            double host_ms = 0.0;
+            rc = check(c, h, hpm_build_planar_prior(h, f, c.opt->params.knn_k, &host_ms), "planar prior");
+            if (rc < 0) return rc;
+            add_ms(c, &c.st->ms_prior_host, host_ms);
+            add_ms(c, &c.st->ms_prior_device, ms_since(t0) - host_ms);
+            if (c.opt->verbose >= 2) {
+                fprintf(stderr, "view %d prior %.2fx on the device: %.1f ms (host %.1f ms)\n", ref, f, ms_since(t0),
+                        host_ms);
+            }
+            rc = check(c, h, hpm_run_internal(h, false), "RunPatchMatch (prior)");
+            if (rc < 0) return rc;
+            account(c, h);
+        }
+        return check(c, h, hpm_store_to_pool(h, 0), "store");
     }
     if (c.opt->planar_prior) {
         hpm_set_planar_prior(h); /* SetPlanarPriorParams (HPM.cpp:477-480) */
@@ -373,6 +399,7 @@ extern "C" void hpm_pipeline_default(hpm_pipeline_options* opt)
     opt->learned_min_accept = 0.3f;
     opt->hierarchy_from = nullptr;
     opt->stream_base = 0;
+    opt->prior_on_device = 1;
 }
 
 extern "C" int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, float* cost)

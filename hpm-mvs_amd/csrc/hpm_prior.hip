@@ -1,0 +1,410 @@
+/*
+ * hpm_prior.hip — the planar prior's per-pixel and per-triangle stages on
+ * the device (main.cpp:84-162, 238-314, 387-454 with HPM.cpp:759-1123):
+ * scale maps, support-point blocks, triangle rasterisation and plane fits,
+ * the KNN supplement and the prior assembly. They compute what the host
+ * functions of hpm_prior.cpp and the assembly loops of hpm_pipeline.cpp
+ * compute, in the same arithmetic (no contraction; float where those use
+ * float, double where they use double), so the prior is the same bit for
+ * bit. Delaunay stays on the host (hpm_delaunay).
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "hpm_device.h"
+
+namespace {
+
+/* hpm_resize_bilinear_c (hpm_io.cpp) for one output texel of a strided map */
+__device__ float resize_texel(const float* src, int stride, int sw, int sh, int dw, int dh, int x, int y)
+{
+    const double sx_scale = (double)sw / dw;
+    const double sy_scale = (double)sh / dh;
+    float fy = (float)((y + 0.5) * sy_scale - 0.5);
+    int y0 = (int)floorf(fy);
+    fy -= (float)y0;
+    int y1 = y0 + 1;
+    if (y0 < 0) {
+        y0 = 0;
+        y1 = 0;
+        fy = 0.0f;
+    }
+    if (y1 >= sh) {
+        y0 = sh - 1;
+        y1 = sh - 1;
+        fy = 0.0f;
+    }
+    float fx = (float)((x + 0.5) * sx_scale - 0.5);
+    int x0 = (int)floorf(fx);
+    fx -= (float)x0;
+    int x1 = x0 + 1;
+    if (x0 < 0) {
+        x0 = 0;
+        x1 = 0;
+        fx = 0.0f;
+    }
+    if (x1 >= sw) {
+        x0 = sw - 1;
+        x1 = sw - 1;
+        fx = 0.0f;
+    }
+    const float a = src[((size_t)y0 * sw + x0) * stride];
+    const float b = src[((size_t)y0 * sw + x1) * stride];
+    const float d = src[((size_t)y1 * sw + x0) * stride];
+    const float e = src[((size_t)y1 * sw + x1) * stride];
+    const float top = a * (1.0f - fx) + b * fx;
+    const float bot = d * (1.0f - fx) + e * fx;
+    return top * (1.0f - fy) + bot * fy;
+}
+
+/* Get3DPointonRefCam_factor (HPM.cpp:236-245) */
+__device__ __forceinline__ void point_ref_factor(const hpm_camera& c, int x, int y, float depth, float factor,
+                                                 float X[3])
+{
+    X[0] = depth * ((float)x - c.K[2] * factor) / (c.K[0] * factor);
+    X[1] = depth * ((float)y - c.K[5] * factor) / (c.K[4] * factor);
+    X[2] = depth;
+}
+
+/* GetPriorPlaneParams_factor (HPM.cpp:811-844) as hpm_plane_from_points:
+ * the null vector of [X_i 1] in double, unit normal with w >= 0 */
+__device__ float4 plane_from_points(const hpm_camera& ref, const int32_t p[6], const float* depth, int width,
+                                    float factor)
+{
+    float X[3][3];
+    for (int i = 0; i < 3; ++i) {
+        const int x = p[2 * i], y = p[2 * i + 1];
+        point_ref_factor(ref, x, y, depth[(size_t)y * width + x], factor, X[i]);
+    }
+    const double a0 = (double)X[1][0] - X[0][0], a1 = (double)X[1][1] - X[0][1], a2 = (double)X[1][2] - X[0][2];
+    const double b0 = (double)X[2][0] - X[0][0], b1 = (double)X[2][1] - X[0][1], b2 = (double)X[2][2] - X[0][2];
+    const double n0 = a1 * b2 - a2 * b1, n1 = a2 * b0 - a0 * b2, n2 = a0 * b1 - a1 * b0;
+    const double w = -(n0 * X[0][0] + n1 * X[0][1] + n2 * X[0][2]);
+    double norm = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    if (w < 0) {
+        norm = -norm;
+    }
+    return make_float4((float)(n0 / norm), (float)(n1 / norm), (float)(n2 / norm), (float)(w / norm));
+}
+
+/* GetDepthFromPlaneParam_factor (HPM.cpp:846-849), hpm_depth_from_plane_factor */
+__device__ __forceinline__ float depth_from_plane_factor(const hpm_camera& c, float4 pl, int x, int y, float factor)
+{
+    return -pl.w * (c.K[0] * factor) /
+           (((float)x - c.K[2] * factor) * pl.x + (c.K[0] / c.K[4]) * ((float)y - c.K[5] * factor) * pl.y +
+            c.K[0] * factor * pl.z);
+}
+
+} // namespace
+
+/* depth (the .w of the finished hypotheses) and cost at the prior's scale:
+ * copied at 1x, resized with the host's bilinear resize otherwise */
+__global__ __launch_bounds__(256) void hpm_k_prior_maps(const float* hyp4, const float* cost, int W, int H,
+                                                        float* depth_s, float* cost_s, int ws, int hs)
+{
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= ws || y >= hs) {
+        return;
+    }
+    const size_t q = (size_t)y * ws + x;
+    if (ws == W && hs == H) {
+        depth_s[q] = hyp4[4 * q + 3];
+        cost_s[q] = cost[q];
+    } else {
+        depth_s[q] = resize_texel(hyp4 + 3, 4, W, H, ws, hs, x, y);
+        cost_s[q] = resize_texel(cost, 1, W, H, ws, hs, x, y);
+    }
+}
+
+/* GetSupportPoints (HPM.cpp:759-785) per 5x5 block, blocks numbered
+ * column-major (the reference's scan order): the first strict minimum of
+ * the costs below 2 in column-major order, kept if below 0.1; packed x |
+ * y << 16, or -1 */
+__global__ __launch_bounds__(256) void hpm_k_support_blocks(const float* cost, int w, int h, int32_t* cand)
+{
+    const int nbx = (w + 4) / 5, nby = (h + 4) / 5;
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= nbx * nby) {
+        return;
+    }
+    const int col = (b / nby) * 5, row = (b % nby) * 5;
+    float min_cost = 2.0f;
+    int bx = -1, by = -1;
+    const int cb = min(w, col + 5), rb = min(h, row + 5);
+    for (int c = col; c < cb; ++c) {
+        for (int r = row; r < rb; ++r) {
+            const float v = cost[(size_t)r * w + c];
+            if (v < 2.0f && min_cost > v) {
+                bx = c;
+                by = r;
+                min_cost = v;
+            }
+        }
+    }
+    cand[b] = min_cost < 0.1f ? (bx | (by << 16)) : -1;
+}
+
+/* the triangle rasterisation of main.cpp:96-116 (hpm_rasterize_triangles):
+ * one lane per triangle; later triangles overwrite earlier ones there, so a
+ * pixel ends with the largest index + 1 that reaches it (atomicMax) */
+__global__ __launch_bounds__(256) void hpm_k_raster(const int32_t* pts, const int32_t* tris, const int32_t* tri_idx,
+                                                    int ntris, uint32_t* mask, int width, int height)
+{
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= ntris || tri_idx[t] < 0) {
+        return;
+    }
+    const uint32_t val = (uint32_t)tri_idx[t] + 1u;
+    const int x1 = pts[2 * tris[3 * t]], y1 = pts[2 * tris[3 * t] + 1];
+    const int x2 = pts[2 * tris[3 * t + 1]], y2 = pts[2 * tris[3 * t + 1] + 1];
+    const int x3 = pts[2 * tris[3 * t + 2]], y3 = pts[2 * tris[3 * t + 2] + 1];
+    const double d01x = (double)(x1 - x2), d01y = (double)(y1 - y2);
+    const double d02x = (double)(x1 - x3), d02y = (double)(y1 - y3);
+    const double d12x = (double)(x2 - x3), d12y = (double)(y2 - y3);
+    const float L01 = (float)sqrt(d01x * d01x + d01y * d01y);
+    const float L02 = (float)sqrt(d02x * d02x + d02y * d02y);
+    const float L12 = (float)sqrt(d12x * d12x + d12y * d12y);
+    const float max_edge = fmaxf(L01, fmaxf(L02, L12));
+    const float step = (float)(1.0 / max_edge);
+    for (float p = 0; p < 1.0; p += step) {
+        for (float q = 0; (double)q < 1.0 - (double)p; q += step) {
+            const int x = (int)((double)(p * (float)x1 + q * (float)x2) + (1.0 - (double)p - (double)q) * x3);
+            const int y = (int)((double)(p * (float)y1 + q * (float)y2) + (1.0 - (double)p - (double)q) * y3);
+            if (x >= 0 && x < width && y >= 0 && y < height) {
+                atomicMax(&mask[(size_t)y * width + x], val);
+            }
+        }
+    }
+}
+
+/* the plane of every triangle (main.cpp:117-121) */
+__global__ __launch_bounds__(256) void hpm_k_tri_planes(const int32_t* pts, const int32_t* tris, int ntris,
+                                                        const float* depth, int width, hpm_camera ref, float factor,
+                                                        float* planes)
+{
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= ntris) {
+        return;
+    }
+    const int32_t p[6] = {pts[2 * tris[3 * t]], pts[2 * tris[3 * t] + 1], pts[2 * tris[3 * t + 1]],
+                          pts[2 * tris[3 * t + 1] + 1], pts[2 * tris[3 * t + 2]], pts[2 * tris[3 * t + 2] + 1]};
+    reinterpret_cast<float4*>(planes)[t] = plane_from_points(ref, p, depth, width, factor);
+}
+
+/* DepthsPredictSupplement (HPM.cpp:923-1097) as hpm_knn_supplement, one lane
+ * per pixel no triangle covers: the k nearest support points in (squared
+ * distance, index) order from a uniform grid (cells of `cell` px, point
+ * indices ascending per cell; rings of cells until the next ring lies
+ * strictly farther than the k-th), the first non-collinear triple of
+ * `order`, its plane, kept when its depth is in range. */
+__global__ __launch_bounds__(256) void hpm_k_knn(const int32_t* pts, int npoints, const int32_t* cell_start,
+                                                 const int32_t* cell_pts, int cell, int gw, int gh,
+                                                 const int32_t* order, int norder, int k, const uint32_t* raster,
+                                                 const float* depth, int width, int height, hpm_camera ref,
+                                                 float factor, float depth_min, float depth_max, uint8_t* knn_flag,
+                                                 float* knn_planes)
+{
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (j >= width || i >= height) {
+        return;
+    }
+    const size_t center = (size_t)i * width + j;
+    knn_flag[center] = 0;
+    if (raster[center] > 0u || npoints < 3) {
+        return;
+    }
+    int64_t bd[32];
+    int bi[32];
+    int found = 0;
+    const int kk = min(k, npoints);
+    const int cx = min(j / cell, gw - 1), cy = min(i / cell, gh - 1);
+    for (int r = 0;; ++r) {
+        const int x_lo = cx - r, x_hi = cx + r, y_lo = cy - r, y_hi = cy + r;
+        for (int gy = max(y_lo, 0); gy <= min(y_hi, gh - 1); ++gy) {
+            for (int gx = max(x_lo, 0); gx <= min(x_hi, gw - 1); ++gx) {
+                if (gx != x_lo && gx != x_hi && gy != y_lo && gy != y_hi) {
+                    continue; /* inner cells: earlier rings */
+                }
+                const int c = gy * gw + gx;
+                for (int e = cell_start[c]; e < cell_start[c + 1]; ++e) {
+                    const int s = cell_pts[e];
+                    const int64_t dx = pts[2 * s] - j, dy = pts[2 * s + 1] - i;
+                    const int64_t d2 = dx * dx + dy * dy;
+                    if (found == kk && (d2 > bd[kk - 1] || (d2 == bd[kk - 1] && s > bi[kk - 1]))) {
+                        continue;
+                    }
+                    int pos = found < kk ? found : kk - 1;
+                    while (pos > 0 && (bd[pos - 1] > d2 || (bd[pos - 1] == d2 && bi[pos - 1] > s))) {
+                        bd[pos] = bd[pos - 1];
+                        bi[pos] = bi[pos - 1];
+                        --pos;
+                    }
+                    bd[pos] = d2;
+                    bi[pos] = s;
+                    if (found < kk) {
+                        ++found;
+                    }
+                }
+            }
+        }
+        if (x_lo <= 0 && y_lo <= 0 && x_hi >= gw - 1 && y_hi >= gh - 1) {
+            break; /* every cell seen */
+        }
+        if (found == kk) {
+            /* a point outside the rings so far lies beyond one of the sides
+             * the rings have not reached the grid's edge on */
+            int64_t gap = INT64_MAX;
+            if (x_lo > 0) gap = min(gap, (int64_t)(j - x_lo * cell + 1));
+            if (x_hi < gw - 1) gap = min(gap, (int64_t)((x_hi + 1) * cell - j));
+            if (y_lo > 0) gap = min(gap, (int64_t)(i - y_lo * cell + 1));
+            if (y_hi < gh - 1) gap = min(gap, (int64_t)((y_hi + 1) * cell - i));
+            if (gap * gap > bd[kk - 1]) {
+                break;
+            }
+        }
+    }
+    if (found < 3) {
+        return;
+    }
+    for (int t = 0; t < norder; ++t) {
+        if (order[3 * t + 2] >= found) {
+            continue;
+        }
+        const int a = bi[order[3 * t]], b = bi[order[3 * t + 1]], c = bi[order[3 * t + 2]];
+        const int64_t cr = (int64_t)(pts[2 * b] - pts[2 * a]) * (pts[2 * c + 1] - pts[2 * a + 1]) -
+                           (int64_t)(pts[2 * b + 1] - pts[2 * a + 1]) * (pts[2 * c] - pts[2 * a]);
+        if (cr != 0) {
+            const int32_t tri[6] = {pts[2 * a], pts[2 * a + 1], pts[2 * b], pts[2 * b + 1], pts[2 * c], pts[2 * c + 1]};
+            const float4 n4 = plane_from_points(ref, tri, depth, width, factor);
+            const float d = depth_from_plane_factor(ref, n4, j, i, factor);
+            if (d <= depth_max && d >= depth_min) {
+                knn_flag[center] = 1;
+                reinterpret_cast<float4*>(knn_planes)[center] = n4;
+            }
+            return;
+        }
+    }
+}
+
+/* The prior maps of one scale (main.cpp:124-162, 277-314 below full size,
+ * 427-454 at full size): the triangle plane (mask idx + 1) or the KNN plane
+ * (mask -233) where its depth passes the range gate. Below full size
+ * (src_nd != null): world normal and depth per pixel for the JBU, the gate
+ * relaxed to [0.6 dmin, 1.2 dmax] for triangles. At full size: camera-frame
+ * planes and masks 1 (the KNN mask counts as active, Q12). */
+__global__ __launch_bounds__(256) void hpm_k_prior_assemble(const uint32_t* raster, const float* tri_planes,
+                                                            const uint8_t* knn_flag, const float* knn_planes,
+                                                            int width, int height, hpm_camera ref, float factor,
+                                                            float depth_min, float depth_max, float* src_nd,
+                                                            float* planes, uint32_t* mask)
+{
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    if (x >= width || y >= height) {
+        return;
+    }
+    const size_t q = (size_t)y * width + x;
+    const uint32_t r = raster[q];
+    float4 plane = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    bool have = false, tri = false;
+    if (r > 0u) {
+        plane = reinterpret_cast<const float4*>(tri_planes)[r - 1u];
+        have = tri = true;
+    } else if (knn_flag[q]) {
+        plane = reinterpret_cast<const float4*>(knn_planes)[q];
+        have = true;
+    }
+    const float d = have ? depth_from_plane_factor(ref, plane, x, y, factor) : 0.0f;
+    if (src_nd) {
+        const bool ok = have && (tri ? (d <= depth_max * 1.2f && d >= depth_min * 0.6f)
+                                     : (d <= depth_max && d >= depth_min));
+        float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ok) {
+            o.x = ref.R[0] * plane.x + ref.R[3] * plane.y + ref.R[6] * plane.z;
+            o.y = ref.R[1] * plane.x + ref.R[4] * plane.y + ref.R[7] * plane.z;
+            o.z = ref.R[2] * plane.x + ref.R[5] * plane.y + ref.R[8] * plane.z;
+            o.w = d;
+        }
+        reinterpret_cast<float4*>(src_nd)[q] = o;
+    } else {
+        const bool ok = have && d <= depth_max && d >= depth_min;
+        reinterpret_cast<float4*>(planes)[q] = ok ? plane : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        mask[q] = ok ? 1u : 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void hpm_k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[i] = v;
+    }
+}
+
+extern "C" int hpm_launch_prior_maps(const float* hyp4, const float* cost, int W, int H, float* depth_s,
+                                     float* cost_s, int ws, int hs, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_prior_maps, dim3((unsigned)((ws + 63) / 64), (unsigned)((hs + 3) / 4), 1), blk, 0,
+                       (hipStream_t)stream, hyp4, cost, W, H, depth_s, cost_s, ws, hs);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_support_blocks(const float* cost, int w, int h, int32_t* cand, void* stream)
+{
+    const int nb = ((w + 4) / 5) * ((h + 4) / 5);
+    hipLaunchKernelGGL(hpm_k_support_blocks, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       cost, w, h, cand);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_raster(const int32_t* pts, const int32_t* tris, const int32_t* tri_idx, int ntris,
+                                 uint32_t* mask, int width, int height, void* stream)
+{
+    const int64_t n = (int64_t)width * height;
+    hipLaunchKernelGGL(hpm_k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, mask, n,
+                       0u);
+    if (ntris > 0) {
+        hipLaunchKernelGGL(hpm_k_raster, dim3((unsigned)((ntris + 63) / 64)), dim3(64), 0, (hipStream_t)stream, pts,
+                           tris, tri_idx, ntris, mask, width, height);
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_tri_planes(const int32_t* pts, const int32_t* tris, int ntris, const float* depth,
+                                     int width, const hpm_camera* ref, float factor, float* planes, void* stream)
+{
+    if (ntris > 0) {
+        hipLaunchKernelGGL(hpm_k_tri_planes, dim3((unsigned)((ntris + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)stream, pts, tris, ntris, depth, width, *ref, factor, planes);
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_knn(const int32_t* pts, int npoints, const int32_t* cell_start, const int32_t* cell_pts,
+                              int cell, int gw, int gh, const int32_t* order, int norder, int k,
+                              const uint32_t* raster, const float* depth, int width, int height,
+                              const hpm_camera* ref, float factor, float depth_min, float depth_max,
+                              uint8_t* knn_flag, float* knn_planes, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_knn, dim3((unsigned)((width + 63) / 64), (unsigned)((height + 3) / 4), 1), blk, 0,
+                       (hipStream_t)stream, pts, npoints, cell_start, cell_pts, cell, gw, gh, order, norder, k, raster,
+                       depth, width, height, *ref, factor, depth_min, depth_max, knn_flag, knn_planes);
+    return (int)hipGetLastError();
+}
+
+extern "C" int hpm_launch_prior_assemble(const uint32_t* raster, const float* tri_planes, const uint8_t* knn_flag,
+                                         const float* knn_planes, int width, int height, const hpm_camera* ref,
+                                         float factor, float depth_min, float depth_max, float* src_nd,
+                                         float* planes, uint32_t* mask, void* stream)
+{
+    const dim3 blk(64, 4, 1);
+    hipLaunchKernelGGL(hpm_k_prior_assemble, dim3((unsigned)((width + 63) / 64), (unsigned)((height + 3) / 4), 1),
+                       blk, 0, (hipStream_t)stream, raster, tri_planes, knn_flag, knn_planes, width, height, *ref,
+                       factor, depth_min, depth_max, src_nd, planes, mask);
+    return (int)hipGetLastError();
+}

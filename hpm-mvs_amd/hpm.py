@@ -142,6 +142,7 @@ class HpmPipelineOptions(C.Structure):
         ("learned_min_accept", C.c_float),
         ("hierarchy_from", C.c_void_p),
         ("stream_base", C.c_int32),
+        ("prior_on_device", C.c_int32),
     ]
 
 
@@ -206,6 +207,7 @@ SIGNATURES = {
     "hpm_set_geom_consistency": (C.c_int, [C.c_void_p, C.c_int]),
     "hpm_set_planar_prior": (C.c_int, [C.c_void_p]),
     "hpm_set_hierarchy_input": (C.c_int, [C.c_void_p, _FP, C.c_int, C.c_int]),
+    "hpm_build_planar_prior": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.POINTER(C.c_double)]),
     "hpm_input_init": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _IP, C.c_int]),
     "hpm_input_init_memory": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(HpmCamera),
                                         C.POINTER(_FP), C.POINTER(_FP)]),

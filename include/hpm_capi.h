@@ -142,6 +142,18 @@ int hpm_depth_range(const hpm_handle* h, float* depth_min, float* depth_max);
  * CudaPlanarPriorInitialization (HPM.cpp:702-722): per-pixel camera-frame
  * prior planes (n, d) and masks; planes are read only where mask > 0. */
 int hpm_upload_prior(hpm_handle* h, const float* planes4, const uint32_t* mask);
+/* One scale of the planar prior built on the device from the handle's last
+ * RunPatchMatch, as main.cpp builds it on the host (main.cpp:84-162 at
+ * factor 0.25, 238-314 at 0.5, 387-454 at 1; HPM.cpp:759-1123): support
+ * points of the (resized) cost map, their Delaunay triangulation (on the
+ * host), the rasterised triangle mask, the triangle planes, the KNN
+ * supplement with knn_k neighbours, then at factor 1 the range-gated planes
+ * and masks, below it their world normals and depths upsampled by JBU
+ * (JointBilateralUpsampling, HPM.cpp:1215-1281) into the prior. The prior
+ * is left on the handle with the planar prior enabled, equal bit for bit to
+ * the host construction. ms_host (may be NULL) accumulates the host part's
+ * milliseconds. */
+int hpm_build_planar_prior(hpm_handle* h, float factor, int knn_k, double* ms_host);
 /* HPM::ReleasePriorCudaMemory (HPM.cpp:1283-1286); idempotent. */
 int hpm_release_prior(hpm_handle* h);
 /* HPM::ReleaseProblemCudaMemory / ReleaseProblemHostMemory

@@ -49,6 +49,7 @@ typedef struct hpm_pipeline_options {
      * (hpm_set_hierarchy_input) instead of a plain pass */
     hpm_pool* hierarchy_from;
     int32_t stream_base;        /* added to every RunPatchMatch call's Philox stream (0) */
+    int32_t prior_on_device;    /* build the planar prior with hpm_build_planar_prior (1), or on the host (0) */
 } hpm_pipeline_options;
 
 typedef struct hpm_pipeline_stats {

@@ -69,6 +69,51 @@ def _prior_scale(ref_cam, depth, cost, f, k, dmin, dmax, threads):
     return mask, tri_planes, knn
 
 
+def prior_scale_maps(ref_cam, depth, cost, guide, f, k, dmin, dmax, threads=0):
+    """The planar prior of one scale from a finished pass's depth and cost
+    (main.cpp:84-162 at f = 0.25, 238-314 at 0.5, 387-454 at 1), built with
+    the library's host functions: (camera-frame planes (H, W, 4), masks
+    (H, W)) at full size."""
+    f32 = np.float32
+    H, W = depth.shape
+    if f < 1.0:
+        ws, hs = int(np.floor(W * f + 0.5)), int(np.floor(H * f + 0.5))
+        ds = _hpm.resize_bilinear(depth, ws, hs)
+        cs_ = _hpm.resize_bilinear(cost, ws, hs)
+        mask, tri_planes, knn = _prior_scale(ref_cam, ds, cs_, f, k, dmin, dmax, threads)
+        ys, xs = np.mgrid[0:hs, 0:ws]
+        src_nd = np.zeros((hs, ws, 4), np.float32)
+        tri = mask > 0
+        pl = np.zeros((hs, ws, 4), np.float32)
+        if len(tri_planes):
+            pl[tri] = tri_planes[mask[tri].astype(np.int64) - 1]
+        d_tri = _depth_from_plane_factor(ref_cam.K, pl, xs, ys, f)
+        ok_tri = tri & (d_tri <= dmax * f32(1.2)) & (d_tri >= dmin * f32(0.6))
+        knn_m = mask < -200
+        d_knn = _depth_from_plane_factor(ref_cam.K, knn, xs, ys, f)
+        ok_knn = knn_m & (d_knn <= dmax) & (d_knn >= dmin)
+        src_nd[ok_tri, 3] = d_tri[ok_tri]
+        src_nd[ok_knn, 3] = d_knn[ok_knn]
+        src_nd[ok_tri, :3] = _to_world(ref_cam.R, pl[ok_tri])
+        src_nd[ok_knn, :3] = _to_world(ref_cam.R, knn[ok_knn])
+        res = _jbu_prior(guide, src_nd, ref_cam, dmin, dmax, threads)
+        if res is None:
+            return np.zeros((H, W, 4), np.float32), np.zeros((H, W), np.uint32)
+        return res
+    mask, tri_planes, knn = _prior_scale(ref_cam, depth, cost, 1.0, k, dmin, dmax, threads)
+    ys, xs = np.mgrid[0:H, 0:W]
+    pl = np.zeros((H, W, 4), np.float32)
+    tri = mask > 0
+    if len(tri_planes):
+        pl[tri] = tri_planes[mask[tri].astype(np.int64) - 1]
+    knn_m = mask < -200
+    pl[knn_m] = knn[knn_m]
+    d = _depth_from_plane_factor(ref_cam.K, pl, xs, ys, 1.0)
+    act = (tri | knn_m) & (d <= dmax) & (d >= dmin)
+    planes = np.where(act[..., None], pl, f32(0)).astype(np.float32)
+    return planes, act.astype(np.uint32)
+
+
 def run_cpu_pipeline(cams, images, problems, opt, threads=0):
     """cams/images indexed by view; problems: [(ref, [sources])]; opt: an
     HpmPipelineOptions. Returns {view: (hyp_world (H, W, 4), cost (H, W))}
@@ -93,48 +138,9 @@ def run_cpu_pipeline(cams, images, problems, opt, threads=0):
                 m //= 2
                 k += 1
             for i in range(2 - k, 3):
-                depth = np.ascontiguousarray(o.hyp[..., 3])
-                normal = np.ascontiguousarray(o.hyp[..., :3])
-                cost = o.cost.copy()
-                if i in (0, 1):
-                    f = 0.25 if i == 0 else 0.5
-                    ws, hs = int(np.floor(W * f + 0.5)), int(np.floor(H * f + 0.5))
-                    ds = _hpm.resize_bilinear(depth, ws, hs)
-                    cs_ = _hpm.resize_bilinear(cost, ws, hs)
-                    mask, tri_planes, knn = _prior_scale(cs[0], ds, cs_, f, params.knn_k, dmin, dmax, threads)
-                    ys, xs = np.mgrid[0:hs, 0:ws]
-                    src_nd = np.zeros((hs, ws, 4), np.float32)
-                    tri = mask > 0
-                    pl = np.zeros((hs, ws, 4), np.float32)
-                    if len(tri_planes):
-                        pl[tri] = tri_planes[mask[tri].astype(np.int64) - 1]
-                    d_tri = _depth_from_plane_factor(cs[0].K, pl, xs, ys, f)
-                    ok_tri = tri & (d_tri <= dmax * f32(1.2)) & (d_tri >= dmin * f32(0.6))
-                    knn_m = mask < -200
-                    d_knn = _depth_from_plane_factor(cs[0].K, knn, xs, ys, f)
-                    ok_knn = knn_m & (d_knn <= dmax) & (d_knn >= dmin)
-                    src_nd[ok_tri, 3] = d_tri[ok_tri]
-                    src_nd[ok_knn, 3] = d_knn[ok_knn]
-                    src_nd[ok_tri, :3] = _to_world(cs[0].R, pl[ok_tri])
-                    src_nd[ok_knn, :3] = _to_world(cs[0].R, knn[ok_knn])
-                    res = _jbu_prior(ims[0], src_nd, cs[0], dmin, dmax, threads)
-                    if res is None:
-                        planes, pmask = np.zeros((H, W, 4), np.float32), np.zeros((H, W), np.uint32)
-                    else:
-                        planes, pmask = res
-                else:
-                    mask, tri_planes, knn = _prior_scale(cs[0], depth, cost, 1.0, params.knn_k, dmin, dmax, threads)
-                    ys, xs = np.mgrid[0:H, 0:W]
-                    pl = np.zeros((H, W, 4), np.float32)
-                    tri = mask > 0
-                    if len(tri_planes):
-                        pl[tri] = tri_planes[mask[tri].astype(np.int64) - 1]
-                    knn_m = mask < -200
-                    pl[knn_m] = knn[knn_m]
-                    d = _depth_from_plane_factor(cs[0].K, pl, xs, ys, 1.0)
-                    act = (tri | knn_m) & (d <= dmax) & (d >= dmin)
-                    planes = np.where(act[..., None], pl, f32(0)).astype(np.float32)
-                    pmask = act.astype(np.uint32)
+                f = (0.25, 0.5, 1.0)[i]
+                planes, pmask = prior_scale_maps(cs[0], np.ascontiguousarray(o.hyp[..., 3]), o.cost.copy(), ims[0], f,
+                                                 params.knn_k, dmin, dmax, threads)
                 o.set_prior(planes, pmask)
                 o.run_patchmatch()
         state[ref] = (o.hyp.copy(), o.cost.copy())

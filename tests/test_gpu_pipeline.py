@@ -142,3 +142,64 @@ def test_executable_on_jpeg_folder(hpm, tmp_path):
     say(f"accuracy (5%) per view: {accs}")
     assert min(accs) > 0.3
     assert (tmp_path / "HPM" / "HPM_model.ply").exists()
+
+
+@pytest.mark.parametrize("size", [(320, 240), (1600, 1066)])
+def test_device_prior_matches_host_construction(hpm, size):
+    """hpm_build_planar_prior (support points, rasterisation, triangle planes,
+    KNN supplement and assembly on the device; Delaunay on the host) against
+    the host construction of the same scale (oracle/pipeline.py:
+    prior_scale_maps, the library's host functions), at 0.5x (through the
+    JBU) and 1x, bit for bit."""
+    import ctypes as C
+
+    from oracle import pipeline
+    from test_gpu_parity import make_problem
+
+    w, h = size
+    f = 0.78 * w
+    cfg = hpm.scene_config(seed=31, width=w, height=h, fx=f, fy=f, cx=w / 2.0, cy=h / 2.0, num_views=4,
+                           num_planes=20, num_boxes=50, low_texture_fraction=0.3)
+    cams, imgs, gd, gn, low = hpm.render_scene(cfg, want_gt=False)
+    params = hpm.default_params()
+    params.knn_k = 8
+    pr = make_problem(hpm, cams, [im.astype(np.float32) for im in imgs], params)
+    pr.RunPatchMatch()
+    hyp, cost = pr.hypotheses(), pr.costs()
+    p = pr.params()
+    for factor in (0.5, 1.0):
+        say(f"case: device planar prior {w}x{h} at {factor}x")
+        host_ms = C.c_double(0.0)
+        rc = hpm.lib().hpm_build_planar_prior(pr._h, factor, 8, C.byref(host_ms))
+        assert rc == 0, hpm.lib().hpm_last_error(pr._h)
+        planes = np.empty((h, w, 4), np.float32)
+        mask = np.empty((h, w), np.uint32)
+        assert hpm.lib().hpm_debug_read_prior(pr._h, hpm.fptr(planes), hpm.uptr(mask)) == 0
+        want_p, want_m = pipeline.prior_scale_maps(cams[0], np.ascontiguousarray(hyp[..., 3]), cost,
+                                                   imgs[0].astype(np.float32), factor, 8, np.float32(p.depth_min),
+                                                   np.float32(p.depth_max))
+        same = np.all(planes.view(np.uint32) == want_p.view(np.uint32), axis=2) & (mask == want_m)
+        say(f"{factor}x: mask on {mask.mean():.4f}, identical pixels {same.mean():.6f}, host part "
+            f"{host_ms.value:.1f} ms")
+        assert mask.mean() > 0.3
+        assert same.all()
+    pr.close()
+
+
+def test_pipeline_device_prior_equals_host_prior(hpm):
+    """The pass schedule with the planar prior built on the device and on the
+    host: the same maps"""
+    cfg, cams, imgs, gd, problems = scene(hpm)
+    say("case: pipeline with the device prior and with the host prior, 4 views 320x240")
+    out = []
+    for dev in (1, 0):
+        opt = hpm.pipeline_options(scale_bound=200, prior_on_device=dev)
+        opt.params.geom_iterations = 1
+        pool = hpm.Pool(cams, imgs)
+        st = hpm.run_pipeline(pool, problems, opt)
+        say(f"prior_on_device={dev}: prior host {st.ms_prior_host:.1f} ms, device {st.ms_prior_device:.1f} ms")
+        out.append([pool.state(v) for v, _ in problems])
+        pool.close()
+    for (a_h, a_c), (b_h, b_c) in zip(*out):
+        assert np.array_equal(a_h.view(np.uint32), b_h.view(np.uint32))
+        assert np.array_equal(a_c.view(np.uint32), b_c.view(np.uint32))

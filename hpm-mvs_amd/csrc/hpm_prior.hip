@@ -147,12 +147,16 @@ __global__ __launch_bounds__(256) void hpm_k_support_blocks(const float* cost, i
 }
 
 /* the triangle rasterisation of main.cpp:96-116 (hpm_rasterize_triangles):
- * one lane per triangle; later triangles overwrite earlier ones there, so a
- * pixel ends with the largest index + 1 that reaches it (atomicMax) */
+ * one wave per triangle, lane l taking the outer steps l, l + 64, ... (the
+ * steps are the float sums p += step, advanced 64 at a time per lane, so
+ * every lane sees the values of the sequential loop); later triangles
+ * overwrite earlier ones there, so a pixel ends with the largest index + 1
+ * that reaches it (atomicMax) */
 __global__ __launch_bounds__(256) void hpm_k_raster(const int32_t* pts, const int32_t* tris, const int32_t* tri_idx,
                                                     int ntris, uint32_t* mask, int width, int height)
 {
-    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int t = (int)(blockIdx.x * 4 + threadIdx.x / 64);
+    const int lane = (int)(threadIdx.x % 64);
     if (t >= ntris || tri_idx[t] < 0) {
         return;
     }
@@ -168,13 +172,22 @@ __global__ __launch_bounds__(256) void hpm_k_raster(const int32_t* pts, const in
     const float L12 = (float)sqrt(d12x * d12x + d12y * d12y);
     const float max_edge = fmaxf(L01, fmaxf(L02, L12));
     const float step = (float)(1.0 / max_edge);
-    for (float p = 0; p < 1.0; p += step) {
+    float p = 0;
+    for (int k = 0; k < lane; ++k) {
+        p += step;
+    }
+    while (p < 1.0) {
         for (float q = 0; (double)q < 1.0 - (double)p; q += step) {
             const int x = (int)((double)(p * (float)x1 + q * (float)x2) + (1.0 - (double)p - (double)q) * x3);
             const int y = (int)((double)(p * (float)y1 + q * (float)y2) + (1.0 - (double)p - (double)q) * y3);
             if (x >= 0 && x < width && y >= 0 && y < height) {
                 atomicMax(&mask[(size_t)y * width + x], val);
             }
+        }
+        /* the sums only grow: past 1 after 64 more steps iff past 1 at the
+         * end of them */
+        for (int k = 0; k < 64; ++k) {
+            p += step;
         }
     }
 }
@@ -368,7 +381,7 @@ extern "C" int hpm_launch_raster(const int32_t* pts, const int32_t* tris, const 
     hipLaunchKernelGGL(hpm_k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, mask, n,
                        0u);
     if (ntris > 0) {
-        hipLaunchKernelGGL(hpm_k_raster, dim3((unsigned)((ntris + 63) / 64)), dim3(64), 0, (hipStream_t)stream, pts,
+        hipLaunchKernelGGL(hpm_k_raster, dim3((unsigned)((ntris + 3) / 4)), dim3(256), 0, (hipStream_t)stream, pts,
                            tris, tri_idx, ntris, mask, width, height);
     }
     return (int)hipGetLastError();

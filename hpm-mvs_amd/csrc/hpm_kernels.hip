@@ -167,17 +167,32 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
  * The three give the same values per lane. */
 enum : int { kPathInner = 0, kPathSmall = 1, kPathAny = 2 };
 
-/* Source coordinate of tap (dx, dy): byte offset of the top-left texel of
- * its bilinear footprint in the padded pool and the bilinear fractions. */
-template <int kPath>
-__device__ __forceinline__ void tap_coord(const Warp& w, float pitchf, float dx, float dy, uint32_t* off, float* fa,
-                                          float* fb)
+/* The dx products of one window column (x offset dx), shared by its taps. */
+struct Column {
+    float ex, gx, gy; /* ex dx, gxx dx, gyx dx */
+};
+
+__device__ __forceinline__ Column column(const Warp& w, float dx)
 {
-    const float u = fmaf(w.ex, dx, w.ey * dy);
+    Column c;
+    c.ex = w.ex * dx;
+    c.gx = w.gxx * dx;
+    c.gy = w.gyx * dx;
+    return c;
+}
+
+/* Source coordinate of tap (dx, dy) of column c: byte offset of the
+ * top-left texel of its bilinear footprint in the padded pool and the
+ * bilinear fractions. */
+template <int kPath>
+__device__ __forceinline__ void tap_coord(const Warp& w, const Column& c, float pitchf, float dy, uint32_t* off,
+                                          float* fa, float* fb)
+{
+    const float u = fmaf(w.ey, dy, c.ex);
     const float cubic = fmaf(u, fmaf(u, 1.0f - u, -1.0f), 1.0f);
     const float inv = (kPath != kPathAny || w.small) ? cubic : 1.0f / (1.0f + u);
-    float lx = fmaf(fmaf(w.gxx, dx, w.gxy * dy), inv, w.fx0);
-    float ly = fmaf(fmaf(w.gyx, dx, w.gyy * dy), inv, w.fy0);
+    float lx = fmaf(fmaf(w.gxy, dy, c.gx), inv, w.fx0);
+    float ly = fmaf(fmaf(w.gyy, dy, c.gy), inv, w.fy0);
     if (kPath != kPathInner) {
         lx = fminf(fmaxf(lx, w.lox), w.hix);
         ly = fminf(fmaxf(ly, w.loy), w.hiy);
@@ -288,9 +303,10 @@ __device__ __forceinline__ void fetch_column(__amdgpu_buffer_rsrc_t pool, uint32
                                              Footprint* quad, float* fa, float* fb)
 {
     uint32_t off[6];
+    const Column c = column(w, dx);
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        tap_coord<kPath>(w, (float)pitch, dx, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
+        tap_coord<kPath>(w, c, (float)pitch, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
     }
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
@@ -330,7 +346,7 @@ __device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, uint32_t p
      * offset (-1, -1)), sampled first: a constant patch sums to exactly 0 */
     uint32_t coff;
     float cfa, cfb;
-    tap_coord<kPath>(w, (float)pitch, -1.0f, -1.0f, &coff, &cfa, &cfb);
+    tap_coord<kPath>(w, column(w, -1.0f), (float)pitch, -1.0f, &coff, &cfa, &cfb);
     const Footprint cq = footprint(pool, coff);
     fetch_column<kPath>(pool, pitch, w, -5.0f, cur, fa, fb);
     const float shift = bilerp(cq, cfa, cfb, one);

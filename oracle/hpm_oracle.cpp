@@ -229,10 +229,11 @@ static inline float tap_inv(float u, bool small)
  * reference samples at +0.5). */
 static inline float sample_tap(const float* img, int W, int H, const Warp& w, float dx, float dy)
 {
-    const float u = fmaf(w.ex, dx, w.ey * dy);
+    /* the dx products are per window column, the dy terms fused per tap */
+    const float u = fmaf(w.ey, dy, w.ex * dx);
     const float inv = tap_inv(u, w.small);
-    float lx = fmaf(fmaf(w.gxx, dx, w.gxy * dy), inv, w.fx0);
-    float ly = fmaf(fmaf(w.gyx, dx, w.gyy * dy), inv, w.fy0);
+    float lx = fmaf(fmaf(w.gxy, dy, w.gxx * dx), inv, w.fx0);
+    float ly = fmaf(fmaf(w.gyy, dy, w.gyx * dx), inv, w.fy0);
     lx = fminf(fmaxf(lx, -1.0f - (float)w.ix0), (float)(W - w.ix0));
     ly = fminf(fmaxf(ly, -1.0f - (float)w.iy0), (float)(H - w.iy0));
     const float flx = floorf(lx);

@@ -59,10 +59,15 @@ __global__ __launch_bounds__(256) void hpm_k_conv_fwd(const hpm_bf16* __restrict
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * 32;
+    /* wave g of the row takes output-channel tile g mod (co / 32) of pixel
+     * tile g div (co / 32): the four waves of a block all work, and those
+     * on one pixel tile share its input loads */
+    const int co_tiles = co / 32;
+    const int g = (int)blockIdx.x * 4 + wave;
+    const int x0 = (g / co_tiles) * 32;
     const int y = blockIdx.y;
-    const int n0 = (blockIdx.z * 4 + wave) * 32;
-    if (n0 >= co) {
+    const int n0 = (g % co_tiles) * 32;
+    if (x0 >= width) {
         return;
     }
     const int r = lane & 31;
@@ -230,7 +235,8 @@ extern "C" int hpm_unet_conv_fwd(const void* in, int ldi, int ci, const void* w,
     if (ci % 16 != 0 || co % 32 != 0 || (ks != 1 && ks != 3) || ldi % 8 != 0 || height < 1 || width < 1) {
         return (int)hipErrorInvalidValue;
     }
-    const dim3 grid((width + 31) / 32, height, (co / 32 + 3) / 4);
+    const int items = ((width + 31) / 32) * (co / 32);
+    const dim3 grid((unsigned)((items + 3) / 4), (unsigned)height, 1);
     hipLaunchKernelGGL(hpm_k_conv_fwd, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)in, ldi, ci,
                        (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, ks, relu);
     return (int)hipGetLastError();

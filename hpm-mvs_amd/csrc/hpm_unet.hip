@@ -50,8 +50,13 @@ __global__ __launch_bounds__(64) void hpm_k_mfma_probe(const bf16x8* a, const bf
 }
 
 /* out[p][co] = act(bias[co] + sum_{tap, ci} in[p + tap][ci] w[co][tap][ci]),
- * zero padding k/2. One wave: 32 consecutive pixels of a row x 32 output
- * channels; a block: 4 waves over 4 output-channel tiles. */
+ * zero padding k/2. One wave: MT tiles of 32 consecutive pixels of a row x
+ * NT tiles of 32 output channels, each input fragment feeding NT products
+ * and each weight fragment MT (every output takes the same sequence of
+ * products, whatever the tiling); a block: 4 waves on consecutive (pixel
+ * group, channel group) items, those on one pixel group sharing its input
+ * loads. */
+template <int MT, int NT>
 __global__ __launch_bounds__(256) void hpm_k_conv_fwd(const hpm_bf16* __restrict__ in, int ldi, int ci,
                                                       const hpm_bf16* __restrict__ w, const float* __restrict__ bias,
                                                       float* out32, hpm_bf16* out16, int ldo, int co, int height,
@@ -59,54 +64,83 @@ __global__ __launch_bounds__(256) void hpm_k_conv_fwd(const hpm_bf16* __restrict
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    /* wave g of the row takes output-channel tile g mod (co / 32) of pixel
-     * tile g div (co / 32): the four waves of a block all work, and those
-     * on one pixel tile share its input loads */
-    const int co_tiles = co / 32;
+    const int groups_n = co / (32 * NT);
     const int g = (int)blockIdx.x * 4 + wave;
-    const int x0 = (g / co_tiles) * 32;
+    const int x0 = (g / groups_n) * 32 * MT;
     const int y = blockIdx.y;
-    const int n0 = (g % co_tiles) * 32;
+    const int n0 = (g % groups_n) * 32 * NT;
     if (x0 >= width) {
         return;
     }
     const int r = lane & 31;
     const int kh = lane >> 5;
-    const int px = x0 + r;
     const int taps = ks * ks;
     const int half = ks / 2;
-    f32x16 acc = {};
+    f32x16 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            acc[m][n] = f32x16{};
+        }
+    }
     const bf16x8 zero = {};
     for (int tap = 0; tap < taps; ++tap) {
         const int sy = y + tap / ks - half;
-        const int sx = px + tap % ks - half;
-        const bool valid = px < width && sy >= 0 && sy < height && sx >= 0 && sx < width;
-        const hpm_bf16* arow = in + ((int64_t)(valid ? sy : 0) * width + (valid ? sx : 0)) * ldi + kh * 8;
+        const hpm_bf16* arow[MT];
+        bool valid[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int px = x0 + 32 * m + r;
+            const int sx = px + tap % ks - half;
+            valid[m] = px < width && sy >= 0 && sy < height && sx >= 0 && sx < width;
+            arow[m] = in + ((int64_t)(valid[m] ? sy : 0) * width + (valid[m] ? sx : 0)) * ldi + kh * 8;
+        }
         const hpm_bf16* brow = w + ((int64_t)(n0 + r) * taps + tap) * ci + kh * 8;
+        const int64_t bstride = (int64_t)32 * taps * ci; /* 32 output channels */
         for (int c0 = 0; c0 < ci; c0 += 16) {
-            const bf16x8 a = valid ? ld8(arow + c0) : zero;
-            const bf16x8 b = ld8(brow + c0);
-            acc = mfma16(a, b, acc);
+            bf16x8 a[MT], b[NT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                a[m] = valid[m] ? ld8(arow[m] + c0) : zero;
+            }
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                b[n] = ld8(brow + n * bstride + c0);
+            }
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+#pragma unroll
+                for (int n = 0; n < NT; ++n) {
+                    acc[m][n] = mfma16(a[m], b[n], acc[m][n]);
+                }
+            }
         }
     }
-    const int n = n0 + r;
-    const float bn = bias ? bias[n] : 0.0f;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int n = 0; n < NT; ++n) {
+        const int ch = n0 + 32 * n + r;
+        const float bn = bias ? bias[ch] : 0.0f;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int x = x0 + 8 * j + 4 * kh + i;
-            if (x < width) {
-                float v = acc[4 * j + i] + bn;
-                if (relu) {
-                    v = fmaxf(v, 0.0f);
-                }
-                const int64_t o = ((int64_t)y * width + x) * ldo + n;
-                if (out32) {
-                    out32[o] = v;
-                }
-                if (out16) {
-                    out16[o] = (hpm_bf16)v;
+        for (int m = 0; m < MT; ++m) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int x = x0 + 32 * m + 8 * j + 4 * kh + i;
+                    if (x < width) {
+                        float v = acc[m][n][4 * j + i] + bn;
+                        if (relu) {
+                            v = fmaxf(v, 0.0f);
+                        }
+                        const int64_t o = ((int64_t)y * width + x) * ldo + ch;
+                        if (out32) {
+                            out32[o] = v;
+                        }
+                        if (out16) {
+                            out16[o] = (hpm_bf16)v;
+                        }
+                    }
                 }
             }
         }
@@ -235,10 +269,23 @@ extern "C" int hpm_unet_conv_fwd(const void* in, int ldi, int ci, const void* w,
     if (ci % 16 != 0 || co % 32 != 0 || (ks != 1 && ks != 3) || ldi % 8 != 0 || height < 1 || width < 1) {
         return (int)hipErrorInvalidValue;
     }
-    const int items = ((width + 31) / 32) * (co / 32);
-    const dim3 grid((unsigned)((items + 3) / 4), (unsigned)height, 1);
-    hipLaunchKernelGGL(hpm_k_conv_fwd, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)in, ldi, ci,
-                       (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, ks, relu);
+    /* tiles per wave: two pixel tiles at 32 output channels, else two or
+     * three channel tiles */
+    auto launch = [&](auto kernel, int mt, int nt) {
+        const int items = ((width + 32 * mt - 1) / (32 * mt)) * (co / (32 * nt));
+        const dim3 grid((unsigned)((items + 3) / 4), (unsigned)height, 1);
+        hipLaunchKernelGGL(kernel, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)in, ldi, ci,
+                           (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, ks, relu);
+    };
+    if (co == 32) {
+        launch(hpm_k_conv_fwd<2, 1>, 2, 1);
+    } else if (co % 64 == 0) {
+        launch(hpm_k_conv_fwd<1, 2>, 1, 2);
+    } else if (co % 96 == 0) {
+        launch(hpm_k_conv_fwd<1, 3>, 1, 3);
+    } else {
+        launch(hpm_k_conv_fwd<1, 1>, 1, 1);
+    }
     return (int)hipGetLastError();
 }
 

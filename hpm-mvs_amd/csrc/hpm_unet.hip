@@ -147,6 +147,103 @@ __global__ __launch_bounds__(256) void hpm_k_conv_fwd(const hpm_bf16* __restrict
     }
 }
 
+/* The 3x3 convolution with its operands staged in LDS: a block computes
+ * one output row, 128 pixels (32 per wave) x 32 NT output channels; for
+ * every chunk of 32 input channels the three input rows with their 1-pixel
+ * halo (130 x 32, zero outside the image) and the weight slice (32 NT x 9
+ * taps x 32) are loaded with coalesced 16-byte loads into LDS (rows padded
+ * to 40 channels against bank conflicts), then every wave runs the
+ * chunk's 9 taps x 2 MFMA steps per channel tile from LDS. Products are
+ * accumulated chunk by chunk, tap by tap. */
+constexpr int kLdsPad = 40;
+template <int NT>
+__global__ __launch_bounds__(256) void hpm_k_conv3_lds(const hpm_bf16* __restrict__ in, int ldi, int ci,
+                                                       const hpm_bf16* __restrict__ w, const float* __restrict__ bias,
+                                                       float* out32, hpm_bf16* out16, int ldo, int co, int height,
+                                                       int width, int relu)
+{
+    __shared__ __align__(16) hpm_bf16 s_in[3 * 130 * kLdsPad];
+    __shared__ __align__(16) hpm_bf16 s_w[32 * NT * 9 * kLdsPad];
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int x_blk = (int)blockIdx.x * 128;
+    const int n0 = (int)blockIdx.z * 32 * NT;
+    const int y = (int)blockIdx.y;
+    const int r = lane & 31;
+    const int kh = lane >> 5;
+    f32x16 acc[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        acc[n] = f32x16{};
+    }
+    const bf16x8 zero = {};
+    for (int c0 = 0; c0 < ci; c0 += 32) {
+        /* input rows y-1..y+1, columns x_blk-1 .. x_blk+128: 3 x 130 pixels
+         * x 4 vectors of 8 channels */
+        for (int e = tid; e < 3 * 130 * 4; e += 256) {
+            const int v = e & 3, pix = e >> 2;
+            const int row = pix / 130, col = pix % 130;
+            const int sy = y + row - 1, sx = x_blk + col - 1;
+            bf16x8 val = zero;
+            if (sy >= 0 && sy < height && sx >= 0 && sx < width) {
+                val = ld8(in + ((int64_t)sy * width + sx) * ldi + c0 + 8 * v);
+            }
+            *reinterpret_cast<bf16x8*>(&s_in[(row * 130 + col) * kLdsPad + 8 * v]) = val;
+        }
+        /* weights of the block's channels: 32 NT x 9 taps x 4 vectors */
+        for (int e = tid; e < 32 * NT * 9 * 4; e += 256) {
+            const int v = e & 3, rt = e >> 2;
+            const int nl = rt / 9, tap = rt % 9;
+            const bf16x8 val = ld8(w + ((int64_t)(n0 + nl) * 9 + tap) * ci + c0 + 8 * v);
+            *reinterpret_cast<bf16x8*>(&s_w[(nl * 9 + tap) * kLdsPad + 8 * v]) = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+#pragma unroll
+            for (int k16 = 0; k16 < 2; ++k16) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(
+                    &s_in[(ky * 130 + wave * 32 + r + kx) * kLdsPad + 16 * k16 + 8 * kh]);
+#pragma unroll
+                for (int n = 0; n < NT; ++n) {
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(
+                        &s_w[((32 * n + r) * 9 + tap) * kLdsPad + 16 * k16 + 8 * kh]);
+                    acc[n] = mfma16(a, b, acc[n]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const int x0 = x_blk + wave * 32;
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int ch = n0 + 32 * n + r;
+        const float bn = bias ? bias[ch] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int x = x0 + 8 * j + 4 * kh + i;
+                if (x < width) {
+                    float v = acc[n][4 * j + i] + bn;
+                    if (relu) {
+                        v = fmaxf(v, 0.0f);
+                    }
+                    const int64_t o = ((int64_t)y * width + x) * ldo + ch;
+                    if (out32) {
+                        out32[o] = v;
+                    }
+                    if (out16) {
+                        out16[o] = (hpm_bf16)v;
+                    }
+                }
+            }
+        }
+    }
+}
+
 /* dw[co][tap][ci] += sum_p dy[co][p] x[p + tap][ci] over one chunk of
  * pixels, FP32 atomics into dw. dy is channel-major ([co][H*W], BF16), x
  * pixel-major with stride ldx. One wave: a 32x32 (co, ci) tile of one tap;
@@ -277,7 +374,19 @@ extern "C" int hpm_unet_conv_fwd(const void* in, int ldi, int ci, const void* w,
         hipLaunchKernelGGL(kernel, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)in, ldi, ci,
                            (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, ks, relu);
     };
-    if (co == 32) {
+    if (ks == 3 && ci >= 64 && ci % 32 == 0 && (co == 32 || co == 64)) {
+        /* operands staged in LDS, all output channels in one block: pays
+         * where a block runs two or more input-channel chunks (with one, the
+         * loads are not hidden) and its LDS leaves two blocks per CU */
+        const dim3 grid((unsigned)((width + 127) / 128), (unsigned)height, 1);
+        if (co == 32) {
+            hipLaunchKernelGGL(hpm_k_conv3_lds<1>, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)in, ldi,
+                               ci, (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, relu);
+        } else {
+            hipLaunchKernelGGL(hpm_k_conv3_lds<2>, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)in, ldi,
+                               ci, (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, relu);
+        }
+    } else if (co == 32) {
         launch(hpm_k_conv_fwd<2, 1>, 2, 1);
     } else if (co % 64 == 0) {
         launch(hpm_k_conv_fwd<1, 2>, 1, 2);

@@ -203,3 +203,44 @@ def test_pipeline_device_prior_equals_host_prior(hpm):
     for (a_h, a_c), (b_h, b_c) in zip(*out):
         assert np.array_equal(a_h.view(np.uint32), b_h.view(np.uint32))
         assert np.array_equal(a_c.view(np.uint32), b_c.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["textureless", "tiny"])
+def test_device_prior_degenerate(hpm, case):
+    """The device planar prior where the host construction degenerates: a
+    textureless reference (every cost 2, no support point, no triangle, no
+    KNN plane) and a 40x30 view (a handful of support points): both
+    constructions agree, also on an empty prior."""
+    import ctypes as C
+
+    from oracle import pipeline
+    from test_gpu_parity import make_problem
+
+    w, h = (160, 120) if case == "textureless" else (40, 30)
+    cfg = hpm.scene_config(seed=33, width=w, height=h, fx=0.78 * w, fy=0.78 * w, cx=w / 2.0, cy=h / 2.0,
+                           num_views=3, num_planes=4, num_boxes=2)
+    cams, imgs, gd, gn, low = hpm.render_scene(cfg, want_gt=False)
+    imgs = [im.astype(np.float32) for im in imgs]
+    if case == "textureless":
+        imgs = [np.full_like(im, 128.0) for im in imgs]
+    params = hpm.default_params()
+    params.knn_k = 6
+    pr = make_problem(hpm, cams, imgs, params)
+    pr.RunPatchMatch()
+    hyp, cost = pr.hypotheses(), pr.costs()
+    p = pr.params()
+    for factor in (0.5, 1.0):
+        say(f"case: device planar prior, {case} {w}x{h}, {factor}x")
+        rc = hpm.lib().hpm_build_planar_prior(pr._h, factor, 6, None)
+        assert rc == 0, hpm.lib().hpm_last_error(pr._h)
+        planes = np.empty((h, w, 4), np.float32)
+        mask = np.empty((h, w), np.uint32)
+        assert hpm.lib().hpm_debug_read_prior(pr._h, hpm.fptr(planes), hpm.uptr(mask)) == 0
+        want_p, want_m = pipeline.prior_scale_maps(cams[0], np.ascontiguousarray(hyp[..., 3]), cost, imgs[0], factor, 6,
+                                                   np.float32(p.depth_min), np.float32(p.depth_max))
+        say(f"mask on {mask.mean():.4f} (host {want_m.mean():.4f})")
+        assert np.array_equal(mask, want_m)
+        assert np.array_equal(planes.view(np.uint32), want_p.view(np.uint32))
+        if case == "textureless":
+            assert mask.sum() == 0
+    pr.close()

@@ -361,4 +361,194 @@ void HPM::ReleaseProblemCudaMemory()
 
 void HPM::ReleaseProblemHostMemory() {}
 
+/* ------------------------------------------------------------------ ACMM */
+
+namespace {
+
+std::string acmm_dir(const std::string& dense_folder, int id)
+{
+    char buf[64];
+    snprintf(buf, sizeof(buf), "/ACMM_NESP/2333_%08d", id);
+    return dense_folder + buf;
+}
+
+/* images/%08d.jpg (or .pgm) as grey and cams/%08d_cam.txt */
+void read_view(const std::string& dense_folder, int id, MatF& img, hpm_camera& cam)
+{
+    char path[4096];
+    int w = 0, h = 0;
+    snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder.c_str(), id);
+    if (hpm_read_image_grey(path, nullptr, &w, &h) != 0) {
+        snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder.c_str(), id);
+        if (hpm_read_image_grey(path, nullptr, &w, &h) != 0) {
+            throw Error(std::string("cannot read image ") + path);
+        }
+    }
+    img = MatF(h, w);
+    if (hpm_read_image_grey(path, img.data.data(), &w, &h) != 0) {
+        throw Error(std::string("cannot read image ") + path);
+    }
+    snprintf(path, sizeof(path), "%s/cams/%08d_cam.txt", dense_folder.c_str(), id);
+    cam = ReadCamera(path);
+    cam.width = w;
+    cam.height = h;
+}
+
+} // namespace
+
+ACMM::ACMM() : h_(hpm_create(0))
+{
+    if (!h_) {
+        throw Error("hpm_create failed: no HIP device");
+    }
+    hpm_params_default(&params_);
+    params_.acmm = 1;
+    memset(&ref_cam_, 0, sizeof(ref_cam_));
+}
+
+ACMM::~ACMM()
+{
+    hpm_destroy(h_);
+}
+
+void ACMM::check(int rc, const char* what)
+{
+    if (rc < 0) {
+        throw Error(std::string(what) + ": " + hpm_last_error(h_));
+    }
+}
+
+/* ACMM.cpp:434-441 */
+void ACMM::SetGeomConsistencyParams(bool multi_geometry)
+{
+    params_.geom_consistency = 1;
+    params_.max_iterations = 2;
+    if (multi_geometry) {
+        params_.multi_geometry = 1;
+    }
+}
+
+/* ACMM.cpp:443-446 */
+void ACMM::SetHierarchyParams()
+{
+    hierarchy_ = true;
+}
+
+/* ACMM.cpp:448-553: images scaled to the level's size of their own
+ * problem (problems are indexed by image id), cameras with them; with
+ * geometric consistency the depth maps of every image */
+void ACMM::InuputInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, const int idx)
+{
+    const Problem& problem = problems[(size_t)idx];
+    std::vector<int> ids = {problem.ref_image_id};
+    ids.insert(ids.end(), problem.src_image_ids.begin(), problem.src_image_ids.end());
+    const size_t n = ids.size();
+    std::vector<MatF> imgs(n);
+    std::vector<hpm_camera> cams(n);
+    std::vector<MatF> depths;
+    for (size_t i = 0; i < n; ++i) {
+        read_view(dense_folder, ids[i], imgs[i], cams[i]);
+        /* ACMM.cpp:487-491 indexes problems by the source's image id */
+        if (ids[i] < 0 || (size_t)ids[i] >= problems.size()) {
+            throw Error("image id outside the problem list");
+        }
+        const int cur = i == 0 ? problems[(size_t)idx].cur_image_size : problems[(size_t)ids[i]].cur_image_size;
+        const int W = imgs[i].cols, H = imgs[i].rows;
+        if (W <= cur && H <= cur) {
+            continue;
+        }
+        const float f = std::min((float)cur / W, (float)cur / H);
+        const int nw = (int)std::round(W * f), nh = (int)std::round(H * f);
+        MatF o(nh, nw);
+        hpm_resize_bilinear(imgs[i].data.data(), W, H, o.data.data(), nw, nh);
+        imgs[i] = o;
+        const float sx = nw / (float)W, sy = nh / (float)H;
+        cams[i].K[0] *= sx;
+        cams[i].K[2] *= sx;
+        cams[i].K[4] *= sy;
+        cams[i].K[5] *= sy;
+        cams[i].width = nw;
+        cams[i].height = nh;
+    }
+    if (params_.geom_consistency) {
+        depths.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            const std::string p = acmm_dir(dense_folder, ids[i]) +
+                                  (params_.multi_geometry ? "/depths_geom.dmb" : "/depths.dmb");
+            if (readDepthDmb(p, depths[i]) != 0) {
+                throw Error("cannot read " + p);
+            }
+        }
+    }
+    std::vector<const float*> ip(n), dp(n);
+    for (size_t i = 0; i < n; ++i) {
+        ip[i] = imgs[i].data.data();
+        dp[i] = depths.empty() ? nullptr : depths[i].data.data();
+    }
+    check(hpm_set_params(h_, &params_), "InuputInitialization");
+    check(hpm_input_init_memory(h_, problem.ref_image_id, (int)n, cams.data(), ip.data(),
+                                depths.empty() ? nullptr : dp.data()),
+          "InuputInitialization");
+    check(hpm_get_camera(h_, 0, &ref_cam_), "InuputInitialization");
+}
+
+/* ACMM.cpp:555-719: device buffers; with geometric consistency the view's
+ * own depth (depths.dmb or depths_geom.dmb), normals.dmb and costs.dmb as
+ * its starting state; with the hierarchy the coarser level's maps */
+void ACMM::CudaSpaceInitialization(const std::string& dense_folder, const Problem& problem)
+{
+    check(hpm_device_init(h_, params_.geom_consistency ? dense_folder.c_str() : nullptr), "CudaSpaceInitialization");
+    const std::string dir = acmm_dir(dense_folder, problem.ref_image_id);
+    if (hierarchy_) {
+        MatF depth;
+        MatV3 normal;
+        if (readDepthDmb(dir + "/depths_geom.dmb", depth) != 0 || readNormalDmb(dir + "/normals.dmb", normal) != 0 ||
+            depth.cols != normal.cols || depth.rows != normal.rows) {
+            throw Error("cannot read the coarser maps of " + dir);
+        }
+        std::vector<float> c4(depth.data.size() * 4);
+        for (size_t k = 0; k < depth.data.size(); ++k) {
+            c4[4 * k] = normal.data[3 * k];
+            c4[4 * k + 1] = normal.data[3 * k + 1];
+            c4[4 * k + 2] = normal.data[3 * k + 2];
+            c4[4 * k + 3] = depth.data[k];
+        }
+        check(hpm_set_hierarchy_input(h_, c4.data(), depth.cols, depth.rows), "CudaSpaceInitialization");
+    }
+}
+
+void ACMM::RunPatchMatch()
+{
+    check(hpm_run_patchmatch(h_), "RunPatchMatch");
+}
+
+int ACMM::GetReferenceImageWidth()
+{
+    return ref_cam_.width;
+}
+
+int ACMM::GetReferenceImageHeight()
+{
+    return ref_cam_.height;
+}
+
+MatF ACMM::GetReferenceImage()
+{
+    MatF m(ref_cam_.height, ref_cam_.width);
+    check(hpm_copy_reference_image(h_, m.data.data()), "GetReferenceImage");
+    return m;
+}
+
+float4 ACMM::GetPlaneHypothesis(const int index)
+{
+    float v[4];
+    check(hpm_get_plane(h_, index, v), "GetPlaneHypothesis");
+    return float4{v[0], v[1], v[2], v[3]};
+}
+
+float ACMM::GetCost(const int index)
+{
+    return hpm_get_cost(h_, index);
+}
+
 } // namespace hpm

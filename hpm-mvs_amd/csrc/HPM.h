@@ -65,10 +65,14 @@ struct PointCloud {
 };
 typedef PointCloud* PointCloudPtr;
 
-/* `Problem` (main.h:54-57) */
+/* `Problem` (main.h:54-57); the last three fields are ACMM+NESP's
+ * (ACMM+NESP/windows/main.h:46-52), used by hpm::ACMM */
 struct Problem {
     int ref_image_id = 0;
     std::vector<int> src_image_ids;
+    int max_image_size = 3200;
+    int num_downscale = 0;
+    int cur_image_size = 3200;
 };
 
 struct Error : std::runtime_error {
@@ -133,6 +137,42 @@ private:
     hpm_handle* h_;
     hpm_params params_;
     hpm_camera ref_cam_;
+};
+
+/* `class ACMM` of ACMM+NESP (ACMM+NESP/windows/ACMM.h:47-62) over a handle
+ * with the ACMM kernel variant (hpm_params.acmm): the same method names and
+ * call order as the reference's ProcessProblem (ACMM+NESP/windows/
+ * main.cpp:74-129). Results go to and come from dense_folder/ACMM_NESP.
+ * For the hierarchy the coarser level's normals.dmb and depths_geom.dmb are
+ * read and upsampled inside RunPatchMatch (the reference reads the JBU
+ * output RunJBU wrote to depths.dmb; the two are the same upsampling). */
+class ACMM {
+public:
+    ACMM();
+    ~ACMM();
+    ACMM(const ACMM&) = delete;
+    ACMM& operator=(const ACMM&) = delete;
+
+    void InuputInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, const int idx);
+    void CudaSpaceInitialization(const std::string& dense_folder, const Problem& problem);
+    void RunPatchMatch();
+    void SetGeomConsistencyParams(bool multi_geometry = false);
+    void SetHierarchyParams();
+    int GetReferenceImageWidth();
+    int GetReferenceImageHeight();
+    MatF GetReferenceImage();
+    float4 GetPlaneHypothesis(const int index);
+    float GetCost(const int index);
+
+    hpm_params& params() { return params_; }
+    hpm_handle* handle() { return h_; }
+
+private:
+    void check(int rc, const char* what);
+    hpm_handle* h_;
+    hpm_params params_;
+    hpm_camera ref_cam_;
+    bool hierarchy_ = false;
 };
 
 } // namespace hpm

@@ -349,9 +349,13 @@ extern "C" int hpm_write_ply(const char* path, const hpm_point* points, int64_t 
     return ok ? 0 : -1;
 }
 
-extern "C" int hpm_fusion_run_folder(const char* dense_folder, int geom_consistency, int64_t* num_points, char* err,
-                                     int err_len)
+namespace {
+
+/* RunFusion as main calls it, over the maps of <dense_folder>/<sub> */
+int fusion_folder(bool acmm, const char* dense_folder, int geom_consistency, int64_t* num_points, char* err,
+                  int err_len)
 {
+    const char* sub = acmm ? "ACMM_NESP" : "HPM";
     if (!dense_folder) {
         return fail(err, err_len, "fusion: null folder");
     }
@@ -404,11 +408,11 @@ extern "C" int hpm_fusion_run_folder(const char* dense_folder, int geom_consiste
         const size_t n = (size_t)pc.width * pc.height;
         std::vector<float> d(n), nrm(n * 3), h4(n * 4);
         int w = 0, h = 0, ch = 0;
-        snprintf(path, sizeof(path), "%s/HPM/2333_%08d/%s", dense_folder, i,
+        snprintf(path, sizeof(path), "%s/%s/2333_%08d/%s", dense_folder, sub, i,
                  geom_consistency ? "depths_geom.dmb" : "depths.dmb");
         bool ok = hpm_read_dmb(path, nullptr, &w, &h, &ch) == 0 && w == pc.width && h == pc.height && ch == 1 &&
                   hpm_read_dmb(path, d.data(), &w, &h, &ch) == 0;
-        snprintf(path, sizeof(path), "%s/HPM/2333_%08d/normals.dmb", dense_folder, i);
+        snprintf(path, sizeof(path), "%s/%s/2333_%08d/normals.dmb", dense_folder, sub, i);
         ok = ok && hpm_read_dmb(path, nullptr, &w, &h, &ch) == 0 && w == pc.width && h == pc.height && ch == 3 &&
              hpm_read_dmb(path, nrm.data(), &w, &h, &ch) == 0;
         if (!ok) {
@@ -429,12 +433,12 @@ extern "C" int hpm_fusion_run_folder(const char* dense_folder, int geom_consiste
     hpm_point* pts = nullptr;
     int64_t n = 0;
     if (rc == HPM_OK) {
-        rc = hpm_fusion_run(pool, np, ns.data(), srcs.data(), HPM_MAX_SOURCES, geom_consistency, &pts, &n, err,
-                            err_len);
+        rc = (acmm ? hpm_fusion_run_acmm : hpm_fusion_run)(pool, np, ns.data(), srcs.data(), HPM_MAX_SOURCES,
+                                                           geom_consistency, &pts, &n, err, err_len);
     }
     hpm_pool_destroy(pool);
     if (rc == HPM_OK) {
-        const std::string ply = base + "/HPM/HPM_model.ply";
+        const std::string ply = base + (acmm ? "/ACMM_NESP/ACMM_NESP_model.ply" : "/HPM/HPM_model.ply");
         if (hpm_write_ply(ply.c_str(), pts, n) != 0) {
             rc = fail(err, err_len, "fusion: cannot write %s", ply.c_str());
         }
@@ -444,4 +448,18 @@ extern "C" int hpm_fusion_run_folder(const char* dense_folder, int geom_consiste
         *num_points = n;
     }
     return rc;
+}
+
+} // namespace
+
+extern "C" int hpm_fusion_run_folder(const char* dense_folder, int geom_consistency, int64_t* num_points, char* err,
+                                     int err_len)
+{
+    return fusion_folder(false, dense_folder, geom_consistency, num_points, err, err_len);
+}
+
+extern "C" int hpm_fusion_run_folder_acmm(const char* dense_folder, int geom_consistency, int64_t* num_points,
+                                          char* err, int err_len)
+{
+    return fusion_folder(true, dense_folder, geom_consistency, num_points, err, err_len);
 }

@@ -15,7 +15,10 @@
  * --acmm: the ACMM+NESP executable instead (ACMM+NESP/windows/main.cpp:
  * 302-378, hpm_acmm_run_folder): coarse-to-fine levels with the ACMM kernel
  * variant, results in ACMM_NESP/, fusion to ACMM_NESP/ACMM_NESP_model.ply;
- * --geom-iterations N sets its geometric passes per level (2).
+ * --geom-iterations N sets its geometric passes per level (2), --scale-bound
+ * N the size bound of its levels (1000). With --class-api the same schedule
+ * runs as the reference's main and ProcessProblem do (main.cpp:74-129,
+ * 302-378), against the drop-in `hpm::ACMM` class, file for file.
  */
 #include <math.h>
 #include <stdio.h>
@@ -261,13 +264,134 @@ void ProcessProblem(const std::string& dense_folder, const Problem& problem, boo
     printf("Processing image %08d done!\n", problem.ref_image_id);
 }
 
+/* ComputeMultiScaleSettings (ACMM+NESP/windows/main.cpp:35-71) */
+int ComputeMultiScaleSettings(const std::string& dense_folder, std::vector<Problem>& problems, int size_bound)
+{
+    hpm_params defaults;
+    hpm_params_default(&defaults);
+    int max_num_downscale = -1;
+    char path[4096];
+    for (Problem& p : problems) {
+        int w = 0, h = 0;
+        snprintf(path, sizeof(path), "%s/images/%08d.jpg", dense_folder.c_str(), p.ref_image_id);
+        if (hpm_read_image_grey(path, nullptr, &w, &h) != 0) {
+            snprintf(path, sizeof(path), "%s/images/%08d.pgm", dense_folder.c_str(), p.ref_image_id);
+            if (hpm_read_image_grey(path, nullptr, &w, &h) != 0) {
+                throw Error(std::string("cannot read image ") + path);
+            }
+        }
+        int max_size = std::min(std::max(w, h), (int)defaults.max_image_size);
+        p.max_image_size = max_size;
+        int k = 0;
+        while (max_size > size_bound) {
+            max_size /= 2;
+            ++k;
+        }
+        max_num_downscale = std::max(max_num_downscale, k);
+        p.num_downscale = k;
+    }
+    return max_num_downscale;
+}
+
+/* ProcessProblem of ACMM+NESP (ACMM+NESP/windows/main.cpp:74-129) */
+void AcmmProcessProblem(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
+                        bool geom_consistency, bool hierarchy, bool multi_geometry = false)
+{
+    const Problem& problem = problems[(size_t)idx];
+    printf("Processing image %08d...\n", problem.ref_image_id);
+    char buf[4096];
+    snprintf(buf, sizeof(buf), "%s/ACMM_NESP/2333_%08d", dense_folder.c_str(), problem.ref_image_id);
+    const std::string result_folder(buf);
+    mkdir(result_folder.c_str(), 0755);
+
+    ACMM acmm;
+    if (geom_consistency) {
+        acmm.SetGeomConsistencyParams(multi_geometry);
+    }
+    if (hierarchy) {
+        acmm.SetHierarchyParams();
+    }
+    acmm.InuputInitialization(dense_folder, problems, idx);
+    acmm.CudaSpaceInitialization(dense_folder, problem);
+    acmm.RunPatchMatch();
+
+    const int width = acmm.GetReferenceImageWidth();
+    const int height = acmm.GetReferenceImageHeight();
+    MatF depths(height, width), costs(height, width);
+    MatV3 normals(height, width);
+    for (int col = 0; col < width; ++col) {
+        for (int row = 0; row < height; ++row) {
+            const int center = row * width + col;
+            const float4 plane = acmm.GetPlaneHypothesis(center);
+            depths.data[(size_t)center] = plane.w;
+            normals.data[3 * (size_t)center] = plane.x;
+            normals.data[3 * (size_t)center + 1] = plane.y;
+            normals.data[3 * (size_t)center + 2] = plane.z;
+            costs.data[(size_t)center] = acmm.GetCost(center);
+        }
+    }
+    const std::string suffix = geom_consistency ? "/depths_geom.dmb" : "/depths.dmb";
+    if (writeDepthDmb(result_folder + suffix, depths) != 0 ||
+        writeNormalDmb(result_folder + "/normals.dmb", normals) != 0 ||
+        writeDepthDmb(result_folder + "/costs.dmb", costs) != 0) {
+        throw Error("cannot write the maps of " + result_folder);
+    }
+    printf("Processing image %08d done!\n", problem.ref_image_id);
+}
+
+/* main of ACMM+NESP (ACMM+NESP/windows/main.cpp:302-378) against the class;
+ * the JointBilateralUpsampling file step is folded into the hierarchy pass,
+ * which upsamples the coarser level's maps itself */
+int RunAcmmClassApi(const std::string& dense_folder, int geom_iterations, int size_bound, bool fusion)
+{
+    std::vector<Problem> problems = GenerateSampleList(dense_folder);
+    if (problems.empty()) {
+        throw Error("cannot read " + dense_folder + "/pair.txt");
+    }
+    mkdir((dense_folder + "/ACMM_NESP").c_str(), 0755);
+    const size_t num_images = problems.size();
+    printf("There are %zu problems needed to be processed!\n", num_images);
+    int max_num_downscale = ComputeMultiScaleSettings(dense_folder, problems, size_bound);
+    bool first = true;
+    while (max_num_downscale >= 0) {
+        printf("Scale: %d\n", max_num_downscale);
+        for (Problem& p : problems) {
+            if (p.num_downscale >= 0) {
+                p.cur_image_size = (int)(p.max_image_size / pow(2, p.num_downscale));
+                p.num_downscale--;
+            }
+        }
+        for (size_t i = 0; i < num_images; ++i) {
+            AcmmProcessProblem(dense_folder, problems, (int)i, false, !first);
+        }
+        for (int g = 0; g < geom_iterations; ++g) {
+            for (size_t i = 0; i < num_images; ++i) {
+                AcmmProcessProblem(dense_folder, problems, (int)i, true, false, g > 0);
+            }
+        }
+        first = false;
+        max_num_downscale--;
+    }
+    if (!fusion) {
+        return 0;
+    }
+    int64_t n = 0;
+    char err[512] = {0};
+    if (hpm_fusion_run_folder_acmm(dense_folder.c_str(), geom_iterations > 0 ? 1 : 0, &n, err, sizeof(err)) != 0) {
+        fprintf(stderr, "hpm_mvs: %s\n", err);
+        return 1;
+    }
+    printf("fusion: %lld points -> %s/ACMM_NESP/ACMM_NESP_model.ply\n", (long long)n, dense_folder.c_str());
+    return 0;
+}
+
 } // namespace
 
 int main(int argc, char** argv)
 {
     if (argc < 2) {
         printf("USAGE: hpm_mvs dense_folder [--class-api] [--geom-iterations N] [--knn-k K] [--no-prior] "
-               "[--no-geom] [--learned-prior FILE] [--no-fusion] [--acmm]\n");
+               "[--no-geom] [--learned-prior FILE] [--no-fusion] [--acmm] [--scale-bound N]\n");
         return -1;
     }
     const std::string dense_folder = argv[1];
@@ -275,6 +399,7 @@ int main(int argc, char** argv)
     const char* learned = nullptr;
     bool fusion = true;
     bool acmm = false;
+    int scale_bound = 1000;
     /* RunFusion (main.cpp:675) on the maps just written */
     auto run_fusion = [&]() -> int {
         if (!fusion) {
@@ -306,19 +431,30 @@ int main(int argc, char** argv)
             fusion = false;
         } else if (!strcmp(argv[i], "--acmm")) {
             acmm = true;
+        } else if (!strcmp(argv[i], "--scale-bound") && i + 1 < argc) {
+            scale_bound = atoi(argv[++i]);
         } else {
             fprintf(stderr, "unknown option %s\n", argv[i]);
             return -1;
         }
     }
     if (acmm) {
-        if (opt.class_api || learned) {
-            fprintf(stderr, "hpm_mvs: --acmm runs the ACMM+NESP schedule alone\n");
+        if (learned || scale_bound < 1) {
+            fprintf(stderr, "hpm_mvs: --acmm takes neither --learned-prior nor a scale bound below 1\n");
             return -1;
+        }
+        if (opt.class_api) {
+            try {
+                return RunAcmmClassApi(dense_folder, opt.geom ? opt.geom_iterations : 0, scale_bound, fusion);
+            } catch (const std::exception& e) {
+                fprintf(stderr, "hpm_mvs: %s\n", e.what());
+                return 1;
+            }
         }
         hpm_acmm_options ao;
         hpm_acmm_default(&ao);
         ao.geom_iterations = opt.geom ? opt.geom_iterations : 0;
+        ao.scale_bound = scale_bound;
         hpm_pipeline_stats st;
         int64_t n = 0;
         char err[512] = {0};

@@ -57,6 +57,12 @@ void hpm_fusion_free(hpm_point* points);
 int hpm_fusion_run_folder(const char* dense_folder, int geom_consistency, int64_t* num_points, char* err,
                           int err_len);
 
+/* RunFusion of ACMM+NESP as its main calls it (ACMM+NESP/windows/
+ * main.cpp:164-297, 385): the same inputs from ACMM_NESP/2333_%08d, fused
+ * with hpm_fusion_run_acmm, written to ACMM_NESP/ACMM_NESP_model.ply. */
+int hpm_fusion_run_folder_acmm(const char* dense_folder, int geom_consistency, int64_t* num_points, char* err,
+                               int err_len);
+
 /* StoreColorPlyFileBinaryPointCloud (HPM.cpp:403-456): binary little-endian
  * PLY, vertex x y z nx ny nz (float) red green blue (uchar), non-finite
  * coordinates written as 0; points in array order (the reference writes

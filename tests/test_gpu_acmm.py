@@ -9,9 +9,10 @@ hierarchy, hpm_set_hierarchy_input) against the CPU oracle.
   two geometric passes each) against oracle/pipeline.py: run_cpu_acmm; bar
   as for the HPM-MVS schedule (at least 99% of pixels within 1% depth and
   5 degrees normal; expected identical).
-- `hpm_mvs <folder> --acmm`: writes ACMM_NESP/2333_%08d/*.dmb and
-  ACMM_NESP/ACMM_NESP_model.ply, the cloud equal to the oracle's fusion of
-  the written maps.
+- `hpm_mvs <folder> --acmm` (device schedule, and with --class-api the
+  reference's main over the `hpm::ACMM` class; one level and two):
+  writes ACMM_NESP/2333_%08d/*.dmb and ACMM_NESP/ACMM_NESP_model.ply, the
+  cloud equal to the oracle's fusion of the written maps.
 - Error paths: a hierarchy input that is not coarser, ACMM problems in the
   batched form.
 """
@@ -140,7 +141,12 @@ def test_acmm_schedule_matches_cpu(hpm):
     pool.close()
 
 
-def test_executable_acmm_on_folder(hpm, tmp_path):
+@pytest.mark.parametrize("mode", [[], ["--class-api"], ["--scale-bound", "100"],
+                                  ["--class-api", "--scale-bound", "100"]])
+def test_executable_acmm_on_folder(hpm, tmp_path, mode):
+    """The device schedule and the reference's main written against the
+    drop-in `hpm::ACMM` class, at one level and at two (scale bound 100:
+    96x72 then 192x144, the second level started by the hierarchy pass)."""
     import subprocess
 
     from oracle import cpu
@@ -148,8 +154,8 @@ def test_executable_acmm_on_folder(hpm, tmp_path):
     exe = os.path.join(os.path.dirname(hpm.LIB_PATH), "hpm_mvs")
     cfg = hpm.scene_config(seed=6, width=192, height=144, fx=150.0, fy=150.0, cx=96.0, cy=72.0, num_views=3)
     hpm.write_scene_folder(cfg, str(tmp_path))
-    say("case: hpm_mvs --acmm on a 3-view 192x144 folder (one level)")
-    r = subprocess.run([exe, str(tmp_path), "--acmm"], capture_output=True, text=True, timeout=300)
+    say(f"case: hpm_mvs --acmm {' '.join(mode)} on a 3-view 192x144 folder")
+    r = subprocess.run([exe, str(tmp_path), "--acmm"] + mode, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     say(r.stdout.strip().splitlines()[-1])
     cams, depths, normals, colors = [], [], [], []
@@ -157,7 +163,9 @@ def test_executable_acmm_on_folder(hpm, tmp_path):
         base = tmp_path / "ACMM_NESP" / f"2333_{v:08d}"
         d = hpm.read_dmb(str(base / "depths_geom.dmb"))
         gt = hpm.read_dmb(str(tmp_path / "gt" / f"{v:08d}_depth.dmb"))
-        assert np.mean(np.abs(d - gt) / gt < 0.05) > 0.3
+        acc = np.mean(np.abs(d - gt) / gt < 0.05)
+        say(f"view {v}: within 5% of the true depth {acc:.4f}")
+        assert d.shape == (144, 192) and acc > 0.3
         for f in ("depths.dmb", "normals.dmb", "costs.dmb"):
             assert (base / f).exists()
         cam = hpm.read_camera(str(tmp_path / "cams" / f"{v:08d}_cam.txt"))

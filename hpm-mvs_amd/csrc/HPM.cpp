@@ -439,6 +439,9 @@ void ACMM::SetHierarchyParams()
  * geometric consistency the depth maps of every image */
 void ACMM::InuputInitialization(const std::string& dense_folder, const std::vector<Problem>& problems, const int idx)
 {
+    if (idx < 0 || (size_t)idx >= problems.size()) {
+        throw Error("InuputInitialization: problem index outside the list");
+    }
     const Problem& problem = problems[(size_t)idx];
     std::vector<int> ids = {problem.ref_image_id};
     ids.insert(ids.end(), problem.src_image_ids.begin(), problem.src_image_ids.end());

@@ -295,7 +295,7 @@ int ComputeMultiScaleSettings(const std::string& dense_folder, std::vector<Probl
 
 /* ProcessProblem of ACMM+NESP (ACMM+NESP/windows/main.cpp:74-129) */
 void AcmmProcessProblem(const std::string& dense_folder, const std::vector<Problem>& problems, int idx,
-                        bool geom_consistency, bool hierarchy, bool multi_geometry = false)
+                        uint32_t stream, bool geom_consistency, bool hierarchy, bool multi_geometry = false)
 {
     const Problem& problem = problems[(size_t)idx];
     printf("Processing image %08d...\n", problem.ref_image_id);
@@ -313,6 +313,11 @@ void AcmmProcessProblem(const std::string& dense_folder, const std::vector<Probl
     }
     acmm.InuputInitialization(dense_folder, problems, idx);
     acmm.CudaSpaceInitialization(dense_folder, problem);
+    /* the Philox stream hpm_acmm_run gives this pass, so that both runs of
+     * the schedule draw the same numbers */
+    if (hpm_set_stream_index(acmm.handle(), stream) != 0) {
+        throw Error(std::string("hpm_set_stream_index: ") + hpm_last_error(acmm.handle()));
+    }
     acmm.RunPatchMatch();
 
     const int width = acmm.GetReferenceImageWidth();
@@ -353,7 +358,7 @@ int RunAcmmClassApi(const std::string& dense_folder, int geom_iterations, int si
     printf("There are %zu problems needed to be processed!\n", num_images);
     int max_num_downscale = ComputeMultiScaleSettings(dense_folder, problems, size_bound);
     bool first = true;
-    while (max_num_downscale >= 0) {
+    for (uint32_t step = 0; max_num_downscale >= 0; ++step) {
         printf("Scale: %d\n", max_num_downscale);
         for (Problem& p : problems) {
             if (p.num_downscale >= 0) {
@@ -362,11 +367,11 @@ int RunAcmmClassApi(const std::string& dense_folder, int geom_iterations, int si
             }
         }
         for (size_t i = 0; i < num_images; ++i) {
-            AcmmProcessProblem(dense_folder, problems, (int)i, false, !first);
+            AcmmProcessProblem(dense_folder, problems, (int)i, 16u * step, false, !first);
         }
         for (int g = 0; g < geom_iterations; ++g) {
             for (size_t i = 0; i < num_images; ++i) {
-                AcmmProcessProblem(dense_folder, problems, (int)i, true, false, g > 0);
+                AcmmProcessProblem(dense_folder, problems, (int)i, 16u * step + 8u + (uint32_t)g, true, false, g > 0);
             }
         }
         first = false;

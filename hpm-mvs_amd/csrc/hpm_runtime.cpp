@@ -772,6 +772,18 @@ extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
     return HPM_OK;
 }
 
+extern "C" int hpm_set_stream_index(hpm_handle* h, uint32_t index)
+{
+    if (!h) {
+        return hpm_fail(h, HPM_EINVAL, "null argument");
+    }
+    if (!h->have_device) {
+        return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
+    }
+    h->call_index = index;
+    return HPM_OK;
+}
+
 extern "C" int hpm_device_init_state(hpm_handle* h, const float* hyp_world, const float* cost)
 {
     if (!h || !hyp_world || !cost) {

@@ -238,6 +238,7 @@ SIGNATURES = {
                                       C.POINTER(C.c_int64), C.c_char_p, C.c_int]),
     "hpm_fusion_free": (None, [C.c_void_p]),
     "hpm_fusion_run_folder": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.c_char_p, C.c_int]),
+    "hpm_set_stream_index": (C.c_int, [C.c_void_p, C.c_uint32]),
     "hpm_fusion_run_folder_acmm": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.c_char_p, C.c_int]),
     "hpm_pool_set_state": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP]),
     "hpm_write_ply": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int64]),

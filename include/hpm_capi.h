@@ -117,6 +117,13 @@ int hpm_device_init(hpm_handle* h, const char* dense_folder);
  * (world normal xyz, depth) per pixel, cost the previous costs. */
 int hpm_device_init_state(hpm_handle* h, const float* hyp_world, const float* cost);
 
+/* Philox stream index of the next RunPatchMatch (each call then advances
+ * it by one; hpm_device_init resets it to 0). No counterpart in the
+ * reference, which seeds every launch from clock64() (HPM.cu:521): it lets
+ * a caller that drives the class API reproduce a schedule's passes bit for
+ * bit (hpm_pipeline_options.stream_base). Call after hpm_device_init. */
+int hpm_set_stream_index(hpm_handle* h, uint32_t index);
+
 /* HPM::RunPatchMatch (HPM.cu:1906-1957) */
 int hpm_run_patchmatch(hpm_handle* h);
 int hpm_get_run_stats(const hpm_handle* h, hpm_run_stats* out);

@@ -189,6 +189,36 @@ def test_executable_acmm_on_folder(hpm, tmp_path, mode):
     assert np.array_equal(rows["rgb"], np.stack([want["r"], want["g"], want["b"]], axis=1))
 
 
+def test_class_api_replays_device_schedule(hpm, tmp_path):
+    """`hpm_mvs --acmm --class-api` (the reference's main over `hpm::ACMM`,
+    .dmb files between passes) with every pass on the Philox stream the
+    device schedule gives it (hpm_set_stream_index): every map file and the
+    fused cloud equal the device schedule's byte for byte, over two levels
+    (96x72, then 192x144 started by the hierarchy pass)."""
+    import shutil
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(hpm.LIB_PATH), "hpm_mvs")
+    cfg = hpm.scene_config(seed=8, width=192, height=144, fx=150.0, fy=150.0, cx=96.0, cy=72.0, num_views=3)
+    dev, cls = tmp_path / "device", tmp_path / "class"
+    hpm.write_scene_folder(cfg, str(dev))
+    shutil.copytree(dev, cls)
+    say("case: hpm_mvs --acmm --scale-bound 100, device schedule and class API, 3 views 192x144")
+    for folder, mode in ((dev, []), (cls, ["--class-api"])):
+        r = subprocess.run([exe, str(folder), "--acmm", "--scale-bound", "100"] + mode, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+    for v in range(3):
+        for f in ("depths.dmb", "depths_geom.dmb", "normals.dmb", "costs.dmb"):
+            x = (dev / "ACMM_NESP" / f"2333_{v:08d}" / f).read_bytes()
+            y = (cls / "ACMM_NESP" / f"2333_{v:08d}" / f).read_bytes()
+            assert x == y, (v, f)
+    x = (dev / "ACMM_NESP" / "ACMM_NESP_model.ply").read_bytes()
+    y = (cls / "ACMM_NESP" / "ACMM_NESP_model.ply").read_bytes()
+    say(f"maps identical; PLY {len(x)} bytes, identical {x == y}")
+    assert x == y
+
+
 def test_acmm_error_paths(hpm, small_scene):
     fine, fimg, gt = small_scene
     say("case: ACMM error paths")

@@ -108,7 +108,7 @@ void rasterize(HPM& hpm, const std::vector<Triangle>& triangles, const Rect& rc,
 
 /* ProcessProblem (main.cpp:35-499) */
 void ProcessProblem(const std::string& dense_folder, const Problem& problem, bool geom_consistency, bool planar_prior,
-                    bool multi_geometry, const Options& opt)
+                    bool multi_geometry, const Options& opt, uint32_t stream = 0)
 {
     printf("Processing image %08d...\n", problem.ref_image_id);
     char buf[4096];
@@ -123,6 +123,11 @@ void ProcessProblem(const std::string& dense_folder, const Problem& problem, boo
     }
     hpm.InuputInitialization(dense_folder, problem);
     hpm.CudaSpaceInitialization(dense_folder, problem);
+    /* the Philox stream hpm_pipeline_run gives the pass (the later prior
+     * passes of this object follow it) */
+    if (hpm_set_stream_index(hpm.handle(), stream) != 0) {
+        throw Error(std::string("hpm_set_stream_index: ") + hpm_last_error(hpm.handle()));
+    }
     hpm.RunPatchMatch();
 
     const int width = hpm.GetReferenceImageWidth();
@@ -509,7 +514,7 @@ int main(int argc, char** argv)
         if (opt.geom) {
             for (int g = 0; g < opt.geom_iterations; ++g) {
                 for (const Problem& p : problems) {
-                    ProcessProblem(dense_folder, p, true, false, g > 0, opt);
+                    ProcessProblem(dense_folder, p, true, false, g > 0, opt, 8u + (uint32_t)g);
                 }
             }
         }

@@ -101,6 +101,37 @@ def test_executable_on_folder(hpm, tmp_path, mode):
     assert np.array_equal(rows["rgb"], np.stack([want["r"], want["g"], want["b"]], axis=1))
 
 
+def test_class_api_replays_device_schedule(hpm, tmp_path):
+    """`hpm_mvs --class-api` (the reference's ProcessProblem over `hpm::HPM`,
+    host prior construction, .dmb files between passes) with every pass on
+    the Philox stream the device schedule gives it: every map file and the
+    fused cloud equal the device schedule's (device prior) byte for byte."""
+    import shutil
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(hpm.LIB_PATH), "hpm_mvs")
+    cfg = hpm.scene_config(seed=6, width=192, height=144, fx=150.0, fy=150.0, cx=96.0, cy=72.0, num_views=3)
+    dev, cls = tmp_path / "device", tmp_path / "class"
+    hpm.write_scene_folder(cfg, str(dev))
+    shutil.copytree(dev, cls)
+    say("case: hpm_mvs --geom-iterations 2, device schedule and class API, 3 views 192x144")
+    for folder, mode in ((dev, []), (cls, ["--class-api"])):
+        r = subprocess.run([exe, str(folder), "--geom-iterations", "2"] + mode, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+    for v in range(3):
+        for f in ("depths.dmb", "depths_geom.dmb", "normals.dmb", "costs.dmb"):
+            x = hpm.read_dmb(str(dev / "HPM" / f"2333_{v:08d}" / f))
+            y = hpm.read_dmb(str(cls / "HPM" / f"2333_{v:08d}" / f))
+            same = np.mean(x.view(np.uint32) == y.view(np.uint32))
+            say(f"view {v} {f}: identical {same:.6f}")
+            assert same == 1.0, (v, f)
+    x = (dev / "HPM" / "HPM_model.ply").read_bytes()
+    y = (cls / "HPM" / "HPM_model.ply").read_bytes()
+    say(f"PLY {len(x)} bytes, identical {x == y}")
+    assert x == y
+
+
 def test_pipeline_folder_round_trip(hpm, tmp_path):
     cfg = hpm.scene_config(seed=4, width=160, height=120, fx=125.0, fy=125.0, cx=80.0, cy=60.0, num_views=3)
     hpm.write_scene_folder(cfg, str(tmp_path))

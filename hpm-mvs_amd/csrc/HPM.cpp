@@ -136,6 +136,11 @@ float4 HPM::GetPlaneHypothesis(const int index)
 
 float HPM::GetCost(const int index)
 {
+    int w = 0, hh = 0;
+    check(hpm_dims(h_, &w, &hh), "GetCost");
+    if (index < 0 || (int64_t)index >= (int64_t)w * hh) {
+        throw Error("GetCost: pixel index " + std::to_string(index) + " outside the reference image");
+    }
     return hpm_get_cost(h_, index);
 }
 
@@ -481,6 +486,9 @@ void ACMM::InuputInitialization(const std::string& dense_folder, const std::vect
             if (readDepthDmb(p, depths[i]) != 0) {
                 throw Error("cannot read " + p);
             }
+            if (depths[i].cols != cams[i].width || depths[i].rows != cams[i].height) {
+                throw Error(p + " is not the size of the view's image");
+            }
         }
     }
     std::vector<const float*> ip(n), dp(n);
@@ -551,6 +559,11 @@ float4 ACMM::GetPlaneHypothesis(const int index)
 
 float ACMM::GetCost(const int index)
 {
+    int w = 0, hh = 0;
+    check(hpm_dims(h_, &w, &hh), "GetCost");
+    if (index < 0 || (int64_t)index >= (int64_t)w * hh) {
+        throw Error("GetCost: pixel index " + std::to_string(index) + " outside the reference image");
+    }
     return hpm_get_cost(h_, index);
 }
 

@@ -1036,7 +1036,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 if (ca[0][i] < good) {
                     ++good_sum;
                 }
-                if ((double)ca[0][i] > 1.2) {
+                if (ca[0][i] > 1.2f) { /* float bad_threshold (HPM.cu:1042) */
                     ++bad_sum;
                 }
             }

@@ -745,19 +745,18 @@ extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
                  h->ref_id);
         const char* suffix = h->params.multi_geometry ? "depths_geom.dmb" : "depths.dmb";
         char path[4200];
-        int w = 0, hh = 0, nb = 0;
         std::vector<float> depth(np), normal(np * 3), cost(np);
-        snprintf(path, sizeof(path), "%s/%s", base, suffix);
-        if (hpm_read_dmb(path, depth.data(), &w, &hh, &nb) != 0 || w != W || hh != H || nb != 1) {
-            return hpm_fail(h, HPM_EIO, "cannot reload %s", path);
-        }
-        snprintf(path, sizeof(path), "%s/normals.dmb", base);
-        if (hpm_read_dmb(path, normal.data(), &w, &hh, &nb) != 0 || w != W || hh != H || nb != 3) {
-            return hpm_fail(h, HPM_EIO, "cannot reload %s", path);
-        }
-        snprintf(path, sizeof(path), "%s/costs.dmb", base);
-        if (hpm_read_dmb(path, cost.data(), &w, &hh, &nb) != 0 || w != W || hh != H || nb != 1) {
-            return hpm_fail(h, HPM_EIO, "cannot reload %s", path);
+        /* the header is read first: a file of another size (an earlier run
+         * at another scale) is an error, never a read past the buffers */
+        auto reload = [&](const char* name, float* dst, int nb_want) {
+            snprintf(path, sizeof(path), "%s/%s", base, name);
+            int w = 0, hh = 0, nb = 0;
+            return hpm_read_dmb(path, nullptr, &w, &hh, &nb) == 0 && w == W && hh == H && nb == nb_want &&
+                   hpm_read_dmb(path, dst, &w, &hh, &nb) == 0;
+        };
+        if (!reload(suffix, depth.data(), 1) || !reload("normals.dmb", normal.data(), 3) ||
+            !reload("costs.dmb", cost.data(), 1)) {
+            return hpm_fail(h, HPM_EIO, "cannot reload %s (missing, or not %dx%d)", path, W, H);
         }
         std::vector<float> hyp(np * 4);
         for (size_t i = 0; i < np; ++i) {

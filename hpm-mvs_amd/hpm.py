@@ -447,6 +447,13 @@ class Problem:
         self._check(lib().hpm_dims(self._h, C.byref(w), C.byref(h)), "hpm_dims")
         return w.value, h.value
 
+    def camera(self, image: int) -> HpmCamera:
+        """Camera of image `image` of the problem (0 = reference) as the
+        device uses it, after the max_image_size downscale."""
+        c = HpmCamera()
+        self._check(lib().hpm_get_camera(self._h, int(image), C.byref(c)), "hpm_get_camera")
+        return c
+
     def GetReferenceImageWidth(self) -> int:
         return self.dims()[0]
 

@@ -96,8 +96,9 @@ int hpm_set_hierarchy_input(hpm_handle* h, const float* coarse_world4, int coars
 /* HPM::InuputInitialization (HPM.cpp:482-589): reads images/%08d.{pgm,jpg}
  * as grey, cams/%08d_cam.txt, downscales images above max_image_size, sets
  * the relaxed depth range; with geometric consistency also reads
- * HPM/2333_%08d/depths.dmb (or depths_geom.dmb) for every view. Only 8-bit
- * PGM (P5) images are decoded here; a .jpg path is tried as PGM content. */
+ * HPM/2333_%08d/depths.dmb (or depths_geom.dmb) for every view. Images are
+ * images/%08d.jpg (sequential-Huffman JPEG, decoded to the luminance that
+ * cv::imread(IMREAD_GRAYSCALE) returns), else images/%08d.pgm (8-bit P5). */
 int hpm_input_init(hpm_handle* h, const char* dense_folder, int ref_image_id, const int* src_image_ids,
                    int num_src);
 /* The same on a pool: the problem's reference and source views are pool

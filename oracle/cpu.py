@@ -44,6 +44,25 @@ _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
 
 
+COST_FN = C.CFUNCTYPE(C.c_float, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _FP)
+
+
+class OracleTrace(C.Structure):
+    """hpm_oracle_trace: one pixel's propagation decisions."""
+
+    _fields_ = [
+        ("px", C.c_int32), ("py", C.c_int32), ("hit", C.c_int32),
+        ("flag", C.c_int32 * 8), ("pos_scan", C.c_int32 * 8), ("judge", C.c_int32 * 3), ("pos", C.c_int32 * 8),
+        ("ca", (C.c_float * 32) * 8), ("vprior", C.c_float * 32), ("pdf", C.c_float * 32),
+        ("cdf", C.c_float * 32), ("draws", C.c_float * 15), ("vw", C.c_float * 32), ("tsel", C.c_uint32),
+        ("wn", C.c_float), ("fc", C.c_float * 8), ("min_idx", C.c_int32), ("cost_now", C.c_float),
+        ("rfc", C.c_float * 8), ("max_idx", C.c_int32), ("rcn", C.c_float), ("adopted", C.c_int32),
+        ("nan_fallback", C.c_int32), ("restricted_cost", C.c_float), ("cand", (C.c_float * 4) * 5),
+        ("cand_depth", C.c_float * 5), ("cand_cost", C.c_float * 5), ("cand_rcost", C.c_float * 5),
+        ("cand_accepted", C.c_int32 * 5),
+    ]
+
+
 class OracleProblem(C.Structure):
     _fields_ = [
         ("num_images", C.c_int32),
@@ -59,6 +78,9 @@ class OracleProblem(C.Structure):
         ("coarse_w", C.c_int32),
         ("coarse_h", C.c_int32),
         ("pre_cost", _FP),
+        ("cost_hook", COST_FN),
+        ("cost_hook_user", C.c_void_p),
+        ("trace", C.POINTER(OracleTrace)),
     ]
 
 
@@ -81,6 +103,8 @@ def lib():
         L.hpm_oracle_run_patchmatch.argtypes = [P, _FP, _FP, _UP, U64, C.c_int]
         L.hpm_oracle_init.argtypes = [P, _FP, _FP, _UP, U64, C.c_int]
         L.hpm_oracle_half_iteration.argtypes = [P, C.c_int, C.c_int, _FP, _FP, _UP, U64, C.c_int]
+        L.hpm_oracle_half_iteration_rows.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, _FP, _FP, _UP, U64,
+                                                     C.c_int]
         L.hpm_oracle_finalize.argtypes = [P, _FP, C.c_int]
         L.hpm_oracle_filter.argtypes = [P, C.c_int, _FP, _FP, C.c_int]
         L.hpm_oracle_cost_vectors.argtypes = [P, _FP, _IP, C.c_int, C.c_int, _DP, C.c_int]
@@ -154,6 +178,25 @@ class Oracle:
         self.pb.pre_cost = _f(self.pre_cost)
         self.pb.params.hierarchy = 1
 
+    def set_cost_hook(self, fn):
+        """Replaces the NCC by fn(px, py, view, plane) -> float (micro-scenes;
+        the oracle then runs on one thread)."""
+
+        def cb(_user, px, py, view, plane):
+            return float(fn(px, py, view, np.ctypeslib.as_array(plane, (4,)).copy()))
+
+        self._hook = COST_FN(cb)
+        self.pb.cost_hook = self._hook
+        self.threads = 1
+
+    def trace_pixel(self, x: int, y: int) -> OracleTrace:
+        """Records the decisions of pixel (x, y) in the following
+        half-iterations."""
+        self._trace = OracleTrace()
+        self._trace.px, self._trace.py = x, y
+        self.pb.trace = C.pointer(self._trace)
+        return self._trace
+
     def _state(self):
         return (_f(self.hyp), _f(self.cost), self.sel.ctypes.data_as(_UP))
 
@@ -174,6 +217,12 @@ class Oracle:
         n = C.c_uint64(0)
         assert lib().hpm_oracle_half_iteration(C.byref(self.pb), it, colour, *self._state(), C.byref(n),
                                                self.threads) == 0
+        return n.value
+
+    def half_iteration_rows(self, it: int, colour: int, y0: int, y1: int):
+        n = C.c_uint64(0)
+        assert lib().hpm_oracle_half_iteration_rows(C.byref(self.pb), it, colour, y0, y1, *self._state(),
+                                                    C.byref(n), self.threads) == 0
         return n.value
 
     def finalize(self):

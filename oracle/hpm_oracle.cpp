@@ -659,6 +659,17 @@ static inline int find_max_idx(const float* c, int n)
     return idx;
 }
 
+/* One (pixel, plane, source view) cost: the bilateral NCC, or the
+ * micro-scene cost hook when a test installed one. */
+static inline float eval_cost(const Ctx& cx, const RefPatch& rp, int px, int py, int v, F4 plane, const double q[3])
+{
+    if (cx.pb->cost_hook != nullptr) {
+        const float pl[4] = {plane.x, plane.y, plane.z, plane.w};
+        return cx.pb->cost_hook(cx.pb->cost_hook_user, px, py, v, pl);
+    }
+    return ncc(cx, rp, px, py, v, q);
+}
+
 /* ComputeMultiViewCostVector (HPM.cu:438-443); with weights given, views
  * of zero weight are skipped (the caller never reads their terms). */
 static void cost_vector(const Ctx& cx, const RefPatch& rp, int px, int py, F4 plane, float* out,
@@ -671,7 +682,7 @@ static void cost_vector(const Ctx& cx, const RefPatch& rp, int px, int py, F4 pl
             out[v - 1] = 0.0f;
             continue;
         }
-        out[v - 1] = ncc(cx, rp, px, py, v, q);
+        out[v - 1] = eval_cost(cx, rp, px, py, v, plane, q);
         ++*nev;
     }
 }
@@ -685,7 +696,7 @@ static float initial_cost(const Ctx& cx, const RefPatch& rp, int px, int py, F4 
     double q[3];
     plane_q(cx.rk, plane, q);
     for (int v = 1; v <= cx.nsrc; ++v) {
-        const float c = ncc(cx, rp, px, py, v, q);
+        const float c = eval_cost(cx, rp, px, py, v, plane, q);
         ++*nev;
         cv[v - 1] = c;
         cvc[v - 1] = c;
@@ -865,13 +876,13 @@ static bool judge_extend(const Ctx& cx, int iter, int k, const float* c, bool fl
         return false;
     }
     const float good = cx.ic.good[iter][k];
-    const double bad = 1.2;
+    const float bad = 1.2f; /* float bad_threshold = 1.2; (HPM.cu:1042): compared in float */
     int good_sum = 0, bad_sum = 0;
     for (int i = 0; i < cx.nsrc; ++i) {
         if (c[i] < good) {
             ++good_sum;
         }
-        if ((double)c[i] > bad) {
+        if (c[i] > bad) {
             ++bad_sum;
         }
     }
@@ -915,6 +926,7 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
         return;
     }
     const int center = py * W + px;
+    hpm_oracle_trace* tr = (pb->trace != nullptr && pb->trace->px == px && pb->trace->py == py) ? pb->trace : nullptr;
     hpm_rng rng;
     hpm_rng_init(&rng, P.seed_lo, P.seed_hi, (uint32_t)center, hpm_stream_update(pb->call_index, iter, colour),
                  pb->salt);
@@ -1114,11 +1126,25 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
         cost_vector(cx, rp, px, py, ld4(hyp, left_down), ca[6], nullptr, nev);
     }
     int pos[8] = {left_up, up_far, right_up, down_far, right_down, left_far, left_down, right_far};
+    if (tr) {
+        tr->hit = 1;
+        for (int i = 0; i < 8; ++i) {
+            tr->flag[i] = flag[i] ? 1 : 0;
+            tr->pos_scan[i] = pos[i];
+        }
+        for (int k = 0; k < 3; ++k) {
+            tr->judge[k] = -1;
+        }
+    }
 
     /* NESP extension: only orientation 0 is ever enabled (HPM.cu:1473-1543);
      * ExtendedLeftUpPropagation (HPM.cu:780-843). */
     for (int k = 0; k < 3; ++k) {
-        if (!judge_extend(cx, iter, k, ca[0], flag[0])) {
+        const bool ext = judge_extend(cx, iter, k, ca[0], flag[0]);
+        if (tr) {
+            tr->judge[k] = ext ? 1 : 0;
+        }
+        if (!ext) {
             break;
         }
         int lu = center - 5 - 6 * W - 4 * 2 * k - 4 * 2 * W * k;
@@ -1158,6 +1184,11 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
             lu = pos[0]; /* flag[0] is true here */
         }
         pos[0] = lu;
+    }
+
+    if (tr) {
+        memcpy(tr->pos, pos, sizeof(pos));
+        memcpy(tr->ca, ca, sizeof(ca));
     }
 
     /* Multi-hypothesis joint view selection (HPM.cu:1548-1613) */
@@ -1202,6 +1233,10 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
         }
         probs[i] = probs[i] * vprior[i];
     }
+    if (tr) {
+        memcpy(tr->vprior, vprior, sizeof(vprior));
+        memcpy(tr->pdf, probs, sizeof(probs));
+    }
     /* TransformPDFToCDF (HPM.cu:86-100) */
     {
         float sum = 0.0f;
@@ -1215,8 +1250,14 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
             probs[i] = cum;
         }
     }
+    if (tr) {
+        memcpy(tr->cdf, probs, sizeof(probs));
+    }
     for (int s = 0; s < 15; ++s) {
         const float u = hpm_rng_uniform(&rng) - FLT_EPSILON;
+        if (tr) {
+            tr->draws[s] = u;
+        }
         for (int v = 0; v < nsrc; ++v) {
             if (probs[v] > u) {
                 vw[v] += 1.0f;
@@ -1231,6 +1272,12 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
             tsel |= (1u << i);
             wn += vw[i];
         }
+    }
+
+    if (tr) {
+        memcpy(tr->vw, vw, sizeof(vw));
+        tr->tsel = tsel;
+        tr->wn = wn;
     }
 
     /* Candidate scoring (HPM.cu:1615-1633) */
@@ -1253,6 +1300,17 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
         fc[i] /= wn;
     }
     const int min_idx = find_min_idx(fc, 8);
+    if (tr) {
+        memcpy(tr->fc, fc, sizeof(fc));
+        tr->min_idx = min_idx;
+        tr->max_idx = -1;
+        tr->rcn = 0.0f;
+        tr->adopted = -1;
+        tr->nan_fallback = 0;
+        for (int i = 0; i < 8; ++i) {
+            tr->rfc[i] = 0.0f;
+        }
+    }
 
     /* Re-score the current hypothesis (HPM.cu:1637-1650); the geometric
      * term weighs 0.1 there and 0.2 in ACMM (ACMM.cu:1596, 651) */
@@ -1273,6 +1331,9 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
     }
     cost_now /= wn;
     cost[center] = cost_now;
+    if (tr) {
+        tr->cost_now = cost_now;
+    }
     float depth_now = depth_from_plane(rc, cur, px, py);
 
     /* Adoption (HPM.cu:1652-1722) */
@@ -1295,6 +1356,11 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
             const float dn = depth_from_plane(rc, cur, px, py);
             const float pr = prior_term(pk, dn - depth_prior, prior_plane, cur);
             const float rcn = hpm_expf(-cost_now * cost_now / pk.beta) * pr;
+            if (tr) {
+                memcpy(tr->rfc, rfc, sizeof(rfc));
+                tr->max_idx = max_idx;
+                tr->rcn = rcn;
+            }
             if (flag[max_idx]) {
                 const F4 hp = ld4(hyp, pos[max_idx]);
                 const float db = depth_from_plane(rc, hp, px, py);
@@ -1304,6 +1370,9 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
                     cost[center] = fc[max_idx];
                     restricted_cost = rfc[max_idx];
                     sel[center] = tsel;
+                    if (tr) {
+                        tr->adopted = max_idx;
+                    }
                 }
             }
         } else if (flag[min_idx]) {
@@ -1313,6 +1382,9 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
                 depth_now = db;
                 st4(hyp, center, hp);
                 cost[center] = fc[min_idx];
+                if (tr) {
+                    tr->adopted = min_idx;
+                }
             }
         }
     }
@@ -1324,12 +1396,19 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
             st4(hyp, center, hp);
             cost[center] = fc[min_idx];
             sel[center] = tsel;
+            if (tr) {
+                tr->adopted = min_idx;
+            }
         }
         if (!P.acmm && cost[center] != cost[center]) { /* no such fallback in ACMM (ACMM.cu:1607-1616) */
             depth_now = db;
             st4(hyp, center, hp);
             cost_now = fc[min_idx];
             sel[center] = tsel;
+            if (tr) {
+                tr->adopted = min_idx;
+                tr->nan_fallback = 1;
+            }
         }
     }
 
@@ -1359,6 +1438,9 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
         const F4 n_pert = perturbed_normal(rc, px, py, plane_now, &rng, (float)((double)perturbation * kPi));
         const float depths[5] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert};
         const F4 normals[5] = {plane_now, n_rand, n_rand, n_pert, plane_now};
+        if (tr) {
+            tr->restricted_cost = restricted_cost;
+        }
         for (int i = 0; i < 5; ++i) {
             float cvt[32];
             F4 t = normals[i];
@@ -1376,21 +1458,35 @@ static void propagate_pixel(const Ctx& cx, int px, int py, int iter, int colour,
             }
             tc /= wn;
             const float db = depth_from_plane(rc, t, px, py);
+            bool accepted = false;
+            float rtc = 0.0f;
             if (use_prior) {
                 const float pr = prior_term(pk, depths[i] - depth_prior, prior_plane, t);
-                const float rtc = hpm_expf(-tc * tc / pk.beta) * pr;
+                rtc = hpm_expf(-tc * tc / pk.beta) * pr;
                 if (db >= P.depth_min && db <= P.depth_max && rtc > restricted_cost) {
                     depth_now = db;
                     st4(hyp, center, t);
                     cost[center] = tc;
                     restricted_cost = rtc;
+                    accepted = true;
                 }
             } else {
                 if (db >= P.depth_min && db <= P.depth_max && tc < cost[center]) {
                     depth_now = db;
                     st4(hyp, center, t);
                     cost[center] = tc;
+                    accepted = true;
                 }
+            }
+            if (tr) {
+                tr->cand[i][0] = t.x;
+                tr->cand[i][1] = t.y;
+                tr->cand[i][2] = t.z;
+                tr->cand[i][3] = t.w;
+                tr->cand_depth[i] = depths[i];
+                tr->cand_cost[i] = tc;
+                tr->cand_rcost[i] = rtc;
+                tr->cand_accepted[i] = accepted ? 1 : 0;
             }
         }
     }
@@ -1570,6 +1666,13 @@ extern "C" int hpm_oracle_init(const hpm_oracle_problem* pb, float* hyp, float* 
 extern "C" int hpm_oracle_half_iteration(const hpm_oracle_problem* pb, int iter, int colour, float* hyp,
                                          float* cost, uint32_t* selmask, uint64_t* n_eval, int num_threads)
 {
+    return hpm_oracle_half_iteration_rows(pb, iter, colour, 0, INT32_MAX, hyp, cost, selmask, n_eval, num_threads);
+}
+
+extern "C" int hpm_oracle_half_iteration_rows(const hpm_oracle_problem* pb, int iter, int colour, int y0, int y1,
+                                              float* hyp, float* cost, uint32_t* selmask, uint64_t* n_eval,
+                                              int num_threads)
+{
     if (check_problem(pb) != 0 || iter < 0 || iter > 2 || colour < 0 || colour > 1) {
         return -1;
     }
@@ -1578,6 +1681,9 @@ extern "C" int hpm_oracle_half_iteration(const hpm_oracle_problem* pb, int iter,
     const int nt = nthreads_or_default(num_threads);
     std::vector<uint64_t> counts((size_t)cx.H, 0);
     for_colour(cx, colour, nt, [&](int x, int y) {
+        if (y < y0 || y >= y1) {
+            return;
+        }
         uint64_t local = 0;
         propagate_pixel(cx, x, y, iter, colour, hyp, cost, selmask, &local);
         counts[(size_t)y] += local;

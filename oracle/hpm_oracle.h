@@ -28,6 +28,50 @@
 extern "C" {
 #endif
 
+/* Micro-scene hooks (tests only). A cost hook, when set, replaces the
+ * bilateral NCC (HPM.cu:325-397) of one (reference pixel, camera-frame
+ * plane (n, d), source view 1..N-1) everywhere the restatement evaluates
+ * it, so that a test can build the cost fields the sampling, view
+ * selection, adoption and refinement logic acts on. Call with one thread
+ * when the hook is not thread-safe. */
+typedef float (*hpm_oracle_cost_fn)(void* user, int32_t px, int32_t py, int32_t view, const float* plane);
+
+/* What one pixel's CheckerboardPropagation + PlaneHypothesisRefinement
+ * (HPM.cu:1222-1725, 1143-1220) decided, recorded when trace->px/py is
+ * updated by a half-iteration. Directions in the reference's order:
+ * 0 left_up, 1 up_far, 2 right_up, 3 down_far, 4 right_down, 5 left_far,
+ * 6 left_down, 7 right_far. */
+typedef struct hpm_oracle_trace {
+    int32_t px, py;           /* in: the pixel to record */
+    int32_t hit;              /* out: 1 once recorded */
+    int32_t flag[8];          /* direction valid */
+    int32_t pos_scan[8];      /* neighbour pixel index chosen by each scan */
+    int32_t judge[3];         /* JudgeExtend per round: 1 extend, 0 stop, -1 not reached */
+    int32_t pos[8];           /* positions after the extension rounds */
+    float ca[8][32];          /* cost_array after the extension rounds */
+    float vprior[32];         /* view_selection_priors */
+    float pdf[32];            /* sampling_probs (times the prior), before the CDF */
+    float cdf[32];            /* after TransformPDFToCDF */
+    float draws[15];          /* curand_uniform - FLT_EPSILON */
+    float vw[32];             /* view_weights */
+    uint32_t tsel;            /* temp_selected_views */
+    float wn;                 /* weight_norm */
+    float fc[8];              /* final_costs */
+    int32_t min_idx;          /* FindMinCostIndex(final_costs) */
+    float cost_now;           /* re-scored current hypothesis */
+    float rfc[8];             /* restricted_final_costs (prior mask > 0) */
+    int32_t max_idx;          /* FindMaxCostIndex(restricted_final_costs), -1 without */
+    float rcn;                /* restricted_cost_now */
+    int32_t adopted;          /* direction whose hypothesis was adopted, -1 none */
+    int32_t nan_fallback;     /* 1 when the NaN-cost fallback adopted */
+    float restricted_cost;    /* restricted_cost entering the refinement */
+    float cand[5][4];         /* refinement planes (camera frame, d from GetDistance2Origin) */
+    float cand_depth[5];      /* depths[i] of the refinement */
+    float cand_cost[5];       /* temp_cost */
+    float cand_rcost[5];      /* restricted_temp_cost (prior), else 0 */
+    int32_t cand_accepted[5];
+} hpm_oracle_trace;
+
 typedef struct hpm_oracle_problem {
     int32_t num_images;                      /* 1 reference + sources */
     hpm_camera cams[HPM_MAX_IMAGES];         /* width/height filled */
@@ -44,6 +88,10 @@ typedef struct hpm_oracle_problem {
     const float* coarse;
     int32_t coarse_w, coarse_h;
     float* pre_cost;
+    /* micro-scene hooks (NULL in every ordinary run) */
+    hpm_oracle_cost_fn cost_hook;
+    void* cost_hook_user;
+    hpm_oracle_trace* trace;
 } hpm_oracle_problem;
 
 /* One full RunPatchMatch (HPM.cu:1906-1957): init, 3 red/black iterations,
@@ -59,6 +107,11 @@ int hpm_oracle_init(const hpm_oracle_problem* pb, float* hyp, float* cost, uint3
                     uint64_t* n_eval, int num_threads);
 int hpm_oracle_half_iteration(const hpm_oracle_problem* pb, int iter, int colour, float* hyp,
                               float* cost, uint32_t* selmask, uint64_t* n_eval, int num_threads);
+/* The same half-iteration restricted to the pixels of rows [y0, y1): the
+ * pixels of one colour read only the other colour's state, so a band
+ * gives the whole half-iteration's bits inside it (large views). */
+int hpm_oracle_half_iteration_rows(const hpm_oracle_problem* pb, int iter, int colour, int y0, int y1,
+                                   float* hyp, float* cost, uint32_t* selmask, uint64_t* n_eval, int num_threads);
 int hpm_oracle_finalize(const hpm_oracle_problem* pb, float* hyp, int num_threads);
 int hpm_oracle_filter(const hpm_oracle_problem* pb, int colour, float* hyp, const float* cost,
                       int num_threads);

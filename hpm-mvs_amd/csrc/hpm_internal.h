@@ -132,4 +132,14 @@ int hpm_load_from_pool(hpm_handle* h);
  * scale is 1. */
 int hpm_jbu_prior(hpm_handle* h, const float* src_nd, int sw, int sh, float factor);
 
+/* The learned prior's network over a caller-provided activation area
+ * (hpm_lprior.cpp): bytes needed at width x height (input and head not
+ * included), the device the weights live on, and the forward pass into
+ * `ws` (asynchronous on `stream`). */
+struct hpm_lprior;
+size_t hpm_lprior_activation_bytes(int width, int height);
+int hpm_lprior_device(const hpm_lprior* lp);
+hipError_t hpm_lprior_forward_ws(const hpm_lprior* lp, const void* x, int height, int width, float* head, void* ws,
+                                 hipStream_t stream);
+
 #endif /* HPM_INTERNAL_H_ */

@@ -106,12 +106,14 @@ extern "C" hpm_lprior* hpm_lprior_load(const char* path, int device)
 
 namespace {
 
-/* activation buffers of one forward pass at H x W, packed in one area */
+/* activation buffers of one forward pass at H x W, packed in one area;
+ * with io = false the network input, head and judge counter are left out
+ * (the caller brings them) */
 struct Plan {
     size_t x, t0, cat0, p1, t1, cat1, p2, t2, cat2, p3, t3, b3, d2, d1, d0, head, cnt, total;
 };
 
-Plan plan(int W, int H)
+Plan plan(int W, int H, bool io = true)
 {
     const int w1 = (W + 1) / 2, h1 = (H + 1) / 2;
     const int w2 = (w1 + 1) / 2, h2 = (h1 + 1) / 2;
@@ -124,7 +126,7 @@ Plan plan(int W, int H)
         total += (bytes + 255) & ~(size_t)255;
         return off;
     };
-    pl.x = take(p0 * 32 * 2);
+    pl.x = io ? take(p0 * 32 * 2) : 0;
     pl.t0 = take(p0 * 32 * 2);
     pl.cat0 = take(p0 * 96 * 2);
     pl.p1 = take(p1 * 32 * 2);
@@ -139,8 +141,8 @@ Plan plan(int W, int H)
     pl.d2 = take(p2 * 96 * 2);
     pl.d1 = take(p1 * 64 * 2);
     pl.d0 = take(p0 * 32 * 2);
-    pl.head = take(p0 * 32 * 4);
-    pl.cnt = take(8);
+    pl.head = io ? take(p0 * 32 * 4) : 0;
+    pl.cnt = io ? take(8) : 0;
     pl.total = total;
     return pl;
 }
@@ -197,6 +199,22 @@ hipError_t forward(const hpm_lprior* lp, const void* x, int W, int H, float* hea
 }
 
 } // namespace
+
+size_t hpm_lprior_activation_bytes(int width, int height)
+{
+    return plan(width, height, false).total;
+}
+
+int hpm_lprior_device(const hpm_lprior* lp)
+{
+    return lp->device;
+}
+
+hipError_t hpm_lprior_forward_ws(const hpm_lprior* lp, const void* x, int height, int width, float* head, void* ws,
+                                 hipStream_t stream)
+{
+    return forward(lp, x, width, height, head, (char*)ws, plan(width, height, false), stream);
+}
 
 extern "C" int hpm_lprior_forward(hpm_lprior* lp, const void* x, int height, int width, float* head, void* stream)
 {

@@ -296,6 +296,8 @@ SIGNATURES = {
     "hpm_lprior_apply": (C.c_int, [C.c_void_p, C.c_void_p, _FP]),
     "hpm_lprior_apply_timed": (C.c_int, [C.c_void_p, C.c_void_p, _FP, _FP]),
     "hpm_lprior_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "hpm_lprior_forward_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "hpm_lprior_noalloc_capacity": (C.c_size_t, []),
     # hpm_batch.h
     "hpm_batch_create": (C.c_void_p, [C.c_int]),
     "hpm_batch_destroy": (None, [C.c_void_p]),

@@ -35,6 +35,7 @@ typedef struct hpm_pool hpm_pool;
 #define HPM_EIO (-2)      /* file missing or malformed */
 #define HPM_EDEVICE (-3)  /* HIP runtime error */
 #define HPM_ENOMEM (-4)   /* allocation failure */
+#define HPM_ETOOBIG (-5)  /* input larger than a fixed-size entry point admits */
 
 /* Timing and work counters of the last hpm_run_patchmatch. */
 typedef struct hpm_run_stats {

@@ -11,6 +11,7 @@
 #ifndef HPM_UNET_H_
 #define HPM_UNET_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -64,6 +65,19 @@ int hpm_lprior_apply_timed(hpm_lprior* lp, hpm_handle* h, float* accepted, float
  * height*width*32, as hpm-mvs_amd/unet_prior.py builds them); synchronous,
  * returns a hipError_t value. */
 int hpm_lprior_forward(hpm_lprior* lp, const void* x, int height, int width, float* head, void* stream);
+/* hpm_lprior_forward without any allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The activations live in one fixed array per GPU of
+ * hpm_lprior_noalloc_capacity() bytes (64 MiB, about 552 bytes per pixel:
+ * up to about 120k pixels); a larger input returns HPM_ETOOBIG (-5) and a
+ * null pointer or empty size HPM_EINVAL (-1), else a hipError_t value.
+ * Asynchronous: the kernels are queued on `stream` and head is complete
+ * when the stream reaches them. Every call on one GPU shares that GPU's
+ * array (nothing in it needs to start at zero), so calls that overlap on
+ * one GPU (other streams, other threads) race. */
+int hpm_lprior_forward_noalloc(hpm_lprior* lp, const void* x, int height, int width, float* head, void* stream);
+size_t hpm_lprior_noalloc_capacity(void);
 
 #ifdef __cplusplus
 }

@@ -74,8 +74,16 @@ uint8_t grey_byte(float v)
  * means of the reference pixel's and its consistent sources', the normal
  * through cv::Vec3f /= float, a product with the float reciprocal; a point
  * needs 2 consistent sources) */
+/* colour channel k of pixel p of view v: the given RGB image, else the grey
+ * image in every channel */
+inline float colour_of(const hpm_pool* pool, const uint8_t* const* colors, int v, size_t p, int k)
+{
+    return colors && colors[v] ? (float)colors[v][3 * p + (size_t)k] : (float)grey_byte(pool->images[(size_t)v][p]);
+}
+
 int fusion_run(bool acmm, hpm_pool* pool, int num_views, const int* num_src, const int* src_ids, int max_src,
-               int geom_consistency, hpm_point** points, int64_t* num_points, char* err, int err_len)
+               int geom_consistency, const uint8_t* const* colors, hpm_point** points, int64_t* num_points, char* err,
+               int err_len)
 {
     if (!pool || !num_src || !src_ids || !points || !num_points || num_views < 1 || num_views > pool->n ||
         max_src < 1) {
@@ -185,7 +193,6 @@ int fusion_run(bool acmm, hpm_pool* pool, int num_views, const int* num_src, con
             uy[j] = -1;
         }
         std::vector<uint8_t>& own = masks[(size_t)i];
-        const std::vector<float>& grey = pool->images[(size_t)i];
         for (size_t p = 0; p < npx; ++p) {
             const uint32_t any = h_any[p];
             if (any == 0u || own[p] == 1) {
@@ -196,7 +203,8 @@ int fusion_run(bool acmm, hpm_pool* pool, int num_views, const int* num_src, con
             const int c = (int)(p % (size_t)cols), r = (int)(p / (size_t)cols);
             P3 sx{0.0f, 0.0f, 0.0f};
             float sn[3] = {h_hyp[4 * p], h_hyp[4 * p + 1], h_hyp[4 * p + 2]};
-            float sc = (float)grey_byte(grey[p]);
+            float sc[3] = {colour_of(pool, colors, i, p, 0), colour_of(pool, colors, i, p, 1),
+                           colour_of(pool, colors, i, p, 2)};
             if (acmm) {
                 sx = world_point(c, r, h_depth[p], rc);
             }
@@ -222,7 +230,9 @@ int fusion_run(bool acmm, hpm_pool* pool, int num_views, const int* num_src, con
                     sn[0] = sn[0] + all_hyp[4 * g];
                     sn[1] = sn[1] + all_hyp[4 * g + 1];
                     sn[2] = sn[2] + all_hyp[4 * g + 2];
-                    sc += (float)grey_byte(pool->images[(size_t)sv][(size_t)q]);
+                    for (int k = 0; k < 3; ++k) {
+                        sc[k] += colour_of(pool, colors, sv, (size_t)q, k);
+                    }
                 }
                 dyn += h_w[p * nn + j];
                 ++num_consistent;
@@ -240,7 +250,9 @@ int fusion_run(bool acmm, hpm_pool* pool, int num_views, const int* num_src, con
                     pt.nx = sn[0] * ik;
                     pt.ny = sn[1] * ik;
                     pt.nz = sn[2] * ik;
-                    pt.r = pt.g = pt.b = (uint8_t)(int)(sc / k);
+                    pt.r = (uint8_t)(int)(sc[0] / k);
+                    pt.g = (uint8_t)(int)(sc[1] / k);
+                    pt.b = (uint8_t)(int)(sc[2] / k);
                 } else {
                     const P3 X = world_point(c, r, h_depth[p], rc);
                     pt.x = X.x;
@@ -249,7 +261,9 @@ int fusion_run(bool acmm, hpm_pool* pool, int num_views, const int* num_src, con
                     pt.nx = sn[0];
                     pt.ny = sn[1];
                     pt.nz = sn[2];
-                    pt.r = pt.g = pt.b = grey_byte(grey[p]);
+                    pt.r = (uint8_t)(int)sc[0];
+                    pt.g = (uint8_t)(int)sc[1];
+                    pt.b = (uint8_t)(int)sc[2];
                 }
                 pt.pad_ = 0;
                 out.push_back(pt);
@@ -293,16 +307,24 @@ int fusion_run(bool acmm, hpm_pool* pool, int num_views, const int* num_src, con
 extern "C" int hpm_fusion_run(hpm_pool* pool, int num_views, const int* num_src, const int* src_ids, int max_src,
                               int geom_consistency, hpm_point** points, int64_t* num_points, char* err, int err_len)
 {
-    return fusion_run(false, pool, num_views, num_src, src_ids, max_src, geom_consistency, points, num_points, err,
-                      err_len);
+    return fusion_run(false, pool, num_views, num_src, src_ids, max_src, geom_consistency, nullptr, points,
+                      num_points, err, err_len);
 }
 
 extern "C" int hpm_fusion_run_acmm(hpm_pool* pool, int num_views, const int* num_src, const int* src_ids,
                                    int max_src, int geom_consistency, hpm_point** points, int64_t* num_points,
                                    char* err, int err_len)
 {
-    return fusion_run(true, pool, num_views, num_src, src_ids, max_src, geom_consistency, points, num_points, err,
-                      err_len);
+    return fusion_run(true, pool, num_views, num_src, src_ids, max_src, geom_consistency, nullptr, points,
+                      num_points, err, err_len);
+}
+
+extern "C" int hpm_fusion_run_rgb(hpm_pool* pool, int num_views, const int* num_src, const int* src_ids, int max_src,
+                                  int geom_consistency, const uint8_t* const* colors, int acmm, hpm_point** points,
+                                  int64_t* num_points, char* err, int err_len)
+{
+    return fusion_run(acmm != 0, pool, num_views, num_src, src_ids, max_src, geom_consistency, colors, points,
+                      num_points, err, err_len);
 }
 
 extern "C" void hpm_fusion_free(hpm_point* points)
@@ -373,6 +395,7 @@ int fusion_folder(bool acmm, const char* dense_folder, int geom_consistency, int
     }
     std::vector<hpm_camera> cams((size_t)np);
     std::vector<std::vector<float>> imgs((size_t)np), depth((size_t)np), hyp((size_t)np);
+    std::vector<std::vector<uint8_t>> rgb((size_t)np);
     char path[4096];
     for (int i = 0; i < np; ++i) {
         int w = 0, h = 0;
@@ -385,6 +408,12 @@ int fusion_folder(bool acmm, const char* dense_folder, int geom_consistency, int
         }
         imgs[(size_t)i].resize((size_t)w * h);
         hpm_read_image_grey(path, imgs[(size_t)i].data(), &w, &h);
+        /* the colours, cv::imread(IMREAD_COLOR) (main.cpp:523) */
+        rgb[(size_t)i].resize((size_t)w * h * 3);
+        int cw = w, ch = h;
+        if (hpm_read_image_rgb(path, rgb[(size_t)i].data(), &cw, &ch) != 0 || cw != w || ch != h) {
+            return fail(err, err_len, "fusion: cannot read the colours of image %d", i);
+        }
         snprintf(path, sizeof(path), "%s/cams/%08d_cam.txt", dense_folder, i);
         if (hpm_read_camera(path, &cams[(size_t)i]) != 0) {
             return fail(err, err_len, "fusion: cannot read %s", path);
@@ -403,6 +432,25 @@ int fusion_folder(bool acmm, const char* dense_folder, int geom_consistency, int
         return fail(err, err_len, "fusion: cannot create the device pool");
     }
     int rc = HPM_OK;
+    std::vector<const uint8_t*> cp((size_t)np);
+    for (int i = 0; i < np; ++i) {
+        /* a view the pool downscaled (max_image_size): its colours resized
+         * as RescaleImageAndCamera does (main.cpp:541-542), with the
+         * library's bilinear resize rounded to bytes */
+        const hpm_camera& pc = pool->cams[(size_t)i];
+        if (pc.width != cams[(size_t)i].width || pc.height != cams[(size_t)i].height) {
+            const int w = cams[(size_t)i].width, h = cams[(size_t)i].height;
+            std::vector<float> f((size_t)w * h * 3), g((size_t)pc.width * pc.height * 3);
+            for (size_t k = 0; k < f.size(); ++k) f[k] = (float)rgb[(size_t)i][k];
+            hpm_resize_bilinear_c(f.data(), w, h, 3, g.data(), pc.width, pc.height);
+            rgb[(size_t)i].resize(g.size());
+            for (size_t k = 0; k < g.size(); ++k) {
+                const long v = lrintf(g[k]);
+                rgb[(size_t)i][k] = (uint8_t)(v < 0 ? 0 : v > 255 ? 255 : v);
+            }
+        }
+        cp[(size_t)i] = rgb[(size_t)i].data();
+    }
     for (int i = 0; i < np && rc == HPM_OK; ++i) {
         const hpm_camera& pc = pool->cams[(size_t)i];
         const size_t n = (size_t)pc.width * pc.height;
@@ -433,8 +481,8 @@ int fusion_folder(bool acmm, const char* dense_folder, int geom_consistency, int
     hpm_point* pts = nullptr;
     int64_t n = 0;
     if (rc == HPM_OK) {
-        rc = (acmm ? hpm_fusion_run_acmm : hpm_fusion_run)(pool, np, ns.data(), srcs.data(), HPM_MAX_SOURCES,
-                                                           geom_consistency, &pts, &n, err, err_len);
+        rc = hpm_fusion_run_rgb(pool, np, ns.data(), srcs.data(), HPM_MAX_SOURCES, geom_consistency, cp.data(),
+                                acmm ? 1 : 0, &pts, &n, err, err_len);
     }
     hpm_pool_destroy(pool);
     if (rc == HPM_OK) {

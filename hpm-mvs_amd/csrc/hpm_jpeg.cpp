@@ -16,6 +16,11 @@
  * extra bits of precision between the column and the row pass, rounded
  * right shifts, and the output level-shifted by 128 and clamped to 0..255,
  * as libjpeg's "islow" method computes it.
+ *
+ * The colour image (cv::imread(path, cv::IMREAD_COLOR), read by RunFusion
+ * for the point colours, HPM-MVS/main.cpp:523) reconstructs every
+ * component, upsamples 2x1 / 2x2 subsampled chroma with libjpeg's default
+ * triangle filter and converts YCbCr to RGB in libjpeg's fixed point.
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -589,6 +594,218 @@ extern "C" int hpm_decode_jpeg_grey(const uint8_t* data, size_t size, uint8_t* o
                 }
             }
         }
+    }
+    return 0;
+}
+
+namespace {
+
+/* one component's samples at its own (downsampled) resolution: the inverse
+ * DCT of all its blocks (the padding blocks included, as libjpeg's sample
+ * buffers hold them) */
+std::vector<uint8_t> component_plane(const Decoder& d, const Component& c)
+{
+    const int pw = c.bw * 8;
+    std::vector<uint8_t> plane((size_t)pw * c.bh * 8);
+    for (int by = 0; by < c.bh; ++by) {
+        for (int bx = 0; bx < c.bw; ++bx) {
+            idct_islow(&c.coef[((size_t)by * c.bw + bx) * 64], d.q[c.tq], &plane[(size_t)by * 8 * pw + bx * 8], pw);
+        }
+    }
+    return plane;
+}
+
+/* A chroma component at full resolution, as libjpeg's default ("fancy")
+ * upsampling computes it: a triangle filter, each output sample 3/4 of the
+ * nearer input sample and 1/4 of the farther one (in both directions for
+ * 2x2 subsampling: 9/16, 3/16, 3/16, 1/16), with its rounding biases that
+ * alternate between the two outputs of one input; the first and last
+ * columns take the nearer sample alone, the rows above the first and below
+ * the last real row repeat those rows; components at most two samples
+ * wide are replicated instead. Factors 1x1, 2x1 and 2x2 only. */
+bool upsample(const std::vector<uint8_t>& in, int pw, int cw, int ch, int hx, int vx, int W, int H,
+              std::vector<uint8_t>* out)
+{
+    out->assign((size_t)W * H, 0);
+    auto at = [&](int r, int col) { return (int)in[(size_t)r * pw + col]; };
+    if (hx == 1 && vx == 1) {
+        for (int y = 0; y < H; ++y) {
+            for (int x = 0; x < W; ++x) {
+                (*out)[(size_t)y * W + x] = (uint8_t)at(y, x);
+            }
+        }
+        return true;
+    }
+    if (hx != 2 || (vx != 1 && vx != 2)) {
+        return false;
+    }
+    if (cw <= 2) {
+        /* components at most 2 samples wide are replicated (libjpeg filters
+         * wider ones only) */
+        for (int y = 0; y < H; ++y) {
+            for (int x = 0; x < W; ++x) {
+                (*out)[(size_t)y * W + x] = (uint8_t)at(y / vx, x / 2);
+            }
+        }
+        return true;
+    }
+    std::vector<int> row((size_t)pw + 1), prow((size_t)2 * cw + 2);
+    for (int r = 0; r < ch; ++r) {
+        for (int half = 0; half < vx; ++half) {
+            const int y = r * vx + half;
+            if (y >= H) {
+                break;
+            }
+            /* column sums (2x2: 3 x this row + the row above or below) */
+            if (vx == 1) {
+                for (int col = 0; col <= cw && col < pw; ++col) row[(size_t)col] = at(r, col);
+            } else {
+                const int other = half == 0 ? (r > 0 ? r - 1 : 0) : (r + 1 < ch ? r + 1 : ch - 1);
+                for (int col = 0; col <= cw && col < pw; ++col) row[(size_t)col] = 3 * at(r, col) + at(other, col);
+            }
+            int* o = prow.data();
+            {
+                for (int col = 0; col < cw; ++col) {
+                    const int v = row[(size_t)col];
+                    int left, right;
+                    if (vx == 1) {
+                        left = col == 0 ? v : (v * 3 + row[(size_t)col - 1] + 1) >> 2;
+                        right = col == cw - 1 ? v : (v * 3 + row[(size_t)col + 1] + 2) >> 2;
+                    } else {
+                        left = col == 0 ? (v * 4 + 8) >> 4 : (v * 3 + row[(size_t)col - 1] + 8) >> 4;
+                        right = col == cw - 1 ? (v * 4 + 7) >> 4 : (v * 3 + row[(size_t)col + 1] + 7) >> 4;
+                    }
+                    o[2 * col] = left;
+                    o[2 * col + 1] = right;
+                }
+            }
+            for (int x = 0; x < W; ++x) {
+                (*out)[(size_t)y * W + x] = (uint8_t)o[x];
+            }
+        }
+    }
+    return true;
+}
+
+/* JFIF YCbCr -> RGB in 16-bit fixed point with rounding, clamped to 0..255
+ * (R = Y + 1.402 Cr', G = Y - 0.34414 Cb' - 0.71414 Cr', B = Y + 1.772 Cb',
+ * Cb' = Cb - 128, Cr' = Cr - 128), the integer form libjpeg computes */
+struct YccTables {
+    int cr_r[256], cb_b[256];
+    int32_t cr_g[256], cb_g[256];
+    YccTables()
+    {
+        const int32_t half = 1 << 15;
+        auto fixp = [](double x) { return (int32_t)(x * 65536.0 + 0.5); };
+        for (int i = 0; i < 256; ++i) {
+            const int32_t x = i - 128;
+            cr_r[i] = (int)((fixp(1.40200) * x + half) >> 16);
+            cb_b[i] = (int)((fixp(1.77200) * x + half) >> 16);
+            cr_g[i] = -fixp(0.71414) * x;
+            cb_g[i] = -fixp(0.34414) * x + half;
+        }
+    }
+};
+
+inline uint8_t clamp255(int v)
+{
+    return (uint8_t)(v < 0 ? 0 : v > 255 ? 255 : v);
+}
+
+} // namespace
+
+extern "C" int hpm_decode_jpeg_rgb(const uint8_t* data, size_t size, uint8_t* out, int* width, int* height)
+{
+    if (!data || !width || !height) {
+        return -1;
+    }
+    Decoder d;
+    d.p = data;
+    d.end = data + size;
+    if (!d.parse()) {
+        return -1;
+    }
+    const int W = d.width, H = d.height;
+    if (!out) {
+        *width = W;
+        *height = H;
+        return 0;
+    }
+    if (*width != W || *height != H) {
+        return -1;
+    }
+    const size_t np = (size_t)W * H;
+    std::vector<uint8_t> planes[3];
+    for (size_t k = 0; k < d.comps.size(); ++k) {
+        const Component& c = d.comps[k];
+        if (d.hmax % c.h != 0 || d.vmax % c.v != 0) {
+            return -1;
+        }
+        const int cw = (W * c.h + d.hmax - 1) / d.hmax, ch = (H * c.v + d.vmax - 1) / d.vmax;
+        if (!upsample(component_plane(d, c), c.bw * 8, cw, ch, d.hmax / c.h, d.vmax / c.v, W, H, &planes[k])) {
+            return -1;
+        }
+    }
+    if (d.comps.size() == 1) {
+        for (size_t i = 0; i < np; ++i) {
+            out[3 * i] = out[3 * i + 1] = out[3 * i + 2] = planes[0][i];
+        }
+        return 0;
+    }
+    static const YccTables t;
+    for (size_t i = 0; i < np; ++i) {
+        const int y = planes[0][i], cb = planes[1][i], cr = planes[2][i];
+        out[3 * i] = clamp255(y + t.cr_r[cr]);
+        out[3 * i + 1] = clamp255(y + (int)((t.cb_g[cb] + t.cr_g[cr]) >> 16));
+        out[3 * i + 2] = clamp255(y + t.cb_b[cb]);
+    }
+    return 0;
+}
+
+extern "C" int hpm_read_image_rgb(const char* path, uint8_t* data, int* width, int* height)
+{
+    if (!path || !width || !height) {
+        return -1;
+    }
+    FILE* f = fopen(path, "rb");
+    if (!f) {
+        return -1;
+    }
+    std::vector<uint8_t> buf;
+    uint8_t chunk[65536];
+    size_t n;
+    while ((n = fread(chunk, 1, sizeof(chunk), f)) > 0) {
+        buf.insert(buf.end(), chunk, chunk + n);
+    }
+    fclose(f);
+    if (buf.size() >= 2 && buf[0] == 0xFF && buf[1] == 0xD8) {
+        int w = *width, h = *height;
+        if (!data) {
+            return hpm_decode_jpeg_rgb(buf.data(), buf.size(), nullptr, width, height);
+        }
+        if (hpm_decode_jpeg_rgb(buf.data(), buf.size(), nullptr, &w, &h) != 0) {
+            return -1;
+        }
+        *width = w;
+        *height = h;
+        return hpm_decode_jpeg_rgb(buf.data(), buf.size(), data, &w, &h);
+    }
+    /* an 8-bit grey PGM: the grey value in all three channels */
+    int w = 0, h = 0;
+    if (hpm_read_pgm_f32(path, nullptr, &w, &h) != 0) {
+        return -1;
+    }
+    *width = w;
+    *height = h;
+    if (!data) {
+        return 0;
+    }
+    std::vector<float> g((size_t)w * h);
+    if (hpm_read_pgm_f32(path, g.data(), &w, &h) != 0) {
+        return -1;
+    }
+    for (size_t i = 0; i < g.size(); ++i) {
+        data[3 * i] = data[3 * i + 1] = data[3 * i + 2] = (uint8_t)g[i];
     }
     return 0;
 }

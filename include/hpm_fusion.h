@@ -29,7 +29,8 @@ typedef struct hpm_point {
  * num_views. Depths are the geometric-consistency maps when
  * geom_consistency is nonzero (depths_geom.dmb), the plain-pass maps
  * otherwise (depths.dmb); normals are the last pass's (normals.dmb);
- * colours are the pool's grey images. The points, in the reference's
+ * colours are the pool's grey images (hpm_fusion_run_rgb takes colour
+ * images). The points, in the reference's
  * order (views in turn, pixels row-major), are returned in *points (free
  * with hpm_fusion_free) and their number in *num_points. The consistency
  * tests of every (pixel, source) pair run on the device; the masks, whose
@@ -46,10 +47,18 @@ int hpm_fusion_run(hpm_pool* pool, int num_views, const int* num_src, const int*
  * does, ACMM.cpp:396-398). */
 int hpm_fusion_run_acmm(hpm_pool* pool, int num_views, const int* num_src, const int* src_ids, int max_src,
                         int geom_consistency, hpm_point** points, int64_t* num_points, char* err, int err_len);
+/* Either rule (acmm nonzero: ACMM+NESP's) with the point colours from
+ * colors[i], width * height RGB byte triples of pool view i at the pool's
+ * size (the reference's IMREAD_COLOR image, main.cpp:523, 573); a NULL
+ * array or entry takes the grey image. */
+int hpm_fusion_run_rgb(hpm_pool* pool, int num_views, const int* num_src, const int* src_ids, int max_src,
+                       int geom_consistency, const uint8_t* const* colors, int acmm, hpm_point** points,
+                       int64_t* num_points, char* err, int err_len);
 void hpm_fusion_free(hpm_point* points);
 
 /* RunFusion as the reference's main calls it (main.cpp:675): reads
- * pair.txt, cams/%08d_cam.txt, images/%08d.jpg or .pgm (grey, as colour) and
+ * pair.txt, cams/%08d_cam.txt, images/%08d.jpg or .pgm (grey for the
+ * consistency tests' cameras, colour for the points: hpm_read_image_rgb) and
  * HPM/2333_%08d/{depths_geom.dmb or depths.dmb, normals.dmb} of every view
  * (views 0..n-1 in pair.txt order), fuses them with hpm_fusion_run and
  * writes HPM/HPM_model.ply. *num_points (may be NULL) receives the point

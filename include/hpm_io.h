@@ -56,6 +56,17 @@ int hpm_read_image_grey(const char* path, float* data, int* width, int* height);
  * NULL returns the size. */
 int hpm_decode_jpeg_grey(const uint8_t* data, size_t size, uint8_t* out, int* width, int* height);
 
+/* cv::imread(path, cv::IMREAD_COLOR) (main.cpp:523, RunFusion's colours)
+ * for JPEG and PGM files: width * height RGB byte triples (the reference's
+ * BGR Mat in RGB order). JPEG chroma subsampled 2x1 or 2x2 is upsampled
+ * with libjpeg's default ("fancy") triangle filter, YCbCr converted in its
+ * fixed point; grey JPEGs and PGMs give the grey value in all channels.
+ * data == NULL reads only the size. Returns 0 or -1. */
+int hpm_read_image_rgb(const char* path, uint8_t* data, int* width, int* height);
+/* The same decode of a JPEG in memory into width * height * 3 bytes; out ==
+ * NULL returns the size. */
+int hpm_decode_jpeg_rgb(const uint8_t* data, size_t size, uint8_t* out, int* width, int* height);
+
 /* cv::resize(..., INTER_LINEAR) of a float image: half-pixel centres,
  * src = (dst + 0.5) * (sw / dw) - 0.5, edge-clamped, exact float weights.
  * Used for the >max_image_size downscale (HPM.cpp:524-551) and the prior

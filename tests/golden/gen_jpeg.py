@@ -2,7 +2,9 @@
 by libjpeg-turbo through Pillow (the JPEG codec OpenCV also builds on; the
 reference reads its images with cv::imread(IMREAD_GRAYSCALE),
 HPM-MVS/HPM.cpp:492), and for each the grey image libjpeg decodes
-(grayscale output: the luminance component, integer inverse DCT), as PGM.
+(grayscale output: the luminance component, integer inverse DCT), as PGM,
+and the colour image it decodes (RGB output, default upsampling, as
+cv::imread(IMREAD_COLOR) asks for in RunFusion, main.cpp:523), as PPM.
 A progressive file has no expected image: the decoder must reject it.
 
     python tests/golden/gen_jpeg.py
@@ -59,6 +61,10 @@ def main():
         with open(os.path.join(HERE, name + ".pgm"), "wb") as f:
             f.write(b"P5\n%d %d\n255\n" % (g.shape[1], g.shape[0]))
             f.write(g.astype(np.uint8).tobytes())
+        c = np.asarray(Image.open(io.BytesIO(data)).convert("RGB"))
+        with open(os.path.join(HERE, name + ".ppm"), "wb") as f:
+            f.write(b"P6\n%d %d\n255\n" % (c.shape[1], c.shape[0]))
+            f.write(c.astype(np.uint8).tobytes())
         print(name, len(data), "bytes", g.shape)
 
 

@@ -72,3 +72,44 @@ def test_decode_random_vs_libjpeg(hpm, tmp_path):
         want = np.asarray(ref)
         got = hpm.read_image_grey(str(path))
         assert np.array_equal(got, want.astype(np.float32)), (k, h, w, opts)
+
+
+def _ppm(path):
+    data = open(path, "rb").read()
+    parts = data.split(b"\n", 3)
+    w, h = map(int, parts[1].split())
+    return np.frombuffer(parts[3], np.uint8).reshape(h, w, 3)
+
+
+COLOUR_CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.ppm")))
+
+
+@pytest.mark.parametrize("name", COLOUR_CASES)
+def test_colour_decode_matches_libjpeg(hpm, name):
+    """cv::imread(IMREAD_COLOR) of RunFusion (main.cpp:523): libjpeg's RGB
+    output (default upsampling of 2x1 / 2x2 chroma, fixed-point YCbCr
+    conversion), byte-exact; grey files give the grey value per channel."""
+    got = hpm.read_image_rgb(os.path.join(HERE, name + ".jpg"))
+    want = _ppm(os.path.join(HERE, name + ".ppm"))
+    assert got.shape == want.shape
+    diff = np.abs(got.astype(int) - want.astype(int))
+    assert diff.max() == 0, f"{name}: {int((diff > 0).any(-1).sum())} pixels differ, max {diff.max()}"
+
+
+def test_colour_decode_random_files(hpm, tmp_path):
+    """40 random colour images in every subsampling Pillow writes, odd
+    sizes, qualities 30-100, against libjpeg-turbo's RGB decode."""
+    Image = pytest.importorskip("PIL.Image")
+    import io
+
+    rng = np.random.default_rng(12)
+    for k in range(40):
+        h, w = int(rng.integers(1, 60)), int(rng.integers(1, 60))
+        img = (rng.integers(0, 256, (h, w, 3)) * rng.uniform(0, 1, (1, 1, 3))).astype(np.uint8)
+        buf = io.BytesIO()
+        Image.fromarray(img).save(buf, "JPEG", quality=int(rng.integers(30, 101)), subsampling=int(k % 3))
+        path = tmp_path / f"{k}.jpg"
+        path.write_bytes(buf.getvalue())
+        want = np.asarray(Image.open(io.BytesIO(buf.getvalue())).convert("RGB"))
+        got = hpm.read_image_rgb(str(path))
+        assert np.array_equal(got, want), (k, h, w, k % 3)

@@ -84,34 +84,104 @@ def view_order(ref, n):
     return [ref] + sorted([i for i in range(n) if i != ref], key=lambda i: (abs(i - ref), i))
 
 
-def cpu_baseline(cams, imgs, ref, evals_per_view, host_ms_per_view, threads):
-    """The CPU restatement (oracle) on a bounded sample of the same workload:
-    initialisation plus the first black half-iteration of one reference view
-    at full size. Seconds per NCC evaluation x the evaluations of a whole
-    view (all its RunPatchMatch calls), plus the host prior construction the
-    CPU run shares with the device run, gives ms per view."""
+def cpu_baseline(hpm, cams, imgs, gd, ref, phase_evals, host_ms_per_view, threads):
+    """The CPU restatement (oracle) timed on measured phases of reference
+    view `ref` of the same scene (the whole plain method of one view is
+    about a minute of CPU work, so two of its phases are sampled):
+      - the plain RunPatchMatch, whole (init, 3 red/black iterations,
+        finalize, filter);
+      - the first half-iteration of a planar-prior RunPatchMatch, with the
+        1x planar prior built from that plain pass by the host construction;
+      - the first half-iteration of a geometric-consistency RunPatchMatch,
+        with the scene's ground-truth depth maps as the previous depths.
+    Each phase gives seconds per NCC evaluation of its kind. ms per view =
+    the plain pass as measured + the device schedule's evaluations of the
+    prior passes and of the geometric passes (phase_evals, per view) at
+    their measured rates + the host prior construction."""
     import numpy as np
-    from oracle import cpu
+    from oracle import cpu, pipeline
 
     order = view_order(ref, len(cams))
     cs = [cams[i] for i in order]
     ims = [imgs[i].astype(np.float32) for i in order]
-    o = cpu.Oracle(cs, ims, cpu.relaxed_params(cs), salt=ref, threads=threads)
+    p = cpu.relaxed_params(cs)
+    o = cpu.Oracle(cs, ims, p, salt=ref, threads=threads)
     t0 = time.perf_counter()
-    n = o.init()
-    n += o.half_iteration(0, 0)
-    dt = time.perf_counter() - t0
-    sec_per_eval = dt / max(n, 1)
+    o.run_patchmatch()
+    t_plain = time.perf_counter() - t0
+    n_plain = o.n_eval
+    log(f"cpu: plain RunPatchMatch {t_plain:.1f} s, {n_plain} evaluations")
+    planes, pmask = pipeline.prior_scale_maps(cs[0], np.ascontiguousarray(o.hyp[..., 3]), o.cost.copy(), ims[0], 1.0,
+                                              8, np.float32(p.depth_min), np.float32(p.depth_max), threads)
+    hyp0, cost0 = o.hyp.copy(), o.cost.copy()
+    op = cpu.Oracle(cs, ims, p, salt=ref, threads=threads)
+    op.pb.params.planar_prior = 1
+    op.set_prior(planes, pmask)
+    op.hyp[...], op.cost[...] = hyp0, cost0
+    op.init()
+    t0 = time.perf_counter()
+    n_prior = op.half_iteration(0, 0)
+    t_prior = time.perf_counter() - t0
+    log(f"cpu: prior half-iteration {t_prior:.1f} s, {n_prior} evaluations")
+    pg = cpu.relaxed_params(cs)
+    pg.geom_consistency = 1
+    og = cpu.Oracle(cs, ims, pg, depths=[gd[i] for i in order], salt=ref, threads=threads)
+    og.hyp[...], og.cost[...] = hyp0, cost0
+    og.init()
+    t0 = time.perf_counter()
+    n_geom = og.half_iteration(0, 0)
+    t_geom = time.perf_counter() - t0
+    log(f"cpu: geometric half-iteration {t_geom:.1f} s, {n_geom} evaluations")
+    r_prior, r_geom = t_prior / max(n_prior, 1), t_geom / max(n_geom, 1)
+    ms = (t_plain + r_prior * phase_evals["prior"] + r_geom * phase_evals["geom"]) * 1e3 + host_ms_per_view
     return {
-        "value": round(sec_per_eval * evals_per_view * 1e3 + host_ms_per_view, 1),
+        "value": round(ms, 1),
         "unit": "ms/ref view",
         "cores": threads,
         "kind": "port",
         "cpu_model": cpu_model(),
-        "sample": f"init + iteration-0 black half of view {ref} at 1600x1066 with 10 sources "
-                  f"({n} NCC evaluations in {dt:.1f} s), scaled to the {evals_per_view} evaluations of a whole "
-                  f"view, plus {host_ms_per_view:.0f} ms of host prior construction",
+        "sample": (f"view {ref} at 1600x1066 with 10 sources: the plain RunPatchMatch in full ({n_plain} NCC "
+                   f"evaluations, {t_plain:.2f} s); a planar-prior half-iteration ({n_prior} evaluations, "
+                   f"{t_prior:.2f} s) and a geometric-consistency half-iteration ({n_geom} evaluations, "
+                   f"{t_geom:.2f} s), their rates applied to the device schedule's {phase_evals['prior']} prior-pass "
+                   f"and {phase_evals['geom']} geometric-pass evaluations per view; plus {host_ms_per_view:.0f} ms "
+                   f"of host prior construction"),
+        "phases_s": {"plain_pass": round(t_plain, 3), "prior_half_iteration": round(t_prior, 3),
+                     "geometric_half_iteration": round(t_geom, 3)},
     }
+
+
+def cpu_c1(hpm, threads):
+    """The 640x480, 4-source workload (one plain RunPatchMatch) entirely on
+    the CPU restatement, on `threads` threads and on one, beside the same
+    RunPatchMatch on the GPU."""
+    import numpy as np
+    from oracle import cpu
+
+    cfg = hpm.workload_config("c1")
+    cams, imgs, gd, gn, low = hpm.render_scene(cfg)
+    order = view_order(2, cfg.num_views)
+    cs = [cams[i] for i in order]
+    ims = [imgs[i].astype(np.float32) for i in order]
+    pr = hpm.Problem(0)
+    pr.input_init_memory(cs, ims, ref_image_id=2)
+    pr.CudaSpaceInitialization()
+    pr.RunPatchMatch()  # warm-up
+    gpu_ms = []
+    for _ in range(5):
+        pr.RunPatchMatch()
+        gpu_ms.append(pr.run_stats().ms_total)
+    params = pr.params()
+    pr.close()
+    out = {"gpu_ms": round(float(np.median(gpu_ms)), 3)}
+    for t in (threads, 1):
+        o = cpu.Oracle(cs, ims, params, salt=2, threads=t)
+        t0 = time.perf_counter()
+        o.run_patchmatch()
+        out[f"cpu_ms_{t}_threads"] = round((time.perf_counter() - t0) * 1e3, 1)
+        log(f"cpu: 640x480 RunPatchMatch on {t} threads {out[f'cpu_ms_{t}_threads']} ms")
+    out["speedup_vs_cpu"] = round(out[f"cpu_ms_{threads}_threads"] / out["gpu_ms"], 1)
+    return out
 
 
 def batch_bench(hpm, args):
@@ -179,6 +249,7 @@ def batch_bench(hpm, args):
             "ms_one_at_a_time": round(ms_seq, 3),
             "speedup_vs_one_at_a_time": round(ms_seq / ms_batch, 3),
             "grid": st.grid,
+            "items_per_pull": st.items,
             "autotune_log": batch.log().strip(),
         },
     }
@@ -409,11 +480,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workers", type=int, default=0, help="host workers of the schedule (0: library default)")
-    ap.add_argument("--learned-prior", default=os.path.join(ROOT, "hpm-mvs_amd", "unet_prior.bin"),
-                    help="U-Net weight file of the learned-prior schedule ('none': time the planar prior only)")
+    ap.add_argument("--learned-prior", default=os.path.join(ROOT, "hpm-mvs_amd", "unet_prior_plain.bin"),
+                    help="U-Net weight file of the learned-prior schedule, trained on the plain method's maps "
+                         "('none': time the planar prior only)")
+    ap.add_argument("--learned-prior-gt", default=os.path.join(ROOT, "hpm-mvs_amd", "unet_prior.bin"),
+                    help="a second U-Net weight file, trained on ground-truth maps, timed beside it ('none': skip)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="time the learned-prior schedules on per-view workers only, not in the batched form")
     ap.add_argument("--learned-geom-iterations", default="0,1,3",
-                    help="geometric-consistency pass counts of the learned-prior schedules timed, comma-separated "
-                         "(the plain method runs 3)")
+                    help="geometric-consistency pass counts of the learned-prior schedules timed, comma-separated")
+    ap.add_argument("--plain-geom-iterations", default="0,1,3",
+                    help="geometric-consistency pass counts of the plain-method schedules timed, comma-separated; "
+                         "the accuracy bar is the plain method's with 3 passes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB on the device 3 times (the known-size kernel that scales the FETCH_SIZE "
                          "and WRITE_SIZE counters) and exit")
@@ -460,7 +538,9 @@ def main():
     problems = [(r, view_order(r, n)[1:]) for r in range(n)]
     log("images resident on the device")
 
-    def options(learned, geom_iterations=3):
+    batch = hpm.Batch(0)
+
+    def options(learned, geom_iterations=3, batched=False):
         opt = hpm.pipeline_options()
         opt.params.knn_k = 8
         opt.params.geom_iterations = geom_iterations
@@ -471,7 +551,9 @@ def main():
         if args.workers > 0:
             opt.workers = args.workers
         if learned:
-            opt.learned_prior = os.path.abspath(args.learned_prior).encode()
+            opt.learned_prior = os.path.abspath(learned).encode()
+        if batched:
+            opt.batch = batch._b
         return opt
 
     def time_method(name, opt):
@@ -513,18 +595,49 @@ def main():
         log(f"{name}: {r['ms_per_view']} ms/view, accuracy {r['accuracy_2pct']}")
         return r
 
-    learned = args.workload in ("full", "large") and args.learned_prior.lower() != "none"
-    plain = time_method("planar prior", options(False))
-    variants = {}
-    if learned:
-        for g in sorted({int(x) for x in str(args.learned_geom_iterations).split(",") if x.strip()}):
-            variants[g] = time_method(f"learned prior, {g} GC passes", options(True, g))
-    # the metric: ms per view at equal or better depth accuracy than the
-    # plain method: the fastest learned-prior schedule that meets the plain
-    # method's accuracy, else the plain method
-    meeting = [g for g, r in variants.items() if r["accuracy_2pct"] >= plain["accuracy_2pct"]]
-    best_g = min(meeting, key=lambda g: variants[g]["ms_per_view"]) if meeting else None
-    chosen = variants[best_g] if best_g is not None else plain
+    def pass_counts(text):
+        return sorted({int(x) for x in str(text).split(",") if x.strip()})
+
+    full_method = args.workload in ("full", "large")
+    plain_variants = {}
+    for g in (pass_counts(args.plain_geom_iterations) if full_method else [3]):
+        plain_variants[g] = time_method(f"planar prior, {g} GC passes", options(False, g))
+    if 3 not in plain_variants:
+        plain_variants[3] = time_method("planar prior, 3 GC passes", options(False, 3))
+    plain = plain_variants[3]
+    # learned-prior families: (name, weight file, batched form); the first
+    # is the one the metric names: the network trained on the plain
+    # method's maps, in the batched form
+    families = []
+    if full_method and args.learned_prior.lower() != "none":
+        if not args.no_batched:
+            families.append(("learned_prior_batched", args.learned_prior, True))
+        families.append(("learned_prior", args.learned_prior, False))
+    if full_method and args.learned_prior_gt.lower() != "none" and os.path.exists(args.learned_prior_gt):
+        families.append(("learned_prior_gt_labels_batched" if not args.no_batched else "learned_prior_gt_labels",
+                         args.learned_prior_gt, not args.no_batched))
+    fam_variants = {}
+    for fam, path, batched in families:
+        fam_variants[fam] = {g: time_method(f"{fam}, {g} GC passes", options(path, g, batched))
+                             for g in pass_counts(args.learned_geom_iterations)}
+    # the metric: ms per view at equal or better depth accuracy. The bar is
+    # the plain method with 3 geometric passes; each method's value is its
+    # fastest schedule (0, 1 or 3 passes) that meets the bar.
+    bar = plain["accuracy_2pct"]
+
+    def fastest(variants):
+        ok = [g for g, r in variants.items() if r["accuracy_2pct"] >= bar]
+        return min(ok, key=lambda g: variants[g]["ms_per_view"]) if ok else None
+
+    plain_best = fastest(plain_variants)
+    value_fam = families[0][0] if families else None
+    best_g = fastest(fam_variants[value_fam]) if value_fam else None
+    if best_g is not None:
+        chosen = fam_variants[value_fam][best_g]
+        value_method = f"{value_fam}_{best_g}_gc_passes"
+    else:
+        chosen = plain_variants[plain_best]
+        value_method = f"plain_gpu_planar_prior_{plain_best}_gc_passes"
     ms_per_step = chosen["ms_per_step"]
     ms_per_view = chosen["ms_per_view"]
 
@@ -545,18 +658,22 @@ def main():
 
     if args.workload in ("full", "large"):
         scales = "0.25x and 0.5x planar priors with JBU" if large else "0.5x planar prior with JBU"
-        workload = ("per reference view: plain RunPatchMatch, then the prior (learned: U-Net proposal + NCC judge, "
-                    "one prior pass, planar prior as fallback, then "
-                    f"{best_g if best_g is not None else '-'} geometric-consistency passes, the fewest of "
-                    f"{sorted(variants)} that meet the plain method's accuracy; plain method: {scales} "
-                    "and 1x planar prior with KNN K=8, then 3 geometric-consistency passes); "
+        workload = ("per reference view: plain RunPatchMatch, then the prior (learned, batched form: every view's "
+                    "plain pass in one batch, U-Net trained on the plain method's maps proposing depth, normal "
+                    "and confidence, NCC judge, every view's prior pass in one batch, planar prior as fallback, "
+                    "then the fewest of 0, 1 or 3 geometric-consistency passes that meet the bar; plain method: "
+                    f"{scales} and 1x planar prior with KNN K=8, then 0, 1 or 3 geometric-consistency passes); "
+                    "bar: the plain method's accuracy with 3 passes; "
                     f"{cfg.width}x{cfg.height}, {n - 1} sources, 11x11 window stride 2")
     else:
         workload = ("one plain RunPatchMatch per reference view (no prior, no geometric consistency); 1600x1066, "
                     "10 sources")
-    methods = {"plain_gpu_planar_prior": {k: v for k, v in plain.items() if k != "ms_per_step"}}
-    for g, r in variants.items():
-        methods[f"learned_prior_{g}_gc_passes"] = {k: v for k, v in r.items() if k != "ms_per_step"}
+    methods = {}
+    for g, r in plain_variants.items():
+        methods[f"plain_gpu_planar_prior_{g}_gc_passes"] = {k: v for k, v in r.items() if k != "ms_per_step"}
+    for fam, variants in fam_variants.items():
+        for g, r in variants.items():
+            methods[f"{fam}_{g}_gc_passes"] = {k: v for k, v in r.items() if k != "ms_per_step"}
     out = {
         "metric": METRIC_LARGE if large else METRIC,
         "value": round(ms_per_view, 3),
@@ -576,10 +693,11 @@ def main():
             "width": cfg.width,
             "height": cfg.height,
             "sources": n - 1,
-            "value_method": (f"learned_prior_{best_g}_gc_passes" if best_g is not None
-                             else "plain_gpu_planar_prior"),
+            "value_method": value_method,
             "accuracy_2pct": chosen["accuracy_2pct"],
-            "speedup_vs_plain": round(plain["ms_per_view"] / ms_per_view, 3),
+            "accuracy_bar": bar,
+            "plain_method_best": f"plain_gpu_planar_prior_{plain_best}_gc_passes",
+            "speedup_vs_plain": round(plain_variants[plain_best]["ms_per_view"] / ms_per_view, 3),
             "methods": methods,
         },
         "roofline": {
@@ -599,9 +717,19 @@ def main():
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or (os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads ...")
-        out["cpu_baseline"] = cpu_baseline(cams, imgs, 5, plain["n_eval_per_view"],
-                                           plain["ms_per_view_prior_host"], threads)
+        # the plain method's evaluations per view by phase, on the device:
+        # the plain pass (the roofline run above), phase A (plain pass and
+        # prior passes, no geometric pass) and the whole schedule
+        n_plain_dev = rst.n_eval / n
+        ast = hpm.run_pipeline(pool, problems, options(False, 0))
+        n_a_dev = ast.n_eval / n
+        phase_evals = {"plain": int(n_plain_dev), "prior": int(n_a_dev - n_plain_dev),
+                       "geom": int(plain["n_eval_per_view"] - n_a_dev)}
+        log(f"cpu baseline on {threads} threads, device evaluations per view by phase {phase_evals} ...")
+        out["cpu_baseline"] = cpu_baseline(hpm, cams, imgs, gd, 5, phase_evals, plain["ms_per_view_prior_host"],
+                                           threads)
+        out["cpu_baseline"]["c1_640x480_one_pass"] = cpu_c1(hpm, threads)
+    batch.close()
     pool.close()
     print(json.dumps(out), flush=True)
 

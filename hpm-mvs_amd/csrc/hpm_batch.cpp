@@ -5,11 +5,15 @@
  *
  * All problems advance through the same schedule (init, 3 x red/black
  * propagation, finalize, red/black filter); each stage is one launch of a
- * fixed grid of resident blocks that walk a table of 64x4 tiles covering
- * every problem (hpm_launch_batch). The 11 launches are captured once per
+ * grid of resident blocks that walk a table of tiles covering every problem
+ * (hpm_launch_batch), in contiguous ranges or pulling work items from
+ * per-XCD queues. The 11 launches are captured once per
  * batch shape in a hipGraph and replayed; the problem table (pointers,
- * sizes, RNG call index) is uploaded before every replay. The grid size is
- * chosen by timing candidates on the first call of a shape, with the
+ * sizes, RNG call index) is uploaded before every replay; up to 8 shapes
+ * keep their graphs (a pass schedule alternates between the shapes of its
+ * passes). The schedule (static ranges over a grid of 2-16 blocks per CU,
+ * or work queues with items of 1, 2 or 4 tiles) is chosen by timing the
+ * candidates on the first call of a shape, with the
  * problems' state saved before and restored after every trial, so tuning
  * leaves the results untouched. Per pixel the work is that of
  * hpm_run_patchmatch, so every problem's maps equal an unbatched run's.
@@ -26,25 +30,34 @@
 #include "hpm_batch.h"
 #include "hpm_internal.h"
 
+/* one batch shape (problem count and sizes): its device tables and the
+ * captured schedule, whose launches point into these tables */
+struct hpm_batch_shape {
+    std::vector<int> key;
+    hpm_dev_problem* d_pbs = nullptr;      /* [2n]: init counters, then propagation counters */
+    unsigned long long* d_neval = nullptr; /* [2n] */
+    hpm_tile* d_tiles = nullptr;           /* full-grid tiles, then colour-grid tiles */
+    unsigned long long* d_queue = nullptr; /* 8 u32 work-queue counters per launch, zeroed per run */
+    hipGraphExec_t exec = nullptr;
+    int grid = 0;
+    int gran = 0;                          /* 0: static ranges; else tiles per work item */
+    int n_full = 0, n_colour = 0;
+    uint64_t last_use = 0;
+};
+
 struct hpm_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::string err;
     std::string log;
-    /* device tables */
-    hpm_dev_problem* d_pbs = nullptr; /* [2n]: init counters, then propagation counters */
-    hpm_tile* d_tiles = nullptr;      /* full-grid tiles, then colour-grid tiles */
-    unsigned long long* d_neval = nullptr;
+    std::vector<hpm_batch_shape> shapes;
+    uint64_t clock = 0;
     float* d_save = nullptr;          /* state saved around tuning trials */
-    size_t cap_pbs = 0, cap_tiles = 0, cap_neval = 0, cap_save = 0;
+    size_t cap_save = 0;
     hpm_dev_problem* h_pbs = nullptr; /* pinned */
     size_t cap_h_pbs = 0;
-    /* captured schedule and the shape it was built for */
-    hipGraphExec_t exec = nullptr;
-    std::vector<int> key;
-    int grid = 0;
-    int n_full = 0, n_colour = 0;
+    int fixed_grid = 0, fixed_items = -1; /* hpm_batch_set_schedule; -1: autotune */
 };
 
 namespace {
@@ -81,13 +94,12 @@ int grow(hpm_batch* b, T** p, size_t* cap, size_t count)
 /* tiles of the full grid (x in [0, W), y in [0, H); 64x4) and of the colour
  * grid (x = 2i + parity, rows below min(H, 2 yy_limit); HPM_COLOUR_TILE_X x
  * HPM_COLOUR_TILE_Y) of every problem */
-void make_tiles(const std::vector<hpm_dev_problem>& pbs, int n, std::vector<hpm_tile>* full,
-                std::vector<hpm_tile>* colour)
+void make_tiles(const hpm_dev_problem* pbs, int n, std::vector<hpm_tile>* full, std::vector<hpm_tile>* colour)
 {
     full->clear();
     colour->clear();
     for (int i = 0; i < n; ++i) {
-        const hpm_dev_problem& pb = pbs[(size_t)i];
+        const hpm_dev_problem& pb = pbs[i];
         for (int by = 0; by < (pb.H + 3) / 4; ++by) {
             for (int bx = 0; bx < (pb.W + 63) / 64; ++bx) {
                 full->push_back({i, bx, by, 0});
@@ -103,24 +115,43 @@ void make_tiles(const std::vector<hpm_dev_problem>& pbs, int n, std::vector<hpm_
     }
 }
 
-/* the 11 launches of one schedule */
-int enqueue(hpm_batch* b, int n, int grid)
+const size_t kShapes = 8;
+
+void free_shape(hpm_batch_shape* sh)
 {
-    const hpm_tile* tf = b->d_tiles;
-    const hpm_tile* tc = b->d_tiles + b->n_full;
-    const hpm_dev_problem* p_init = b->d_pbs;
-    const hpm_dev_problem* p_prop = b->d_pbs + n;
-    BHIP(b, (hipError_t)hpm_launch_zero_u64(b->d_neval, 2 * n, b->stream));
-    BHIP(b, (hipError_t)hpm_launch_batch(HPM_STAGE_INIT, p_init, tf, b->n_full, grid, 0, 0, b->stream));
+    if (sh->exec) (void)hipGraphExecDestroy(sh->exec);
+    if (sh->d_pbs) (void)hipFree(sh->d_pbs);
+    if (sh->d_neval) (void)hipFree(sh->d_neval);
+    if (sh->d_tiles) (void)hipFree(sh->d_tiles);
+    if (sh->d_queue) (void)hipFree(sh->d_queue);
+    *sh = hpm_batch_shape();
+}
+
+const int kLaunches = 11;
+
+/* the 11 launches of one schedule */
+int enqueue(hpm_batch* b, const hpm_batch_shape& sh, int n, int grid, int gran)
+{
+    const hpm_tile* tf = sh.d_tiles;
+    const hpm_tile* tc = sh.d_tiles + sh.n_full;
+    const hpm_dev_problem* p_init = sh.d_pbs;
+    const hpm_dev_problem* p_prop = sh.d_pbs + n;
+    unsigned int* q = (unsigned int*)sh.d_queue;
+    int li = 0;
+    auto stage = [&](int st, const hpm_dev_problem* pbs, const hpm_tile* tiles, int nt, int it, int colour) {
+        return (hipError_t)hpm_launch_batch(st, pbs, tiles, nt, grid, it, colour, gran, q + 8 * li++, b->stream);
+    };
+    BHIP(b, (hipError_t)hpm_launch_zero_u64(sh.d_neval, 2 * n, b->stream));
+    BHIP(b, (hipError_t)hpm_launch_zero_u64(sh.d_queue, 4 * kLaunches, b->stream));
+    BHIP(b, stage(HPM_STAGE_INIT, p_init, tf, sh.n_full, 0, 0));
     for (int it = 0; it < 3; ++it) { /* hard-coded 3 (HPM.cu:1933) */
         for (int colour = 0; colour < 2; ++colour) {
-            BHIP(b, (hipError_t)hpm_launch_batch(HPM_STAGE_PROPAGATE, p_prop, tc, b->n_colour, grid, it, colour,
-                                                 b->stream));
+            BHIP(b, stage(HPM_STAGE_PROPAGATE, p_prop, tc, sh.n_colour, it, colour));
         }
     }
-    BHIP(b, (hipError_t)hpm_launch_batch(HPM_STAGE_FINALIZE, p_prop, tf, b->n_full, grid, 0, 0, b->stream));
-    BHIP(b, (hipError_t)hpm_launch_batch(HPM_STAGE_FILTER, p_prop, tc, b->n_colour, grid, 0, 0, b->stream));
-    BHIP(b, (hipError_t)hpm_launch_batch(HPM_STAGE_FILTER, p_prop, tc, b->n_colour, grid, 0, 1, b->stream));
+    BHIP(b, stage(HPM_STAGE_FINALIZE, p_prop, tf, sh.n_full, 0, 0));
+    BHIP(b, stage(HPM_STAGE_FILTER, p_prop, tc, sh.n_colour, 0, 0));
+    BHIP(b, stage(HPM_STAGE_FILTER, p_prop, tc, sh.n_colour, 0, 1));
     return HPM_OK;
 }
 
@@ -147,6 +178,11 @@ int save_state(hpm_batch* b, hpm_handle* const* hs, int n, bool restore)
 
 } // namespace
 
+void hpm_make_tiles(const hpm_dev_problem* pbs, int n, std::vector<hpm_tile>* full, std::vector<hpm_tile>* colour)
+{
+    make_tiles(pbs, n, full, colour);
+}
+
 extern "C" hpm_batch* hpm_batch_create(int device)
 {
     hpm_batch* b = new hpm_batch();
@@ -169,12 +205,9 @@ extern "C" void hpm_batch_destroy(hpm_batch* b)
     if (b->stream) {
         (void)hipStreamSynchronize(b->stream);
     }
-    if (b->exec) {
-        (void)hipGraphExecDestroy(b->exec);
+    for (hpm_batch_shape& sh : b->shapes) {
+        free_shape(&sh);
     }
-    if (b->d_pbs) (void)hipFree(b->d_pbs);
-    if (b->d_tiles) (void)hipFree(b->d_tiles);
-    if (b->d_neval) (void)hipFree(b->d_neval);
     if (b->d_save) (void)hipFree(b->d_save);
     if (b->h_pbs) (void)hipHostFree(b->h_pbs);
     for (hipEvent_t e : b->ev) {
@@ -184,6 +217,20 @@ extern "C" void hpm_batch_destroy(hpm_batch* b)
         (void)hipStreamDestroy(b->stream);
     }
     delete b;
+}
+
+extern "C" int hpm_batch_set_schedule(hpm_batch* b, int grid, int items)
+{
+    if (!b || !(items == -1 || items == 0 || items == 1 || items == 2 || items == 4) || grid < 0) {
+        return HPM_EINVAL;
+    }
+    b->fixed_grid = grid;
+    b->fixed_items = items;
+    for (hpm_batch_shape& sh : b->shapes) {
+        free_shape(&sh);
+    }
+    b->shapes.clear();
+    return HPM_OK;
 }
 
 extern "C" const char* hpm_batch_last_error(const hpm_batch* b)
@@ -227,10 +274,33 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         key.push_back(pbs[(size_t)i].H);
         save_floats += (size_t)pbs[(size_t)i].W * pbs[(size_t)i].H * 6;
     }
+    /* the shape's entry: a new one gets its tables, the oldest is evicted */
+    hpm_batch_shape* sh = nullptr;
+    for (hpm_batch_shape& x : b->shapes) {
+        if (x.key == key) {
+            sh = &x;
+        }
+    }
+    const bool reused = sh != nullptr;
+    if (!sh) {
+        if (b->shapes.size() >= kShapes) {
+            auto old = std::min_element(b->shapes.begin(), b->shapes.end(),
+                                        [](const hpm_batch_shape& a, const hpm_batch_shape& c) {
+                                            return a.last_use < c.last_use;
+                                        });
+            free_shape(&*old);
+            b->shapes.erase(old);
+        }
+        b->shapes.emplace_back();
+        sh = &b->shapes.back();
+        sh->key = key;
+        BHIP(b, hipMalloc((void**)&sh->d_pbs, 2 * (size_t)n * sizeof(hpm_dev_problem)));
+        BHIP(b, hipMalloc((void**)&sh->d_neval, 2 * (size_t)n * sizeof(unsigned long long)));
+        BHIP(b, hipMalloc((void**)&sh->d_queue, 4 * (size_t)kLaunches * sizeof(unsigned long long)));
+    }
+    sh->last_use = ++b->clock;
+    int rc = HPM_OK;
     /* problem tables: init counts into 2i, propagation into 2i + 1 */
-    int rc = grow(b, &b->d_neval, &b->cap_neval, 2 * (size_t)n);
-    if (rc == HPM_OK) rc = grow(b, &b->d_pbs, &b->cap_pbs, 2 * (size_t)n);
-    if (rc != HPM_OK) return rc;
     if (b->cap_h_pbs < 2 * (size_t)n) {
         if (b->h_pbs) (void)hipHostFree(b->h_pbs);
         b->h_pbs = nullptr;
@@ -238,62 +308,75 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         b->cap_h_pbs = 2 * (size_t)n;
     }
     for (int i = 0; i < n; ++i) {
-        pbs[(size_t)i].n_eval = b->d_neval + 2 * i;
+        pbs[(size_t)i].n_eval = sh->d_neval + 2 * i;
         b->h_pbs[i] = pbs[(size_t)i];
         b->h_pbs[n + i] = pbs[(size_t)i];
-        b->h_pbs[n + i].n_eval = b->d_neval + 2 * i + 1;
+        b->h_pbs[n + i].n_eval = sh->d_neval + 2 * i + 1;
     }
-    BHIP(b, hipMemcpyAsync(b->d_pbs, b->h_pbs, 2 * (size_t)n * sizeof(hpm_dev_problem), hipMemcpyHostToDevice,
+    BHIP(b, hipMemcpyAsync(sh->d_pbs, b->h_pbs, 2 * (size_t)n * sizeof(hpm_dev_problem), hipMemcpyHostToDevice,
                            b->stream));
     /* every handle's earlier work (uploads, reloads) precedes the batch */
     for (int i = 0; i < n; ++i) {
         BHIP(b, hipStreamSynchronize(hs[i]->stream));
     }
 
-    bool reused = b->exec && key == b->key;
     if (!reused) {
-        if (b->exec) {
-            BHIP(b, hipGraphExecDestroy(b->exec));
-            b->exec = nullptr;
-        }
         std::vector<hpm_tile> full, colour;
-        make_tiles(pbs, n, &full, &colour);
-        b->n_full = (int)full.size();
-        b->n_colour = (int)colour.size();
-        rc = grow(b, &b->d_tiles, &b->cap_tiles, full.size() + colour.size());
-        if (rc != HPM_OK) return rc;
-        BHIP(b, hipMemcpyAsync(b->d_tiles, full.data(), full.size() * sizeof(hpm_tile), hipMemcpyHostToDevice,
+        make_tiles(pbs.data(), n, &full, &colour);
+        sh->n_full = (int)full.size();
+        sh->n_colour = (int)colour.size();
+        BHIP(b, hipMalloc((void**)&sh->d_tiles, (full.size() + colour.size()) * sizeof(hpm_tile)));
+        BHIP(b, hipMemcpyAsync(sh->d_tiles, full.data(), full.size() * sizeof(hpm_tile), hipMemcpyHostToDevice,
                                b->stream));
-        BHIP(b, hipMemcpyAsync(b->d_tiles + full.size(), colour.data(), colour.size() * sizeof(hpm_tile),
+        BHIP(b, hipMemcpyAsync(sh->d_tiles + full.size(), colour.data(), colour.size() * sizeof(hpm_tile),
                                hipMemcpyHostToDevice, b->stream));
         BHIP(b, hipStreamSynchronize(b->stream));
-        /* autotuner: resident blocks per stage launch; 2 blocks of 256
-         * lanes fit a CU (LDS), more blocks balance uneven tiles */
+        /* autotuner over the schedule: static ranges for grids of 2-16
+         * blocks per CU (2 blocks of 256 lanes fit a CU, LDS-bound; more
+         * blocks balance uneven tiles through the hardware scheduler), or
+         * one resident grid pulling work items of 1, 2 or 4 tiles from the
+         * per-XCD queues */
         hipDeviceProp_t prop;
         BHIP(b, hipGetDeviceProperties(&prop, b->device));
         const int cus = std::max(1, prop.multiProcessorCount);
-        const int cands[4] = {2 * cus, 4 * cus, 8 * cus, 16 * cus};
+        struct Cand {
+            int grid, gran;
+        };
+        const Cand cands[7] = {{2 * cus, 0}, {4 * cus, 0}, {8 * cus, 0}, {16 * cus, 0},
+                               {2 * cus, 1}, {2 * cus, 2}, {2 * cus, 4}};
         rc = grow(b, &b->d_save, &b->cap_save, save_floats);
         if (rc != HPM_OK) return rc;
         rc = save_state(b, hs, n, false);
         if (rc != HPM_OK) return rc;
         char line[256];
-        snprintf(line, sizeof(line), "batch of %d problems, %d full / %d colour tiles, %d CUs:", n, b->n_full,
-                 b->n_colour, cus);
+        snprintf(line, sizeof(line), "batch of %d problems, %d full / %d colour tiles, %d CUs:", n, sh->n_full,
+                 sh->n_colour, cus);
         std::string log = line;
         float best_ms = -1.0f;
-        int best = cands[0];
-        for (int c : cands) {
+        Cand best = cands[0];
+        const bool fixed = b->fixed_items >= 0;
+        if (fixed) {
+            best = Cand{b->fixed_grid > 0 ? b->fixed_grid : 2 * cus, b->fixed_items};
+            log += " fixed schedule;";
+        }
+        for (const Cand& c : cands) {
+            if (fixed) {
+                break;
+            }
             rc = save_state(b, hs, n, true);
             if (rc != HPM_OK) return rc;
             BHIP(b, hipEventRecord(b->ev[0], b->stream));
-            rc = enqueue(b, n, c);
+            rc = enqueue(b, *sh, n, c.grid, c.gran);
             if (rc != HPM_OK) return rc;
             BHIP(b, hipEventRecord(b->ev[1], b->stream));
             BHIP(b, hipEventSynchronize(b->ev[1]));
             float ms = 0.0f;
             BHIP(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
-            snprintf(line, sizeof(line), " grid %d: %.3f ms;", c, ms);
+            if (c.gran) {
+                snprintf(line, sizeof(line), " queue items of %d tiles, grid %d: %.3f ms;", c.gran, c.grid, ms);
+            } else {
+                snprintf(line, sizeof(line), " static grid %d: %.3f ms;", c.grid, ms);
+            }
             log += line;
             if (best_ms < 0.0f || ms < best_ms) {
                 best_ms = ms;
@@ -302,30 +385,31 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         }
         rc = save_state(b, hs, n, true);
         if (rc != HPM_OK) return rc;
-        snprintf(line, sizeof(line), " chosen grid %d\n", best);
+        snprintf(line, sizeof(line), " chosen %s grid %d items %d\n", best.gran ? "queue" : "static", best.grid,
+                 best.gran);
         log += line;
         b->log += log;
-        b->grid = best;
+        sh->grid = best.grid;
+        sh->gran = best.gran;
         /* capture the schedule */
         hipGraph_t graph = nullptr;
         BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-        rc = enqueue(b, n, b->grid);
+        rc = enqueue(b, *sh, n, sh->grid, sh->gran);
         hipError_t ce = hipStreamEndCapture(b->stream, &graph);
         if (rc != HPM_OK) {
             if (graph) (void)hipGraphDestroy(graph);
             return rc;
         }
         BHIP(b, ce);
-        hipError_t ie = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+        hipError_t ie = hipGraphInstantiate(&sh->exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         BHIP(b, ie);
-        b->key = key;
     }
     BHIP(b, hipEventRecord(b->ev[0], b->stream));
-    BHIP(b, hipGraphLaunch(b->exec, b->stream));
+    BHIP(b, hipGraphLaunch(sh->exec, b->stream));
     BHIP(b, hipEventRecord(b->ev[1], b->stream));
     std::vector<unsigned long long> nev(2 * (size_t)n, 0ull);
-    BHIP(b, hipMemcpyAsync(nev.data(), b->d_neval, nev.size() * 8, hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipMemcpyAsync(nev.data(), sh->d_neval, nev.size() * 8, hipMemcpyDeviceToHost, b->stream));
     if (want_readback) {
         for (int i = 0; i < n; ++i) {
             hpm_handle* h = hs[i];
@@ -355,9 +439,10 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         stats->ms_host = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host).count();
         stats->n_eval = total;
         stats->problems = n;
-        stats->tiles_full = b->n_full;
-        stats->tiles_colour = b->n_colour;
-        stats->grid = b->grid;
+        stats->tiles_full = sh->n_full;
+        stats->tiles_colour = sh->n_colour;
+        stats->grid = sh->grid;
+        stats->items = sh->gran;
         stats->graph_reused = reused ? 1 : 0;
     }
     return HPM_OK;

@@ -123,15 +123,17 @@ int hpm_launch_jbu_prior(const float* guide, int w, int h, const float* src_nd, 
                          void* stream);
 
 /* One stage of the batched schedule over a tile table: `grid` resident
- * blocks each take a contiguous range of the ntiles tiles. pbs and tiles
- * are device arrays. */
+ * blocks; gran 0: each takes a contiguous range of the ntiles tiles; gran
+ * 1, 2 or 4: they pull work items of gran tiles from per-XCD queues
+ * (queue: 8 zeroed counters for this launch). pbs, tiles and queue are
+ * device arrays. */
 int hpm_launch_batch(int stage, const hpm_dev_problem* pbs, const hpm_tile* tiles, int ntiles, int grid, int iter,
-                     int colour, void* stream);
+                     int colour, int gran, unsigned int* queue, void* stream);
 /* learned prior: network input (x: W*H*32 BF16) of a finished plain pass,
  * then prediction + judge into prior planes / masks (head: W*H*32 FP32) */
 int hpm_launch_lprior_features(const hpm_dev_problem* pb, void* x, void* stream);
 int hpm_launch_lprior_judge(const hpm_dev_problem* pb, const void* x, const float* head, float* planes,
-                            uint32_t* mask, unsigned long long* accepted, void* stream);
+                            uint32_t* mask, unsigned long long* accepted, int use_conf, void* stream);
 /* Fusion (main.cpp:547-604): the consistency tests of reference view ref
  * against its sources, one lane per reference pixel. rec_pix[p * nn + j]
  * receives the source pixel of source j when the pair passes the

@@ -101,6 +101,11 @@ int hpm_fail(hpm_handle* h, int code, const char* fmt, ...);
         }                                                                                           \
     } while (0)
 
+/* The tiles of the batched and work-queue schedules: 64x4 tiles of every
+ * problem's full grid (init, finalize) and colour-grid tiles (propagate,
+ * filter), problem by problem, rows outer (hpm_batch.cpp). */
+void hpm_make_tiles(const hpm_dev_problem* pbs, int n, std::vector<hpm_tile>* full, std::vector<hpm_tile>* colour);
+
 /* The device problem description of a ready handle (its pool, views, state,
  * call index), and the readiness check of RunPatchMatch. */
 void hpm_fill_problem(const hpm_handle* h, hpm_dev_problem* pb);

@@ -1647,7 +1647,7 @@ __global__ __launch_bounds__(256) void hpm_k_lprior_features(hpm_dev_problem pb,
  * hypothesis at the pixel. accepted[0] counts kept pixels. */
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_lprior_judge(
     hpm_dev_problem pb, const __bf16* __restrict__ x, const float* __restrict__ head, float* planes, uint32_t* mask,
-    unsigned long long* accepted)
+    unsigned long long* accepted, int use_conf)
 {
     int bx, by;
     xcd_block(&bx, &by);
@@ -1665,7 +1665,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAV
         float4 n = make_float4((float)xi[2] + hi[1], (float)xi[3] + hi[2], (float)xi[4] + hi[3], 0.0f);
         const float len2 = n.x * n.x + n.y * n.y + n.z * n.z;
         float4 prop = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        bool ok = len2 > 1e-12f && isfinite(len2);
+        /* head channel 4, when trained, is the logit of the proposal being
+         * right: below 0 (probability 0.5) the proposal is not judged */
+        bool ok = len2 > 1e-12f && isfinite(len2) && (!use_conf || hi[4] > 0.0f);
         if (ok) {
             normalize3(&n);
             const float4 vd = view_direction(rc, px, py, depth);
@@ -1701,11 +1703,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAV
 }
 
 /* ---------------------------------------------------------- batched form
- * Several independent problems in one launch per stage: a fixed grid of
- * resident blocks walks a table of 64x4 tiles (problem, tile x, tile y).
- * Logical block l (XCD-renumbered) takes the contiguous tile range
- * [l n / G, (l + 1) n / G), so each XCD works through a contiguous stretch
- * of problems and rows. Per pixel the work is the unbatched kernels' own. */
+ * Several independent problems in one launch per stage: a grid of resident
+ * blocks walks a table of tiles (problem, tile x, tile y). Two schedules,
+ * chosen by the autotuner:
+ *  - static: logical block l (XCD-renumbered) takes the contiguous tile
+ *    range [l n / G, (l + 1) n / G), so each XCD works through a
+ *    contiguous stretch of problems and rows;
+ *  - work queue: the tiles are cut into 8 contiguous bands, one per XCD;
+ *    a block pulls work items of kGran consecutive tiles from its XCD's
+ *    band with an atomic counter and, once that band is empty, from the
+ *    next XCDs' bands in turn, so uneven tiles balance while each XCD's L2
+ *    keeps its band. Every block leaves once every band is empty.
+ * Per pixel the work is the unbatched kernels' own. */
 
 __device__ __forceinline__ hpm_tile load_tile(const hpm_tile* tiles, int t)
 {
@@ -1729,12 +1738,53 @@ __device__ __forceinline__ void tile_range(int ntiles, int* first, int* last)
     *last = (int)((int64_t)(l + 1) * ntiles / g);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_init_batch(
-    const hpm_dev_problem* __restrict__ pbs, const hpm_tile* __restrict__ tiles, int ntiles)
+/* Calls body(t) for every tile t this block takes under the schedule:
+ * kGran == 0 the static ranges, else work items of kGran tiles from the
+ * per-XCD queues (queue: kXcds zeroed counters of this launch). The loop
+ * condition is uniform across the block (the item goes through LDS). */
+template <int kGran, typename F>
+__device__ __forceinline__ void for_tiles(int ntiles, unsigned int* queue, F&& body)
 {
-    int first, last;
-    tile_range(ntiles, &first, &last);
-    for (int t = first; t < last; ++t) {
+    if (kGran == 0) {
+        int first, last;
+        tile_range(ntiles, &first, &last);
+        for (int t = first; t < last; ++t) {
+            body(t);
+        }
+        return;
+    }
+    __shared__ int s_item;
+    const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
+    const int home = (int)blockIdx.x % kXcds;
+    for (int k = 0; k < kXcds; ++k) {
+        const int band = (home + k) % kXcds;
+        const int b0 = (int)((int64_t)band * ntiles / kXcds);
+        const int b1 = (int)((int64_t)(band + 1) * ntiles / kXcds);
+        const int items = (b1 - b0 + kGran - 1) / kGran;
+        for (;;) {
+            __syncthreads(); /* the previous item's s_item reads are done */
+            if (leader) {
+                s_item = (int)atomicAdd(&queue[band], 1u);
+            }
+            __syncthreads();
+            const int item = s_item;
+            if (item >= items) {
+                break;
+            }
+            const int t0 = b0 + item * kGran;
+            const int t1 = min(t0 + kGran, b1);
+            for (int t = t0; t < t1; ++t) {
+                body(t);
+            }
+        }
+    }
+}
+
+template <int kGran>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_init_batch(
+    const hpm_dev_problem* __restrict__ pbs, const hpm_tile* __restrict__ tiles, int ntiles, unsigned int* queue)
+{
+    for_tiles<kGran>(ntiles, queue, [&](int t) {
         const hpm_tile tl = load_tile(tiles, t);
         const hpm_dev_problem& pb = pbs[tl.problem];
         const int px = tl.bx * blockDim.x + threadIdx.x;
@@ -1744,30 +1794,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAV
             nev = init_pixel(pb, px, py);
         }
         count_evals(pb, nev);
-    }
+    });
 }
 
+template <int kGran>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_propagate_batch(
-    const hpm_dev_problem* __restrict__ pbs, const hpm_tile* __restrict__ tiles, int ntiles, int iter, int colour)
+    const hpm_dev_problem* __restrict__ pbs, const hpm_tile* __restrict__ tiles, int ntiles, int iter, int colour,
+    unsigned int* queue)
 {
-    int first, last;
-    tile_range(ntiles, &first, &last);
-    for (int t = first; t < last; ++t) {
+    for_tiles<kGran>(ntiles, queue, [&](int t) {
         const hpm_tile tl = load_tile(tiles, t);
         const hpm_dev_problem& pb = pbs[tl.problem];
         int px, py;
         const bool valid = colour_pixel_at(pb, colour, tl.bx, tl.by, &px, &py);
         const uint32_t nev = propagate_pixel(pb, px, py, iter, colour, valid);
         count_evals(pb, nev);
-    }
+    });
 }
 
+template <int kGran>
 __global__ __launch_bounds__(256) void hpm_k_finalize_batch(const hpm_dev_problem* __restrict__ pbs,
-                                                            const hpm_tile* __restrict__ tiles, int ntiles)
+                                                            const hpm_tile* __restrict__ tiles, int ntiles,
+                                                            unsigned int* queue)
 {
-    int first, last;
-    tile_range(ntiles, &first, &last);
-    for (int t = first; t < last; ++t) {
+    for_tiles<kGran>(ntiles, queue, [&](int t) {
         const hpm_tile tl = load_tile(tiles, t);
         const hpm_dev_problem& pb = pbs[tl.problem];
         const int px = tl.bx * blockDim.x + threadIdx.x;
@@ -1775,22 +1825,22 @@ __global__ __launch_bounds__(256) void hpm_k_finalize_batch(const hpm_dev_proble
         if (px < pb.W && py < pb.H) {
             finalize_pixel(pb, px, py);
         }
-    }
+    });
 }
 
+template <int kGran>
 __global__ __launch_bounds__(256) void hpm_k_filter_batch(const hpm_dev_problem* __restrict__ pbs,
-                                                          const hpm_tile* __restrict__ tiles, int ntiles, int colour)
+                                                          const hpm_tile* __restrict__ tiles, int ntiles, int colour,
+                                                          unsigned int* queue)
 {
-    int first, last;
-    tile_range(ntiles, &first, &last);
-    for (int t = first; t < last; ++t) {
+    for_tiles<kGran>(ntiles, queue, [&](int t) {
         const hpm_tile tl = load_tile(tiles, t);
         const hpm_dev_problem& pb = pbs[tl.problem];
         int px, py;
         if (colour_pixel_at(pb, colour, tl.bx, tl.by, &px, &py)) {
             filter_pixel(pb, px, py);
         }
-    }
+    });
 }
 
 /* zeroes n counters (a kernel, so the schedule can be captured in a graph) */
@@ -1972,29 +2022,48 @@ extern "C" int hpm_launch_jbu(const float* guide, int w, int h, const float* src
     return (int)hipGetLastError();
 }
 
+namespace {
+
+template <int kGran>
+void launch_batch_stage(int stage, dim3 g, dim3 blk, hipStream_t s, const hpm_dev_problem* pbs,
+                        const hpm_tile* tiles, int ntiles, int iter, int colour, unsigned int* queue)
+{
+    switch (stage) {
+    case HPM_STAGE_INIT:
+        hipLaunchKernelGGL(hpm_k_init_batch<kGran>, g, blk, 0, s, pbs, tiles, ntiles, queue);
+        break;
+    case HPM_STAGE_PROPAGATE:
+        hipLaunchKernelGGL(hpm_k_propagate_batch<kGran>, g, blk, 0, s, pbs, tiles, ntiles, iter, colour, queue);
+        break;
+    case HPM_STAGE_FINALIZE:
+        hipLaunchKernelGGL(hpm_k_finalize_batch<kGran>, g, blk, 0, s, pbs, tiles, ntiles, queue);
+        break;
+    default:
+        hipLaunchKernelGGL(hpm_k_filter_batch<kGran>, g, blk, 0, s, pbs, tiles, ntiles, colour, queue);
+        break;
+    }
+}
+
+} // namespace
+
 extern "C" int hpm_launch_batch(int stage, const hpm_dev_problem* pbs, const hpm_tile* tiles, int ntiles, int grid,
-                                int iter, int colour, void* stream)
+                                int iter, int colour, int gran, unsigned int* queue, void* stream)
 {
     if (ntiles <= 0) {
         return 0;
     }
+    if (stage < HPM_STAGE_INIT || stage > HPM_STAGE_FILTER || (gran != 0 && !queue)) {
+        return (int)hipErrorInvalidValue;
+    }
     const dim3 blk(64, 4, 1);
     const dim3 g(grid > 0 ? grid : 1, 1, 1);
-    switch (stage) {
-    case HPM_STAGE_INIT:
-        hipLaunchKernelGGL(hpm_k_init_batch, g, blk, 0, (hipStream_t)stream, pbs, tiles, ntiles);
-        break;
-    case HPM_STAGE_PROPAGATE:
-        hipLaunchKernelGGL(hpm_k_propagate_batch, g, blk, 0, (hipStream_t)stream, pbs, tiles, ntiles, iter, colour);
-        break;
-    case HPM_STAGE_FINALIZE:
-        hipLaunchKernelGGL(hpm_k_finalize_batch, g, blk, 0, (hipStream_t)stream, pbs, tiles, ntiles);
-        break;
-    case HPM_STAGE_FILTER:
-        hipLaunchKernelGGL(hpm_k_filter_batch, g, blk, 0, (hipStream_t)stream, pbs, tiles, ntiles, colour);
-        break;
-    default:
-        return (int)hipErrorInvalidValue;
+    const hipStream_t s = (hipStream_t)stream;
+    switch (gran) {
+    case 0: launch_batch_stage<0>(stage, g, blk, s, pbs, tiles, ntiles, iter, colour, queue); break;
+    case 1: launch_batch_stage<1>(stage, g, blk, s, pbs, tiles, ntiles, iter, colour, queue); break;
+    case 2: launch_batch_stage<2>(stage, g, blk, s, pbs, tiles, ntiles, iter, colour, queue); break;
+    case 4: launch_batch_stage<4>(stage, g, blk, s, pbs, tiles, ntiles, iter, colour, queue); break;
+    default: return (int)hipErrorInvalidValue;
     }
     return (int)hipGetLastError();
 }
@@ -2008,11 +2077,11 @@ extern "C" int hpm_launch_lprior_features(const hpm_dev_problem* pb, void* x, vo
 }
 
 extern "C" int hpm_launch_lprior_judge(const hpm_dev_problem* pb, const void* x, const float* head, float* planes,
-                                       uint32_t* mask, unsigned long long* accepted, void* stream)
+                                       uint32_t* mask, unsigned long long* accepted, int use_conf, void* stream)
 {
     const dim3 blk(64, 4, 1);
     hipLaunchKernelGGL(hpm_k_lprior_judge, full_grid(pb->W, pb->H, blk), blk, 0, (hipStream_t)stream, *pb,
-                       (const __bf16*)x, head, planes, mask, accepted);
+                       (const __bf16*)x, head, planes, mask, accepted, use_conf);
     return (int)hipGetLastError();
 }
 

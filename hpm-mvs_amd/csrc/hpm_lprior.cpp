@@ -43,6 +43,7 @@ enum { E0A, E0B, E1A, E1B, E2A, E2B, E3A, E3B, D2, D1, D0, HEAD };
 
 struct hpm_lprior {
     int device = 0;
+    int has_conf = 0; /* head channel 4 is a confidence logit (weight file HPMUNET2) */
     Layer layers[kLayers];
     std::string err;
 };
@@ -74,8 +75,10 @@ extern "C" hpm_lprior* hpm_lprior_load(const char* path, int device)
     bool ok = hipSetDevice(lp->device) == hipSuccess;
     char magic[8];
     int32_t n = 0;
-    ok = ok && fread(magic, 1, 8, f) == 8 && memcmp(magic, "HPMUNET1", 8) == 0 && fread(&n, 4, 1, f) == 1 &&
+    ok = ok && fread(magic, 1, 8, f) == 8 &&
+         (memcmp(magic, "HPMUNET1", 8) == 0 || memcmp(magic, "HPMUNET2", 8) == 0) && fread(&n, 4, 1, f) == 1 &&
          n == kLayers;
+    lp->has_conf = ok && memcmp(magic, "HPMUNET2", 8) == 0;
     for (int i = 0; ok && i < kLayers; ++i) {
         int32_t hdr[4];
         ok = fread(hdr, 4, 4, f) == 4 && hdr[0] == kShape[i][0] && hdr[1] == kShape[i][1] &&
@@ -276,7 +279,7 @@ extern "C" int hpm_lprior_apply_timed(hpm_lprior* lp, hpm_handle* h, float* acce
     if (e == hipSuccess) e = mark(2);
     /* judge into the handle's prior */
     if (e == hipSuccess) e = (hipError_t)hpm_launch_zero_u64(cnt, 1, s);
-    if (e == hipSuccess) e = (hipError_t)hpm_launch_lprior_judge(&pb, base + pl.x, head, h->d_prior, h->d_mask, cnt, s);
+    if (e == hipSuccess) e = (hipError_t)hpm_launch_lprior_judge(&pb, base + pl.x, head, h->d_prior, h->d_mask, cnt, lp->has_conf, s);
     if (e == hipSuccess) e = mark(3);
     if (e == hipSuccess) e = hipMemcpyAsync(&kept, cnt, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);

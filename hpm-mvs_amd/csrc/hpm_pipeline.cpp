@@ -19,6 +19,7 @@
 #include <thread>
 #include <vector>
 
+#include "hpm_batch.h"
 #include "hpm_internal.h"
 #include "hpm_io.h"
 #include "hpm_prior.h"
@@ -70,6 +71,16 @@ void add_ms(Ctx& c, double* field, double ms)
 {
     std::lock_guard<std::mutex> lk(c.mu);
     *field += ms;
+}
+
+/* one hpm_run_batch call over n problems */
+void account_batch(Ctx& c, const hpm_batch_stats& bs, int n)
+{
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.st->ms_device += bs.ms_device;
+    c.st->n_eval += bs.n_eval;
+    c.st->runs += n;
+    c.st->batches += 1;
 }
 
 void account(Ctx& c, const hpm_handle* h)
@@ -166,7 +177,8 @@ void build_prior_scale(Ctx& c, const hpm_camera& ref, const float* depth, const 
 }
 
 /* ProcessProblem (main.cpp:35-499) for phase A. */
-int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
+/* ProcessProblem up to the plain pass: the handle set up for it */
+int phase_a_setup(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
 {
     hpm_params p = c.opt->params;
     p.planar_prior = 0;
@@ -190,43 +202,48 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
                    "hierarchy input");
         if (rc < 0) return rc;
     }
-    /* the host builds the planar prior from the maps it reads back */
-    const bool host_prior = c.opt->planar_prior != 0 && !c.opt->prior_on_device;
-    rc = check(c, h, hpm_run_internal(h, host_prior), "RunPatchMatch");
+    return HPM_OK;
+}
+
+/* the learned prior on a finished plain pass (network + judge); *use tells
+ * whether the view keeps it for one prior pass, else the handle is left for
+ * the planar prior */
+int learned_apply(Ctx& c, hpm_handle* h, int ref, bool* use)
+{
+    auto t0 = clk::now();
+    float kept = 0.0f;
+    int rc = check(c, h, hpm_lprior_apply(c.lprior, h, &kept), "learned prior");
     if (rc < 0) return rc;
-    account(c, h);
-    const hpm_camera ref_cam = h->cams[0];
-    const int W = ref_cam.width, H = ref_cam.height;
-    const float dmin = h->params.depth_min, dmax = h->params.depth_max;
-    if (c.opt->planar_prior && c.lprior) {
-        /* learned prior: network + judge on the plain pass, one prior pass */
-        auto t0 = clk::now();
-        float kept = 0.0f;
-        rc = check(c, h, hpm_lprior_apply(c.lprior, h, &kept), "learned prior");
-        if (rc < 0) return rc;
-        add_ms(c, &c.st->ms_prior_device, ms_since(t0));
-        {
-            std::lock_guard<std::mutex> lk(c.mu);
-            c.st->learned_accept += kept;
-            if (kept >= c.opt->learned_min_accept) {
-                c.st->learned_views += 1;
-            } else {
-                c.st->learned_fallbacks += 1;
-            }
+    add_ms(c, &c.st->ms_prior_device, ms_since(t0));
+    *use = kept >= c.opt->learned_min_accept;
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        c.st->learned_accept += kept;
+        if (*use) {
+            c.st->learned_views += 1;
+        } else {
+            c.st->learned_fallbacks += 1;
         }
-        if (c.opt->verbose >= 2) {
-            fprintf(stderr, "view %d learned prior: %.3f of the pixels kept\n", ref, kept);
-        }
-        if (kept >= c.opt->learned_min_accept) {
-            rc = check(c, h, hpm_run_internal(h, false), "RunPatchMatch (learned prior)");
-            if (rc < 0) return rc;
-            account(c, h);
-            return check(c, h, hpm_store_to_pool(h, 0), "store");
-        }
-        /* fallback: the planar prior below, from the plain pass */
+    }
+    if (c.opt->verbose >= 2) {
+        fprintf(stderr, "view %d learned prior: %.3f of the pixels kept\n", ref, kept);
+    }
+    if (!*use) {
+        /* fallback: the planar prior, from the plain pass */
         h->params.planar_prior = 0;
         h->have_prior = false;
     }
+    return HPM_OK;
+}
+
+/* ProcessProblem after the plain pass: the planar prior passes (device or
+ * host construction), then the maps go to the pool */
+int phase_a_planar(Ctx& c, hpm_handle* h, int ref)
+{
+    int rc = HPM_OK;
+    const hpm_camera ref_cam = h->cams[0];
+    const int W = ref_cam.width, H = ref_cam.height;
+    const float dmin = h->params.depth_min, dmax = h->params.depth_max;
     if (c.opt->planar_prior && c.opt->prior_on_device) {
         hpm_set_planar_prior(h); /* SetPlanarPriorParams (HPM.cpp:477-480) */
         int k = 0;
@@ -361,8 +378,32 @@ int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
     return check(c, h, hpm_store_to_pool(h, 0), "store");
 }
 
-/* ProcessProblem with geometric consistency (main.cpp:44-51, 481-490) */
-int process_phase_b(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc, int pass)
+int process_phase_a(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc)
+{
+    int rc = phase_a_setup(c, h, ref, srcs, nsrc);
+    if (rc < 0) return rc;
+    /* the host builds the planar prior from the maps it reads back */
+    const bool host_prior = c.opt->planar_prior != 0 && !c.opt->prior_on_device;
+    rc = check(c, h, hpm_run_internal(h, host_prior), "RunPatchMatch");
+    if (rc < 0) return rc;
+    account(c, h);
+    if (c.opt->planar_prior && c.lprior) {
+        bool use = false;
+        rc = learned_apply(c, h, ref, &use);
+        if (rc < 0) return rc;
+        if (use) {
+            rc = check(c, h, hpm_run_internal(h, false), "RunPatchMatch (learned prior)");
+            if (rc < 0) return rc;
+            account(c, h);
+            return check(c, h, hpm_store_to_pool(h, 0), "store");
+        }
+    }
+    return phase_a_planar(c, h, ref);
+}
+
+/* ProcessProblem with geometric consistency (main.cpp:44-51, 481-490) up
+ * to its RunPatchMatch */
+int phase_b_setup(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc, int pass)
 {
     const bool multi = pass > 0;
     hpm_params p = c.opt->params;
@@ -378,10 +419,32 @@ int process_phase_b(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc, i
     rc = check(c, h, hpm_load_from_pool(h), "reload");
     if (rc < 0) return rc;
     h->call_index = (uint32_t)c.opt->stream_base + 8u + (uint32_t)pass;
+    return HPM_OK;
+}
+
+int process_phase_b(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc, int pass)
+{
+    int rc = phase_b_setup(c, h, ref, srcs, nsrc, pass);
+    if (rc < 0) return rc;
     rc = check(c, h, hpm_run_internal(h, false), "RunPatchMatch (geometric)");
     if (rc < 0) return rc;
     account(c, h);
     return check(c, h, hpm_store_to_pool(h, 1), "store");
+}
+
+/* RunPatchMatch of the listed problems in one hpm_run_batch call */
+int run_batch(Ctx& c, const std::vector<hpm_handle*>& hs, bool readback)
+{
+    if (hs.empty()) {
+        return HPM_OK;
+    }
+    hpm_batch_stats bs;
+    const int rc = hpm_run_batch(c.opt->batch, hs.data(), (int)hs.size(), readback ? 1 : 0, &bs);
+    if (rc < 0) {
+        return fail(c, "batch: %s", hpm_batch_last_error(c.opt->batch));
+    }
+    account_batch(c, bs, (int)hs.size());
+    return HPM_OK;
 }
 
 } // namespace
@@ -400,6 +463,7 @@ extern "C" void hpm_pipeline_default(hpm_pipeline_options* opt)
     opt->hierarchy_from = nullptr;
     opt->stream_base = 0;
     opt->prior_on_device = 1;
+    opt->batch = nullptr;
 }
 
 extern "C" int hpm_pool_get_state(hpm_pool* pool, int view, float* hyp_world4, float* cost)
@@ -510,7 +574,101 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
         c.threads = all_threads;
         return worst.load();
     };
-    if (rc == HPM_OK) {
+    if (rc == HPM_OK && c.opt->batch && c.opt->hierarchy_from) {
+        rc = fail(c, "the batched schedule runs HPM-MVS problems (no hierarchy)");
+    }
+    /* the batched form: one handle per problem, each pass whose views are
+     * independent runs as one hpm_run_batch call over all of them */
+    std::vector<hpm_handle*> ph;
+    if (rc == HPM_OK && c.opt->batch) {
+        for (int i = 0; i < num_problems && rc == HPM_OK; ++i) {
+            ph.push_back(hpm_create(pool->device));
+            if (!ph.back()) {
+                rc = fail(c, "cannot create a problem handle");
+            }
+        }
+    }
+    /* runs fn(handle of problem i, i) for every problem on the workers */
+    auto each = [&](auto&& fn) {
+        std::atomic<int> next(0);
+        std::atomic<int> worst(HPM_OK);
+        auto work = [&]() {
+            for (;;) {
+                const int i = next.fetch_add(1);
+                if (i >= num_problems || worst.load() < 0) {
+                    return;
+                }
+                const int r = fn(ph[(size_t)i], i);
+                if (r < 0) {
+                    worst.store(r);
+                }
+            }
+        };
+        c.threads = std::max(1, all_threads / workers);
+        std::vector<std::thread> ts;
+        for (int w = 1; w < workers; ++w) {
+            ts.emplace_back(work);
+        }
+        work();
+        for (auto& t : ts) {
+            t.join();
+        }
+        c.threads = all_threads;
+        return worst.load();
+    };
+    auto srcs_of = [&](int i) { return src_ids + (size_t)i * max_src; };
+    if (rc == HPM_OK && c.opt->batch) {
+        auto t0 = clk::now();
+        const bool host_prior = c.opt->planar_prior != 0 && !c.opt->prior_on_device;
+        for (int i = 0; i < num_problems && rc >= 0; ++i) {
+            rc = phase_a_setup(c, ph[(size_t)i], ref_ids[i], srcs_of(i), num_src[i]);
+        }
+        if (rc >= 0) rc = run_batch(c, ph, host_prior);
+        std::vector<char> use((size_t)num_problems, 0);
+        if (rc >= 0 && c.opt->planar_prior && c.lprior) {
+            rc = each([&](hpm_handle* hh, int i) {
+                bool u = false;
+                const int r = learned_apply(c, hh, ref_ids[i], &u);
+                use[(size_t)i] = u ? 1 : 0;
+                return r;
+            });
+            std::vector<hpm_handle*> learned;
+            for (int i = 0; i < num_problems; ++i) {
+                if (use[(size_t)i]) learned.push_back(ph[(size_t)i]);
+            }
+            if (rc >= 0) rc = run_batch(c, learned, false);
+        }
+        if (rc >= 0) {
+            rc = each([&](hpm_handle* hh, int i) {
+                if (use[(size_t)i]) {
+                    return check(c, hh, hpm_store_to_pool(hh, 0), "store");
+                }
+                return phase_a_planar(c, hh, ref_ids[i]);
+            });
+        }
+        st->ms_phase_a = ms_since(t0);
+        if (rc >= 0 && c.opt->geom_consistency) {
+            t0 = clk::now();
+            for (int g = 0; g < c.opt->params.geom_iterations && rc >= 0; ++g) {
+                if (g == 0) {
+                    /* the first pass reads only phase-A maps: one batch */
+                    for (int i = 0; i < num_problems && rc >= 0; ++i) {
+                        rc = phase_b_setup(c, ph[(size_t)i], ref_ids[i], srcs_of(i), num_src[i], g);
+                    }
+                    if (rc >= 0) rc = run_batch(c, ph, false);
+                    for (int i = 0; i < num_problems && rc >= 0; ++i) {
+                        rc = check(c, ph[(size_t)i], hpm_store_to_pool(ph[(size_t)i], 1), "store");
+                    }
+                } else {
+                    /* later passes read maps of this pass: problem order */
+                    for (int i = 0; i < num_problems && rc >= 0; ++i) {
+                        rc = process_phase_b(c, ph[(size_t)i], ref_ids[i], srcs_of(i), num_src[i], g);
+                    }
+                }
+            }
+            st->ms_phase_b = ms_since(t0);
+        }
+    } else if (rc == HPM_OK) {
         /* phase A: views are independent (main.cpp:658-661) */
         auto t0 = clk::now();
         rc = parallel([&](hpm_handle* wh, int i) {
@@ -521,7 +679,7 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
         });
         st->ms_phase_a = ms_since(t0);
     }
-    if (rc >= 0 && c.opt->geom_consistency) {
+    if (rc >= 0 && c.opt->geom_consistency && !c.opt->batch) {
         auto t0 = clk::now();
         for (int g = 0; g < c.opt->params.geom_iterations && rc >= 0; ++g) {
             auto pass = [&](hpm_handle* wh, int i) {
@@ -548,6 +706,11 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
     for (int w = 1; w < workers; ++w) {
         if (handles[(size_t)w]) {
             hpm_destroy(handles[(size_t)w]);
+        }
+    }
+    for (hpm_handle* x : ph) {
+        if (x) {
+            hpm_destroy(x);
         }
     }
     if (h) {

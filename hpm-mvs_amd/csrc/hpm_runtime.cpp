@@ -809,7 +809,7 @@ int hpm_run_internal(hpm_handle* h, bool want_readback)
     HPM_HIP(h, hipSetDevice(h->device));
     hpm_dev_problem pb;
     fill_problem(h, &pb);
-    HPM_HIP(h, hipMemsetAsync(h->d_neval, 0, 2 * sizeof(unsigned long long), h->stream));
+    HPM_HIP(h, (hipError_t)hpm_launch_zero_u64(h->d_neval, 2, h->stream));
     HPM_HIP(h, hipEventRecord(h->ev[0], h->stream));
     HPM_HIP(h, (hipError_t)hpm_launch_init(&pb, h->stream));
     HPM_HIP(h, hipEventRecord(h->ev[1], h->stream));

@@ -143,6 +143,7 @@ class HpmPipelineOptions(C.Structure):
         ("hierarchy_from", C.c_void_p),
         ("stream_base", C.c_int32),
         ("prior_on_device", C.c_int32),
+        ("batch", C.c_void_p),
     ]
 
 
@@ -174,6 +175,7 @@ class HpmPipelineStats(C.Structure):
         ("learned_views", C.c_int32),
         ("learned_fallbacks", C.c_int32),
         ("learned_accept", C.c_double),
+        ("batches", C.c_int32),
     ]
 
 
@@ -187,6 +189,7 @@ class HpmBatchStats(C.Structure):
         ("tiles_colour", C.c_int32),
         ("grid", C.c_int32),
         ("graph_reused", C.c_int32),
+        ("items", C.c_int32),
     ]
 
 
@@ -296,6 +299,12 @@ SIGNATURES = {
     "hpm_lprior_apply": (C.c_int, [C.c_void_p, C.c_void_p, _FP]),
     "hpm_lprior_apply_timed": (C.c_int, [C.c_void_p, C.c_void_p, _FP, _FP]),
     "hpm_lprior_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "hpm_fusion_run_rgb": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                     C.c_void_p, C.POINTER(C.c_int64), C.c_char_p, C.c_int]),
+    "hpm_read_image_rgb": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "hpm_decode_jpeg_rgb": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int32)]),
+    "hpm_batch_set_schedule": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "hpm_lprior_forward_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "hpm_lprior_noalloc_capacity": (C.c_size_t, []),
     # hpm_batch.h
@@ -590,6 +599,12 @@ class Batch:
     def log(self) -> str:
         return lib().hpm_batch_log(self._b).decode()
 
+    def set_schedule(self, grid: int, items: int):
+        """hpm_batch_set_schedule: items 0 static ranges over `grid` blocks,
+        1/2/4 work-queue items of that many tiles, -1 autotune."""
+        if lib().hpm_batch_set_schedule(self._b, int(grid), int(items)) != 0:
+            raise HpmError("hpm_batch_set_schedule: bad schedule")
+
 
 def read_camera(path: str) -> HpmCamera:
     c = HpmCamera()
@@ -626,6 +641,17 @@ def read_pgm(path: str) -> np.ndarray:
         raise HpmError(f"cannot read {path}")
     out = np.empty((h.value, w.value), np.float32)
     lib().hpm_read_pgm_f32(path.encode(), fptr(out), C.byref(w), C.byref(h))
+    return out
+
+
+def read_image_rgb(path: str) -> np.ndarray:
+    """cv::imread(IMREAD_COLOR) as (H, W, 3) RGB bytes (hpm_read_image_rgb)."""
+    w, h = C.c_int32(), C.c_int32()
+    if lib().hpm_read_image_rgb(path.encode(), None, C.byref(w), C.byref(h)) != 0:
+        raise HpmError(f"cannot read {path}")
+    out = np.empty((h.value, w.value, 3), np.uint8)
+    if lib().hpm_read_image_rgb(path.encode(), bptr(out), C.byref(w), C.byref(h)) != 0:
+        raise HpmError(f"cannot read {path}")
     return out
 
 

@@ -11,8 +11,11 @@ Network (NHWC, channel counts as the BF16 MFMA kernels need them): input 32
 channels (6 used), encoder 32-64-96-128 with two 3x3 convolutions + ReLU per
 level and 2x2 average pooling between levels, decoder with nearest 2x
 upsampling, concatenation of the skip, one 3x3 convolution + ReLU per level,
-and a 1x1 head of 32 channels of which 4 are used (depth residual, normal
-residual). 0.78 M parameters.
+and a 1x1 head of 32 channels of which 5 are used (depth residual, normal
+residual, and the logit of the proposal's confidence: the probability that
+its depth lies within 2% of the label). 0.78 M parameters. Weight files of
+networks trained without the confidence channel carry MAGIC_NO_CONF; the
+pipeline then judges every proposal.
 
 Training runs the convolutions (forward, input gradient, weight gradient) on
 the hand-written kernels of include/hpm_unet.h through a torch autograd
@@ -40,7 +43,8 @@ LAYERS: List[Tuple[str, int, int, int, int]] = [
     ("head", 32, 32, 1, 0),
 ]
 IN_CHANNELS = 32
-MAGIC = b"HPMUNET1"
+MAGIC = b"HPMUNET2"          # head channel 4 is the confidence logit
+MAGIC_NO_CONF = b"HPMUNET1"  # no confidence channel
 
 
 def num_params() -> int:
@@ -61,13 +65,13 @@ def features(image: np.ndarray, depth: np.ndarray, normal_cam: np.ndarray, cost:
     return x
 
 
-def save(path: str, weights: Sequence, biases: Sequence):
+def save(path: str, weights: Sequence, biases: Sequence, confidence: bool = True):
     """Weight file: magic, layer count, then per layer (ci, co, ks, relu) as
     int32, BF16 weights [co][ks*ks][ci] and FP32 biases [co]."""
     import torch
 
     with open(path, "wb") as f:
-        f.write(MAGIC)
+        f.write(MAGIC if confidence else MAGIC_NO_CONF)
         f.write(struct.pack("<i", len(LAYERS)))
         for (_, ci, co, ks, relu), w, b in zip(LAYERS, weights, biases):
             f.write(struct.pack("<4i", ci, co, ks, relu))
@@ -82,7 +86,7 @@ def load(path: str):
 
     ws, bs = [], []
     with open(path, "rb") as f:
-        if f.read(8) != MAGIC:
+        if f.read(8) not in (MAGIC, MAGIC_NO_CONF):
             raise ValueError(f"{path}: not a learned-prior weight file")
         (n,) = struct.unpack("<i", f.read(4))
         if n != len(LAYERS):

@@ -22,6 +22,7 @@
 
 #include <stdint.h>
 
+#include "hpm_batch.h"
 #include "hpm_capi.h"
 #include "hpm_types.h"
 
@@ -50,6 +51,14 @@ typedef struct hpm_pipeline_options {
     hpm_pool* hierarchy_from;
     int32_t stream_base;        /* added to every RunPatchMatch call's Philox stream (0) */
     int32_t prior_on_device;    /* build the planar prior with hpm_build_planar_prior (1), or on the host (0) */
+    /* the batched form (no reference counterpart): when set, every pass
+     * whose views are independent (the plain pass, the learned-prior pass,
+     * the first geometric pass) runs as one hpm_run_batch call over all
+     * problems on this caller-owned object, which keeps the tuned and
+     * captured schedules across runs; the planar prior passes stay per
+     * view on the workers. NULL: every pass per view on the workers. The
+     * maps equal those of the per-view schedule bit for bit. */
+    hpm_batch* batch;
 } hpm_pipeline_options;
 
 typedef struct hpm_pipeline_stats {
@@ -67,6 +76,7 @@ typedef struct hpm_pipeline_stats {
     int32_t learned_views;      /* views whose prior pass used the learned prior */
     int32_t learned_fallbacks;  /* views that fell back to the planar prior */
     double learned_accept;      /* mean kept fraction over the views that ran the network */
+    int32_t batches;            /* hpm_run_batch calls of the batched form */
 } hpm_pipeline_stats;
 
 void hpm_pipeline_default(hpm_pipeline_options* opt);

@@ -41,7 +41,9 @@ int hpm_unet_upsample2(const void* in, int ldi, void* out, int ldo, int c, int h
 int hpm_unet_to_bf16(const float* in, void* out, int64_t n, void* stream);
 
 /* ------------------------------------------------------ the learned prior
- * Weight file of hpm-mvs_amd/unet_prior.py (train_unet_prior.py writes it).
+ * Weight file of hpm-mvs_amd/unet_prior.py (train_unet_prior.py writes it);
+ * in files marked HPMUNET2 the head's channel 4 is the logit of the
+ * proposal's confidence, and proposals below 0.5 are not kept.
  * hpm_lprior_load returns NULL if the file is missing or of another
  * network. hpm_lprior_apply runs on a handle whose last hpm_run_patchmatch
  * was a plain pass: the network proposes a depth and normal per pixel from

@@ -72,7 +72,18 @@ def test_batch_matches_unbatched(hpm, geom):
             assert a.run_stats().n_eval == b.run_stats().n_eval, (call, i)
     log = batch.log()
     say(log.strip())
-    assert "chosen grid" in log
+    assert "chosen" in log and "queue items of 2 tiles" in log
+    # every schedule the tuner may choose gives the same bits
+    for items in (0, 1, 2, 4):
+        batch.set_schedule(0, items)
+        for pr in singles:
+            pr.RunPatchMatch()
+        st = batch.run(batched)
+        say(f"fixed schedule, items {items}: grid {st.grid}, {st.ms_device:.2f} ms device")
+        assert st.items == items
+        for i, (a, b) in enumerate(zip(singles, batched)):
+            assert np.array_equal(a.hypotheses().view(np.uint32), b.hypotheses().view(np.uint32)), (items, i)
+            assert np.array_equal(a.costs().view(np.uint32), b.costs().view(np.uint32)), (items, i)
     batch.close()
     for pr in singles + batched:
         pr.close()

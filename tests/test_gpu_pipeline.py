@@ -154,8 +154,11 @@ def test_pipeline_folder_round_trip(hpm, tmp_path):
 
 
 def test_executable_on_jpeg_folder(hpm, tmp_path):
-    """The reference's folder layout with images/%08d.jpg (as real datasets
-    ship them): hpm_mvs decodes the JPEGs (luminance) and reconstructs."""
+    """The reference's folder layout with colour images/%08d.jpg (as real
+    datasets ship them): hpm_mvs decodes the JPEGs (luminance for the
+    matching, IMREAD_GRAYSCALE; colour for the fused points,
+    IMREAD_COLOR, main.cpp:523) and reconstructs; the PLY equals the
+    oracle's fusion of the written maps with the decoded colours."""
     import subprocess
 
     Image = pytest.importorskip("PIL.Image")
@@ -164,8 +167,10 @@ def test_executable_on_jpeg_folder(hpm, tmp_path):
     hpm.write_scene_folder(cfg, str(tmp_path))
     for v in range(3):
         pgm = tmp_path / "images" / f"{v:08d}.pgm"
-        img = hpm.read_image_grey(str(pgm)).astype(np.uint8)
-        Image.fromarray(img).save(str(tmp_path / "images" / f"{v:08d}.jpg"), "JPEG", quality=95)
+        img = hpm.read_image_grey(str(pgm)).astype(np.int32)
+        rgb = np.stack([img + 25, img, img - 25 + (np.arange(img.shape[1]) % 7)], -1)
+        Image.fromarray(np.clip(rgb, 0, 255).astype(np.uint8)).save(str(tmp_path / "images" / f"{v:08d}.jpg"),
+                                                                   "JPEG", quality=95)
         pgm.unlink()
     say("case: hpm_mvs on a 3-view 192x144 folder of JPEG images")
     r = subprocess.run([exe, str(tmp_path), "--geom-iterations", "1"], capture_output=True, text=True, timeout=300)
@@ -177,7 +182,28 @@ def test_executable_on_jpeg_folder(hpm, tmp_path):
         accs.append(float(np.mean(np.abs(d - gt) / gt < 0.05)))
     say(f"accuracy (5%) per view: {accs}")
     assert min(accs) > 0.3
-    assert (tmp_path / "HPM" / "HPM_model.ply").exists()
+    from oracle import cpu
+
+    ply = (tmp_path / "HPM" / "HPM_model.ply").read_bytes()
+    head, _, body = ply.partition(b"end_header\n")
+    n = int([ln for ln in head.split(b"\n") if ln.startswith(b"element vertex")][0].split()[-1])
+    rows = np.frombuffer(body, dtype=np.dtype([("xyz", "<f4", 3), ("n", "<f4", 3), ("rgb", "u1", 3)]))
+    problems = hpm.read_pair(str(tmp_path / "pair.txt"))
+    cams, depths, normals, colors = [], [], [], []
+    for v in range(3):
+        cam = hpm.read_camera(str(tmp_path / "cams" / f"{v:08d}_cam.txt"))
+        col = hpm.read_image_rgb(str(tmp_path / "images" / f"{v:08d}.jpg"))
+        cam.height, cam.width = col.shape[:2]
+        base = tmp_path / "HPM" / f"2333_{v:08d}"
+        cams.append(cam)
+        depths.append(hpm.read_dmb(str(base / "depths_geom.dmb")))
+        normals.append(hpm.read_dmb(str(base / "normals.dmb")))
+        colors.append(col)
+    want = cpu.fusion(cams, depths, normals, colors, [(r, list(s)) for r, s in problems], hpm.POINT_DTYPE)
+    say(f"fused points: {n} (oracle {len(want)}), distinct colours {len(np.unique(rows['rgb'], axis=0))}")
+    assert n == len(want) and n > 0
+    assert np.array_equal(rows["rgb"], np.stack([want["r"], want["g"], want["b"]], axis=1))
+    assert np.mean(rows["rgb"][:, 0].astype(int) - rows["rgb"][:, 2].astype(int)) > 30  # red above blue
 
 
 @pytest.mark.parametrize("size", [(320, 240), (1600, 1066)])
@@ -317,3 +343,42 @@ def test_device_init_rejects_a_mis_sized_reload(hpm, tmp_path):
     pr.RunPatchMatch()
     assert pr.hypotheses().shape == (H, W, 4)
     pr.close()
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_batched_schedule_equals_per_view_schedule(hpm, learned):
+    """hpm_pipeline_options.batch: the plain pass, the learned-prior pass and
+    the first geometric pass of all views as one hpm_run_batch call each
+    give the per-view schedule's maps bit for bit, on the first run (tuned
+    and captured) and on a second run (the captured schedules replayed)."""
+    cfg, cams, imgs, gd, problems = scene(hpm)
+
+    def options(batch=None):
+        opt = hpm.pipeline_options(scale_bound=200)
+        opt.params.geom_iterations = 2
+        if learned:
+            opt.learned_prior = os.path.abspath(WEIGHTS).encode()
+        if batch is not None:
+            opt.batch = batch._b
+        return opt
+
+    say(f"case: batched schedule, 4 views 320x240, learned prior {learned}")
+    ref_pool = hpm.Pool(cams, imgs)
+    hpm.run_pipeline(ref_pool, problems, options())
+    batch = hpm.Batch(0)
+    pool = hpm.Pool(cams, imgs)
+    for run in range(2):
+        st = hpm.run_pipeline(pool, problems, options(batch))
+        say(f"run {run}: {st.batches} batches, {st.runs} RunPatchMatch, learned views {st.learned_views}")
+        assert st.batches == 2 + (1 if st.learned_views else 0)
+        assert st.learned_views == (4 if learned else 0) or st.learned_fallbacks > 0
+        for v, _ in problems:
+            a, b = ref_pool.state(v), pool.state(v)
+            assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)), (run, v)
+            assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)), (run, v)
+            assert np.array_equal(ref_pool.depth(v, 0), pool.depth(v, 0))
+            assert np.array_equal(ref_pool.depth(v, 1), pool.depth(v, 1))
+    say(batch.log().strip())
+    batch.close()
+    pool.close()
+    ref_pool.close()

@@ -41,6 +41,7 @@ struct hpm_batch_shape {
     hipGraphExec_t exec = nullptr;
     int grid = 0;
     int gran = 0;                          /* 0: static ranges; else tiles per work item */
+    bool persistent = false;               /* one cooperative launch for the whole schedule */
     int n_full = 0, n_colour = 0;
     uint64_t last_use = 0;
 };
@@ -58,6 +59,7 @@ struct hpm_batch {
     hpm_dev_problem* h_pbs = nullptr; /* pinned */
     size_t cap_h_pbs = 0;
     int fixed_grid = 0, fixed_items = -1; /* hpm_batch_set_schedule; -1: autotune */
+    bool fixed_persistent = false;
 };
 
 namespace {
@@ -129,9 +131,17 @@ void free_shape(hpm_batch_shape* sh)
 
 const int kLaunches = 11;
 
-/* the 11 launches of one schedule */
-int enqueue(hpm_batch* b, const hpm_batch_shape& sh, int n, int grid, int gran)
+/* one schedule: 11 stage launches, or the zeroing kernels and one
+ * persistent cooperative launch */
+int enqueue(hpm_batch* b, const hpm_batch_shape& sh, int n, int grid, int gran, bool persistent)
 {
+    if (persistent) {
+        BHIP(b, (hipError_t)hpm_launch_zero_u64(sh.d_neval, 2 * n, b->stream));
+        BHIP(b, (hipError_t)hpm_launch_zero_u64(sh.d_queue, 4 * kLaunches, b->stream));
+        BHIP(b, (hipError_t)hpm_launch_batch_persistent(sh.d_pbs, n, sh.d_tiles, sh.n_full, sh.n_colour, gran,
+                                                        (unsigned int*)sh.d_queue, grid, b->stream));
+        return HPM_OK;
+    }
     const hpm_tile* tf = sh.d_tiles;
     const hpm_tile* tc = sh.d_tiles + sh.n_full;
     const hpm_dev_problem* p_init = sh.d_pbs;
@@ -219,13 +229,15 @@ extern "C" void hpm_batch_destroy(hpm_batch* b)
     delete b;
 }
 
-extern "C" int hpm_batch_set_schedule(hpm_batch* b, int grid, int items)
+extern "C" int hpm_batch_set_schedule(hpm_batch* b, int grid, int items, int persistent)
 {
-    if (!b || !(items == -1 || items == 0 || items == 1 || items == 2 || items == 4) || grid < 0) {
+    if (!b || !(items == -1 || items == 0 || items == 1 || items == 2 || items == 4) || grid < 0 ||
+        (persistent && items != 1 && items != 2)) {
         return HPM_EINVAL;
     }
     b->fixed_grid = grid;
     b->fixed_items = items;
+    b->fixed_persistent = persistent != 0;
     for (hpm_batch_shape& sh : b->shapes) {
         free_shape(&sh);
     }
@@ -341,9 +353,22 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         const int cus = std::max(1, prop.multiProcessorCount);
         struct Cand {
             int grid, gran;
+            bool persistent;
         };
-        const Cand cands[7] = {{2 * cus, 0}, {4 * cus, 0}, {8 * cus, 0}, {16 * cus, 0},
-                               {2 * cus, 1}, {2 * cus, 2}, {2 * cus, 4}};
+        int pgrid[3] = {0, 0, 0};
+        for (int g = 1; g <= 2; ++g) {
+            if (hpm_batch_persistent_grid(b->device, g, &pgrid[g]) != 0) {
+                pgrid[g] = 0;
+            }
+        }
+        std::vector<Cand> cands = {{2 * cus, 0, false}, {4 * cus, 0, false}, {8 * cus, 0, false},
+                                   {16 * cus, 0, false}, {2 * cus, 1, false}, {2 * cus, 2, false},
+                                   {2 * cus, 4, false}};
+        for (int g = 1; g <= 2; ++g) {
+            if (pgrid[g] > 0) {
+                cands.push_back({pgrid[g], g, true});
+            }
+        }
         rc = grow(b, &b->d_save, &b->cap_save, save_floats);
         if (rc != HPM_OK) return rc;
         rc = save_state(b, hs, n, false);
@@ -356,7 +381,11 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         Cand best = cands[0];
         const bool fixed = b->fixed_items >= 0;
         if (fixed) {
-            best = Cand{b->fixed_grid > 0 ? b->fixed_grid : 2 * cus, b->fixed_items};
+            if (b->fixed_persistent && pgrid[b->fixed_items] <= 0) {
+                return bfail(b, HPM_EDEVICE, "the persistent schedule cannot be launched on this device");
+            }
+            best = b->fixed_persistent ? Cand{pgrid[b->fixed_items], b->fixed_items, true}
+                                       : Cand{b->fixed_grid > 0 ? b->fixed_grid : 2 * cus, b->fixed_items, false};
             log += " fixed schedule;";
         }
         for (const Cand& c : cands) {
@@ -366,13 +395,16 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
             rc = save_state(b, hs, n, true);
             if (rc != HPM_OK) return rc;
             BHIP(b, hipEventRecord(b->ev[0], b->stream));
-            rc = enqueue(b, *sh, n, c.grid, c.gran);
+            rc = enqueue(b, *sh, n, c.grid, c.gran, c.persistent);
             if (rc != HPM_OK) return rc;
             BHIP(b, hipEventRecord(b->ev[1], b->stream));
             BHIP(b, hipEventSynchronize(b->ev[1]));
             float ms = 0.0f;
             BHIP(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
-            if (c.gran) {
+            if (c.persistent) {
+                snprintf(line, sizeof(line), " persistent launch, items of %d tiles, grid %d: %.3f ms;", c.gran,
+                         c.grid, ms);
+            } else if (c.gran) {
                 snprintf(line, sizeof(line), " queue items of %d tiles, grid %d: %.3f ms;", c.gran, c.grid, ms);
             } else {
                 snprintf(line, sizeof(line), " static grid %d: %.3f ms;", c.grid, ms);
@@ -385,16 +417,19 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         }
         rc = save_state(b, hs, n, true);
         if (rc != HPM_OK) return rc;
-        snprintf(line, sizeof(line), " chosen %s grid %d items %d\n", best.gran ? "queue" : "static", best.grid,
-                 best.gran);
+        snprintf(line, sizeof(line), " chosen %s grid %d items %d\n",
+                 best.persistent ? "persistent" : (best.gran ? "queue" : "static"), best.grid, best.gran);
         log += line;
         b->log += log;
         sh->grid = best.grid;
         sh->gran = best.gran;
-        /* capture the schedule */
+        sh->persistent = best.persistent;
+    }
+    if (!reused && !sh->persistent) {
+        /* capture the schedule (the persistent one is a single launch) */
         hipGraph_t graph = nullptr;
         BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-        rc = enqueue(b, *sh, n, sh->grid, sh->gran);
+        rc = enqueue(b, *sh, n, sh->grid, sh->gran, false);
         hipError_t ce = hipStreamEndCapture(b->stream, &graph);
         if (rc != HPM_OK) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -406,7 +441,12 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         BHIP(b, ie);
     }
     BHIP(b, hipEventRecord(b->ev[0], b->stream));
-    BHIP(b, hipGraphLaunch(sh->exec, b->stream));
+    if (sh->persistent) {
+        rc = enqueue(b, *sh, n, sh->grid, sh->gran, true);
+        if (rc != HPM_OK) return rc;
+    } else {
+        BHIP(b, hipGraphLaunch(sh->exec, b->stream));
+    }
     BHIP(b, hipEventRecord(b->ev[1], b->stream));
     std::vector<unsigned long long> nev(2 * (size_t)n, 0ull);
     BHIP(b, hipMemcpyAsync(nev.data(), sh->d_neval, nev.size() * 8, hipMemcpyDeviceToHost, b->stream));
@@ -443,6 +483,7 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         stats->tiles_colour = sh->n_colour;
         stats->grid = sh->grid;
         stats->items = sh->gran;
+        stats->persistent = sh->persistent ? 1 : 0;
         stats->graph_reused = reused ? 1 : 0;
     }
     return HPM_OK;

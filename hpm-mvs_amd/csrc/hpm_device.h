@@ -156,6 +156,13 @@ typedef struct hpm_fusion_args {
 int hpm_launch_fusion_geometry(const hpm_fusion_args* a, int width, int height, void* stream);
 /* p[0..n) = 0 as a kernel (graph-capturable, unlike a memset node) */
 int hpm_launch_zero_u64(unsigned long long* p, int n, void* stream);
+/* The batched schedule in one persistent cooperative launch (work items of
+ * gran 1 or 2 tiles from per-XCD queues, grid barriers between the 10
+ * stages; queue: 80 zeroed u32 counters). grid must not exceed the resident
+ * blocks hpm_batch_persistent_grid reports. */
+int hpm_batch_persistent_grid(int device, int gran, int* grid);
+int hpm_launch_batch_persistent(const hpm_dev_problem* pbs, int n, const hpm_tile* tiles, int n_full, int n_colour,
+                                int gran, unsigned int* queue, int grid, void* stream);
 
 /* kind 0..3: exp/sin/cos/acos of in[i]; kind 4: Philox blocks */
 int hpm_launch_selftest(int kind, const void* in, void* out, int n, void* stream);

@@ -17,6 +17,7 @@
  * VGPRs for the whole pixel update; source taps are bilinear gathers from
  * the FP32 image pool (cache resident: a 1600x1066 view set is ~75 MB).
  */
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <float.h>
@@ -1843,6 +1844,71 @@ __global__ __launch_bounds__(256) void hpm_k_filter_batch(const hpm_dev_problem*
     });
 }
 
+/* The whole batched RunPatchMatch (init, 3 x red/black propagation,
+ * finalize, red/black filter) in one persistent cooperative launch: every
+ * block stays resident, pulls work items of kGran tiles from the per-XCD
+ * queues of each stage (queue: 8 zeroed counters per stage), and the grid
+ * synchronises between stages, whose reads depend on the other colour's or
+ * the previous stage's writes anywhere in the problem. The grid barrier
+ * makes those writes visible across XCDs. */
+template <int kGran>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HPM_NCC_WAVES))) void hpm_k_batch_persistent(
+    const hpm_dev_problem* __restrict__ pbs, int n, const hpm_tile* __restrict__ tiles, int n_full, int n_colour,
+    unsigned int* queue)
+{
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const hpm_dev_problem* p_init = pbs;
+    const hpm_dev_problem* p_prop = pbs + n;
+    const hpm_tile* tf = tiles;
+    const hpm_tile* tc = tiles + n_full;
+    for_tiles<kGran>(n_full, queue, [&](int t) {
+        const hpm_tile tl = load_tile(tf, t);
+        const hpm_dev_problem& pb = p_init[tl.problem];
+        const int px = tl.bx * blockDim.x + threadIdx.x;
+        const int py = tl.by * blockDim.y + threadIdx.y;
+        uint32_t nev = 0;
+        if (px < pb.W && py < pb.H) {
+            nev = init_pixel(pb, px, py);
+        }
+        count_evals(pb, nev);
+    });
+    grid.sync();
+#pragma unroll 1
+    for (int k = 0; k < 6; ++k) { /* 3 iterations (HPM.cu:1933) x black, red */
+        const int iter = k / 2, colour = k % 2;
+        for_tiles<kGran>(n_colour, queue + 8 * (1 + k), [&](int t) {
+            const hpm_tile tl = load_tile(tc, t);
+            const hpm_dev_problem& pb = p_prop[tl.problem];
+            int px, py;
+            const bool valid = colour_pixel_at(pb, colour, tl.bx, tl.by, &px, &py);
+            const uint32_t nev = propagate_pixel(pb, px, py, iter, colour, valid);
+            count_evals(pb, nev);
+        });
+        grid.sync();
+    }
+    for_tiles<kGran>(n_full, queue + 8 * 7, [&](int t) {
+        const hpm_tile tl = load_tile(tf, t);
+        const hpm_dev_problem& pb = p_prop[tl.problem];
+        const int px = tl.bx * blockDim.x + threadIdx.x;
+        const int py = tl.by * blockDim.y + threadIdx.y;
+        if (px < pb.W && py < pb.H) {
+            finalize_pixel(pb, px, py);
+        }
+    });
+#pragma unroll 1
+    for (int colour = 0; colour < 2; ++colour) {
+        grid.sync();
+        for_tiles<kGran>(n_colour, queue + 8 * (8 + colour), [&](int t) {
+            const hpm_tile tl = load_tile(tc, t);
+            const hpm_dev_problem& pb = p_prop[tl.problem];
+            int px, py;
+            if (colour_pixel_at(pb, colour, tl.bx, tl.by, &px, &py)) {
+                filter_pixel(pb, px, py);
+            }
+        });
+    }
+}
+
 /* zeroes n counters (a kernel, so the schedule can be captured in a graph) */
 __global__ __launch_bounds__(64) void hpm_k_zero_u64(unsigned long long* p, int n)
 {
@@ -2066,6 +2132,40 @@ extern "C" int hpm_launch_batch(int stage, const hpm_dev_problem* pbs, const hpm
     default: return (int)hipErrorInvalidValue;
     }
     return (int)hipGetLastError();
+}
+
+namespace {
+
+template <int kGran>
+const void* persistent_kernel()
+{
+    return (const void*)hpm_k_batch_persistent<kGran>;
+}
+
+} // namespace
+
+extern "C" int hpm_batch_persistent_grid(int device, int gran, int* grid)
+{
+    const void* k = gran == 1 ? persistent_kernel<1>() : gran == 2 ? persistent_kernel<2>() : nullptr;
+    if (!k || !grid) {
+        return (int)hipErrorInvalidValue;
+    }
+    int per_cu = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    *grid = per_cu * cus;
+    return (int)e;
+}
+
+extern "C" int hpm_launch_batch_persistent(const hpm_dev_problem* pbs, int n, const hpm_tile* tiles, int n_full,
+                                           int n_colour, int gran, unsigned int* queue, int grid, void* stream)
+{
+    const void* k = gran == 1 ? persistent_kernel<1>() : gran == 2 ? persistent_kernel<2>() : nullptr;
+    if (!k || !queue || grid < 1) {
+        return (int)hipErrorInvalidValue;
+    }
+    void* args[] = {(void*)&pbs, (void*)&n, (void*)&tiles, (void*)&n_full, (void*)&n_colour, (void*)&queue};
+    return (int)hipLaunchCooperativeKernel(k, dim3(grid, 1, 1), dim3(64, 4, 1), args, 0, (hipStream_t)stream);
 }
 
 extern "C" int hpm_launch_lprior_features(const hpm_dev_problem* pb, void* x, void* stream)

@@ -190,6 +190,7 @@ class HpmBatchStats(C.Structure):
         ("grid", C.c_int32),
         ("graph_reused", C.c_int32),
         ("items", C.c_int32),
+        ("persistent", C.c_int32),
     ]
 
 
@@ -304,7 +305,7 @@ SIGNATURES = {
     "hpm_read_image_rgb": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "hpm_decode_jpeg_rgb": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.POINTER(C.c_int32),
                                       C.POINTER(C.c_int32)]),
-    "hpm_batch_set_schedule": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "hpm_batch_set_schedule": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "hpm_lprior_forward_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "hpm_lprior_noalloc_capacity": (C.c_size_t, []),
     # hpm_batch.h
@@ -599,10 +600,11 @@ class Batch:
     def log(self) -> str:
         return lib().hpm_batch_log(self._b).decode()
 
-    def set_schedule(self, grid: int, items: int):
+    def set_schedule(self, grid: int, items: int, persistent: bool = False):
         """hpm_batch_set_schedule: items 0 static ranges over `grid` blocks,
-        1/2/4 work-queue items of that many tiles, -1 autotune."""
-        if lib().hpm_batch_set_schedule(self._b, int(grid), int(items)) != 0:
+        1/2/4 work-queue items of that many tiles (persistent: one
+        cooperative launch, items 1 or 2), -1 autotune."""
+        if lib().hpm_batch_set_schedule(self._b, int(grid), int(items), int(persistent)) != 0:
             raise HpmError("hpm_batch_set_schedule: bad schedule")
 
 

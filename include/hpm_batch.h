@@ -6,9 +6,10 @@
  * problems (e.g. 32 references of 640x480 to 1920x1080 with 4-10 sources):
  * every stage of the schedule is one launch over all problems' tiles, the
  * schedule is captured once per batch shape in a hipGraph and replayed, and
- * the tile schedule (static ranges over a chosen grid, or per-XCD work
- * queues with a chosen item size) is picked by timing the candidates on the
- * first call of a shape (logged by hpm_batch_log). Each problem's results equal those of
+ * the tile schedule (static ranges over a chosen grid, per-XCD work queues
+ * with a chosen item size, or the whole schedule as one persistent
+ * cooperative launch with grid barriers between stages) is picked by timing
+ * the candidates on the first call of a shape (logged by hpm_batch_log). Each problem's results equal those of
  * hpm_run_patchmatch on its handle (same seeds and call index); tuning
  * trials save and restore the problems' state.
  */
@@ -35,6 +36,7 @@ typedef struct hpm_batch_stats {
     int32_t grid;            /* blocks per stage launch (autotuned) */
     int32_t graph_reused;    /* 1 when the captured schedule was replayed */
     int32_t items;           /* autotuned: 0 static tile ranges, else tiles per work-queue item */
+    int32_t persistent;      /* 1: the whole schedule ran as one persistent launch */
 } hpm_batch_stats;
 
 hpm_batch* hpm_batch_create(int device);
@@ -49,9 +51,10 @@ int hpm_run_batch(hpm_batch* b, hpm_handle* const* handles, int n, int want_read
 
 /* Fixes the schedule instead of autotuning it: items 0 for static tile
  * ranges over `grid` blocks (0: 2 per CU), 1, 2 or 4 for work-queue items
- * of that many tiles; items -1 returns to autotuning. Drops the captured
- * schedules. */
-int hpm_batch_set_schedule(hpm_batch* b, int grid, int items);
+ * of that many tiles; persistent (items 1 or 2): the whole schedule as one
+ * cooperative launch of every resident block, grid barriers between its
+ * stages; items -1 returns to autotuning. Drops the captured schedules. */
+int hpm_batch_set_schedule(hpm_batch* b, int grid, int items, int persistent);
 const char* hpm_batch_last_error(const hpm_batch* b);
 /* one line per tuned batch shape: candidates with their times, the choice */
 const char* hpm_batch_log(const hpm_batch* b);

@@ -74,13 +74,13 @@ def test_batch_matches_unbatched(hpm, geom):
     say(log.strip())
     assert "chosen" in log and "queue items of 2 tiles" in log
     # every schedule the tuner may choose gives the same bits
-    for items in (0, 1, 2, 4):
-        batch.set_schedule(0, items)
+    for items, persistent in ((0, False), (1, False), (2, False), (4, False), (1, True), (2, True)):
+        batch.set_schedule(0, items, persistent)
         for pr in singles:
             pr.RunPatchMatch()
         st = batch.run(batched)
-        say(f"fixed schedule, items {items}: grid {st.grid}, {st.ms_device:.2f} ms device")
-        assert st.items == items
+        say(f"fixed schedule, items {items}, persistent {persistent}: grid {st.grid}, {st.ms_device:.2f} ms device")
+        assert st.items == items and st.persistent == int(persistent)
         for i, (a, b) in enumerate(zip(singles, batched)):
             assert np.array_equal(a.hypotheses().view(np.uint32), b.hypotheses().view(np.uint32)), (items, i)
             assert np.array_equal(a.costs().view(np.uint32), b.costs().view(np.uint32)), (items, i)

@@ -231,7 +231,9 @@ class Oracle:
     def filter(self, colour: int):
         assert lib().hpm_oracle_filter(C.byref(self.pb), colour, _f(self.hyp), _f(self.cost), self.threads) == 0
 
-    def cost_vectors(self, planes: np.ndarray, pix: np.ndarray, fp64: bool) -> np.ndarray:
+    def cost_vectors(self, planes: np.ndarray, pix: np.ndarray, fp64) -> np.ndarray:
+        """fp64: False/0 the HIP path's FP32 arithmetic, True/1 the FP64
+        reference formula, 2 the reference's FP32 order, 3 the FP32 warp."""
         pl = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
         px = np.ascontiguousarray(pix, dtype=np.int32).reshape(-1, 2)
         out = np.empty((len(pl), self.pb.num_images - 1), np.float64)

@@ -360,6 +360,182 @@ static float ncc(const Ctx& cx, const RefPatch& rp, int px, int py, int v, const
     return fmaxf(0.0f, fminf(2.0f, 1.0f - cov / den));
 }
 
+/* Bilinear sample at an absolute source coordinate (x, y), clamped to
+ * [-1, W] x [-1, H] with edge texels repeated: the sampling of sample_tap. */
+static inline float sample_abs(const float* img, int W, int H, float x, float y)
+{
+    x = fminf(fmaxf(x, -1.0f), (float)W);
+    y = fminf(fmaxf(y, -1.0f), (float)H);
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int ix = (int)fx, iy = (int)fy;
+    const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
+    const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
+    const float t00 = img[(size_t)y0 * W + x0], t10 = img[(size_t)y0 * W + x1];
+    const float t01 = img[(size_t)y1 * W + x0], t11 = img[(size_t)y1 * W + x1];
+    const float v0 = fmaf(a, t10 - t00, t00);
+    const float v1 = fmaf(a, t11 - t01, t01);
+    return fmaf(b, v1 - v0, v0);
+}
+
+/* Accuracy evidence only: ComputeBilateralNCC (HPM.cu:227-397) in the
+ * reference's own FP32 arithmetic order: the homography rebuilt in float
+ * per call, each tap's source point divided out in float
+ * (ComputeCorrespondingPoint), raw window sums E[xy] - E[x]E[y] in float.
+ * Sampling as the FP32 path's (FP16 texels, exact FP32 bilinear). */
+static double ncc32_reference_order(const hpm_oracle_problem* pb, const Ctx& cx, int px, int py, int v, F4 pl)
+{
+    const hpm_camera& rc = pb->cams[0];
+    const hpm_camera& sc = pb->cams[v];
+    const hpm_params& P = pb->params;
+    const int SW = sc.width, SH = sc.height;
+    const float* rimg = pb->images[0];
+    const float* simg = cx.src16[v].data();
+    float refC[3], srcC[3], Rr[9], Cr[3], tr[3], H[9], tmp[9];
+    for (int a = 0; a < 3; ++a) {
+        refC[a] = -(rc.R[a] * rc.t[0] + rc.R[3 + a] * rc.t[1] + rc.R[6 + a] * rc.t[2]);
+        srcC[a] = -(sc.R[a] * sc.t[0] + sc.R[3 + a] * sc.t[1] + sc.R[6 + a] * sc.t[2]);
+    }
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) {
+            Rr[3 * r + c] = sc.R[3 * r] * rc.R[3 * c] + sc.R[3 * r + 1] * rc.R[3 * c + 1] + sc.R[3 * r + 2] * rc.R[3 * c + 2];
+        }
+        Cr[r] = refC[r] - srcC[r];
+    }
+    for (int r = 0; r < 3; ++r) {
+        tr[r] = sc.R[3 * r] * Cr[0] + sc.R[3 * r + 1] * Cr[1] + sc.R[3 * r + 2] * Cr[2];
+    }
+    const float nn[3] = {pl.x, pl.y, pl.z};
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) {
+            H[3 * r + c] = Rr[3 * r + c] - tr[r] * nn[c] / pl.w;
+        }
+    }
+    for (int r = 0; r < 3; ++r) {
+        tmp[3 * r + 0] = H[3 * r + 0] / rc.K[0];
+        tmp[3 * r + 1] = H[3 * r + 1] / rc.K[4];
+        tmp[3 * r + 2] = -H[3 * r + 0] * rc.K[2] / rc.K[0] - H[3 * r + 1] * rc.K[5] / rc.K[4] + H[3 * r + 2];
+    }
+    H[0] = sc.K[0] * tmp[0] + sc.K[2] * tmp[6];
+    H[1] = sc.K[0] * tmp[1] + sc.K[2] * tmp[7];
+    H[2] = sc.K[0] * tmp[2] + sc.K[2] * tmp[8];
+    H[3] = sc.K[4] * tmp[3] + sc.K[5] * tmp[6];
+    H[4] = sc.K[4] * tmp[4] + sc.K[5] * tmp[7];
+    H[5] = sc.K[4] * tmp[5] + sc.K[5] * tmp[8];
+    H[6] = sc.K[8] * tmp[6];
+    H[7] = sc.K[8] * tmp[7];
+    H[8] = sc.K[8] * tmp[8];
+    auto point = [&](int x, int y, float* ox, float* oy) {
+        const float X = (float)x, Y = (float)y;
+        const float hx = H[0] * X + H[1] * Y + H[2];
+        const float hy = H[3] * X + H[4] * Y + H[5];
+        const float hz = H[6] * X + H[7] * Y + H[8];
+        *ox = hx / hz;
+        *oy = hy / hz;
+    };
+    float cxp, cyp;
+    point(px, py, &cxp, &cyp);
+    if (cxp >= SW || cxp < 0.0f || cyp >= SH || cyp < 0.0f) {
+        return 2.0;
+    }
+    const int radius = P.patch_size / 2;
+    const float c = texel(rimg, rc.width, rc.height, px, py);
+    float sw = 0.0f, sr = 0.0f, srr = 0.0f, ss = 0.0f, sss = 0.0f, srs = 0.0f;
+    for (int i = -radius; i < radius + 1; i += P.radius_increment) {
+        float swr = 0.0f, srw = 0.0f, srrw = 0.0f, ssw = 0.0f, sssw = 0.0f, srsw = 0.0f;
+        for (int j = -radius; j < radius + 1; j += P.radius_increment) {
+            const float r = texel(rimg, rc.width, rc.height, px + i, py + j);
+            float sx, sy;
+            point(px + i, py + j, &sx, &sy);
+            const float sv = sample_abs(simg, SW, SH, sx, sy);
+            const float w = hpm_expf(-sqrtf((float)(i * i + j * j)) / (2.0f * P.sigma_spatial * P.sigma_spatial) -
+                                     fabsf(r - c) / (2.0f * P.sigma_color * P.sigma_color));
+            srw += w * r;
+            srrw += w * r * r;
+            ssw += w * sv;
+            sssw += w * sv * sv;
+            srsw += w * r * sv;
+            swr += w;
+        }
+        sr += srw;
+        srr += srrw;
+        ss += ssw;
+        sss += sssw;
+        srs += srsw;
+        sw += swr;
+    }
+    const float inv = 1.0f / sw;
+    sr *= inv;
+    srr *= inv;
+    ss *= inv;
+    sss *= inv;
+    srs *= inv;
+    const float vr = srr - sr * sr, vs = sss - ss * ss;
+    if (vr < 1e-5f || vs < 1e-5f) {
+        return 2.0;
+    }
+    return fmaxf(0.0f, fminf(2.0f, 1.0f - (srs - sr * ss) / sqrtf(vr * vs)));
+}
+
+/* Accuracy evidence only: this build's FP32 window sums with the warp
+ * (homography H = A - b q^T, warped centre, tap gradients) evaluated in
+ * FP32 instead of FP64. */
+static double ncc_warp32(const Ctx& cx, const RefPatch& rp, int px, int py, int v, const double qd[3])
+{
+    const hpm_oracle_problem* pb = cx.pb;
+    const hpm_camera& sc = pb->cams[v];
+    const int SW = sc.width, SH = sc.height;
+    const float* simg = cx.src16[v].data();
+    const ViewRel& vr = cx.rel[v];
+    const float q[3] = {(float)qd[0], (float)qd[1], (float)qd[2]};
+    float H[9];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) {
+            H[3 * r + c] = fmaf(-(float)vr.b[r], q[c], (float)vr.A[3 * r + c]);
+        }
+    }
+    const float X = (float)px, Y = (float)py;
+    const float hx = fmaf(H[0], X, fmaf(H[1], Y, H[2]));
+    const float hy = fmaf(H[3], X, fmaf(H[4], Y, H[5]));
+    const float hz = fmaf(H[6], X, fmaf(H[7], Y, H[8]));
+    const float icz = 1.0f / hz;
+    const float sx0 = hx * icz, sy0 = hy * icz;
+    if (!(sx0 >= 0.0f && sx0 < (float)SW && sy0 >= 0.0f && sy0 < (float)SH)) {
+        return 2.0;
+    }
+    Warp w;
+    const float fx = floorf(sx0), fy = floorf(sy0);
+    w.ix0 = (int)fx;
+    w.iy0 = (int)fy;
+    w.fx0 = sx0 - fx;
+    w.fy0 = sy0 - fy;
+    w.ex = H[6] * icz;
+    w.ey = H[7] * icz;
+    w.gxx = (H[0] - sx0 * H[6]) * icz;
+    w.gxy = (H[1] - sx0 * H[7]) * icz;
+    w.gyx = (H[3] - sy0 * H[6]) * icz;
+    w.gyy = (H[4] - sy0 * H[7]) * icz;
+    w.small = (fabsf(w.ex) + fabsf(w.ey)) * (float)(pb->params.patch_size / 2) <= 0.01f;
+    float ss_ = 0.0f, sss = 0.0f, srs = 0.0f;
+    const int kc = rp.centre_tap;
+    const float shift = sample_tap(simg, SW, SH, w, rp.dx[kc], rp.dy[kc]);
+    for (int k = 0; k < rp.ntap; ++k) {
+        const float raw = k == kc ? shift : sample_tap(simg, SW, SH, w, rp.dx[k], rp.dy[k]);
+        const float s = raw - shift;
+        const float ws = rp.w[k] * s;
+        ss_ = ss_ + ws;
+        sss = fmaf(ws, s, sss);
+        srs = fmaf(rp.wr[k], s, srs);
+    }
+    const float ms = ss_ * rp.inv_w;
+    const float var_s = sss * rp.inv_w - ms * ms;
+    if (rp.var_r < 1e-5f || var_s < 1e-5f) {
+        return 2.0;
+    }
+    const float cov = srs * rp.inv_w - rp.mr * ms;
+    return fmaxf(0.0f, fminf(2.0f, 1.0f - cov / sqrtf(rp.var_r * var_s)));
+}
+
 /* ComputeBilateralNCC in double precision, reference formula (raw sums). */
 static double ncc64(const hpm_oracle_problem* pb, const Ctx& cx, int px, int py, int v, F4 plane)
 {
@@ -1762,9 +1938,13 @@ extern "C" int hpm_oracle_cost_vectors(const hpm_oracle_problem* pb, const float
     for (int i = 0; i < npix; ++i) {
         const int px = pix[2 * i], py = pix[2 * i + 1];
         const F4 pl = ld4(planes, i);
-        if (fp64) {
+        if (fp64 == 1) {
             for (int v = 1; v <= nsrc; ++v) {
                 out[(size_t)i * nsrc + v - 1] = ncc64(pb, cx, px, py, v, pl);
+            }
+        } else if (fp64 == 2) {
+            for (int v = 1; v <= nsrc; ++v) {
+                out[(size_t)i * nsrc + v - 1] = ncc32_reference_order(pb, cx, px, py, v, pl);
             }
         } else {
             RefPatch rp;
@@ -1772,7 +1952,7 @@ extern "C" int hpm_oracle_cost_vectors(const hpm_oracle_problem* pb, const float
             double q[3];
             plane_q(cx.rk, pl, q);
             for (int v = 1; v <= nsrc; ++v) {
-                out[(size_t)i * nsrc + v - 1] = ncc(cx, rp, px, py, v, q);
+                out[(size_t)i * nsrc + v - 1] = fp64 == 3 ? ncc_warp32(cx, rp, px, py, v, q) : ncc(cx, rp, px, py, v, q);
             }
         }
     }

@@ -118,9 +118,11 @@ int hpm_oracle_filter(const hpm_oracle_problem* pb, int colour, float* hyp, cons
 
 /* Multi-view NCC cost vectors for fixed hypotheses (HPM.cu:325-397,438-443).
  * planes: npix camera-frame planes (n, d); pix: npix (x, y) pairs.
- * out: npix x (num_images-1) costs. fp64 selects the double-precision
- * evaluation of the reference formula; otherwise the FP32 arithmetic of the
- * HIP path. */
+ * out: npix x (num_images-1) costs. fp64: 0 the FP32 arithmetic of the HIP
+ * path; 1 the double-precision evaluation of the reference formula; for
+ * accuracy evidence only, 2 the reference's own FP32 order (homography in
+ * float per call, a float division per tap, raw sums, HPM.cu:227-397) and
+ * 3 the HIP path's formulation with its warp in FP32 instead of FP64. */
 int hpm_oracle_cost_vectors(const hpm_oracle_problem* pb, const float* planes, const int32_t* pix,
                             int npix, int fp64, double* out, int num_threads);
 

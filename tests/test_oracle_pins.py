@@ -306,3 +306,34 @@ def test_plain_run_reaches_ground_truth(hpm, cpu, c1_small):
     # a random hypothesis is within 2% about 2.5% of the time; at 160x120
     # with a 7.5 cm baseline 2% of depth is under 0.1 px of disparity
     assert acc > 0.3, acc
+
+
+def test_fp64_warp_is_needed_for_the_1e4_bar(hpm, cpu):
+    """The FP32 cost must stay within 1e-4 of the FP64 evaluation of the
+    reference formula with fixed hypotheses. On the 1600x1066, 10-source
+    scene, planes near the ground truth (1% and 30% perturbations): the
+    HIP path's arithmetic (homography and warped centre in FP64, taps and
+    sums in FP32) stays within 1e-4; the same formulation with the warp in
+    FP32, and the reference's own FP32 order (HPM.cu:227-397: homography
+    rebuilt in float, a float division per tap, raw sums), do not. This is
+    why the per-(hypothesis, view) warp is FP64, not only the per-view
+    setup."""
+    from conftest import problem_inputs
+    from test_gpu_parity import _planes_near_truth
+
+    cfg = hpm.workload_config("c2")
+    cams, images, gd, gn, low, order = problem_inputs(hpm, cfg, 5)
+    rng = np.random.default_rng(1)
+    p1, x1 = _planes_near_truth(hpm, cams, gd, gn, order, rng, 5000, 0.01)
+    p2, x2 = _planes_near_truth(hpm, cams, gd, gn, order, rng, 5000, 0.3)
+    planes, pix = np.concatenate([p1, p2]), np.concatenate([x1, x2])
+    o = cpu.Oracle(cams, images, cpu.relaxed_params(cams))
+    f64 = o.cost_vectors(planes, pix, 1)
+    err = {}
+    for mode in (0, 2, 3):
+        got = o.cost_vectors(planes, pix, mode)
+        boundary = (got == 2.0) != (f64 == 2.0)
+        err[mode] = float(np.abs(got - f64)[~boundary].max())
+    print(f"max |cost - FP64|: HIP path {err[0]:.2e}, FP32 warp {err[3]:.2e}, reference FP32 order {err[2]:.2e}")
+    assert err[0] <= 1e-4
+    assert err[3] > 1e-4 and err[2] > 1e-4

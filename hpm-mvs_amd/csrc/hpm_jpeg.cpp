@@ -4,12 +4,17 @@
  * which asks libjpeg for grayscale output, i.e. the decoded luminance (Y)
  * component, computed with libjpeg's default integer inverse DCT.
  *
- * Supported: baseline and extended sequential Huffman JPEG (SOF0, SOF1),
- * 8-bit samples, 1 (grey) or 3 (YCbCr) components with any sampling
- * factors, interleaved or single-component scans, restart intervals. Only
- * the Y component is reconstructed (it is at full resolution in every
- * sampling layout). Progressive, lossless, arithmetic-coded, 12-bit and
- * 4-component (CMYK/YCCK) files are rejected.
+ * Supported: baseline, extended sequential and progressive Huffman JPEG
+ * (SOF0, SOF1, SOF2), 8-bit samples, 1 (grey) or 3 (YCbCr) components with
+ * any sampling factors, interleaved or single-component scans, restart
+ * intervals. Progressive scans follow ITU-T T.81 annex G: DC first and
+ * refinement scans, AC spectral-selection first scans with end-of-band
+ * runs, and AC successive-approximation refinement (correction bits of
+ * the coefficients already nonzero, new coefficients of +-1 at the current
+ * bit position). For the grey image only the Y component is reconstructed
+ * (it is at full resolution in every sampling layout). Lossless,
+ * arithmetic-coded, 12-bit and 4-component (CMYK/YCCK) files are
+ * rejected.
  *
  * The inverse DCT is the integer form of the Loeffler-Ligtenberg-Moschytz
  * factorisation (ICASSP 1989) with 13-bit fixed-point constants and two
@@ -64,6 +69,8 @@ struct Decoder {
     std::vector<Component> comps;
     int width = 0, height = 0, hmax = 1, vmax = 1;
     int restart = 0;
+    bool progressive = false;
+    int eobrun = 0; /* progressive AC scans: blocks left in an end-of-band run */
     /* bit reader */
     uint32_t bits = 0;
     int nbits = 0;
@@ -219,9 +226,10 @@ struct Decoder {
 
     bool read_sof(int len, int marker)
     {
-        if (marker != 0xC0 && marker != 0xC1) {
-            return fail("unsupported JPEG process (only baseline / extended sequential Huffman)");
+        if (marker != 0xC0 && marker != 0xC1 && marker != 0xC2) {
+            return fail("unsupported JPEG process (only baseline, extended sequential and progressive Huffman)");
         }
+        progressive = marker == 0xC2;
         const int prec = byte();
         height = u16();
         width = u16();
@@ -295,6 +303,125 @@ struct Decoder {
         return true;
     }
 
+    /* progressive DC first scan: the DC difference, scaled by 2^al */
+    bool decode_dc_first(Component& c, int16_t* blk, int al)
+    {
+        const int t = decode(dc[c.td]);
+        if (t < 0 || t > 11) {
+            return fail("corrupt DC code");
+        }
+        c.pred += t ? extend(getbits(t), t) : 0;
+        blk[0] = (int16_t)(c.pred * (1 << al));
+        return true;
+    }
+
+    /* progressive DC refinement: one more bit of the DC coefficient */
+    void decode_dc_refine(int16_t* blk, int al)
+    {
+        if (getbits(1)) {
+            blk[0] = (int16_t)(blk[0] | (1 << al));
+        }
+    }
+
+    /* progressive AC first scan over the band ss..se, values scaled by 2^al,
+     * with end-of-band runs across blocks */
+    bool decode_ac_first(Component& c, int16_t* blk, int ss, int se, int al)
+    {
+        if (eobrun > 0) {
+            --eobrun;
+            return true;
+        }
+        const Huffman& ha = ac[c.ta];
+        for (int k = ss; k <= se; ++k) {
+            const int rs = decode(ha);
+            if (rs < 0) {
+                return fail("corrupt AC code");
+            }
+            const int r = rs >> 4, sz = rs & 15;
+            if (sz == 0) {
+                if (r < 15) { /* EOBr: this block and (2^r - 1 + bits) more end here */
+                    eobrun = (1 << r) - 1;
+                    if (r) {
+                        eobrun += getbits(r);
+                    }
+                    break;
+                }
+                k += 15; /* ZRL */
+                continue;
+            }
+            k += r;
+            if (k > se) {
+                return fail("AC run past the band");
+            }
+            blk[kZigzag[k]] = (int16_t)(extend(getbits(sz), sz) * (1 << al));
+        }
+        return true;
+    }
+
+    /* progressive AC refinement of the band ss..se at bit al: a correction
+     * bit for every coefficient already nonzero that the scan passes, and
+     * new coefficients of magnitude 2^al after runs of zero coefficients */
+    bool decode_ac_refine(Component& c, int16_t* blk, int ss, int se, int al)
+    {
+        const int p1 = 1 << al, m1 = -(1 << al);
+        auto correct = [&](int k) {
+            int16_t& v = blk[kZigzag[k]];
+            if (getbits(1) && (v & p1) == 0) {
+                v = (int16_t)(v >= 0 ? v + p1 : v + m1);
+            }
+        };
+        int k = ss;
+        if (eobrun == 0) {
+            const Huffman& ha = ac[c.ta];
+            for (; k <= se; ++k) {
+                const int rs = decode(ha);
+                if (rs < 0) {
+                    return fail("corrupt AC code");
+                }
+                int r = rs >> 4;
+                const int sz = rs & 15;
+                int val = 0;
+                if (sz) {
+                    if (sz != 1) {
+                        return fail("bad AC refinement value");
+                    }
+                    val = getbits(1) ? p1 : m1;
+                } else if (r != 15) { /* EOBr */
+                    eobrun = 1 << r;
+                    if (r) {
+                        eobrun += getbits(r);
+                    }
+                    break;
+                }
+                /* pass r zero coefficients (correcting the nonzero ones on
+                 * the way), then place the new value (ZRL: 16 zeros) */
+                for (; k <= se; ++k) {
+                    if (blk[kZigzag[k]] != 0) {
+                        correct(k);
+                    } else {
+                        if (r == 0) {
+                            break;
+                        }
+                        --r;
+                    }
+                }
+                if (val && k <= se) {
+                    blk[kZigzag[k]] = (int16_t)val;
+                }
+            }
+        }
+        if (eobrun > 0) {
+            /* inside an end-of-band run: correction bits only */
+            for (; k <= se; ++k) {
+                if (blk[kZigzag[k]] != 0) {
+                    correct(k);
+                }
+            }
+            --eobrun;
+        }
+        return true;
+    }
+
     /* after a restart interval: the RSTn marker, then fresh predictors */
     bool restart_marker()
     {
@@ -311,6 +438,7 @@ struct Decoder {
         for (Component& c : comps) {
             c.pred = 0;
         }
+        eobrun = 0;
         return true;
     }
 
@@ -334,15 +462,42 @@ struct Decoder {
             }
             c->td = t >> 4;
             c->ta = t & 15;
-            if (c->td > 3 || c->ta > 3 || !dc[c->td].defined || !ac[c->ta].defined) {
+            if (c->td > 3 || c->ta > 3) {
                 return fail("scan uses an undefined Huffman table");
             }
             sc.push_back(c);
         }
         const int ss = byte(), se = byte(), ahal = byte();
-        if (ss != 0 || se != 63 || ahal != 0) {
-            return fail("not a sequential scan");
+        const int ah = ahal >> 4, al = ahal & 15;
+        if (!progressive) {
+            if (ss != 0 || se != 63 || ahal != 0) {
+                return fail("not a sequential scan");
+            }
+        } else if (ss > se || se > 63 || al > 13 || (ss == 0 && se != 0) || (ss > 0 && ns != 1)) {
+            return fail("bad progressive scan");
         }
+        /* the Huffman tables this scan decodes with must be defined */
+        const bool dc_scan = ss == 0, first = ah == 0;
+        for (const Component* c : sc) {
+            if ((dc_scan && first && !dc[c->td].defined) || (!dc_scan && !ac[c->ta].defined) ||
+                (!progressive && (!dc[c->td].defined || !ac[c->ta].defined))) {
+                return fail("scan uses an undefined Huffman table");
+            }
+        }
+        eobrun = 0;
+        auto block = [&](Component& c, int16_t* blk) {
+            if (!progressive) {
+                return decode_block(c, blk);
+            }
+            if (dc_scan) {
+                if (first) {
+                    return decode_dc_first(c, blk, al);
+                }
+                decode_dc_refine(blk, al);
+                return true;
+            }
+            return first ? decode_ac_first(c, blk, ss, se, al) : decode_ac_refine(c, blk, ss, se, al);
+        };
         bits = 0;
         nbits = 0;
         hit_marker = false;
@@ -370,7 +525,7 @@ struct Decoder {
                 }
                 if (ns == 1) {
                     Component& c = *sc[0];
-                    if (!decode_block(c, &c.coef[((size_t)my * c.bw + mx) * 64])) {
+                    if (!block(c, &c.coef[((size_t)my * c.bw + mx) * 64])) {
                         return false;
                     }
                 } else {
@@ -378,7 +533,7 @@ struct Decoder {
                         for (int by = 0; by < c->v; ++by) {
                             for (int bx = 0; bx < c->h; ++bx) {
                                 const size_t b = (size_t)(my * c->v + by) * c->bw + (size_t)(mx * c->h + bx);
-                                if (!decode_block(*c, &c->coef[b * 64])) {
+                                if (!block(*c, &c->coef[b * 64])) {
                                     return false;
                                 }
                             }

@@ -5,7 +5,8 @@ HPM-MVS/HPM.cpp:492), and for each the grey image libjpeg decodes
 (grayscale output: the luminance component, integer inverse DCT), as PGM,
 and the colour image it decodes (RGB output, default upsampling, as
 cv::imread(IMREAD_COLOR) asks for in RunFusion, main.cpp:523), as PPM.
-A progressive file has no expected image: the decoder must reject it.
+Progressive files (spectral selection and successive approximation, as
+libjpeg's default progression script writes them) have expected images too.
 
     python tests/golden/gen_jpeg.py
 """
@@ -41,6 +42,11 @@ CASES = [
     ("grey_restart_rows_q100_33x17", lambda: Image.fromarray(content(17, 33, 6)),
      dict(quality=100, restart_marker_rows=1)),
     ("progressive_48x32", lambda: Image.fromarray(rgb(32, 48, 7)), dict(quality=80, progressive=True)),
+    ("progressive_grey_q90_53x41", lambda: Image.fromarray(content(41, 53, 8)), dict(quality=90, progressive=True)),
+    ("progressive_ycc420_optimized_101x77", lambda: Image.fromarray(rgb(77, 101, 9)),
+     dict(quality=70, subsampling=2, progressive=True, optimize=True)),
+    ("progressive_ycc444_restart_q95_66x50", lambda: Image.fromarray(rgb(50, 66, 10)),
+     dict(quality=95, subsampling=0, progressive=True, restart_marker_blocks=3)),
 ]
 
 
@@ -53,8 +59,6 @@ def main():
         data = buf.getvalue()
         with open(os.path.join(HERE, name + ".jpg"), "wb") as f:
             f.write(data)
-        if "progressive" in opts:
-            continue
         im = Image.open(io.BytesIO(data))
         im.draft("L", im.size)
         g = np.asarray(im.convert("L") if im.mode != "L" else im)

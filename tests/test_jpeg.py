@@ -2,8 +2,9 @@
 cv::imread(path, IMREAD_GRAYSCALE) gives (HPM-MVS/HPM.cpp:492), i.e.
 libjpeg's grayscale decode, on fixtures written by tests/golden/gen_jpeg.py
 (libjpeg-turbo through Pillow): grey and YCbCr files, 4:2:0 / 4:2:2 / 4:4:4,
-odd sizes, restart markers, optimised Huffman tables, qualities 50-100.
-Bar: byte-exact. Progressive files are rejected."""
+odd sizes, restart markers, optimised Huffman tables, qualities 50-100,
+progressive files (spectral selection, successive approximation).
+Bar: byte-exact. Other processes (lossless, arithmetic) are rejected."""
 import glob
 import os
 
@@ -30,9 +31,19 @@ def test_decode_matches_libjpeg(hpm, name):
     assert diff.max() == 0, f"{name}: {int((diff > 0).sum())} pixels differ, max {diff.max()}"
 
 
-def test_progressive_rejected(hpm):
+def test_lossless_process_rejected(hpm, tmp_path):
+    """A frame of an unsupported process (the lossless SOF3 marker written
+    over a baseline file's SOF0) is an error, not a wrong image."""
+    data = bytearray(open(os.path.join(HERE, "grey_q75_37x23.jpg"), "rb").read())
+    i = data.find(b"\xff\xc0")
+    assert i > 0
+    data[i + 1] = 0xC3
+    p = tmp_path / "lossless.jpg"
+    p.write_bytes(bytes(data))
     with pytest.raises(hpm.HpmError):
-        hpm.read_image_grey(os.path.join(HERE, "progressive_48x32.jpg"))
+        hpm.read_image_grey(str(p))
+    with pytest.raises(hpm.HpmError):
+        hpm.read_image_rgb(str(p))
 
 
 def test_pgm_still_read(hpm, tmp_path):
@@ -98,7 +109,8 @@ def test_colour_decode_matches_libjpeg(hpm, name):
 
 def test_colour_decode_random_files(hpm, tmp_path):
     """40 random colour images in every subsampling Pillow writes, odd
-    sizes, qualities 30-100, against libjpeg-turbo's RGB decode."""
+    sizes, qualities 30-100, sequential and progressive, against
+    libjpeg-turbo's RGB decode."""
     Image = pytest.importorskip("PIL.Image")
     import io
 
@@ -107,7 +119,8 @@ def test_colour_decode_random_files(hpm, tmp_path):
         h, w = int(rng.integers(1, 60)), int(rng.integers(1, 60))
         img = (rng.integers(0, 256, (h, w, 3)) * rng.uniform(0, 1, (1, 1, 3))).astype(np.uint8)
         buf = io.BytesIO()
-        Image.fromarray(img).save(buf, "JPEG", quality=int(rng.integers(30, 101)), subsampling=int(k % 3))
+        Image.fromarray(img).save(buf, "JPEG", quality=int(rng.integers(30, 101)), subsampling=int(k % 3),
+                                  progressive=bool(k % 2), optimize=bool(k % 4 == 1))
         path = tmp_path / f"{k}.jpg"
         path.write_bytes(buf.getvalue())
         want = np.asarray(Image.open(io.BytesIO(buf.getvalue())).convert("RGB"))

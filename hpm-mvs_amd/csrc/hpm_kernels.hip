@@ -833,6 +833,19 @@ __device__ __forceinline__ int lane_rank(uint64_t m)
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+/* the reference sums of lane `owner` of wave wv; its window weights are
+ * read from LDS at its lane index */
+__device__ __forceinline__ RefPatch shfl_ref_patch(const RefPatch& rp, int wv, int owner)
+{
+    RefPatch o;
+    o.lane = wv * 64 + owner;
+    o.c = __shfl(rp.c, owner);
+    o.inv_w = __shfl(rp.inv_w, owner);
+    o.mr = __shfl(rp.mr, owner);
+    o.var_r = __shfl(rp.var_r, owner);
+    return o;
+}
+
 /* Every lane of the wave calls this (the refinement packs evaluations of
  * all lanes into the wave's trips); lanes without a pixel (valid false)
  * only help. */
@@ -1078,15 +1091,60 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 pos[0] = lu;
             }
         }
-        if (__any(extend)) {
+        const uint64_t em = __ballot(extend);
+        if (em != 0ull) {
             if (extend) {
                 plane_q(pb, ld4(pb.hyp, pos[0]), q);
                 nev += (uint32_t)nsrc;
             }
-#pragma unroll 1
-            for (int v = 0; v < nsrc; ++v) {
+            const int n = __popcll(em);
+            const int total = n * nsrc;
+            if ((total + 63) / 64 < nsrc) {
+                /* few lanes extend: their (view, lane) evaluations are
+                 * packed into full trips, view-major (task t = v n + r
+                 * scores view v for the r-th extending lane) */
+                const int wv = (int)threadIdx.y, ln = (int)threadIdx.x; /* blockDim = (64, 4) */
+                const int rank = lane_rank(em);
                 if (extend) {
-                    ca[0][v] = ncc(pb, rp, px, py, v + 1, q);
+                    s_own[wv][rank] = ln;
+                }
+                __builtin_amdgcn_wave_barrier();
+                const int pxy = px | (py << 16);
+#pragma unroll 1
+                for (int t0 = 0; t0 < total; t0 += 64) {
+                    const int t = t0 + ln;
+                    const bool act = t < total;
+                    const int v = act ? t / n : 0;
+                    const int owner = act ? s_own[wv][t - v * n] : ln;
+                    const int opxy = __shfl(pxy, owner);
+                    const RefPatch orp = shfl_ref_patch(rp, wv, owner);
+                    double tq[3];
+#pragma unroll
+                    for (int k2 = 0; k2 < 3; ++k2) {
+                        tq[k2] = __shfl(q[k2], owner);
+                    }
+                    float term = 0.0f;
+                    if (act) {
+                        term = ncc(pb, orp, opxy & 0xffff, opxy >> 16, v + 1, tq);
+                    }
+                    const int v_hi = min(nsrc - 1, (t0 + 63) / n);
+#pragma unroll 1
+                    for (int vv = t0 / n; vv <= v_hi; ++vv) {
+                        const int src = vv * n + rank - t0; /* lane that scored (vv, this lane) */
+                        const bool got = extend && src >= 0 && src < 64;
+                        const float val = __shfl(term, got ? src : ln);
+                        if (got) {
+                            ca[0][vv] = val;
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            } else {
+#pragma unroll 1
+                for (int v = 0; v < nsrc; ++v) {
+                    if (extend) {
+                        ca[0][v] = ncc(pb, rp, px, py, v + 1, q);
+                    }
                 }
             }
         }
@@ -1372,12 +1430,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 const int c_lo = t0 / n;
                 const int c_hi = (t0 + 63) / n < 4 ? (t0 + 63) / n : 4;
                 const int opxy = __shfl(pxy, owner);
-                RefPatch orp;
-                orp.lane = wv * 64 + owner;
-                orp.c = __shfl(rp.c, owner);
-                orp.inv_w = __shfl(rp.inv_w, owner);
-                orp.mr = __shfl(rp.mr, owner);
-                orp.var_r = __shfl(rp.var_r, owner);
+                const RefPatch orp = shfl_ref_patch(rp, wv, owner);
                 float4 plane = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
                 for (int cc = 0; cc < 5; ++cc) {

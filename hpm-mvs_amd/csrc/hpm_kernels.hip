@@ -106,12 +106,14 @@ __device__ __forceinline__ void plane_q(const hpm_dev_problem& pb, float4 n, dou
  * into an integer texel and a float fraction, plus the float coefficients of
  * the tap offsets s(p + d) - s0 = (g d) / (1 + e . d). */
 struct Warp {
-    uint32_t base;        /* padded-pool element of texel (ix0, iy0) (the pool holds < 2^30 texels) */
+    uint32_t base4;       /* byte offset of texel (ix0, iy0) in the padded pool (< 2^30 texels) */
     float fx0, fy0;
     float ex, ey, gxx, gxy, gyx, gyy;
     float lox, hix, loy, hiy; /* clamp of the offsets: source coordinate in [-1, W] x [-1, H] */
+    float cub;            /* 1 for the cubic reciprocal, 0 for the quadratic (tiny) */
     bool small;           /* |e . d| <= 0.01 over the window: cubic reciprocal */
-    bool inner;           /* small, and every tap lies inside the clamp: the clamp is the identity */
+    bool tiny;            /* |e . d| <= 0.002 over the window: quadratic reciprocal */
+    bool inner;           /* tiny, and every tap lies inside the clamp: the clamp is the identity */
 };
 
 /* false when the warped centre leaves the source image (HPM.cu:333-335) */
@@ -138,7 +140,7 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
     }
     const double fx = floor(sx0), fy = floor(sy0);
     const int ix0 = (int)fx, iy0 = (int)fy;
-    w->base = (uint32_t)sv->pad_off + (uint32_t)((iy0 + 1) * sv->pitch) + (uint32_t)(ix0 + 1);
+    w->base4 = ((uint32_t)sv->pad_off + (uint32_t)((iy0 + 1) * sv->pitch) + (uint32_t)(ix0 + 1)) * 4u;
     w->fx0 = (float)(sx0 - fx);
     w->fy0 = (float)(sy0 - fy);
     w->ex = (float)(H[6] * icz);
@@ -152,20 +154,23 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
     w->loy = -1.0f - (float)iy0;
     w->hiy = (float)(SH - iy0);
     w->small = (fabsf(w->ex) + fabsf(w->ey)) * 5.0f <= 0.01f;
+    w->tiny = (fabsf(w->ex) + fabsf(w->ey)) * 5.0f <= 0.002f;
+    w->cub = w->tiny ? 0.0f : 1.0f;
     /* |tap offset| <= 5 (|g.x| + |g.y|) / (1 - 0.01) for small warps; 1.0102
-     * covers the cubic's maximum (1.010101) and the FP32 roundings, 1e-3 px
-     * the roundings of these bounds */
+     * covers the maxima of the cubic (1.010101) and the quadratic (1.002004)
+     * and the FP32 roundings, 1e-3 px the roundings of these bounds */
     const float bx = fmaf(5.0f * (fabsf(w->gxx) + fabsf(w->gxy)), 1.0102f, 1e-3f);
     const float by = fmaf(5.0f * (fabsf(w->gyx) + fabsf(w->gyy)), 1.0102f, 1e-3f);
-    w->inner = w->small && w->fx0 - bx >= w->lox && w->fx0 + bx <= w->hix && w->fy0 - by >= w->loy &&
+    w->inner = w->tiny && w->fx0 - bx >= w->lox && w->fx0 + bx <= w->hix && w->fy0 - by >= w->loy &&
                w->fy0 + by <= w->hiy;
     return true;
 }
 
 /* Paths of the window sums, chosen per wave: every active lane has an inner
- * warp (the cubic reciprocal, no clamp), a small warp (the cubic, with the
- * clamp), or any warp (per lane the cubic or the quotient, with the clamp).
- * The three give the same values per lane. */
+ * warp (the quadratic reciprocal, no clamp), a small warp (per lane the
+ * quadratic or the cubic, with the clamp), or any warp (per lane the
+ * quadratic, the cubic or the quotient, with the clamp). The three give the
+ * same values per lane. */
 enum : int { kPathInner = 0, kPathSmall = 1, kPathAny = 2 };
 
 /* The dx products of one window column (x offset dx), shared by its taps. */
@@ -182,31 +187,57 @@ __device__ __forceinline__ Column column(const Warp& w, float dx)
     return c;
 }
 
+/* floor(x) as an integer, one instruction */
+__device__ __forceinline__ int floor_int(float x)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+/* a * b + c for 24-bit signed a and b, one instruction */
+__device__ __forceinline__ int mad_i24(int a, int b, int c)
+{
+    int r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
+/* the bilinear fraction: x - floor(x), capped at the largest float below 1
+ * (v_fract_f32) */
+__device__ __forceinline__ float fract1(float x)
+{
+    return __builtin_amdgcn_fractf(x);
+}
+
 /* Source coordinate of tap (dx, dy) of column c: byte offset of the
  * top-left texel of its bilinear footprint in the padded pool and the
  * bilinear fractions. */
 template <int kPath>
-__device__ __forceinline__ void tap_coord(const Warp& w, const Column& c, float pitchf, float dy, uint32_t* off,
+__device__ __forceinline__ void tap_coord(const Warp& w, const Column& c, int pitch4, float dy, uint32_t* off,
                                           float* fa, float* fb)
 {
     const float u = fmaf(w.ey, dy, c.ex);
-    const float cubic = fmaf(u, fmaf(u, 1.0f - u, -1.0f), 1.0f);
-    const float inv = (kPath != kPathAny || w.small) ? cubic : 1.0f / (1.0f + u);
+    float inv;
+    if (kPath == kPathInner) {
+        inv = fmaf(u, u - 1.0f, 1.0f);
+    } else {
+        /* 1 + u (u k - 1): k = 1 the quadratic, k = 1 - u the cubic */
+        const float poly = fmaf(u, fmaf(u, fmaf(-u, w.cub, 1.0f), -1.0f), 1.0f);
+        inv = (kPath != kPathAny || w.small) ? poly : 1.0f / (1.0f + u);
+    }
     float lx = fmaf(fmaf(w.gxy, dy, c.gx), inv, w.fx0);
     float ly = fmaf(fmaf(w.gyy, dy, c.gy), inv, w.fy0);
     if (kPath != kPathInner) {
         lx = fminf(fmaxf(lx, w.lox), w.hix);
         ly = fminf(fmaxf(ly, w.loy), w.hiy);
     }
-    const float flx = floorf(lx);
-    const float fly = floorf(ly);
-    *fa = lx - flx;
-    *fb = ly - fly;
-    /* (ix0 + flx, iy0 + fly) lies in [-1, W] x [-1, H]: its footprint is
-     * inside the padded image, whose border repeats the edge texels. The
-     * texel offset fly * pitch + flx from (ix0, iy0) is an integer below the
-     * image's padded size (< 2^24), so the FP32 fma is exact. */
-    *off = (w.base + (uint32_t)(int)fmaf(fly, pitchf, flx)) * 4u;
+    *fa = fract1(lx);
+    *fb = fract1(ly);
+    /* (ix0 + floor(lx), iy0 + floor(ly)) lies in [-1, W] x [-1, H]: its
+     * footprint is inside the padded image, whose border repeats the edge
+     * texels; |floor(ly)| <= 8 and pitch4 < 2^23 fit the 24-bit product */
+    *off = ((uint32_t)floor_int(lx) << 2) + (uint32_t)mad_i24(floor_int(ly), pitch4, (int)w.base4);
 }
 
 /* The padded pool as a buffer resource (gfx9 word 3). */
@@ -300,14 +331,14 @@ __device__ __forceinline__ void make_ref_patch(const float* img, int W, int H, i
 /* One column of the window (6 taps at x offset dx): coordinates, then the
  * 6 gathers. */
 template <int kPath>
-__device__ __forceinline__ void fetch_column(__amdgpu_buffer_rsrc_t pool, uint32_t pitch, const Warp& w, float dx,
+__device__ __forceinline__ void fetch_column(__amdgpu_buffer_rsrc_t pool, int pitch4, const Warp& w, float dx,
                                              Footprint* quad, float* fa, float* fb)
 {
     uint32_t off[6];
     const Column c = column(w, dx);
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        tap_coord<kPath>(w, c, (float)pitch, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
+        tap_coord<kPath>(w, c, pitch4, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
     }
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
@@ -337,7 +368,7 @@ __device__ __forceinline__ void accumulate_column(const RefPatch& rp, int a, flo
  * column: the gathers of column a + 1 are in flight while column a is
  * accumulated. Tap k = 6 a + b sits at offset (2a - 5, 2b - 5). */
 template <int kPath>
-__device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, uint32_t pitch, float one, const RefPatch& rp,
+__device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, int pitch4, float one, const RefPatch& rp,
                                          const Warp& w, float* ss_out, float* sss_out, float* srs_out)
 {
     float ss = 0.0f, sss = 0.0f, srs = 0.0f;
@@ -347,21 +378,21 @@ __device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, uint32_t p
      * offset (-1, -1)), sampled first: a constant patch sums to exactly 0 */
     uint32_t coff;
     float cfa, cfb;
-    tap_coord<kPath>(w, column(w, -1.0f), (float)pitch, -1.0f, &coff, &cfa, &cfb);
+    tap_coord<kPath>(w, column(w, -1.0f), pitch4, -1.0f, &coff, &cfa, &cfb);
     const Footprint cq = footprint(pool, coff);
-    fetch_column<kPath>(pool, pitch, w, -5.0f, cur, fa, fb);
+    fetch_column<kPath>(pool, pitch4, w, -5.0f, cur, fa, fb);
     const float shift = bilerp(cq, cfa, cfb, one);
     /* two columns per trip with the buffers alternating; the last two
      * columns follow the loop, so every prefetch is unconditional and the
      * waits stay on the older loads */
 #pragma unroll 1
     for (int a = 0; a < 4; a += 2) {
-        fetch_column<kPath>(pool, pitch, w, (float)(2 * a - 3), nxt, nfa, nfb);
+        fetch_column<kPath>(pool, pitch4, w, (float)(2 * a - 3), nxt, nfa, nfb);
         accumulate_column(rp, a, shift, cur, fa, fb, one, &ss, &sss, &srs);
-        fetch_column<kPath>(pool, pitch, w, (float)(2 * a - 1), cur, fa, fb);
+        fetch_column<kPath>(pool, pitch4, w, (float)(2 * a - 1), cur, fa, fb);
         accumulate_column(rp, a + 1, shift, nxt, nfa, nfb, one, &ss, &sss, &srs);
     }
-    fetch_column<kPath>(pool, pitch, w, 5.0f, nxt, nfa, nfb);
+    fetch_column<kPath>(pool, pitch4, w, 5.0f, nxt, nfa, nfb);
     accumulate_column(rp, 4, shift, cur, fa, fb, one, &ss, &sss, &srs);
     accumulate_column(rp, 5, shift, nxt, nfa, nfb, one, &ss, &sss, &srs);
     *ss_out = ss;
@@ -378,15 +409,15 @@ __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& 
     if (!make_warp(sv, q, px, py, &w)) {
         return 2.0f;
     }
-    const uint32_t pitch = (uint32_t)sv->pitch;
+    const int pitch4 = sv->pitch * 4;
     const __amdgpu_buffer_rsrc_t pool = pool_rsrc(pb.images_pad);
     float ss, sss, srs;
     if (__all(w.inner)) {
-        ncc_sums<kPathInner>(pool, pitch, pb.one, rp, w, &ss, &sss, &srs);
+        ncc_sums<kPathInner>(pool, pitch4, pb.one, rp, w, &ss, &sss, &srs);
     } else if (__all(w.small)) {
-        ncc_sums<kPathSmall>(pool, pitch, pb.one, rp, w, &ss, &sss, &srs);
+        ncc_sums<kPathSmall>(pool, pitch4, pb.one, rp, w, &ss, &sss, &srs);
     } else {
-        ncc_sums<kPathAny>(pool, pitch, pb.one, rp, w, &ss, &sss, &srs);
+        ncc_sums<kPathAny>(pool, pitch4, pb.one, rp, w, &ss, &sss, &srs);
     }
     const float ms = ss * rp.inv_w;
     const float var_s = sss * rp.inv_w - ms * ms;
@@ -803,17 +834,6 @@ __device__ __forceinline__ float prior_term(const PriorConsts& k, float depth_di
     return k.gamma + hpm_expf(-depth_diff * depth_diff / k.tdss) * hpm_expf(-ad * ad / k.tass);
 }
 
-/* the lowest source view some lane of the wave still needs */
-__device__ __forceinline__ int lowest_view(uint32_t todo, int nsrc)
-{
-    for (int i = 0; i < nsrc; ++i) {
-        if (__ballot((todo >> i) & 1u) != 0ull) {
-            return i;
-        }
-    }
-    return 0;
-}
-
 /* CheckerboardPropagation (HPM.cu:1222-1725) + PlaneHypothesisRefinement
  * (HPM.cu:1143-1220).
  *
@@ -824,10 +844,6 @@ __device__ __forceinline__ int lowest_view(uint32_t todo, int nsrc)
  * once per wave rather than once for every group of lanes that reaches it.
  * Per-view results are stored and summed in view order, so every value and
  * every sum order is the sequential one. */
-/* per wave of a 256-lane block: the lanes that selected the view being
- * refined, in lane order */
-__shared__ int s_own[4][64];
-
 __device__ __forceinline__ int lane_rank(uint64_t m)
 {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -844,6 +860,163 @@ __device__ __forceinline__ RefPatch shfl_ref_patch(const RefPatch& rp, int wv, i
     o.mr = __shfl(rp.mr, owner);
     o.var_r = __shfl(rp.var_r, owner);
     return o;
+}
+
+/* the number of the 15 view-selection draws per source view, 4 bits per
+ * view (views 0-15 in lo, 16-31 in hi): registers rather than an array */
+struct DrawCounts {
+    uint64_t lo, hi;
+};
+
+__device__ __forceinline__ float draws_of(const DrawCounts& d, int v)
+{
+    const uint64_t w = v < 16 ? d.lo : d.hi;
+    return (float)(uint32_t)((w >> (4 * (v & 15))) & 15u);
+}
+
+__device__ __forceinline__ void set_draws(DrawCounts* d, int v, float n)
+{
+    const uint64_t b = (uint64_t)(uint32_t)n << (4 * (v & 15));
+    if (v < 16) {
+        d->lo |= b;
+    } else {
+        d->hi |= b;
+    }
+}
+
+/* the candidates c of a view whose tasks [pv + c n, pv + (c + 1) n) meet
+ * the trip [t0, t0 + 64): cl .. ch */
+template <int C>
+__device__ __forceinline__ void cand_range(int pv, int n, int t0, int* cl, int* ch)
+{
+    int lo = 0, hi = 0;
+#pragma unroll
+    for (int c = 1; c < C; ++c) {
+        if (pv + c * n <= t0) {
+            lo = c;
+        }
+        if (pv + c * n < t0 + 64) {
+            hi = c;
+        }
+    }
+    *cl = lo;
+    *ch = hi;
+}
+
+/* per wave of a 256-lane block: the lanes that selected the view being
+ * refined, in lane order */
+__shared__ int s_own[4][64];
+
+/* per wave of a 256-lane block: the tasks of the current packed trip,
+ * owner lane | view << 8 | candidate << 16 */
+__shared__ int s_task[4][64];
+
+/* Packed evaluation in view-major order. For each source view v, in
+ * ascending order, the lanes with bit v of vmask each score C candidate
+ * planes on v; task t runs over (v, c, r): candidate c of the r-th such lane
+ * in lane order. Every trip of the wave takes the next 64 tasks: lane j
+ * scores task t0 + j for its owner, whose pixel, reference sums and
+ * candidate come by lane permutes (the window weights are in LDS at the
+ * owner's index), so only the last trip runs partly empty. take(v, c, value)
+ * then runs on the owner for each of its tasks in the trip, views
+ * ascending: per candidate, every owner receives its views in ascending
+ * order. Every lane of the wave calls this. */
+template <int C, typename PlaneOf, typename Score, typename Take>
+__device__ __forceinline__ void packed_views(int nsrc, const RefPatch& rp, int px, int py, uint32_t vmask,
+                                             PlaneOf&& plane_of, Score&& score, Take&& take)
+{
+    const int wv = (int)threadIdx.y, ln = (int)threadIdx.x; /* blockDim = (64, 4) */
+    int total = 0;
+#pragma unroll 1
+    for (int v = 0; v < nsrc; ++v) {
+        total += C * __popcll(__ballot((vmask >> v) & 1u));
+    }
+    const int pxy = px | (py << 16);
+    int va = 0, pa = 0; /* the first view with tasks at or after t0, and its first task */
+#pragma unroll 1
+    for (int t0 = 0; t0 < total; t0 += 64) {
+#pragma unroll 1
+        for (;;) {
+            const int na = C * __popcll(__ballot((vmask >> va) & 1u));
+            if (pa + na > t0) {
+                break;
+            }
+            pa += na;
+            ++va;
+        }
+        /* the trip's task table and the candidates it covers */
+        int c_lo = C - 1, c_hi = 0;
+        int vb = va;
+#pragma unroll 1
+        for (int pv = pa; vb < nsrc && pv < t0 + 64; ++vb) {
+            const bool mine = ((vmask >> vb) & 1u) != 0u;
+            const uint64_t m = __ballot(mine);
+            const int n = __popcll(m);
+            if (n > 0) {
+                int cl, ch;
+                cand_range<C>(pv, n, t0, &cl, &ch);
+                c_lo = min(c_lo, cl);
+                c_hi = max(c_hi, ch);
+                const int slot = pv + lane_rank(m) - t0;
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const int i = slot + c * n;
+                    if (mine && c >= cl && c <= ch && i >= 0 && i < 64) {
+                        s_task[wv][i] = ln | (vb << 8) | (c << 16);
+                    }
+                }
+            }
+            pv += C * n;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const bool act = t0 + ln < total;
+        const int e = act ? s_task[wv][ln] : ln;
+        const int owner = e & 0xff, v = (e >> 8) & 0xff, c = e >> 16;
+        const int opxy = __shfl(pxy, owner);
+        const RefPatch orp = shfl_ref_patch(rp, wv, owner);
+        float4 plane = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc) {
+            if (cc >= c_lo && cc <= c_hi) {
+                const float4 mp = plane_of(cc);
+                const float4 pc =
+                    make_float4(__shfl(mp.x, owner), __shfl(mp.y, owner), __shfl(mp.z, owner), __shfl(mp.w, owner));
+                if (c == cc) {
+                    plane = pc;
+                }
+            }
+        }
+        float term = 0.0f;
+        if (act) {
+            term = score(orp, opxy & 0xffff, opxy >> 16, v, plane);
+        }
+        /* each owner takes back its terms */
+        int pv = pa;
+#pragma unroll 1
+        for (int vv = va; vv < vb; ++vv) {
+            const bool mine = ((vmask >> vv) & 1u) != 0u;
+            const uint64_t m = __ballot(mine);
+            const int n = __popcll(m);
+            if (n > 0) {
+                int cl, ch;
+                cand_range<C>(pv, n, t0, &cl, &ch);
+                const int slot = pv + lane_rank(m) - t0;
+#pragma unroll
+                for (int cc = 0; cc < C; ++cc) {
+                    if (cc >= cl && cc <= ch) {
+                        const int src = slot + cc * n; /* the lane that scored (vv, cc) for this lane */
+                        const bool got = mine && src >= 0 && src < 64;
+                        const float val = __shfl(term, got ? src : ln);
+                        if (got) {
+                            take(vv, cc, val);
+                        }
+                    }
+                }
+            }
+            pv += C * n;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
 }
 
 /* Every lane of the wave calls this (the refinement packs evaluations of
@@ -987,7 +1160,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     const uint32_t all_views = nsrc >= 32 ? 0xffffffffu : ((1u << nsrc) - 1u);
 
     /* state handed between the phases */
-    float vw[HPM_MAX_SOURCES];
+    DrawCounts vw = {0ull, 0ull}; /* draws per source view */
     uint32_t sel_views = 0u; /* views with vw > 0 */
     uint32_t tsel = 0u;
     float wn = 0.0f;
@@ -1031,10 +1204,9 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     /* The rest: up to three NESP extension rounds, joint view selection,
      * the rescore of the current hypothesis, adoption and five refinement
      * candidates. The phases run in step across the wave, so each decision
-     * runs once per wave; inside the rescore and refinement phases every
-     * trip scores one source view (the one most lanes still need) for the
-     * lanes that selected it. Values, sum orders and decisions per lane are
-     * those of the sequential order. */
+     * runs once per wave; the extension, rescore and refinement evaluations
+     * of all lanes are packed into full trips (packed_views). Values, sum
+     * orders and decisions per lane are those of the sequential order. */
 
     /* NESP extension, orientation 0 only (HPM.cu:1473-1543, 780-843): up to
      * 3 rounds, each judged on ca[0] */
@@ -1091,62 +1263,19 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 pos[0] = lu;
             }
         }
-        const uint64_t em = __ballot(extend);
-        if (em != 0ull) {
+        if (__any(extend)) {
             if (extend) {
-                plane_q(pb, ld4(pb.hyp, pos[0]), q);
                 nev += (uint32_t)nsrc;
             }
-            const int n = __popcll(em);
-            const int total = n * nsrc;
-            if ((total + 63) / 64 < nsrc) {
-                /* few lanes extend: their (view, lane) evaluations are
-                 * packed into full trips, view-major (task t = v n + r
-                 * scores view v for the r-th extending lane) */
-                const int wv = (int)threadIdx.y, ln = (int)threadIdx.x; /* blockDim = (64, 4) */
-                const int rank = lane_rank(em);
-                if (extend) {
-                    s_own[wv][rank] = ln;
-                }
-                __builtin_amdgcn_wave_barrier();
-                const int pxy = px | (py << 16);
-#pragma unroll 1
-                for (int t0 = 0; t0 < total; t0 += 64) {
-                    const int t = t0 + ln;
-                    const bool act = t < total;
-                    const int v = act ? t / n : 0;
-                    const int owner = act ? s_own[wv][t - v * n] : ln;
-                    const int opxy = __shfl(pxy, owner);
-                    const RefPatch orp = shfl_ref_patch(rp, wv, owner);
+            const float4 ext_plane = extend ? ld4(pb.hyp, pos[0]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            packed_views<1>(
+                nsrc, rp, px, py, extend ? all_views : 0u, [&](int) { return ext_plane; },
+                [&](const RefPatch& orp, int opx, int opy, int v, float4 plane) {
                     double tq[3];
-#pragma unroll
-                    for (int k2 = 0; k2 < 3; ++k2) {
-                        tq[k2] = __shfl(q[k2], owner);
-                    }
-                    float term = 0.0f;
-                    if (act) {
-                        term = ncc(pb, orp, opxy & 0xffff, opxy >> 16, v + 1, tq);
-                    }
-                    const int v_hi = min(nsrc - 1, (t0 + 63) / n);
-#pragma unroll 1
-                    for (int vv = t0 / n; vv <= v_hi; ++vv) {
-                        const int src = vv * n + rank - t0; /* lane that scored (vv, this lane) */
-                        const bool got = extend && src >= 0 && src < 64;
-                        const float val = __shfl(term, got ? src : ln);
-                        if (got) {
-                            ca[0][vv] = val;
-                        }
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-            } else {
-#pragma unroll 1
-                for (int v = 0; v < nsrc; ++v) {
-                    if (extend) {
-                        ca[0][v] = ncc(pb, rp, px, py, v + 1, q);
-                    }
-                }
-            }
+                    plane_q(pb, plane, tq);
+                    return ncc(pb, orp, opx, opy, v + 1, tq);
+                },
+                [&](int v, int, float val) { ca[0][v] = val; });
         }
     }
 
@@ -1221,13 +1350,13 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                         n += 1.0f;
                     }
                 }
-                vw[i] = n;
+                set_draws(&vw, i, n);
             }
         }
         for (int i = 0; i < nsrc; ++i) {
-            if (vw[i] > 0) {
+            if (draws_of(vw, i) > 0) {
                 tsel |= (1u << i);
-                wn += vw[i];
+                wn += draws_of(vw, i);
             }
         }
         sel_views = valid ? tsel : 0u;
@@ -1249,7 +1378,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 }
                 for (int j = 0; j < nsrc; ++j) {
                     if ((tsel >> j) & 1u) {
-                        const float w = vw[j];
+                        const float w = draws_of(vw, j);
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             const float g = flag[i] ? geom_cost(pb, j + 1, hp[i], px, py) : 5.0f;
@@ -1260,7 +1389,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             } else {
                 for (int j = 0; j < nsrc; ++j) {
                     if ((tsel >> j) & 1u) {
-                        const float w = vw[j];
+                        const float w = draws_of(vw, j);
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             a[i] += w * ca[i][j];
@@ -1276,27 +1405,27 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
         min_idx = find_min_idx8(fc);
     }
 
-    /* re-score the current hypothesis on the selected views
-     * (HPM.cu:1637-1650); the trips take the views in ascending order, so
-     * the weighted sum accumulates in view order */
-    plane_q(pb, cur, q);
     /* weight of the geometric term in the re-score and the refinement:
      * 0.1 (HPM.cu:1640, 1194), 0.2 in ACMM (ACMM.cu:1596, 651) */
     const float gc_weight = P.acmm ? 0.2f : 0.1f;
+    /* the term of one hypothesis on one selected view: the NCC cost, plus
+     * the weighted geometric cost in geometric-consistency mode */
+    auto selected_term = [&](const RefPatch& orp, int opx, int opy, int v, float4 plane) {
+        double tq[3];
+        plane_q(pb, plane, tq);
+        const float cv = ncc(pb, orp, opx, opy, v + 1, tq);
+        return P.geom_consistency ? cv + gc_weight * geom_cost(pb, v + 1, plane, opx, opy) : cv;
+    };
+
+    /* re-score the current hypothesis on the selected views
+     * (HPM.cu:1637-1650), packed over the wave's lanes and views; the views
+     * arrive in ascending order, so the weighted sum accumulates in view
+     * order */
     float rescore_sum = 0.0f;
-    {
-        uint32_t todo = sel_views;
-        while (__any(todo != 0u)) {
-            const int vi = lowest_view(todo, nsrc);
-            if ((todo >> vi) & 1u) {
-                todo &= ~(1u << vi);
-                const float c = ncc(pb, rp, px, py, vi + 1, q);
-                ++nev;
-                const float term = P.geom_consistency ? c + gc_weight * geom_cost(pb, vi + 1, cur, px, py) : c;
-                rescore_sum += vw[vi] * term;
-            }
-        }
-    }
+    packed_views<1>(
+        nsrc, rp, px, py, sel_views, [&](int) { return cur; }, selected_term,
+        [&](int v, int, float val) { rescore_sum += draws_of(vw, v) * val; });
+    nev += (uint32_t)__popc(sel_views);
 
     {
         cost_now = rescore_sum / wn;
@@ -1419,7 +1548,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 s_own[wv][rank] = ln;
             }
             __builtin_amdgcn_wave_barrier();
-            const float wview = vw[vi];
+            const float wview = draws_of(vw, vi);
             const int total = 5 * n;
 #pragma unroll 1
             for (int t0 = 0; t0 < total; t0 += 64) {
@@ -2072,6 +2201,8 @@ __global__ __launch_bounds__(256) void hpm_k_selftest(int kind, const void* in, 
     case 0: y = hpm_expf(x); break;
     case 1: y = hpm_sinf(x); break;
     case 2: y = hpm_cosf(x); break;
+    case 5: y = fract1(x); break;
+    case 6: y = __int_as_float(floor_int(x)); break;
     default: y = hpm_acosf(x); break;
     }
     static_cast<float*>(out)[i] = y;

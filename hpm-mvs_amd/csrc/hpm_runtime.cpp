@@ -1236,7 +1236,7 @@ extern "C" int hpm_debug_write_state(hpm_handle* h, const float* hyp4, const flo
 
 extern "C" int hpm_device_selftest(hpm_handle* h, int kind, const void* in, void* out, int n)
 {
-    if (!h || !in || !out || n < 0 || kind < 0 || kind > 4) {
+    if (!h || !in || !out || n < 0 || kind < 0 || kind > 6) {
         return hpm_fail(h, HPM_EINVAL, "bad selftest arguments");
     }
     if (n == 0) {

@@ -15,9 +15,13 @@
  *     (pixel, launch);
  *   - the homography and the warped window centre in FP64 per (hypothesis,
  *     source view), the 36 tap offsets from the centre in FP32 with one
- *     reciprocal 1 / (1 + u) per tap: the cubic 1 - u + u^2 - u^3 when the
- *     window bounds |u| by 0.01 (truncation below 1.02e-8 relative, under
- *     half an FP32 ulp), the correctly rounded quotient otherwise;
+ *     reciprocal 1 / (1 + u) per tap: the quadratic 1 - u + u^2 when the
+ *     window bounds |u| by 0.002 (truncation below 8.1e-9 relative), else
+ *     the cubic 1 - u + u^2 - u^3 when it bounds |u| by 0.01 (below
+ *     1.02e-8), both under half an FP32 ulp, the correctly rounded quotient
+ *     otherwise; bilinear fractions x - floor(x) capped at the largest float
+ *     below 1 (they differ from the plain difference only where it rounds
+ *     up to 1, for x just below an integer);
  *   - NCC window sums of both signals accumulated relative to the reference
  *     centre value: NCC is invariant to a shift of either signal, and the
  *     shifted sums keep FP32 accurate on low-texture patches;
@@ -177,6 +181,7 @@ struct Warp {
     float fx0, fy0;
     float ex, ey, gxx, gxy, gyx, gyy;
     bool small; /* |e . d| <= 0.01 over the window: cubic reciprocal */
+    bool tiny;  /* |e . d| <= 0.002 over the window: quadratic reciprocal */
 };
 
 /* Returns false when the warped centre falls outside the source image (the
@@ -213,15 +218,27 @@ static inline bool make_warp(const ViewRel& v, const double q[3], int px, int py
     w->gyx = (float)((H[3] - sy0 * H[6]) * icz);
     w->gyy = (float)((H[4] - sy0 * H[7]) * icz);
     w->small = (fabsf(w->ex) + fabsf(w->ey)) * radius <= 0.01f;
+    w->tiny = (fabsf(w->ex) + fabsf(w->ey)) * radius <= 0.002f;
     return true;
 }
 
-/* 1 / (1 + u) of a tap: the cubic 1 - u + u^2 - u^3 in Horner form when the
- * warp bounds |u| by 0.01 (truncation u^4 / (1 + u) <= 1.02e-8 relative),
- * else the correctly rounded quotient. */
-static inline float tap_inv(float u, bool small)
+/* 1 / (1 + u) of a tap: the quadratic 1 - u + u^2 in Horner form when the
+ * warp bounds |u| by 0.002 (truncation |u|^3 / (1 + u) <= 8.1e-9 relative),
+ * the cubic 1 - u + u^2 - u^3 when it bounds |u| by 0.01 (u^4 / (1 + u) <=
+ * 1.02e-8), else the correctly rounded quotient. */
+static inline float tap_inv(float u, const Warp& w)
 {
-    return small ? fmaf(u, fmaf(u, 1.0f - u, -1.0f), 1.0f) : 1.0f / (1.0f + u);
+    if (w.tiny) {
+        return fmaf(u, u - 1.0f, 1.0f);
+    }
+    return w.small ? fmaf(u, fmaf(u, 1.0f - u, -1.0f), 1.0f) : 1.0f / (1.0f + u);
+}
+
+/* The bilinear fraction of x: x - floor(x), capped at the largest float
+ * below 1 (the difference rounds to 1 for x just below an integer). */
+static inline float fract1(float x)
+{
+    return fminf(x - floorf(x), 0x1.fffffep-1f);
 }
 
 /* tex2D with cudaFilterModeLinear and clamp addressing (HPM.cpp:599-617) at
@@ -231,17 +248,15 @@ static inline float sample_tap(const float* img, int W, int H, const Warp& w, fl
 {
     /* the dx products are per window column, the dy terms fused per tap */
     const float u = fmaf(w.ey, dy, w.ex * dx);
-    const float inv = tap_inv(u, w.small);
+    const float inv = tap_inv(u, w);
     float lx = fmaf(fmaf(w.gxy, dy, w.gxx * dx), inv, w.fx0);
     float ly = fmaf(fmaf(w.gyy, dy, w.gyx * dx), inv, w.fy0);
     lx = fminf(fmaxf(lx, -1.0f - (float)w.ix0), (float)(W - w.ix0));
     ly = fminf(fmaxf(ly, -1.0f - (float)w.iy0), (float)(H - w.iy0));
-    const float flx = floorf(lx);
-    const float fly = floorf(ly);
-    const float a = lx - flx;
-    const float b = ly - fly;
-    const int ix = w.ix0 + (int)flx;
-    const int iy = w.iy0 + (int)fly;
+    const float a = fract1(lx);
+    const float b = fract1(ly);
+    const int ix = w.ix0 + (int)floorf(lx);
+    const int iy = w.iy0 + (int)floorf(ly);
     const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
     const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
     const float t00 = img[(size_t)y0 * W + x0];
@@ -366,9 +381,8 @@ static inline float sample_abs(const float* img, int W, int H, float x, float y)
 {
     x = fminf(fmaxf(x, -1.0f), (float)W);
     y = fminf(fmaxf(y, -1.0f), (float)H);
-    const float fx = floorf(x), fy = floorf(y);
-    const float a = x - fx, b = y - fy;
-    const int ix = (int)fx, iy = (int)fy;
+    const float a = fract1(x), b = fract1(y);
+    const int ix = (int)floorf(x), iy = (int)floorf(y);
     const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
     const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
     const float t00 = img[(size_t)y0 * W + x0], t10 = img[(size_t)y0 * W + x1];
@@ -516,6 +530,7 @@ static double ncc_warp32(const Ctx& cx, const RefPatch& rp, int px, int py, int 
     w.gyx = (H[3] - sy0 * H[6]) * icz;
     w.gyy = (H[4] - sy0 * H[7]) * icz;
     w.small = (fabsf(w.ex) + fabsf(w.ey)) * (float)(pb->params.patch_size / 2) <= 0.01f;
+    w.tiny = (fabsf(w.ex) + fabsf(w.ey)) * (float)(pb->params.patch_size / 2) <= 0.002f;
     float ss_ = 0.0f, sss = 0.0f, srs = 0.0f;
     const int kc = rp.centre_tap;
     const float shift = sample_tap(simg, SW, SH, w, rp.dx[kc], rp.dy[kc]);
@@ -2000,6 +2015,12 @@ extern "C" int hpm_oracle_fmath(int fn, const float* x, float* y, int n)
         case 1: y[i] = hpm_sinf(x[i]); break;
         case 2: y[i] = hpm_cosf(x[i]); break;
         case 3: y[i] = hpm_acosf(x[i]); break;
+        case 5: y[i] = fract1(x[i]); break;
+        case 6: {
+            const int32_t f = (int32_t)floorf(x[i]);
+            memcpy(&y[i], &f, sizeof(f));
+            break;
+        }
         default: return -1;
         }
     }

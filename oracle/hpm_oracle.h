@@ -151,7 +151,9 @@ int hpm_oracle_jbu_prior(const float* guide, int w, int h, const float* src_nd, 
                          int num_threads);
 
 /* Host evaluation of the shared elementary functions (hpm_fmath.h):
- * fn 0 exp, 1 sin, 2 cos, 3 acos. */
+ * fn 0 exp, 1 sin, 2 cos, 3 acos; and of the tap sampling's integer and
+ * fraction split: fn 5 the bilinear fraction (x - floor(x), capped at the
+ * largest float below 1), fn 6 floor(x) as an int32 stored in y's bits. */
 int hpm_oracle_fmath(int fn, const float* x, float* y, int n);
 /* The value a source texel is sampled as (hpm_half.h): nearest FP16. */
 int hpm_oracle_round_f16(const float* x, float* y, int n);

@@ -48,6 +48,19 @@ def test_shared_math_bitwise(hpm, cpu):
                            len(x))
         got = pr.device_selftest(kind, x)
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), kind
+    # the tap sampling's fraction (5) and floor (6): random values, values
+    # just below and at integers, tiny and subnormal negatives, signed zeros
+    edge = [np.nextafter(np.float32(k), np.float32(-np.inf)) for k in range(-9, 10)] + list(range(-9, 10))
+    edge += [-1e-9, -1e-30, -1e-40, -0.0, 0.0, 1e-40, 0.99999994, -0.99999994, 4194303.5, -4194303.5]
+    x = np.concatenate([np.array(edge, np.float32), rng.uniform(-12.0, 12.0, 200000).astype(np.float32),
+                        (rng.integers(-2**20, 2**20, 50000) / np.float32(4096.0) - 1e-7).astype(np.float32)])
+    for kind in (5, 6):
+        ref = np.empty_like(x)
+        L.hpm_oracle_fmath(kind, x.ctypes.data_as(C.POINTER(C.c_float)), ref.ctypes.data_as(C.POINTER(C.c_float)),
+                           len(x))
+        got = pr.device_selftest(kind, x)
+        bad = np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))
+        assert bad.size == 0, (kind, x[bad[:5]], got[bad[:5]], ref[bad[:5]])
     blocks = rng.integers(0, 2**32, size=(4096, 6), dtype=np.uint64).astype(np.uint32)
     got = pr.device_selftest(4, blocks)
     for i in range(0, len(blocks), 97):

@@ -3,6 +3,8 @@
  */
 #include "hpm_prior.h"
 
+#include "hpm_dt.h"
+
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -12,290 +14,87 @@
 
 namespace {
 
-typedef __int128 i128;
+/* The host grid of the star walk (hpm_dt.h): cells of cs pixels over the
+ * points' bounding box, each listing its points; the convex hull by
+ * Andrew's monotone chain. */
+struct HostGrid {
+    const std::vector<hpm_dt::P2>* P;
+    int x0, y0, gw, gh, cs;
+    std::vector<int> start, items;
+    std::vector<hpm_dt::P2> H;
+    int nhull;
 
-struct Pt {
-    int64_t x, y;
+    hpm_dt::P2 pt(int i) const { return (*P)[(size_t)i]; }
+    int cx(hpm_dt::P2 p) const { return (p.x - x0) / cs; }
+    int cy(hpm_dt::P2 p) const { return (p.y - y0) / cs; }
+    int cxd(double x) const { return (int)fmax(-1.0, fmin((double)gw, floor((x - x0) / cs))); }
+    int cyd(double y) const { return (int)fmax(-1.0, fmin((double)gh, floor((y - y0) / cs))); }
+    hpm_dt::P2 hull(int k) const { return H[(size_t)k]; }
+    template <class F>
+    void visit(int x, int y, F&& f) const
+    {
+        const size_t c = (size_t)y * gw + x;
+        for (int k = start[c]; k < start[c + 1]; ++k) {
+            f(items[(size_t)k]);
+        }
+    }
+
+    /* keep: the points to triangulate (distinct coordinates) */
+    HostGrid(const std::vector<hpm_dt::P2>& pts, const std::vector<int>& keep)
+    {
+        P = &pts;
+        int x1 = pts[(size_t)keep[0]].x, y1 = pts[(size_t)keep[0]].y;
+        x0 = x1;
+        y0 = y1;
+        for (int i : keep) {
+            x0 = std::min(x0, pts[(size_t)i].x);
+            y0 = std::min(y0, pts[(size_t)i].y);
+            x1 = std::max(x1, pts[(size_t)i].x);
+            y1 = std::max(y1, pts[(size_t)i].y);
+        }
+        /* about one point per cell */
+        const double area = ((double)x1 - x0 + 1) * ((double)y1 - y0 + 1);
+        cs = std::max(1, (int)ceil(sqrt(area / (double)keep.size())));
+        gw = (x1 - x0) / cs + 1;
+        gh = (y1 - y0) / cs + 1;
+        start.assign((size_t)gw * gh + 1, 0);
+        for (int i : keep) {
+            start[(size_t)cy(pts[(size_t)i]) * gw + cx(pts[(size_t)i]) + 1] += 1;
+        }
+        for (size_t c = 0; c < (size_t)gw * gh; ++c) {
+            start[c + 1] += start[c];
+        }
+        items.resize(keep.size());
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int i : keep) {
+            items[(size_t)fill[(size_t)cy(pts[(size_t)i]) * gw + cx(pts[(size_t)i])]++] = i;
+        }
+        /* monotone chain over (x, y) */
+        std::vector<hpm_dt::P2> s;
+        s.reserve(keep.size());
+        for (int i : keep) {
+            s.push_back(pts[(size_t)i]);
+        }
+        std::sort(s.begin(), s.end(),
+                  [](hpm_dt::P2 a, hpm_dt::P2 b) { return a.x < b.x || (a.x == b.x && a.y < b.y); });
+        std::vector<hpm_dt::P2> h(2 * s.size() + 1);
+        size_t k = 0;
+        for (size_t i = 0; i < s.size(); ++i) {
+            while (k >= 2 && hpm_dt::orient(h[k - 2], h[k - 1], s[i]) <= 0) {
+                --k;
+            }
+            h[k++] = s[i];
+        }
+        for (size_t i = s.size() - 1, t = k + 1; i > 0; --i) {
+            while (k >= t && hpm_dt::orient(h[k - 2], h[k - 1], s[i - 1]) <= 0) {
+                --k;
+            }
+            h[k++] = s[i - 1];
+        }
+        H.assign(h.begin(), h.begin() + (long)(k > 1 ? k - 1 : k));
+        nhull = (int)H.size();
+    }
 };
-
-/* > 0 when c lies to the left of a->b (counter-clockwise in a y-up frame) */
-inline int64_t orient(const Pt& a, const Pt& b, const Pt& c)
-{
-    return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
-}
-
-/* > 0 when d lies strictly inside the circumcircle of the positively
- * oriented triangle (a, b, c); exact in 64 bits while every coordinate
- * difference is below 2^14 (12 D^4 < 2^63) */
-inline int incircle64(const Pt& a, const Pt& b, const Pt& c, const Pt& d)
-{
-    const int64_t adx = a.x - d.x, ady = a.y - d.y;
-    const int64_t bdx = b.x - d.x, bdy = b.y - d.y;
-    const int64_t cdx = c.x - d.x, cdy = c.y - d.y;
-    const int64_t ad = adx * adx + ady * ady;
-    const int64_t bd = bdx * bdx + bdy * bdy;
-    const int64_t cd = cdx * cdx + cdy * cdy;
-    const int64_t det = adx * (bdy * cd - bd * cdy) - ady * (bdx * cd - bd * cdx) + ad * (bdx * cdy - bdy * cdx);
-    return det > 0 ? 1 : (det < 0 ? -1 : 0);
-}
-
-/* the same with 128-bit products, for triangles of the enclosing triangle */
-inline int incircle(const Pt& a, const Pt& b, const Pt& c, const Pt& d)
-{
-    const i128 adx = a.x - d.x, ady = a.y - d.y;
-    const i128 bdx = b.x - d.x, bdy = b.y - d.y;
-    const i128 cdx = c.x - d.x, cdy = c.y - d.y;
-    const i128 ad = adx * adx + ady * ady;
-    const i128 bd = bdx * bdx + bdy * bdy;
-    const i128 cd = cdx * cdx + cdy * cdy;
-    const i128 det = adx * (bdy * cd - bd * cdy) - ady * (bdx * cd - bd * cdx) + ad * (bdx * cdy - bdy * cdx);
-    return det > 0 ? 1 : (det < 0 ? -1 : 0);
-}
-
-struct Tri {
-    int v[3];   /* positively oriented */
-    int nb[3];  /* neighbour across the edge opposite v[i], -1 if none */
-    bool alive;
-};
-
-/* Bowyer-Watson insertion with a visibility walk for point location. */
-class Delaunay {
-public:
-    explicit Delaunay(const std::vector<Pt>& pts) : P(pts)
-    {
-        int64_t maxc = 1;
-        for (const Pt& p : P) {
-            const int64_t ax = p.x < 0 ? -p.x : p.x, ay = p.y < 0 ? -p.y : p.y;
-            maxc = std::max(maxc, std::max(ax, ay));
-        }
-        const int64_t S = 64 * (maxc + 16);
-        n_real = (int)P.size();
-        small_coords = 2 * maxc < ((int64_t)1 << 14);
-        P.push_back({-S, -S});
-        P.push_back({3 * S, -S});
-        P.push_back({-S, 3 * S});
-        T.reserve(2 * (size_t)n_real + 16);
-        marks.assign(2 * (size_t)n_real + 16, 0);
-        Tri t;
-        t.v[0] = n_real;
-        t.v[1] = n_real + 1;
-        t.v[2] = n_real + 2;
-        t.nb[0] = t.nb[1] = t.nb[2] = -1;
-        t.alive = true;
-        T.push_back(t);
-        last = 0;
-    }
-
-    bool insert(int pi)
-    {
-        const Pt& p = P[pi];
-        int t = locate(p);
-        if (t < 0) {
-            return false;
-        }
-        for (int i = 0; i < 3; ++i) {
-            const Pt& q = P[T[t].v[i]];
-            if (q.x == p.x && q.y == p.y) {
-                return false; /* duplicate point */
-            }
-        }
-        /* cavity: triangles whose circumcircle strictly contains p */
-        cavity.clear();
-        stack.clear();
-        mark_gen++;
-        if (marks.size() < T.size()) {
-            marks.resize(T.size() * 2, 0);
-        }
-        stack.push_back(t);
-        marks[t] = mark_gen;
-        while (!stack.empty()) {
-            const int c = stack.back();
-            stack.pop_back();
-            cavity.push_back(c);
-            for (int e = 0; e < 3; ++e) {
-                const int n = T[c].nb[e];
-                if (n < 0 || marks[n] == mark_gen) {
-                    continue;
-                }
-                const Tri& tn = T[n];
-                const bool real = tn.v[0] < n_real && tn.v[1] < n_real && tn.v[2] < n_real && small_coords;
-                const int in = real ? incircle64(P[tn.v[0]], P[tn.v[1]], P[tn.v[2]], p)
-                                    : incircle(P[tn.v[0]], P[tn.v[1]], P[tn.v[2]], p);
-                if (in > 0) {
-                    marks[n] = mark_gen;
-                    stack.push_back(n);
-                }
-            }
-        }
-        /* boundary edges (a, b) of the cavity with the outside neighbour */
-        boundary.clear();
-        for (int c : cavity) {
-            for (int e = 0; e < 3; ++e) {
-                const int n = T[c].nb[e];
-                if (n >= 0 && marks[n] == mark_gen) {
-                    continue;
-                }
-                Edge ed;
-                ed.a = T[c].v[(e + 1) % 3];
-                ed.b = T[c].v[(e + 2) % 3];
-                ed.outside = n;
-                ed.outside_edge = -1;
-                if (n >= 0) {
-                    for (int k = 0; k < 3; ++k) {
-                        if (T[n].nb[k] == c) {
-                            ed.outside_edge = k;
-                        }
-                    }
-                }
-                boundary.push_back(ed);
-            }
-        }
-        for (int c : cavity) {
-            T[c].alive = false;
-            free_slots.push_back(c);
-        }
-        /* new triangles (a, b, p), linked around p through the map a -> tri */
-        created.clear();
-        for (const Edge& ed : boundary) {
-            int slot;
-            if (!free_slots.empty()) {
-                slot = free_slots.back();
-                free_slots.pop_back();
-            } else {
-                slot = (int)T.size();
-                T.push_back(Tri());
-            }
-            Tri& nt = T[slot];
-            nt.v[0] = ed.a;
-            nt.v[1] = ed.b;
-            nt.v[2] = pi;
-            nt.nb[2] = ed.outside;
-            nt.nb[0] = -1;
-            nt.nb[1] = -1;
-            nt.alive = true;
-            if (ed.outside >= 0 && ed.outside_edge >= 0) {
-                T[ed.outside].nb[ed.outside_edge] = slot;
-            }
-            created.push_back(slot);
-        }
-        /* edge (b, p) of triangle (a, b, p) is opposite a (nb[0]); it is
-         * shared with the triangle whose first vertex is b */
-        first_of.clear();
-        for (int s : created) {
-            first_of.emplace_back(T[s].v[0], s);
-        }
-        std::sort(first_of.begin(), first_of.end());
-        for (int s : created) {
-            const int b = T[s].v[1];
-            auto it = std::lower_bound(first_of.begin(), first_of.end(), std::make_pair(b, -1));
-            if (it != first_of.end() && it->first == b) {
-                const int o = it->second;
-                T[s].nb[0] = o;
-                T[o].nb[1] = s;
-            }
-        }
-        if (marks.size() < T.size()) {
-            marks.resize(T.size() * 2, 0);
-        }
-        last = created.empty() ? last : created[0];
-        return true;
-    }
-
-    /* triangles made only of real points, in slot order */
-    void output(std::vector<int32_t>& out) const
-    {
-        out.reserve(3 * T.size());
-        for (const Tri& t : T) {
-            if (!t.alive || t.v[0] >= n_real || t.v[1] >= n_real || t.v[2] >= n_real) {
-                continue;
-            }
-            out.push_back(t.v[0]);
-            out.push_back(t.v[1]);
-            out.push_back(t.v[2]);
-        }
-    }
-
-private:
-    struct Edge {
-        int a, b, outside, outside_edge;
-    };
-
-    int locate(const Pt& p)
-    {
-        int t = last;
-        if (t < 0 || t >= (int)T.size() || !T[t].alive) {
-            t = -1;
-            for (int i = (int)T.size() - 1; i >= 0; --i) {
-                if (T[i].alive) {
-                    t = i;
-                    break;
-                }
-            }
-        }
-        const size_t limit = 4 * T.size() + 64;
-        for (size_t step = 0; step < limit && t >= 0; ++step) {
-            int next = -1;
-            for (int e = 0; e < 3; ++e) {
-                const Pt& a = P[T[t].v[(e + 1) % 3]];
-                const Pt& b = P[T[t].v[(e + 2) % 3]];
-                if (orient(a, b, p) < 0) {
-                    next = T[t].nb[e];
-                    break;
-                }
-            }
-            if (next < 0) {
-                return t; /* inside or on the boundary of t */
-            }
-            t = next;
-        }
-        /* fall back to a scan (never expected on a Delaunay triangulation) */
-        for (int i = 0; i < (int)T.size(); ++i) {
-            if (!T[i].alive) {
-                continue;
-            }
-            bool in = true;
-            for (int e = 0; e < 3 && in; ++e) {
-                in = orient(P[T[i].v[(e + 1) % 3]], P[T[i].v[(e + 2) % 3]], p) >= 0;
-            }
-            if (in) {
-                return i;
-            }
-        }
-        return -1;
-    }
-
-    std::vector<Pt> P;
-    std::vector<Tri> T;
-    int n_real;
-    bool small_coords; /* real points differ by less than 2^14 per axis */
-    int last;
-    std::vector<int> cavity, stack, free_slots, created;
-    std::vector<Edge> boundary;
-    std::vector<std::pair<int, int>> first_of;
-    std::vector<uint32_t> marks;
-    uint32_t mark_gen = 0;
-};
-
-/* position of (x, y) along a Hilbert curve over [0, 2^16)^2 */
-inline uint64_t hilbert_key(uint32_t x, uint32_t y)
-{
-    uint64_t d = 0;
-    for (uint32_t s = 1u << 15; s > 0; s >>= 1) {
-        const uint32_t rx = (x & s) ? 1u : 0u;
-        const uint32_t ry = (y & s) ? 1u : 0u;
-        d += (uint64_t)s * s * ((3u * rx) ^ ry);
-        if (ry == 0) {
-            if (rx == 1) {
-                x = s - 1 - (x & (s - 1)) + (x & ~(s - 1));
-                y = s - 1 - (y & (s - 1)) + (y & ~(s - 1));
-            }
-            const uint32_t t = x;
-            x = y;
-            y = t;
-        }
-    }
-    return d;
-}
 
 /* Static 2-D tree over integer points for exact k-nearest-neighbour
  * queries ordered by (squared distance, index), the order the KNN
@@ -461,31 +260,68 @@ extern "C" int hpm_delaunay(const int32_t* points_xy, int n, int32_t* tris, int 
     if (n < 3) {
         return 0;
     }
-    std::vector<Pt> pts((size_t)n);
+    std::vector<hpm_dt::P2> pts((size_t)n);
     for (int i = 0; i < n; ++i) {
-        pts[i] = {points_xy[2 * i], points_xy[2 * i + 1]};
+        pts[(size_t)i] = {points_xy[2 * i], points_xy[2 * i + 1]};
     }
-    Delaunay dt(pts);
-    /* insertion along a Hilbert curve: short walks and small cavities (in
-     * grid order the cavities of near-lattice support points are large) */
-    int32_t minx = (int32_t)pts[0].x, miny = (int32_t)pts[0].y;
-    for (const Pt& p : pts) {
-        minx = (int32_t)std::min<int64_t>(minx, p.x);
-        miny = (int32_t)std::min<int64_t>(miny, p.y);
-    }
-    std::vector<std::pair<uint64_t, int>> order((size_t)n);
+    /* of points with equal coordinates the first one is triangulated */
+    std::vector<int> byxy((size_t)n);
     for (int i = 0; i < n; ++i) {
-        order[i] = {hilbert_key((uint32_t)(pts[i].x - minx) & 0xffffu, (uint32_t)(pts[i].y - miny) & 0xffffu), i};
+        byxy[(size_t)i] = i;
     }
-    std::sort(order.begin(), order.end());
-    for (const auto& o : order) {
-        dt.insert(o.second);
+    std::sort(byxy.begin(), byxy.end(), [&](int a, int b) {
+        const hpm_dt::P2 p = pts[(size_t)a], q = pts[(size_t)b];
+        return p.x < q.x || (p.x == q.x && (p.y < q.y || (p.y == q.y && a < b)));
+    });
+    std::vector<char> dup((size_t)n, 0);
+    for (size_t k = 1; k < byxy.size(); ++k) {
+        const hpm_dt::P2 p = pts[(size_t)byxy[k - 1]], q = pts[(size_t)byxy[k]];
+        if (p.x == q.x && p.y == q.y) {
+            dup[(size_t)byxy[k]] = 1;
+        }
     }
-    std::vector<int32_t> out;
-    dt.output(out);
-    const int nt = (int)(out.size() / 3);
+    std::vector<int> keep;
+    for (int i = 0; i < n; ++i) {
+        if (!dup[(size_t)i]) {
+            keep.push_back(i);
+        }
+    }
+    if (keep.size() < 3) {
+        return 0;
+    }
+    const HostGrid g(pts, keep);
+    /* the stars are independent: count, place, then write them in parallel */
+    const int nk = (int)keep.size();
+    std::vector<int> count((size_t)nk + 1, 0);
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : bad)
+    for (int k = 0; k < nk; ++k) {
+        int c = 0;
+        const int r = hpm_dt::star(g, keep[(size_t)k], 4 * n + 16, [&](int, int) { ++c; });
+        count[(size_t)k + 1] = c;
+        bad += r < 0;
+    }
+    if (bad) {
+        return -2;
+    }
+    for (int k = 0; k < nk; ++k) {
+        count[(size_t)k + 1] += count[(size_t)k];
+    }
+    const int nt = count[(size_t)nk];
     if (tris) {
-        memcpy(tris, out.data(), sizeof(int32_t) * 3 * (size_t)std::min(nt, max_tris));
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int k = 0; k < nk; ++k) {
+            const int p = keep[(size_t)k];
+            int at = count[(size_t)k];
+            hpm_dt::star(g, p, 4 * n + 16, [&](int a, int b) {
+                if (at < max_tris) {
+                    tris[3 * at] = p;
+                    tris[3 * at + 1] = a;
+                    tris[3 * at + 2] = b;
+                }
+                ++at;
+            });
+        }
     }
     return nt;
 }

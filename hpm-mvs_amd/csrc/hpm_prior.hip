@@ -6,13 +6,15 @@
  * functions of hpm_prior.cpp and the assembly loops of hpm_pipeline.cpp
  * compute, in the same arithmetic (no contraction; float where those use
  * float, double where they use double), so the prior is the same bit for
- * bit. Delaunay stays on the host (hpm_delaunay).
+ * bit. The Delaunay triangulation runs the star walk of hpm_dt.h, the
+ * same algorithm as the host's hpm_delaunay, over the support-point blocks.
  */
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
 
 #include "hpm_device.h"
+#include "hpm_dt.h"
 
 namespace {
 
@@ -357,6 +359,266 @@ __global__ __launch_bounds__(256) void hpm_k_fill_u32(uint32_t* p, int64_t n, ui
     }
 }
 
+/* ---- support points, their Delaunay triangulation and the KNN cells ---- */
+
+/* one workgroup: out[0] = 0, out[i + 1] = in[0] + ... + in[i] (in may be
+ * 0/1 flags of cand >= 0 when flags is set) */
+__global__ __launch_bounds__(1024) void hpm_k_scan(const int32_t* in, int n, int flags, int32_t* out)
+{
+    __shared__ int32_t part[1024];
+    const int t = (int)threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = min(n, t * per), e = min(n, b + per);
+    int32_t sum = 0;
+    for (int i = b; i < e; ++i) {
+        sum += flags ? (in[i] >= 0 ? 1 : 0) : in[i];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int32_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int32_t run = t > 0 ? part[t - 1] : 0;
+    if (t == 0) {
+        out[0] = 0;
+    }
+    for (int i = b; i < e; ++i) {
+        run += flags ? (in[i] >= 0 ? 1 : 0) : in[i];
+        out[i + 1] = run;
+    }
+}
+
+/* the support points in block order (GetSupportPoints' scan): pts[k] for
+ * the k-th non-empty block, and each block's point index (or -1) */
+__global__ __launch_bounds__(256) void hpm_k_support_compact(const int32_t* cand, const int32_t* pos, int nb,
+                                                             int32_t* pts, int32_t* bpoint)
+{
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= nb) {
+        return;
+    }
+    const int32_t c = cand[b];
+    if (c >= 0) {
+        const int k = pos[b];
+        pts[2 * k] = c & 0xffff;
+        pts[2 * k + 1] = c >> 16;
+        bpoint[b] = k;
+    } else {
+        bpoint[b] = -1;
+    }
+}
+
+/* per pixel column x, the points with that x of least and greatest y
+ * (their point indices, -1 if none): the convex hull's vertices are among
+ * them */
+__global__ __launch_bounds__(256) void hpm_k_hull_columns(const int32_t* pts, const int32_t* bpoint, int nby, int ws,
+                                                          int32_t* col_lo, int32_t* col_hi)
+{
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (x >= ws) {
+        return;
+    }
+    const int bc = x / 5;
+    int lo = -1, hi = -1;
+    for (int br = 0; br < nby; ++br) {
+        const int i = bpoint[bc * nby + br];
+        if (i >= 0 && pts[2 * i] == x) {
+            if (lo < 0) {
+                lo = i;
+            }
+            hi = i;
+        }
+    }
+    col_lo[x] = lo;
+    col_hi[x] = hi;
+}
+
+/* Andrew's monotone chain over the column extremes, which come sorted by
+ * (x, y): the hull counter-clockwise (orient > 0), as the host's chain over
+ * all points gives it. One thread. hull has room for 4 ws + 2 points (the
+ * chain's stack over at most 2 ws points). */
+__global__ void hpm_k_hull_chain(const int32_t* pts, const int32_t* col_lo, const int32_t* col_hi, int ws,
+                                 int32_t* hull, int32_t* nhull)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) {
+        return;
+    }
+    using hpm_dt::P2;
+    auto pt = [&](int i) { return P2{pts[2 * i], pts[2 * i + 1]}; };
+    int k = 0, n = 0;
+    /* lower chain */
+    for (int x = 0; x < ws; ++x) {
+        for (int e = 0; e < 2; ++e) {
+            const int i = e == 0 ? col_lo[x] : col_hi[x];
+            if (i < 0 || (e == 1 && i == col_lo[x])) {
+                continue;
+            }
+            ++n;
+            const P2 p = pt(i);
+            while (k >= 2 && hpm_dt::orient(P2{hull[2 * (k - 2)], hull[2 * (k - 2) + 1]},
+                                            P2{hull[2 * (k - 1)], hull[2 * (k - 1) + 1]}, p) <= 0) {
+                --k;
+            }
+            hull[2 * k] = p.x;
+            hull[2 * k + 1] = p.y;
+            ++k;
+        }
+    }
+    /* upper chain, from the second-largest point down */
+    const int t = k + 1;
+    bool first = true;
+    for (int x = ws - 1; x >= 0; --x) {
+        for (int e = 1; e >= 0; --e) {
+            const int i = e == 0 ? col_lo[x] : col_hi[x];
+            if (i < 0 || (e == 1 && i == col_lo[x])) {
+                continue;
+            }
+            if (first) { /* the largest point ends the lower chain */
+                first = false;
+                continue;
+            }
+            const P2 p = pt(i);
+            while (k >= t && hpm_dt::orient(P2{hull[2 * (k - 2)], hull[2 * (k - 2) + 1]},
+                                            P2{hull[2 * (k - 1)], hull[2 * (k - 1) + 1]}, p) <= 0) {
+                --k;
+            }
+            hull[2 * k] = p.x;
+            hull[2 * k + 1] = p.y;
+            ++k;
+        }
+    }
+    *nhull = n < 3 ? 0 : (k > 1 ? k - 1 : k);
+}
+
+/* the star walk's grid over the support blocks: 5-pixel cells, at most one
+ * point each (blocks numbered column-major) */
+struct BlockGrid {
+    const int32_t* P;
+    const int32_t* bpoint;
+    const int32_t* H;
+    int gw, gh, cs, nhull;
+
+    __device__ hpm_dt::P2 pt(int i) const { return hpm_dt::P2{P[2 * i], P[2 * i + 1]}; }
+    __device__ int cx(hpm_dt::P2 p) const { return p.x / 5; }
+    __device__ int cy(hpm_dt::P2 p) const { return p.y / 5; }
+    __device__ int cxd(double x) const { return (int)fmax(-1.0, fmin((double)gw, floor(x / 5.0))); }
+    __device__ int cyd(double y) const { return (int)fmax(-1.0, fmin((double)gh, floor(y / 5.0))); }
+    __device__ hpm_dt::P2 hull(int k) const { return hpm_dt::P2{H[2 * k], H[2 * k + 1]}; }
+    template <class F>
+    __device__ void visit(int x, int y, F&& f) const
+    {
+        const int i = bpoint[x * gh + y];
+        if (i >= 0) {
+            f(i);
+        }
+    }
+};
+
+/* pass 0: the number of triangles each point emits (count[p + 1]); pass 1:
+ * the triangles, at the offsets the scan of the counts gives */
+__global__ __launch_bounds__(256) void hpm_k_delaunay(const int32_t* pts, const int32_t* npts_p, const int32_t* bpoint,
+                                                      int nbx, int nby, const int32_t* hull, const int32_t* nhull,
+                                                      int pass, int32_t* count, int32_t* tris, int max_tris,
+                                                      int32_t* err)
+{
+    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int npts = *npts_p;
+    if (p >= nbx * nby) {
+        return;
+    }
+    if (p >= npts) { /* the scan runs over one count per block */
+        if (pass == 0) {
+            count[p + 1] = 0;
+        }
+        return;
+    }
+    BlockGrid g;
+    g.P = pts;
+    g.bpoint = bpoint;
+    g.H = hull;
+    g.gw = nbx;
+    g.gh = nby;
+    g.cs = 5;
+    g.nhull = *nhull;
+    if (npts < 3) {
+        if (pass == 0) {
+            count[p + 1] = 0;
+        }
+        return;
+    }
+    if (pass == 0) {
+        int c = 0;
+        const int r = hpm_dt::star(g, p, 4 * npts + 16, [&](int, int) { ++c; });
+        count[p + 1] = c;
+        if (r < 0) {
+            atomicOr(err, 1);
+        }
+    } else {
+        int at = count[p];
+        hpm_dt::star(g, p, 4 * npts + 16, [&](int a, int b) {
+            if (at < max_tris) {
+                tris[3 * at] = p;
+                tris[3 * at + 1] = a;
+                tris[3 * at + 2] = b;
+            } else {
+                atomicOr(err, 2);
+            }
+            ++at;
+        });
+    }
+}
+
+/* the rasteriser's running index: every support point lies in the image,
+ * so every triangle is counted (imageRC.contains, main.cpp:97) */
+__global__ __launch_bounds__(256) void hpm_k_iota(int32_t* p, const int32_t* n_p)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < *n_p) {
+        p[i] = i;
+    }
+}
+
+/* The KNN search grid: cells of `cell` px, point indices ascending within a
+ * cell. A cell's points come from the blocks it meets, visited column-major
+ * (ascending point index); pass 0 counts, pass 1 writes at the scanned
+ * offsets. */
+__global__ __launch_bounds__(256) void hpm_k_knn_cells(const int32_t* pts, const int32_t* bpoint, int nbx, int nby,
+                                                       int cell, int gw, int gh, int pass, int32_t* count,
+                                                       int32_t* items)
+{
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= gw * gh) {
+        return;
+    }
+    const int cx = c % gw, cy = c / gw;
+    const int bc0 = cx * cell / 5, bc1 = min(nbx - 1, (cx * cell + cell - 1) / 5);
+    const int br0 = cy * cell / 5, br1 = min(nby - 1, (cy * cell + cell - 1) / 5);
+    int n = 0;
+    int at = pass == 1 ? count[c] : 0;
+    for (int bc = bc0; bc <= bc1; ++bc) {
+        for (int br = br0; br <= br1; ++br) {
+            const int i = bpoint[bc * nby + br];
+            if (i < 0) {
+                continue;
+            }
+            const int x = pts[2 * i], y = pts[2 * i + 1];
+            if (min(x / cell, gw - 1) != cx || min(y / cell, gh - 1) != cy) {
+                continue;
+            }
+            if (pass == 1) {
+                items[at++] = i;
+            }
+            ++n;
+        }
+    }
+    if (pass == 0) {
+        count[c + 1] = n;
+    }
+}
+
 extern "C" int hpm_launch_prior_maps(const float* hyp4, const float* cost, int W, int H, float* depth_s,
                                      float* cost_s, int ws, int hs, void* stream)
 {
@@ -420,4 +682,53 @@ extern "C" int hpm_launch_prior_assemble(const uint32_t* raster, const float* tr
                        blk, 0, (hipStream_t)stream, raster, tri_planes, knn_flag, knn_planes, width, height, *ref,
                        factor, depth_min, depth_max, src_nd, planes, mask);
     return (int)hipGetLastError();
+}
+
+/* The support points of the block candidates, their Delaunay triangulation
+ * and the KNN cells, all on the device. ws_i32 has room for the scans and
+ * the hull (3 nb + 10 ws + 16 int32). nums receives (npts, ntris, error). */
+extern "C" int hpm_launch_support_delaunay(const int32_t* cand, int ws, int hs, int cell, int gw, int gh,
+                                           int32_t* pts, int32_t* tris, int32_t* tri_idx, int32_t* cell_start,
+                                           int32_t* cell_pts, int32_t* ws_i32, int32_t* nums, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    const int nbx = (ws + 4) / 5, nby = (hs + 4) / 5, nb = nbx * nby;
+    int32_t* pos = ws_i32;                /* nb + 1 */
+    int32_t* bpoint = pos + nb + 1;       /* nb */
+    int32_t* col_lo = bpoint + nb;        /* ws */
+    int32_t* col_hi = col_lo + ws;        /* ws */
+    int32_t* hull = col_hi + ws;          /* 2 (4 ws + 2): the chain's stack over <= 2 ws points */
+    int32_t* nhull = hull + 2 * (4 * ws + 2);
+    int32_t* tcount = nhull + 1;          /* nb + 1 (points <= blocks) */
+    int32_t* err = nums + 2;
+    hipLaunchKernelGGL(hpm_k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)err, (int64_t)1, 0u);
+    hipLaunchKernelGGL(hpm_k_scan, dim3(1), dim3(1024), 0, s, cand, nb, 1, pos);
+    hipLaunchKernelGGL(hpm_k_support_compact, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, cand, pos, nb, pts,
+                       bpoint);
+    hipLaunchKernelGGL(hpm_k_hull_columns, dim3((unsigned)((ws + 255) / 256)), dim3(256), 0, s, pts, bpoint, nby, ws,
+                       col_lo, col_hi);
+    hipLaunchKernelGGL(hpm_k_hull_chain, dim3(1), dim3(64), 0, s, pts, col_lo, col_hi, ws, hull, nhull);
+    const int32_t* npts_p = pos + nb;
+    /* at most nb points: the grids cover every block, threads past npts return */
+    const unsigned gp = (unsigned)((nb + 255) / 256);
+    /* a triangulation of n points has at most 2 n - 5 triangles */
+    const int max_tris = 2 * nb;
+    hipLaunchKernelGGL(hpm_k_delaunay, dim3(gp), dim3(256), 0, s, pts, npts_p, bpoint, nbx, nby, hull, nhull, 0,
+                       tcount, tris, max_tris, err);
+    hipLaunchKernelGGL(hpm_k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)tcount, (int64_t)1, 0u);
+    hipLaunchKernelGGL(hpm_k_scan, dim3(1), dim3(1024), 0, s, tcount + 1, nb, 0, tcount);
+    hipLaunchKernelGGL(hpm_k_delaunay, dim3(gp), dim3(256), 0, s, pts, npts_p, bpoint, nbx, nby, hull, nhull, 1,
+                       tcount, tris, max_tris, err);
+    hipLaunchKernelGGL(hpm_k_iota, dim3((unsigned)((max_tris + 255) / 256)), dim3(256), 0, s, tri_idx, tcount + nb);
+    hipLaunchKernelGGL(hpm_k_knn_cells, dim3((unsigned)((gw * gh + 255) / 256)), dim3(256), 0, s, pts, bpoint, nbx,
+                       nby, cell, gw, gh, 0, cell_start, cell_pts);
+    hipLaunchKernelGGL(hpm_k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)cell_start, (int64_t)1, 0u);
+    hipLaunchKernelGGL(hpm_k_scan, dim3(1), dim3(1024), 0, s, cell_start + 1, gw * gh, 0, cell_start);
+    hipLaunchKernelGGL(hpm_k_knn_cells, dim3((unsigned)((gw * gh + 255) / 256)), dim3(256), 0, s, pts, bpoint, nbx,
+                       nby, cell, gw, gh, 1, cell_start, cell_pts);
+    hipError_t e = hipMemcpyAsync(nums, npts_p, 4, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) {
+        e = hipMemcpyAsync(nums + 1, tcount + nb, 4, hipMemcpyDeviceToDevice, s);
+    }
+    return e != hipSuccess ? (int)e : (int)hipGetLastError();
 }

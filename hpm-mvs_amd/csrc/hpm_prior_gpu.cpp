@@ -3,10 +3,12 @@
  * the handle's last RunPatchMatch (main.cpp:84-162 at 0.25x, 238-314 at
  * 0.5x, 387-454 at 1x; HPM.cpp:759-1123): the scale maps, the support-point
  * blocks, the rasterised triangle mask, the triangle planes, the KNN
- * supplement and the prior maps run as kernels (hpm_prior.hip); the
- * support points are gathered and triangulated on the host (hpm_delaunay),
- * which also sorts them into the KNN search grid. The result equals the
- * host construction of hpm_pipeline.cpp bit for bit.
+ * supplement and the prior maps run as kernels (hpm_prior.hip), as do the
+ * gathering of the support points, their Delaunay triangulation (the star
+ * walk of hpm_dt.h, which the host's hpm_delaunay also runs) and the KNN
+ * search grid; only the point and triangle counts come back to size the
+ * launches. The result equals the host construction of hpm_pipeline.cpp bit
+ * for bit.
  */
 #include <hip/hip_runtime.h>
 
@@ -32,6 +34,9 @@ int hpm_launch_knn(const int32_t* pts, int npoints, const int32_t* cell_start, c
                    int gw, int gh, const int32_t* order, int norder, int k, const uint32_t* raster, const float* depth,
                    int width, int height, const hpm_camera* ref, float factor, float depth_min, float depth_max,
                    uint8_t* knn_flag, float* knn_planes, void* stream);
+int hpm_launch_support_delaunay(const int32_t* cand, int ws, int hs, int cell, int gw, int gh, int32_t* pts,
+                                int32_t* tris, int32_t* tri_idx, int32_t* cell_start, int32_t* cell_pts,
+                                int32_t* ws_i32, int32_t* nums, void* stream);
 int hpm_launch_prior_assemble(const uint32_t* raster, const float* tri_planes, const uint8_t* knn_flag,
                               const float* knn_planes, int width, int height, const hpm_camera* ref, float factor,
                               float depth_min, float depth_max, float* src_nd, float* planes, uint32_t* mask,
@@ -109,7 +114,8 @@ extern "C" int hpm_build_planar_prior(hpm_handle* h, float factor, int knn_k, do
     const size_t maxp = (size_t)nb, maxt = 2 * (size_t)nb;
     auto layout = [&](Carve& c, float** depth_s, float** cost_s, int32_t** cand, int32_t** pts, int32_t** tris,
                       int32_t** tidx, uint32_t** raster, float** tplanes, int32_t** cstart, int32_t** cpts,
-                      int32_t** ord, uint8_t** kflag, float** kplanes, float** src_nd) {
+                      int32_t** ord, uint8_t** kflag, float** kplanes, float** src_nd, int32_t** scratch,
+                      int32_t** nums) {
         *depth_s = c.take<float>(ns);
         *cost_s = c.take<float>(ns);
         *cand = c.take<int32_t>((size_t)nb);
@@ -124,14 +130,16 @@ extern "C" int hpm_build_planar_prior(hpm_handle* h, float factor, int knn_k, do
         *kflag = c.take<uint8_t>(ns);
         *kplanes = c.take<float>(4 * ns);
         *src_nd = c.take<float>(factor == 1.0f ? 0 : 4 * ns);
+        *scratch = c.take<int32_t>(3 * (size_t)nb + 10 * (size_t)ws + 16);
+        *nums = c.take<int32_t>(4);
     };
     float *depth_s, *cost_s, *tplanes, *kplanes, *src_nd;
-    int32_t *cand, *pts, *tris, *tidx, *cstart, *cpts, *ord;
+    int32_t *cand, *pts, *tris, *tidx, *cstart, *cpts, *ord, *scratch, *nums;
     uint32_t* raster;
     uint8_t* kflag;
     Carve sizing{nullptr};
     layout(sizing, &depth_s, &cost_s, &cand, &pts, &tris, &tidx, &raster, &tplanes, &cstart, &cpts, &ord, &kflag,
-           &kplanes, &src_nd);
+           &kplanes, &src_nd, &scratch, &nums);
     void* ws_ptr = nullptr;
     int rc = hpm_workspace(h, sizing.off, &ws_ptr);
     if (rc != HPM_OK) {
@@ -139,71 +147,26 @@ extern "C" int hpm_build_planar_prior(hpm_handle* h, float factor, int knn_k, do
     }
     Carve carve{(char*)ws_ptr};
     layout(carve, &depth_s, &cost_s, &cand, &pts, &tris, &tidx, &raster, &tplanes, &cstart, &cpts, &ord, &kflag,
-           &kplanes, &src_nd);
+           &kplanes, &src_nd, &scratch, &nums);
     hipStream_t s = h->stream;
-    std::vector<int32_t> h_cand((size_t)nb);
     HPM_HIP(h, (hipError_t)hpm_launch_prior_maps(h->d_hyp, h->d_cost, W, H, depth_s, cost_s, ws, hs, s));
     HPM_HIP(h, (hipError_t)hpm_launch_support_blocks(cost_s, ws, hs, cand, s));
-    HPM_HIP(h, hipMemcpyAsync(h_cand.data(), cand, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
-    HPM_HIP(h, hipStreamSynchronize(s));
+    HPM_HIP(h, (hipError_t)hpm_launch_support_delaunay(cand, ws, hs, cell, gw, gh, pts, tris, tidx, cstart, cpts,
+                                                        scratch, nums, s));
+    /* the counts size the per-triangle and per-point launches */
+    int32_t h_nums[3] = {0, 0, 0};
     auto t0 = clk::now();
-    /* support points in block order (GetSupportPoints' scan) */
-    std::vector<int32_t> h_pts;
-    h_pts.reserve(2 * maxp);
-    for (int b = 0; b < nb; ++b) {
-        if (h_cand[(size_t)b] >= 0) {
-            h_pts.push_back(h_cand[(size_t)b] & 0xffff);
-            h_pts.push_back(h_cand[(size_t)b] >> 16);
-        }
-    }
-    const int npts = (int)(h_pts.size() / 2);
-    std::vector<int32_t> h_tris;
-    if (npts >= 3) {
-        h_tris.resize((size_t)npts * 2 * 3);
-        const int nt = hpm_delaunay(h_pts.data(), npts, h_tris.data(), npts * 2);
-        h_tris.resize((size_t)std::max(nt, 0) * 3);
-    }
-    const int ntris = (int)(h_tris.size() / 3);
-    /* the rasteriser's running index over triangles with all three
-     * vertices inside the image (imageRC.contains, main.cpp:97) */
-    std::vector<int32_t> h_tidx((size_t)ntris);
-    int next = 0;
-    for (int t = 0; t < ntris; ++t) {
-        bool in = true;
-        for (int v = 0; v < 3; ++v) {
-            const int x = h_pts[2 * (size_t)h_tris[3 * (size_t)t + v]], y = h_pts[2 * (size_t)h_tris[3 * (size_t)t + v] + 1];
-            in = in && x >= 0 && x < ws && y >= 0 && y < hs;
-        }
-        h_tidx[(size_t)t] = in ? next++ : -1;
-    }
-    /* the KNN grid: point indices by cell, ascending within a cell */
-    std::vector<int32_t> h_cstart((size_t)gw * gh + 1, 0), h_cpts((size_t)std::max(npts, 1));
-    for (int i = 0; i < npts; ++i) {
-        const int c = std::min(h_pts[2 * (size_t)i + 1] / cell, gh - 1) * gw + std::min(h_pts[2 * (size_t)i] / cell, gw - 1);
-        h_cstart[(size_t)c + 1] += 1;
-    }
-    for (size_t c = 0; c < (size_t)gw * gh; ++c) {
-        h_cstart[c + 1] += h_cstart[c];
-    }
-    {
-        std::vector<int32_t> fill(h_cstart.begin(), h_cstart.end() - 1);
-        for (int i = 0; i < npts; ++i) {
-            const int c = std::min(h_pts[2 * (size_t)i + 1] / cell, gh - 1) * gw + std::min(h_pts[2 * (size_t)i] / cell, gw - 1);
-            h_cpts[(size_t)fill[(size_t)c]++] = i;
-        }
-    }
+    HPM_HIP(h, hipMemcpyAsync(h_nums, nums, sizeof(h_nums), hipMemcpyDeviceToHost, s));
+    HPM_HIP(h, hipStreamSynchronize(s));
     if (ms_host) {
         *ms_host += std::chrono::duration<double, std::milli>(clk::now() - t0).count();
     }
-    if (npts > 0) {
-        HPM_HIP(h, hipMemcpyAsync(pts, h_pts.data(), h_pts.size() * 4, hipMemcpyHostToDevice, s));
-        HPM_HIP(h, hipMemcpyAsync(cpts, h_cpts.data(), (size_t)npts * 4, hipMemcpyHostToDevice, s));
+    if (h_nums[2] != 0 || h_nums[1] > (int)maxt) {
+        return hpm_fail(h, HPM_EDEVICE, "planar prior: the Delaunay star walk failed (flags %d, %d triangles)",
+                        h_nums[2], h_nums[1]);
     }
-    if (ntris > 0) {
-        HPM_HIP(h, hipMemcpyAsync(tris, h_tris.data(), h_tris.size() * 4, hipMemcpyHostToDevice, s));
-        HPM_HIP(h, hipMemcpyAsync(tidx, h_tidx.data(), h_tidx.size() * 4, hipMemcpyHostToDevice, s));
-    }
-    HPM_HIP(h, hipMemcpyAsync(cstart, h_cstart.data(), h_cstart.size() * 4, hipMemcpyHostToDevice, s));
+    const int npts = h_nums[0];
+    const int ntris = h_nums[1];
     HPM_HIP(h, hipMemcpyAsync(ord, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
     HPM_HIP(h, (hipError_t)hpm_launch_raster(pts, tris, tidx, ntris, raster, ws, hs, s));
     HPM_HIP(h, (hipError_t)hpm_launch_tri_planes(pts, tris, ntris, depth_s, ws, &ref, factor, tplanes, s));

@@ -34,9 +34,14 @@ extern "C" {
 int hpm_support_points(const float* cost, int width, int height, int32_t* points_xy, int max_points);
 
 /* DelaunayTriangulation (HPM.cpp:787-809): triangles whose three vertices
- * are input points (indices into points_xy, counter-clockwise in image
- * coordinates). Returns the triangle count or a negative error; writes at
- * most max_tris triples. */
+ * are input points (indices into points_xy, positively oriented:
+ * (b - a) x (c - a) > 0). Co-circular ties are broken by point index (a
+ * symbolic perturbation of the lifted coordinate), so the triangulation is
+ * unique; of points with equal coordinates the first is used. Canonical
+ * order: by smallest vertex index, listed first, then around that vertex
+ * from its nearest neighbour (or its clockwise hull edge). Coordinate
+ * differences must stay below 2^14. Returns the triangle count or a
+ * negative error; writes at most max_tris triples. */
 int hpm_delaunay(const int32_t* points_xy, int n, int32_t* tris, int max_tris);
 
 /* Triangle rasterisation of main.cpp:96-116: barycentric stepping with

@@ -208,8 +208,9 @@ def test_executable_on_jpeg_folder(hpm, tmp_path):
 
 @pytest.mark.parametrize("size", [(320, 240), (1600, 1066)])
 def test_device_prior_matches_host_construction(hpm, size):
-    """hpm_build_planar_prior (support points, rasterisation, triangle planes,
-    KNN supplement and assembly on the device; Delaunay on the host) against
+    """hpm_build_planar_prior (support points, their Delaunay triangulation,
+    rasterisation, triangle planes, KNN grid and supplement and assembly, all
+    on the device) against
     the host construction of the same scale (oracle/pipeline.py:
     prior_scale_maps, the library's host functions), at 0.5x (through the
     JBU) and 1x, bit for bit."""

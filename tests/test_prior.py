@@ -1,9 +1,10 @@
 """Host prior construction against independent checks:
   - support points: brute force over the 5x5 blocks;
   - Delaunay: every triangle's circumcircle is empty of other points (exact
-    integer test), the triangles tile the convex hull (area), and the
-    triangle count matches scipy.spatial.Delaunay for points in general
-    position;
+    integer test), the triangles tile the convex hull (area), the triangles
+    equal scipy.spatial.Delaunay's (Qhull) for points in general position,
+    and on degenerate sets (lattices, co-circular points, collinear rows) a
+    brute force over all triples with the index tie-break;
   - rasterisation: brute-force replay of the reference's float stepping;
   - plane fit: the numpy SVD null vector of [X 1];
   - KNN supplement: neighbours against a brute-force sort by
@@ -76,6 +77,97 @@ def test_delaunay_empty_circumcircle(hpm, kind):
     if kind == "random":
         # general position: the Delaunay triangulation is unique
         assert len(tris) == len(Delaunay(pts.astype(np.float64)).simplices)
+
+
+def _orient(a, b, c):
+    return (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0])
+
+
+def _incircle_perturbed(P, ia, ib, ic, id_):
+    """sign of the incircle determinant of rows (x, y, x^2 + y^2 + eps_i, 1)
+    with eps_0 >> eps_1 >> ...: the exact determinant, else the first
+    non-zero cofactor of the lifted column by ascending point index"""
+    a, b, c, d = P[ia], P[ib], P[ic], P[id_]
+    det = _incircle_exact(a, b, c, d)
+    if det:
+        return 1 if det > 0 else -1
+    cof = {ia: _orient(b, c, d), ib: -_orient(a, c, d), ic: _orient(a, b, d), id_: -_orient(a, b, c)}
+    for i in sorted(cof):
+        if cof[i]:
+            return 1 if cof[i] > 0 else -1
+    return 0
+
+
+def _brute_force_triangulation(P):
+    """every positively oriented triple whose perturbed circumcircle holds
+    no other point, as (smallest index first, positively oriented)"""
+    n = len(P)
+    out = set()
+    for i in range(n):
+        for j in range(i + 1, n):
+            for k in range(j + 1, n):
+                o = _orient(P[i], P[j], P[k])
+                if o == 0:
+                    continue
+                a, b, c = (i, j, k) if o > 0 else (i, k, j)
+                if all(_incircle_perturbed(P, a, b, c, d) < 0 for d in range(n) if d not in (a, b, c)):
+                    out.add((a, b, c))
+    return out
+
+
+def _point_sets():
+    rng = np.random.default_rng(7)
+    yield "random", rng.integers(0, 60, (35, 2))
+    yield "lattice", np.array([(x, y) for x in range(0, 30, 5) for y in range(0, 25, 5)])
+    g = np.array([(x, y) for x in range(0, 40, 5) for y in range(0, 30, 5)])
+    yield "lattice_gaps", g[rng.uniform(size=len(g)) < 0.7]
+    # eight points on one circle, its centre, and two outside
+    yield "cocircular", np.array([(10, 0), (-10, 0), (0, 10), (0, -10), (6, 8), (-6, 8), (6, -8), (-8, -6), (0, 0),
+                                  (20, 3), (-3, 21)]) + 30
+    yield "collinear_rows", np.array([(x, y) for y in (0, 7, 14) for x in range(0, 40, 4)] + [(13, 3), (21, 10)])
+
+
+@pytest.mark.parametrize("case", [c for c, _ in _point_sets()])
+def test_delaunay_brute_force_with_tie_break(hpm, case):
+    """the triangulation equals the brute-force set of triangles with an
+    empty (perturbed) circumcircle, on degenerate sets: lattices (co-circular
+    quads everywhere), points on one circle, collinear rows; and it is in the
+    canonical order: smallest vertex index first, groups by ascending first
+    vertex"""
+    P = np.array(dict(_point_sets())[case], np.int64)
+    pts = P.astype(np.int32)
+    tris = hpm.delaunay(pts)
+    got = [tuple(int(v) for v in t) for t in tris]
+    assert len(set(got)) == len(got)
+    want = _brute_force_triangulation([tuple(p) for p in P])
+    assert set(got) == want
+    assert all(t[0] < t[1] and t[0] < t[2] for t in got)
+    firsts = [t[0] for t in got]
+    assert firsts == sorted(firsts)
+    # a permutation of the input gives the same triangles up to the relabel
+    # when the tie-break order is carried along (the perturbation follows the
+    # index, so it is relabelled too): here a reversal
+    rev = pts[::-1].copy()
+    n = len(pts)
+    got_rev = {tuple(sorted((n - 1 - v for v in t))) for t in hpm.delaunay(rev)}
+    want_rev = {tuple(sorted(t)) for t in _brute_force_triangulation([tuple(p) for p in rev.astype(np.int64)])}
+    assert {tuple(sorted(n - 1 - v for v in t)) for t in want_rev} == got_rev
+
+
+def test_delaunay_general_position_equals_qhull(hpm):
+    rng = np.random.default_rng(3)
+    pts = np.unique(rng.integers(0, 5000, (3000, 2)), axis=0).astype(np.int32)
+    rng.shuffle(pts)
+    got = {tuple(sorted(int(v) for v in t)) for t in hpm.delaunay(pts)}
+    want = {tuple(sorted(int(v) for v in t)) for t in Delaunay(pts.astype(np.float64)).simplices}
+    assert got == want
+
+
+def test_delaunay_duplicates_use_the_first(hpm):
+    pts = np.array([[0, 0], [10, 0], [0, 10], [10, 0], [10, 10], [0, 0]], np.int32)
+    tris = hpm.delaunay(pts)
+    used = set(int(v) for t in tris for v in t)
+    assert used == {0, 1, 2, 4}
 
 
 def test_rasterize_brute_force(hpm):

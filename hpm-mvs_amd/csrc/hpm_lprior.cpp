@@ -173,9 +173,14 @@ hipError_t forward(const hpm_lprior* lp, const void* x, int W, int H, float* hea
             e = (hipError_t)hpm_unet_avgpool2(in, ldi, out, c, c, hh, ww, s);
         }
     };
-    auto up = [&](const void* in, int c, void* out, int ldo, int hh, int ww) {
+    /* decoder layers: the upsampled half of the concatenation is read from
+     * the coarser map directly (no upsampled copy) */
+    auto conv_up = [&](int li, const void* up, int cup, const void* in, int ldi, void* out, int ldo, int hh,
+                       int ww) {
         if (e == hipSuccess) {
-            e = (hipError_t)hpm_unet_upsample2(in, c, out, ldo, c, hh, ww, s);
+            const Layer& l = L[li];
+            e = (hipError_t)hpm_unet_conv_fwd_up(in, ldi, l.ci, up, cup, cup, l.w, l.b, nullptr, out, ldo, l.co, hh,
+                                                 ww, l.relu, s);
         }
     };
     /* encoder; s0 = cat0[64:96], s1 = cat1[96:160], s2 = cat2[128:224] */
@@ -191,12 +196,9 @@ hipError_t forward(const hpm_lprior* lp, const void* x, int W, int H, float* hea
     conv(E3A, at(pl.p3), 96, at(pl.t3), 128, h3, w3, nullptr);
     conv(E3B, at(pl.t3), 128, at(pl.b3), 128, h3, w3, nullptr);
     /* decoder */
-    up(at(pl.b3), 128, at(pl.cat2), 224, h2, w2);
-    conv(D2, at(pl.cat2), 224, at(pl.d2), 96, h2, w2, nullptr);
-    up(at(pl.d2), 96, at(pl.cat1), 160, h1, w1);
-    conv(D1, at(pl.cat1), 160, at(pl.d1), 64, h1, w1, nullptr);
-    up(at(pl.d1), 64, at(pl.cat0), 96, H, W);
-    conv(D0, at(pl.cat0), 96, at(pl.d0), 32, H, W, nullptr);
+    conv_up(D2, at(pl.b3), 128, at(pl.cat2), 224, at(pl.d2), 96, h2, w2);
+    conv_up(D1, at(pl.d2), 96, at(pl.cat1), 160, at(pl.d1), 64, h1, w1);
+    conv_up(D0, at(pl.d1), 64, at(pl.cat0), 96, at(pl.d0), 32, H, W);
     conv(HEAD, at(pl.d0), 32, nullptr, 32, H, W, head);
     return e;
 }

@@ -148,20 +148,29 @@ __global__ __launch_bounds__(256) void hpm_k_conv_fwd(const hpm_bf16* __restrict
 }
 
 /* The 3x3 convolution with its operands staged in LDS: a block computes
- * one output row, 128 pixels (32 per wave) x 32 NT output channels; for
- * every chunk of 32 input channels the three input rows with their 1-pixel
- * halo (130 x 32, zero outside the image) and the weight slice (32 NT x 9
- * taps x 32) are loaded with coalesced 16-byte loads into LDS (rows padded
- * to 40 channels against bank conflicts), then every wave runs the
- * chunk's 9 taps x 2 MFMA steps per channel tile from LDS. Products are
- * accumulated chunk by chunk, tap by tap. */
+ * one output row, 128 pixels (32 per wave) x 32 NT output channels (channel
+ * group blockIdx.z); for every chunk of 32 input channels the three input
+ * rows with their 1-pixel halo (130 x 32, zero outside the image) and the
+ * weight slice (32 NT x 9 taps x 32) sit in LDS (rows padded to 40
+ * channels against bank conflicts) while every wave runs the chunk's 9 taps
+ * x 2 MFMA steps per channel tile. The next chunk's operands are loaded
+ * into registers while the current one is multiplied, then stored to LDS:
+ * the global loads overlap the matrix work. Products are accumulated chunk
+ * by chunk, tap by tap. With up != NULL, input channels [0, cup) are the
+ * nearest 2x upsampling of up ((h+1)/2 x (w+1)/2, stride ldu), read at
+ * (y/2, x/2) in place of an upsampled copy; cup is a multiple of 32. */
 constexpr int kLdsPad = 40;
 template <int NT>
 __global__ __launch_bounds__(256) void hpm_k_conv3_lds(const hpm_bf16* __restrict__ in, int ldi, int ci,
+                                                       const hpm_bf16* __restrict__ up, int ldu, int cup,
                                                        const hpm_bf16* __restrict__ w, const float* __restrict__ bias,
                                                        float* out32, hpm_bf16* out16, int ldo, int co, int height,
                                                        int width, int relu)
 {
+    constexpr int kInVec = 3 * 130 * 4;      /* 16-byte vectors of the input tile */
+    constexpr int kWVec = 32 * NT * 9 * 4;   /* and of the weight slice */
+    constexpr int kIn = (kInVec + 255) / 256;
+    constexpr int kW = (kWVec + 255) / 256;
     __shared__ __align__(16) hpm_bf16 s_in[3 * 130 * kLdsPad];
     __shared__ __align__(16) hpm_bf16 s_w[32 * NT * 9 * kLdsPad];
     const int tid = (int)threadIdx.x;
@@ -178,27 +187,60 @@ __global__ __launch_bounds__(256) void hpm_k_conv3_lds(const hpm_bf16* __restric
         acc[n] = f32x16{};
     }
     const bf16x8 zero = {};
-    for (int c0 = 0; c0 < ci; c0 += 32) {
-        /* input rows y-1..y+1, columns x_blk-1 .. x_blk+128: 3 x 130 pixels
-         * x 4 vectors of 8 channels */
-        for (int e = tid; e < 3 * 130 * 4; e += 256) {
+    bf16x8 rin[kIn], rw[kW];
+    /* this thread's vectors of the chunk at input channel c0 */
+    const int wu = (width + 1) / 2;
+    auto fetch = [&](int c0) {
+        const bool from_up = c0 < cup;
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) {
+            const int e = tid + 256 * k;
             const int v = e & 3, pix = e >> 2;
             const int row = pix / 130, col = pix % 130;
             const int sy = y + row - 1, sx = x_blk + col - 1;
-            bf16x8 val = zero;
-            if (sy >= 0 && sy < height && sx >= 0 && sx < width) {
-                val = ld8(in + ((int64_t)sy * width + sx) * ldi + c0 + 8 * v);
+            rin[k] = zero;
+            if (e < kInVec && sy >= 0 && sy < height && sx >= 0 && sx < width) {
+                rin[k] = from_up ? ld8(up + ((int64_t)(sy >> 1) * wu + (sx >> 1)) * ldu + c0 + 8 * v)
+                                 : ld8(in + ((int64_t)sy * width + sx) * ldi + c0 + 8 * v);
             }
-            *reinterpret_cast<bf16x8*>(&s_in[(row * 130 + col) * kLdsPad + 8 * v]) = val;
         }
-        /* weights of the block's channels: 32 NT x 9 taps x 4 vectors */
-        for (int e = tid; e < 32 * NT * 9 * 4; e += 256) {
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            const int e = tid + 256 * k;
             const int v = e & 3, rt = e >> 2;
             const int nl = rt / 9, tap = rt % 9;
-            const bf16x8 val = ld8(w + ((int64_t)(n0 + nl) * 9 + tap) * ci + c0 + 8 * v);
-            *reinterpret_cast<bf16x8*>(&s_w[(nl * 9 + tap) * kLdsPad + 8 * v]) = val;
+            rw[k] = zero;
+            if (e < kWVec) {
+                rw[k] = ld8(w + ((int64_t)(n0 + nl) * 9 + tap) * ci + c0 + 8 * v);
+            }
         }
-        __syncthreads();
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) {
+            const int e = tid + 256 * k;
+            if (e < kInVec) {
+                const int v = e & 3, pix = e >> 2;
+                *reinterpret_cast<bf16x8*>(&s_in[pix * kLdsPad + 8 * v]) = rin[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kW; ++k) {
+            const int e = tid + 256 * k;
+            if (e < kWVec) {
+                const int v = e & 3, rt = e >> 2;
+                *reinterpret_cast<bf16x8*>(&s_w[rt * kLdsPad + 8 * v]) = rw[k];
+            }
+        }
+    };
+    fetch(0);
+    stash();
+    __syncthreads();
+    for (int c0 = 0; c0 < ci; c0 += 32) {
+        const bool more = c0 + 32 < ci;
+        if (more) {
+            fetch(c0 + 32);
+        }
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
             const int ky = tap / 3, kx = tap % 3;
@@ -215,6 +257,10 @@ __global__ __launch_bounds__(256) void hpm_k_conv3_lds(const hpm_bf16* __restric
             }
         }
         __syncthreads();
+        if (more) {
+            stash();
+            __syncthreads();
+        }
     }
     const int x0 = x_blk + wave * 32;
 #pragma unroll
@@ -360,6 +406,49 @@ extern "C" int hpm_unet_mfma_probe(const void* a, const void* b, float* d, void*
     return (int)hipGetLastError();
 }
 
+namespace {
+
+bool lds_eligible(int ci, int ks)
+{
+    return ks == 3 && ci >= 64 && ci % 32 == 0;
+}
+
+void launch_conv3_lds(const void* in, int ldi, int ci, const void* up, int ldu, int cup, const void* w,
+                      const float* bias, float* out32, void* out16, int ldo, int co, int height, int width, int relu,
+                      hipStream_t stream)
+{
+    /* 64 channels per block where co divides by 64 and the map gives four
+     * blocks per CU (LDS for two blocks per CU), else 32: the channel groups
+     * spread small maps over more blocks */
+    const int64_t blocks64 = (int64_t)((width + 127) / 128) * height * (co / 64);
+    const int nt = co % 64 == 0 && blocks64 >= 1024 ? 2 : 1;
+    const dim3 grid((unsigned)((width + 127) / 128), (unsigned)height, (unsigned)(co / (32 * nt)));
+    if (nt == 1) {
+        hipLaunchKernelGGL(hpm_k_conv3_lds<1>, grid, dim3(256), 0, stream, (const hpm_bf16*)in, ldi, ci,
+                           (const hpm_bf16*)up, ldu, cup, (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co,
+                           height, width, relu);
+    } else {
+        hipLaunchKernelGGL(hpm_k_conv3_lds<2>, grid, dim3(256), 0, stream, (const hpm_bf16*)in, ldi, ci,
+                           (const hpm_bf16*)up, ldu, cup, (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co,
+                           height, width, relu);
+    }
+}
+
+} // namespace
+
+extern "C" int hpm_unet_conv_fwd_up(const void* in, int ldi, int ci, const void* up, int ldu, int cup, const void* w,
+                                    const float* bias, float* out32, void* out16, int ldo, int co, int height,
+                                    int width, int relu, void* stream)
+{
+    if (!lds_eligible(ci, 3) || co % 32 != 0 || cup < 0 || cup % 32 != 0 || cup > ci || ldi % 8 != 0 ||
+        ldu % 8 != 0 || height < 1 || width < 1 || (cup > 0 && !up)) {
+        return (int)hipErrorInvalidValue;
+    }
+    launch_conv3_lds(in, ldi, ci, up, ldu, cup, w, bias, out32, out16, ldo, co, height, width, relu,
+                     (hipStream_t)stream);
+    return (int)hipGetLastError();
+}
+
 extern "C" int hpm_unet_conv_fwd(const void* in, int ldi, int ci, const void* w, const float* bias, float* out32,
                                  void* out16, int ldo, int co, int height, int width, int ks, int relu, void* stream)
 {
@@ -374,18 +463,11 @@ extern "C" int hpm_unet_conv_fwd(const void* in, int ldi, int ci, const void* w,
         hipLaunchKernelGGL(kernel, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)in, ldi, ci,
                            (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, ks, relu);
     };
-    if (ks == 3 && ci >= 64 && ci % 32 == 0 && (co == 32 || co == 64)) {
-        /* operands staged in LDS, all output channels in one block: pays
-         * where a block runs two or more input-channel chunks (with one, the
-         * loads are not hidden) and its LDS leaves two blocks per CU */
-        const dim3 grid((unsigned)((width + 127) / 128), (unsigned)height, 1);
-        if (co == 32) {
-            hipLaunchKernelGGL(hpm_k_conv3_lds<1>, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)in, ldi,
-                               ci, (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, relu);
-        } else {
-            hipLaunchKernelGGL(hpm_k_conv3_lds<2>, grid, dim3(256), 0, (hipStream_t)stream, (const hpm_bf16*)in, ldi,
-                               ci, (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, relu);
-        }
+    if (lds_eligible(ci, ks)) {
+        /* operands staged in LDS with the next chunk prefetched: pays where a
+         * block runs two or more input-channel chunks */
+        launch_conv3_lds(in, ldi, ci, nullptr, 0, 0, w, bias, out32, out16, ldo, co, height, width, relu,
+                         (hipStream_t)stream);
     } else if (co == 32) {
         launch(hpm_k_conv_fwd<2, 1>, 2, 1);
     } else if (co % 64 == 0) {

@@ -62,7 +62,9 @@ def _conv_ref(x_bf, w_bf, bias, ks, relu):
     return torch.relu(y) if relu else y
 
 
-@pytest.mark.parametrize("ks,relu,ci,co,h,w", [(3, 1, 32, 64, 13, 45), (3, 0, 16, 32, 7, 33), (1, 0, 96, 32, 5, 70)])
+@pytest.mark.parametrize("ks,relu,ci,co,h,w", [(3, 1, 32, 64, 13, 45), (3, 0, 16, 32, 7, 33), (1, 0, 96, 32, 5, 70),
+                                                (3, 1, 96, 96, 9, 131), (3, 0, 128, 128, 6, 257), (3, 1, 224, 96, 5, 140),
+                                                (3, 1, 64, 96, 3, 300), (3, 0, 160, 64, 4, 129), (3, 1, 96, 32, 7, 127)])
 def test_conv_forward(hpm, ks, relu, ci, co, h, w):
     say(f"case: conv forward ks={ks} relu={relu} ci={ci} co={co} {h}x{w}")
     g = torch.Generator().manual_seed(ks * 100 + ci)
@@ -84,6 +86,36 @@ def test_conv_forward(hpm, ks, relu, ci, co, h, w):
     scale = ref.abs().max().item()
     say(f"max abs err {err:.3e} of {scale:.3e}")
     assert err <= 1e-4 * max(scale, 1.0)
+
+
+@pytest.mark.parametrize("ci,cup,co,h,w", [(224, 128, 96, 9, 131), (160, 96, 64, 8, 130), (96, 64, 32, 7, 257),
+                                           (96, 64, 32, 6, 129)])
+def test_conv_with_fused_upsampling(hpm, ci, cup, co, h, w):
+    """hpm_unet_conv_fwd_up (channels [0, cup) read as the nearest 2x
+    upsampling of a coarser map) equals hpm_unet_conv_fwd on the explicitly
+    upsampled concatenation, bit for bit"""
+    say(f"case: conv with fused upsampling ci={ci} cup={cup} co={co} {h}x{w}")
+    g = torch.Generator().manual_seed(ci + cup)
+    hu, wu = (h + 1) // 2, (w + 1) // 2
+    up = torch.randn(hu, wu, cup, generator=g).to(torch.bfloat16)
+    skip = torch.randn(h, w, ci - cup, generator=g).to(torch.bfloat16)
+    cat = torch.cat([up.repeat_interleave(2, 0).repeat_interleave(2, 1)[:h, :w], skip], dim=2).contiguous()
+    wt = (0.1 * torch.randn(co, 9, ci, generator=g)).to(torch.bfloat16)
+    bias = torch.randn(co, generator=g).float()
+    a = torch.zeros(h, w, co, dtype=torch.bfloat16, device="cuda")
+    b = torch.zeros(h, w, co, dtype=torch.bfloat16, device="cuda")
+    cat_d, up_d, w_d, b_d = cat.cuda(), up.cuda(), wt.cuda(), bias.cuda()
+    # the fused call reads the skip channels from a buffer whose first cup
+    # channels were never written
+    holey = cat.clone()
+    holey[:, :, :cup] = float("nan")
+    holey_d = holey.cuda()
+    assert hpm.lib().hpm_unet_conv_fwd(_p(cat_d), ci, ci, _p(w_d), _p(b_d), None, _p(a), co, co, h, w, 3, 1,
+                                       _stream()) == 0
+    assert hpm.lib().hpm_unet_conv_fwd_up(_p(holey_d), ci, ci, _p(up_d), cup, cup, _p(w_d), _p(b_d), None, _p(b), co,
+                                          co, h, w, 1, _stream()) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(a.cpu().view(torch.int16), b.cpu().view(torch.int16))
 
 
 def test_conv_weight_gradient(hpm):

@@ -33,8 +33,13 @@
 
 #if defined(__HIPCC__)
 #define HPM_DT_FN __host__ __device__ inline
+/* one copy of the larger predicates: divergent lanes of a wave run
+ * different parts of the walk, and inlined copies would overflow the
+ * instruction cache */
+#define HPM_DT_CALL __host__ __device__ __attribute__((noinline))
 #else
 #define HPM_DT_FN inline
+#define HPM_DT_CALL inline
 #endif
 
 namespace hpm_dt {
@@ -72,7 +77,7 @@ HPM_DT_FN int64_t incircle_det(P2 a, P2 b, P2 c, P2 d)
  * rows (x, y, x^2 + y^2 + eps_i, 1) in the order a, b, c, d; the cofactors
  * of the lifted column are +O(b,c,d), -O(a,c,d), +O(a,b,d), -O(a,b,c).
  * Non-zero whenever (a, b, c) is a proper triangle. */
-HPM_DT_FN int incircle(P2 a, int ia, P2 b, int ib, P2 c, int ic, P2 d, int id)
+HPM_DT_CALL int incircle(P2 a, int ia, P2 b, int ib, P2 c, int ic, P2 d, int id)
 {
     const int64_t det = incircle_det(a, b, c, d);
     if (det != 0) {
@@ -102,7 +107,12 @@ HPM_DT_FN int incircle(P2 a, int ia, P2 b, int ib, P2 c, int ic, P2 d, int id)
  *   int cx(P2 p) const, cy(P2 p) const;   the cell of a point
  *   int cxd(double x) const, cyd(double y) const;   the cell column / row of
  *       a coordinate, clamped to [-1, gw] / [-1, gh]
- *   template <class F> void visit(int cx, int cy, F f) const;  f(i) for the points of a cell
+ *   double rowy(int y) const, colx(int x) const;   the first pixel row /
+ *       column of cell row y / column x
+ *   template <class F> void visit_row(int y, int x0, int x1, F f) const;
+ *   template <class F> void visit_col(int x, int y0, int y1, F f) const;
+ *       f(i) for the points of the cells (x0..x1, y) / (x, y0..y1), ranges
+ *       inside the grid; a grid may skip empty cells in one step
  *   int nhull; P2 hull(int k) const; the convex hull's vertices
  * A point lies at most cs - 1 from its cell's corner in each axis. */
 
@@ -129,29 +139,23 @@ HPM_DT_FN bool visit_ring(const G& g, int cx, int cy, int r, F&& f)
         return false;
     }
     if (r == 0) {
-        g.visit(cx, cy, f);
+        g.visit_row(cy, cx, cx, f);
         return true;
     }
-    for (int x = x0; x <= x1; ++x) {
-        if (x < 0 || x >= g.gw) {
-            continue;
-        }
-        if (y0 >= 0) {
-            g.visit(x, y0, f);
-        }
-        if (y1 < g.gh) {
-            g.visit(x, y1, f);
-        }
+    const int xa = x0 < 0 ? 0 : x0, xb = x1 >= g.gw ? g.gw - 1 : x1;
+    if (y0 >= 0) {
+        g.visit_row(y0, xa, xb, f);
     }
-    for (int y = y0 + 1; y <= y1 - 1; ++y) {
-        if (y < 0 || y >= g.gh) {
-            continue;
-        }
+    if (y1 < g.gh) {
+        g.visit_row(y1, xa, xb, f);
+    }
+    const int ya = y0 + 1 < 0 ? 0 : y0 + 1, yb = y1 - 1 >= g.gh ? g.gh - 1 : y1 - 1;
+    if (ya <= yb) {
         if (x0 >= 0) {
-            g.visit(x0, y, f);
+            g.visit_col(x0, ya, yb, f);
         }
         if (x1 < g.gw) {
-            g.visit(x1, y, f);
+            g.visit_col(x1, ya, yb, f);
         }
     }
     return true;
@@ -159,7 +163,7 @@ HPM_DT_FN bool visit_ring(const G& g, int cx, int cy, int r, F&& f)
 
 /* the nearest other point of p (ties to the smaller index), -1 if none */
 template <class G>
-HPM_DT_FN int nearest(const G& g, int p)
+HPM_DT_CALL int nearest(const G& g, int p)
 {
     const P2 pp = g.pt(p);
     const int cx = g.cx(pp), cy = g.cy(pp);
@@ -208,7 +212,7 @@ HPM_DT_FN bool on_hull(const G& g, P2 p)
  * through an interior point has hull vertices on both sides); boundary
  * says whether both lie on the hull's boundary. */
 template <class G>
-HPM_DT_FN int third(const G& g, int a, int b, bool boundary)
+HPM_DT_CALL int third(const G& g, int a, int b, bool boundary)
 {
     const P2 pa = g.pt(a), pb = g.pt(b);
     if (boundary && nothing_left(g, pa, pb)) {
@@ -228,32 +232,145 @@ HPM_DT_FN int third(const G& g, int a, int b, bool boundary)
             best = i;
         }
     };
-    /* a first candidate from the rings around a (at least rings 0 and 1) */
+    const double abx = (double)pb.x - pa.x, aby = (double)pb.y - pa.y;
+    /* the cells of row y that can hold points left of a->b, clipped to
+     * [xs, xe]; false when none */
+    auto left_row = [&](int y, int* xs, int* xe) {
+        const double y0 = g.rowy(y), y1 = y0 + (g.cs - 1);
+        double lo = -1e300, hi = 1e300;
+        if (aby > 0.0) {
+            hi = pa.x + abx * ((abx >= 0.0 ? y1 : y0) - pa.y) / aby + 1e-6;
+        } else if (aby < 0.0) {
+            lo = pa.x + abx * ((abx >= 0.0 ? y1 : y0) - pa.y) / aby - 1e-6;
+        } else if ((abx > 0.0 && !(y1 > pa.y)) || (abx < 0.0 && !(y0 < pa.y))) {
+            return false;
+        }
+        const int a0 = g.cxd(lo), a1 = g.cxd(hi);
+        *xs = a0 > *xs ? a0 : *xs;
+        *xe = a1 < *xe ? a1 : *xe;
+        return *xs <= *xe;
+    };
+    /* the same for the rows of column x (a column band of cs pixels) */
+    auto left_col = [&](int x, int* ys, int* ye) {
+        const double x0 = g.colx(x), x1 = x0 + (g.cs - 1);
+        double lo = -1e300, hi = 1e300;
+        /* (b - a) x (p - a) > 0: abx (y - a.y) > aby (x - a.x) */
+        if (abx > 0.0) {
+            lo = pa.y + aby * ((aby >= 0.0 ? x0 : x1) - pa.x) / abx - 1e-6;
+        } else if (abx < 0.0) {
+            hi = pa.y + aby * ((aby >= 0.0 ? x0 : x1) - pa.x) / abx + 1e-6;
+        } else if ((aby > 0.0 && !(x0 < pa.x)) || (aby < 0.0 && !(x1 > pa.x))) {
+            return false;
+        }
+        const int a0 = g.cyd(lo), a1 = g.cyd(hi);
+        *ys = a0 > *ys ? a0 : *ys;
+        *ye = a1 < *ye ? a1 : *ye;
+        return *ys <= *ye;
+    };
+    /* a first candidate from the rings around a (at least rings 0 and 1),
+     * each side of a ring clipped to the left half-plane */
     for (int r = 0;; ++r) {
-        if (!visit_ring(g, cx, cy, r, consider)) {
+        const int x0 = cx - r, x1 = cx + r, y0 = cy - r, y1 = cy + r;
+        if (x0 < 0 && y0 < 0 && x1 >= g.gw && y1 >= g.gh) {
             return best;
+        }
+        const int xa = x0 < 0 ? 0 : x0, xb = x1 >= g.gw ? g.gw - 1 : x1;
+        for (int side = 0; side < 2; ++side) {
+            const int y = side == 0 ? y0 : y1;
+            if (y < 0 || y >= g.gh || (side == 1 && r == 0)) {
+                continue;
+            }
+            int xs = xa, xe = xb;
+            if (left_row(y, &xs, &xe)) {
+                g.visit_row(y, xs, xe, consider);
+            }
+        }
+        const int ya = y0 + 1 < 0 ? 0 : y0 + 1, yb = y1 - 1 >= g.gh ? g.gh - 1 : y1 - 1;
+        for (int side = 0; side < 2 && ya <= yb; ++side) {
+            const int x = side == 0 ? x0 : x1;
+            if (x < 0 || x >= g.gw) {
+                continue;
+            }
+            int ys = ya, ye = yb;
+            if (left_col(x, &ys, &ye)) {
+                g.visit_col(x, ys, ye, consider);
+            }
         }
         if (best >= 0 && r >= 1) {
             break;
         }
     }
     /* On the left of a->b the circles through a and b are nested, so the
-     * left part of the final circle lies inside the left part of this
-     * one: scanning the cells of its bounding box (the radius widened for
-     * the roundings) completes the search. */
-    const P2 pc = g.pt(best);
-    const double bx = (double)pb.x - pa.x, by = (double)pb.y - pa.y;
-    const double qx = (double)pc.x - pa.x, qy = (double)pc.y - pa.y;
-    const double den = 2.0 * (double)orient(pa, pb, pc);
-    const double b2 = bx * bx + by * by, q2 = qx * qx + qy * qy;
-    const double ux = pa.x + (qy * b2 - by * q2) / den;
-    const double uy = pa.y + (bx * q2 - qx * b2) / den;
-    const double rad = sqrt((ux - pa.x) * (ux - pa.x) + (uy - pa.y) * (uy - pa.y)) * (1.0 + 1e-9) + 1e-6;
-    const int gx0 = g.cxd(ux - rad), gx1 = g.cxd(ux + rad);
-    const int gy0 = g.cyd(uy - rad), gy1 = g.cyd(uy + rad);
-    for (int y = gy0 < 0 ? 0 : gy0; y <= gy1 && y < g.gh; ++y) {
-        for (int x = gx0 < 0 ? 0 : gx0; x <= gx1 && x < g.gw; ++x) {
-            g.visit(x, y, consider);
+     * left part of the final circle lies inside the left part of the
+     * current one, and when the current centre lies right of a->b that
+     * part (a minor segment: every point sees a b at 90 degrees or more)
+     * lies inside the disk on a b as diameter. The scan covers that
+     * region row by row, outwards from a's row, each row over the cells its
+     * chord of the region and the left half-plane reach (bounds widened for
+     * the roundings); the region is recomputed as better candidates shrink
+     * it, so every row scanned, and every row left out, is judged against
+     * a region that contains the final one. */
+    const double lab = sqrt(abx * abx + aby * aby);
+    double ux = 0.0, uy = 0.0, rad = 0.0;
+    int cur = -1, gy0 = 0, gy1 = -1;
+    auto region = [&]() {
+        cur = best;
+        const P2 pc = g.pt(best);
+        const double qx = (double)pc.x - pa.x, qy = (double)pc.y - pa.y;
+        const double den = 2.0 * (double)orient(pa, pb, pc);
+        const double b2 = abx * abx + aby * aby, q2 = qx * qx + qy * qy;
+        const double cx = (qy * b2 - aby * q2) / den, cy = (abx * q2 - qx * b2) / den;
+        /* signed distance of the centre from the line, > 0 on the left */
+        const double side = (abx * cy - aby * cx) / lab;
+        if (side < -1e-6 * (lab + 1.0)) {
+            ux = pa.x + 0.5 * abx;
+            uy = pa.y + 0.5 * aby;
+            rad = 0.5 * lab * (1.0 + 1e-9) + 1e-6;
+        } else {
+            ux = pa.x + cx;
+            uy = pa.y + cy;
+            rad = sqrt(cx * cx + cy * cy) * (1.0 + 1e-9) + 1e-6;
+        }
+        gy0 = g.cyd(uy - rad);
+        gy1 = g.cyd(uy + rad);
+    };
+    auto scan_row = [&](int y) {
+        if (best != cur) {
+            region();
+        }
+        if (y < gy0 || y > gy1 || y < 0 || y >= g.gh) {
+            return;
+        }
+        const double y0 = g.rowy(y), y1 = y0 + (g.cs - 1);
+        const double dyc = uy < y0 ? y0 - uy : (uy > y1 ? uy - y1 : 0.0);
+        if (dyc > rad) {
+            return;
+        }
+        const double hw = sqrt(rad * rad - dyc * dyc);
+        double lo = ux - hw, hi = ux + hw;
+        /* the left half-plane within the row band: (b - a) x (p - a) > 0 */
+        if (aby > 0.0) {
+            hi = fmin(hi, pa.x + abx * ((abx >= 0.0 ? y1 : y0) - pa.y) / aby + 1e-6);
+        } else if (aby < 0.0) {
+            lo = fmax(lo, pa.x + abx * ((abx >= 0.0 ? y1 : y0) - pa.y) / aby - 1e-6);
+        } else if ((abx > 0.0 && !(y1 > pa.y)) || (abx < 0.0 && !(y0 < pa.y))) {
+            return;
+        }
+        const int xa = g.cxd(lo), xb = g.cxd(hi);
+        const int xs = xa < 0 ? 0 : xa, xe = xb >= g.gw ? g.gw - 1 : xb;
+        if (xs <= xe) {
+            g.visit_row(y, xs, xe, consider);
+        }
+    };
+    region();
+    const int ya = cy;
+    for (int d = 0;; ++d) {
+        if (ya - d < gy0 && ya + d > gy1) {
+            break;
+        }
+        scan_row(ya - d);
+        if (d > 0) {
+            scan_row(ya + d);
         }
     }
     return best;

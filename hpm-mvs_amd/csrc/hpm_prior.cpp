@@ -29,6 +29,8 @@ struct HostGrid {
     int cy(hpm_dt::P2 p) const { return (p.y - y0) / cs; }
     int cxd(double x) const { return (int)fmax(-1.0, fmin((double)gw, floor((x - x0) / cs))); }
     int cyd(double y) const { return (int)fmax(-1.0, fmin((double)gh, floor((y - y0) / cs))); }
+    double rowy(int y) const { return (double)y0 + (double)y * cs; }
+    double colx(int x) const { return (double)x0 + (double)x * cs; }
     hpm_dt::P2 hull(int k) const { return H[(size_t)k]; }
     template <class F>
     void visit(int x, int y, F&& f) const
@@ -36,6 +38,20 @@ struct HostGrid {
         const size_t c = (size_t)y * gw + x;
         for (int k = start[c]; k < start[c + 1]; ++k) {
             f(items[(size_t)k]);
+        }
+    }
+    template <class F>
+    void visit_row(int y, int xa, int xb, F&& f) const
+    {
+        for (int x = xa; x <= xb; ++x) {
+            visit(x, y, f);
+        }
+    }
+    template <class F>
+    void visit_col(int x, int ya, int yb, F&& f) const
+    {
+        for (int y = ya; y <= yb; ++y) {
+            visit(x, y, f);
         }
     }
 

@@ -437,68 +437,96 @@ __global__ __launch_bounds__(256) void hpm_k_hull_columns(const int32_t* pts, co
 }
 
 /* Andrew's monotone chain over the column extremes, which come sorted by
- * (x, y): the hull counter-clockwise (orient > 0), as the host's chain over
- * all points gives it. One thread. hull has room for 4 ws + 2 points (the
- * chain's stack over at most 2 ws points). */
-__global__ void hpm_k_hull_chain(const int32_t* pts, const int32_t* col_lo, const int32_t* col_hi, int ws,
-                                 int32_t* hull, int32_t* nhull)
+ * (x, y): the lower chain over the least-y points, the upper over the
+ * greatest-y points (a lower-hull vertex is its column's lowest point, an
+ * upper one its highest). The hull counter-clockwise (orient > 0) from the
+ * least (x, y) point, as the host's chain over all points gives it. One
+ * workgroup stages the extremes in LDS (points packed x | y << 16, x and y
+ * below 2^15) and one thread runs the chain there. hull has room for
+ * 2 ws + 2 points. */
+constexpr int kHullMaxW = 4096;
+__global__ __launch_bounds__(256) void hpm_k_hull_chain(const int32_t* pts, const int32_t* col_lo,
+                                                        const int32_t* col_hi, int ws, int32_t* hull, int32_t* nhull)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) {
+    __shared__ int32_t s_lo[kHullMaxW], s_hi[kHullMaxW], s_st[kHullMaxW + 2];
+    for (int x = (int)threadIdx.x; x < ws; x += 256) {
+        const int lo = col_lo[x], hi = col_hi[x];
+        s_lo[x] = lo < 0 ? -1 : (pts[2 * lo] | (pts[2 * lo + 1] << 16));
+        s_hi[x] = hi < 0 ? -1 : (pts[2 * hi] | (pts[2 * hi + 1] << 16));
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) {
         return;
     }
     using hpm_dt::P2;
-    auto pt = [&](int i) { return P2{pts[2 * i], pts[2 * i + 1]}; };
-    int k = 0, n = 0;
-    /* lower chain */
+    auto un = [](int32_t v) { return P2{v & 0xffff, v >> 16}; };
+    /* distinct points and the extreme ones */
+    int n = 0, first = -1, last = -1;
     for (int x = 0; x < ws; ++x) {
-        for (int e = 0; e < 2; ++e) {
-            const int i = e == 0 ? col_lo[x] : col_hi[x];
-            if (i < 0 || (e == 1 && i == col_lo[x])) {
-                continue;
+        if (s_lo[x] >= 0) {
+            n += s_hi[x] != s_lo[x] ? 2 : 1;
+            if (first < 0) {
+                first = x;
             }
-            ++n;
-            const P2 p = pt(i);
-            while (k >= 2 && hpm_dt::orient(P2{hull[2 * (k - 2)], hull[2 * (k - 2) + 1]},
-                                            P2{hull[2 * (k - 1)], hull[2 * (k - 1) + 1]}, p) <= 0) {
-                --k;
-            }
-            hull[2 * k] = p.x;
-            hull[2 * k + 1] = p.y;
-            ++k;
+            last = x;
         }
     }
-    /* upper chain, from the second-largest point down */
-    const int t = k + 1;
-    bool first = true;
-    for (int x = ws - 1; x >= 0; --x) {
-        for (int e = 1; e >= 0; --e) {
-            const int i = e == 0 ? col_lo[x] : col_hi[x];
-            if (i < 0 || (e == 1 && i == col_lo[x])) {
-                continue;
-            }
-            if (first) { /* the largest point ends the lower chain */
-                first = false;
-                continue;
-            }
-            const P2 p = pt(i);
-            while (k >= t && hpm_dt::orient(P2{hull[2 * (k - 2)], hull[2 * (k - 2) + 1]},
-                                            P2{hull[2 * (k - 1)], hull[2 * (k - 1) + 1]}, p) <= 0) {
-                --k;
-            }
-            hull[2 * k] = p.x;
-            hull[2 * k + 1] = p.y;
-            ++k;
+    if (n < 3) {
+        *nhull = 0;
+        return;
+    }
+    int k = 0;
+    auto push = [&](int32_t v, int floor_k) {
+        while (k >= floor_k + 2 && hpm_dt::orient(un(s_st[k - 2]), un(s_st[k - 1]), un(v)) <= 0) {
+            --k;
+        }
+        s_st[k++] = v;
+    };
+    /* lower chain: least (x, y) .. greatest (x, y) over the lowest points,
+     * then the greatest column's highest point */
+    for (int x = first; x <= last; ++x) {
+        if (s_lo[x] >= 0) {
+            push(s_lo[x], 0);
         }
     }
-    *nhull = n < 3 ? 0 : (k > 1 ? k - 1 : k);
+    if (s_hi[last] != s_lo[last]) {
+        push(s_hi[last], 0);
+    }
+    const int lower = k;
+    for (int e = 0; e < lower; ++e) {
+        hull[2 * e] = s_st[e] & 0xffff;
+        hull[2 * e + 1] = s_st[e] >> 16;
+    }
+    /* upper chain from the greatest point back to the least */
+    k = 0;
+    s_st[k++] = s_hi[last];
+    for (int x = last - 1; x >= first; --x) {
+        if (s_hi[x] >= 0) {
+            push(s_hi[x], 0);
+        }
+    }
+    if (s_lo[first] != s_hi[first]) {
+        push(s_lo[first], 0);
+    }
+    /* the upper chain's ends are the lower chain's ends */
+    int m = lower;
+    for (int e = 1; e < k - 1; ++e) {
+        hull[2 * m] = s_st[e] & 0xffff;
+        hull[2 * m + 1] = s_st[e] >> 16;
+        ++m;
+    }
+    *nhull = m;
 }
 
 /* the star walk's grid over the support blocks: 5-pixel cells, at most one
- * point each (blocks numbered column-major) */
+ * point each (blocks numbered column-major); nx / ny give the next occupied
+ * cell along a row / column, so runs of empty cells cost one step */
 struct BlockGrid {
     const int32_t* P;
     const int32_t* bpoint;
     const int32_t* H;
+    const int32_t* nx; /* [y][x]: least x' >= x with a point in row y, else gw */
+    const int32_t* ny; /* [x][y]: least y' >= y with a point in column x, else gh */
     int gw, gh, cs, nhull;
 
     __device__ hpm_dt::P2 pt(int i) const { return hpm_dt::P2{P[2 * i], P[2 * i + 1]}; }
@@ -506,22 +534,60 @@ struct BlockGrid {
     __device__ int cy(hpm_dt::P2 p) const { return p.y / 5; }
     __device__ int cxd(double x) const { return (int)fmax(-1.0, fmin((double)gw, floor(x / 5.0))); }
     __device__ int cyd(double y) const { return (int)fmax(-1.0, fmin((double)gh, floor(y / 5.0))); }
+    __device__ double rowy(int y) const { return 5.0 * y; }
+    __device__ double colx(int x) const { return 5.0 * x; }
     __device__ hpm_dt::P2 hull(int k) const { return hpm_dt::P2{H[2 * k], H[2 * k + 1]}; }
     template <class F>
-    __device__ void visit(int x, int y, F&& f) const
+    __device__ void visit_row(int y, int xa, int xb, F&& f) const
     {
-        const int i = bpoint[x * gh + y];
-        if (i >= 0) {
-            f(i);
+        for (int x = nx[y * gw + xa]; x <= xb; x = x + 1 < gw ? nx[y * gw + x + 1] : gw) {
+            f(bpoint[x * gh + y]);
+        }
+    }
+    template <class F>
+    __device__ void visit_col(int x, int ya, int yb, F&& f) const
+    {
+        for (int y = ny[x * gh + ya]; y <= yb; y = y + 1 < gh ? ny[x * gh + y + 1] : gh) {
+            f(bpoint[x * gh + y]);
         }
     }
 };
 
-/* pass 0: the number of triangles each point emits (count[p + 1]); pass 1:
- * the triangles, at the offsets the scan of the counts gives */
+/* the next-occupied-cell tables of BlockGrid: one thread per row (first
+ * nby threads) and per column (next nbx threads) */
+__global__ __launch_bounds__(256) void hpm_k_block_spans(const int32_t* bpoint, int nbx, int nby, int32_t* nx,
+                                                         int32_t* ny)
+{
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t < nby) {
+        int next = nbx;
+        for (int x = nbx - 1; x >= 0; --x) {
+            if (bpoint[x * nby + t] >= 0) {
+                next = x;
+            }
+            nx[t * nbx + x] = next;
+        }
+    } else if (t < nby + nbx) {
+        const int x = t - nby;
+        int next = nby;
+        for (int y = nby - 1; y >= 0; --y) {
+            if (bpoint[x * nby + y] >= 0) {
+                next = y;
+            }
+            ny[x * nby + y] = next;
+        }
+    }
+}
+
+/* pass 0: the number of triangles each point emits (count[p + 1]), the
+ * first kKeep of them kept in keep[p]; pass 1: the triangles, at the
+ * offsets the scan of the counts gives, copied from keep, or walked again
+ * for the rare points that emit more */
+constexpr int kKeep = 8;
 __global__ __launch_bounds__(256) void hpm_k_delaunay(const int32_t* pts, const int32_t* npts_p, const int32_t* bpoint,
-                                                      int nbx, int nby, const int32_t* hull, const int32_t* nhull,
-                                                      int pass, int32_t* count, int32_t* tris, int max_tris,
+                                                      const int32_t* nx, const int32_t* ny, int nbx, int nby,
+                                                      const int32_t* hull, const int32_t* nhull, int pass,
+                                                      int32_t* count, int32_t* keep, int32_t* tris, int max_tris,
                                                       int32_t* err)
 {
     const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -538,6 +604,8 @@ __global__ __launch_bounds__(256) void hpm_k_delaunay(const int32_t* pts, const 
     BlockGrid g;
     g.P = pts;
     g.bpoint = bpoint;
+    g.nx = nx;
+    g.ny = ny;
     g.H = hull;
     g.gw = nbx;
     g.gh = nby;
@@ -549,16 +617,24 @@ __global__ __launch_bounds__(256) void hpm_k_delaunay(const int32_t* pts, const 
         }
         return;
     }
+    int32_t* kp = keep + (size_t)p * 2 * kKeep;
     if (pass == 0) {
         int c = 0;
-        const int r = hpm_dt::star(g, p, 4 * npts + 16, [&](int, int) { ++c; });
+        const int r = hpm_dt::star(g, p, 4 * npts + 16, [&](int a, int b) {
+            if (c < kKeep) {
+                kp[2 * c] = a;
+                kp[2 * c + 1] = b;
+            }
+            ++c;
+        });
         count[p + 1] = c;
         if (r < 0) {
             atomicOr(err, 1);
         }
     } else {
         int at = count[p];
-        hpm_dt::star(g, p, 4 * npts + 16, [&](int a, int b) {
+        const int c = count[p + 1] - at;
+        auto put = [&](int a, int b) {
             if (at < max_tris) {
                 tris[3 * at] = p;
                 tris[3 * at + 1] = a;
@@ -567,7 +643,14 @@ __global__ __launch_bounds__(256) void hpm_k_delaunay(const int32_t* pts, const 
                 atomicOr(err, 2);
             }
             ++at;
-        });
+        };
+        if (c <= kKeep) {
+            for (int k = 0; k < c; ++k) {
+                put(kp[2 * k], kp[2 * k + 1]);
+            }
+        } else {
+            hpm_dt::star(g, p, 4 * npts + 16, put);
+        }
     }
 }
 
@@ -685,8 +768,9 @@ extern "C" int hpm_launch_prior_assemble(const uint32_t* raster, const float* tr
 }
 
 /* The support points of the block candidates, their Delaunay triangulation
- * and the KNN cells, all on the device. ws_i32 has room for the scans and
- * the hull (3 nb + 10 ws + 16 int32). nums receives (npts, ntris, error). */
+ * and the KNN cells, all on the device. ws_i32 has room for the scans, the
+ * span tables, the kept triangles and the hull (21 nb + 10 ws + 16 int32).
+ * nums receives (npts, ntris, error). Widths up to 4096. */
 extern "C" int hpm_launch_support_delaunay(const int32_t* cand, int ws, int hs, int cell, int gw, int gh,
                                            int32_t* pts, int32_t* tris, int32_t* tri_idx, int32_t* cell_start,
                                            int32_t* cell_pts, int32_t* ws_i32, int32_t* nums, void* stream)
@@ -700,6 +784,9 @@ extern "C" int hpm_launch_support_delaunay(const int32_t* cand, int ws, int hs, 
     int32_t* hull = col_hi + ws;          /* 2 (4 ws + 2): the chain's stack over <= 2 ws points */
     int32_t* nhull = hull + 2 * (4 * ws + 2);
     int32_t* tcount = nhull + 1;          /* nb + 1 (points <= blocks) */
+    int32_t* nxt = tcount + nb + 1;       /* nb */
+    int32_t* nyt = nxt + nb;              /* nb */
+    int32_t* keep = nyt + nb;             /* 2 kKeep nb */
     int32_t* err = nums + 2;
     hipLaunchKernelGGL(hpm_k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)err, (int64_t)1, 0u);
     hipLaunchKernelGGL(hpm_k_scan, dim3(1), dim3(1024), 0, s, cand, nb, 1, pos);
@@ -707,18 +794,23 @@ extern "C" int hpm_launch_support_delaunay(const int32_t* cand, int ws, int hs, 
                        bpoint);
     hipLaunchKernelGGL(hpm_k_hull_columns, dim3((unsigned)((ws + 255) / 256)), dim3(256), 0, s, pts, bpoint, nby, ws,
                        col_lo, col_hi);
-    hipLaunchKernelGGL(hpm_k_hull_chain, dim3(1), dim3(64), 0, s, pts, col_lo, col_hi, ws, hull, nhull);
+    if (ws > kHullMaxW) {
+        return (int)hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(hpm_k_hull_chain, dim3(1), dim3(256), 0, s, pts, col_lo, col_hi, ws, hull, nhull);
+    hipLaunchKernelGGL(hpm_k_block_spans, dim3((unsigned)((nbx + nby + 255) / 256)), dim3(256), 0, s, bpoint, nbx,
+                       nby, nxt, nyt);
     const int32_t* npts_p = pos + nb;
     /* at most nb points: the grids cover every block, threads past npts return */
     const unsigned gp = (unsigned)((nb + 255) / 256);
     /* a triangulation of n points has at most 2 n - 5 triangles */
     const int max_tris = 2 * nb;
-    hipLaunchKernelGGL(hpm_k_delaunay, dim3(gp), dim3(256), 0, s, pts, npts_p, bpoint, nbx, nby, hull, nhull, 0,
-                       tcount, tris, max_tris, err);
+    hipLaunchKernelGGL(hpm_k_delaunay, dim3(gp), dim3(256), 0, s, pts, npts_p, bpoint, nxt, nyt, nbx, nby, hull,
+                       nhull, 0, tcount, keep, tris, max_tris, err);
     hipLaunchKernelGGL(hpm_k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t*)tcount, (int64_t)1, 0u);
     hipLaunchKernelGGL(hpm_k_scan, dim3(1), dim3(1024), 0, s, tcount + 1, nb, 0, tcount);
-    hipLaunchKernelGGL(hpm_k_delaunay, dim3(gp), dim3(256), 0, s, pts, npts_p, bpoint, nbx, nby, hull, nhull, 1,
-                       tcount, tris, max_tris, err);
+    hipLaunchKernelGGL(hpm_k_delaunay, dim3(gp), dim3(256), 0, s, pts, npts_p, bpoint, nxt, nyt, nbx, nby, hull,
+                       nhull, 1, tcount, keep, tris, max_tris, err);
     hipLaunchKernelGGL(hpm_k_iota, dim3((unsigned)((max_tris + 255) / 256)), dim3(256), 0, s, tri_idx, tcount + nb);
     hipLaunchKernelGGL(hpm_k_knn_cells, dim3((unsigned)((gw * gh + 255) / 256)), dim3(256), 0, s, pts, bpoint, nbx,
                        nby, cell, gw, gh, 0, cell_start, cell_pts);

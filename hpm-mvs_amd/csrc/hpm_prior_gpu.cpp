@@ -130,7 +130,7 @@ extern "C" int hpm_build_planar_prior(hpm_handle* h, float factor, int knn_k, do
         *kflag = c.take<uint8_t>(ns);
         *kplanes = c.take<float>(4 * ns);
         *src_nd = c.take<float>(factor == 1.0f ? 0 : 4 * ns);
-        *scratch = c.take<int32_t>(3 * (size_t)nb + 10 * (size_t)ws + 16);
+        *scratch = c.take<int32_t>(21 * (size_t)nb + 10 * (size_t)ws + 16);
         *nums = c.take<int32_t>(4);
     };
     float *depth_s, *cost_s, *tplanes, *kplanes, *src_nd;

@@ -2,26 +2,30 @@
 
 Workload (default, "full"): the 1600x1066 scene with 11 views (20 plane
 patches, 50 boxes, 30% low-texture area); every view is a reference with the
-other 10 as sources. One step is the whole per-scene schedule, timed for two
-methods:
-- plain method with the planar prior: per view a plain RunPatchMatch, a 0.5x
-  planar prior upsampled with JBU and a 1x planar prior (KNN K=8), each
-  followed by RunPatchMatch, then 3 geometric-consistency passes over all
-  views;
-- learned prior: per view a plain RunPatchMatch, the U-Net proposal
-  (hpm-mvs_amd/unet_prior.bin) judged by the NCC cost, one RunPatchMatch with
-  the kept proposals as prior (planar prior where too few are kept), then
-  0, 1 or 3 geometric-consistency passes (--learned-geom-iterations), each
-  timed.
+other 10 as sources. One step is the whole per-scene schedule, timed for:
+- the plain method with the planar prior: per view a plain RunPatchMatch, a
+  0.5x planar prior upsampled with JBU and a 1x planar prior (KNN K=8), both
+  built on the device and each followed by RunPatchMatch, then 0, 1 or 3
+  geometric-consistency passes over all views (--plain-geom-iterations);
+  the plain method with 3 passes sets the accuracy bar;
+- the learned prior in the batched form: every view's plain RunPatchMatch
+  as one batch, per view the U-Net proposal (--learned-prior, by default
+  hpm-mvs_amd/unet_prior_plain.bin, trained on the plain method's maps)
+  judged by its confidence and the NCC cost, every view's prior pass as one
+  batch (planar prior where too few proposals are kept), then 0, 1 or 3
+  geometric-consistency passes (--learned-geom-iterations);
+- the same learned schedule on per-view workers, and in the batched form
+  with the network trained on ground-truth maps (--learned-prior-gt).
 value = ms per reference view (step time / 11) of the fastest learned-prior
-schedule whose depth accuracy (fraction of pixels within 2% of ground truth)
-is at least the plain method's, else of the plain method; all are in config.
-Images are resident in HBM before the timed region. Other workloads:
-"plain" (one plain RunPatchMatch per view), "batch" (the batched form over 32
-mixed-size references), "fusion" (RunFusion of the 11 views), "large" (21
-views of 3840x2160, 20 sources), "lprior" (the learned prior alone on one
-view, with the MFMA roofline of its network), "acmm" (the ACMM+NESP
-coarse-to-fine schedule). Prints one JSON line.
+schedule in the batched form whose depth accuracy (fraction of pixels within
+2% of ground truth) is at least the bar, else of the fastest plain schedule
+that meets it; every schedule is in config.methods. Images are resident in
+HBM before the timed region. Other workloads: "plain" (one plain
+RunPatchMatch per view), "batch" (the batched form over 32 mixed-size
+references), "fusion" (RunFusion of the 11 views), "large" (21 views of
+3840x2160, 20 sources), "lprior" (the learned prior alone on one view, with
+the MFMA roofline of its network), "acmm" (the ACMM+NESP coarse-to-fine
+schedule). Prints one JSON line.
 """
 import argparse
 import importlib.util

@@ -318,8 +318,9 @@ extern "C" hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera
     p->have_state.assign((size_t)num_views, 0);
     bool fits = p->pad_total < ((int64_t)1 << 29); /* 31-bit byte offsets into the padded pool */
     for (const hpm_camera& c : p->cams) {
-        /* texel offsets inside one padded image are exact FP32 integers */
-        fits = fits && (int64_t)(c.width + 3) * (c.height + 2) < ((int64_t)1 << 24);
+        /* the tap offset's 24-bit signed multiply takes the row pitch in
+         * bytes and the row index */
+        fits = fits && (int64_t)(c.width + 3) * 4 < ((int64_t)1 << 23) && c.height + 2 < (1 << 23);
     }
     if (!fits) {
         delete p;

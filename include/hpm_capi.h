@@ -57,9 +57,9 @@ typedef struct hpm_run_stats {
  * Returns NULL on bad arguments, device errors, or when the padded source
  * copies, (width + 3) * (height + 2) entries per view, exceed 2^29 entries
  * together (the kernels address them with 31-bit byte offsets; split larger
- * scenes into several pools) or 2^24 entries in one view (about 16
- * megapixels). Source texels are sampled as their nearest FP16 values
- * (exact for 8-bit grey images). */
+ * scenes into several pools) or a view's width or height exceeds 2^21 - 3
+ * or 2^23 - 3 pixels. Source texels are sampled as their nearest FP16
+ * values (exact for 8-bit grey images). */
 hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera* cams, const float* const* images,
                           int max_image_size);
 void hpm_pool_destroy(hpm_pool* pool);
@@ -98,8 +98,9 @@ int hpm_set_hierarchy_input(hpm_handle* h, const float* coarse_world4, int coars
  * as grey, cams/%08d_cam.txt, downscales images above max_image_size, sets
  * the relaxed depth range; with geometric consistency also reads
  * HPM/2333_%08d/depths.dmb (or depths_geom.dmb) for every view. Images are
- * images/%08d.jpg (sequential-Huffman JPEG, decoded to the luminance that
- * cv::imread(IMREAD_GRAYSCALE) returns), else images/%08d.pgm (8-bit P5). */
+ * images/%08d.jpg (sequential or progressive Huffman JPEG, decoded to the
+ * luminance that cv::imread(IMREAD_GRAYSCALE) returns), else
+ * images/%08d.pgm (8-bit P5). */
 int hpm_input_init(hpm_handle* h, const char* dense_folder, int ref_image_id, const int* src_image_ids,
                    int num_src);
 /* The same on a pool: the problem's reference and source views are pool
@@ -153,15 +154,16 @@ int hpm_depth_range(const hpm_handle* h, float* depth_min, float* depth_max);
 int hpm_upload_prior(hpm_handle* h, const float* planes4, const uint32_t* mask);
 /* One scale of the planar prior built on the device from the handle's last
  * RunPatchMatch, as main.cpp builds it on the host (main.cpp:84-162 at
- * factor 0.25, 238-314 at 0.5, 387-454 at 1; HPM.cpp:759-1123): support
- * points of the (resized) cost map, their Delaunay triangulation (on the
- * host), the rasterised triangle mask, the triangle planes, the KNN
- * supplement with knn_k neighbours, then at factor 1 the range-gated planes
- * and masks, below it their world normals and depths upsampled by JBU
- * (JointBilateralUpsampling, HPM.cpp:1215-1281) into the prior. The prior
- * is left on the handle with the planar prior enabled, equal bit for bit to
- * the host construction. ms_host (may be NULL) accumulates the host part's
- * milliseconds. */
+ * factor 0.25, 238-314 at 0.5, 387-454 at 1; HPM.cpp:759-1123), every
+ * stage on the device: support points of the (resized) cost map, their
+ * Delaunay triangulation (the one of hpm_delaunay), the rasterised triangle
+ * mask, the triangle planes, the KNN supplement with knn_k neighbours, then
+ * at factor 1 the range-gated planes and masks, below it their world
+ * normals and depths upsampled by JBU (JointBilateralUpsampling,
+ * HPM.cpp:1215-1281) into the prior. The prior is left on the handle with
+ * the planar prior enabled, equal bit for bit to the host construction.
+ * ms_host (may be NULL) accumulates the milliseconds the host waits for
+ * the point and triangle counts, which size the later launches. */
 int hpm_build_planar_prior(hpm_handle* h, float factor, int knn_k, double* ms_host);
 /* HPM::ReleasePriorCudaMemory (HPM.cpp:1283-1286); idempotent. */
 int hpm_release_prior(hpm_handle* h);

@@ -461,6 +461,11 @@ extern "C" int hpm_run_batch(hpm_batch* b, hpm_handle* const* hs, int n, int wan
         }
     }
     BHIP(b, hipStreamSynchronize(b->stream));
+    if (want_readback) {
+        for (int i = 0; i < n; ++i) {
+            hs[i]->mirror_filled = true;
+        }
+    }
     float ms = 0.0f;
     BHIP(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     uint64_t total = 0;

@@ -17,6 +17,7 @@
 
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,15 @@ struct hpm_pool {
     float* d_state_cost = nullptr;
     std::vector<char> have_state;
     std::string err;
+    /* problem handles of finished pass schedules over this pool, with their
+     * streams and device buffers, reused by the next schedule (creating a
+     * stream and allocating per problem cost more than a 1600x1066 launch);
+     * and the learned prior the last schedule loaded, keyed by its file.
+     * Schedules over one pool do not run concurrently: they write its maps. */
+    std::mutex spare_mu;
+    std::vector<hpm_handle*> spare;
+    std::string lprior_key;
+    struct hpm_lprior* lprior = nullptr;
 };
 
 struct hpm_handle {
@@ -82,9 +92,11 @@ struct hpm_handle {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool have_prior = false;
 
-    /* host mirrors */
+    /* host mirrors, sized and filled by the first readback after a device
+     * initialisation; until then the getters read zeros */
     std::vector<float> h_hyp;
     std::vector<float> h_cost;
+    bool mirror_filled = false;
     hpm_run_stats stats;
 };
 

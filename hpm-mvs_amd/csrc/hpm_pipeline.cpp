@@ -432,6 +432,66 @@ int process_phase_b(Ctx& c, hpm_handle* h, int ref, const int* srcs, int nsrc, i
     return check(c, h, hpm_store_to_pool(h, 1), "store");
 }
 
+/* a problem handle for a schedule over pool: a spare one left by an earlier
+ * schedule (its stream and buffers kept), else a new one */
+hpm_handle* take_handle(hpm_pool* pool)
+{
+    {
+        std::lock_guard<std::mutex> lk(pool->spare_mu);
+        if (!pool->spare.empty()) {
+            hpm_handle* h = pool->spare.back();
+            pool->spare.pop_back();
+            return h;
+        }
+    }
+    return hpm_create(pool->device);
+}
+
+/* gives a finished schedule's handle back to the pool; beyond 64 spare
+ * handles it is destroyed */
+void give_handle(hpm_pool* pool, hpm_handle* h)
+{
+    if (!h) {
+        return;
+    }
+    (void)hipStreamSynchronize(h->stream);
+    h->err.clear();
+    {
+        std::lock_guard<std::mutex> lk(pool->spare_mu);
+        if (pool->spare.size() < 64) {
+            pool->spare.push_back(h);
+            return;
+        }
+    }
+    hpm_destroy(h);
+}
+
+/* the learned prior of a weight file: the pool's loaded copy while the file
+ * (path, size, modification time) is unchanged, else loaded anew. The pool
+ * keeps it; NULL if it cannot be loaded. */
+hpm_lprior* pool_lprior(hpm_pool* pool, const char* path)
+{
+    struct stat sb;
+    if (stat(path, &sb) != 0) {
+        return nullptr;
+    }
+    char key[64];
+    snprintf(key, sizeof(key), "|%lld|%lld.%09ld", (long long)sb.st_size, (long long)sb.st_mtim.tv_sec,
+             (long)sb.st_mtim.tv_nsec);
+    const std::string k = std::string(path) + key;
+    if (pool->lprior && pool->lprior_key == k) {
+        return pool->lprior;
+    }
+    hpm_lprior* lp = hpm_lprior_load(path, pool->device);
+    if (!lp) {
+        return nullptr;
+    }
+    hpm_lprior_destroy(pool->lprior);
+    pool->lprior = lp;
+    pool->lprior_key = k;
+    return lp;
+}
+
 /* RunPatchMatch of the listed problems in one hpm_run_batch call */
 int run_batch(Ctx& c, const std::vector<hpm_handle*>& hs, bool readback)
 {
@@ -527,10 +587,10 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
     hpm_handle* h = nullptr;
     if (!pool || num_problems < 1 || !ref_ids || !num_src || !src_ids || max_src < 1) {
         rc = fail(c, "bad pipeline arguments");
-    } else if (!(h = hpm_create(pool->device))) {
+    } else if (!(h = take_handle(pool))) {
         rc = fail(c, "cannot create a problem handle");
     } else if (c.opt->learned_prior && c.opt->planar_prior &&
-               !(c.lprior = hpm_lprior_load(c.opt->learned_prior, pool->device))) {
+               !(c.lprior = pool_lprior(pool, c.opt->learned_prior))) {
         rc = fail(c, "cannot load the learned prior %s", c.opt->learned_prior);
     }
     /* host workers, each with its own handle and stream: views that do not
@@ -539,8 +599,9 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
     const int workers = std::max(1, std::min(c.opt->workers, num_problems));
     std::vector<hpm_handle*> handles((size_t)workers, nullptr);
     handles[0] = h;
-    for (int w = 1; rc == HPM_OK && w < workers; ++w) {
-        handles[(size_t)w] = hpm_create(pool->device);
+    /* the batched form runs its host threads over the problems' own handles */
+    for (int w = 1; rc == HPM_OK && !c.opt->batch && w < workers; ++w) {
+        handles[(size_t)w] = take_handle(pool);
         if (!handles[(size_t)w]) {
             rc = fail(c, "cannot create a problem handle");
         }
@@ -582,7 +643,7 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
     std::vector<hpm_handle*> ph;
     if (rc == HPM_OK && c.opt->batch) {
         for (int i = 0; i < num_problems && rc == HPM_OK; ++i) {
-            ph.push_back(hpm_create(pool->device));
+            ph.push_back(take_handle(pool));
             if (!ph.back()) {
                 rc = fail(c, "cannot create a problem handle");
             }
@@ -620,9 +681,8 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
     if (rc == HPM_OK && c.opt->batch) {
         auto t0 = clk::now();
         const bool host_prior = c.opt->planar_prior != 0 && !c.opt->prior_on_device;
-        for (int i = 0; i < num_problems && rc >= 0; ++i) {
-            rc = phase_a_setup(c, ph[(size_t)i], ref_ids[i], srcs_of(i), num_src[i]);
-        }
+        /* the problems' set-up (allocations, view tables) on the host threads */
+        rc = each([&](hpm_handle* hh, int i) { return phase_a_setup(c, hh, ref_ids[i], srcs_of(i), num_src[i]); });
         if (rc >= 0) rc = run_batch(c, ph, host_prior);
         std::vector<char> use((size_t)num_problems, 0);
         if (rc >= 0 && c.opt->planar_prior && c.lprior) {
@@ -652,9 +712,9 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
             for (int g = 0; g < c.opt->params.geom_iterations && rc >= 0; ++g) {
                 if (g == 0) {
                     /* the first pass reads only phase-A maps: one batch */
-                    for (int i = 0; i < num_problems && rc >= 0; ++i) {
-                        rc = phase_b_setup(c, ph[(size_t)i], ref_ids[i], srcs_of(i), num_src[i], g);
-                    }
+                    rc = each([&](hpm_handle* hh, int i) {
+                        return phase_b_setup(c, hh, ref_ids[i], srcs_of(i), num_src[i], g);
+                    });
                     if (rc >= 0) rc = run_batch(c, ph, false);
                     for (int i = 0; i < num_problems && rc >= 0; ++i) {
                         rc = check(c, ph[(size_t)i], hpm_store_to_pool(ph[(size_t)i], 1), "store");
@@ -704,21 +764,12 @@ extern "C" int hpm_pipeline_run(hpm_pool* pool, int num_problems, const int* ref
         st->ms_phase_b = ms_since(t0);
     }
     for (int w = 1; w < workers; ++w) {
-        if (handles[(size_t)w]) {
-            hpm_destroy(handles[(size_t)w]);
-        }
+        give_handle(pool, handles[(size_t)w]);
     }
     for (hpm_handle* x : ph) {
-        if (x) {
-            hpm_destroy(x);
-        }
+        give_handle(pool, x);
     }
-    if (h) {
-        hpm_destroy(h);
-    }
-    if (c.lprior) {
-        hpm_lprior_destroy(c.lprior);
-    }
+    give_handle(pool, h);
     if (st->learned_views + st->learned_fallbacks > 0) {
         st->learned_accept /= (double)(st->learned_views + st->learned_fallbacks);
     }

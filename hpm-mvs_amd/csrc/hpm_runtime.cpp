@@ -31,6 +31,7 @@
 #include "hpm_fmath.h"
 #include "hpm_internal.h"
 #include "hpm_io.h"
+#include "hpm_unet.h"
 
 int hpm_fail(hpm_handle* h, int code, const char* fmt, ...)
 {
@@ -221,6 +222,7 @@ int readback(hpm_handle* h)
     HPM_HIP(h, hipMemcpyAsync(h->h_hyp.data(), h->d_hyp, np * 16, hipMemcpyDeviceToHost, h->stream));
     HPM_HIP(h, hipMemcpyAsync(h->h_cost.data(), h->d_cost, np * 4, hipMemcpyDeviceToHost, h->stream));
     HPM_HIP(h, hipStreamSynchronize(h->stream));
+    h->mirror_filled = true;
     return HPM_OK;
 }
 
@@ -374,6 +376,12 @@ extern "C" void hpm_pool_destroy(hpm_pool* p)
         return;
     }
     (void)hipSetDevice(p->device);
+    for (hpm_handle* h : p->spare) {
+        hpm_destroy(h);
+    }
+    p->spare.clear();
+    hpm_lprior_destroy(p->lprior);
+    p->lprior = nullptr;
     if (p->stream) {
         (void)hipStreamSynchronize(p->stream);
     }
@@ -733,8 +741,9 @@ extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
     HPM_HIP(h, hipMemsetAsync(h->d_hyp, 0, np * 16, h->stream));
     HPM_HIP(h, hipMemsetAsync(h->d_cost, 0, np * 4, h->stream));
     HPM_HIP(h, hipMemsetAsync(h->d_sel, 0, np * 4, h->stream));
-    h->h_hyp.assign(np * 4, 0.0f);
-    h->h_cost.assign(np, 0.0f);
+    /* the host mirrors are filled by the first readback (a pass schedule
+     * that keeps its maps on the device never pays for them) */
+    h->mirror_filled = false;
     h->have_device = true;
     h->have_prior = false;
     h->have_coarse = false;
@@ -793,8 +802,11 @@ extern "C" int hpm_device_init_state(hpm_handle* h, const float* hyp_world, cons
         return hpm_fail(h, HPM_EINVAL, "device initialisation must come first");
     }
     const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
+    h->h_hyp.resize(np * 4);
+    h->h_cost.resize(np);
     memcpy(h->h_hyp.data(), hyp_world, np * 16);
     memcpy(h->h_cost.data(), cost, np * 4);
+    h->mirror_filled = true;
     HPM_HIP(h, hipMemcpyAsync(h->d_hyp, h->h_hyp.data(), np * 16, hipMemcpyHostToDevice, h->stream));
     HPM_HIP(h, hipMemcpyAsync(h->d_cost, h->h_cost.data(), np * 4, hipMemcpyHostToDevice, h->stream));
     HPM_HIP(h, hipStreamSynchronize(h->stream));
@@ -978,10 +990,20 @@ extern "C" int hpm_copy_reference_image(const hpm_handle* h, float* out)
     return HPM_OK;
 }
 
+/* pixels the getters cover: the reference view's, once the device state exists */
+static size_t mirror_pixels(const hpm_handle* h)
+{
+    return h->have_device ? (size_t)h->cams[0].width * h->cams[0].height : 0;
+}
+
 extern "C" int hpm_get_plane(const hpm_handle* h, int index, float out[4])
 {
-    if (!h || !out || index < 0 || (size_t)index * 4 >= h->h_hyp.size()) {
+    if (!h || !out || index < 0 || (size_t)index >= mirror_pixels(h)) {
         return HPM_EINVAL;
+    }
+    if (!h->mirror_filled) {
+        memset(out, 0, 16);
+        return HPM_OK;
     }
     memcpy(out, &h->h_hyp[(size_t)index * 4], 16);
     return HPM_OK;
@@ -989,27 +1011,37 @@ extern "C" int hpm_get_plane(const hpm_handle* h, int index, float out[4])
 
 extern "C" float hpm_get_cost(const hpm_handle* h, int index)
 {
-    if (!h || index < 0 || (size_t)index >= h->h_cost.size()) {
+    if (!h || index < 0 || (size_t)index >= mirror_pixels(h)) {
         return hpm_qnan();
     }
-    return h->h_cost[(size_t)index];
+    return h->mirror_filled ? h->h_cost[(size_t)index] : 0.0f;
 }
 
 extern "C" int hpm_copy_hypotheses(const hpm_handle* h, float* out4)
 {
-    if (!h || !out4 || h->h_hyp.empty()) {
+    const size_t np = h ? mirror_pixels(h) : 0;
+    if (!h || !out4 || np == 0) {
         return HPM_EINVAL;
     }
-    memcpy(out4, h->h_hyp.data(), h->h_hyp.size() * sizeof(float));
+    if (!h->mirror_filled) {
+        memset(out4, 0, np * 16);
+        return HPM_OK;
+    }
+    memcpy(out4, h->h_hyp.data(), np * 16);
     return HPM_OK;
 }
 
 extern "C" int hpm_copy_costs(const hpm_handle* h, float* out)
 {
-    if (!h || !out || h->h_cost.empty()) {
+    const size_t np = h ? mirror_pixels(h) : 0;
+    if (!h || !out || np == 0) {
         return HPM_EINVAL;
     }
-    memcpy(out, h->h_cost.data(), h->h_cost.size() * sizeof(float));
+    if (!h->mirror_filled) {
+        memset(out, 0, np * 4);
+        return HPM_OK;
+    }
+    memcpy(out, h->h_cost.data(), np * 4);
     return HPM_OK;
 }
 

@@ -6,12 +6,13 @@
  *   image pool   float, all images back to back, row-major, one grey value
  *                per pixel (the reference converts 8-bit JPEG to float,
  *                HPM.cpp:492-494);
- *   padded pool  the same images with a border of 1 texel left and top, 2
- *                right and bottom repeating the edge texels (pitch W + 3):
- *                the bilinear footprint of every clamped source coordinate
- *                in [-1, W] x [-1, H] is in bounds, so a source tap is two
- *                8-byte row-pair gathers with 32-bit offsets and no integer
- *                clamps;
+ *   padded pool  per image and texel (x, y), x in [-1, W], y in [-1, H]
+ *                (pitch W + 2), the FP16 bilinear coefficients of the 2x2
+ *                footprint at (x, y), 8 bytes (hpm_half.h), the border
+ *                repeating the edge texels: every clamped source coordinate
+ *                in [-1, W] x [-1, H] has its footprint in bounds, so a source
+ *                tap is one 8-byte gather with a 32-bit offset, no integer
+ *                clamps, and three fused multiply-adds;
  *   depth pool   float, previous-pass depth map of every image (geometric
  *                consistency only, HPM.cpp:561-588);
  *   hyp          float4 per reference pixel: camera-frame plane (n, d)
@@ -40,7 +41,7 @@ typedef struct hpm_dev_view {
     double A[9];
     double b[3];
     hpm_camera cam;
-    int32_t pitch;      /* row pitch of the padded image in entries (width + 3) */
+    int32_t pitch;      /* row pitch of the padded image in entries (width + 2) */
     int64_t img_off;    /* offset of this image in the image pool (elements) */
     int64_t depth_off;  /* offset of this view's depth map in the depth pool */
     int64_t pad_off;    /* offset of entry (-1, -1) of this image in the padded pool */
@@ -58,9 +59,7 @@ typedef struct hpm_dev_problem {
     float thr[3];              /* view-selection thresholds per iteration */
     float fallback[3];         /* expf(thr^2 / -0.32) */
     const float* images;
-    const uint32_t* images_pad; /* padded pool: entry (x, y) = FP16 texels (x, y), (x, y + 1) */
-    float one;                  /* 1.0f, opaque to the compiler: keeps p1 - p0 of two FP16 texels one
-                                   v_fma_mix_f32 (p1 * one - p0) */
+    const uint32_t* images_pad; /* padded pool: entry (x, y) = FP16 coefficients c0 | c1 << 16, c2 | c3 << 16 */
     const float* depths;
     const hpm_dev_view* views;
     float* hyp;                /* float4 per pixel */

@@ -35,8 +35,9 @@ struct hpm_pool {
     float* d_images = nullptr;
     std::vector<int64_t> pad_off;             /* offset of texel (-1, -1) of each view, padded pool */
     int64_t pad_total = 0;
-    uint32_t* d_images_pad = nullptr;         /* entry (x, y): FP16 texels (x, y), (x, y + 1) (hpm_half.h);
-                                                 x in [-1, W + 1], y in [-1, H], edges replicated */
+    uint32_t* d_images_pad = nullptr;         /* entry (x, y), 8 bytes: the FP16 bilinear coefficients of
+                                                 texels (x..x+1, y..y+1) (hpm_half.h); x in [-1, W],
+                                                 y in [-1, H], edges replicated */
     float* d_depth_plain = nullptr;           /* depths.dmb of each view */
     float* d_depth_geom = nullptr;            /* depths_geom.dmb of each view */
     float* d_state_hyp = nullptr;             /* float4: world normal, depth of the last pass */

@@ -106,7 +106,7 @@ __device__ __forceinline__ void plane_q(const hpm_dev_problem& pb, float4 n, dou
  * into an integer texel and a float fraction, plus the float coefficients of
  * the tap offsets s(p + d) - s0 = (g d) / (1 + e . d). */
 struct Warp {
-    uint32_t base4;       /* byte offset of texel (ix0, iy0) in the padded pool (< 2^30 texels) */
+    uint32_t base8;       /* byte offset of entry (ix0, iy0) in the padded pool (< 2^29 entries) */
     float fx0, fy0;
     float ex, ey, gxx, gxy, gyx, gyy;
     float lox, hix, loy, hiy; /* clamp of the offsets: source coordinate in [-1, W] x [-1, H] */
@@ -140,7 +140,7 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
     }
     const double fx = floor(sx0), fy = floor(sy0);
     const int ix0 = (int)fx, iy0 = (int)fy;
-    w->base4 = ((uint32_t)sv->pad_off + (uint32_t)((iy0 + 1) * sv->pitch) + (uint32_t)(ix0 + 1)) * 4u;
+    w->base8 = ((uint32_t)sv->pad_off + (uint32_t)((iy0 + 1) * sv->pitch) + (uint32_t)(ix0 + 1)) * 8u;
     w->fx0 = (float)(sx0 - fx);
     w->fy0 = (float)(sy0 - fy);
     w->ex = (float)(H[6] * icz);
@@ -214,7 +214,7 @@ __device__ __forceinline__ float fract1(float x)
  * top-left texel of its bilinear footprint in the padded pool and the
  * bilinear fractions. */
 template <int kPath>
-__device__ __forceinline__ void tap_coord(const Warp& w, const Column& c, int pitch4, float dy, uint32_t* off,
+__device__ __forceinline__ void tap_coord(const Warp& w, const Column& c, int pitch8, float dy, uint32_t* off,
                                           float* fa, float* fb)
 {
     const float u = fmaf(w.ey, dy, c.ex);
@@ -235,22 +235,23 @@ __device__ __forceinline__ void tap_coord(const Warp& w, const Column& c, int pi
     *fa = fract1(lx);
     *fb = fract1(ly);
     /* (ix0 + floor(lx), iy0 + floor(ly)) lies in [-1, W] x [-1, H]: its
-     * footprint is inside the padded image, whose border repeats the edge
-     * texels; |floor(ly)| <= 8 and pitch4 < 2^23 fit the 24-bit product */
-    *off = ((uint32_t)floor_int(lx) << 2) + (uint32_t)mad_i24(floor_int(ly), pitch4, (int)w.base4);
+     * footprint entry is inside the padded image, whose border repeats the
+     * edge texels; |floor(ly)| <= 8 and pitch8 < 2^23 fit the 24-bit
+     * product; the sum is the 32-bit unsigned byte offset */
+    *off = ((uint32_t)floor_int(lx) << 3) + (uint32_t)mad_i24(floor_int(ly), pitch8, (int)w.base8);
 }
 
-/* The padded pool as a buffer resource (gfx9 word 3). */
+/* The padded pool as a buffer resource (gfx9 word 3): offsets are unsigned
+ * 32-bit, the pool is below 2^29 entries of 8 bytes. */
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t pool_rsrc(const uint32_t* pool)
 {
-    return __builtin_amdgcn_make_buffer_rsrc((void*)pool, (short)0, 0x7fffffff, 0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)pool, (short)0, (int)0xffffffffu, 0x00020000);
 }
 
-/* the 2x2 texels of a footprint: entry (x, y) of the pool holds the FP16
- * texels (x, y) and (x, y + 1), so the footprint at byte offset off is the
- * two entries (x, y), (x + 1, y): one 8-byte load */
+/* the bilinear coefficients of a footprint (hpm_half.h): entry (x, y) holds
+ * c0 | c1 << 16 and c2 | c3 << 16 as FP16, one 8-byte load */
 struct Footprint {
-    uint32_t left, right;
+    uint32_t lo, hi;
 };
 
 __device__ __forceinline__ Footprint footprint(__amdgpu_buffer_rsrc_t pool, uint32_t off)
@@ -258,25 +259,25 @@ __device__ __forceinline__ Footprint footprint(__amdgpu_buffer_rsrc_t pool, uint
     typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
     const u32x2 v = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(pool, off, 0, 0));
     Footprint f;
-    f.left = v.x;
-    f.right = v.y;
+    f.lo = v.x;
+    f.hi = v.y;
     return f;
 }
 
 typedef _Float16 hpm_h2 __attribute__((ext_vector_type(2)));
 
 /* linear-filtered, clamp-addressed fetch (the reference's tex2D on a
- * cudaFilterModeLinear texture, HPM.cpp:599-617) from a footprint: texels
- * (x, y), (x + 1, y), (x, y + 1), (x + 1, y + 1), in FP32 arithmetic on the
- * (exact) FP16 values; one = 1.0f lets each row difference be one
- * v_fma_mix_f32 on the FP16 halves */
-__device__ __forceinline__ float bilerp(Footprint q, float fa, float fb, float one)
+ * cudaFilterModeLinear texture, HPM.cpp:599-617) at fractions (fa, fb) of a
+ * footprint: (c0 + fa c1) + fb (c2 + fa c3) as hpm_bilinear_eval, each
+ * step one fused multiply-add, the first two v_fma_mix_f32 on the FP16
+ * coefficients */
+__device__ __forceinline__ float bilerp(Footprint q, float fa, float fb)
 {
-    const hpm_h2 l = __builtin_bit_cast(hpm_h2, q.left);
-    const hpm_h2 r = __builtin_bit_cast(hpm_h2, q.right);
-    const float v0 = fmaf(fa, fmaf((float)r.x, one, -(float)l.x), (float)l.x);
-    const float v1 = fmaf(fa, fmaf((float)r.y, one, -(float)l.y), (float)l.y);
-    return fmaf(fb, v1 - v0, v0);
+    const hpm_h2 l = __builtin_bit_cast(hpm_h2, q.lo);
+    const hpm_h2 h = __builtin_bit_cast(hpm_h2, q.hi);
+    const float v0 = fmaf(fa, (float)l.y, (float)l.x);
+    const float v = fmaf(fa, (float)h.y, (float)h.x);
+    return fmaf(fb, v, v0);
 }
 
 /* Bilateral weights of the reference taps, (w, w * (r - c)) per tap and
@@ -331,14 +332,14 @@ __device__ __forceinline__ void make_ref_patch(const float* img, int W, int H, i
 /* One column of the window (6 taps at x offset dx): coordinates, then the
  * 6 gathers. */
 template <int kPath>
-__device__ __forceinline__ void fetch_column(__amdgpu_buffer_rsrc_t pool, int pitch4, const Warp& w, float dx,
+__device__ __forceinline__ void fetch_column(__amdgpu_buffer_rsrc_t pool, int pitch8, const Warp& w, float dx,
                                              Footprint* quad, float* fa, float* fb)
 {
     uint32_t off[6];
     const Column c = column(w, dx);
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        tap_coord<kPath>(w, c, pitch4, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
+        tap_coord<kPath>(w, c, pitch8, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
     }
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
@@ -349,12 +350,11 @@ __device__ __forceinline__ void fetch_column(__amdgpu_buffer_rsrc_t pool, int pi
 /* Adds the 6 taps of window column a (taps 6a .. 6a + 5) to the sums, in
  * tap order; tap 14 is the shift itself. */
 __device__ __forceinline__ void accumulate_column(const RefPatch& rp, int a, float shift, const Footprint* quad,
-                                                  const float* fa, const float* fb, float one, float* ss, float* sss,
-                                                  float* srs)
+                                                  const float* fa, const float* fb, float* ss, float* sss, float* srs)
 {
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
-        const float raw = (a == 2 && b == 2) ? shift : bilerp(quad[b], fa[b], fb[b], one);
+        const float raw = (a == 2 && b == 2) ? shift : bilerp(quad[b], fa[b], fb[b]);
         const float s = raw - shift;
         const float2 wk = s_refw[6 * a + b][rp.lane];
         const float ws = wk.x * s;
@@ -368,8 +368,8 @@ __device__ __forceinline__ void accumulate_column(const RefPatch& rp, int a, flo
  * column: the gathers of column a + 1 are in flight while column a is
  * accumulated. Tap k = 6 a + b sits at offset (2a - 5, 2b - 5). */
 template <int kPath>
-__device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, int pitch4, float one, const RefPatch& rp,
-                                         const Warp& w, float* ss_out, float* sss_out, float* srs_out)
+__device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, int pitch8, const RefPatch& rp, const Warp& w,
+                                         float* ss_out, float* sss_out, float* srs_out)
 {
     float ss = 0.0f, sss = 0.0f, srs = 0.0f;
     Footprint cur[6], nxt[6];
@@ -378,23 +378,23 @@ __device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, int pitch4
      * offset (-1, -1)), sampled first: a constant patch sums to exactly 0 */
     uint32_t coff;
     float cfa, cfb;
-    tap_coord<kPath>(w, column(w, -1.0f), pitch4, -1.0f, &coff, &cfa, &cfb);
+    tap_coord<kPath>(w, column(w, -1.0f), pitch8, -1.0f, &coff, &cfa, &cfb);
     const Footprint cq = footprint(pool, coff);
-    fetch_column<kPath>(pool, pitch4, w, -5.0f, cur, fa, fb);
-    const float shift = bilerp(cq, cfa, cfb, one);
+    fetch_column<kPath>(pool, pitch8, w, -5.0f, cur, fa, fb);
+    const float shift = bilerp(cq, cfa, cfb);
     /* two columns per trip with the buffers alternating; the last two
      * columns follow the loop, so every prefetch is unconditional and the
      * waits stay on the older loads */
 #pragma unroll 1
     for (int a = 0; a < 4; a += 2) {
-        fetch_column<kPath>(pool, pitch4, w, (float)(2 * a - 3), nxt, nfa, nfb);
-        accumulate_column(rp, a, shift, cur, fa, fb, one, &ss, &sss, &srs);
-        fetch_column<kPath>(pool, pitch4, w, (float)(2 * a - 1), cur, fa, fb);
-        accumulate_column(rp, a + 1, shift, nxt, nfa, nfb, one, &ss, &sss, &srs);
+        fetch_column<kPath>(pool, pitch8, w, (float)(2 * a - 3), nxt, nfa, nfb);
+        accumulate_column(rp, a, shift, cur, fa, fb, &ss, &sss, &srs);
+        fetch_column<kPath>(pool, pitch8, w, (float)(2 * a - 1), cur, fa, fb);
+        accumulate_column(rp, a + 1, shift, nxt, nfa, nfb, &ss, &sss, &srs);
     }
-    fetch_column<kPath>(pool, pitch4, w, 5.0f, nxt, nfa, nfb);
-    accumulate_column(rp, 4, shift, cur, fa, fb, one, &ss, &sss, &srs);
-    accumulate_column(rp, 5, shift, nxt, nfa, nfb, one, &ss, &sss, &srs);
+    fetch_column<kPath>(pool, pitch8, w, 5.0f, nxt, nfa, nfb);
+    accumulate_column(rp, 4, shift, cur, fa, fb, &ss, &sss, &srs);
+    accumulate_column(rp, 5, shift, nxt, nfa, nfb, &ss, &sss, &srs);
     *ss_out = ss;
     *sss_out = sss;
     *srs_out = srs;
@@ -409,15 +409,15 @@ __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& 
     if (!make_warp(sv, q, px, py, &w)) {
         return 2.0f;
     }
-    const int pitch4 = sv->pitch * 4;
+    const int pitch8 = sv->pitch * 8;
     const __amdgpu_buffer_rsrc_t pool = pool_rsrc(pb.images_pad);
     float ss, sss, srs;
     if (__all(w.inner)) {
-        ncc_sums<kPathInner>(pool, pitch4, pb.one, rp, w, &ss, &sss, &srs);
+        ncc_sums<kPathInner>(pool, pitch8, rp, w, &ss, &sss, &srs);
     } else if (__all(w.small)) {
-        ncc_sums<kPathSmall>(pool, pitch4, pb.one, rp, w, &ss, &sss, &srs);
+        ncc_sums<kPathSmall>(pool, pitch8, rp, w, &ss, &sss, &srs);
     } else {
-        ncc_sums<kPathAny>(pool, pitch4, pb.one, rp, w, &ss, &sss, &srs);
+        ncc_sums<kPathAny>(pool, pitch8, rp, w, &ss, &sss, &srs);
     }
     const float ms = ss * rp.inv_w;
     const float var_s = sss * rp.inv_w - ms * ms;

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "hpm_capi.h"
@@ -196,7 +197,6 @@ void fill_problem(const hpm_handle* h, hpm_dev_problem* pb)
     iter_consts(pb);
     pb->images = h->pool->d_images;
     pb->images_pad = h->pool->d_images_pad;
-    pb->one = 1.0f;
     pb->depths = h->params.multi_geometry ? h->pool->d_depth_geom : h->pool->d_depth_plain;
     pb->views = h->d_views;
     pb->hyp = h->d_hyp;
@@ -315,14 +315,14 @@ extern "C" hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera
         p->off.push_back(p->total);
         p->total += (int64_t)p->cams.back().width * p->cams.back().height;
         p->pad_off.push_back(p->pad_total);
-        p->pad_total += (int64_t)(p->cams.back().width + 3) * (p->cams.back().height + 2);
+        p->pad_total += (int64_t)(p->cams.back().width + 2) * (p->cams.back().height + 2);
     }
     p->have_state.assign((size_t)num_views, 0);
-    bool fits = p->pad_total < ((int64_t)1 << 29); /* 31-bit byte offsets into the padded pool */
+    bool fits = p->pad_total < ((int64_t)1 << 29); /* 32-bit byte offsets of 8-byte entries */
     for (const hpm_camera& c : p->cams) {
         /* the tap offset's 24-bit signed multiply takes the row pitch in
          * bytes and the row index */
-        fits = fits && (int64_t)(c.width + 3) * 4 < ((int64_t)1 << 23) && c.height + 2 < (1 << 23);
+        fits = fits && (int64_t)(c.width + 2) * 8 < ((int64_t)1 << 23) && c.height + 2 < (1 << 23);
     }
     if (!fits) {
         delete p;
@@ -331,33 +331,62 @@ extern "C" hpm_pool* hpm_pool_create(int device, int num_views, const hpm_camera
     bool ok = hipSetDevice(p->device) == hipSuccess &&
               hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc((void**)&p->d_images, (size_t)p->total * 4) == hipSuccess &&
-              hipMalloc((void**)&p->d_images_pad, (size_t)p->pad_total * 4) == hipSuccess &&
+              hipMalloc((void**)&p->d_images_pad, (size_t)p->pad_total * 8) == hipSuccess &&
               hipMalloc((void**)&p->d_depth_plain, (size_t)p->total * 4) == hipSuccess &&
               hipMalloc((void**)&p->d_depth_geom, (size_t)p->total * 4) == hipSuccess &&
               hipMalloc((void**)&p->d_state_hyp, (size_t)p->total * 16) == hipSuccess &&
               hipMalloc((void**)&p->d_state_cost, (size_t)p->total * 4) == hipSuccess;
     std::vector<uint32_t> padded;
+    std::vector<float> t16;
+    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     for (int i = 0; ok && i < num_views; ++i) {
         ok = hipMemcpyAsync(p->d_images + p->off[i], p->images[i].data(), p->images[i].size() * 4,
                             hipMemcpyHostToDevice, p->stream) == hipSuccess;
         const int W = p->cams[i].width, H = p->cams[i].height;
         const float* img = p->images[i].data();
-        auto at = [&](int x, int y) {
-            return img[(size_t)std::min(std::max(y, 0), H - 1) * W + std::min(std::max(x, 0), W - 1)];
-        };
-        /* padded FP16 copy with vertical pairs: entry (x, y), x in [-1, W + 1],
-         * y in [-1, H], at (y + 1) * (W + 3) + x + 1, holds texel (x, y) in
-         * its low half and (x, y + 1) in its high half; borders repeat the
-         * edge texels */
-        const int pitch = W + 3;
-        padded.assign((size_t)pitch * (H + 2), 0u);
-        for (int y = -1; y <= H; ++y) {
-            uint32_t* dst = padded.data() + (size_t)(y + 1) * pitch;
-            for (int x = -1; x <= W + 1; ++x) {
-                dst[x + 1] = (uint32_t)hpm_f16_bits(at(x, y)) | ((uint32_t)hpm_f16_bits(at(x, y + 1)) << 16);
+        /* padded pool of bilinear footprints: entry (x, y), x in [-1, W],
+         * y in [-1, H], at (y + 1) * (W + 2) + x + 1, holds the FP16
+         * coefficients of the footprint of texels (x..x+1, y..y+1)
+         * (hpm_bilinear_coeffs of the FP16-rounded texels), c0 | c1 << 16
+         * then c2 | c3 << 16; borders repeat the edge texels (clamp
+         * addressing). Rows are built on host threads. */
+        const int pitch = W + 2;
+        t16.resize((size_t)W * H);
+        padded.assign((size_t)pitch * (H + 2) * 2, 0u);
+        auto rows = [&](int t) {
+            for (int y = t; y < H; y += nthreads) {
+                for (int x = 0; x < W; ++x) {
+                    t16[(size_t)y * W + x] = hpm_round_f16(img[(size_t)y * W + x]);
+                }
             }
+        };
+        auto entries = [&](int t) {
+            auto at = [&](int x, int y) {
+                return t16[(size_t)std::min(std::max(y, 0), H - 1) * W + std::min(std::max(x, 0), W - 1)];
+            };
+            for (int y = -1 + t; y <= H; y += nthreads) {
+                uint32_t* dst = padded.data() + (size_t)(y + 1) * pitch * 2;
+                for (int x = -1; x <= W; ++x) {
+                    uint16_t c[4];
+                    hpm_bilinear_coeffs(at(x, y), at(x + 1, y), at(x, y + 1), at(x + 1, y + 1), c);
+                    dst[2 * (x + 1)] = (uint32_t)c[0] | ((uint32_t)c[1] << 16);
+                    dst[2 * (x + 1) + 1] = (uint32_t)c[2] | ((uint32_t)c[3] << 16);
+                }
+            }
+        };
+        {
+            std::vector<std::thread> ts;
+            for (int t = 1; t < nthreads; ++t) ts.emplace_back(rows, t);
+            rows(0);
+            for (auto& th : ts) th.join();
         }
-        ok = ok && hipMemcpy(p->d_images_pad + p->pad_off[i], padded.data(), padded.size() * 4,
+        {
+            std::vector<std::thread> ts;
+            for (int t = 1; t < nthreads; ++t) ts.emplace_back(entries, t);
+            entries(0);
+            for (auto& th : ts) th.join();
+        }
+        ok = ok && hipMemcpy(p->d_images_pad + 2 * p->pad_off[i], padded.data(), padded.size() * 4,
                              hipMemcpyHostToDevice) == hipSuccess;
     }
     ok = ok && hipMemsetAsync(p->d_depth_plain, 0, (size_t)p->total * 4, p->stream) == hipSuccess &&
@@ -725,7 +754,7 @@ extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
         views[i].img_off = h->pool->off[(size_t)h->views[i]];
         views[i].depth_off = views[i].img_off;
         views[i].pad_off = h->pool->pad_off[(size_t)h->views[i]];
-        views[i].pitch = views[i].cam.width + 3;
+        views[i].pitch = views[i].cam.width + 2;
     }
     if (h->alloc_pixels != np) {
         release_problem(h);

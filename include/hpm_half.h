@@ -12,6 +12,7 @@
 #ifndef HPM_HALF_H
 #define HPM_HALF_H
 
+#include <math.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -72,6 +73,33 @@ static inline float hpm_f16_value(uint16_t h)
 static inline float hpm_round_f16(float f)
 {
     return hpm_f16_value(hpm_f16_bits(f));
+}
+
+/* The bilinear coefficients of a 2x2 texel footprint, as FP16: with the
+ * texels t00 = (x, y), t10 = (x + 1, y), t01 = (x, y + 1), t11 = (x + 1,
+ * y + 1) at their sampled values (hpm_round_f16: the arguments must be
+ * FP16 values already),
+ *   c[0] = t00, c[1] = t10 - t00, c[2] = t01 - t00,
+ *   c[3] = (t11 - t01) - (t10 - t00),
+ * differences in FP32 then rounded to FP16. For 8-bit images every
+ * coefficient is an integer of magnitude <= 510, exact in FP16. A sample at
+ * fractions (a, b) is then hpm_bilinear_eval: three fused multiply-adds. */
+static inline void hpm_bilinear_coeffs(float t00, float t10, float t01, float t11, uint16_t c[4])
+{
+    c[0] = hpm_f16_bits(t00);
+    c[1] = hpm_f16_bits(t10 - t00);
+    c[2] = hpm_f16_bits(t01 - t00);
+    c[3] = hpm_f16_bits((t11 - t01) - (t10 - t00));
+}
+
+/* the sample at fractions (a, b) of a footprint's coefficients:
+ * (c0 + a c1) + b (c2 + a c3), each bracket and the sum one fused
+ * multiply-add */
+static inline float hpm_bilinear_eval(const uint16_t c[4], float a, float b)
+{
+    const float v0 = fmaf(a, hpm_f16_value(c[1]), hpm_f16_value(c[0]));
+    const float v = fmaf(a, hpm_f16_value(c[3]), hpm_f16_value(c[2]));
+    return fmaf(b, v, v0);
 }
 
 #endif

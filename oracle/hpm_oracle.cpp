@@ -9,8 +9,11 @@
  * clock64()), this file and the HIP kernels implement one shared definition
  * instead:
  *   - exp/sin/cos/acos: hpm_fmath.h; 1/sqrt(x) as 1.0f / sqrtf(x);
- *   - bilinear sampling with exact FP32 weights, clamp-to-edge addressing,
- *     texel centres at integer coordinates (the reference samples at +0.5);
+ *   - bilinear sampling as (c0 + a c1) + b (c2 + a c3) of the footprint's
+ *     coefficients (c0 = t00, c1 = t10 - t00, c2 = t01 - t00,
+ *     c3 = t11 - t10 - t01 + t00, hpm_half.h), three fused multiply-adds
+ *     with exact FP32 fractions, clamp-to-edge addressing, texel centres at
+ *     integer coordinates (the reference samples at +0.5);
  *   - random numbers: Philox4x32-10 (hpm_philox.h), one counter stream per
  *     (pixel, launch);
  *   - the homography and the warped window centre in FP64 per (hypothesis,
@@ -25,8 +28,9 @@
  *   - NCC window sums of both signals accumulated relative to the reference
  *     centre value: NCC is invariant to a shift of either signal, and the
  *     shifted sums keep FP32 accurate on low-texture patches;
- *   - source images sampled as their FP16-rounded texels (hpm_half.h; exact
- *     for 8-bit grey images), the reference image at full FP32.
+ *   - source images sampled through FP16 coefficients of their FP16-rounded
+ *     texels (hpm_half.h; exact for 8-bit grey images), the reference image
+ *     at full FP32.
  * The file must be compiled with -ffp-contract=off and without fast-math.
  */
 #include "hpm_oracle.h"
@@ -241,10 +245,42 @@ static inline float fract1(float x)
     return fminf(x - floorf(x), 0x1.fffffep-1f);
 }
 
+/* A source image as it is sampled: per footprint entry (x, y), x in
+ * [-1, W], y in [-1, H] (row pitch W + 2), the four bilinear coefficients
+ * of texels (x..x+1, y..y+1) with the edge texels repeated (clamp
+ * addressing), hpm_bilinear_coeffs of the FP16-rounded texels img, held as
+ * floats. */
+static void make_coef_plane(const float* img, int W, int H, std::vector<float>* out)
+{
+    const int pitch = W + 2;
+    out->assign((size_t)pitch * (H + 2) * 4, 0.0f);
+    auto at = [&](int x, int y) { return img[(size_t)clampi(y, 0, H - 1) * W + clampi(x, 0, W - 1)]; };
+    for (int y = -1; y <= H; ++y) {
+        for (int x = -1; x <= W; ++x) {
+            uint16_t c[4];
+            hpm_bilinear_coeffs(at(x, y), at(x + 1, y), at(x, y + 1), at(x + 1, y + 1), c);
+            float* d = out->data() + 4 * ((size_t)(y + 1) * pitch + (x + 1));
+            for (int k = 0; k < 4; ++k) {
+                d[k] = hpm_f16_value(c[k]);
+            }
+        }
+    }
+}
+
+/* the sample at fractions (a, b) of footprint entry (ix, iy), ix in
+ * [-1, W], iy in [-1, H]: hpm_bilinear_eval */
+static inline float sample_coef(const float* coef, int W, int ix, int iy, float a, float b)
+{
+    const float* c = coef + 4 * ((size_t)(iy + 1) * (W + 2) + (ix + 1));
+    const float v0 = fmaf(a, c[1], c[0]);
+    const float v = fmaf(a, c[3], c[2]);
+    return fmaf(b, v, v0);
+}
+
 /* tex2D with cudaFilterModeLinear and clamp addressing (HPM.cpp:599-617) at
  * the tap (dx, dy) of a warp; texel centres at integer coordinates (the
  * reference samples at +0.5). */
-static inline float sample_tap(const float* img, int W, int H, const Warp& w, float dx, float dy)
+static inline float sample_tap(const float* coef, int W, int H, const Warp& w, float dx, float dy)
 {
     /* the dx products are per window column, the dy terms fused per tap */
     const float u = fmaf(w.ey, dy, w.ex * dx);
@@ -257,15 +293,7 @@ static inline float sample_tap(const float* img, int W, int H, const Warp& w, fl
     const float b = fract1(ly);
     const int ix = w.ix0 + (int)floorf(lx);
     const int iy = w.iy0 + (int)floorf(ly);
-    const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
-    const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
-    const float t00 = img[(size_t)y0 * W + x0];
-    const float t10 = img[(size_t)y0 * W + x1];
-    const float t01 = img[(size_t)y1 * W + x0];
-    const float t11 = img[(size_t)y1 * W + x1];
-    const float v0 = fmaf(a, t10 - t00, t00);
-    const float v1 = fmaf(a, t11 - t01, t01);
-    return fmaf(b, v1 - v0, v0);
+    return sample_coef(coef, W, ix, iy, a, b);
 }
 
 static inline float texel(const float* img, int W, int H, int x, int y)
@@ -331,7 +359,8 @@ static void make_ref_patch(const float* img, int W, int H, int px, int py, const
 
 struct Ctx {
     const hpm_oracle_problem* pb;
-    std::vector<float> src16[HPM_MAX_IMAGES]; /* source images as sampled: FP16-rounded texels */
+    std::vector<float> src16[HPM_MAX_IMAGES]; /* source images at their FP16-rounded texels */
+    std::vector<float> coef[HPM_MAX_IMAGES];  /* source images as sampled (make_coef_plane) */
     int W, H, nsrc;
     ViewRel rel[HPM_MAX_IMAGES];
     RefK rk;
@@ -346,7 +375,7 @@ static float ncc(const Ctx& cx, const RefPatch& rp, int px, int py, int v, const
     const hpm_oracle_problem* pb = cx.pb;
     const hpm_camera& sc = pb->cams[v];
     const int SW = sc.width, SH = sc.height;
-    const float* simg = cx.src16[v].data();
+    const float* simg = cx.coef[v].data(); /* as sampled */
     Warp w;
     if (!make_warp(cx.rel[v], q, px, py, SW, SH, (float)(pb->params.patch_size / 2), &w)) {
         return 2.0f;
@@ -377,19 +406,12 @@ static float ncc(const Ctx& cx, const RefPatch& rp, int px, int py, int v, const
 
 /* Bilinear sample at an absolute source coordinate (x, y), clamped to
  * [-1, W] x [-1, H] with edge texels repeated: the sampling of sample_tap. */
-static inline float sample_abs(const float* img, int W, int H, float x, float y)
+static inline float sample_abs(const float* coef, int W, int H, float x, float y)
 {
     x = fminf(fmaxf(x, -1.0f), (float)W);
     y = fminf(fmaxf(y, -1.0f), (float)H);
     const float a = fract1(x), b = fract1(y);
-    const int ix = (int)floorf(x), iy = (int)floorf(y);
-    const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
-    const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
-    const float t00 = img[(size_t)y0 * W + x0], t10 = img[(size_t)y0 * W + x1];
-    const float t01 = img[(size_t)y1 * W + x0], t11 = img[(size_t)y1 * W + x1];
-    const float v0 = fmaf(a, t10 - t00, t00);
-    const float v1 = fmaf(a, t11 - t01, t01);
-    return fmaf(b, v1 - v0, v0);
+    return sample_coef(coef, W, (int)floorf(x), (int)floorf(y), a, b);
 }
 
 /* Accuracy evidence only: ComputeBilateralNCC (HPM.cu:227-397) in the
@@ -404,7 +426,7 @@ static double ncc32_reference_order(const hpm_oracle_problem* pb, const Ctx& cx,
     const hpm_params& P = pb->params;
     const int SW = sc.width, SH = sc.height;
     const float* rimg = pb->images[0];
-    const float* simg = cx.src16[v].data();
+    const float* simg = cx.coef[v].data(); /* as sampled */
     float refC[3], srcC[3], Rr[9], Cr[3], tr[3], H[9], tmp[9];
     for (int a = 0; a < 3; ++a) {
         refC[a] = -(rc.R[a] * rc.t[0] + rc.R[3 + a] * rc.t[1] + rc.R[6 + a] * rc.t[2]);
@@ -499,7 +521,7 @@ static double ncc_warp32(const Ctx& cx, const RefPatch& rp, int px, int py, int 
     const hpm_oracle_problem* pb = cx.pb;
     const hpm_camera& sc = pb->cams[v];
     const int SW = sc.width, SH = sc.height;
-    const float* simg = cx.src16[v].data();
+    const float* simg = cx.coef[v].data(); /* as sampled */
     const ViewRel& vr = cx.rel[v];
     const float q[3] = {(float)qd[0], (float)qd[1], (float)qd[2]};
     float H[9];
@@ -930,6 +952,7 @@ static void make_ctx(const hpm_oracle_problem* pb, Ctx* cx)
         for (size_t i = 0; i < n; ++i) {
             dst[i] = hpm_round_f16(src[i]);
         }
+        make_coef_plane(dst.data(), pb->cams[v].width, pb->cams[v].height, &cx->coef[v]);
     }
     cx->rk = make_ref_k(pb->cams[0]);
     cx->ic = make_iter_consts();

@@ -14,7 +14,10 @@ max_image_size downscale of InuputInitialization.
     bracket, and the depth accuracy against ground truth;
   - views above max_image_size are resized and their cameras scaled
     (HPM.cpp:524-551) before anything else: the run equals the oracle's on
-    the resized images and scaled cameras, bit for bit.
+    the resized images and scaled cameras, bit for bit;
+  - a pool above 2 GiB (its entries are addressed by unsigned 32-bit byte
+    offsets): a problem whose views lie past 2^31 bytes gives the same maps
+    as the same views at small offsets.
 """
 import numpy as np
 import pytest
@@ -178,3 +181,35 @@ def test_downscale_above_max_image_size(hpm, cpu):
     say(f"identical pixels {same.mean():.6f}, n_eval gpu={pr.run_stats().n_eval} cpu={o.n_eval}")
     assert same.all() and pr.run_stats().n_eval == o.n_eval
     pr.close()
+
+
+def test_pool_beyond_2gib(hpm):
+    """38 views: 35 fillers of 3200x2400 put the last three views' footprint
+    entries (8 bytes each) past byte 2^31 of the pool; the same three views
+    behind 35 fillers of 16x16 sit at small offsets. The pass schedule's
+    plain RunPatchMatch of reference 37 with sources 36 and 35 (same ids, so
+    the same random streams) must give bit-identical maps in both pools."""
+    import copy
+
+    W, H = 3200, 2400
+    cfg = hpm.scene_config(seed=31, width=W, height=H, fx=2500.0, fy=2500.0, cx=1600.0, cy=1200.0, num_views=3)
+    cams, imgs, gd, gn, low = hpm.render_scene(cfg, want_gt=False)
+    imgs = [im.astype(np.float32) for im in imgs]
+    small = copy.copy(cams[0])
+    small.width, small.height = 16, 16
+    tiny = np.zeros((16, 16), np.float32)
+    problems = [(37, [36, 35])]
+    opt = hpm.pipeline_options(planar_prior=0, geom_consistency=0, workers=1)
+    opt.params.max_image_size = 3200
+    states = []
+    for filler_cam, filler_img in ((cams[0], imgs[0]), (small, tiny)):
+        say(f"case: 38-view pool, fillers {filler_cam.width}x{filler_cam.height}, problem 37 <- 36, 35 at {W}x{H}")
+        pool = hpm.Pool([filler_cam] * 35 + list(cams), [filler_img] * 35 + imgs, max_image_size=3200)
+        st = hpm.run_pipeline(pool, problems, opt)
+        say(f"  {st.runs} RunPatchMatch, {st.n_eval} evaluations")
+        states.append(pool.state(37))
+        pool.close()
+    (h0, c0), (h1, c1) = states
+    assert np.array_equal(h0.view(np.uint32), h1.view(np.uint32))
+    assert np.array_equal(c0.view(np.uint32), c1.view(np.uint32))
+    assert np.mean(c0 < 1.0) > 0.5  # a textured view was matched, not a blank one

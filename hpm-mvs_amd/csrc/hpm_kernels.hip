@@ -90,11 +90,53 @@ __device__ __forceinline__ float texel(const float* img, int W, int H, int x, in
     return img[(size_t)clampi(y, 0, H - 1) * W + clampi(x, 0, W - 1)];
 }
 
+/* Correctly rounded square root, quotient and reciprocal for operands that
+ * need no range scaling: the hardware's approximations refined exactly as
+ * the general (scaling, special-case fixing) instruction sequences refine
+ * them, without the steps that only act on denormal, infinite, zero or
+ * extreme-exponent operands. For operands in the stated ranges the results
+ * are the IEEE ones (checked against the host's on the GPU). */
+
+/* sqrt(x) for normal positive finite x: the hardware root s, then whichever
+ * of s - 1 ulp, s, s + 1 ulp the exact residuals x - t s select */
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    const float t = fmaf(-sd, s, x) <= 0.0f ? sd : s;
+    return fmaf(-su, s, x) > 0.0f ? su : t;
+}
+
+/* n / d for normal d and |n| / |d| far from the exponent range's ends (n may
+ * be zero): reciprocal, one Newton step, the quotient and two exact
+ * residual corrections */
+__device__ __forceinline__ float div_rn(float n, float d)
+{
+    float r = __builtin_amdgcn_rcpf(d);
+    r = fmaf(fmaf(-d, r, 1.0f), r, r);
+    float q = n * r;
+    q = fmaf(fmaf(-d, q, n), r, q);
+    return fmaf(fmaf(-d, q, n), r, q);
+}
+
+/* 1 / d for normal d of moderate exponent: reciprocal, two Newton steps and
+ * an exact residual correction. A zero d gives NaN (the general sequence
+ * gives an infinity); both make every warp built from it fail its bounds
+ * test, so a hypothesis scores the same. */
+__device__ __forceinline__ double rcp_rn(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(fma(-d, r, 1.0), r, r);
+}
+
 /* q = Kr^-T n / w of one hypothesis (double); the homography of source
  * view v is then H = A_v - b_v q^T (HPM.cu:227-287) */
 __device__ __forceinline__ void plane_q(const hpm_dev_problem& pb, float4 n, double q[3])
 {
-    const double iw = 1.0 / (double)n.w;
+    const double iw = rcp_rn((double)n.w);
     const double nx = n.x, ny = n.y, nz = n.z;
     q[0] = (nx * pb.rk_ifx) * iw;
     q[1] = (ny * pb.rk_ify) * iw;
@@ -114,6 +156,7 @@ struct Warp {
     bool small;           /* |e . d| <= 0.01 over the window: cubic reciprocal */
     bool tiny;            /* |e . d| <= 0.002 over the window: quadratic reciprocal */
     bool inner;           /* tiny, and every tap lies inside the clamp: the clamp is the identity */
+    bool lin;             /* inner, and the linear reciprocal 1 - e . d moves no tap by more than 2e-6 px */
 };
 
 /* false when the warped centre leaves the source image (HPM.cu:333-335) */
@@ -132,7 +175,7 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
     const double hx = fma(H[0], X, fma(H[1], Y, H[2]));
     const double hy = fma(H[3], X, fma(H[4], Y, H[5]));
     const double hz = fma(H[6], X, fma(H[7], Y, H[8]));
-    const double icz = 1.0 / hz;
+    const double icz = rcp_rn(hz);
     const double sx0 = hx * icz;
     const double sy0 = hy * icz;
     if (!(sx0 >= 0.0 && sx0 < (double)SW && sy0 >= 0.0 && sy0 < (double)SH)) {
@@ -163,25 +206,35 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
     const float by = fmaf(5.0f * (fabsf(w->gyx) + fabsf(w->gyy)), 1.0102f, 1e-3f);
     w->inner = w->tiny && w->fx0 - bx >= w->lox && w->fx0 + bx <= w->hix && w->fy0 - by >= w->loy &&
                w->fy0 + by <= w->hiy;
+    /* 1 - u for 1 / (1 + u) errs by u^2 / (1 + u): a tap offset of at most
+     * max(bx, by) px moves by at most bu^2 max(bx, by) / (1 - bu) */
+    const float bu = (fabsf(w->ex) + fabsf(w->ey)) * 5.0f;
+    w->lin = w->inner && (bu * bu) * fmaxf(bx, by) <= 2e-6f;
     return true;
 }
 
-/* Paths of the window sums, chosen per wave: every active lane has an inner
- * warp (the quadratic reciprocal, no clamp), a small warp (per lane the
- * quadratic or the cubic, with the clamp), or any warp (per lane the
- * quadratic, the cubic or the quotient, with the clamp). The three give the
- * same values per lane. */
-enum : int { kPathInner = 0, kPathSmall = 1, kPathAny = 2 };
+/* Paths of the window sums, chosen per wave: every active lane has a
+ * linear warp (the linear reciprocal, no clamp), an inner warp (per lane the
+ * linear or the quadratic reciprocal, no clamp), a small warp (per lane the
+ * linear, quadratic or cubic, with the clamp), or any warp (per lane the
+ * linear, quadratic, cubic or the quotient, with the clamp). The four give
+ * the same values per lane. */
+enum : int { kPathLinear = 0, kPathInner = 1, kPathSmall = 2, kPathAny = 3 };
 
 /* The dx products of one window column (x offset dx), shared by its taps. */
 struct Column {
     float ex, gx, gy; /* ex dx, gxx dx, gyx dx */
+    float omex;       /* 1 - ex dx, one rounding: the linear reciprocal's column term */
 };
 
+template <int kPath>
 __device__ __forceinline__ Column column(const Warp& w, float dx)
 {
     Column c;
-    c.ex = w.ex * dx;
+    if (kPath != kPathLinear) {
+        c.ex = w.ex * dx;
+    }
+    c.omex = fmaf(-w.ex, dx, 1.0f);
     c.gx = w.gxx * dx;
     c.gy = w.gyx * dx;
     return c;
@@ -217,18 +270,23 @@ template <int kPath>
 __device__ __forceinline__ void tap_coord(const Warp& w, const Column& c, int pitch8, float dy, uint32_t* off,
                                           float* fa, float* fb)
 {
-    const float u = fmaf(w.ey, dy, c.ex);
+    /* the linear reciprocal 1 - ex dx - ey dy, one fused multiply-add */
+    const float lin = fmaf(-w.ey, dy, c.omex);
     float inv;
-    if (kPath == kPathInner) {
-        inv = fmaf(u, u - 1.0f, 1.0f);
+    if (kPath == kPathLinear) {
+        inv = lin;
+    } else if (kPath == kPathInner) {
+        const float u = fmaf(w.ey, dy, c.ex);
+        inv = w.lin ? lin : fmaf(u, u - 1.0f, 1.0f);
     } else {
         /* 1 + u (u k - 1): k = 1 the quadratic, k = 1 - u the cubic */
+        const float u = fmaf(w.ey, dy, c.ex);
         const float poly = fmaf(u, fmaf(u, fmaf(-u, w.cub, 1.0f), -1.0f), 1.0f);
-        inv = (kPath != kPathAny || w.small) ? poly : 1.0f / (1.0f + u);
+        inv = w.lin ? lin : ((kPath != kPathAny || w.small) ? poly : 1.0f / (1.0f + u));
     }
     float lx = fmaf(fmaf(w.gxy, dy, c.gx), inv, w.fx0);
     float ly = fmaf(fmaf(w.gyy, dy, c.gy), inv, w.fy0);
-    if (kPath != kPathInner) {
+    if (kPath == kPathSmall || kPath == kPathAny) {
         lx = fminf(fmaxf(lx, w.lox), w.hix);
         ly = fminf(fmaxf(ly, w.loy), w.hiy);
     }
@@ -336,7 +394,7 @@ __device__ __forceinline__ void fetch_column(__amdgpu_buffer_rsrc_t pool, int pi
                                              Footprint* quad, float* fa, float* fb)
 {
     uint32_t off[6];
-    const Column c = column(w, dx);
+    const Column c = column<kPath>(w, dx);
 #pragma unroll
     for (int b = 0; b < 6; ++b) {
         tap_coord<kPath>(w, c, pitch8, (float)(2 * b - 5), &off[b], &fa[b], &fb[b]);
@@ -378,7 +436,7 @@ __device__ __forceinline__ void ncc_sums(__amdgpu_buffer_rsrc_t pool, int pitch8
      * offset (-1, -1)), sampled first: a constant patch sums to exactly 0 */
     uint32_t coff;
     float cfa, cfb;
-    tap_coord<kPath>(w, column(w, -1.0f), pitch8, -1.0f, &coff, &cfa, &cfb);
+    tap_coord<kPath>(w, column<kPath>(w, -1.0f), pitch8, -1.0f, &coff, &cfa, &cfb);
     const Footprint cq = footprint(pool, coff);
     fetch_column<kPath>(pool, pitch8, w, -5.0f, cur, fa, fb);
     const float shift = bilerp(cq, cfa, cfb);
@@ -412,7 +470,9 @@ __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& 
     const int pitch8 = sv->pitch * 8;
     const __amdgpu_buffer_rsrc_t pool = pool_rsrc(pb.images_pad);
     float ss, sss, srs;
-    if (__all(w.inner)) {
+    if (__all(w.lin)) {
+        ncc_sums<kPathLinear>(pool, pitch8, rp, w, &ss, &sss, &srs);
+    } else if (__all(w.inner)) {
         ncc_sums<kPathInner>(pool, pitch8, rp, w, &ss, &sss, &srs);
     } else if (__all(w.small)) {
         ncc_sums<kPathSmall>(pool, pitch8, rp, w, &ss, &sss, &srs);
@@ -426,8 +486,10 @@ __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& 
         return 2.0f;
     }
     const float cov = srs * rp.inv_w - rp.mr * ms;
-    const float den = sqrtf(rp.var_r * var_s);
-    return fmaxf(0.0f, fminf(2.0f, 1.0f - cov / den));
+    /* var_r, var_s >= 1e-5 and both below 255^2: the product and its root are
+     * normal, the quotient needs no scaling */
+    const float den = sqrt_rn(rp.var_r * var_s);
+    return fmaxf(0.0f, fminf(2.0f, 1.0f - div_rn(cov, den)));
 }
 
 /* HPM.cu:102-133 */
@@ -1550,11 +1612,16 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             __builtin_amdgcn_wave_barrier();
             const float wview = draws_of(vw, vi);
             const int total = 5 * n;
+            /* t / n for t < 5 n <= 320: (t + 1/2) / n lies at least 1 / (2 n)
+             * >= 1/128 inside an integer interval, far beyond the float
+             * rounding of the product, so truncation gives the quotient */
+            const float rn = 1.0f / (float)n;
+            const float hrn = 0.5f * rn;
 #pragma unroll 1
             for (int t0 = 0; t0 < total; t0 += 64) {
                 const int t = t0 + ln;
                 const bool act = t < total;
-                const int c = act ? t / n : 0;
+                const int c = act ? (int)fmaf((float)t, rn, hrn) : 0;
                 const int owner = act ? s_own[wv][t - c * n] : ln;
                 const int c_lo = t0 / n;
                 const int c_hi = (t0 + 63) / n < 4 ? (t0 + 63) / n : 4;
@@ -2195,9 +2262,19 @@ __global__ __launch_bounds__(256) void hpm_k_selftest(int kind, const void* in, 
         o[3] = r[3];
         return;
     }
+    if (kind == 8) {
+        const float* b = static_cast<const float*>(in) + 2 * (size_t)i;
+        static_cast<float*>(out)[i] = div_rn(b[0], b[1]);
+        return;
+    }
+    if (kind == 9) {
+        static_cast<double*>(out)[i] = rcp_rn(static_cast<const double*>(in)[i]);
+        return;
+    }
     const float x = static_cast<const float*>(in)[i];
     float y;
     switch (kind) {
+    case 7: y = sqrt_rn(x); break;
     case 0: y = hpm_expf(x); break;
     case 1: y = hpm_sinf(x); break;
     case 2: y = hpm_cosf(x); break;

@@ -1298,15 +1298,15 @@ extern "C" int hpm_debug_write_state(hpm_handle* h, const float* hyp4, const flo
 
 extern "C" int hpm_device_selftest(hpm_handle* h, int kind, const void* in, void* out, int n)
 {
-    if (!h || !in || !out || n < 0 || kind < 0 || kind > 6) {
+    if (!h || !in || !out || n < 0 || kind < 0 || kind > 9) {
         return hpm_fail(h, HPM_EINVAL, "bad selftest arguments");
     }
     if (n == 0) {
         return HPM_OK;
     }
     HPM_HIP(h, hipSetDevice(h->device));
-    const size_t in_bytes = (size_t)n * (kind == 4 ? 24 : 4);
-    const size_t out_bytes = (size_t)n * (kind == 4 ? 16 : 4);
+    const size_t in_bytes = (size_t)n * (kind == 4 ? 24 : (kind == 8 || kind == 9 ? 8 : 4));
+    const size_t out_bytes = (size_t)n * (kind == 4 ? 16 : (kind == 9 ? 8 : 4));
     void *d_in = nullptr, *d_out = nullptr;
     hipError_t e = hipMalloc(&d_in, in_bytes);
     if (e == hipSuccess) e = hipMalloc(&d_out, out_bytes);

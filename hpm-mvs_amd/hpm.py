@@ -553,6 +553,12 @@ class Problem:
         if kind == 4:
             a = np.ascontiguousarray(data, dtype=np.uint32).reshape(-1, 6)
             out = np.empty((len(a), 4), np.uint32)
+        elif kind == 8:  # quotients of (numerator, denominator) pairs
+            a = np.ascontiguousarray(data, dtype=np.float32).reshape(-1, 2)
+            out = np.empty(len(a), np.float32)
+        elif kind == 9:  # double reciprocals
+            a = np.ascontiguousarray(data, dtype=np.float64).reshape(-1)
+            out = np.empty(len(a), np.float64)
         else:
             a = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
             out = np.empty(len(a), np.float32)

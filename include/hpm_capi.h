@@ -199,8 +199,12 @@ int hpm_debug_read_prior(hpm_handle* h, float* planes4, uint32_t* mask);
 /* Evaluates the shared device arithmetic for comparison with a host build:
  * kind 0..3 = exp, sin, cos, acos of the n floats in `in` (out: n floats);
  * kind 4 = Philox4x32-10 of n blocks, each 6 uint32 (ctr0..3, key0..1) in
- * `in`, 4 uint32 out per block. No reference counterpart: a check of the
- * replacement for cuRAND and fast-math intrinsics. */
+ * `in`, 4 uint32 out per block; kind 5, 6 = the tap fraction and floor of
+ * n floats; kind 7 = the NCC's square root of n floats; kind 8 = the NCC's
+ * quotient of n (numerator, denominator) float pairs (out: n floats);
+ * kind 9 = the warp's double reciprocal of n doubles (out: n doubles). No
+ * reference counterpart: a check of the replacement for cuRAND and
+ * fast-math intrinsics and of the correctly rounded device operations. */
 int hpm_device_selftest(hpm_handle* h, int kind, const void* in, void* out, int n);
 
 #ifdef __cplusplus

@@ -186,7 +186,30 @@ struct Warp {
     float ex, ey, gxx, gxy, gyx, gyy;
     bool small; /* |e . d| <= 0.01 over the window: cubic reciprocal */
     bool tiny;  /* |e . d| <= 0.002 over the window: quadratic reciprocal */
+    bool lin;   /* tiny, every tap inside the clamp, and the linear reciprocal 1 - e . d moves no
+                   tap by more than 2e-6 px */
 };
+
+/* The reciprocal tiers of a warp of a SW x SH source, radius the largest
+ * tap offset of the window. The linear tier: 1 - u errs from 1 / (1 + u) by
+ * u^2 / (1 + u), so with |u| <= bu = radius (|ex| + |ey|) and tap offsets
+ * below bx = radius (|gxx| + |gxy|) (by for y) a tap moves by at most
+ * bu^2 max(bx, by) / (1 - bu); the bounds bx, by (with margins for the
+ * quadratic's and cubic's maxima and FP32 rounding) also show that every tap
+ * stays inside the clamp, which the tier requires. */
+static inline void set_tiers(Warp* w, int SW, int SH, float radius)
+{
+    const float bu = (fabsf(w->ex) + fabsf(w->ey)) * radius;
+    w->small = bu <= 0.01f;
+    w->tiny = bu <= 0.002f;
+    const float bx = fmaf(radius * (fabsf(w->gxx) + fabsf(w->gxy)), 1.0102f, 1e-3f);
+    const float by = fmaf(radius * (fabsf(w->gyx) + fabsf(w->gyy)), 1.0102f, 1e-3f);
+    const float lox = -1.0f - (float)w->ix0, hix = (float)(SW - w->ix0);
+    const float loy = -1.0f - (float)w->iy0, hiy = (float)(SH - w->iy0);
+    const bool inner = w->tiny && w->fx0 - bx >= lox && w->fx0 + bx <= hix && w->fy0 - by >= loy &&
+                       w->fy0 + by <= hiy;
+    w->lin = inner && (bu * bu) * fmaxf(bx, by) <= 2e-6f;
+}
 
 /* Returns false when the warped centre falls outside the source image (the
  * reference returns the maximum cost there, HPM.cu:333-335). radius is the
@@ -221,17 +244,21 @@ static inline bool make_warp(const ViewRel& v, const double q[3], int px, int py
     w->gxy = (float)((H[1] - sx0 * H[7]) * icz);
     w->gyx = (float)((H[3] - sy0 * H[6]) * icz);
     w->gyy = (float)((H[4] - sy0 * H[7]) * icz);
-    w->small = (fabsf(w->ex) + fabsf(w->ey)) * radius <= 0.01f;
-    w->tiny = (fabsf(w->ex) + fabsf(w->ey)) * radius <= 0.002f;
+    set_tiers(w, SW, SH, radius);
     return true;
 }
 
-/* 1 / (1 + u) of a tap: the quadratic 1 - u + u^2 in Horner form when the
+/* 1 / (1 + u) of a tap (u = ex dx + ey dy): in the linear tier
+ * 1 - ex dx - ey dy as two fused multiply-adds (the column term
+ * 1 - ex dx first); else the quadratic 1 - u + u^2 in Horner form when the
  * warp bounds |u| by 0.002 (truncation |u|^3 / (1 + u) <= 8.1e-9 relative),
  * the cubic 1 - u + u^2 - u^3 when it bounds |u| by 0.01 (u^4 / (1 + u) <=
  * 1.02e-8), else the correctly rounded quotient. */
-static inline float tap_inv(float u, const Warp& w)
+static inline float tap_inv(float u, const Warp& w, float dx, float dy)
 {
+    if (w.lin) {
+        return fmaf(-w.ey, dy, fmaf(-w.ex, dx, 1.0f));
+    }
     if (w.tiny) {
         return fmaf(u, u - 1.0f, 1.0f);
     }
@@ -284,7 +311,7 @@ static inline float sample_tap(const float* coef, int W, int H, const Warp& w, f
 {
     /* the dx products are per window column, the dy terms fused per tap */
     const float u = fmaf(w.ey, dy, w.ex * dx);
-    const float inv = tap_inv(u, w);
+    const float inv = tap_inv(u, w, dx, dy);
     float lx = fmaf(fmaf(w.gxy, dy, w.gxx * dx), inv, w.fx0);
     float ly = fmaf(fmaf(w.gyy, dy, w.gyx * dx), inv, w.fy0);
     lx = fminf(fmaxf(lx, -1.0f - (float)w.ix0), (float)(W - w.ix0));
@@ -551,8 +578,7 @@ static double ncc_warp32(const Ctx& cx, const RefPatch& rp, int px, int py, int 
     w.gxy = (H[1] - sx0 * H[7]) * icz;
     w.gyx = (H[3] - sy0 * H[6]) * icz;
     w.gyy = (H[4] - sy0 * H[7]) * icz;
-    w.small = (fabsf(w.ex) + fabsf(w.ey)) * (float)(pb->params.patch_size / 2) <= 0.01f;
-    w.tiny = (fabsf(w.ex) + fabsf(w.ey)) * (float)(pb->params.patch_size / 2) <= 0.002f;
+    set_tiers(&w, SW, SH, (float)(pb->params.patch_size / 2));
     float ss_ = 0.0f, sss = 0.0f, srs = 0.0f;
     const int kc = rp.centre_tap;
     const float shift = sample_tap(simg, SW, SH, w, rp.dx[kc], rp.dy[kc]);

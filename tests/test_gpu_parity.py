@@ -61,6 +61,24 @@ def test_shared_math_bitwise(hpm, cpu):
         got = pr.device_selftest(kind, x)
         bad = np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))
         assert bad.size == 0, (kind, x[bad[:5]], got[bad[:5]], ref[bad[:5]])
+    # the NCC finish's square root and quotient and the warp's double
+    # reciprocal, over their operand ranges and beyond: IEEE results (numpy
+    # rounds correctly), bit for bit
+    x = np.concatenate([np.exp(rng.uniform(np.log(1e-10), np.log(4.3e9), 300000)),
+                        rng.uniform(1e-10, 1e-9, 20000), rng.uniform(1.0, 4.0, 20000)]).astype(np.float32)
+    x = np.concatenate([x, np.array([1e-10, 4.3e9, 1.0, 2.0, 0.25, 65025.0**2], np.float32)])
+    got = pr.device_selftest(7, x)
+    assert np.array_equal(got.view(np.uint32), np.sqrt(x).view(np.uint32))
+    num = np.concatenate([rng.uniform(-70000.0, 70000.0, 200000), rng.uniform(-1e-3, 1e-3, 50000),
+                          np.exp(rng.uniform(np.log(6e-8), np.log(7e4), 100000)) * rng.choice([-1, 1], 100000),
+                          np.zeros(100)]).astype(np.float32)
+    den = np.exp(rng.uniform(np.log(1e-5), np.log(65025.0), len(num))).astype(np.float32)
+    got = pr.device_selftest(8, np.stack([num, den], 1))
+    assert np.array_equal(got, num / den)
+    d = np.concatenate([np.exp(rng.uniform(np.log(1e-30), np.log(1e30), 300000)) * rng.choice([-1, 1], 300000),
+                        rng.uniform(0.5, 2.0, 50000), np.array([1.0, -1.0, 3.0, 1e-300, 1e300])])
+    got = pr.device_selftest(9, d)
+    assert np.array_equal(got.view(np.uint64), (1.0 / d).view(np.uint64))
     blocks = rng.integers(0, 2**32, size=(4096, 6), dtype=np.uint64).astype(np.uint32)
     got = pr.device_selftest(4, blocks)
     for i in range(0, len(blocks), 97):

@@ -1211,12 +1211,13 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
         }
     }
 
-    /* cost_array[8][32] = { 2.0f } (HPM.cu:1244) */
-    float ca[8][HPM_MAX_SOURCES];
-    for (int d = 0; d < 8; ++d) {
-        for (int v = 0; v < nsrc; ++v) {
-            ca[d][v] = 0.0f;
-        }
+    /* cost_array[8][32] = { 2.0f } (HPM.cu:1244), held view-major here:
+     * ca[v][d] is direction d on source view v, so a view's eight costs are
+     * two 16-byte loads in the view selection and the candidate scoring */
+    float ca[HPM_MAX_SOURCES][8] __attribute__((aligned(16)));
+    for (int v = 0; v < nsrc; ++v) {
+        reinterpret_cast<float4*>(ca[v])[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        reinterpret_cast<float4*>(ca[v])[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     ca[0][0] = 2.0f;
     const uint32_t all_views = nsrc >= 32 ? 0xffffffffu : ((1u << nsrc) - 1u);
@@ -1257,7 +1258,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             plane_q(pb, ld4(pb.hyp, pos[d]), q);
 #pragma unroll 1
             for (int v = 0; v < nsrc; ++v) {
-                ca[d][v] = ncc(pb, rp, px, py, v + 1, q);
+                ca[v][d] = ncc(pb, rp, px, py, v + 1, q);
             }
             nev += (uint32_t)nsrc;
         }
@@ -1271,7 +1272,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
      * orders and decisions per lane are those of the sequential order. */
 
     /* NESP extension, orientation 0 only (HPM.cu:1473-1543, 780-843): up to
-     * 3 rounds, each judged on ca[0] */
+     * 3 rounds, each judged on direction 0's costs */
     bool ext_on = flag[0];
 #pragma unroll 1
     for (int k = 0; k < 3; ++k) {
@@ -1281,10 +1282,10 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             const float good = pb.good[iter][k];
             int good_sum = 0, bad_sum = 0;
             for (int i = 0; i < nsrc; ++i) {
-                if (ca[0][i] < good) {
+                if (ca[i][0] < good) {
                     ++good_sum;
                 }
-                if (ca[0][i] > 1.2f) { /* float bad_threshold (HPM.cu:1042) */
+                if (ca[i][0] > 1.2f) { /* float bad_threshold (HPM.cu:1042) */
                     ++bad_sum;
                 }
             }
@@ -1337,7 +1338,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                     plane_q(pb, plane, tq);
                     return ncc(pb, orp, opx, opy, v + 1, tq);
                 },
-                [&](int v, int, float val) { ca[0][v] = val; });
+                [&](int v, int, float val) { ca[v][0] = val; });
         }
     }
 
@@ -1367,8 +1368,12 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 float count = 0.0f;
                 int count_false = 0;
                 float tmpw = 0.0f;
+                const float4 c_lo = reinterpret_cast<const float4*>(ca[i])[0];
+                const float4 c_hi = reinterpret_cast<const float4*>(ca[i])[1];
+                const float cs[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
+#pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    const float c = ca[j][i];
+                    const float c = cs[j];
                     if (c < thr) {
                         tmpw += hpm_expf(c * c / (-0.18f));
                         count++;
@@ -1441,10 +1446,13 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 for (int j = 0; j < nsrc; ++j) {
                     if ((tsel >> j) & 1u) {
                         const float w = draws_of(vw, j);
+                        const float4 c_lo = reinterpret_cast<const float4*>(ca[j])[0];
+                        const float4 c_hi = reinterpret_cast<const float4*>(ca[j])[1];
+                        const float cs[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             const float g = flag[i] ? geom_cost(pb, j + 1, hp[i], px, py) : 5.0f;
-                            a[i] += w * (ca[i][j] + 0.1f * g);
+                            a[i] += w * (cs[i] + 0.1f * g);
                         }
                     }
                 }
@@ -1452,9 +1460,12 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 for (int j = 0; j < nsrc; ++j) {
                     if ((tsel >> j) & 1u) {
                         const float w = draws_of(vw, j);
+                        const float4 c_lo = reinterpret_cast<const float4*>(ca[j])[0];
+                        const float4 c_hi = reinterpret_cast<const float4*>(ca[j])[1];
+                        const float cs[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
-                            a[i] += w * ca[i][j];
+                            a[i] += w * cs[i];
                         }
                     }
                 }

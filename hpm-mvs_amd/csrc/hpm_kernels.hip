@@ -1114,59 +1114,43 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     pos[6] = center - 6 + 5 * W;  /* left_down */
     pos[7] = center + 5;          /* right_far */
 
-    /* axis strips (HPM.cu:1251-1341): start sample, up to 4 more two apart;
-     * keeps the minimum, the maximum for right_far */
+    /* axis strips (HPM.cu:1251-1341): start sample, up to 4 more two apart,
+     * the strip ending at the first sample outside the image; keeps the
+     * first minimum, the first maximum for right_far. A strip leaves the
+     * image for good once it leaves it, so the samples inside are a prefix:
+     * all five are loaded at once (those outside read the start sample and
+     * are not compared) */
     flag[1] = py > 4;
-    if (flag[1]) {
-        float cmin = cost[pos[1]];
-        int cmp = pos[1];
-        for (int i = 1; i < 5 && py > 4 + 2 * i; ++i) {
-            const int pt = pos[1] - 2 * i * W;
-            if (cost[pt] < cmin) {
-                cmin = cost[pt];
-                cmp = pt;
-            }
-        }
-        pos[1] = cmp;
-    }
     flag[3] = py < H - 5;
-    if (flag[3]) {
-        float cmin = cost[pos[3]];
-        int cmp = pos[3];
-        for (int i = 1; i < 5 && py < H - 5 - 2 * i; ++i) {
-            const int pt = pos[3] + 2 * i * W;
-            if (cost[pt] < cmin) {
-                cmin = cost[pt];
-                cmp = pt;
-            }
-        }
-        pos[3] = cmp;
-    }
     flag[5] = px > 4;
-    if (flag[5]) {
-        float cmin = cost[pos[5]];
-        int cmp = pos[5];
-        for (int i = 1; i < 5 && px > 4 + 2 * i; ++i) {
-            const int pt = pos[5] - 2 * i;
-            if (cost[pt] < cmin) {
-                cmin = cost[pt];
-                cmp = pt;
-            }
-        }
-        pos[5] = cmp;
-    }
     flag[7] = px < W - 5;
-    if (flag[7]) {
-        float cmin = cost[pos[7]];
-        int cmp = pos[7];
-        for (int i = 1; i < 5 && px < W - 5 - 2 * i; ++i) {
-            const int pt = pos[7] + 2 * i;
-            if (cmin < cost[pt]) { /* keeps the maximum (HPM.cu:1330) */
-                cmin = cost[pt];
-                cmp = pt;
+    {
+        const int step[4] = {-2 * W, 2 * W, -2, 2};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int s = 2 * k + 1;
+            if (!flag[s]) {
+                continue;
             }
+            float cv[5];
+            bool in[5];
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                in[i] = k == 0 ? py > 4 + 2 * i : (k == 1 ? py < H - 5 - 2 * i : (k == 2 ? px > 4 + 2 * i : px < W - 5 - 2 * i));
+                cv[i] = cost[in[i] ? pos[s] + i * step[k] : pos[s]];
+            }
+            float cmin = cv[0];
+            int cmp = pos[s];
+#pragma unroll
+            for (int i = 1; i < 5; ++i) {
+                const bool better = k == 3 ? cmin < cv[i] : cv[i] < cmin; /* right_far keeps the maximum (HPM.cu:1330) */
+                if (in[i] && better) {
+                    cmin = cv[i];
+                    cmp = pos[s] + i * step[k];
+                }
+            }
+            pos[s] = cmp;
         }
-        pos[7] = cmp;
     }
     /* diagonal zig-zags (HPM.cu:1344-1471): start offset, first step axis,
      * per-axis step signs */
@@ -1187,9 +1171,17 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             if (!flag[s]) {
                 continue;
             }
-            float cmin = cost[pos[s]];
-            int cmp = pos[s];
+            /* the path moves away from the image on both axes, so once a
+             * step leaves the image every later one does: breaking there
+             * (HPM.cu:1344-1460) and skipping the outside steps (left_down,
+             * which never breaks, HPM.cu:1461-1470) select the same; the
+             * seven samples are loaded at once */
+            const float c0 = cost[pos[s]];
+            int pts[7];
+            bool in[7];
+            float cv[7];
             int qx = px + sx0[d], qy = py + sy0[d];
+#pragma unroll
             for (int i = 0; i < 7; i++) {
                 const bool move_x = ((i % 2 == 0) == (first_x[d] == 1));
                 if (move_x) {
@@ -1197,14 +1189,17 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 } else {
                     qy += dys[d];
                 }
-                if (qx >= 0 && qx < W && qy < H && qy >= 0) {
-                    const int pt = qy * W + qx;
-                    if (cost[pt] < cmin) {
-                        cmin = cost[pt];
-                        cmp = pt;
-                    }
-                } else if (d != 3) {
-                    break; /* left_down never breaks (HPM.cu:1461-1470) */
+                in[i] = qx >= 0 && qx < W && qy < H && qy >= 0;
+                pts[i] = qy * W + qx;
+                cv[i] = cost[in[i] ? pts[i] : pos[s]];
+            }
+            float cmin = c0;
+            int cmp = pos[s];
+#pragma unroll
+            for (int i = 0; i < 7; i++) {
+                if (in[i] && cv[i] < cmin) {
+                    cmin = cv[i];
+                    cmp = pts[i];
                 }
             }
             pos[s] = cmp;
@@ -1295,26 +1290,36 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 int lu = center - 5 - 6 * W - 4 * 2 * k - 4 * 2 * W * k;
                 if (px > 4 + 8 * k && py > 5 + 8 * k) {
                     bool symbol = true;
-                    float cmin = cost[lu];
-                    int cmp = lu;
+                    /* a left-up path: once outside, always outside, so the
+                     * break (HPM.cu:797-815) selects as skipping; the samples
+                     * are loaded at once */
+                    const float c0 = cost[lu];
+                    const float cprev = cost[pos[0]];
+                    int pts[7];
+                    bool in[7];
+                    float cv[7];
                     int qx = px - 5 - 8 * k, qy = py - 6 - 8 * k;
+#pragma unroll
                     for (int i = 0; i < 7; i++) {
                         if (i % 2 == 0) {
                             qx -= 2;
                         } else {
                             qy -= 2;
                         }
-                        if (qx >= 0 && qx < W && qy < H && qy >= 0) {
-                            const int pt = qy * W + qx;
-                            if (cost[pt] < cmin) {
-                                cmin = cost[pt];
-                                cmp = pt;
-                            }
-                        } else {
-                            break;
+                        in[i] = qx >= 0 && qx < W && qy < H && qy >= 0;
+                        pts[i] = qy * W + qx;
+                        cv[i] = cost[in[i] ? pts[i] : lu];
+                    }
+                    float cmin = c0;
+                    int cmp = lu;
+#pragma unroll
+                    for (int i = 0; i < 7; i++) {
+                        if (in[i] && cv[i] < cmin) {
+                            cmin = cv[i];
+                            cmp = pts[i];
                         }
                     }
-                    if (cmin > cost[pos[0]]) {
+                    if (cmin > cprev) {
                         cmp = pos[0];
                         symbol = false;
                     }

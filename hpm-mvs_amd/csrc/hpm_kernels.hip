@@ -188,10 +188,10 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
     w->fy0 = (float)(sy0 - fy);
     w->ex = (float)(H[6] * icz);
     w->ey = (float)(H[7] * icz);
-    w->gxx = (float)((H[0] - sx0 * H[6]) * icz);
-    w->gxy = (float)((H[1] - sx0 * H[7]) * icz);
-    w->gyx = (float)((H[3] - sy0 * H[6]) * icz);
-    w->gyy = (float)((H[4] - sy0 * H[7]) * icz);
+    w->gxx = (float)(fma(-sx0, H[6], H[0]) * icz);
+    w->gxy = (float)(fma(-sx0, H[7], H[1]) * icz);
+    w->gyx = (float)(fma(-sy0, H[6], H[3]) * icz);
+    w->gyy = (float)(fma(-sy0, H[7], H[4]) * icz);
     w->lox = -1.0f - (float)ix0;
     w->hix = (float)(SW - ix0);
     w->loy = -1.0f - (float)iy0;

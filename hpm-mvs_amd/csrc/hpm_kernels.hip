@@ -1728,13 +1728,43 @@ __device__ void filter_pixel(const hpm_dev_problem& pb, int px, int py)
         return;
     }
     const float* hyp = pb.hyp;
-    float f[21];
-    int n = 0;
-    f[n++] = hyp[4 * center + 3];
     const int left = center - 1, leftleft = center - 3;
     const int up = center - W, upup = center - 3 * W;
     const int down = center + W, downdown = center + 3 * W;
     const int right = center + 1, rightright = center + 3;
+    if (px > 4 && px < W - 5 && py > 4 && py < H - 5) {
+        /* away from the border every tap exists: 21 values, and when they
+         * are all positive finite depths (no NaN, no signed zeros) their
+         * median is selected by a min/max sorting network in registers,
+         * the same value as the insertion sort's middle element */
+        const int idx[21] = {center, up, upup, upup - W * 2, down, downdown, downdown + W * 2, left, leftleft,
+                             leftleft - 2, right, rightright, rightright + 2, up + 2, down + 2, up - 2, down - 2,
+                             left - W * 2, right - W * 2, left + W * 2, right + W * 2};
+        float v[21];
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 21; ++k) {
+            v[k] = hyp[4 * idx[k] + 3];
+            ok = ok && v[k] > 0.0f && v[k] < INFINITY;
+        }
+        if (ok) {
+#pragma unroll
+            for (int i = 1; i < 21; ++i) {
+#pragma unroll
+                for (int j = i; j > 0; --j) {
+                    const float lo = fminf(v[j - 1], v[j]);
+                    const float hi = fmaxf(v[j - 1], v[j]);
+                    v[j - 1] = lo;
+                    v[j] = hi;
+                }
+            }
+            pb.hyp[4 * center + 3] = v[10];
+            return;
+        }
+    }
+    float f[21];
+    int n = 0;
+    f[n++] = hyp[4 * center + 3];
 #define HPM_TAP(cond, idx)                  \
     if (cond) {                             \
         f[n++] = hyp[4 * (idx) + 3];        \

@@ -360,6 +360,12 @@ __device__ __forceinline__ void make_ref_patch(const float* img, int W, int H, i
     const float c = texel(img, W, H, px, py);
     const float two_ss = 2.0f * P.sigma_spatial * P.sigma_spatial;
     const float two_sc = 2.0f * P.sigma_color * P.sigma_color;
+    /* the 36 reads are issued together */
+    float tex[HPM_NTAP];
+#pragma unroll
+    for (int k = 0; k < HPM_NTAP; ++k) {
+        tex[k] = texel(img, W, H, px + 2 * (k / 6) - 5, py + 2 * (k % 6) - 5);
+    }
     float wsum = 0.0f, sr = 0.0f, srr = 0.0f;
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
@@ -367,7 +373,7 @@ __device__ __forceinline__ void make_ref_patch(const float* img, int W, int H, i
         for (int b = 0; b < 6; ++b) {
             const int i = 2 * a - 5, j = 2 * b - 5;
             const int k = 6 * a + b;
-            const float r = texel(img, W, H, px + i, py + j);
+            const float r = tex[k];
             const float fi = (float)i, fj = (float)j;
             const float sd = sqrtf(fi * fi + fj * fj);
             const float cd = fabsf(r - c);
@@ -1081,6 +1087,29 @@ __device__ __forceinline__ void packed_views(int nsrc, const RefPatch& rp, int p
     }
 }
 
+/* one source view's eight direction costs */
+struct CostRow {
+    float4 lo, hi;
+};
+
+/* f(i, load(i)) for i = 0 .. n - 1 in order (n >= 1): the loads of rows
+ * i + 1 and i + 2 are in flight while row i is used, in two buffers, so no
+ * copy between trips waits on a load. load(i) must be valid for i < n. */
+template <typename Load, typename F>
+__device__ __forceinline__ void pipelined(int n, Load&& load, F&& f)
+{
+    auto a = load(0);
+#pragma unroll 1
+    for (int i = 0; i < n; i += 2) {
+        const auto b = load(min(i + 1, n - 1));
+        f(i, a);
+        if (i + 1 < n) {
+            a = load(min(i + 2, n - 1));
+            f(i + 1, b);
+        }
+    }
+}
+
 /* Every lane of the wave calls this (the refinement packs evaluations of
  * all lanes into the wave's trips); lanes without a pixel (valid false)
  * only help. */
@@ -1216,6 +1245,13 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     }
     ca[0][0] = 2.0f;
     const uint32_t all_views = nsrc >= 32 ? 0xffffffffu : ((1u << nsrc) - 1u);
+    /* a view's eight costs */
+    auto cost_row = [&](int v) {
+        CostRow r;
+        r.lo = reinterpret_cast<const float4*>(ca[v])[0];
+        r.hi = reinterpret_cast<const float4*>(ca[v])[1];
+        return r;
+    };
 
     /* state handed between the phases */
     DrawCounts vw = {0ull, 0ull}; /* draws per source view */
@@ -1276,14 +1312,12 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             /* JudgeExtend (HPM.cu:1036-1059) */
             const float good = pb.good[iter][k];
             int good_sum = 0, bad_sum = 0;
-            for (int i = 0; i < nsrc; ++i) {
-                if (ca[i][0] < good) {
-                    ++good_sum;
-                }
-                if (ca[i][0] > 1.2f) { /* float bad_threshold (HPM.cu:1042) */
-                    ++bad_sum;
-                }
-            }
+            pipelined(
+                nsrc, [&](int v) { return ca[v][0]; },
+                [&](int, float c) {
+                    good_sum += c < good ? 1 : 0;
+                    bad_sum += c > 1.2f ? 1 : 0; /* float bad_threshold (HPM.cu:1042) */
+                });
             if (good_sum >= 1 && bad_sum <= 2) {
                 ext_on = false;
             } else {
@@ -1362,7 +1396,11 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 nbm[k] = nbf[k] ? pb.sel[nb[k]] : 0u;
             }
             const float thr = pb.thr[iter];
-            for (int i = 0; i < nsrc; i++) {
+            const float fallback = pb.fallback[iter];
+            /* one view's probability from its eight costs (c in [0, 2]:
+             * c^2 is 0 or at least 2^-48, so div_rn is the IEEE quotient;
+             * every term is computed and the ones below thr summed in order) */
+            auto view_prob = [&](int i, float4 c_lo, float4 c_hi) {
                 float vprior = 0.0f;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
@@ -1373,28 +1411,25 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 float count = 0.0f;
                 int count_false = 0;
                 float tmpw = 0.0f;
-                const float4 c_lo = reinterpret_cast<const float4*>(ca[i])[0];
-                const float4 c_hi = reinterpret_cast<const float4*>(ca[i])[1];
                 const float cs[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
                     const float c = cs[j];
-                    if (c < thr) {
-                        tmpw += hpm_expf(c * c / (-0.18f));
-                        count++;
-                    }
-                    if (c > 1.2f) {
-                        count_false++;
-                    }
+                    const float e = hpm_expf(div_rn(c * c, -0.18f));
+                    const bool in = c < thr;
+                    tmpw = in ? tmpw + e : tmpw;
+                    count = in ? count + 1.0f : count;
+                    count_false += c > 1.2f ? 1 : 0;
                 }
                 float pr = 0.0f;
                 if (count > 2 && count_false < 3) {
-                    pr = tmpw / count;
+                    pr = div_rn(tmpw, count);
                 } else if (count_false < 3) {
-                    pr = pb.fallback[iter];
+                    pr = fallback;
                 }
                 probs[i] = pr * vprior;
-            }
+            };
+            pipelined(nsrc, cost_row, [&](int i, CostRow r) { view_prob(i, r.lo, r.hi); });
         }
         {
             /* TransformPDFToCDF (HPM.cu:86-100) and the 15 draws: every draw
@@ -1448,32 +1483,28 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 for (int i = 0; i < 8; ++i) {
                     hp[i] = flag[i] ? ld4(pb.hyp, pos[i]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
-                for (int j = 0; j < nsrc; ++j) {
+                pipelined(nsrc, cost_row, [&](int j, CostRow r) {
                     if ((tsel >> j) & 1u) {
                         const float w = draws_of(vw, j);
-                        const float4 c_lo = reinterpret_cast<const float4*>(ca[j])[0];
-                        const float4 c_hi = reinterpret_cast<const float4*>(ca[j])[1];
-                        const float cs[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
+                        const float cs[8] = {r.lo.x, r.lo.y, r.lo.z, r.lo.w, r.hi.x, r.hi.y, r.hi.z, r.hi.w};
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             const float g = flag[i] ? geom_cost(pb, j + 1, hp[i], px, py) : 5.0f;
                             a[i] += w * (cs[i] + 0.1f * g);
                         }
                     }
-                }
+                });
             } else {
-                for (int j = 0; j < nsrc; ++j) {
+                pipelined(nsrc, cost_row, [&](int j, CostRow r) {
                     if ((tsel >> j) & 1u) {
                         const float w = draws_of(vw, j);
-                        const float4 c_lo = reinterpret_cast<const float4*>(ca[j])[0];
-                        const float4 c_hi = reinterpret_cast<const float4*>(ca[j])[1];
-                        const float cs[8] = {c_lo.x, c_lo.y, c_lo.z, c_lo.w, c_hi.x, c_hi.y, c_hi.z, c_hi.w};
+                        const float cs[8] = {r.lo.x, r.lo.y, r.lo.z, r.lo.w, r.hi.x, r.hi.y, r.hi.z, r.hi.w};
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             a[i] += w * cs[i];
                         }
                     }
-                }
+                });
             }
 #pragma unroll
             for (int i = 0; i < 8; ++i) {

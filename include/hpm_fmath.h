@@ -45,19 +45,15 @@ static inline HPM_HD_F float hpm_qnan(void)
     return hpm_bits_to_float(0x7fc00000u);
 }
 
+/* Branch-free: the core runs on x clamped to [-103.97, 88.72] (x itself
+ * inside), and NaN, overflow and underflow are selected afterwards, so the
+ * GPU schedules calls side by side instead of behind per-call branches. */
 static inline HPM_HD_F float hpm_expf(float x)
 {
-    if (!(x == x)) {
-        return x;
-    }
-    if (x > 88.72283935546875f) {
-        return hpm_bits_to_float(0x7f800000u);
-    }
-    if (x < -103.972084045410156f) {
-        return 0.0f;
-    }
-    const float n = rintf(x * 1.44269502162933349609f);
-    float r = fmaf(n, -0.693145751953125f, x);
+    const float lo = -103.972084045410156f, hi = 88.72283935546875f;
+    const float xc = fminf(fmaxf(x, lo), hi);
+    const float n = rintf(xc * 1.44269502162933349609f);
+    float r = fmaf(n, -0.693145751953125f, xc);
     r = fmaf(n, -1.428606765330187045e-06f, r);
     float p = 1.98412698412698413e-04f;
     p = fmaf(p, r, 1.38888888888888889e-03f);
@@ -72,7 +68,10 @@ static inline HPM_HD_F float hpm_expf(float x)
     const int n2 = ni - n1;
     const float s1 = hpm_bits_to_float((uint32_t)(n1 + 127) << 23);
     const float s2 = hpm_bits_to_float((uint32_t)(n2 + 127) << 23);
-    return (p * s1) * s2;
+    float y = (p * s1) * s2;
+    y = x > hi ? hpm_bits_to_float(0x7f800000u) : y;
+    y = x < lo ? 0.0f : y;
+    return x == x ? y : x;
 }
 
 static inline HPM_HD_F float hpm_sin_poly(float r)

@@ -176,6 +176,11 @@ int save_state(hpm_batch* b, hpm_handle* const* hs, int n, bool restore)
             BHIP(b, hipMemcpyAsync(hs[i]->d_hyp, s, np * 16, hipMemcpyDeviceToDevice, b->stream));
             BHIP(b, hipMemcpyAsync(hs[i]->d_cost, s + 4 * np, np * 4, hipMemcpyDeviceToDevice, b->stream));
             BHIP(b, hipMemcpyAsync(hs[i]->d_sel, s + 5 * np, np * 4, hipMemcpyDeviceToDevice, b->stream));
+            /* the restored hypotheses are not the ones the cost cache holds */
+            if (hs[i]->d_vcost) {
+                BHIP(b, (hipError_t)hpm_launch_fill_u32((uint32_t*)hs[i]->d_vcost, (int64_t)np, 0xffffffffu,
+                                                        b->stream));
+            }
         } else {
             BHIP(b, hipMemcpyAsync(s, hs[i]->d_hyp, np * 16, hipMemcpyDeviceToDevice, b->stream));
             BHIP(b, hipMemcpyAsync(s + 4 * np, hs[i]->d_cost, np * 4, hipMemcpyDeviceToDevice, b->stream));

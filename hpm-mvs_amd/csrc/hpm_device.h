@@ -65,6 +65,10 @@ typedef struct hpm_dev_problem {
     float* hyp;                /* float4 per pixel */
     float* cost;
     uint32_t* sel;
+    /* photometric cost of each pixel's current hypothesis on every source
+     * view, [view][pixel]; NaN in view 0's entry when not known (may be
+     * null: no cache) */
+    float* vcost;
     const float* prior_planes; /* float4 per pixel */
     const uint32_t* prior_mask;
     unsigned long long* n_eval; /* NCC evaluation counter (may be null) */
@@ -112,6 +116,7 @@ int hpm_launch_jbu(const float* guide, int w, int h, const float* src_nd, int sw
 
 /* Copies n finished hypotheses (float4) and costs into a pool slot and the
  * depth (.w) into a depth map. */
+int hpm_launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, void* stream);
 int hpm_launch_store_state(const float* hyp, const float* cost, int64_t n, float* out_hyp, float* out_cost,
                            float* out_depth, void* stream);
 /* JBU of a low-resolution prior followed by the reload into camera-frame

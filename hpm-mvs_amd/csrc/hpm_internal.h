@@ -77,6 +77,8 @@ struct hpm_handle {
     float* d_hyp = nullptr;
     float* d_cost = nullptr;
     uint32_t* d_sel = nullptr;
+    float* d_vcost = nullptr;     /* cost cache, nsrc x pixels (hpm_dev_problem::vcost) */
+    size_t vcost_cap = 0;         /* floats */
     float* d_prior = nullptr;
     uint32_t* d_mask = nullptr;
     unsigned long long* d_neval = nullptr;
@@ -123,6 +125,9 @@ void hpm_make_tiles(const hpm_dev_problem* pbs, int n, std::vector<hpm_tile>* fu
  * call index), and the readiness check of RunPatchMatch. */
 void hpm_fill_problem(const hpm_handle* h, hpm_dev_problem* pb);
 int hpm_ensure_ready(hpm_handle* h);
+/* Marks the handle's cost cache unknown for every pixel: required after
+ * anything but the init and propagation kernels writes d_hyp. */
+int hpm_invalidate_cost_cache(hpm_handle* h);
 /* Allocates the prior planes / masks if needed and enables the planar prior
  * as if a prior had been uploaded (the caller fills d_prior / d_mask). */
 int hpm_prior_on_device(hpm_handle* h);

@@ -699,9 +699,12 @@ __device__ void cost_vector(const hpm_dev_problem& pb, const RefPatch& rp, int p
 }
 
 /* ComputeMultiViewInitialCostandSelectedViews (HPM.cu:399-436) */
+/* The initial cost and view mask of a hypothesis; with vc (the pixel's
+ * cost-cache entry, views pb.W * pb.H apart) its per-view costs are kept. */
 __device__ float initial_cost(const hpm_dev_problem& pb, const RefPatch& rp, int px, int py, float4 plane,
-                              uint32_t* sel, uint32_t* nev)
+                              uint32_t* sel, uint32_t* nev, float* vc = nullptr)
 {
+    const size_t np = (size_t)pb.W * pb.H;
     float cv[HPM_MAX_SOURCES], cvc[HPM_MAX_SOURCES];
     int count = 0, valid = 0;
     double q[3];
@@ -709,6 +712,9 @@ __device__ float initial_cost(const hpm_dev_problem& pb, const RefPatch& rp, int
     for (int v = 1; v <= pb.nsrc; ++v) {
         const float c = ncc(pb, rp, px, py, v, q);
         ++*nev;
+        if (vc) {
+            vc[(size_t)(v - 1) * np] = c;
+        }
         cv[v - 1] = c;
         cvc[v - 1] = c;
         ++count;
@@ -874,7 +880,7 @@ __device__ uint32_t init_pixel(const hpm_dev_problem& pb, int px, int py)
     }
     st4(pb.hyp, center, plane);
     uint32_t m;
-    pb.cost[center] = initial_cost(pb, rp, px, py, plane, &m, &nev);
+    pb.cost[center] = initial_cost(pb, rp, px, py, plane, &m, &nev, pb.vcost ? pb.vcost + center : nullptr);
     pb.sel[center] = m;
     return nev;
 }
@@ -1261,6 +1267,16 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     float fc[8];
     int min_idx = 0;
     const float4 cur = ld4(pb.hyp, center);
+    /* the cost cache: when view 0's entry is not NaN, every view's entry is
+     * the photometric cost of cur on that view (computed when cur was set,
+     * by the same function), and the rescore takes it from there */
+    const size_t np = (size_t)W * H;
+    float* const vc = pb.vcost ? pb.vcost + center : nullptr;
+    const bool known = valid && vc != nullptr && vc[0] == vc[0];
+    /* where the stored hypothesis comes from: -1 cur, 0-7 that neighbour
+     * direction (its costs on every view are in ca), 8 a refinement
+     * candidate */
+    int plane_src = -1;
     float cost_now = 0.0f;
     float cost_center = 0.0f;
     float4 plane_center = cur;
@@ -1532,8 +1548,28 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
      * order */
     float rescore_sum = 0.0f;
     packed_views<1>(
-        nsrc, rp, px, py, sel_views, [&](int) { return cur; }, selected_term,
+        nsrc, rp, px, py, known ? 0u : sel_views, [&](int) { return cur; }, selected_term,
         [&](int v, int, float val) { rescore_sum += draws_of(vw, v) * val; });
+    if (known) {
+        /* the same terms from the cache, in the same view order; eight
+         * views' entries are read at once */
+#pragma unroll 1
+        for (int v0 = 0; v0 < nsrc; v0 += 8) {
+            float c[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                c[k] = vc[(size_t)min(v0 + k, nsrc - 1) * np];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int v = v0 + k;
+                if (v < nsrc && ((sel_views >> v) & 1u)) {
+                    const float term = P.geom_consistency ? c[k] + gc_weight * geom_cost(pb, v + 1, cur, px, py) : c[k];
+                    rescore_sum += draws_of(vw, v) * term;
+                }
+            }
+        }
+    }
     nev += (uint32_t)__popc(sel_views);
 
     {
@@ -1569,6 +1605,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                     if (db >= P.depth_min && db <= P.depth_max && rfc[max_idx] > rcn) {
                         depth_now = db;
                         plane_center = hp;
+                        plane_src = max_idx;
                         cost_center = fc[max_idx];
                         restricted_cost = rfc[max_idx];
                         sel_center = tsel;
@@ -1580,6 +1617,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
                     depth_now = db;
                     plane_center = hp;
+                    plane_src = min_idx;
                     cost_center = fc[min_idx];
                 }
             }
@@ -1590,12 +1628,14 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
                 depth_now = db;
                 plane_center = hp;
+                plane_src = min_idx;
                 cost_center = fc[min_idx];
                 sel_center = tsel;
             }
             if (!P.acmm && cost_center != cost_center) {
                 depth_now = db;
                 plane_center = hp;
+                plane_src = min_idx;
                 cost_now = fc[min_idx];
                 sel_center = tsel;
             }
@@ -1724,6 +1764,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             if (db >= P.depth_min && db <= P.depth_max && rtc > restricted_cost) {
                 depth_now = db;
                 plane_center = t;
+                plane_src = 8;
                 cost_center = tc;
                 restricted_cost = rtc;
             }
@@ -1731,6 +1772,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             if (db >= P.depth_min && db <= P.depth_max && tc < cost_center) {
                 depth_now = db;
                 plane_center = t;
+                plane_src = 8;
                 cost_center = tc;
             }
         }
@@ -1745,6 +1787,25 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             st4(pb.hyp, center, plane_center);
             pb.cost[center] = cost_center;
             pb.sel[center] = sel_center;
+            if (vc != nullptr && plane_src >= 8) {
+                vc[0] = __uint_as_float(0xffffffffu); /* a candidate's costs are not kept */
+            } else if (vc != nullptr && plane_src >= 0) {
+                /* the adopted direction's costs, eight views at a time */
+#pragma unroll 1
+                for (int v0 = 0; v0 < nsrc; v0 += 8) {
+                    float c[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        c[k] = ca[min(v0 + k, nsrc - 1)][plane_src];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        if (v0 + k < nsrc) {
+                            vc[(size_t)(v0 + k) * np] = c[k];
+                        }
+                    }
+                }
+            }
         }
     }
     return nev;

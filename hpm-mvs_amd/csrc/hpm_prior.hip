@@ -702,6 +702,16 @@ __global__ __launch_bounds__(256) void hpm_k_knn_cells(const int32_t* pts, const
     }
 }
 
+extern "C" int hpm_launch_fill_u32(uint32_t* p, int64_t n, uint32_t value, void* stream)
+{
+    if (n <= 0) {
+        return 0;
+    }
+    hipLaunchKernelGGL(hpm_k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, p, n,
+                       value);
+    return (int)hipGetLastError();
+}
+
 extern "C" int hpm_launch_prior_maps(const float* hyp4, const float* cost, int W, int H, float* depth_s,
                                      float* cost_s, int ws, int hs, void* stream)
 {

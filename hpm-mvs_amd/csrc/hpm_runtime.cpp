@@ -144,6 +144,8 @@ void release_problem(hpm_handle* h)
     dfree(h->d_hyp);
     dfree(h->d_cost);
     dfree(h->d_sel);
+    dfree(h->d_vcost);
+    h->vcost_cap = 0;
     dfree(h->d_prior);
     dfree(h->d_mask);
     dfree(h->d_neval);
@@ -202,6 +204,7 @@ void fill_problem(const hpm_handle* h, hpm_dev_problem* pb)
     pb->hyp = h->d_hyp;
     pb->cost = h->d_cost;
     pb->sel = h->d_sel;
+    pb->vcost = h->d_vcost;
     pb->prior_planes = h->d_prior;
     pb->prior_mask = h->d_mask;
     pb->n_eval = h->d_neval;
@@ -765,8 +768,18 @@ extern "C" int hpm_device_init(hpm_handle* h, const char* dense_folder)
         HPM_HIP(h, hipMalloc((void**)&h->d_views, HPM_MAX_IMAGES * sizeof(hpm_dev_view)));
         h->alloc_pixels = np;
     }
+    /* the cost cache: one slot per source view and pixel (at least one
+     * view, whose slot carries the unknown mark) */
+    const size_t vc_need = np * (size_t)std::max(n - 1, 1);
+    if (h->vcost_cap < vc_need) {
+        dfree(h->d_vcost);
+        h->vcost_cap = 0;
+        HPM_HIP(h, hipMalloc((void**)&h->d_vcost, vc_need * 4));
+        h->vcost_cap = vc_need;
+    }
     HPM_HIP(h, hipMemcpyAsync(h->d_views, views.data(), views.size() * sizeof(hpm_dev_view), hipMemcpyHostToDevice,
                               h->stream));
+    HPM_HIP(h, (hipError_t)hpm_launch_fill_u32((uint32_t*)h->d_vcost, (int64_t)np, 0xffffffffu, h->stream));
     HPM_HIP(h, hipMemsetAsync(h->d_hyp, 0, np * 16, h->stream));
     HPM_HIP(h, hipMemsetAsync(h->d_cost, 0, np * 4, h->stream));
     HPM_HIP(h, hipMemsetAsync(h->d_sel, 0, np * 4, h->stream));
@@ -839,7 +852,7 @@ extern "C" int hpm_device_init_state(hpm_handle* h, const float* hyp_world, cons
     HPM_HIP(h, hipMemcpyAsync(h->d_hyp, h->h_hyp.data(), np * 16, hipMemcpyHostToDevice, h->stream));
     HPM_HIP(h, hipMemcpyAsync(h->d_cost, h->h_cost.data(), np * 4, hipMemcpyHostToDevice, h->stream));
     HPM_HIP(h, hipStreamSynchronize(h->stream));
-    return HPM_OK;
+    return hpm_invalidate_cost_cache(h);
 }
 
 int hpm_run_internal(hpm_handle* h, bool want_readback)
@@ -924,6 +937,17 @@ int hpm_load_from_pool(hpm_handle* h)
     const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
     HPM_HIP(h, hipMemcpyAsync(h->d_hyp, p->d_state_hyp + 4 * off, np * 16, hipMemcpyDeviceToDevice, h->stream));
     HPM_HIP(h, hipMemcpyAsync(h->d_cost, p->d_state_cost + off, np * 4, hipMemcpyDeviceToDevice, h->stream));
+    HPM_HIP(h, hipStreamSynchronize(h->stream));
+    return hpm_invalidate_cost_cache(h);
+}
+
+int hpm_invalidate_cost_cache(hpm_handle* h)
+{
+    if (!h->d_vcost) {
+        return HPM_OK;
+    }
+    const size_t np = (size_t)h->cams[0].width * h->cams[0].height;
+    HPM_HIP(h, (hipError_t)hpm_launch_fill_u32((uint32_t*)h->d_vcost, (int64_t)np, 0xffffffffu, h->stream));
     HPM_HIP(h, hipStreamSynchronize(h->stream));
     return HPM_OK;
 }
@@ -1293,7 +1317,7 @@ extern "C" int hpm_debug_write_state(hpm_handle* h, const float* hyp4, const flo
     if (cost) HPM_HIP(h, hipMemcpyAsync(h->d_cost, cost, np * 4, hipMemcpyHostToDevice, h->stream));
     if (selmask) HPM_HIP(h, hipMemcpyAsync(h->d_sel, selmask, np * 4, hipMemcpyHostToDevice, h->stream));
     HPM_HIP(h, hipStreamSynchronize(h->stream));
-    return HPM_OK;
+    return hyp4 ? hpm_invalidate_cost_cache(h) : HPM_OK;
 }
 
 extern "C" int hpm_device_selftest(hpm_handle* h, int kind, const void* in, void* out, int n)

@@ -212,6 +212,38 @@ def test_stages_bitwise(hpm, cpu, c1_small):
     pr.close()
 
 
+def test_state_write_over_initialised_problem(hpm, cpu, c1_small):
+    """The device keeps each pixel's per-view costs of its hypothesis for the
+    rescore; a state written over an initialised problem (other planes,
+    costs and views) must propagate as the oracle does from that state, over
+    several half-iterations without a further write."""
+    cfg, (cams, images, gd, gn, low, order) = c1_small
+    say("case: state write over an initialised problem, 160x120, 4 sources")
+    pr = make_problem(hpm, cams, images)
+    pr.debug_stage(0)
+    dev_hyp, _, _ = pr.debug_state()
+    o = cpu.Oracle(cams, images, pr.params(), salt=0)
+    other = cpu.Oracle(cams, images, pr.params(), salt=1)
+    other.init()
+    o.hyp[...] = other.hyp
+    o.cost[...] = other.cost
+    o.sel[...] = other.sel
+    moved = np.any(o.hyp.view(np.uint32) != dev_hyp.view(np.uint32), axis=2).mean()
+    say(f"written planes differ from the device's at {moved:.4f} of the pixels")
+    assert moved > 0.9
+    hpm.lib().hpm_debug_write_state(pr._h, hpm.fptr(o.hyp), hpm.fptr(o.cost), hpm.uptr(o.sel))
+    for it, colour in ((0, 0), (0, 1), (1, 0)):
+        o.half_iteration(it, colour)
+        pr.debug_stage(1, it, colour)
+        hyp, cost, sel = pr.debug_state()
+        same = (np.all(hyp.view(np.uint32) == o.hyp.view(np.uint32), axis=2)
+                & ((cost.view(np.uint32) == o.cost.view(np.uint32)) | (np.isnan(cost) & np.isnan(o.cost)))
+                & (sel == o.sel))
+        say(f"iter {it} colour {colour}: identical pixels {same.mean():.6f}")
+        assert same.all()
+    pr.close()
+
+
 def depth_normal_agreement(a_hyp, b_hyp):
     da, db = a_hyp[..., 3], b_hyp[..., 3]
     rel = np.abs(da - db) / np.maximum(np.abs(db), 1e-12)

@@ -1093,6 +1093,33 @@ __device__ __forceinline__ void packed_views(int nsrc, const RefPatch& rp, int p
     }
 }
 
+/* a[i] for an index known only at run time, by selects: the array is
+ * never indexed dynamically, so it stays in registers. The first element
+ * passes through an empty asm statement: a select between two loads would
+ * otherwise be folded into one load from a selected address, which indexes
+ * the array after all. */
+__device__ __forceinline__ int pick8(const int (&a)[8], int i)
+{
+    int r = a[0];
+    asm volatile("" : "+v"(r));
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+        r = i == k ? a[k] : r;
+    }
+    return r;
+}
+
+/* f[i] for an index known only at run time, from the flags' bit mask */
+__device__ __forceinline__ bool pick8(const bool (&f)[8], int i)
+{
+    uint32_t m = 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        m |= f[k] ? 1u << k : 0u;
+    }
+    return ((m >> i) & 1u) != 0u;
+}
+
 /* one source view's eight direction costs */
 struct CostRow {
     float4 lo, hi;
@@ -1298,11 +1325,17 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     }
     double q[3];
     /* the eight neighbour hypotheses on every view: the same work for every
-     * lane away from the image border, so the wave runs it in lockstep */
+     * lane away from the image border, so the wave runs it in lockstep; the
+     * next direction's hypothesis is read while this one is scored */
+    float4 next_plane = ld4(pb.hyp, flag[0] ? pos[0] : center);
 #pragma unroll 1
     for (int d = 0; d < 8; ++d) {
-        if (flag[d]) {
-            plane_q(pb, ld4(pb.hyp, pos[d]), q);
+        const float4 plane_d = next_plane;
+        if (d + 1 < 8) {
+            next_plane = ld4(pb.hyp, pick8(flag, d + 1) ? pick8(pos, d + 1) : center);
+        }
+        if (pick8(flag, d)) {
+            plane_q(pb, plane_d, q);
 #pragma unroll 1
             for (int v = 0; v < nsrc; ++v) {
                 ca[v][d] = ncc(pb, rp, px, py, v + 1, q);
@@ -1599,8 +1632,8 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 const float dn = depth_from_plane(rc, cur, px, py);
                 const float pr = prior_term(pk, dn - depth_prior, prior_plane, cur);
                 const float rcn = hpm_expf(-cost_now * cost_now / pk.beta) * pr;
-                if (flag[max_idx]) {
-                    const float4 hp = ld4(pb.hyp, pos[max_idx]);
+                if (pick8(flag, max_idx)) {
+                    const float4 hp = ld4(pb.hyp, pick8(pos, max_idx));
                     const float db = depth_from_plane(rc, hp, px, py);
                     if (db >= P.depth_min && db <= P.depth_max && rfc[max_idx] > rcn) {
                         depth_now = db;
@@ -1611,8 +1644,8 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                         sel_center = tsel;
                     }
                 }
-            } else if (flag[min_idx]) {
-                const float4 hp = ld4(pb.hyp, pos[min_idx]);
+            } else if (pick8(flag, min_idx)) {
+                const float4 hp = ld4(pb.hyp, pick8(pos, min_idx));
                 const float db = depth_from_plane(rc, hp, px, py);
                 if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
                     depth_now = db;
@@ -1622,8 +1655,8 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 }
             }
         }
-        if (!prior_on && flag[min_idx]) {
-            const float4 hp = ld4(pb.hyp, pos[min_idx]);
+        if (!prior_on && pick8(flag, min_idx)) {
+            const float4 hp = ld4(pb.hyp, pick8(pos, min_idx));
             const float db = depth_from_plane(rc, hp, px, py);
             if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
                 depth_now = db;

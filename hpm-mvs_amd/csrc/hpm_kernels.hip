@@ -653,9 +653,13 @@ __device__ float geom_cost(const hpm_dev_problem& pb, int v, float4 plane, int p
     return fminf(max_cost, sqrtf(dx * dx + dy * dy));
 }
 
+/* the empty asm keeps the first element a value: a select between two
+ * loads of the array would be folded into one load from a selected
+ * address, and the array (indexed at run time) would stay in memory */
 __device__ __forceinline__ int find_min_idx8(const float* c)
 {
     float m = c[0];
+    asm volatile("" : "+v"(m));
     int idx = 0;
 #pragma unroll
     for (int i = 1; i < 8; ++i) {
@@ -670,6 +674,7 @@ __device__ __forceinline__ int find_min_idx8(const float* c)
 __device__ __forceinline__ int find_max_idx8(const float* c)
 {
     float m = c[0];
+    asm volatile("" : "+v"(m));
     int idx = 0;
 #pragma unroll
     for (int i = 1; i < 8; ++i) {
@@ -1098,9 +1103,10 @@ __device__ __forceinline__ void packed_views(int nsrc, const RefPatch& rp, int p
  * passes through an empty asm statement: a select between two loads would
  * otherwise be folded into one load from a selected address, which indexes
  * the array after all. */
-__device__ __forceinline__ int pick8(const int (&a)[8], int i)
+template <typename T>
+__device__ __forceinline__ T pick8(const T (&a)[8], int i)
 {
-    int r = a[0];
+    T r = a[0];
     asm volatile("" : "+v"(r));
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
@@ -1433,6 +1439,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     {
         /* joint view selection (HPM.cu:1548-1613) */
         float probs[HPM_MAX_SOURCES];
+        float psum = 0.0f; /* their sum in view order */
         {
             /* view priors from the four axis neighbours' selections, summed
              * in neighbour order per view */
@@ -1477,6 +1484,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                     pr = fallback;
                 }
                 probs[i] = pr * vprior;
+                psum += probs[i];
             };
             pipelined(nsrc, cost_row, [&](int i, CostRow r) { view_prob(i, r.lo, r.hi); });
         }
@@ -1484,11 +1492,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             /* TransformPDFToCDF (HPM.cu:86-100) and the 15 draws: every draw
              * goes to the first view whose CDF value exceeds it; the draws
              * are made first and assigned in one pass over the views */
-            float sum = 0.0f;
-            for (int i = 0; i < nsrc; ++i) {
-                sum += probs[i];
-            }
-            const float inv = 1.0f / sum;
+            const float inv = 1.0f / psum;
             float u[15];
 #pragma unroll
             for (int k = 0; k < 15; ++k) {
@@ -1496,8 +1500,8 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             }
             uint32_t open = 0x7fffu; /* draws not yet assigned */
             float cum = 0.0f;
-            for (int i = 0; i < nsrc; ++i) {
-                cum += probs[i] * inv;
+            pipelined(nsrc, [&](int i) { return probs[i]; }, [&](int i, float p) {
+                cum += p * inv;
                 float n = 0.0f;
 #pragma unroll
                 for (int k = 0; k < 15; ++k) {
@@ -1507,7 +1511,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                     }
                 }
                 set_draws(&vw, i, n);
-            }
+            });
         }
         for (int i = 0; i < nsrc; ++i) {
             if (draws_of(vw, i) > 0) {
@@ -1635,41 +1639,41 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 if (pick8(flag, max_idx)) {
                     const float4 hp = ld4(pb.hyp, pick8(pos, max_idx));
                     const float db = depth_from_plane(rc, hp, px, py);
-                    if (db >= P.depth_min && db <= P.depth_max && rfc[max_idx] > rcn) {
+                    if (db >= P.depth_min && db <= P.depth_max && pick8(rfc, max_idx) > rcn) {
                         depth_now = db;
                         plane_center = hp;
                         plane_src = max_idx;
-                        cost_center = fc[max_idx];
-                        restricted_cost = rfc[max_idx];
+                        cost_center = pick8(fc, max_idx);
+                        restricted_cost = pick8(rfc, max_idx);
                         sel_center = tsel;
                     }
                 }
             } else if (pick8(flag, min_idx)) {
                 const float4 hp = ld4(pb.hyp, pick8(pos, min_idx));
                 const float db = depth_from_plane(rc, hp, px, py);
-                if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
+                if (db >= P.depth_min && db <= P.depth_max && pick8(fc, min_idx) < cost_now) {
                     depth_now = db;
                     plane_center = hp;
                     plane_src = min_idx;
-                    cost_center = fc[min_idx];
+                    cost_center = pick8(fc, min_idx);
                 }
             }
         }
         if (!prior_on && pick8(flag, min_idx)) {
             const float4 hp = ld4(pb.hyp, pick8(pos, min_idx));
             const float db = depth_from_plane(rc, hp, px, py);
-            if (db >= P.depth_min && db <= P.depth_max && fc[min_idx] < cost_now) {
+            if (db >= P.depth_min && db <= P.depth_max && pick8(fc, min_idx) < cost_now) {
                 depth_now = db;
                 plane_center = hp;
                 plane_src = min_idx;
-                cost_center = fc[min_idx];
+                cost_center = pick8(fc, min_idx);
                 sel_center = tsel;
             }
             if (!P.acmm && cost_center != cost_center) {
                 depth_now = db;
                 plane_center = hp;
                 plane_src = min_idx;
-                cost_now = fc[min_idx];
+                cost_now = pick8(fc, min_idx);
                 sel_center = tsel;
             }
         }

@@ -152,11 +152,9 @@ struct Warp {
     float fx0, fy0;
     float ex, ey, gxx, gxy, gyx, gyy;
     float lox, hix, loy, hiy; /* clamp of the offsets: source coordinate in [-1, W] x [-1, H] */
-    float cub;            /* 1 for the cubic reciprocal, 0 for the quadratic (tiny) */
     bool small;           /* |e . d| <= 0.01 over the window: cubic reciprocal */
-    bool tiny;            /* |e . d| <= 0.002 over the window: quadratic reciprocal */
+    bool tiny;            /* |e . d| <= 0.002 over the window: linear reciprocal */
     bool inner;           /* tiny, and every tap lies inside the clamp: the clamp is the identity */
-    bool lin;             /* inner, and the linear reciprocal 1 - e . d moves no tap by more than 2e-6 px */
 };
 
 /* false when the warped centre leaves the source image (HPM.cu:333-335) */
@@ -198,28 +196,22 @@ __device__ __forceinline__ bool make_warp(hpm_cview sv, const double q[3], int p
     w->hiy = (float)(SH - iy0);
     w->small = (fabsf(w->ex) + fabsf(w->ey)) * 5.0f <= 0.01f;
     w->tiny = (fabsf(w->ex) + fabsf(w->ey)) * 5.0f <= 0.002f;
-    w->cub = w->tiny ? 0.0f : 1.0f;
     /* |tap offset| <= 5 (|g.x| + |g.y|) / (1 - 0.01) for small warps; 1.0102
-     * covers the maxima of the cubic (1.010101) and the quadratic (1.002004)
+     * covers the maxima of the cubic (1.010101) and the linear form (1.002)
      * and the FP32 roundings, 1e-3 px the roundings of these bounds */
     const float bx = fmaf(5.0f * (fabsf(w->gxx) + fabsf(w->gxy)), 1.0102f, 1e-3f);
     const float by = fmaf(5.0f * (fabsf(w->gyx) + fabsf(w->gyy)), 1.0102f, 1e-3f);
     w->inner = w->tiny && w->fx0 - bx >= w->lox && w->fx0 + bx <= w->hix && w->fy0 - by >= w->loy &&
                w->fy0 + by <= w->hiy;
-    /* 1 - u for 1 / (1 + u) errs by u^2 / (1 + u): a tap offset of at most
-     * max(bx, by) px moves by at most bu^2 max(bx, by) / (1 - bu) */
-    const float bu = (fabsf(w->ex) + fabsf(w->ey)) * 5.0f;
-    w->lin = w->inner && (bu * bu) * fmaxf(bx, by) <= 2e-6f;
     return true;
 }
 
-/* Paths of the window sums, chosen per wave: every active lane has a
- * linear warp (the linear reciprocal, no clamp), an inner warp (per lane the
- * linear or the quadratic reciprocal, no clamp), a small warp (per lane the
- * linear, quadratic or cubic, with the clamp), or any warp (per lane the
- * linear, quadratic, cubic or the quotient, with the clamp). The four give
- * the same values per lane. */
-enum : int { kPathLinear = 0, kPathInner = 1, kPathSmall = 2, kPathAny = 3 };
+/* Paths of the window sums, chosen per wave: every active lane has an
+ * inner warp (the linear reciprocal, no clamp), a small warp (per lane the
+ * linear or the cubic, with the clamp), or any warp (per lane the linear,
+ * the cubic or the quotient, with the clamp). The three give the same
+ * values per lane. */
+enum : int { kPathLinear = 0, kPathSmall = 1, kPathAny = 2 };
 
 /* The dx products of one window column (x offset dx), shared by its taps. */
 struct Column {
@@ -275,14 +267,11 @@ __device__ __forceinline__ void tap_coord(const Warp& w, const Column& c, int pi
     float inv;
     if (kPath == kPathLinear) {
         inv = lin;
-    } else if (kPath == kPathInner) {
-        const float u = fmaf(w.ey, dy, c.ex);
-        inv = w.lin ? lin : fmaf(u, u - 1.0f, 1.0f);
     } else {
-        /* 1 + u (u k - 1): k = 1 the quadratic, k = 1 - u the cubic */
+        /* the cubic 1 + u (u (1 - u) - 1) */
         const float u = fmaf(w.ey, dy, c.ex);
-        const float poly = fmaf(u, fmaf(u, fmaf(-u, w.cub, 1.0f), -1.0f), 1.0f);
-        inv = w.lin ? lin : ((kPath != kPathAny || w.small) ? poly : 1.0f / (1.0f + u));
+        const float cubic = fmaf(u, fmaf(u, 1.0f - u, -1.0f), 1.0f);
+        inv = w.tiny ? lin : ((kPath != kPathAny || w.small) ? cubic : 1.0f / (1.0f + u));
     }
     float lx = fmaf(fmaf(w.gxy, dy, c.gx), inv, w.fx0);
     float ly = fmaf(fmaf(w.gyy, dy, c.gy), inv, w.fy0);
@@ -476,10 +465,8 @@ __device__ __forceinline__ float ncc(const hpm_dev_problem& pb, const RefPatch& 
     const int pitch8 = sv->pitch * 8;
     const __amdgpu_buffer_rsrc_t pool = pool_rsrc(pb.images_pad);
     float ss, sss, srs;
-    if (__all(w.lin)) {
+    if (__all(w.inner)) {
         ncc_sums<kPathLinear>(pool, pitch8, rp, w, &ss, &sss, &srs);
-    } else if (__all(w.inner)) {
-        ncc_sums<kPathInner>(pool, pitch8, rp, w, &ss, &sss, &srs);
     } else if (__all(w.small)) {
         ncc_sums<kPathSmall>(pool, pitch8, rp, w, &ss, &sss, &srs);
     } else {

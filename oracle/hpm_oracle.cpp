@@ -18,11 +18,11 @@
  *     (pixel, launch);
  *   - the homography and the warped window centre in FP64 per (hypothesis,
  *     source view), the 36 tap offsets from the centre in FP32 with one
- *     reciprocal 1 / (1 + u) per tap: the quadratic 1 - u + u^2 when the
- *     window bounds |u| by 0.002 (truncation below 8.1e-9 relative), else
- *     the cubic 1 - u + u^2 - u^3 when it bounds |u| by 0.01 (below
- *     1.02e-8), both under half an FP32 ulp, the correctly rounded quotient
- *     otherwise; bilinear fractions x - floor(x) capped at the largest float
+ *     reciprocal 1 / (1 + u) per tap: the linear 1 - u when the window
+ *     bounds |u| by 0.002 (a tap moves by at most 4.1e-6 of its offset),
+ *     else the cubic 1 - u + u^2 - u^3 when it bounds |u| by 0.01 (below
+ *     1.02e-8 relative, under half an FP32 ulp), the correctly rounded
+ *     quotient otherwise; bilinear fractions x - floor(x) capped at the largest float
  *     below 1 (they differ from the plain difference only where it rounds
  *     up to 1, for x just below an integer);
  *   - NCC window sums of both signals accumulated relative to the reference
@@ -185,30 +185,18 @@ struct Warp {
     float fx0, fy0;
     float ex, ey, gxx, gxy, gyx, gyy;
     bool small; /* |e . d| <= 0.01 over the window: cubic reciprocal */
-    bool tiny;  /* |e . d| <= 0.002 over the window: quadratic reciprocal */
-    bool lin;   /* tiny, every tap inside the clamp, and the linear reciprocal 1 - e . d moves no
-                   tap by more than 2e-6 px */
+    bool tiny;  /* |e . d| <= 0.002 over the window: linear reciprocal */
 };
 
-/* The reciprocal tiers of a warp of a SW x SH source, radius the largest
- * tap offset of the window. The linear tier: 1 - u errs from 1 / (1 + u) by
- * u^2 / (1 + u), so with |u| <= bu = radius (|ex| + |ey|) and tap offsets
- * below bx = radius (|gxx| + |gxy|) (by for y) a tap moves by at most
- * bu^2 max(bx, by) / (1 - bu); the bounds bx, by (with margins for the
- * quadratic's and cubic's maxima and FP32 rounding) also show that every tap
- * stays inside the clamp, which the tier requires. */
-static inline void set_tiers(Warp* w, int SW, int SH, float radius)
+/* The reciprocal tiers of a warp, radius the largest tap offset of the
+ * window: with |u| <= bu = radius (|ex| + |ey|), the linear 1 - u errs from
+ * 1 / (1 + u) by u^2 / (1 + u), so a tap moves by at most bu^2 / (1 - bu)
+ * of its offset (4.1e-6 at bu = 0.002). */
+static inline void set_tiers(Warp* w, float radius)
 {
     const float bu = (fabsf(w->ex) + fabsf(w->ey)) * radius;
     w->small = bu <= 0.01f;
     w->tiny = bu <= 0.002f;
-    const float bx = fmaf(radius * (fabsf(w->gxx) + fabsf(w->gxy)), 1.0102f, 1e-3f);
-    const float by = fmaf(radius * (fabsf(w->gyx) + fabsf(w->gyy)), 1.0102f, 1e-3f);
-    const float lox = -1.0f - (float)w->ix0, hix = (float)(SW - w->ix0);
-    const float loy = -1.0f - (float)w->iy0, hiy = (float)(SH - w->iy0);
-    const bool inner = w->tiny && w->fx0 - bx >= lox && w->fx0 + bx <= hix && w->fy0 - by >= loy &&
-                       w->fy0 + by <= hiy;
-    w->lin = inner && (bu * bu) * fmaxf(bx, by) <= 2e-6f;
 }
 
 /* Returns false when the warped centre falls outside the source image (the
@@ -244,23 +232,19 @@ static inline bool make_warp(const ViewRel& v, const double q[3], int px, int py
     w->gxy = (float)(fma(-sx0, H[7], H[1]) * icz);
     w->gyx = (float)(fma(-sy0, H[6], H[3]) * icz);
     w->gyy = (float)(fma(-sy0, H[7], H[4]) * icz);
-    set_tiers(w, SW, SH, radius);
+    set_tiers(w, radius);
     return true;
 }
 
-/* 1 / (1 + u) of a tap (u = ex dx + ey dy): in the linear tier
- * 1 - ex dx - ey dy as two fused multiply-adds (the column term
- * 1 - ex dx first); else the quadratic 1 - u + u^2 in Horner form when the
- * warp bounds |u| by 0.002 (truncation |u|^3 / (1 + u) <= 8.1e-9 relative),
- * the cubic 1 - u + u^2 - u^3 when it bounds |u| by 0.01 (u^4 / (1 + u) <=
- * 1.02e-8), else the correctly rounded quotient. */
+/* 1 / (1 + u) of a tap (u = ex dx + ey dy): when the warp bounds |u| by
+ * 0.002, the linear 1 - ex dx - ey dy as two fused multiply-adds (the
+ * column term 1 - ex dx first); else the cubic 1 - u + u^2 - u^3 in Horner
+ * form when it bounds |u| by 0.01 (u^4 / (1 + u) <= 1.02e-8), else the
+ * correctly rounded quotient. */
 static inline float tap_inv(float u, const Warp& w, float dx, float dy)
 {
-    if (w.lin) {
-        return fmaf(-w.ey, dy, fmaf(-w.ex, dx, 1.0f));
-    }
     if (w.tiny) {
-        return fmaf(u, u - 1.0f, 1.0f);
+        return fmaf(-w.ey, dy, fmaf(-w.ex, dx, 1.0f));
     }
     return w.small ? fmaf(u, fmaf(u, 1.0f - u, -1.0f), 1.0f) : 1.0f / (1.0f + u);
 }
@@ -578,7 +562,7 @@ static double ncc_warp32(const Ctx& cx, const RefPatch& rp, int px, int py, int 
     w.gxy = (H[1] - sx0 * H[7]) * icz;
     w.gyx = (H[3] - sy0 * H[6]) * icz;
     w.gyy = (H[4] - sy0 * H[7]) * icz;
-    set_tiers(&w, SW, SH, (float)(pb->params.patch_size / 2));
+    set_tiers(&w, (float)(pb->params.patch_size / 2));
     float ss_ = 0.0f, sss = 0.0f, srs = 0.0f;
     const int kc = rp.centre_tap;
     const float shift = sample_tap(simg, SW, SH, w, rp.dx[kc], rp.dy[kc]);

@@ -697,22 +697,78 @@ __device__ float initial_cost(const hpm_dev_problem& pb, const RefPatch& rp, int
                               uint32_t* sel, uint32_t* nev, float* vc = nullptr)
 {
     const size_t np = (size_t)pb.W * pb.H;
+    const int nsrc = pb.nsrc;
+    const int kparam = pb.params.top_k;
     float cv[HPM_MAX_SOURCES], cvc[HPM_MAX_SOURCES];
+    /* for top_k <= 8 the eight smallest costs in ascending order, kept in
+     * registers by a compare-and-swap insertion (an equal cost goes after
+     * the ones already kept, as in the insertion sort) */
+    const bool small_k = kparam <= 8;
+    float best[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        best[j] = 3.0f; /* above every cost */
+    }
     int count = 0, valid = 0;
     double q[3];
     plane_q(pb, plane, q);
-    for (int v = 1; v <= pb.nsrc; ++v) {
+    for (int v = 1; v <= nsrc; ++v) {
         const float c = ncc(pb, rp, px, py, v, q);
         ++*nev;
         if (vc) {
             vc[(size_t)(v - 1) * np] = c;
         }
-        cv[v - 1] = c;
         cvc[v - 1] = c;
         ++count;
         if (c < 2.0f) {
             ++valid;
         }
+        if (small_k) {
+            float x = c;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const bool lt = x < best[j];
+                const float keep = lt ? x : best[j];
+                x = lt ? best[j] : x;
+                best[j] = keep;
+            }
+        } else {
+            cv[v - 1] = c;
+        }
+    }
+    uint32_t m = 0u;
+    const int top_k = min(valid, kparam);
+    float result = 2.0f;
+    if (small_k) {
+        if (top_k > 0) {
+            float cost = 0.0f;
+            float thr = best[0];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i < top_k) {
+                    cost += best[i];
+                    thr = best[i];
+                }
+            }
+            /* the views at or below the k-th cost, eight costs read at once */
+#pragma unroll 1
+            for (int v0 = 0; v0 < nsrc; v0 += 8) {
+                float c[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    c[k] = cvc[min(v0 + k, nsrc - 1)];
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (v0 + k < nsrc && c[k] <= thr) {
+                        m |= (1u << (v0 + k));
+                    }
+                }
+            }
+            result = cost / (float)top_k;
+        }
+        *sel = m;
+        return result;
     }
     /* sort_small (HPM.cu:3-12) */
     for (int i = 1; i < count; i++) {
@@ -723,16 +779,13 @@ __device__ float initial_cost(const hpm_dev_problem& pb, const RefPatch& rp, int
         }
         cv[j] = tmp;
     }
-    uint32_t m = 0u;
-    const int top_k = min(valid, pb.params.top_k);
-    float result = 2.0f;
     if (top_k > 0) {
         float cost = 0.0f;
         for (int i = 0; i < top_k; ++i) {
             cost += cv[i];
         }
         const float thr = cv[top_k - 1];
-        for (int i = 0; i < pb.nsrc; ++i) {
+        for (int i = 0; i < nsrc; ++i) {
             if (cvc[i] <= thr) {
                 m |= (1u << i);
             }

@@ -280,6 +280,38 @@ def test_full_run_parity_c1(hpm, cpu, c1_full):
     pr.close()
 
 
+@pytest.mark.parametrize("top_k", [1, 8, 9, 12])
+def test_init_top_k_bitwise(hpm, cpu, top_k):
+    """The initial cost (mean of the top_k smallest view costs) and view mask
+    with 12 sources: top_k up to 8 keeps the smallest costs in registers,
+    above 8 sorts them; both give the oracle's bits, and one half-iteration
+    from that state too."""
+    from conftest import problem_inputs
+
+    w, h, nv = 64, 48, 13
+    f = 0.8 * w
+    cfg = hpm.scene_config(seed=71, width=w, height=h, fx=f, fy=f, cx=w / 2.0, cy=h / 2.0, num_views=nv)
+    cams, images, gd, gn, low, order = problem_inputs(hpm, cfg, 6)
+    say(f"case: init with top_k {top_k}: {w}x{h}, {len(cams) - 1} sources")
+    p = hpm.default_params()
+    p.top_k = top_k
+    pr = make_problem(hpm, cams, images, p)
+    o = cpu.Oracle(cams, images, pr.params(), salt=0)
+    o.init()
+    pr.debug_stage(0)
+    for tag in ("init", "iter 0 colour 0"):
+        if tag != "init":
+            o.half_iteration(0, 0)
+            pr.debug_stage(1, 0, 0)
+        hyp, cost, sel = pr.debug_state()
+        same = (np.all(hyp.view(np.uint32) == o.hyp.view(np.uint32), axis=2)
+                & ((cost.view(np.uint32) == o.cost.view(np.uint32)) | (np.isnan(cost) & np.isnan(o.cost)))
+                & (sel == o.sel))
+        say(f"{tag}: identical pixels {same.mean():.6f}, mean views selected {np.mean([bin(int(s)).count('1') for s in sel.ravel()]):.2f}")
+        assert same.all(), tag
+    pr.close()
+
+
 def _resized_view(hpm, cam, img, w, h):
     """a source view at another resolution: the image resized, K scaled"""
     import copy

@@ -1226,8 +1226,8 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
      * the strip ending at the first sample outside the image; keeps the
      * first minimum, the first maximum for right_far. A strip leaves the
      * image for good once it leaves it, so the samples inside are a prefix:
-     * all five are loaded at once (those outside read the start sample and
-     * are not compared) */
+     * all five are loaded at once (those outside read the pixel's own cost,
+     * a distinct address, and are not compared) */
     flag[1] = py > 4;
     flag[3] = py < H - 5;
     flag[5] = px > 4;
@@ -1245,7 +1245,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
                 in[i] = k == 0 ? py > 4 + 2 * i : (k == 1 ? py < H - 5 - 2 * i : (k == 2 ? px > 4 + 2 * i : px < W - 5 - 2 * i));
-                cv[i] = cost[in[i] ? pos[s] + i * step[k] : pos[s]];
+                cv[i] = cost[in[i] ? pos[s] + i * step[k] : center];
             }
             float cmin = cv[0];
             int cmp = pos[s];
@@ -1299,7 +1299,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 }
                 in[i] = qx >= 0 && qx < W && qy < H && qy >= 0;
                 pts[i] = qy * W + qx;
-                cv[i] = cost[in[i] ? pts[i] : pos[s]];
+                cv[i] = cost[in[i] ? pts[i] : center];
             }
             float cmin = c0;
             int cmp = pos[s];
@@ -1437,7 +1437,7 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                         }
                         in[i] = qx >= 0 && qx < W && qy < H && qy >= 0;
                         pts[i] = qy * W + qx;
-                        cv[i] = cost[in[i] ? pts[i] : lu];
+                        cv[i] = cost[in[i] ? pts[i] : center];
                     }
                     float cmin = c0;
                     int cmp = lu;

@@ -612,27 +612,43 @@ __device__ __forceinline__ void project(const float X[3], const hpm_camera& c, f
     *v = (c.K[3] * t[0] + c.K[4] * t[1] + c.K[5] * t[2]) / *d;
 }
 
-/* ComputeGeomConsistencyCost (HPM.cu:483-508; ACMM.cu:494-519 caps it at 3) */
-__device__ float geom_cost(const hpm_dev_problem& pb, int v, float4 plane, int px, int py)
+/* ComputeGeomConsistencyCost (HPM.cu:483-508; ACMM.cu:494-519 caps it at
+ * 3), in two steps so that several depth reads can be in flight: the
+ * projection into source view v and the read of its depth there (the
+ * clamped index stays inside the map for any plane), then the
+ * reprojection and the capped distance */
+struct GeomSample {
+    float sx, sy, src_depth;
+};
+
+__device__ __forceinline__ GeomSample geom_fetch(const hpm_dev_problem& pb, int v, float4 plane, int px, int py)
 {
-    const float max_cost = pb.params.acmm ? 3.0f : 5.0f;
     const hpm_camera rc = camera(pb, 0);
     const hpm_camera sc = camera(pb, v);
     const float depth = depth_from_plane(rc, plane, px, py);
     float fw[3];
     point_world((float)px, (float)py, depth, rc, fw);
-    float sx, sy, sd;
-    project(fw, sc, &sx, &sy, &sd);
-    const float cxf = fminf(fmaxf(sx, -1.0f), (float)sc.width);
-    const float cyf = fminf(fmaxf(sy, -1.0f), (float)sc.height);
+    GeomSample g;
+    float sd;
+    project(fw, sc, &g.sx, &g.sy, &sd);
+    const float cxf = fminf(fmaxf(g.sx, -1.0f), (float)sc.width);
+    const float cyf = fminf(fmaxf(g.sy, -1.0f), (float)sc.height);
     const int ix = clampi((int)cxf, 0, sc.width - 1);
     const int iy = clampi((int)cyf, 0, sc.height - 1);
-    const float src_depth = pb.depths[cview(pb, v)->depth_off + (int64_t)iy * sc.width + ix];
-    if (src_depth == 0.0f) {
+    g.src_depth = pb.depths[cview(pb, v)->depth_off + (int64_t)iy * sc.width + ix];
+    return g;
+}
+
+__device__ __forceinline__ float geom_finish(const hpm_dev_problem& pb, int v, GeomSample g, int px, int py)
+{
+    const float max_cost = pb.params.acmm ? 3.0f : 5.0f;
+    if (g.src_depth == 0.0f) {
         return max_cost;
     }
+    const hpm_camera rc = camera(pb, 0);
+    const hpm_camera sc = camera(pb, v);
     float bw[3];
-    point_world(sx, sy, src_depth, sc, bw);
+    point_world(g.sx, g.sy, g.src_depth, sc, bw);
     float bx, by, bd;
     project(bw, rc, &bx, &by, &bd);
     const float dx = (float)px - bx;
@@ -1580,9 +1596,14 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                     if ((tsel >> j) & 1u) {
                         const float w = draws_of(vw, j);
                         const float cs[8] = {r.lo.x, r.lo.y, r.lo.z, r.lo.w, r.hi.x, r.hi.y, r.hi.z, r.hi.w};
+                        GeomSample gs[8];
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
-                            const float g = flag[i] ? geom_cost(pb, j + 1, hp[i], px, py) : 5.0f;
+                            gs[i] = geom_fetch(pb, j + 1, hp[i], px, py);
+                        }
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            const float g = flag[i] ? geom_finish(pb, j + 1, gs[i], px, py) : 5.0f;
                             a[i] += w * (cs[i] + 0.1f * g);
                         }
                     }
@@ -1613,10 +1634,14 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
     /* the term of one hypothesis on one selected view: the NCC cost, plus
      * the weighted geometric cost in geometric-consistency mode */
     auto selected_term = [&](const RefPatch& orp, int opx, int opy, int v, float4 plane) {
+        GeomSample gs = {0.0f, 0.0f, 0.0f};
+        if (P.geom_consistency) {
+            gs = geom_fetch(pb, v + 1, plane, opx, opy); /* in flight during the NCC */
+        }
         double tq[3];
         plane_q(pb, plane, tq);
         const float cv = ncc(pb, orp, opx, opy, v + 1, tq);
-        return P.geom_consistency ? cv + gc_weight * geom_cost(pb, v + 1, plane, opx, opy) : cv;
+        return P.geom_consistency ? cv + gc_weight * geom_finish(pb, v + 1, gs, opx, opy) : cv;
     };
 
     /* re-score the current hypothesis on the selected views
@@ -1637,11 +1662,19 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             for (int k = 0; k < 8; ++k) {
                 c[k] = vc[(size_t)min(v0 + k, nsrc - 1) * np];
             }
+            GeomSample gs[8];
+            if (P.geom_consistency) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    gs[k] = geom_fetch(pb, min(v0 + k, nsrc - 1) + 1, cur, px, py);
+                }
+            }
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const int v = v0 + k;
                 if (v < nsrc && ((sel_views >> v) & 1u)) {
-                    const float term = P.geom_consistency ? c[k] + gc_weight * geom_cost(pb, v + 1, cur, px, py) : c[k];
+                    const float term = P.geom_consistency ? c[k] + gc_weight * geom_finish(pb, v + 1, gs[k], px, py)
+                                                          : c[k];
                     rescore_sum += draws_of(vw, v) * term;
                 }
             }
@@ -1662,11 +1695,16 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
             depth_prior = depth_from_plane(rc, prior_plane, px, py);
             if (prior_pix) {
                 float rfc[8];
+                float4 hps[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    hps[i] = ld4(pb.hyp, flag[i] ? pos[i] : center);
+                }
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
                     rfc[i] = 0.0f;
                     if (flag[i]) {
-                        const float4 hp = ld4(pb.hyp, pos[i]);
+                        const float4 hp = hps[i];
                         const float dn = depth_from_plane(rc, hp, px, py);
                         const float pr = prior_term(pk, dn - depth_prior, prior_plane, hp);
                         rfc[i] = hpm_expf(-fc[i] * fc[i] / pk.beta) * pr;
@@ -1805,10 +1843,14 @@ __device__ __forceinline__ uint32_t propagate_pixel(const hpm_dev_problem& pb, i
                 float term = 0.0f;
                 if (act) {
                     const int opx = opxy & 0xffff, opy = opxy >> 16;
+                    GeomSample gs = {0.0f, 0.0f, 0.0f};
+                    if (P.geom_consistency) {
+                        gs = geom_fetch(pb, vi + 1, plane, opx, opy); /* in flight during the NCC */
+                    }
                     double tq[3];
                     plane_q(pb, plane, tq);
                     const float cv = ncc(pb, orp, opx, opy, vi + 1, tq);
-                    term = P.geom_consistency ? cv + gc_weight * geom_cost(pb, vi + 1, plane, opx, opy) : cv;
+                    term = P.geom_consistency ? cv + gc_weight * geom_finish(pb, vi + 1, gs, opx, opy) : cv;
                 }
 #pragma unroll
                 for (int cc = 0; cc < 5; ++cc) {

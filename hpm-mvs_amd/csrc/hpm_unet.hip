@@ -147,14 +147,17 @@ __global__ __launch_bounds__(256) void hpm_k_conv_fwd(const hpm_bf16* __restrict
     }
 }
 
-/* The 3x3 convolution with its operands staged in LDS: a block computes
- * one output row, 128 pixels (32 per wave) x 32 NT output channels (channel
- * group blockIdx.z); for every chunk of 32 input channels the three input
- * rows with their 1-pixel halo (130 x 32, zero outside the image) and the
- * weight slice (32 NT x 9 taps x 32) sit in LDS (rows padded to 40
- * channels against bank conflicts) while every wave runs the chunk's 9 taps
- * x 2 MFMA steps per channel tile. The next chunk's operands are loaded
- * into registers while the current one is multiplied, then stored to LDS:
+/* The 3x3 convolution with its operands staged in LDS: a block computes a
+ * tile of 4 output rows x 32 pixels (one row per wave) x 32 NT output
+ * channels (channel group blockIdx.z); for every chunk of 32 input channels
+ * the 6 input rows with their 1-pixel halo (6 x 34 x 32, zero outside the
+ * image) and the weight slice (32 NT x 9 taps x 32) sit in LDS (rows padded
+ * to 40 channels against bank conflicts) while every wave runs the chunk's
+ * 9 taps x 2 MFMA steps per channel tile. The square-ish tile reads 1.6
+ * input pixels per output (a row of 128 reads 3.05) and wastes at most 31
+ * columns per map row (a row of 128: up to 127, 28% of a 200-pixel map).
+ * The next chunk's operands are loaded into registers while the current one
+ * is multiplied, then stored to LDS:
  * the global loads overlap the matrix work. Products are accumulated chunk
  * by chunk, tap by tap. With up != NULL, input channels [0, cup) are the
  * nearest 2x upsampling of up ((h+1)/2 x (w+1)/2, stride ldu), read at
@@ -167,18 +170,20 @@ __global__ __launch_bounds__(256) void hpm_k_conv3_lds(const hpm_bf16* __restric
                                                        float* out32, hpm_bf16* out16, int ldo, int co, int height,
                                                        int width, int relu)
 {
-    constexpr int kInVec = 3 * 130 * 4;      /* 16-byte vectors of the input tile */
+    constexpr int kTileRows = 4, kCols = 34, kRows = kTileRows + 2;
+    constexpr int kInVec = kRows * kCols * 4; /* 16-byte vectors of the input tile */
     constexpr int kWVec = 32 * NT * 9 * 4;   /* and of the weight slice */
     constexpr int kIn = (kInVec + 255) / 256;
     constexpr int kW = (kWVec + 255) / 256;
-    __shared__ __align__(16) hpm_bf16 s_in[3 * 130 * kLdsPad];
+    __shared__ __align__(16) hpm_bf16 s_in[kRows * kCols * kLdsPad];
     __shared__ __align__(16) hpm_bf16 s_w[32 * NT * 9 * kLdsPad];
     const int tid = (int)threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int x_blk = (int)blockIdx.x * 128;
+    const int x_blk = (int)blockIdx.x * 32;
     const int n0 = (int)blockIdx.z * 32 * NT;
-    const int y = (int)blockIdx.y;
+    const int y_blk = (int)blockIdx.y * kTileRows;
+    const int y = y_blk + wave;
     const int r = lane & 31;
     const int kh = lane >> 5;
     f32x16 acc[NT];
@@ -196,8 +201,8 @@ __global__ __launch_bounds__(256) void hpm_k_conv3_lds(const hpm_bf16* __restric
         for (int k = 0; k < kIn; ++k) {
             const int e = tid + 256 * k;
             const int v = e & 3, pix = e >> 2;
-            const int row = pix / 130, col = pix % 130;
-            const int sy = y + row - 1, sx = x_blk + col - 1;
+            const int row = pix / kCols, col = pix % kCols;
+            const int sy = y_blk + row - 1, sx = x_blk + col - 1;
             rin[k] = zero;
             if (e < kInVec && sy >= 0 && sy < height && sx >= 0 && sx < width) {
                 rin[k] = from_up ? ld8(up + ((int64_t)(sy >> 1) * wu + (sx >> 1)) * ldu + c0 + 8 * v)
@@ -247,7 +252,7 @@ __global__ __launch_bounds__(256) void hpm_k_conv3_lds(const hpm_bf16* __restric
 #pragma unroll
             for (int k16 = 0; k16 < 2; ++k16) {
                 const bf16x8 a = *reinterpret_cast<const bf16x8*>(
-                    &s_in[(ky * 130 + wave * 32 + r + kx) * kLdsPad + 16 * k16 + 8 * kh]);
+                    &s_in[((ky + wave) * kCols + r + kx) * kLdsPad + 16 * k16 + 8 * kh]);
 #pragma unroll
                 for (int n = 0; n < NT; ++n) {
                     const bf16x8 b = *reinterpret_cast<const bf16x8*>(
@@ -262,7 +267,10 @@ __global__ __launch_bounds__(256) void hpm_k_conv3_lds(const hpm_bf16* __restric
             __syncthreads();
         }
     }
-    const int x0 = x_blk + wave * 32;
+    const int x0 = x_blk;
+    if (y >= height) {
+        return;
+    }
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
         const int ch = n0 + 32 * n + r;
@@ -410,7 +418,7 @@ namespace {
 
 bool lds_eligible(int ci, int ks)
 {
-    return ks == 3 && ci >= 64 && ci % 32 == 0;
+    return ks == 3 && ci >= 32 && ci % 32 == 0;
 }
 
 void launch_conv3_lds(const void* in, int ldi, int ci, const void* up, int ldu, int cup, const void* w,
@@ -420,17 +428,17 @@ void launch_conv3_lds(const void* in, int ldi, int ci, const void* up, int ldu, 
     /* 64 channels per block where co divides by 64 and the map gives four
      * blocks per CU (LDS for two blocks per CU), else 32: the channel groups
      * spread small maps over more blocks */
-    const int64_t blocks64 = (int64_t)((width + 127) / 128) * height * (co / 64);
-    const int nt = co % 64 == 0 && blocks64 >= 1024 ? 2 : 1;
-    const dim3 grid((unsigned)((width + 127) / 128), (unsigned)height, (unsigned)(co / (32 * nt)));
+    const int64_t tiles = (int64_t)((width + 31) / 32) * ((height + 3) / 4);
+    const int nt = co % 64 == 0 && tiles * (co / 64) >= 1024 ? 2 : 1;
+    const dim3 grid((unsigned)((width + 31) / 32), (unsigned)((height + 3) / 4), (unsigned)(co / (32 * nt)));
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(256), 0, stream, (const hpm_bf16*)in, ldi, ci, (const hpm_bf16*)up, ldu,
+                           cup, (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, relu);
+    };
     if (nt == 1) {
-        hipLaunchKernelGGL(hpm_k_conv3_lds<1>, grid, dim3(256), 0, stream, (const hpm_bf16*)in, ldi, ci,
-                           (const hpm_bf16*)up, ldu, cup, (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co,
-                           height, width, relu);
+        go(hpm_k_conv3_lds<1>);
     } else {
-        hipLaunchKernelGGL(hpm_k_conv3_lds<2>, grid, dim3(256), 0, stream, (const hpm_bf16*)in, ldi, ci,
-                           (const hpm_bf16*)up, ldu, cup, (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co,
-                           height, width, relu);
+        go(hpm_k_conv3_lds<2>);
     }
 }
 
@@ -452,7 +460,8 @@ extern "C" int hpm_unet_conv_fwd_up(const void* in, int ldi, int ci, const void*
 extern "C" int hpm_unet_conv_fwd(const void* in, int ldi, int ci, const void* w, const float* bias, float* out32,
                                  void* out16, int ldo, int co, int height, int width, int ks, int relu, void* stream)
 {
-    if (ci % 16 != 0 || co % 32 != 0 || (ks != 1 && ks != 3) || ldi % 8 != 0 || height < 1 || width < 1) {
+    if (ci < 16 || ci % 16 != 0 || co < 32 || co % 32 != 0 || (ks != 1 && ks != 3) || ldi % 8 != 0 || height < 1 ||
+        width < 1) {
         return (int)hipErrorInvalidValue;
     }
     /* tiles per wave: two pixel tiles at 32 output channels, else two or
@@ -464,8 +473,9 @@ extern "C" int hpm_unet_conv_fwd(const void* in, int ldi, int ci, const void* w,
                            (const hpm_bf16*)w, bias, out32, (hpm_bf16*)out16, ldo, co, height, width, ks, relu);
     };
     if (lds_eligible(ci, ks)) {
-        /* operands staged in LDS with the next chunk prefetched: pays where a
-         * block runs two or more input-channel chunks */
+        /* operands staged in LDS with the next chunk prefetched; also ahead
+         * of the tiled kernel for a single 32-channel chunk (the staged
+         * input tile is read by 9 taps x the block's channel tiles) */
         launch_conv3_lds(in, ldi, ci, nullptr, 0, 0, w, bias, out32, out16, ldo, co, height, width, relu,
                          (hipStream_t)stream);
     } else if (co == 32) {

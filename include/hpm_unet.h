@@ -31,7 +31,7 @@ int hpm_unet_conv_fwd(const void* in, int ldi, int ci, const void* w, const floa
 /* The 3x3 convolution of hpm_unet_conv_fwd on an input whose channels
  * [0, cup) are the nearest 2x upsampling of up ((height+1)/2 x (width+1)/2,
  * stride ldu) and whose channels [cup, ci) are in[.. + cup ..]: the
- * decoder's concatenation without its upsampled copy. ci >= 64, ci and cup
+ * decoder's concatenation without its upsampled copy. ci >= 32, ci and cup
  * multiples of 32. */
 int hpm_unet_conv_fwd_up(const void* in, int ldi, int ci, const void* up, int ldu, int cup, const void* w,
                          const float* bias, float* out32, void* out16, int ldo, int co, int height, int width,

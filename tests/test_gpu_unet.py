@@ -64,7 +64,9 @@ def _conv_ref(x_bf, w_bf, bias, ks, relu):
 
 @pytest.mark.parametrize("ks,relu,ci,co,h,w", [(3, 1, 32, 64, 13, 45), (3, 0, 16, 32, 7, 33), (1, 0, 96, 32, 5, 70),
                                                 (3, 1, 96, 96, 9, 131), (3, 0, 128, 128, 6, 257), (3, 1, 224, 96, 5, 140),
-                                                (3, 1, 64, 96, 3, 300), (3, 0, 160, 64, 4, 129), (3, 1, 96, 32, 7, 127)])
+                                                (3, 1, 64, 96, 3, 300), (3, 0, 160, 64, 4, 129), (3, 1, 96, 32, 7, 127),
+                                                (3, 1, 32, 128, 37, 1700), (3, 0, 64, 96, 5, 1601),
+                                                (3, 1, 32, 128, 257, 200)])
 def test_conv_forward(hpm, ks, relu, ci, co, h, w):
     say(f"case: conv forward ks={ks} relu={relu} ci={ci} co={co} {h}x{w}")
     g = torch.Generator().manual_seed(ks * 100 + ci)

@@ -17,9 +17,11 @@ other 10 as sources. One step is the whole per-scene schedule, timed for:
 - the same learned schedule on per-view workers, and in the batched form
   with the network trained on ground-truth maps (--learned-prior-gt).
 value = ms per reference view (step time / 11) of the fastest learned-prior
-schedule in the batched form whose depth accuracy (fraction of pixels within
-2% of ground truth) is at least the bar, else of the fastest plain schedule
-that meets it; every schedule is in config.methods. Images are resident in
+schedule (network trained on the plain method's maps, batched form or
+per-view workers, whichever is faster in the run) whose depth accuracy
+(fraction of pixels within 2% of ground truth) is at least the bar, else of
+the fastest plain schedule that meets it; every schedule is in
+config.methods. Images are resident in
 HBM before the timed region. Other workloads: "plain" (one plain
 RunPatchMatch per view), "batch" (the batched form over 32 mixed-size
 references), "fusion" (RunFusion of the 11 views), "large" (21 views of
@@ -609,9 +611,9 @@ def main():
     if 3 not in plain_variants:
         plain_variants[3] = time_method("planar prior, 3 GC passes", options(False, 3))
     plain = plain_variants[3]
-    # learned-prior families: (name, weight file, batched form); the first
-    # is the one the metric names: the network trained on the plain
-    # method's maps, in the batched form
+    # learned-prior families: (name, weight file, batched form): the network
+    # trained on the plain method's maps in the batched form and on per-view
+    # workers, and the network trained on ground-truth maps
     families = []
     if full_method and args.learned_prior.lower() != "none":
         if not args.no_batched:
@@ -634,8 +636,16 @@ def main():
         return min(ok, key=lambda g: variants[g]["ms_per_view"]) if ok else None
 
     plain_best = fastest(plain_variants)
-    value_fam = families[0][0] if families else None
-    best_g = fastest(fam_variants[value_fam]) if value_fam else None
+    # the learned prior's value: the faster of its two execution forms in
+    # this run (every view's pass in one batched launch, or per-view launches
+    # from host workers: the same kernels' arithmetic on the same inputs);
+    # the ground-truth-label network is reported beside it, not chosen
+    value_fam, best_g = None, None
+    for fam, _, _ in families:
+        g = None if fam.startswith("learned_prior_gt") else fastest(fam_variants[fam])
+        if g is not None and (best_g is None or fam_variants[fam][g]["ms_per_view"] <
+                              fam_variants[value_fam][best_g]["ms_per_view"]):
+            value_fam, best_g = fam, g
     if best_g is not None:
         chosen = fam_variants[value_fam][best_g]
         value_method = f"{value_fam}_{best_g}_gc_passes"
@@ -664,10 +674,11 @@ def main():
 
     if args.workload in ("full", "large"):
         scales = "0.25x and 0.5x planar priors with JBU" if large else "0.5x planar prior with JBU"
-        workload = ("per reference view: plain RunPatchMatch, then the prior (learned, batched form: every view's "
-                    "plain pass in one batch, U-Net trained on the plain method's maps proposing depth, normal "
-                    "and confidence, NCC judge, every view's prior pass in one batch, planar prior as fallback, "
-                    "then the fewest of 0, 1 or 3 geometric-consistency passes that meet the bar; plain method: "
+        workload = ("per reference view: plain RunPatchMatch, then the prior (learned: U-Net trained on the "
+                    "plain method's maps proposing depth, normal and confidence, NCC judge, the prior pass, "
+                    "planar prior as fallback, then the fewest of 0, 1 or 3 geometric-consistency passes that "
+                    "meet the bar; run in the batched form, every view's pass in one launch, and on per-view "
+                    "workers, the faster form giving the value; plain method: "
                     f"{scales} and 1x planar prior with KNN K=8, then 0, 1 or 3 geometric-consistency passes); "
                     "bar: the plain method's accuracy with 3 passes; "
                     f"{cfg.width}x{cfg.height}, {n - 1} sources, 11x11 window stride 2")
